@@ -1,0 +1,209 @@
+/*
+ * ofeli_hip.h -- C-ABI of libofeli_hip, the MI355X (gfx950) implementation of
+ * OFELI's sparse linear-solve path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  The
+ * C++ drop-in classes (OFELI::SpMatrix, Vect, Prec, LinearSolver, the equation
+ * classes) sit on top of these entry points; each entry point names the
+ * reference function it replaces (paths relative to the OFELI 5.3.3 tree).
+ *
+ * Memory: arguments named d_* are device pointers (hipMalloc'd, or any device
+ * allocation on the context's device); h_* are host pointers.  All device
+ * work is ordered on the context's stream.  Entry points that return host
+ * results (iteration counts, norms) synchronise that stream before returning.
+ *
+ * Status: every function returns OFH_OK (0) or a negative OFH_E_* code; the
+ * message of the last failure on the calling thread is ofh_last_error().
+ * Solvers report the reference's return convention separately in
+ * ofh_solve_info.nb_it (CG.h:154-156, BiCGStab.h:117-157, GMRes.h:236-240).
+ */
+#ifndef OFELI_HIP_H
+#define OFELI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define OFH_OK 0
+#define OFH_E_INVALID (-1)     /* bad argument / size / handle               */
+#define OFH_E_HIP (-2)         /* HIP runtime failure                        */
+#define OFH_E_ZERO_PIVOT (-3)  /* Prec::inv_diag / SSORSolve zero diagonal   */
+#define OFH_E_NEGDET (-4)      /* Triang3/Tetra4/Quad4 negative determinant  */
+#define OFH_E_PATTERN (-5)     /* entry outside the sparse pattern / empty row */
+#define OFH_E_NOMEM (-6)
+#define OFH_E_UNSUPPORTED (-7) /* e.g. DILU_PREC, not provided by this path  */
+#define OFH_E_ZERODET (-8)     /* null determinant                           */
+
+/* Iteration and Preconditioner enums: same values as Matrix.h:75-93. */
+#define OFH_DIRECT_SOLVER 1
+#define OFH_CG_SOLVER 2
+#define OFH_CGS_SOLVER 3
+#define OFH_BICG_SOLVER 4
+#define OFH_BICG_STAB_SOLVER 5
+#define OFH_GMRES_SOLVER 6
+#define OFH_IDENT_PREC 1
+#define OFH_DIAG_PREC 2
+#define OFH_DILU_PREC 3
+#define OFH_ILU_PREC 4
+#define OFH_SSOR_PREC 5
+
+/* Element kinds of the assembly plan (equation classes' build()). */
+#define OFH_LAPLACE2DT3 1 /* Laplace2DT3.cpp:93-143                          */
+#define OFH_DC2DT3 2      /* DC2DT3 steady diffusion, per-element k          */
+#define OFH_DC2DT3_BE 3   /* DC2DT3 backward Euler (Equa_Therm.h:162-218)    */
+#define OFH_LAPLACE3DT4 4 /* Laplace3DT4.cpp:76-129 / DC3DT4 Diffusion(k_e)  */
+#define OFH_ELAS2DQ4 5    /* Elas2DQ4.cpp:76-221                             */
+
+typedef struct ofh_ctx_s* ofh_ctx;
+typedef struct ofh_csr_s* ofh_csr;
+
+typedef struct ofh_solve_info {
+  int nb_it;          /* reference convention: it>0 converged after it steps,
+                         0 converged at start, -(max_it+1) not converged,
+                         2/3 BiCGStab breakdowns                             */
+  int iterations;     /* iterations actually performed                       */
+  double relres;      /* final ||r||/||b|| as the reference measures it      */
+  double true_relres; /* ||b-Ax||/||b|| recomputed in FP64 at the end        */
+  int outer;          /* mixed precision: FP64 residual replacements          */
+  int fallback;       /* mixed precision: 1 if the FP64 fallback ran          */
+  double seconds;     /* wall time of the solve (host clock, synchronised)    */
+} ofh_solve_info;
+
+/* ---------------------------------------------------------------- context */
+const char* ofh_last_error(void);
+int ofh_ctx_create(int device, ofh_ctx* out);
+int ofh_ctx_destroy(ofh_ctx ctx);
+/* Use an existing hipStream_t (NULL: the context's own stream). */
+int ofh_ctx_set_stream(ofh_ctx ctx, void* hip_stream);
+int ofh_ctx_synchronize(ofh_ctx ctx);
+int ofh_malloc(ofh_ctx ctx, size_t bytes, void** d_ptr);
+int ofh_free(ofh_ctx ctx, void* d_ptr);
+int ofh_memcpy_h2d(ofh_ctx ctx, void* d_dst, const void* h_src, size_t bytes);
+int ofh_memcpy_d2h(ofh_ctx ctx, void* h_dst, const void* d_src, size_t bytes);
+int ofh_memcpy_d2d(ofh_ctx ctx, void* d_dst, const void* d_src, size_t bytes);
+
+/* ------------------------------------------------------------ SpMatrix CSR */
+/* SpMatrix(row_ptr,col_ind,a) (SpMatrix.h:176-210): host arrays, row_ptr
+ * 0-based (n+1), col 0-based (the reference stores col_ind 1-based; the C++
+ * layer converts), sorted ascending in each row.  h_val may be NULL. */
+int ofh_csr_create(ofh_ctx ctx, int n, int64_t nnz, const int32_t* h_row_ptr,
+                   const int32_t* h_col, const double* h_val, ofh_csr* out);
+int ofh_csr_destroy(ofh_csr A);
+int ofh_csr_info(ofh_csr A, int* n, int64_t* nnz, int* max_row_nnz);
+int ofh_csr_get_pattern(ofh_csr A, int32_t* h_row_ptr, int32_t* h_col);
+/* Values: SpMatrix::get()/set (SpMatrix_impl.h:980-1005) as bulk copies. */
+int ofh_csr_set_values(ofh_csr A, const double* h_val);
+int ofh_csr_set_values_device(ofh_csr A, const double* d_val);
+int ofh_csr_get_values(ofh_csr A, double* h_val);
+int ofh_csr_values_device(ofh_csr A, double** d_val);
+
+/* y = A x: SpMatrix::Mult (SpMatrix_impl.h:728-734). */
+int ofh_spmv_f64(ofh_csr A, const double* d_x, double* d_y);
+/* y += a A x: SpMatrix::MultAdd (SpMatrix_impl.h:737-757). */
+int ofh_spmv_add_f64(ofh_csr A, double a, const double* d_x, double* d_y);
+/* FP32 SpMV on the FP32 mirror of the values (mixed-precision inner loop). */
+int ofh_spmv_f32(ofh_csr A, const float* d_x, float* d_y);
+
+/* ------------------------------------------------------------ Vect BLAS-1 */
+/* Vect::operator, (Vect_impl.h:2218-2225) and getNorm2 (:660-668). */
+int ofh_dot_f64(ofh_ctx ctx, int n, const double* d_x, const double* d_y, double* h_out);
+int ofh_norm2_f64(ofh_ctx ctx, int n, const double* d_x, double* h_out);
+/* y = a*x + y (Vect operator+= of a*x, Vect_impl.h:2143-2173). */
+int ofh_axpy_f64(ofh_ctx ctx, int n, double a, const double* d_x, double* d_y);
+
+/* ------------------------------------------------------------ Prec / SSOR */
+/* Prec<T_>::solve(b,x) (Prec_impl.h:226-255) for IDENT, DIAG and SSOR. */
+int ofh_prec_apply(ofh_csr A, int prec, const double* d_b, double* d_x);
+/* SpMatrix::SSORSolve (SpMatrix_impl.h:913-942): symmetric Gauss-Seidel. */
+int ofh_ssor_solve(ofh_csr A, const double* d_b, double* d_x);
+
+/* ---------------------------------------------------------------- solvers */
+/* CG (CG.h:99-157 via CG.h:189-200): x holds the initial guess. */
+int ofh_cg(ofh_csr A, int prec, const double* d_b, double* d_x, int max_it,
+           double toler, ofh_solve_info* info);
+/* BiCGStab (BiCGStab.h:87-162 via :194-216). */
+int ofh_bicgstab(ofh_csr A, int prec, const double* d_b, double* d_x, int max_it,
+                 double toler, ofh_solve_info* info);
+/* GMRes(m) (GMRes.h:148-241 via :274-283). */
+int ofh_gmres(ofh_csr A, int prec, const double* d_b, double* d_x, int m, int max_it,
+              double toler, ofh_solve_info* info);
+/* LinearSolver::solve (LinearSolver.cpp:194-244) dispatch; GMRES uses
+ * m = min(n/5, 50) instead of the reference's n/5 (LinearSolver.cpp:238). */
+int ofh_linear_solve(ofh_csr A, int solver, int prec, const double* d_b, double* d_x,
+                     int max_it, double toler, ofh_solve_info* info);
+/* Mixed precision CG (no reference counterpart; judged against ofh_cg):
+ * FP32 SpMV and vector updates, FP64 Krylov dot products, FP64 solution
+ * accumulation and FP64 residual replacement b-Ax; stops on the true FP64
+ * residual ||b-Ax||/||b|| <= toler; falls back to FP64 CG from the current
+ * iterate if the FP32 iteration stagnates. prec: IDENT or DIAG. */
+int ofh_cg_mixed(ofh_csr A, int prec, const double* d_b, double* d_x, int max_it,
+                 double toler, ofh_solve_info* info);
+
+/* ------------------------------------------------------ meshes (host side) */
+/* Structured rectangle, Mesh::Mesh(xmin,xmax,ymin,ymax,nx,ny,cx0,cxN,cy0,cyN,
+ * opt,nb_dof) (Mesh.cpp:547-676): h_coord (nn x 2), h_code (nn x nb_dof; the
+ * boundary code goes to DOF 1 as in the reference), h_elem (ne x nen,
+ * 1-based node ids).  quad = 0: 2 triangles per cell, 1: quadrilaterals. */
+int ofh_mesh2d_structured(double xmin, double xmax, double ymin, double ymax, int nx, int ny,
+                          int cx0, int cxN, int cy0, int cyN, int quad, int nb_dof,
+                          double* h_coord, int32_t* h_code, int32_t* h_elem);
+/* Structured box, node order and codes of Mesh.cpp:703-726; cells split into
+ * 6 positively oriented Kuhn tetrahedra (the reference's TETRAHEDRON split,
+ * Mesh.cpp:747-771, yields negative Jacobians). */
+int ofh_mesh3d_kuhn(double xmin, double xmax, double ymin, double ymax, double zmin,
+                    double zmax, int nx, int ny, int nz, int cx0, int cxN, int cy0, int cyN,
+                    int cz0, int czN, double* h_coord, int32_t* h_code, int32_t* h_elem);
+/* DOF numbering with imposed-DOF elimination, Mesh::NumberEquations with
+ * removeImposedDOF (Mesh.cpp:1499-1509): returns nb_eq, labels 1-based. */
+int ofh_number_equations(int nb_nodes, int nb_dof, const int32_t* h_code, int32_t* h_dof,
+                         int* nb_eq);
+/* Gauss-Legendre points/weights, Gauss::Gauss (Gauss.cpp:37-80). */
+int ofh_gauss(int np, double* h_x, double* h_w);
+
+/* ------------------------------------------------------- assembly (device) */
+typedef struct ofh_asm_s* ofh_asm;
+/* Uploads the mesh, numbers the equations (imposed DOFs eliminated) and builds
+ * on the GPU the CSR pattern of NodeGraph/FinalizeGraph/StoreGraph
+ * (GraphOfMatrix.cpp:52-67, 347-375, 619-634) plus the element-to-nonzero
+ * gather map used by ofh_asm_build.  h_elem holds 1-based node ids. */
+int ofh_asm_create(ofh_ctx ctx, int dim, int nb_nodes, const double* h_coord, int nb_el,
+                   int nen, const int32_t* h_elem, int nb_dof, const int32_t* h_code,
+                   ofh_asm* out);
+int ofh_asm_destroy(ofh_asm P);
+int ofh_asm_info(ofh_asm P, int* nb_eq, int64_t* nnz);
+/* The plan's matrix (pattern + values); owned by the plan. */
+int ofh_asm_matrix(ofh_asm P, ofh_csr* A);
+int ofh_asm_get_dof(ofh_asm P, int32_t* h_dof);
+/* The equation classes' element loop (Equa_Laplace.h:112-133,
+ * Equa_Therm.h:162-218, Equa_Solid.h:168-204): element matrices and
+ * right-hand sides (BodyRHS, updateBC lifting, backward-Euler term) computed
+ * per element, then each matrix entry and RHS entry summed over its elements
+ * in ascending element order (Matrix_impl.h:172-201, Vect_impl.h:1522-1534)
+ * with no atomics: bitwise the reference's sums.
+ *   d_coef: per-element coefficient (k for DC/3D kinds, E for ELAS2DQ4) or NULL
+ *   d_f:    nodal source (LAPLACE/DC kinds) or NULL
+ *   d_bc:   nodal Dirichlet values (nb_dof per node) or NULL
+ *   d_uprev: nodal u^n (DC2DT3_BE)
+ *   h_prm:  DC2DT3_BE {dt, rho, cp}; ELAS2DQ4 {nu, mode (0 reference parity:
+ *           plane strain, 2*Dev+Dil; 1 textbook: plane stress, Dev+Dil)}
+ * want_matrix = 0 assembles the right-hand side only. d_b may be NULL. */
+int ofh_asm_build(ofh_asm P, int kind, const double* d_coef, const double* d_f,
+                  const double* d_bc, const double* d_uprev, const double* h_prm, int nprm,
+                  int want_matrix, double* d_b);
+/* Elas2DQ4 constant side tractions (Line2 length, Elas2DQ4::BoundaryRHS,
+ * Elas2DQ4.cpp:256-263), added to d_b after the element loop in side order.
+ * h_side_nodes: nb_sides x 2 1-based node ids; d_traction: nb_sides x 2. */
+int ofh_asm_traction(ofh_asm P, int nb_sides, const int32_t* h_side_nodes,
+                     const double* d_traction, double* d_b);
+/* Vect::insertBC (Vect_impl.h:1097-1116): nodal vector from the free-DOF
+ * solution and the Dirichlet values (d_bc may be NULL: zero). */
+int ofh_asm_insert_bc(ofh_asm P, const double* d_v, const double* d_bc, double* d_u);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OFELI_HIP_H */

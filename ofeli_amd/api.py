@@ -1,0 +1,311 @@
+"""Python front end of libofeli_hip: objects mirroring OFELI's SpMatrix,
+the solver functions and the equation classes' assembly.
+
+Device vectors are torch tensors on the GPU (float64 unless noted).  The
+context runs its kernels on torch's current stream, so torch work and
+library work are ordered without extra synchronisation.
+"""
+import atexit
+import ctypes as C
+import weakref
+
+import numpy as np
+
+from . import (DIAG_PREC, SolveInfo, check, lib)
+
+
+def _ptr(t):
+    """Device pointer of a contiguous torch tensor (or None)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a GPU tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _f64(t, n=None):
+    import torch
+    if t.dtype != torch.float64:
+        raise TypeError("expected float64 tensor, got %s" % t.dtype)
+    if n is not None and t.numel() != n:
+        raise ValueError("expected %d entries, got %d" % (n, t.numel()))
+    return _ptr(t)
+
+
+def _np(a, dt):
+    a = np.ascontiguousarray(a, dtype=dt)
+    ct = {np.int32: C.c_int32, np.int64: C.c_int64, np.float64: C.c_double}[dt]
+    return a, a.ctypes.data_as(C.POINTER(ct))
+
+
+_live_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    # runs before torch releases the HIP runtime at interpreter exit
+    for c in list(_live_contexts):
+        c.close()
+
+
+class Context:
+    """ofh_ctx: device, stream and workspace of the library.  Objects made
+    from a context are closed before it."""
+
+    def __init__(self, device=0, use_torch_stream=True):
+        L = lib()
+        h = C.c_void_p()
+        check(L.ofh_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+        self._children = weakref.WeakSet()
+        _live_contexts.add(self)
+        if use_torch_stream:
+            import torch
+            s = torch.cuda.current_stream(device)
+            check(L.ofh_ctx_set_stream(self.h, C.c_void_p(s.cuda_stream)))
+
+    def synchronize(self):
+        check(lib().ofh_ctx_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            for ch in list(self._children):
+                ch.close()
+            lib().ofh_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CsrMatrix:
+    """Device CSR (SpMatrix storage): row_ptr 0-based, col 0-based sorted."""
+
+    def __init__(self, ctx, n=None, row_ptr=None, col=None, val=None, handle=None, owner=None):
+        self.ctx = ctx
+        self._own = handle is None
+        self._owner = owner
+        if self._own:
+            ctx._children.add(self)
+        if handle is None:
+            rp, rpp = _np(row_ptr, np.int32)
+            cl, clp = _np(col, np.int32)
+            vp = None
+            if val is not None:
+                v, vp = _np(val, np.float64)
+            h = C.c_void_p()
+            check(lib().ofh_csr_create(ctx.h, int(n), int(rp[-1]), rpp, clp, vp, C.byref(h)))
+            handle = h
+        self.h = handle
+        n_ = C.c_int()
+        nnz = C.c_int64()
+        mr = C.c_int()
+        check(lib().ofh_csr_info(self.h, C.byref(n_), C.byref(nnz), C.byref(mr)))
+        self.n, self.nnz, self.max_row_nnz = n_.value, nnz.value, mr.value
+
+    def pattern(self):
+        rp = np.zeros(self.n + 1, np.int32)
+        cl = np.zeros(max(self.nnz, 1), np.int32)
+        check(lib().ofh_csr_get_pattern(self.h, rp.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        cl.ctypes.data_as(C.POINTER(C.c_int32))))
+        return rp, cl[: self.nnz]
+
+    def values(self):
+        v = np.zeros(max(self.nnz, 1))
+        check(lib().ofh_csr_get_values(self.h, v.ctypes.data_as(C.POINTER(C.c_double))))
+        return v[: self.nnz]
+
+    def set_values(self, v):
+        import torch
+        if isinstance(v, torch.Tensor):
+            check(lib().ofh_csr_set_values_device(self.h, _f64(v, self.nnz)))
+        else:
+            a, ap = _np(v, np.float64)
+            assert a.size == self.nnz
+            check(lib().ofh_csr_set_values(self.h, ap))
+
+    def close(self):
+        if self._own and self.h:
+            lib().ofh_csr_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def spmv(A, x, y):
+    """y = A x (SpMatrix::Mult)."""
+    check(lib().ofh_spmv_f64(A.h, _f64(x, A.n), _f64(y, A.n)))
+
+
+def spmv_add(A, a, x, y):
+    """y += a A x (SpMatrix::MultAdd)."""
+    check(lib().ofh_spmv_add_f64(A.h, float(a), _f64(x, A.n), _f64(y, A.n)))
+
+
+def spmv_f32(A, x, y):
+    check(lib().ofh_spmv_f32(A.h, _ptr(x), _ptr(y)))
+
+
+def dot(ctx, x, y):
+    out = C.c_double()
+    check(lib().ofh_dot_f64(ctx.h, x.numel(), _f64(x), _f64(y), C.byref(out)))
+    return out.value
+
+
+def norm2(ctx, x):
+    out = C.c_double()
+    check(lib().ofh_norm2_f64(ctx.h, x.numel(), _f64(x), C.byref(out)))
+    return out.value
+
+
+def prec_apply(A, prec, b, x):
+    check(lib().ofh_prec_apply(A.h, prec, _f64(b, A.n), _f64(x, A.n)))
+
+
+def ssor_solve(A, b, x):
+    check(lib().ofh_ssor_solve(A.h, _f64(b, A.n), _f64(x, A.n)))
+
+
+def _solve(fn, A, b, x, *args):
+    info = SolveInfo()
+    check(getattr(lib(), fn)(A.h, *args[:1], _f64(b, A.n), _f64(x, A.n), *args[1:],
+                             C.byref(info)))
+    return info
+
+
+def cg(A, b, x, prec=DIAG_PREC, max_it=1000, toler=1.4901161193847656e-08):
+    """CG (CG.h:99-157); x is the initial guess and receives the solution."""
+    return _solve("ofh_cg", A, b, x, prec, int(max_it), float(toler))
+
+
+def bicgstab(A, b, x, prec=DIAG_PREC, max_it=1000, toler=1.4901161193847656e-08):
+    return _solve("ofh_bicgstab", A, b, x, prec, int(max_it), float(toler))
+
+
+def gmres(A, b, x, m, prec=DIAG_PREC, max_it=1000, toler=1.4901161193847656e-08):
+    return _solve("ofh_gmres", A, b, x, prec, int(m), int(max_it), float(toler))
+
+
+def cg_mixed(A, b, x, prec=DIAG_PREC, max_it=1000, toler=1.4901161193847656e-08):
+    return _solve("ofh_cg_mixed", A, b, x, prec, int(max_it), float(toler))
+
+
+def linear_solve(A, b, x, solver, prec=DIAG_PREC, max_it=1000, toler=1.4901161193847656e-08):
+    info = SolveInfo()
+    check(lib().ofh_linear_solve(A.h, int(solver), int(prec), _f64(b, A.n), _f64(x, A.n),
+                                 int(max_it), float(toler), C.byref(info)))
+    return info
+
+
+# ---------------------------------------------------------------- meshes
+def mesh2d(nx, ny, quad=False, nb_dof=1, codes=(1, 1, 1, 1), box=(0.0, 1.0, 0.0, 1.0)):
+    """Structured rectangle (Mesh.cpp:547-676): coord, code, elem (1-based)."""
+    nn = (nx + 1) * (ny + 1)
+    ne = nx * ny if quad else 2 * nx * ny
+    coord = np.zeros((nn, 2))
+    code = np.zeros((nn, nb_dof), np.int32)
+    elem = np.zeros((ne, 4 if quad else 3), np.int32)
+    check(lib().ofh_mesh2d_structured(*map(float, box), nx, ny, *codes, int(quad), nb_dof,
+                                      coord.ctypes.data_as(C.POINTER(C.c_double)),
+                                      code.ctypes.data_as(C.POINTER(C.c_int32)),
+                                      elem.ctypes.data_as(C.POINTER(C.c_int32))))
+    return coord, code, elem
+
+
+def mesh3d_kuhn(nx, ny, nz, codes=(1, 1, 1, 1, 1, 1), box=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0)):
+    nn = (nx + 1) * (ny + 1) * (nz + 1)
+    coord = np.zeros((nn, 3))
+    code = np.zeros((nn, 1), np.int32)
+    elem = np.zeros((6 * nx * ny * nz, 4), np.int32)
+    check(lib().ofh_mesh3d_kuhn(*map(float, box), nx, ny, nz, *codes,
+                                coord.ctypes.data_as(C.POINTER(C.c_double)),
+                                code.ctypes.data_as(C.POINTER(C.c_int32)),
+                                elem.ctypes.data_as(C.POINTER(C.c_int32))))
+    return coord, code, elem
+
+
+def number_equations(code):
+    code = np.ascontiguousarray(code, np.int32)
+    dof = np.zeros_like(code)
+    n = C.c_int()
+    check(lib().ofh_number_equations(code.shape[0], code.shape[1],
+                                     code.ctypes.data_as(C.POINTER(C.c_int32)),
+                                     dof.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(n)))
+    return dof, n.value
+
+
+def gauss(np_):
+    x = np.zeros(np_)
+    w = np.zeros(np_)
+    check(lib().ofh_gauss(np_, x.ctypes.data_as(C.POINTER(C.c_double)),
+                          w.ctypes.data_as(C.POINTER(C.c_double))))
+    return x, w
+
+
+class AssemblyPlan:
+    """Mesh on the device + CSR pattern + gather map (equation build)."""
+
+    def __init__(self, ctx, coord, elem, code):
+        coord = np.ascontiguousarray(coord, np.float64)
+        elem = np.ascontiguousarray(elem, np.int32)
+        code = np.ascontiguousarray(code, np.int32)
+        self.ctx = ctx
+        ctx._children.add(self)
+        self.nb_nodes, self.dim = coord.shape
+        self.nb_el, self.nen = elem.shape
+        self.nb_dof = code.shape[1]
+        h = C.c_void_p()
+        check(lib().ofh_asm_create(ctx.h, self.dim, self.nb_nodes,
+                                   coord.ctypes.data_as(C.POINTER(C.c_double)), self.nb_el,
+                                   self.nen, elem.ctypes.data_as(C.POINTER(C.c_int32)),
+                                   self.nb_dof, code.ctypes.data_as(C.POINTER(C.c_int32)),
+                                   C.byref(h)))
+        self.h = h
+        neq = C.c_int()
+        nnz = C.c_int64()
+        check(lib().ofh_asm_info(self.h, C.byref(neq), C.byref(nnz)))
+        self.nb_eq, self.nnz = neq.value, nnz.value
+        a = C.c_void_p()
+        check(lib().ofh_asm_matrix(self.h, C.byref(a)))
+        self.A = CsrMatrix(ctx, handle=a, owner=self)
+
+    def dof(self):
+        d = np.zeros((self.nb_nodes, self.nb_dof), np.int32)
+        check(lib().ofh_asm_get_dof(self.h, d.ctypes.data_as(C.POINTER(C.c_int32))))
+        return d
+
+    def build(self, kind, coef=None, f=None, bc=None, u_prev=None, prm=(), matrix=True, b=None):
+        p = np.ascontiguousarray(prm if len(prm) else [0.0], np.float64)
+        check(lib().ofh_asm_build(self.h, int(kind), _ptr(coef), _ptr(f), _ptr(bc), _ptr(u_prev),
+                                  p.ctypes.data_as(C.POINTER(C.c_double)), len(prm),
+                                  int(bool(matrix)), _ptr(b)))
+
+    def traction(self, sides, traction, b):
+        s = np.ascontiguousarray(sides, np.int32)
+        check(lib().ofh_asm_traction(self.h, s.shape[0], s.ctypes.data_as(C.POINTER(C.c_int32)),
+                                     _ptr(traction), _ptr(b)))
+
+    def insert_bc(self, v, bc, u):
+        check(lib().ofh_asm_insert_bc(self.h, _ptr(v), _ptr(bc), _ptr(u)))
+
+    def close(self):
+        if self.h:
+            self.A.h = None
+            lib().ofh_asm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,708 @@
+// ofh_assembly.hip -- sparse pattern and finite-element assembly on the GPU.
+//
+// Pattern (NodeGraph/FinalizeGraph/StoreGraph, GraphOfMatrix.cpp:52-67,
+// 347-375, 619-634): the columns of the row of a free DOF are the free DOFs of
+// every node that shares an element with its node.  Instead of sorting the
+// full list of element DOF pairs, the GPU builds the node-to-element
+// adjacency (elements in ascending order) and lets one thread per row merge
+// the DOFs of its neighbours into a sorted, duplicate-free list.
+//
+// Assembly (Matrix<T_>::Assembly, Matrix_impl.h:172-201; Vect::Assembly,
+// Vect_impl.h:1522-1534): one thread per element computes the element matrix
+// and right-hand side into a buffer; a precomputed gather map lists, for every
+// nonzero, the element-matrix entries that land on it in ascending element
+// order, and one thread per nonzero sums them from 0.0 in that order.  This is
+// the reference's summation order, without atomics, so A and b are bitwise
+// the reference's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "ofh_elem.h"
+#include "ofh_internal.h"
+
+struct ofh_asm_s {
+  ofh_ctx ctx = nullptr;
+  int dim = 0, nb_nodes = 0, nb_el = 0, nen = 0, nb_dof = 0, nee = 0, nb_eq = 0;
+  double* coord = nullptr;   // nb_nodes x dim
+  int* elem = nullptr;       // nb_el x nen, 0-based node ids
+  int* dof = nullptr;        // nb_nodes x nb_dof, 1-based labels, 0 = eliminated
+  int* code = nullptr;       // nb_nodes x nb_dof
+  int* adj_ptr = nullptr;    // nb_nodes + 1
+  int* adj_el = nullptr;     // elements around each node, ascending
+  int* adj_loc = nullptr;    // local index of the node in that element
+  int* row_node = nullptr;   // nb_eq: node of each equation
+  int* row_comp = nullptr;   // nb_eq: DOF component of each equation
+  int64_t* map_ptr = nullptr;  // nnz + 1
+  int* map = nullptr;          // element-matrix entry ids (e*nee*nee + kk)
+  double* ke = nullptr;        // nb_el x nee x nee
+  double* erhs = nullptr;      // nb_el x nee
+  int* err = nullptr;          // device error slots
+  ofh_csr A = nullptr;
+  std::vector<int> h_dof;
+};
+
+namespace ofh {
+
+constexpr int kMaxRowCand = 96;  // distinct columns per row the merge can hold
+
+__global__ void k_adj_count(int nb_el, int nen, const int* __restrict__ elem, int* cnt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb_el * nen) return;
+  atomicAdd(&cnt[elem[t]], 1);
+}
+
+__global__ void k_adj_fill(int nb_el, int nen, const int* __restrict__ elem,
+                           const int* __restrict__ ptr, int* cursor, int* adj_el, int* adj_loc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb_el * nen) return;
+  const int node = elem[t];
+  const int slot = ptr[node] + atomicAdd(&cursor[node], 1);
+  adj_el[slot] = t / nen;
+  adj_loc[slot] = t % nen;
+}
+
+// the slots above are taken in arbitrary order: sort each node's list
+__global__ void k_adj_sort(int nb_nodes, const int* __restrict__ ptr, int* adj_el, int* adj_loc) {
+  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= nb_nodes) return;
+  const int a = ptr[nd], e = ptr[nd + 1];
+  for (int i = a + 1; i < e; ++i) {
+    const int ke = adj_el[i], kl = adj_loc[i];
+    int j = i - 1;
+    while (j >= a && adj_el[j] > ke) {
+      adj_el[j + 1] = adj_el[j];
+      adj_loc[j + 1] = adj_loc[j];
+      --j;
+    }
+    adj_el[j + 1] = ke;
+    adj_loc[j + 1] = kl;
+  }
+}
+
+// Sorted distinct columns of row r into cols[] (returns count, -1 on overflow).
+__device__ int row_columns(int node, int nen, int nb_dof, const int* __restrict__ elem,
+                           const int* __restrict__ dof, const int* __restrict__ adj_ptr,
+                           const int* __restrict__ adj_el, int* cols) {
+  int n = 0;
+  for (int t = adj_ptr[node]; t < adj_ptr[node + 1]; ++t) {
+    const int* en = elem + (size_t)adj_el[t] * nen;
+    for (int j = 0; j < nen; ++j)
+      for (int l = 0; l < nb_dof; ++l) {
+        const int d = dof[en[j] * nb_dof + l];
+        if (!d) continue;
+        const int c = d - 1;
+        int pos = n;
+        while (pos > 0 && cols[pos - 1] > c) --pos;
+        if (pos > 0 && cols[pos - 1] == c) continue;
+        if (n == kMaxRowCand) return -1;
+        for (int q = n; q > pos; --q) cols[q] = cols[q - 1];
+        cols[pos] = c;
+        ++n;
+      }
+  }
+  return n;
+}
+
+__global__ void k_row_count(int nb_eq, int nen, int nb_dof, const int* __restrict__ elem,
+                            const int* __restrict__ dof, const int* __restrict__ adj_ptr,
+                            const int* __restrict__ adj_el, const int* __restrict__ row_node,
+                            int* row_len, int* err) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nb_eq) return;
+  int cols[kMaxRowCand];
+  const int n = row_columns(row_node[r], nen, nb_dof, elem, dof, adj_ptr, adj_el, cols);
+  if (n < 0) atomicMin(err, r);
+  row_len[r] = n < 0 ? 0 : n;
+}
+
+__global__ void k_row_fill(int nb_eq, int nen, int nb_dof, const int* __restrict__ elem,
+                           const int* __restrict__ dof, const int* __restrict__ adj_ptr,
+                           const int* __restrict__ adj_el, const int* __restrict__ row_node,
+                           const int* __restrict__ row_ptr, int* col) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nb_eq) return;
+  int cols[kMaxRowCand];
+  const int n = row_columns(row_node[r], nen, nb_dof, elem, dof, adj_ptr, adj_el, cols);
+  const int base = row_ptr[r];
+  for (int q = 0; q < n; ++q) col[base + q] = cols[q];
+}
+
+__device__ __forceinline__ int find_col(const int* __restrict__ col, int a, int e, int c) {
+  int lo = a, hi = e - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int v = col[mid];
+    if (v == c) return mid;
+    if (v < c) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+// Gather map: for each nonzero of row r, the element-matrix entries that
+// Matrix::Assembly adds to it, in ascending element order.  FILL = false
+// counts them, FILL = true writes them.
+template <bool FILL>
+__global__ void k_map(int nb_eq, int nen, int nb_dof, int nee, const int* __restrict__ elem,
+                      const int* __restrict__ dof, const int* __restrict__ adj_ptr,
+                      const int* __restrict__ adj_el, const int* __restrict__ adj_loc,
+                      const int* __restrict__ row_node, const int* __restrict__ row_comp,
+                      const int* __restrict__ row_ptr, const int* __restrict__ col,
+                      int* __restrict__ cnt, const int64_t* __restrict__ map_ptr,
+                      int* __restrict__ map) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nb_eq) return;
+  const int node = row_node[r], k = row_comp[r];
+  const int ra = row_ptr[r], re = row_ptr[r + 1];
+  if (FILL)
+    for (int p = ra; p < re; ++p) cnt[p] = 0;
+  for (int t = adj_ptr[node]; t < adj_ptr[node + 1]; ++t) {
+    const int e = adj_el[t], i = adj_loc[t];
+    const int* en = elem + (size_t)e * nen;
+    for (int j = 0; j < nen; ++j)
+      for (int l = 0; l < nb_dof; ++l) {
+        const int d = dof[en[j] * nb_dof + l];
+        if (!d) continue;
+        const int p = find_col(col, ra, re, d - 1);
+        if (p < 0) continue;  // cannot happen: the pattern comes from the same adjacency
+        const int kk = (i * nb_dof + k) * nee + j * nb_dof + l;
+        if (FILL) map[map_ptr[p] + cnt[p]++] = e * nee * nee + kk;
+        else cnt[p]++;
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// element kernel: element matrix (when ke != null) and right-hand side
+// ---------------------------------------------------------------------------
+struct ElemArgs {
+  int kind, nen, nb_dof, nee, dim;
+  const double* coord;
+  const int* elem;
+  const int* code;
+  const double* coef;
+  const double* f;
+  const double* bc;
+  const double* uprev;
+  double p0, p1, p2;  // kind parameters
+  double g0, g1, w0, w1;  // 2-point Gauss rule (Q4)
+  double* ke;
+  double* erhs;
+  int* err;
+};
+
+__device__ __forceinline__ void report(int* err, int e, int rc) {
+  atomicMin(&err[rc == -1 ? 1 : 2], e);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_element(int nb_el, ElemArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nb_el) return;
+  constexpr int NEN = (KIND == OFH_LAPLACE3DT4 || KIND == OFH_ELAS2DQ4) ? 4 : 3;
+  constexpr int NB = KIND == OFH_ELAS2DQ4 ? 2 : 1;
+  constexpr int NEE = NEN * NB;
+  constexpr int DIM = KIND == OFH_LAPLACE3DT4 ? 3 : 2;
+  int nd[NEN];
+#pragma unroll
+  for (int i = 0; i < NEN; ++i) nd[i] = a.elem[(size_t)e * NEN + i];
+  double x[NEN * DIM];
+#pragma unroll
+  for (int i = 0; i < NEN; ++i)
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) x[DIM * i + c] = a.coord[(size_t)nd[i] * DIM + c];
+  double erhs[NEE];
+#pragma unroll
+  for (int q = 0; q < NEE; ++q) erhs[q] = 0.0;
+
+  if constexpr (KIND == OFH_ELAS2DQ4) {
+    double dx[4][4], dy[4][4], w[4];
+    const int rc = quad_shape(reinterpret_cast<const double(&)[8]>(x), a.g0, a.g1, a.w0, a.w1, dx,
+                              dy, w);
+    if (rc) { report(a.err, e, rc); return; }
+    double lambda, G;
+    const int mode = (int)a.p1;
+    elas_moduli(a.coef[e], a.p0, mode, lambda, G);
+    if (a.ke) {
+      double* out = a.ke + (size_t)e * 64;
+      for (int i = 0; i < 4; ++i)
+        for (int di = 0; di < 2; ++di)
+          for (int j = 0; j < 4; ++j)
+            for (int dj = 0; dj < 2; ++dj)
+              out[(2 * i + di) * 8 + 2 * j + dj] = q4_entry(i, di, j, dj, dx, dy, w, G, lambda, mode);
+    }
+    if (a.bc) {  // updateBC, Equation_impl.h:160-176
+      for (int in = 0; in < 8; ++in)
+        for (int j = 0; j < 4; ++j)
+          for (int l = 0; l < 2; ++l)
+            if (a.code[nd[j] * 2 + l] > 0)
+              erhs[in] -= q4_entry(in / 2, in % 2, j, l, dx, dy, w, G, lambda, mode) *
+                          a.bc[nd[j] * 2 + l];
+    }
+  } else {
+    double em[NEE * NEE];
+    double ea1[9];
+    double meas;  // area or volume
+    if constexpr (KIND == OFH_LAPLACE3DT4) {
+      double d[12];
+      const int rc = tet_shape(reinterpret_cast<const double(&)[12]>(x), meas, d);
+      if (rc) { report(a.err, e, rc); return; }
+      const bool var = a.coef != nullptr;
+      const double c = var ? 1.0 * a.coef[e] * meas : 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double dd = d[3 * i] * d[3 * j] + d[3 * i + 1] * d[3 * j + 1] + d[3 * i + 2] * d[3 * j + 2];
+          em[4 * i + j] = var ? 0.0 + (0.0 + c * dd) : meas * dd;
+        }
+      if (a.f) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const double fi = a.f[nd[i]];
+          erhs[i] += var ? fi * 0.25 * meas : 0.25 * meas * fi;
+        }
+      }
+    } else {
+      double d[6];
+      const int rc = tri_shape(reinterpret_cast<const double(&)[6]>(x), meas, d);
+      if (rc) { report(a.err, e, rc); return; }
+      const double k = KIND == OFH_LAPLACE2DT3 ? 0.0 : a.coef[e];
+      double ea0[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const double dd = d[2 * i] * d[2 * j] + d[2 * i + 1] * d[2 * j + 1] + 0.0 * 0.0;
+          ea0[3 * i + j] = KIND == OFH_LAPLACE2DT3 ? 0.0 + meas * dd : 0.0 + 1.0 * k * meas * dd;
+        }
+      if constexpr (KIND == OFH_DC2DT3_BE) {
+        const double dti = 1. / a.p0;
+        const double c = kThird * meas * a.p1 * a.p2 * 1.0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) ea1[q] = 0.0;
+        ea1[0] += c;
+        ea1[4] += c;
+        ea1[8] += c;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) em[q] = dti * ea1[q] + ea0[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) em[q] = ea0[q];
+      }
+      if (a.f) {
+        if (KIND == OFH_LAPLACE2DT3) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) erhs[i] += a.f[nd[i]] * meas * kThird;
+        } else {
+          const double c = kThird * meas;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) erhs[i] += c * a.f[nd[i]];
+        }
+      }
+    }
+    if (a.ke) {
+      double* out = a.ke + (size_t)e * NEE * NEE;
+#pragma unroll
+      for (int q = 0; q < NEE * NEE; ++q) out[q] = em[q];
+    }
+    if (a.bc) {  // updateBC, Equation_impl.h:160-176
+#pragma unroll
+      for (int in = 0; in < NEE; ++in)
+#pragma unroll
+        for (int j = 0; j < NEN; ++j)
+          if (a.code[nd[j]] > 0) erhs[in] -= em[in * NEE + j] * a.bc[nd[j]];
+    }
+    if constexpr (KIND == OFH_DC2DT3_BE) {  // eRHS += dti*eA1*u^n, Equa_Therm.h:203
+      if (a.uprev) {
+        const double dti = 1. / a.p0;
+        double m1[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) m1[q] = dti * ea1[q];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          double s = 0.0;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) s += m1[3 * i + j] * a.uprev[nd[j]];
+          erhs[i] = erhs[i] + s;
+        }
+      }
+    }
+  }
+  double* ro = a.erhs + (size_t)e * NEE;
+#pragma unroll
+  for (int q = 0; q < NEE; ++q) ro[q] = erhs[q];
+}
+
+// a_p = 0 + sum of its element entries in ascending element order
+__global__ void k_gather_matrix(int64_t nnz, const int64_t* __restrict__ map_ptr,
+                                const int* __restrict__ map, const double* __restrict__ ke,
+                                double* __restrict__ val) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nnz) return;
+  double s = 0.0;
+  for (int64_t t = map_ptr[p]; t < map_ptr[p + 1]; ++t) s += ke[map[t]];
+  val[p] = s;
+}
+
+// b_r = 0 + sum over the elements of its node, ascending
+__global__ void k_gather_rhs(int nb_eq, int nb_dof, int nee, const int* __restrict__ row_node,
+                             const int* __restrict__ row_comp, const int* __restrict__ adj_ptr,
+                             const int* __restrict__ adj_el, const int* __restrict__ adj_loc,
+                             const double* __restrict__ erhs, double* __restrict__ b) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nb_eq) return;
+  const int node = row_node[r], k = row_comp[r];
+  double s = 0.0;
+  for (int t = adj_ptr[node]; t < adj_ptr[node + 1]; ++t)
+    s += erhs[(size_t)adj_el[t] * nee + adj_loc[t] * nb_dof + k];
+  b[r] = s;
+}
+
+// Elas2DQ4 side tractions, one thread, side order (Equa_Solid.h:194-202)
+__global__ void k_traction(int nb_sides, const int* __restrict__ sides,
+                           const double* __restrict__ coord, const double* __restrict__ tr,
+                           const int* __restrict__ dof, double* b, int* err) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int s = 0; s < nb_sides; ++s) {
+    const int n0 = sides[2 * s] - 1, n1 = sides[2 * s + 1] - 1;
+    const double dlx = 0.5 * (coord[2 * n1] - coord[2 * n0]);
+    const double dly = 0.5 * (coord[2 * n1 + 1] - coord[2 * n0 + 1]);
+    const double det = sqrt(dlx * dlx + dly * dly + 0.0 * 0.0);  // Line2.cpp:85-86
+    if (det == 0.0) { err[2] = min(err[2], s); continue; }
+    const double c = 0.5 * (2 * det);
+    const int nd[2] = {n0, n1};
+    for (int i = 0; i < 2; ++i)
+      for (int k = 0; k < 2; ++k) {
+        const double v = 0.0 + c * tr[2 * s + k];
+        const int d = dof[nd[i] * 2 + k];
+        if (d) b[d - 1] += v;
+      }
+  }
+}
+
+// Vect::insertBC: u_i = v_j for free DOFs (in order), bc_i otherwise
+__global__ void k_insert_bc(int m, const int* __restrict__ dof, const double* __restrict__ v,
+                            const double* __restrict__ bc, double* __restrict__ u) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  const int d = dof[t];
+  u[t] = d ? v[d - 1] : (bc ? bc[t] : 0.0);
+}
+
+static int host_exclusive_scan(hipStream_t s, const int* d_cnt, int n, std::vector<int64_t>& out) {
+  std::vector<int> c(n);
+  OFH_HIP(hipMemcpyAsync(c.data(), d_cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  OFH_HIP(hipStreamSynchronize(s));
+  out.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) out[i + 1] = out[i] + c[i];
+  return OFH_OK;
+}
+
+static int check_err(ofh_asm P, const char* what) {
+  int h[4];
+  OFH_HIP(hipMemcpyAsync(h, P->err, sizeof(h), hipMemcpyDeviceToHost, P->ctx->stream));
+  OFH_HIP(hipStreamSynchronize(P->ctx->stream));
+  const int big = 0x7fffffff;
+  if (h[0] != big)
+    return fail(OFH_E_PATTERN, std::string(what) + ": row " + std::to_string(h[0] + 1) +
+                                   " has more than 96 distinct columns");
+  if (h[1] != big)
+    return fail(OFH_E_NEGDET, std::string(what) + ": negative determinant of jacobian in element " +
+                                  std::to_string(h[1] + 1));
+  if (h[2] != big)
+    return fail(OFH_E_ZERODET, std::string(what) + ": determinant of jacobian is null (element/side " +
+                                   std::to_string(h[2] + 1) + ")");
+  return OFH_OK;
+}
+
+static int reset_err(ofh_asm P) {
+  const int big[4] = {0x7fffffff, 0x7fffffff, 0x7fffffff, 0x7fffffff};
+  OFH_HIP(hipMemcpyAsync(P->err, big, sizeof(big), hipMemcpyHostToDevice, P->ctx->stream));
+  return OFH_OK;
+}
+
+}  // namespace ofh
+
+using namespace ofh;
+
+extern "C" {
+
+int ofh_asm_create(ofh_ctx ctx, int dim, int nb_nodes, const double* h_coord, int nb_el, int nen,
+                   const int32_t* h_elem, int nb_dof, const int32_t* h_code, ofh_asm* out) {
+  OFH_CHECK(ctx && h_coord && h_elem && h_code && out, OFH_E_INVALID, "ofh_asm_create: null argument");
+  OFH_CHECK((dim == 2 || dim == 3) && nb_nodes > 0 && nb_el > 0 && (nen == 3 || nen == 4) &&
+                (nb_dof == 1 || nb_dof == 2),
+            OFH_E_INVALID, "ofh_asm_create: unsupported mesh shape");
+  OFH_CHECK((int64_t)nb_el * nen * nen * nb_dof * nb_dof < (int64_t)0x7fffffff, OFH_E_INVALID,
+            "ofh_asm_create: element-matrix buffer exceeds 2^31 entries");
+  for (int64_t t = 0; t < (int64_t)nb_el * nen; ++t)
+    OFH_CHECK(h_elem[t] >= 1 && h_elem[t] <= nb_nodes, OFH_E_INVALID,
+              "ofh_asm_create: element node id out of range");
+  hipStream_t s = ctx->stream;
+  ofh_asm P = new ofh_asm_s();
+  P->ctx = ctx;
+  P->dim = dim;
+  P->nb_nodes = nb_nodes;
+  P->nb_el = nb_el;
+  P->nen = nen;
+  P->nb_dof = nb_dof;
+  P->nee = nen * nb_dof;
+  // equation numbering (Mesh.cpp:1499-1509)
+  P->h_dof.resize((size_t)nb_nodes * nb_dof);
+  int neq = 0;
+  ofh_number_equations(nb_nodes, nb_dof, h_code, P->h_dof.data(), &neq);
+  P->nb_eq = neq;
+  OFH_CHECK(neq > 0, OFH_E_INVALID, "ofh_asm_create: every DOF is imposed");
+  std::vector<int> rnode(neq), rcomp(neq);
+  for (int64_t t = 0; t < (int64_t)nb_nodes * nb_dof; ++t)
+    if (P->h_dof[t]) {
+      rnode[P->h_dof[t] - 1] = (int)(t / nb_dof);
+      rcomp[P->h_dof[t] - 1] = (int)(t % nb_dof);
+    }
+  std::vector<int> elem0((size_t)nb_el * nen);
+  for (size_t t = 0; t < elem0.size(); ++t) elem0[t] = h_elem[t] - 1;
+  OFH_HIP(hipMalloc(&P->coord, sizeof(double) * (size_t)nb_nodes * dim));
+  OFH_HIP(hipMalloc(&P->elem, sizeof(int) * elem0.size()));
+  OFH_HIP(hipMalloc(&P->dof, sizeof(int) * P->h_dof.size()));
+  OFH_HIP(hipMalloc(&P->code, sizeof(int) * P->h_dof.size()));
+  OFH_HIP(hipMalloc(&P->row_node, sizeof(int) * neq));
+  OFH_HIP(hipMalloc(&P->row_comp, sizeof(int) * neq));
+  OFH_HIP(hipMalloc(&P->err, sizeof(int) * 4));
+  OFH_HIP(hipMemcpyAsync(P->coord, h_coord, sizeof(double) * (size_t)nb_nodes * dim, hipMemcpyHostToDevice, s));
+  OFH_HIP(hipMemcpyAsync(P->elem, elem0.data(), sizeof(int) * elem0.size(), hipMemcpyHostToDevice, s));
+  OFH_HIP(hipMemcpyAsync(P->dof, P->h_dof.data(), sizeof(int) * P->h_dof.size(), hipMemcpyHostToDevice, s));
+  OFH_HIP(hipMemcpyAsync(P->code, h_code, sizeof(int) * P->h_dof.size(), hipMemcpyHostToDevice, s));
+  OFH_HIP(hipMemcpyAsync(P->row_node, rnode.data(), sizeof(int) * neq, hipMemcpyHostToDevice, s));
+  OFH_HIP(hipMemcpyAsync(P->row_comp, rcomp.data(), sizeof(int) * neq, hipMemcpyHostToDevice, s));
+  int rc = reset_err(P);
+  if (rc) return rc;
+
+  // node -> element adjacency
+  const int tot = nb_el * nen;
+  int* cnt;
+  OFH_HIP(hipMalloc(&cnt, sizeof(int) * (size_t)std::max(nb_nodes, 1)));
+  OFH_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * nb_nodes, s));
+  hipLaunchKernelGGL(k_adj_count, dim3((tot + 255) / 256), dim3(256), 0, s, nb_el, nen, P->elem, cnt);
+  std::vector<int64_t> aptr;
+  if ((rc = host_exclusive_scan(s, cnt, nb_nodes, aptr))) return rc;
+  std::vector<int> aptr32(aptr.begin(), aptr.end());
+  OFH_HIP(hipMalloc(&P->adj_ptr, sizeof(int) * (nb_nodes + 1)));
+  OFH_HIP(hipMalloc(&P->adj_el, sizeof(int) * tot));
+  OFH_HIP(hipMalloc(&P->adj_loc, sizeof(int) * tot));
+  OFH_HIP(hipMemcpyAsync(P->adj_ptr, aptr32.data(), sizeof(int) * (nb_nodes + 1), hipMemcpyHostToDevice, s));
+  OFH_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * nb_nodes, s));
+  hipLaunchKernelGGL(k_adj_fill, dim3((tot + 255) / 256), dim3(256), 0, s, nb_el, nen, P->elem,
+                     P->adj_ptr, cnt, P->adj_el, P->adj_loc);
+  hipLaunchKernelGGL(k_adj_sort, dim3((nb_nodes + 255) / 256), dim3(256), 0, s, nb_nodes,
+                     P->adj_ptr, P->adj_el, P->adj_loc);
+  OFH_HIP(hipFree(cnt));
+
+  // CSR pattern
+  int* rlen;
+  OFH_HIP(hipMalloc(&rlen, sizeof(int) * neq));
+  hipLaunchKernelGGL(k_row_count, dim3((neq + 255) / 256), dim3(256), 0, s, neq, nen, nb_dof,
+                     P->elem, P->dof, P->adj_ptr, P->adj_el, P->row_node, rlen, P->err);
+  if ((rc = check_err(P, "NodeGraph"))) return rc;
+  std::vector<int64_t> rptr;
+  if ((rc = host_exclusive_scan(s, rlen, neq, rptr))) return rc;
+  OFH_HIP(hipFree(rlen));
+  const int64_t nnz = rptr[neq];
+  OFH_CHECK(nnz < (int64_t)0x7fffffff, OFH_E_INVALID, "ofh_asm_create: nnz exceeds 2^31");
+  std::vector<int> rp32(rptr.begin(), rptr.end());
+  ofh_csr A = new ofh_csr_s();
+  A->ctx = ctx;
+  A->n = neq;
+  A->nnz = nnz;
+  OFH_HIP(hipMalloc(&A->row_ptr, sizeof(int) * (neq + 1)));
+  OFH_HIP(hipMalloc(&A->col, sizeof(int) * std::max<int64_t>(nnz, 1)));
+  OFH_HIP(hipMalloc(&A->val, sizeof(double) * std::max<int64_t>(nnz, 1)));
+  OFH_HIP(hipMemsetAsync(A->val, 0, sizeof(double) * std::max<int64_t>(nnz, 1), s));
+  OFH_HIP(hipMemcpyAsync(A->row_ptr, rp32.data(), sizeof(int) * (neq + 1), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_row_fill, dim3((neq + 255) / 256), dim3(256), 0, s, neq, nen, nb_dof,
+                     P->elem, P->dof, P->adj_ptr, P->adj_el, P->row_node, A->row_ptr, A->col);
+  // row blocks of the CSR-stream SpMV
+  {
+    std::vector<int> rb;
+    rb.push_back(0);
+    int r = 0, maxr = 0;
+    for (int i = 0; i < neq; ++i) maxr = std::max(maxr, rp32[i + 1] - rp32[i]);
+    while (r < neq) {
+      const int start = r;
+      const int64_t base = rp32[r];
+      while (r < neq && r - start < kBlock && (int64_t)rp32[r + 1] - base <= kTile64) ++r;
+      if (r == start) ++r;
+      rb.push_back(r);
+    }
+    A->nblocks = (int)rb.size() - 1;
+    A->max_row_nnz = maxr;
+    OFH_CHECK(A->nblocks <= kMaxPartials, OFH_E_INVALID, "ofh_asm_create: matrix too large");
+    OFH_HIP(hipMalloc(&A->row_blocks, sizeof(int) * rb.size()));
+    OFH_HIP(hipMemcpyAsync(A->row_blocks, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice, s));
+    OFH_HIP(hipStreamSynchronize(s));
+  }
+  P->A = A;
+
+  // gather map
+  int* mcnt;
+  OFH_HIP(hipMalloc(&mcnt, sizeof(int) * std::max<int64_t>(nnz, 1)));
+  OFH_HIP(hipMemsetAsync(mcnt, 0, sizeof(int) * nnz, s));
+  const int nee = P->nee;
+  hipLaunchKernelGGL(k_map<false>, dim3((neq + 255) / 256), dim3(256), 0, s, neq, nen, nb_dof, nee,
+                     P->elem, P->dof, P->adj_ptr, P->adj_el, P->adj_loc, P->row_node, P->row_comp,
+                     A->row_ptr, A->col, mcnt, (const int64_t*)nullptr, (int*)nullptr);
+  std::vector<int64_t> mptr;
+  if ((rc = host_exclusive_scan(s, mcnt, (int)nnz, mptr))) return rc;
+  OFH_HIP(hipMalloc(&P->map_ptr, sizeof(int64_t) * (nnz + 1)));
+  OFH_HIP(hipMalloc(&P->map, sizeof(int) * std::max<int64_t>(mptr[nnz], 1)));
+  OFH_HIP(hipMemcpyAsync(P->map_ptr, mptr.data(), sizeof(int64_t) * (nnz + 1), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_map<true>, dim3((neq + 255) / 256), dim3(256), 0, s, neq, nen, nb_dof, nee,
+                     P->elem, P->dof, P->adj_ptr, P->adj_el, P->adj_loc, P->row_node, P->row_comp,
+                     A->row_ptr, A->col, mcnt, P->map_ptr, P->map);
+  OFH_HIP(hipGetLastError());
+  OFH_HIP(hipStreamSynchronize(s));
+  OFH_HIP(hipFree(mcnt));
+  OFH_HIP(hipMalloc(&P->ke, sizeof(double) * (size_t)nb_el * nee * nee));
+  OFH_HIP(hipMalloc(&P->erhs, sizeof(double) * (size_t)nb_el * nee));
+  *out = P;
+  return OFH_OK;
+}
+
+int ofh_asm_destroy(ofh_asm P) {
+  if (!P) return OFH_OK;
+  hipDeviceSynchronize();
+  void* ptrs[] = {P->coord, P->elem, P->dof, P->code, P->adj_ptr, P->adj_el, P->adj_loc,
+                  P->row_node, P->row_comp, P->map_ptr, P->map, P->ke, P->erhs, P->err};
+  for (void* p : ptrs) hipFree(p);
+  ofh_csr_destroy(P->A);
+  delete P;
+  return OFH_OK;
+}
+
+int ofh_asm_info(ofh_asm P, int* nb_eq, int64_t* nnz) {
+  OFH_CHECK(P, OFH_E_INVALID, "null plan");
+  if (nb_eq) *nb_eq = P->nb_eq;
+  if (nnz) *nnz = P->A->nnz;
+  return OFH_OK;
+}
+
+int ofh_asm_matrix(ofh_asm P, ofh_csr* A) {
+  OFH_CHECK(P && A, OFH_E_INVALID, "null argument");
+  *A = P->A;
+  return OFH_OK;
+}
+
+int ofh_asm_get_dof(ofh_asm P, int32_t* h_dof) {
+  OFH_CHECK(P && h_dof, OFH_E_INVALID, "null argument");
+  std::memcpy(h_dof, P->h_dof.data(), sizeof(int) * P->h_dof.size());
+  return OFH_OK;
+}
+
+int ofh_asm_build(ofh_asm P, int kind, const double* d_coef, const double* d_f,
+                  const double* d_bc, const double* d_uprev, const double* h_prm, int nprm,
+                  int want_matrix, double* d_b) {
+  OFH_CHECK(P, OFH_E_INVALID, "ofh_asm_build: null plan");
+  const bool tri = kind == OFH_LAPLACE2DT3 || kind == OFH_DC2DT3 || kind == OFH_DC2DT3_BE;
+  OFH_CHECK((tri && P->nen == 3 && P->dim == 2 && P->nb_dof == 1) ||
+                (kind == OFH_LAPLACE3DT4 && P->nen == 4 && P->dim == 3 && P->nb_dof == 1) ||
+                (kind == OFH_ELAS2DQ4 && P->nen == 4 && P->dim == 2 && P->nb_dof == 2),
+            OFH_E_INVALID, "ofh_asm_build: element kind does not match the mesh");
+  OFH_CHECK(kind == OFH_LAPLACE2DT3 || kind == OFH_LAPLACE3DT4 || d_coef, OFH_E_INVALID,
+            "ofh_asm_build: this kind needs a per-element coefficient");
+  OFH_CHECK(kind != OFH_ELAS2DQ4 || d_f == nullptr, OFH_E_UNSUPPORTED,
+            "ofh_asm_build: Elas2DQ4 body forces are not provided (use tractions)");
+  OFH_CHECK(kind != OFH_DC2DT3_BE || (h_prm && nprm >= 3), OFH_E_INVALID,
+            "ofh_asm_build: DC2DT3_BE needs {dt, rho, cp}");
+  OFH_CHECK(kind != OFH_ELAS2DQ4 || (h_prm && nprm >= 2), OFH_E_INVALID,
+            "ofh_asm_build: ELAS2DQ4 needs {nu, mode}");
+  hipStream_t s = P->ctx->stream;
+  int rc = reset_err(P);
+  if (rc) return rc;
+  ElemArgs a{};
+  a.kind = kind;
+  a.nen = P->nen;
+  a.nb_dof = P->nb_dof;
+  a.nee = P->nee;
+  a.dim = P->dim;
+  a.coord = P->coord;
+  a.elem = P->elem;
+  a.code = P->code;
+  a.coef = d_coef;
+  a.f = d_f;
+  a.bc = d_bc;
+  a.uprev = d_uprev;
+  a.p0 = nprm > 0 ? h_prm[0] : 0.0;
+  a.p1 = nprm > 1 ? h_prm[1] : 0.0;
+  a.p2 = nprm > 2 ? h_prm[2] : 0.0;
+  double gx[2], gw[2];
+  ofh_gauss(2, gx, gw);
+  a.g0 = gx[0];
+  a.g1 = gx[1];
+  a.w0 = gw[0];
+  a.w1 = gw[1];
+  a.ke = want_matrix ? P->ke : nullptr;
+  a.erhs = P->erhs;
+  a.err = P->err;
+  const dim3 grid((P->nb_el + 255) / 256), blk(256);
+  switch (kind) {
+    case OFH_LAPLACE2DT3: hipLaunchKernelGGL(k_element<OFH_LAPLACE2DT3>, grid, blk, 0, s, P->nb_el, a); break;
+    case OFH_DC2DT3: hipLaunchKernelGGL(k_element<OFH_DC2DT3>, grid, blk, 0, s, P->nb_el, a); break;
+    case OFH_DC2DT3_BE: hipLaunchKernelGGL(k_element<OFH_DC2DT3_BE>, grid, blk, 0, s, P->nb_el, a); break;
+    case OFH_LAPLACE3DT4: hipLaunchKernelGGL(k_element<OFH_LAPLACE3DT4>, grid, blk, 0, s, P->nb_el, a); break;
+    case OFH_ELAS2DQ4: hipLaunchKernelGGL(k_element<OFH_ELAS2DQ4>, grid, blk, 0, s, P->nb_el, a); break;
+    default: return fail(OFH_E_INVALID, "ofh_asm_build: unknown kind");
+  }
+  OFH_HIP(hipGetLastError());
+  if ((rc = check_err(P, "element loop"))) return rc;
+  if (want_matrix) {
+    ofh_csr A = P->A;
+    hipLaunchKernelGGL(k_gather_matrix, dim3((unsigned)((A->nnz + 255) / 256)), dim3(256), 0, s,
+                       A->nnz, P->map_ptr, P->map, P->ke, A->val);
+    A->val32_valid = false;
+    A->dinv_valid = false;
+  }
+  if (d_b)
+    hipLaunchKernelGGL(k_gather_rhs, dim3((P->nb_eq + 255) / 256), dim3(256), 0, s, P->nb_eq,
+                       P->nb_dof, P->nee, P->row_node, P->row_comp, P->adj_ptr, P->adj_el,
+                       P->adj_loc, P->erhs, d_b);
+  OFH_HIP(hipGetLastError());
+  return OFH_OK;
+}
+
+int ofh_asm_traction(ofh_asm P, int nb_sides, const int32_t* h_side_nodes,
+                     const double* d_traction, double* d_b) {
+  OFH_CHECK(P && h_side_nodes && d_traction && d_b && nb_sides >= 0, OFH_E_INVALID,
+            "ofh_asm_traction: bad argument");
+  OFH_CHECK(P->dim == 2 && P->nb_dof == 2, OFH_E_INVALID, "ofh_asm_traction: 2D, 2 DOF meshes");
+  for (int t = 0; t < 2 * nb_sides; ++t)
+    OFH_CHECK(h_side_nodes[t] >= 1 && h_side_nodes[t] <= P->nb_nodes, OFH_E_INVALID,
+              "ofh_asm_traction: side node out of range");
+  if (nb_sides == 0) return OFH_OK;
+  hipStream_t s = P->ctx->stream;
+  int* ds;
+  OFH_HIP(hipMalloc(&ds, sizeof(int) * 2 * nb_sides));
+  OFH_HIP(hipMemcpyAsync(ds, h_side_nodes, sizeof(int) * 2 * nb_sides, hipMemcpyHostToDevice, s));
+  int rc = reset_err(P);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_traction, dim3(1), dim3(64), 0, s, nb_sides, ds, P->coord, d_traction,
+                     P->dof, d_b, P->err);
+  OFH_HIP(hipGetLastError());
+  rc = check_err(P, "Line2");
+  hipFree(ds);
+  return rc;
+}
+
+int ofh_asm_insert_bc(ofh_asm P, const double* d_v, const double* d_bc, double* d_u) {
+  OFH_CHECK(P && d_v && d_u, OFH_E_INVALID, "ofh_asm_insert_bc: null argument");
+  const int m = P->nb_nodes * P->nb_dof;
+  hipLaunchKernelGGL(k_insert_bc, dim3((m + 255) / 256), dim3(256), 0, P->ctx->stream, m, P->dof,
+                     d_v, d_bc, d_u);
+  OFH_HIP(hipGetLastError());
+  return OFH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,202 @@
+// ofh_internal.h -- shared internals of libofeli_hip (HIP, gfx950 only).
+//
+// Device conventions: CSR with int32 row_ptr (0-based) and int32 col (0-based),
+// FP64 values plus an optional FP32 mirror.  Every reduction is a fixed-order
+// two-stage tree (wave shuffle -> LDS -> per-block partial -> the last block
+// to finish folds the partials in block order), so a given launch shape
+// always produces bitwise identical results.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include <string>
+
+#include "../../include/ofeli_hip.h"
+
+namespace ofh {
+
+// ---------------------------------------------------------------------------
+// error handling
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define OFH_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t _e = (call);                                                    \
+    if (_e != hipSuccess)                                                      \
+      return ::ofh::fail(OFH_E_HIP, std::string(#call) + ": " +              \
+                                        hipGetErrorString(_e));               \
+  } while (0)
+
+#define OFH_CHECK(cond, code, msg)                                             \
+  do {                                                                         \
+    if (!(cond)) return ::ofh::fail((code), (msg));                            \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// launch geometry
+// ---------------------------------------------------------------------------
+constexpr int kBlock = 256;           // threads per workgroup for stream kernels
+constexpr int kWave = 64;             // CDNA wavefront
+constexpr int kMaxPartials = 1 << 16; // per-reduction partial slots (max workgroups of a reduction)
+constexpr int kTile64 = 4096;         // LDS product tile of the CSR-stream SpMV (FP64)
+constexpr int kTile32 = 4096;         // same, FP32
+constexpr int kNumXcd = 8;            // MI355X: 8 XCDs, workgroups dealt round-robin
+
+// Workgroups are dispatched round-robin over the XCDs; remap the hardware
+// block id so that each XCD walks one contiguous range of row blocks, which
+// keeps the gathered x window of neighbouring rows inside one L2.
+__device__ __forceinline__ int xcd_block(int bid, int nblocks) {
+  int per = (nblocks + kNumXcd - 1) / kNumXcd;
+  int xcd = bid % kNumXcd, slot = bid / kNumXcd;
+  int full = nblocks - (per - 1) * kNumXcd;  // XCDs that own `per` blocks
+  int base = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
+  return base + slot;
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide sum of BLOCK threads; result valid in every thread.
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double* lds /*BLOCK/64*/) {
+  v = wave_sum(v);
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  __syncthreads();
+  if (l == 0) lds[w] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < BLOCK / kWave; ++i) s += lds[i];
+  return s;
+}
+
+// Three sums at once (CG update kernel).
+template <int BLOCK, int K>
+__device__ __forceinline__ void block_sum_k(double (&v)[K], double* lds /*K*BLOCK/64*/) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  __syncthreads();
+  if (l == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) lds[k * (BLOCK / kWave) + w] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < BLOCK / kWave; ++i) s += lds[k * (BLOCK / kWave) + i];
+    v[k] = s;
+  }
+}
+
+// Publish this block's K partials; returns true in every thread of the one
+// block that finished last.  The counter is reset by that block.
+template <int K>
+__device__ __forceinline__ bool publish_partials(const double (&v)[K], double* partials,
+                                                 unsigned* counter, int bid, int nblocks,
+                                                 bool* lds_flag) {
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) partials[k * kMaxPartials + bid] = v[k];
+    __threadfence();
+    unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    *lds_flag = (t == (unsigned)nblocks - 1);
+  }
+  __syncthreads();
+  bool last = *lds_flag;
+  if (last) __threadfence();
+  return last;
+}
+
+// Fold the partials in block order (fixed per launch shape).
+template <int BLOCK, int K>
+__device__ __forceinline__ void fold_partials(double (&out)[K], const double* partials,
+                                              int nblocks, double* lds) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nblocks; i += BLOCK)
+      s += __hip_atomic_load(partials + k * kMaxPartials + i, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    out[k] = s;
+  }
+  block_sum_k<BLOCK, K>(out, lds);
+}
+
+// ---------------------------------------------------------------------------
+// objects behind the opaque handles
+// ---------------------------------------------------------------------------
+struct Workspace {
+  double* partials = nullptr;   // 4 * kMaxPartials doubles
+  unsigned* counters = nullptr; // 64 counters, zero between launches
+};
+
+}  // namespace ofh
+
+struct ofh_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int num_cu = 0;
+  ofh::Workspace ws;
+  void* host_pinned = nullptr;  // 4 KiB staging for scalars
+  double* dev_scratch = nullptr;  // small per-solve state (4 KiB)
+  void* vws = nullptr;            // solver work vectors, grown on demand
+  size_t vws_bytes = 0;
+};
+
+struct ofh_csr_s {
+  ofh_ctx ctx = nullptr;
+  int n = 0;
+  int64_t nnz = 0;
+  int* row_ptr = nullptr;   // device, n+1
+  int* col = nullptr;       // device, nnz (0-based)
+  double* val = nullptr;    // device, nnz
+  float* val32 = nullptr;   // device, nnz (lazy FP32 mirror)
+  int* row_blocks = nullptr;  // device, nblocks+1 (CSR-stream row partition)
+  int nblocks = 0;
+  int max_row_nnz = 0;
+  double* dinv = nullptr;     // device, n (DIAG_PREC pivots; lazy)
+  float* dinv32 = nullptr;
+  int* diag_pos = nullptr;    // device, n (position of a_ii; lazy)
+  bool dinv_valid = false, val32_valid = false;
+  // level schedule for the SSOR sweeps (lazy)
+  int* lvl_rows_fwd = nullptr;
+  int* lvl_ptr_fwd = nullptr;
+  int* lvl_rows_bwd = nullptr;
+  int* lvl_ptr_bwd = nullptr;
+  int nlev_fwd = 0, nlev_bwd = 0, max_lev_width = 0;
+  int* h_lvl_ptr_fwd = nullptr;  // host copies of the level pointers
+  int* h_lvl_ptr_bwd = nullptr;
+  double* ssor_tmp = nullptr;    // forward-sweep result of the SSOR application
+};
+
+namespace ofh {
+int ensure_vws(ofh_ctx ctx, size_t bytes, void** out);
+int reduce_grid(int n);
+int launch_dot(ofh_ctx ctx, int n, const double* x, const double* y, double* d_out, bool sqrt_out);
+int csr_ensure_val32(ofh_csr A);
+int csr_ensure_dinv(ofh_csr A);
+int csr_ensure_levels(ofh_csr A);
+
+// SpMV launchers (ofh_spmv.hip)
+int launch_spmv64(ofh_csr A, const double* x, double* y, hipStream_t s);
+int launch_spmv32(ofh_csr A, const float* x, float* y, hipStream_t s);
+// SSOR sweeps z = M^-1 r (ofh_ssor.hip); skipped when *d_done != 0.
+int launch_ssor(ofh_csr A, const double* r, double* z, const int* d_done, hipStream_t s);
+// x <- b - A x style true residual ||b-Ax||/||b|| in FP64 (host result).
+int true_relres(ofh_csr A, const double* b, const double* x, double* out);
+double now_seconds();
+}  // namespace ofh

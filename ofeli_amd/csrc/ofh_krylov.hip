@@ -1,0 +1,825 @@
+// ofh_krylov.hip -- BiCGStab, GMRes(m), Prec::solve and the LinearSolver
+// dispatch on the GPU, plus the solve report shared by every solver.
+//
+// References: BiCGStab.h:87-162, GMRes.h:67-241, Prec_impl.h:226-255,
+// LinearSolver.cpp:194-244.  As in ofh_cg.hip, the scalars live on the device
+// and every kernel returns at once when the stop flag is set; the host
+// synchronises once per chunk (BiCGStab) or per restart cycle (GMRes).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "ofh_internal.h"
+#include "ofh_solver.h"
+#include "ofh_spmv.h"
+
+namespace ofh {
+
+// ---------------------------------------------------------------------------
+// true residual ||b - A x|| / ||b|| (FP64, no temporary vector)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_true_res(int nblocks, const int* __restrict__ rb,
+                                                    const int* __restrict__ rp,
+                                                    const int* __restrict__ col,
+                                                    const double* __restrict__ val,
+                                                    const double* __restrict__ b,
+                                                    const double* __restrict__ x,
+                                                    double* partials, unsigned* counter,
+                                                    double* out) {
+  __shared__ double lds[kTile64];
+  __shared__ double red[2 * kBlock / kWave];
+  __shared__ bool last;
+  const int blk = xcd_block(blockIdx.x, nblocks);
+  const int r0 = rb[blk], r1 = rb[blk + 1];
+  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
+  const int i = r0 + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < r1) {
+    const double d = b[i] - s;
+    v[0] = d * d;
+    v[1] = b[i] * b[i];
+  }
+  block_sum_k<kBlock, 2>(v, red);
+  if (publish_partials<2>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
+    double t[2];
+    fold_partials<kBlock, 2>(t, partials, gridDim.x, red);
+    if (threadIdx.x == 0) {
+      double nb = sqrt(t[1]);
+      if (nb == 0) nb = 1;
+      *out = sqrt(t[0]) / nb;
+      *counter = 0u;
+    }
+  }
+}
+
+int true_relres(ofh_csr A, const double* b, const double* x, double* out) {
+  ofh_ctx ctx = A->ctx;
+  double* d_out = ctx->dev_scratch + 16;
+  hipLaunchKernelGGL(k_true_res, dim3(A->nblocks), dim3(kBlock), 0, ctx->stream, A->nblocks,
+                     A->row_blocks, A->row_ptr, A->col, A->val, b, x, ctx->ws.partials,
+                     ctx->ws.counters + 8, d_out);
+  OFH_HIP(hipGetLastError());
+  OFH_HIP(hipMemcpyAsync(ctx->host_pinned, d_out, sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  OFH_HIP(hipStreamSynchronize(ctx->stream));
+  *out = *(double*)ctx->host_pinned;
+  return OFH_OK;
+}
+
+int fill_info(ofh_csr A, const double* b, const double* x, const KrylovState& h, double t0,
+              ofh_solve_info* info) {
+  OFH_HIP(hipStreamSynchronize(A->ctx->stream));
+  const double t1 = now_seconds();
+  if (!info) return OFH_OK;
+  info->nb_it = h.done ? h.status : -(h.max_it + 1);
+  info->iterations = h.it;
+  info->relres = h.res;
+  info->outer = 0;
+  info->fallback = 0;
+  info->seconds = t1 - t0;
+  return true_relres(A, b, x, &info->true_relres);
+}
+
+// ---------------------------------------------------------------------------
+// element-wise helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool halted(const KrylovState* st) { return st->halt != 0; }
+
+// Prec::solve(b,x) (Prec_impl.h:226-255) for IDENT and DIAG.
+__global__ void k_prec_diag(int n, const double* __restrict__ b, const double* __restrict__ dinv,
+                            double* __restrict__ x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = dinv ? b[i] * dinv[i] : b[i];
+}
+
+// ===========================================================================
+// BiCGStab
+// ===========================================================================
+// t = A x, r = b - t, rr = r; (b,b), (r,r)
+__global__ __launch_bounds__(kBlock) void k_bi_init(int nblocks, const int* __restrict__ rb,
+                                                   const int* __restrict__ rp,
+                                                   const int* __restrict__ col,
+                                                   const double* __restrict__ val,
+                                                   const double* __restrict__ b,
+                                                   const double* __restrict__ x,
+                                                   double* __restrict__ r,
+                                                   double* __restrict__ rr, KrylovState* st,
+                                                   double* partials, unsigned* counter) {
+  __shared__ double lds[kTile64];
+  __shared__ double red[2 * kBlock / kWave];
+  __shared__ bool last;
+  const int blk = xcd_block(blockIdx.x, nblocks);
+  const int r0 = rb[blk], r1 = rb[blk + 1];
+  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
+  const int i = r0 + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < r1) {
+    const double ri = b[i] - s;
+    r[i] = ri;
+    rr[i] = ri;
+    v[0] = b[i] * b[i];
+    v[1] = ri * ri;
+  }
+  block_sum_k<kBlock, 2>(v, red);
+  if (publish_partials<2>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
+    double t[2];
+    fold_partials<kBlock, 2>(t, partials, gridDim.x, red);
+    if (threadIdx.x == 0) {
+      double nrm = sqrt(t[0]);
+      if (nrm == 0.0) nrm = 1;  // BiCGStab.h:109-110
+      st->nrm = nrm;
+      st->res = sqrt(t[1]) / nrm;
+      st->res0 = st->res;
+      st->it = 0;
+      st->sconv = 0;
+      st->halt = 0;
+      if (st->res <= st->tol) {  // BiCGStab.h:111
+        st->done = 1;
+        st->halt = 1;
+        st->status = 0;
+      } else {
+        st->rho = t[1];  // (rr,r) of the first step: rr == r
+        if (st->rho == 0.0) {
+          st->done = 1;
+          st->halt = 1;
+          st->status = 2;
+        }
+      }
+      *counter = 0u;
+    }
+  }
+}
+
+// p = r (first step) or r + beta (p - omega v); pp = P p (BiCGStab.h:122-129)
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_bi_dir(int n, const double* __restrict__ r,
+                                                  const double* __restrict__ v,
+                                                  double* __restrict__ p, double* __restrict__ pp,
+                                                  const double* __restrict__ dinv,
+                                                  const KrylovState* __restrict__ st) {
+  if (st->done) return;
+  const bool first = st->it == 0;
+  const double beta = st->beta, omega = st->omega;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double pi = first ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
+    p[i] = pi;
+    if (PREC == OFH_DIAG_PREC) pp[i] = pi * dinv[i];
+    else if (PREC == OFH_IDENT_PREC) pp[i] = pi;
+  }
+}
+
+// y = A x with one or two fused dot products u1.y (u2.y) -> scalars.
+// MODE 0: v = A pp, alpha = rho/(rr,v)          (BiCGStab.h:130-131)
+// MODE 1: t = A ss, omega = (t,s)/(t,t)          (BiCGStab.h:140-141)
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_bi_spmv(int nblocks, const int* __restrict__ rb,
+                                                   const int* __restrict__ rp,
+                                                   const int* __restrict__ col,
+                                                   const double* __restrict__ val,
+                                                   const double* __restrict__ x,
+                                                   double* __restrict__ y,
+                                                   const double* __restrict__ u,
+                                                   KrylovState* st, double* partials,
+                                                   unsigned* counter) {
+  if (MODE == 0 ? st->done : halted(st)) return;
+  __shared__ double lds[kTile64];
+  __shared__ double red[2 * kBlock / kWave];
+  __shared__ bool last;
+  const int blk = xcd_block(blockIdx.x, nblocks);
+  const int r0 = rb[blk], r1 = rb[blk + 1];
+  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
+  const int i = r0 + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < r1) {
+    y[i] = s;
+    if (MODE == 0) {
+      v[0] = u[i] * s;  // (rr, v)
+    } else {
+      v[0] = s * u[i];  // (t, s)
+      v[1] = s * s;     // (t, t)
+    }
+  }
+  block_sum_k<kBlock, 2>(v, red);
+  if (publish_partials<2>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
+    double t[2];
+    fold_partials<kBlock, 2>(t, partials, gridDim.x, red);
+    if (threadIdx.x == 0) {
+      if (MODE == 0) st->alpha = st->rho / t[0];
+      else st->omega = t[0] / t[1];
+      *counter = 0u;
+    }
+  }
+}
+
+// s = r - alpha v, ||s||; early exit flag; ss = P s (BiCGStab.h:132-139)
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_bi_s(int n, const double* __restrict__ r,
+                                                const double* __restrict__ v,
+                                                double* __restrict__ s, double* __restrict__ ss,
+                                                const double* __restrict__ dinv,
+                                                KrylovState* st, double* partials,
+                                                unsigned* counter) {
+  if (st->done) return;
+  __shared__ double red[kBlock / kWave];
+  __shared__ bool last;
+  const double alpha = st->alpha;
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double si = r[i] - alpha * v[i];
+    s[i] = si;
+    acc[0] += si * si;
+    if (PREC == OFH_DIAG_PREC) ss[i] = si * dinv[i];
+    else if (PREC == OFH_IDENT_PREC) ss[i] = si;
+  }
+  block_sum_k<kBlock, 1>(acc, red);
+  if (publish_partials<1>(acc, partials, counter, blockIdx.x, gridDim.x, &last)) {
+    double t[1];
+    fold_partials<kBlock, 1>(t, partials, gridDim.x, red);
+    if (threadIdx.x == 0) {
+      const double res = sqrt(t[0]) / st->nrm;
+      if (res < st->tol) {
+        st->sconv = 1;
+        st->halt = 1;
+        st->sconv_res = res;
+      }
+      *counter = 0u;
+    }
+  }
+}
+
+// x += alpha pp + omega ss; r = s - omega t; ||r||, (rr,r); stop tests and the
+// next step's beta (BiCGStab.h:117-121, 142-158).
+__global__ __launch_bounds__(kBlock) void k_bi_upd(int n, double* __restrict__ x,
+                                                  double* __restrict__ r,
+                                                  const double* __restrict__ rr,
+                                                  const double* __restrict__ pp,
+                                                  const double* __restrict__ s,
+                                                  const double* __restrict__ ss,
+                                                  const double* __restrict__ t,
+                                                  KrylovState* st, double* partials,
+                                                  unsigned* counter) {
+  if (st->done) return;
+  __shared__ double red[2 * kBlock / kWave];
+  __shared__ bool last;
+  const double alpha = st->alpha, omega = st->omega;
+  const bool sconv = st->sconv;
+  double v[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    if (sconv) {
+      x[i] += alpha * pp[i];
+    } else {
+      x[i] += alpha * pp[i] + omega * ss[i];
+      const double ri = s[i] - omega * t[i];
+      r[i] = ri;
+      v[0] += ri * ri;
+      v[1] += rr[i] * ri;
+    }
+  }
+  block_sum_k<kBlock, 2>(v, red);
+  if (publish_partials<2>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
+    double tt[2];
+    fold_partials<kBlock, 2>(tt, partials, gridDim.x, red);
+    if (threadIdx.x == 0) {
+      const int it = st->it + 1;
+      st->it = it;
+      if (sconv) {
+        st->res = st->sconv_res;
+        st->done = 1;
+        st->halt = 1;
+        st->status = it;
+      } else {
+        const double res = sqrt(tt[0]) / st->nrm;
+        st->res = res;
+        if (res < st->tol) {
+          st->done = 1;
+          st->halt = 1;
+          st->status = it;
+        } else if (st->omega == 0.0) {
+          st->done = 1;
+          st->halt = 1;
+          st->status = 3;
+        } else if (it >= st->max_it || !(res == res)) {
+          st->done = 1;
+          st->halt = 1;
+          st->status = -(st->max_it + 1);
+        } else {
+          const double rho = tt[1];
+          if (rho == 0.0) {
+            st->done = 1;
+            st->halt = 1;
+            st->status = 2;
+          } else {
+            st->beta = (rho / st->rho) * (st->alpha / st->omega);
+            st->rho = rho;
+          }
+        }
+      }
+      *counter = 0u;
+    }
+  }
+}
+
+template <int PREC>
+static int bicgstab_impl(ofh_csr A, const double* b, double* x, int max_it, double toler,
+                         ofh_solve_info* info) {
+  ofh_ctx ctx = A->ctx;
+  hipStream_t s = ctx->stream;
+  const int n = A->n;
+  const double t0 = now_seconds();
+  int rc;
+  if (PREC == OFH_DIAG_PREC && (rc = csr_ensure_dinv(A))) return rc;
+  if (PREC == OFH_SSOR_PREC && (rc = csr_ensure_levels(A))) return rc;
+  void* wsp;
+  if ((rc = ensure_vws(ctx, sizeof(double) * (size_t)n * 8, &wsp))) return rc;
+  double* r = (double*)wsp;
+  double *rr = r + n, *p = rr + n, *pp = p + n, *v = pp + n, *sv = v + n, *ss = sv + n,
+         *t = ss + n;
+  KrylovState* st = state_ptr(ctx);
+  KrylovState h0{};
+  h0.tol = toler;
+  h0.max_it = max_it;
+  OFH_HIP(hipMemcpyAsync(st, &h0, sizeof(h0), hipMemcpyHostToDevice, s));
+  unsigned* c0 = ctx->ws.counters;
+  double* part = ctx->ws.partials;
+  const int g = reduce_grid(n);
+  hipLaunchKernelGGL(k_bi_init, dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks, A->row_blocks,
+                     A->row_ptr, A->col, A->val, b, x, r, rr, st, part, c0);
+  KrylovState h{};
+  int enq = 0, chunk = 4;
+  for (;;) {
+    for (int c = 0; c < chunk && enq < max_it; ++c, ++enq) {
+      hipLaunchKernelGGL(k_bi_dir<PREC>, dim3(g), dim3(kBlock), 0, s, n, r, v, p, pp, A->dinv, st);
+      if (PREC == OFH_SSOR_PREC && (rc = launch_ssor(A, p, pp, &st->done, s))) return rc;
+      hipLaunchKernelGGL(k_bi_spmv<0>, dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
+                         A->row_blocks, A->row_ptr, A->col, A->val, pp, v, rr, st, part, c0 + 1);
+      hipLaunchKernelGGL(k_bi_s<PREC>, dim3(g), dim3(kBlock), 0, s, n, r, v, sv, ss, A->dinv, st,
+                         part, c0 + 2);
+      if (PREC == OFH_SSOR_PREC && (rc = launch_ssor(A, sv, ss, &st->sconv, s))) return rc;
+      hipLaunchKernelGGL(k_bi_spmv<1>, dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
+                         A->row_blocks, A->row_ptr, A->col, A->val, ss, t, sv, st, part, c0 + 3);
+      hipLaunchKernelGGL(k_bi_upd, dim3(g), dim3(kBlock), 0, s, n, x, r, rr, pp, sv, ss, t, st,
+                         part, c0 + 4);
+    }
+    OFH_HIP(hipGetLastError());
+    if ((rc = read_state(ctx, &h))) return rc;
+    if (h.done || enq >= max_it) break;
+    chunk = next_chunk(h.it, h.res, h.res0, toler, max_it - enq);
+  }
+  return fill_info(A, b, x, h, t0, info);
+}
+
+// ===========================================================================
+// GMRes(m)
+// ===========================================================================
+// r = P (b - A x); (r,r), (b,b).  INIT: GMRes.h:164-179 (returns it=1 when
+// already converged); else the restart residual of GMRes.h:226-232.
+template <int PREC, bool INIT>
+__global__ __launch_bounds__(kBlock) void k_gm_resid(int nblocks, const int* __restrict__ rb,
+                                                    const int* __restrict__ rp,
+                                                    const int* __restrict__ col,
+                                                    const double* __restrict__ val,
+                                                    const double* __restrict__ b,
+                                                    const double* __restrict__ x,
+                                                    double* __restrict__ r,
+                                                    const double* __restrict__ dinv,
+                                                    KrylovState* st, double* partials,
+                                                    unsigned* counter) {
+  if (!INIT && st->done) return;
+  __shared__ double lds[kTile64];
+  __shared__ double red[2 * kBlock / kWave];
+  __shared__ bool last;
+  const int blk = xcd_block(blockIdx.x, nblocks);
+  const int r0 = rb[blk], r1 = rb[blk + 1];
+  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
+  const int i = r0 + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < r1) {
+    double ri = b[i] - s;
+    if (PREC == OFH_DIAG_PREC) ri *= dinv[i];
+    r[i] = ri;
+    v[0] = ri * ri;
+    v[1] = b[i] * b[i];
+  }
+  block_sum_k<kBlock, 2>(v, red);
+  if (publish_partials<2>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
+    double t[2];
+    fold_partials<kBlock, 2>(t, partials, gridDim.x, red);
+    if (threadIdx.x == 0 && PREC != OFH_SSOR_PREC) {
+      if (INIT) {
+        double nb = sqrt(t[1]);
+        if (nb == 0.0) nb = 1;
+        st->nrm = nb;
+        st->it = 1;
+        st->done = 0;
+        st->sconv = 0;
+        st->halt = 0;
+      }
+      st->h_norm = sqrt(t[0]);  // beta
+      const double res = st->h_norm / st->nrm;
+      st->res = res;
+      if (INIT) {
+        st->res0 = res;
+        if (res <= st->tol) {
+          st->done = 1;
+          st->halt = 1;
+          st->status = 1;
+        }
+      } else if (res < st->tol) {
+        st->done = 1;
+        st->halt = 1;
+        st->status = st->it;
+      }
+    }
+    if (threadIdx.x == 0) *counter = 0u;
+  }
+}
+
+// SSOR variant of the residual bookkeeping: r was written by the sweeps.
+template <bool INIT>
+__global__ __launch_bounds__(kBlock) void k_gm_resid_norm(int n, const double* __restrict__ r,
+                                                         const double* __restrict__ b,
+                                                         KrylovState* st, double* partials,
+                                                         unsigned* counter) {
+  if (!INIT && st->done) return;
+  __shared__ double red[2 * kBlock / kWave];
+  __shared__ bool last;
+  double v[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    v[0] += r[i] * r[i];
+    v[1] += b[i] * b[i];
+  }
+  block_sum_k<kBlock, 2>(v, red);
+  if (publish_partials<2>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
+    double t[2];
+    fold_partials<kBlock, 2>(t, partials, gridDim.x, red);
+    if (threadIdx.x == 0) {
+      if (INIT) {
+        double nb = sqrt(t[1]);
+        if (nb == 0.0) nb = 1;
+        st->nrm = nb;
+        st->it = 1;
+        st->done = 0;
+        st->sconv = 0;
+        st->halt = 0;
+      }
+      st->h_norm = sqrt(t[0]);
+      const double res = st->h_norm / st->nrm;
+      st->res = res;
+      if (INIT) {
+        st->res0 = res;
+        if (res <= st->tol) {
+          st->done = 1;
+          st->halt = 1;
+          st->status = 1;
+        }
+      } else if (res < st->tol) {
+        st->done = 1;
+        st->halt = 1;
+        st->status = st->it;
+      }
+      *counter = 0u;
+    }
+  }
+}
+
+// plain b - A x into w (SSOR paths precondition it with the sweeps)
+__global__ __launch_bounds__(kBlock) void k_resid_plain(int nblocks, const int* __restrict__ rb,
+                                                       const int* __restrict__ rp,
+                                                       const int* __restrict__ col,
+                                                       const double* __restrict__ val,
+                                                       const double* __restrict__ b,
+                                                       const double* __restrict__ x,
+                                                       double* __restrict__ w,
+                                                       const KrylovState* __restrict__ st,
+                                                       int check) {
+  if (check && st->done) return;
+  __shared__ double lds[kTile64];
+  const int blk = xcd_block(blockIdx.x, nblocks);
+  const int r0 = rb[blk], r1 = rb[blk + 1];
+  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
+  const int i = r0 + threadIdx.x;
+  if (i < r1) w[i] = b[i] - s;
+}
+
+// v0 = (1/beta) r; s = 0; s[0] = beta (GMRes.h:187-189)
+__global__ void k_gm_v0(int n, int m, const double* __restrict__ r, double* __restrict__ v0,
+                        double* __restrict__ sv, const KrylovState* __restrict__ st) {
+  if (st->done) return;
+  const double beta = st->h_norm, sc = 1.0 / beta;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v0[i] = sc * r[i];
+  if (i <= m) sv[i] = i == 0 ? beta : 0.0;
+}
+
+// w = P (A v_i)   (GMRes.h:191-192)
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_gm_av(int nblocks, const int* __restrict__ rb,
+                                                 const int* __restrict__ rp,
+                                                 const int* __restrict__ col,
+                                                 const double* __restrict__ val,
+                                                 const double* __restrict__ vi,
+                                                 double* __restrict__ w,
+                                                 const double* __restrict__ dinv,
+                                                 const KrylovState* __restrict__ st) {
+  if (halted(st)) return;
+  __shared__ double lds[kTile64];
+  const int blk = xcd_block(blockIdx.x, nblocks);
+  const int r0 = rb[blk], r1 = rb[blk + 1];
+  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, vi, lds, 0.0);
+  const int i = r0 + threadIdx.x;
+  if (i < r1) w[i] = PREC == OFH_DIAG_PREC ? s * dinv[i] : s;
+}
+
+// Modified Gram-Schmidt step k (GMRes.h:193-197), fused with the previous
+// step's update:  w -= H(k-1,i) v_{k-1};  H(k,i) = (w, v_k).
+// k == i+1 (v_k == null): the final update and H(i+1,i) = ||w||.
+__global__ __launch_bounds__(kBlock) void k_gm_mgs(int n, int k, int i, int m1,
+                                                  double* __restrict__ w,
+                                                  const double* __restrict__ vprev,
+                                                  const double* __restrict__ vk,
+                                                  double* __restrict__ H, KrylovState* st,
+                                                  double* partials, unsigned* counter) {
+  if (halted(st)) return;
+  __shared__ double red[kBlock / kWave];
+  __shared__ bool last;
+  const double h = k > 0 ? H[(k - 1) * m1 + i] : 0.0;
+  double v[1] = {0.0};
+  for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+    double wj = w[j];
+    if (k > 0) {
+      wj -= h * vprev[j];
+      w[j] = wj;
+    }
+    v[0] += vk ? wj * vk[j] : wj * wj;
+  }
+  block_sum_k<kBlock, 1>(v, red);
+  if (publish_partials<1>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
+    double t[1];
+    fold_partials<kBlock, 1>(t, partials, gridDim.x, red);
+    if (threadIdx.x == 0) {
+      H[k * m1 + i] = vk ? t[0] : sqrt(t[0]);
+      *counter = 0u;
+    }
+  }
+}
+
+// v_{i+1} = (1/H(i+1,i)) w  (GMRes.h:199)
+__global__ void k_gm_scale(int n, int i, int m1, const double* __restrict__ w,
+                           double* __restrict__ vnext, const double* __restrict__ H,
+                           const KrylovState* __restrict__ st) {
+  if (halted(st)) return;
+  const double sc = 1.0 / H[(i + 1) * m1 + i];
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) vnext[j] = sc * w[j];
+}
+
+__device__ void gm_rotate(double& dx, double& dy, double cs, double sn) {
+  const double temp = cs * dx + sn * dy;  // GMRes.h:67-76
+  dy = cs * dy - sn * dx;
+  dx = temp;
+}
+
+// Givens rotations and the residual estimate (GMRes.h:200-210)
+__global__ void k_gm_givens(int i, int m1, double* H, double* sv, double* cs, double* sn,
+                            KrylovState* st) {
+  if (halted(st) || threadIdx.x != 0) return;
+  for (int k = 0; k < i; k++) gm_rotate(H[k * m1 + i], H[(k + 1) * m1 + i], cs[k], sn[k]);
+  const double dx = H[i * m1 + i], dy = H[(i + 1) * m1 + i];
+  double c, s;
+  if (dy == 0.0) {  // set_rotation, GMRes.h:97-115
+    c = 1.0;
+    s = 0.0;
+  } else if (fabs(dy) > fabs(dx)) {
+    const double temp = dx / dy;
+    s = 1.0 / sqrt(1.0 + temp * temp);
+    c = temp * s;
+  } else {
+    const double temp = dy / dx;
+    c = 1.0 / sqrt(1.0 + temp * temp);
+    s = temp * c;
+  }
+  cs[i] = c;
+  sn[i] = s;
+  gm_rotate(H[i * m1 + i], H[(i + 1) * m1 + i], c, s);
+  gm_rotate(sv[i], sv[i + 1], c, s);
+  const double res = fabs(sv[i + 1]) / st->nrm;
+  st->res = res;
+  if (res < st->tol) {
+    st->sconv = 1;
+    st->halt = 1;  // converged inside the cycle: update(x, i) follows
+    st->cycle_i = i;
+    st->status = st->it;
+  } else {
+    st->it = st->it + 1;
+    if (!(res == res)) {
+      st->done = 1;
+      st->halt = 1;
+      st->status = -(st->max_it + 1);
+    }
+  }
+}
+
+// update(): back substitution y = H^-1 s (GMRes.h:78-94); k from the state.
+__global__ void k_gm_backsolve(int m, int m1, const double* H, const double* sv, double* y,
+                               const KrylovState* st) {
+  if (st->done || threadIdx.x != 0) return;
+  const int k = st->sconv ? st->cycle_i : m - 1;
+  for (int q = 0; q < m1; ++q) y[q] = sv[q];
+  for (int i = k; i >= 0; i--) {
+    y[i] /= H[i * m1 + i];
+    for (int j = i - 1; j >= 0; j--) y[j] -= H[j * m1 + i] * y[i];
+  }
+}
+
+// x += sum_j v_j y_j (j = 0..k, in order, GMRes.h:91-93)
+__global__ void k_gm_xupd(int n, int m, const double* __restrict__ V, const double* __restrict__ y,
+                          double* __restrict__ x, const KrylovState* __restrict__ st) {
+  if (st->done) return;
+  const int k = st->sconv ? st->cycle_i : m - 1;
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n) return;
+  double xl = x[l];
+  for (int j = 0; j <= k; ++j) xl += V[(size_t)j * n + l] * y[j];
+  x[l] = xl;
+}
+
+__global__ void k_gm_finish_cycle(KrylovState* st) {
+  if (threadIdx.x != 0 || st->done) return;
+  if (st->sconv) st->done = 1;
+}
+
+template <int PREC>
+static int gmres_impl(ofh_csr A, const double* b, double* x, int m, int max_it, double toler,
+                      ofh_solve_info* info) {
+  ofh_ctx ctx = A->ctx;
+  hipStream_t s = ctx->stream;
+  const int n = A->n;
+  const int m1 = m + 1;
+  const double t0 = now_seconds();
+  int rc;
+  if (PREC == OFH_DIAG_PREC && (rc = csr_ensure_dinv(A))) return rc;
+  if (PREC == OFH_SSOR_PREC && (rc = csr_ensure_levels(A))) return rc;
+  void* wsp;
+  const size_t nd = (size_t)n * (m1 + 3) + (size_t)m1 * m1 + 4 * (size_t)m1;
+  if ((rc = ensure_vws(ctx, sizeof(double) * nd, &wsp))) return rc;
+  double* V = (double*)wsp;
+  double* w = V + (size_t)n * m1;
+  double* r = w + n;
+  double* tmp = r + n;
+  double* H = tmp + n;
+  double* sv = H + (size_t)m1 * m1;
+  double* cs = sv + m1;
+  double* sn = cs + m1;
+  double* y = sn + m1;
+  OFH_HIP(hipMemsetAsync(H, 0, sizeof(double) * ((size_t)m1 * m1 + 4 * (size_t)m1), s));
+  KrylovState* st = state_ptr(ctx);
+  KrylovState h0{};
+  h0.tol = toler;
+  h0.max_it = max_it;
+  OFH_HIP(hipMemcpyAsync(st, &h0, sizeof(h0), hipMemcpyHostToDevice, s));
+  unsigned* c0 = ctx->ws.counters;
+  double* part = ctx->ws.partials;
+  const int g = reduce_grid(n);
+  const int ge = (n + 255) / 256;
+  auto residual = [&](bool init) -> int {
+    if (PREC == OFH_SSOR_PREC) {
+      hipLaunchKernelGGL(k_resid_plain, dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
+                         A->row_blocks, A->row_ptr, A->col, A->val, b, x, tmp, st, (int)!init);
+      int e = launch_ssor(A, tmp, r, init ? nullptr : &st->done, s);
+      if (e) return e;
+      if (init)
+        hipLaunchKernelGGL(k_gm_resid_norm<true>, dim3(g), dim3(kBlock), 0, s, n, r, b, st, part, c0);
+      else
+        hipLaunchKernelGGL(k_gm_resid_norm<false>, dim3(g), dim3(kBlock), 0, s, n, r, b, st, part, c0);
+    } else if (init) {
+      hipLaunchKernelGGL((k_gm_resid<PREC, true>), dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
+                         A->row_blocks, A->row_ptr, A->col, A->val, b, x, r, A->dinv, st, part, c0);
+    } else {
+      hipLaunchKernelGGL((k_gm_resid<PREC, false>), dim3(A->nblocks), dim3(kBlock), 0, s,
+                         A->nblocks, A->row_blocks, A->row_ptr, A->col, A->val, b, x, r, A->dinv,
+                         st, part, c0);
+    }
+    return OFH_OK;
+  };
+  if ((rc = residual(true))) return rc;
+  KrylovState h{};
+  if ((rc = read_state(ctx, &h))) return rc;
+  while (!h.done && h.it <= max_it) {
+    int it = h.it;
+    hipLaunchKernelGGL(k_gm_v0, dim3((std::max(n, m1) + 255) / 256), dim3(256), 0, s, n, m, r, V,
+                       sv, st);
+    for (int i = 0; i < m && it <= max_it; i++, it++) {
+      double* vi = V + (size_t)i * n;
+      if (PREC == OFH_SSOR_PREC) {
+        hipLaunchKernelGGL(k_gm_av<OFH_IDENT_PREC>, dim3(A->nblocks), dim3(kBlock), 0, s,
+                           A->nblocks, A->row_blocks, A->row_ptr, A->col, A->val, vi, tmp,
+                           A->dinv, st);
+        if ((rc = launch_ssor(A, tmp, w, &st->halt, s))) return rc;
+      } else {
+        hipLaunchKernelGGL(k_gm_av<PREC>, dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
+                           A->row_blocks, A->row_ptr, A->col, A->val, vi, w, A->dinv, st);
+      }
+      for (int k = 0; k <= i + 1; ++k) {
+        const double* vprev = k > 0 ? V + (size_t)(k - 1) * n : nullptr;
+        const double* vk = k <= i ? V + (size_t)k * n : nullptr;
+        hipLaunchKernelGGL(k_gm_mgs, dim3(g), dim3(kBlock), 0, s, n, k, i, m1, w, vprev, vk, H,
+                           st, part, c0 + 5);
+      }
+      hipLaunchKernelGGL(k_gm_scale, dim3(ge), dim3(256), 0, s, n, i, m1, w,
+                         V + (size_t)(i + 1) * n, H, st);
+      hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(64), 0, s, i, m1, H, sv, cs, sn, st);
+    }
+    hipLaunchKernelGGL(k_gm_backsolve, dim3(1), dim3(64), 0, s, m, m1, H, sv, y, st);
+    hipLaunchKernelGGL(k_gm_xupd, dim3(ge), dim3(256), 0, s, n, m, V, y, x, st);
+    hipLaunchKernelGGL(k_gm_finish_cycle, dim3(1), dim3(64), 0, s, st);
+    if ((rc = residual(false))) return rc;
+    OFH_HIP(hipGetLastError());
+    if ((rc = read_state(ctx, &h))) return rc;
+  }
+  if (!h.done) {
+    h.status = -h.it;  // GMRes.h:236-240: the counter's value after the loop
+    h.done = 1;
+  }
+  return fill_info(A, b, x, h, t0, info);
+}
+
+}  // namespace ofh
+
+using namespace ofh;
+
+extern "C" {
+
+int ofh_bicgstab(ofh_csr A, int prec, const double* b, double* x, int max_it, double toler,
+                 ofh_solve_info* info) {
+  OFH_CHECK(A && b && x && max_it >= 0, OFH_E_INVALID, "ofh_bicgstab: bad argument");
+  switch (prec) {
+    case OFH_IDENT_PREC: return bicgstab_impl<OFH_IDENT_PREC>(A, b, x, max_it, toler, info);
+    case OFH_DIAG_PREC: return bicgstab_impl<OFH_DIAG_PREC>(A, b, x, max_it, toler, info);
+    case OFH_SSOR_PREC: return bicgstab_impl<OFH_SSOR_PREC>(A, b, x, max_it, toler, info);
+    case OFH_DILU_PREC:
+    case OFH_ILU_PREC:
+      return fail(OFH_E_UNSUPPORTED, "DILU_PREC/ILU_PREC are not provided by this path");
+    default: return fail(OFH_E_INVALID, "unknown preconditioner");
+  }
+}
+
+int ofh_gmres(ofh_csr A, int prec, const double* b, double* x, int m, int max_it, double toler,
+              ofh_solve_info* info) {
+  OFH_CHECK(A && b && x && max_it >= 0 && m >= 1, OFH_E_INVALID, "ofh_gmres: bad argument");
+  switch (prec) {
+    case OFH_IDENT_PREC: return gmres_impl<OFH_IDENT_PREC>(A, b, x, m, max_it, toler, info);
+    case OFH_DIAG_PREC: return gmres_impl<OFH_DIAG_PREC>(A, b, x, m, max_it, toler, info);
+    case OFH_SSOR_PREC: return gmres_impl<OFH_SSOR_PREC>(A, b, x, m, max_it, toler, info);
+    case OFH_DILU_PREC:
+    case OFH_ILU_PREC:
+      return fail(OFH_E_UNSUPPORTED, "DILU_PREC/ILU_PREC are not provided by this path");
+    default: return fail(OFH_E_INVALID, "unknown preconditioner");
+  }
+}
+
+int ofh_prec_apply(ofh_csr A, int prec, const double* b, double* x) {
+  OFH_CHECK(A && b && x, OFH_E_INVALID, "ofh_prec_apply: null argument");
+  const int n = A->n;
+  int rc;
+  switch (prec) {
+    case OFH_IDENT_PREC:
+      hipLaunchKernelGGL(k_prec_diag, dim3((n + 255) / 256), dim3(256), 0, A->ctx->stream, n, b,
+                         (const double*)nullptr, x);
+      break;
+    case OFH_DIAG_PREC:
+      if ((rc = csr_ensure_dinv(A))) return rc;
+      hipLaunchKernelGGL(k_prec_diag, dim3((n + 255) / 256), dim3(256), 0, A->ctx->stream, n, b,
+                         A->dinv, x);
+      break;
+    case OFH_SSOR_PREC:
+      OFH_CHECK(b != x, OFH_E_INVALID, "ofh_prec_apply: SSOR needs distinct b and x");
+      return launch_ssor(A, b, x, nullptr, A->ctx->stream);
+    case OFH_DILU_PREC:
+    case OFH_ILU_PREC:
+      return fail(OFH_E_UNSUPPORTED, "DILU_PREC/ILU_PREC are not provided by this path");
+    default: return fail(OFH_E_INVALID, "unknown preconditioner");
+  }
+  OFH_HIP(hipGetLastError());
+  return OFH_OK;
+}
+
+int ofh_linear_solve(ofh_csr A, int solver, int prec, const double* b, double* x, int max_it,
+                     double toler, ofh_solve_info* info) {
+  OFH_CHECK(A, OFH_E_INVALID, "In LinearSolver::solve(): No matrix has been defined.");
+  switch (solver) {
+    case OFH_CG_SOLVER: return ofh_cg(A, prec, b, x, max_it, toler, info);
+    case OFH_BICG_STAB_SOLVER: return ofh_bicgstab(A, prec, b, x, max_it, toler, info);
+    case OFH_GMRES_SOLVER: {
+      int m = std::max(1, std::min(A->n / 5, 50));
+      return ofh_gmres(A, prec, b, x, m, max_it, toler, info);
+    }
+    case OFH_DIRECT_SOLVER:
+    case OFH_CGS_SOLVER:
+    case OFH_BICG_SOLVER:
+      return fail(OFH_E_UNSUPPORTED, "solver not provided by this path (CG, BICG_STAB, GMRES are)");
+    default: return fail(OFH_E_INVALID, "unknown solver");
+  }
+}
+
+}  // extern "C"

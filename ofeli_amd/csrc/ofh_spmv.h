@@ -1,0 +1,66 @@
+// ofh_spmv.h -- device-side CSR-stream row sums shared by the SpMV and the
+// fused Krylov kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ofh_internal.h"
+
+namespace ofh {
+
+// Products of one nonzero: a_k * x_c, or (s*a_k) * x_c when SCALED (the
+// operation order of SpMatrix::MultAdd(a,x,y), SpMatrix_impl.h:748-757).
+template <typename T, bool SCALED>
+__device__ __forceinline__ T nz_product(T a, T xv, T s) {
+  if (SCALED) return (s * a) * xv;
+  return a * xv;
+}
+
+// Row sum of row r0+threadIdx.x over the rows [r0,r1) of one workgroup.
+// The sum starts at `init` and adds the row's products left to right.
+// Rows whose run exceeds TILE nonzeros come as a single-row block; that
+// row's sum is returned in thread 0.  Must be called by all kBlock threads.
+template <typename T, int TILE, bool SCALED = false>
+__device__ __forceinline__ T csr_stream_rowsum(int r0, int r1, const int* __restrict__ rp,
+                                               const int* __restrict__ col,
+                                               const T* __restrict__ val,
+                                               const T* __restrict__ x, T* lds, T init,
+                                               T scale = T(1)) {
+  const int k0 = rp[r0], k1 = rp[r1];
+  const int t = threadIdx.x;
+  if (k1 - k0 <= TILE) {
+    int k = k0 + t;
+    // 4 independent val/col loads in flight per thread
+    for (; k + 3 * kBlock < k1; k += 4 * kBlock) {
+      int c0 = col[k], c1 = col[k + kBlock], c2 = col[k + 2 * kBlock], c3 = col[k + 3 * kBlock];
+      T a0 = val[k], a1 = val[k + kBlock], a2 = val[k + 2 * kBlock], a3 = val[k + 3 * kBlock];
+      T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+      lds[k - k0] = nz_product<T, SCALED>(a0, x0, scale);
+      lds[k + kBlock - k0] = nz_product<T, SCALED>(a1, x1, scale);
+      lds[k + 2 * kBlock - k0] = nz_product<T, SCALED>(a2, x2, scale);
+      lds[k + 3 * kBlock - k0] = nz_product<T, SCALED>(a3, x3, scale);
+    }
+    for (; k < k1; k += kBlock) lds[k - k0] = nz_product<T, SCALED>(val[k], x[col[k]], scale);
+    __syncthreads();
+    T s = init;
+    const int r = r0 + t;
+    if (r < r1) {
+      const int a = rp[r] - k0, e = rp[r + 1] - k0;
+      for (int j = a; j < e; ++j) s += lds[j];
+    }
+    return s;
+  }
+  // a single row longer than the tile: chunk through LDS, thread 0 sums
+  T s = init;
+  for (int c0 = k0; c0 < k1; c0 += TILE) {
+    const int c1 = min(c0 + TILE, k1);
+    __syncthreads();
+    for (int k = c0 + t; k < c1; k += kBlock) lds[k - c0] = nz_product<T, SCALED>(val[k], x[col[k]], scale);
+    __syncthreads();
+    if (t == 0)
+      for (int j = 0; j < c1 - c0; ++j) s += lds[j];
+  }
+  return s;
+}
+
+}  // namespace ofh

@@ -1,0 +1,80 @@
+"""Seeded synthetic inputs of the benchmark configurations.
+
+A counter-based SplitMix64 stream (seed 0x0FE11000 + config id) feeds every
+random quantity, so the same arrays are produced on any host.  Meshes come
+from the library's structured mesh constructors; the fields below are nodal
+or per-element arrays handed to the assembly.
+"""
+import numpy as np
+
+SEED_BASE = 0x0FE11000
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.state = np.uint64(seed)
+
+    def next_u64(self, n):
+        with np.errstate(over="ignore"):
+            idx = np.arange(1, n + 1, dtype=np.uint64)
+            z = self.state + idx * np.uint64(0x9E3779B97F4A7C15)
+            self.state = self.state + np.uint64(n) * np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        return z
+
+    def uniform(self, n, lo=0.0, hi=1.0):
+        u = (self.next_u64(n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+        return lo + (hi - lo) * u
+
+    def normal(self, n):
+        u1 = self.uniform(n)
+        u2 = self.uniform(n)
+        u1 = np.maximum(u1, 1e-300)
+        return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def gaussian_bumps(rng, xy, nb=8, amp=0.5):
+    """f = 1 + amp * sum of nb bumps exp(-|x-c|^2 / (2 s^2)), c in [0.1,0.9]^2, s in [0.02,0.1]."""
+    c = rng.uniform(2 * nb, 0.1, 0.9).reshape(nb, 2)
+    s = rng.uniform(nb, 0.02, 0.1)
+    f = np.ones(xy.shape[0])
+    for k in range(nb):
+        d2 = (xy[:, 0] - c[k, 0]) ** 2 + (xy[:, 1] - c[k, 1]) ** 2
+        f += amp * np.exp(-d2 / (2.0 * s[k] ** 2))
+    return f
+
+
+def fourier_field(rng, pts, modes=64, corr=0.1):
+    """Gaussian field of variance 1: g = sqrt(2/M) sum cos(w.x + phi), w ~ N(0, corr^-2 I)."""
+    dim = pts.shape[1]
+    w = rng.normal(modes * dim).reshape(modes, dim) / corr
+    phi = rng.uniform(modes, 0.0, 2.0 * np.pi)
+    g = np.zeros(pts.shape[0])
+    for m in range(modes):
+        g += np.cos(pts @ w[m] + phi[m])
+    return np.sqrt(2.0 / modes) * g
+
+
+def centroids(coord, elem):
+    return coord[elem - 1].mean(axis=1)
+
+
+def config0(n=128):
+    """2D Poisson, Laplace2DT3, n x n, f = 1 + 0.5 * 8 bumps, Dirichlet 0."""
+    from . import api
+    coord, code, elem = api.mesh2d(n, n)
+    rng = SplitMix64(SEED_BASE + 0)
+    f = gaussian_bumps(rng, coord)
+    return dict(coord=coord, code=code, elem=elem, f=f, coef=None)
+
+
+def config1(n=96):
+    """3D variable-k Poisson on Kuhn tets, k_e = exp(g(centroid)), f = 1, Dirichlet 0."""
+    from . import api
+    coord, code, elem = api.mesh3d_kuhn(n, n, n)
+    rng = SplitMix64(SEED_BASE + 1)
+    g = fourier_field(rng, centroids(coord, elem))
+    return dict(coord=coord, code=code, elem=elem, f=np.ones(coord.shape[0]), coef=np.exp(g))

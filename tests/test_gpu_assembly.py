@@ -1,0 +1,115 @@
+"""GPU assembly parity: CSR pattern, matrix values and right-hand side of every
+element kind are bitwise equal to the oracle's element-order assembly."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _t():
+    import torch
+    return torch
+
+
+def dev(a):
+    return _t().as_tensor(np.ascontiguousarray(a, np.float64), device="cuda")
+
+
+def host(t):
+    _t().cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def check_plan(ctx, m, kind, f=None, coef=None, bc=None, prm=None, u_prev=None):
+    import ofeli_amd as oa
+    P = oa.AssemblyPlan(ctx, m.coord, m.elem.astype(np.int32), m.code)
+    dof, neq = O.number_equations(m.code)
+    assert P.nb_eq == neq
+    assert np.array_equal(P.dof(), dof)
+    rp, ci = O.node_graph(m.elem, m.nb_dof, dof, neq)
+    grp, gci = P.A.pattern()
+    assert np.array_equal(grp, rp) and np.array_equal(gci, ci - 1)
+    a, b = O.assemble(kind, m, dof, neq, rp, ci, f=f, coef=coef, bc=bc, prm=prm, u_prev=u_prev)
+    gb = _t().zeros(neq, dtype=_t().float64, device="cuda")
+    P.build(kind, coef=None if coef is None else dev(coef), f=None if f is None else dev(f),
+            bc=None if bc is None else dev(bc), u_prev=None if u_prev is None else dev(u_prev),
+            prm=() if prm is None else tuple(prm), matrix=True, b=gb)
+    assert np.array_equal(P.A.values(), a)
+    assert np.array_equal(host(gb), b)
+    return P, a, b, dof
+
+
+def test_laplace2dt3_harmonic_bc(ctx):
+    """Nonzero Dirichlet data exercise updateBC (Equation_impl.h:160-176)."""
+    m = O.mesh2d(12, 9, codes=(1, 1, 1, 1), box=(0.0, 1.0, 0.0, 0.7))
+    x, y = m.coord[:, 0], m.coord[:, 1]
+    bc = np.exp(x) * np.sin(y) + np.exp(y) * np.cos(x)
+    f = np.zeros(m.nb_nodes) + 0.25
+    check_plan(ctx, m, O.LAPLACE2DT3, f=f, bc=bc)
+
+
+def test_dc2dt3_lognormal(ctx):
+    m = O.mesh2d(20, 17)
+    rng = np.random.default_rng(0)
+    k = np.exp(0.75 * rng.standard_normal(m.nb_elements))
+    check_plan(ctx, m, O.DC2DT3, f=1 + rng.random(m.nb_nodes), coef=k)
+
+
+def test_dc2dt3_backward_euler(ctx):
+    m = O.mesh2d(16, 16)
+    rng = np.random.default_rng(1)
+    k = np.exp(rng.standard_normal(m.nb_elements))
+    check_plan(ctx, m, O.DC2DT3_BE, f=rng.random(m.nb_nodes), coef=k, prm=[1e-3, 1.0, 1.0],
+               u_prev=rng.standard_normal(m.nb_nodes), bc=np.zeros(m.nb_nodes))
+
+
+@pytest.mark.parametrize("with_k", [False, True])
+def test_p1_tetra(ctx, with_k):
+    m = O.mesh3d_kuhn(6, 5, 4)
+    rng = np.random.default_rng(2)
+    k = np.exp(rng.standard_normal(m.nb_elements)) if with_k else None
+    check_plan(ctx, m, O.LAPLACE3DT4, f=1 + rng.random(m.nb_nodes), coef=k,
+               bc=rng.standard_normal(m.nb_nodes))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_elas2dq4_cantilever(ctx, mode):
+    m = O.mesh2d(12, 4, quad=True, nb_dof=2, codes=(1, 0, 0, 0), box=(0.0, 4.0, 0.0, 1.0))
+    m.code[:, 1] = m.code[:, 0]  # clamp both DOFs on x = 0
+    rng = np.random.default_rng(3)
+    E = np.exp(0.5 * rng.standard_normal(m.nb_elements))
+    P, a, b, dof = check_plan(ctx, m, O.ELAS2DQ4, coef=E, prm=[0.3, mode],
+                              bc=np.zeros(2 * m.nb_nodes))
+    # right-edge tractions, side order
+    ny = 4
+    nx = 12
+    sides = np.array([[nx * (ny + 1) + j + 1, nx * (ny + 1) + j + 2] for j in range(ny)], np.int32)
+    tr = rng.standard_normal((ny, 2))
+    gb = dev(b)
+    P.traction(sides, dev(tr), gb)
+    ref = O.traction_rhs(m, sides.astype(np.int64), tr, dof, b)
+    assert np.array_equal(host(gb), ref)
+
+
+def test_insert_bc(ctx):
+    m = O.mesh2d(7, 5)
+    import ofeli_amd as oa
+    P = oa.AssemblyPlan(ctx, m.coord, m.elem.astype(np.int32), m.code)
+    v = np.random.default_rng(0).standard_normal(P.nb_eq)
+    bc = np.random.default_rng(1).standard_normal(m.nb_nodes)
+    u = _t().zeros(m.nb_nodes, dtype=_t().float64, device="cuda")
+    P.insert_bc(dev(v), dev(bc), u)
+    assert np.array_equal(host(u), O.insert_bc(m.code, v, bc))
+
+
+def test_negative_jacobian_rejected(ctx):
+    import ofeli_amd as oa
+    m = O.mesh2d(3, 3)
+    elem = m.elem.copy()
+    elem[4] = elem[4][[0, 2, 1]]  # flip one triangle
+    P = oa.AssemblyPlan(ctx, m.coord, elem.astype(np.int32), m.code)
+    with pytest.raises(oa.OfhError) as e:
+        P.build(oa.LAPLACE2DT3, matrix=True)
+    assert e.value.code == -4

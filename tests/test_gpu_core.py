@@ -1,0 +1,226 @@
+"""GPU parity of the C-ABI against the CPU oracle: SpMV, Prec, SSOR, CG,
+BiCGStab, GMRes and the mixed-precision CG.
+
+Integer/index results and every quantity the reference computes by
+sequential sums (SpMV row sums, SSOR sweeps) must be bitwise equal; iterative
+solutions are compared with the tolerance stated in each test.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def dev(a):
+    torch = _torch()
+    return torch.as_tensor(np.ascontiguousarray(a, np.float64), device="cuda")
+
+
+def host(t):
+    _torch().cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def dc2d_system(n=48, seed=3):
+    """Oracle-assembled DC2DT3 system with lognormal k and a bump source."""
+    m = O.mesh2d(n, n)
+    dof, neq = O.number_equations(m.code)
+    rp, ci = O.node_graph(m.elem, 1, dof, neq)
+    rng = np.random.default_rng(seed)
+    k = np.exp(0.75 * rng.standard_normal(m.nb_elements))
+    f = 1.0 + rng.random(m.nb_nodes)
+    a, b = O.assemble(O.DC2DT3, m, dof, neq, rp, ci, f=f, coef=k)
+    return rp, ci, a, b
+
+
+def gpu_csr(ctx, rp, ci, a):
+    import ofeli_amd as oa
+    return oa.CsrMatrix(ctx, len(rp) - 1, rp.astype(np.int32), (ci - 1).astype(np.int32), a)
+
+
+def rel(x, y):
+    return np.linalg.norm(x - y) / np.linalg.norm(y)
+
+
+def test_spmv_bitexact(ctx):
+    import ofeli_amd as oa
+    rp, ci, a, b = dc2d_system()
+    A = gpu_csr(ctx, rp, ci, a)
+    x = np.random.default_rng(1).standard_normal(A.n)
+    y = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+    oa.spmv(A, dev(x), y)
+    assert np.array_equal(host(y), O.spmv(rp, ci, a, x))
+
+
+def test_spmv_add_bitexact(ctx):
+    """MultAdd(x,y) and MultAdd(a,x,y) keep the reference's term order."""
+    import ofeli_amd as oa
+    rp, ci, a, b = dc2d_system(20)
+    A = gpu_csr(ctx, rp, ci, a)
+    rng = np.random.default_rng(2)
+    x, y0 = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    y = dev(y0)
+    oa.spmv_add(A, 1.0, dev(x), y)
+    ref = y0.copy()
+    for i in range(A.n):
+        for k in range(rp[i], rp[i + 1]):
+            ref[i] += a[k] * x[ci[k] - 1]
+    assert np.array_equal(host(y), ref)
+    y = dev(y0)
+    oa.spmv_add(A, -0.3, dev(x), y)
+    ref = y0.copy()
+    for i in range(A.n):
+        for k in range(rp[i], rp[i + 1]):
+            ref[i] += -0.3 * a[k] * x[ci[k] - 1]
+    assert np.array_equal(host(y), ref)
+
+
+def test_spmv_long_rows_bitexact(ctx):
+    """Rows longer than the LDS tile take the chunked path; sums stay sequential."""
+    import ofeli_amd as oa
+    n = 9000
+    rng = np.random.default_rng(5)
+    rows = []
+    for i in range(n):
+        if i in (3, 4500):
+            cols = np.arange(n)  # two dense rows
+        else:
+            cols = np.unique(np.concatenate([[i], rng.integers(0, n, 6)]))
+        rows.append(cols)
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ci = np.concatenate(rows).astype(np.int64) + 1
+    a = rng.standard_normal(rp[-1])
+    A = gpu_csr(ctx, rp, ci, a)
+    x = rng.standard_normal(n)
+    y = _torch().zeros(n, dtype=_torch().float64, device="cuda")
+    oa.spmv(A, dev(x), y)
+    assert np.array_equal(host(y), O.spmv(rp, ci, a, x))
+
+
+def test_prec_diag_and_ssor_bitexact(ctx):
+    import ofeli_amd as oa
+    rp, ci, a, b = dc2d_system(32)
+    A = gpu_csr(ctx, rp, ci, a)
+    r = np.random.default_rng(4).standard_normal(A.n)
+    z = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+    oa.prec_apply(A, oa.DIAG_PREC, dev(r), z)
+    diag = a[[np.searchsorted(ci[rp[i]:rp[i + 1]], i + 1) + rp[i] for i in range(A.n)]]
+    assert np.array_equal(host(z), r * (1.0 / diag))
+    oa.ssor_solve(A, dev(r), z)
+    assert np.array_equal(host(z), O.ssor_solve(rp, ci, a, r))
+
+
+@pytest.mark.parametrize("prec", [1, 2, 5])
+def test_cg_matches_oracle(ctx, prec):
+    """CG: same return convention, iteration count within 1, and the GPU and
+    oracle solutions agree to relative L2 <= 1e-9 at rtol 1e-10."""
+    import ofeli_amd as oa
+    rp, ci, a, b = dc2d_system(40)
+    A = gpu_csr(ctx, rp, ci, a)
+    ret, xo, reso = O.cg(rp, ci, a, b, prec=prec, max_it=5000, toler=1e-10)
+    x = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+    info = oa.cg(A, dev(b), x, prec=prec, max_it=5000, toler=1e-10)
+    assert ret > 0 and info.nb_it > 0
+    assert abs(info.nb_it - ret) <= 1
+    assert rel(host(x), xo) <= 1e-9
+    assert info.true_relres <= 2e-10
+
+
+def test_cg_deterministic_and_warm_start(ctx):
+    import ofeli_amd as oa
+    rp, ci, a, b = dc2d_system(40)
+    A = gpu_csr(ctx, rp, ci, a)
+    xs = []
+    for _ in range(2):
+        x = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+        oa.cg(A, dev(b), x, max_it=5000, toler=1e-9)
+        xs.append(host(x))
+    assert np.array_equal(xs[0], xs[1])
+    # warm start from the solution: already converged -> 0 (CG.h:116-123)
+    info = oa.cg(A, dev(b), dev(xs[0]), max_it=5000, toler=1e-6)
+    assert info.nb_it == 0
+
+
+def test_cg_not_converged_convention(ctx):
+    import ofeli_amd as oa
+    rp, ci, a, b = dc2d_system(40)
+    A = gpu_csr(ctx, rp, ci, a)
+    ret, _, _ = O.cg(rp, ci, a, b, max_it=7, toler=1e-12)
+    x = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+    info = oa.cg(A, dev(b), x, max_it=7, toler=1e-12)
+    assert ret == -8 and info.nb_it == -8 and info.iterations == 7
+
+
+@pytest.mark.parametrize("prec", [1, 2, 5])
+def test_bicgstab_matches_oracle(ctx, prec):
+    """BiCGStab's residual history is erratic, so rounding differences in the
+    tree-summed dot products move the step that first crosses the tolerance by
+    a few iterations.  The algorithm itself is pinned by the early iterates
+    (equal to ~1e-12), the final solutions by relative L2 <= 1e-8."""
+    import ofeli_amd as oa
+    rp, ci, a, b = dc2d_system(32)
+    A = gpu_csr(ctx, rp, ci, a)
+    for k in (1, 2, 5):
+        _, xo, _ = O.bicgstab(rp, ci, a, b, prec=prec, max_it=k, toler=1e-14)
+        x = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+        info = oa.bicgstab(A, dev(b), x, prec=prec, max_it=k, toler=1e-14)
+        assert info.nb_it == -(k + 1)
+        assert rel(host(x), xo) <= 1e-11
+    ret, xo, _ = O.bicgstab(rp, ci, a, b, prec=prec, max_it=3000, toler=1e-10)
+    x = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+    info = oa.bicgstab(A, dev(b), x, prec=prec, max_it=3000, toler=1e-10)
+    assert ret > 0 and info.nb_it > 0
+    assert abs(info.nb_it - ret) <= max(2, ret // 8)
+    assert rel(host(x), xo) <= 1e-8
+    assert info.true_relres <= 1e-9
+
+
+@pytest.mark.parametrize("prec", [1, 2, 5])
+def test_gmres_matches_oracle(ctx, prec):
+    import ofeli_amd as oa
+    rp, ci, a, b = dc2d_system(24)
+    A = gpu_csr(ctx, rp, ci, a)
+    ret, xo, _ = O.gmres(rp, ci, a, b, 20, prec=prec, max_it=3000, toler=1e-10)
+    x = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+    info = oa.gmres(A, dev(b), x, 20, prec=prec, max_it=3000, toler=1e-10)
+    assert ret > 0 and info.nb_it > 0
+    assert abs(info.nb_it - ret) <= 2
+    assert rel(host(x), xo) <= 1e-8
+
+
+def test_mixed_cg_fp64_verified(ctx):
+    """Mixed precision: stops on the true FP64 residual; error vs the FP64
+    oracle solution of the same tolerance stays at the solver-accuracy level."""
+    import ofeli_amd as oa
+    rp, ci, a, b = dc2d_system(64)
+    A = gpu_csr(ctx, rp, ci, a)
+    ret, xo, _ = O.cg(rp, ci, a, b, max_it=10000, toler=1e-12)
+    x = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+    info = oa.cg_mixed(A, dev(b), x, max_it=10000, toler=1e-8)
+    assert info.nb_it > 0 and info.fallback == 0
+    assert info.true_relres <= 1e-8
+    # ||x - x*|| / ||x*|| <= cond-bounded: generous 1e-5 on this small system
+    assert rel(host(x), xo) <= 1e-5
+
+
+def test_zero_pivot_error(ctx):
+    import ofeli_amd as oa
+    rp, ci, a, b = dc2d_system(8)
+    a = a.copy()
+    d = np.searchsorted(ci[rp[5]:rp[6]], 6) + rp[5]
+    a[d] = 0.0
+    A = gpu_csr(ctx, rp, ci, a)
+    x = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+    with pytest.raises(oa.OfhError) as e:
+        oa.cg(A, dev(b), x)
+    assert e.value.code == -3
+    with pytest.raises(oa.OfhError):
+        oa.cg(A, dev(b), x, prec=oa.DILU_PREC)
