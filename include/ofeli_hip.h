@@ -77,7 +77,8 @@ typedef struct ofh_solve_info {
 const char* ofh_last_error(void);
 int ofh_ctx_create(int device, ofh_ctx* out);
 int ofh_ctx_destroy(ofh_ctx ctx);
-/* Use an existing hipStream_t (NULL: the context's own stream). */
+/* Order all further work on an existing hipStream_t (NULL is the null /
+ * default stream); the context's own stream is released. */
 int ofh_ctx_set_stream(ofh_ctx ctx, void* hip_stream);
 int ofh_ctx_synchronize(ofh_ctx ctx);
 int ofh_malloc(ofh_ctx ctx, size_t bytes, void** d_ptr);
@@ -188,8 +189,10 @@ int ofh_asm_get_dof(ofh_asm P, int32_t* h_dof);
  *   d_f:    nodal source (LAPLACE/DC kinds) or NULL
  *   d_bc:   nodal Dirichlet values (nb_dof per node) or NULL
  *   d_uprev: nodal u^n (DC2DT3_BE)
- *   h_prm:  DC2DT3_BE {dt, rho, cp}; ELAS2DQ4 {nu, mode (0 reference parity:
- *           plane strain, 2*Dev+Dil; 1 textbook: plane stress, Dev+Dil)}
+ *   h_prm:  DC2DT3_BE {dt, rho, cp}; ELAS2DQ4 {nu, mode}: mode 0 reference
+ *           parity (plane strain, 2*Dev+Dil, as Elas2DQ4::set always ends
+ *           with PlaneStrain() and both routines add the cumulative eA0),
+ *           1 plane stress Dev+Dil, 2 plane strain Dev+Dil
  * want_matrix = 0 assembles the right-hand side only. d_b may be NULL. */
 int ofh_asm_build(ofh_asm P, int kind, const double* d_coef, const double* d_f,
                   const double* d_bc, const double* d_uprev, const double* h_prm, int nprm,

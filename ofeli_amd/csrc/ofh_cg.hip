@@ -1,18 +1,24 @@
 // ofh_cg.hip -- preconditioned CG on the GPU (CG.h:99-157 replacement).
 //
-// Per iteration three kernels, all stream-ordered and all reading their
-// scalars (alpha, beta, the stop flag) from a device-side state block, so the
-// host only synchronises once per chunk of iterations:
-//   k_cg_dir   p = z + beta p          (z = D^-1 r fused for DIAG_PREC)
-//   k_cg_spmv  q = A p, (p,q)  -> alpha = rho/(p,q)     (CSR-stream SpMV)
-//   k_cg_upd   x += alpha p, r -= alpha q, (r,r), z = P r, (r,z)
-//              -> stop test ||r||/||b|| <= toler, beta for the next step
-// SSOR_PREC inserts the level-scheduled sweeps and a (r,z) reduction between
-// k_cg_upd and the next k_cg_dir.  Every kernel returns at once when the stop
-// flag is set, so a chunk enqueued past convergence costs only launch slots.
-// The arithmetic of each vector entry is the reference's (p_i = z_i+beta*p_i,
-// x_i += alpha*p_i, r_i -= alpha*q_i, q = A p by sequential row sums); only
-// the dot products are summed as fixed-order trees instead of left to right.
+// Launch sequence (host enqueues, the device decides):
+//   k_cg_init         r = b - A x; partial (b,b) (r,r) (r,z)
+//   k_cg_dir(0)       fold; stop test of the initial guess (CG.h:116); p = z
+//   per iteration it = 1, 2, ...:
+//     k_cg_spmv(it)   q = A p; partial (p,q)                    (CSR-stream)
+//     k_cg_upd(it)    fold (p,q): alpha = rho/(p,q); x += alpha p;
+//                     r -= alpha q; partial (r,r) (r,z) with z = P r
+//     k_cg_dir(it)    fold: ||r||/||b|| <= toler -> stop (CG.h:144); else
+//                     beta = rho_it/rho_{it-1}, p = z + beta p (CG.h:127-133)
+// SSOR_PREC: k_cg_upd stores no (r,z); k_cg_check folds (r,r) and takes the
+// stop decision, the level-scheduled sweeps compute z, k_cg_rz writes (r,z).
+// A reduction is folded by every block of the kernel that needs it (no
+// intra-kernel cross-block traffic); all blocks fold the same partials in the
+// same order and take the same decision; block 0 records it.  Once the stop
+// flag is set every later kernel returns at once, so the iterations the host
+// enqueues past convergence cost only launch slots.
+// Entry-wise arithmetic is the reference's (p_i = z_i + beta p_i,
+// x_i += alpha p_i, r_i -= alpha q_i, q = A p as sequential row sums); only
+// the dot products are summed as fixed-order trees.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,15 +31,14 @@
 namespace ofh {
 
 template <int PREC>
-__device__ __forceinline__ double prec_z(int i, const double* __restrict__ r,
-                                         const double* __restrict__ dinv,
-                                         const double* __restrict__ z) {
-  if (PREC == OFH_DIAG_PREC) return r[i] * dinv[i];  // Prec_impl.h:245-247
-  if (PREC == OFH_IDENT_PREC) return r[i];
+__device__ __forceinline__ double zval(int i, double ri, const double* __restrict__ dinv,
+                                       const double* __restrict__ z) {
+  if (PREC == OFH_DIAG_PREC) return ri * dinv[i];  // Prec_impl.h:245-247
+  if (PREC == OFH_IDENT_PREC) return ri;
   return z[i];
 }
 
-// r = b - A x, z = P r, (b,b) (r,r) (r,z); the last block sets up the state.
+// r = b - A x; partials (b,b) -> slot 0, (r,r) -> 1, (r,z) -> 2
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_cg_init(int nblocks, const int* __restrict__ rb,
                                                    const int* __restrict__ rp,
@@ -43,171 +48,203 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int nblocks, const int* __re
                                                    const double* __restrict__ x,
                                                    double* __restrict__ r,
                                                    const double* __restrict__ dinv,
-                                                   KrylovState* st, double* partials,
-                                                   unsigned* counter) {
+                                                   double* __restrict__ partials) {
   __shared__ double lds[kTile64];
   __shared__ double red[3 * kBlock / kWave];
-  __shared__ bool last;
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
   const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
   double v[3] = {0.0, 0.0, 0.0};
-  const int i = r0 + threadIdx.x;
-  const bool own = i < r1;  // a long-row block holds one row, summed in thread 0
-  if (own) {
-    const double bi = b[i], ri = bi - s;
+  const int i = r0 + threadIdx.x;  // a long-row block holds one row, summed in thread 0
+  if (i < r1) {
+    const double bi = b[i], ri = bi - s;  // r = b - A*x (CG.h:115)
     r[i] = ri;
     v[0] = bi * bi;
     v[1] = ri * ri;
-    if (PREC != OFH_SSOR_PREC) {
-      const double zi = PREC == OFH_DIAG_PREC ? ri * dinv[i] : ri;
-      v[2] = ri * zi;
-    }
+    if (PREC != OFH_SSOR_PREC) v[2] = ri * zval<PREC>(i, ri, dinv, nullptr);
   }
   block_sum_k<kBlock, 3>(v, red);
-  if (publish_partials<3>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
-    double t[3];
-    fold_partials<kBlock, 3>(t, partials, gridDim.x, red);
-    if (threadIdx.x == 0) {
-      double nrm = sqrt(t[0]);
-      if (nrm == 0) nrm = 1;  // CG.h:113-114
-      st->nrm = nrm;
-      st->res = sqrt(t[1]) / nrm;
-      st->res0 = st->res;
-      st->rho = t[2];
-      st->it = 0;
-      st->done = 0;
-      if (st->res <= st->tol) {  // CG.h:116
-        st->done = 1;
-        st->status = 0;
-      }
-      *counter = 0u;
-    }
-  }
+  const int slot[3] = {0, 1, 2};
+  write_partials<3>(v, slot, partials, blockIdx.x);
 }
 
-// p = z + beta p (p = z on the first step), CG.h:127-133
+// Stop decision after `it` iterations and the new direction.
+//   it == 0: folds the init partials (nparts0 blocks): ||b||, ||r0||, (r0,z0)
+//   it >  0: folds (r,r) (slot 0) and (r,z) (slot 1 or, SSOR, slot 2)
+// SSOR: the stop decision was taken by k_cg_check before the sweeps.
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_cg_dir(int n, const double* __restrict__ r,
+__global__ __launch_bounds__(kBlock) void k_cg_dir(int n, int it, int nparts0, int nparts,
+                                                  const double* __restrict__ r,
                                                   const double* __restrict__ dinv,
                                                   const double* __restrict__ z,
-                                                  double* __restrict__ p,
-                                                  const KrylovState* __restrict__ st) {
+                                                  double* __restrict__ p, KrylovState* st,
+                                                  const double* __restrict__ partials) {
   if (st->done) return;
-  const bool first = st->it == 0;
-  const double beta = st->beta;
+  __shared__ double red[3 * kBlock / kWave];
+  double t[3];
+  double nrm;
+  if (PREC == OFH_SSOR_PREC) {
+    const int slot[1] = {2};
+    double u[1];
+    fold_partials<kBlock, 1>(u, slot, partials, nparts, red);  // written by k_cg_rz
+    t[2] = u[0];
+    nrm = st->nrm;
+  } else if (it == 0) {
+    const int slot[3] = {0, 1, 2};
+    fold_partials<kBlock, 3>(t, slot, partials, nparts0, red);
+    nrm = sqrt(t[0]);
+    if (nrm == 0) nrm = 1;  // CG.h:113-114
+  } else {
+    const int slot[2] = {0, 1};
+    double u[2];
+    fold_partials<kBlock, 2>(u, slot, partials, nparts, red);
+    t[1] = u[0];
+    t[2] = u[1];
+    nrm = st->nrm;
+  }
+  bool stop = false;
+  int status = 0;
+  double res = 0;
+  if (PREC != OFH_SSOR_PREC) {
+    res = sqrt(t[1]) / nrm;
+    if (res <= st->tol) {
+      stop = true;
+      status = it;  // 0 when the initial guess passes (CG.h:116-123)
+    } else if (it >= st->max_it || !(res == res)) {
+      stop = true;  // the loop counter ends at max_it+1 (CG.h:154-156)
+      status = -(st->max_it + 1);
+    }
+  }
+  const double rho = t[2];
+  const double beta = it > 0 ? rho / st->rho_buf[(it - 1) & 1] : 0.0;  // CG.h:131
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->it = it;
+    if (PREC != OFH_SSOR_PREC) {
+      st->res = res;
+      if (it == 0) {
+        st->nrm = nrm;
+        st->res0 = res;
+      }
+    }
+    if (stop) {
+      st->done = 1;
+      st->status = status;
+    } else {
+      st->rho_buf[it & 1] = rho;
+      st->beta = beta;
+    }
+  }
+  if (stop) return;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const double zi = prec_z<PREC>(i, r, dinv, z);
-    p[i] = first ? zi : zi + beta * p[i];
+    const double zi = zval<PREC>(i, r[i], dinv, z);
+    p[i] = it == 0 ? zi : zi + beta * p[i];
   }
 }
 
-// q = A p and (p,q); alpha = rho/(p,q), CG.h:134-135
+// q = A p; partial (p,q) -> slot 3 (CG.h:134-135)
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(int nblocks, const int* __restrict__ rb,
                                                    const int* __restrict__ rp,
                                                    const int* __restrict__ col,
                                                    const double* __restrict__ val,
                                                    const double* __restrict__ p,
-                                                   double* __restrict__ q, KrylovState* st,
-                                                   double* partials, unsigned* counter) {
+                                                   double* __restrict__ q,
+                                                   const KrylovState* __restrict__ st,
+                                                   double* __restrict__ partials) {
   if (st->done) return;
   __shared__ double lds[kTile64];
   __shared__ double red[kBlock / kWave];
-  __shared__ bool last;
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
   const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, p, lds, 0.0);
   const int i = r0 + threadIdx.x;
-  const bool own = i < r1;  // a long-row block holds one row, summed in thread 0
   double v[1] = {0.0};
-  if (own) {
+  if (i < r1) {
     q[i] = s;
     v[0] = p[i] * s;
   }
   block_sum_k<kBlock, 1>(v, red);
-  if (publish_partials<1>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
-    double t[1];
-    fold_partials<kBlock, 1>(t, partials, gridDim.x, red);
-    if (threadIdx.x == 0) {
-      st->alpha = st->rho / t[0];
-      *counter = 0u;
-    }
-  }
+  const int slot[1] = {3};
+  write_partials<1>(v, slot, partials, blockIdx.x);
 }
 
-// x += alpha p; r -= alpha q; res = ||r||/||b||; z = P r and (r,z) for the
-// next step (CG.h:136-152, 126).
+// alpha = rho/(p,q); x += alpha p; r -= alpha q; partials (r,r) -> 0 and
+// (r,z) -> 1 (CG.h:135-138)
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_cg_upd(int n, double* __restrict__ x,
-                                                  double* __restrict__ r,
+__global__ __launch_bounds__(kBlock) void k_cg_upd(int n, int it, int nparts_pq,
+                                                  double* __restrict__ x, double* __restrict__ r,
                                                   const double* __restrict__ p,
                                                   const double* __restrict__ q,
                                                   const double* __restrict__ dinv,
-                                                  KrylovState* st, double* partials,
-                                                  unsigned* counter) {
+                                                  KrylovState* st, double* __restrict__ partials) {
   if (st->done) return;
   __shared__ double red[2 * kBlock / kWave];
-  __shared__ bool last;
-  const double alpha = st->alpha;
+  double pq[1];
+  const int sl[1] = {3};
+  fold_partials<kBlock, 1>(pq, sl, partials, nparts_pq, red);
+  const double alpha = st->rho_buf[(it - 1) & 1] / pq[0];
   double v[2] = {0.0, 0.0};
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     x[i] += alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     v[0] += ri * ri;
-    if (PREC != OFH_SSOR_PREC) {
-      const double zi = PREC == OFH_DIAG_PREC ? ri * dinv[i] : ri;
-      v[1] += ri * zi;
-    }
+    if (PREC != OFH_SSOR_PREC) v[1] += ri * zval<PREC>(i, ri, dinv, nullptr);
   }
   block_sum_k<kBlock, 2>(v, red);
-  if (publish_partials<2>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
-    double t[2];
-    fold_partials<kBlock, 2>(t, partials, gridDim.x, red);
-    if (threadIdx.x == 0) {
-      const int it = st->it + 1;
-      st->it = it;
-      const double res = sqrt(t[0]) / st->nrm;
-      st->res = res;
-      if (res <= st->tol) {
-        st->done = 1;
-        st->status = it;
-      } else if (it >= st->max_it || !(res == res)) {
-        // CG.h:154-156: the loop counter ends at max_it+1.  A NaN residual
-        // can never pass the test, so the reference's outcome is already known.
-        st->done = 1;
-        st->status = -(st->max_it + 1);
-      } else if (PREC != OFH_SSOR_PREC) {
-        const double rho = t[1];
-        st->beta = rho / st->rho;  // CG.h:131
-        st->rho = rho;
-      }
-      *counter = 0u;
-    }
+  const int slot[2] = {0, 1};
+  write_partials<2>(v, slot, partials, blockIdx.x);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = alpha;
+}
+
+// SSOR: stop decision on (r,r) before the sweeps (one block).
+__global__ __launch_bounds__(kBlock) void k_cg_check(int it, int nparts, int init,
+                                                    KrylovState* st,
+                                                    const double* __restrict__ partials) {
+  if (st->done) return;
+  __shared__ double red[2 * kBlock / kWave];
+  double t[2] = {0.0, 0.0};
+  if (init) {
+    const int s2[2] = {0, 1};  // (b,b), (r,r) of k_cg_init
+    fold_partials<kBlock, 2>(t, s2, partials, nparts, red);
+  } else {
+    const int s1[1] = {0};  // (r,r) of k_cg_upd
+    double u[1];
+    fold_partials<kBlock, 1>(u, s1, partials, nparts, red);
+    t[1] = u[0];
+  }
+  if (threadIdx.x != 0) return;
+  double nrm = st->nrm;
+  if (init) {
+    nrm = sqrt(t[0]);
+    if (nrm == 0) nrm = 1;
+    st->nrm = nrm;
+  }
+  const double res = sqrt(t[1]) / nrm;
+  st->res = res;
+  if (init) st->res0 = res;
+  st->it = it;
+  if (res <= st->tol) {
+    st->done = 1;
+    st->status = it;
+  } else if (it >= st->max_it || !(res == res)) {
+    st->done = 1;
+    st->status = -(st->max_it + 1);
   }
 }
 
-// SSOR: rho = (r,z) after the sweeps; beta for the next direction.
+// SSOR: (r,z) after the sweeps -> slot 2
 __global__ __launch_bounds__(kBlock) void k_cg_rz(int n, const double* __restrict__ r,
                                                  const double* __restrict__ z,
-                                                 KrylovState* st, double* partials,
-                                                 unsigned* counter) {
+                                                 const KrylovState* __restrict__ st,
+                                                 double* __restrict__ partials) {
   if (st->done) return;
   __shared__ double red[kBlock / kWave];
-  __shared__ bool last;
   double v[1] = {0.0};
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
     v[0] += r[i] * z[i];
   block_sum_k<kBlock, 1>(v, red);
-  if (publish_partials<1>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
-    double t[1];
-    fold_partials<kBlock, 1>(t, partials, gridDim.x, red);
-    if (threadIdx.x == 0) {
-      if (st->it > 0) st->beta = t[0] / st->rho;
-      st->rho = t[0];
-      *counter = 0u;
-    }
-  }
+  const int slot[1] = {2};
+  write_partials<1>(v, slot, partials, blockIdx.x);
 }
 
 template <int PREC>
@@ -231,38 +268,48 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   KrylovState h0{};
   h0.tol = toler;
   h0.max_it = max_it;
-  h0.done = 0;
   OFH_HIP(hipMemcpyAsync(st, &h0, sizeof(h0), hipMemcpyHostToDevice, s));
-  unsigned* c0 = ctx->ws.counters;
   double* part = ctx->ws.partials;
   const int g = reduce_grid(n);
-  hipLaunchKernelGGL(k_cg_init<PREC>, dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
-                     A->row_blocks, A->row_ptr, A->col, A->val, b, x, r, A->dinv, st, part,
-                     c0);
+  const int nb = A->nblocks;
+  hipLaunchKernelGGL(k_cg_init<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
+                     A->row_ptr, A->col, A->val, b, x, r, A->dinv, part);
+  auto ssor_z = [&](int it, bool init) -> int {
+    hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(kBlock), 0, s, it, init ? nb : g, (int)init, st,
+                       part);
+    int e = launch_ssor(A, r, z, &st->done, s);
+    if (e) return e;
+    hipLaunchKernelGGL(k_cg_rz, dim3(g), dim3(kBlock), 0, s, n, r, z, st, part);
+    return OFH_OK;
+  };
+  if (PREC == OFH_SSOR_PREC && (rc = ssor_z(0, true))) return rc;
+  hipLaunchKernelGGL(k_cg_dir<PREC>, dim3(g), dim3(kBlock), 0, s, n, 0, nb, g, r, A->dinv, z, p,
+                     st, part);
   OFH_HIP(hipGetLastError());
-  const int* d_done = &st->done;
-  if (PREC == OFH_SSOR_PREC) {
-    if ((rc = launch_ssor(A, r, z, d_done, s))) return rc;
-    hipLaunchKernelGGL(k_cg_rz, dim3(g), dim3(kBlock), 0, s, n, r, z, st, part, c0 + 1);
-  }
   KrylovState h{};
-  int enq = 0, chunk = 4;
-  for (;;) {
-    for (int c = 0; c < chunk && enq < max_it; ++c, ++enq) {
-      hipLaunchKernelGGL(k_cg_dir<PREC>, dim3(g), dim3(kBlock), 0, s, n, r, A->dinv, z, p, st);
-      hipLaunchKernelGGL(k_cg_spmv, dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
-                         A->row_blocks, A->row_ptr, A->col, A->val, p, q, st, part, c0 + 2);
-      hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, s, n, x, r, p, q, A->dinv,
-                         st, part, c0 + 3);
-      if (PREC == OFH_SSOR_PREC) {
-        if ((rc = launch_ssor(A, r, z, d_done, s))) return rc;
-        hipLaunchKernelGGL(k_cg_rz, dim3(g), dim3(kBlock), 0, s, n, r, z, st, part, c0 + 1);
-      }
+  if ((rc = read_state(ctx, &h))) return rc;
+  int enq = 0, it_prev = 0;
+  double res_prev = h.res;
+  while (!h.done && enq < max_it) {
+    const int chunk = next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq);
+    it_prev = h.it;
+    res_prev = h.res;
+    for (int c = 0; c < chunk; ++c) {
+      const int it = ++enq;
+      hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr,
+                         A->col, A->val, p, q, st, part);
+      hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, nb, x, r, p, q,
+                         A->dinv, st, part);
+      if (PREC == OFH_SSOR_PREC && (rc = ssor_z(it, false))) return rc;
+      hipLaunchKernelGGL(k_cg_dir<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, nb, g, r, A->dinv,
+                         z, p, st, part);
     }
     OFH_HIP(hipGetLastError());
     if ((rc = read_state(ctx, &h))) return rc;
-    if (h.done || enq >= max_it) break;
-    chunk = next_chunk(h.it, h.res, h.res0, toler, max_it - enq);
+  }
+  if (!h.done) {  // max_it == 0 with a non-converged guess
+    h.done = 1;
+    h.status = -(max_it + 1);
   }
   return fill_info(A, b, x, h, t0, info);
 }

@@ -83,28 +83,28 @@ __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restr
   for (; i < n; i += stride) b[i] = (float)a[i];
 }
 
-// Fixed-order dot product: grid-stride per thread, block tree, last block
-// folds the per-block partials in order.
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_dot(int n, const double* __restrict__ x,
-                                              const double* __restrict__ y,
-                                              double* __restrict__ partials,
-                                              unsigned* __restrict__ counter,
-                                              double* __restrict__ out, int sqrt_out) {
-  __shared__ double lds[4 * BLOCK / kWave];
-  __shared__ bool last;
+// Fixed-order dot product: grid-stride per thread and block tree into slot 0
+// of the partials; k_fold_one folds them in block order.
+__global__ __launch_bounds__(kBlock) void k_dot(int n, const double* __restrict__ x,
+                                               const double* __restrict__ y,
+                                               double* __restrict__ partials) {
+  __shared__ double lds[kBlock / kWave];
   double v[1] = {0.0};
-  for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK)
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
     v[0] += x[i] * y[i];
-  block_sum_k<BLOCK, 1>(v, lds);
-  if (publish_partials<1>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
-    double s[1];
-    fold_partials<BLOCK, 1>(s, partials, gridDim.x, lds);
-    if (threadIdx.x == 0) {
-      *out = sqrt_out ? sqrt(s[0]) : s[0];
-      *counter = 0u;
-    }
-  }
+  block_sum_k<kBlock, 1>(v, lds);
+  const int slot[1] = {0};
+  write_partials<1>(v, slot, partials, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fold_one(const double* __restrict__ partials,
+                                                    int nparts, double* __restrict__ out,
+                                                    int sqrt_out) {
+  __shared__ double lds[kBlock / kWave];
+  double s[1];
+  const int slot[1] = {0};
+  fold_partials<kBlock, 1>(s, slot, partials, nparts, lds);
+  if (threadIdx.x == 0) *out = sqrt_out ? sqrt(s[0]) : s[0];
 }
 
 __global__ void k_axpy(int n, double a, const double* __restrict__ x, double* __restrict__ y) {
@@ -114,14 +114,15 @@ __global__ void k_axpy(int n, double a, const double* __restrict__ x, double* __
 
 int reduce_grid(int n) {  // fixed function of n: deterministic partial layout
   int g = (n + kBlock * 4 - 1) / (kBlock * 4);
-  return std::max(1, std::min(g, 2048));
+  return std::max(1, std::min(g, 1024));
 }
 
 int launch_dot(ofh_ctx ctx, int n, const double* x, const double* y, double* d_out,
                bool sqrt_out) {
   int g = reduce_grid(n);
-  hipLaunchKernelGGL(k_dot<kBlock>, dim3(g), dim3(kBlock), 0, ctx->stream, n, x, y,
-                     ctx->ws.partials, ctx->ws.counters, d_out, (int)sqrt_out);
+  hipLaunchKernelGGL(k_dot, dim3(g), dim3(kBlock), 0, ctx->stream, n, x, y, ctx->ws.partials);
+  hipLaunchKernelGGL(k_fold_one, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->ws.partials, g,
+                     d_out, (int)sqrt_out);
   OFH_HIP(hipGetLastError());
   return OFH_OK;
 }
@@ -205,9 +206,7 @@ int ofh_ctx_create(int device, ofh_ctx* out) {
   c->num_cu = prop.multiProcessorCount;
   OFH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   c->own_stream = true;
-  OFH_HIP(hipMalloc(&c->ws.partials, sizeof(double) * 4 * kMaxPartials));
-  OFH_HIP(hipMalloc(&c->ws.counters, sizeof(unsigned) * 64));
-  OFH_HIP(hipMemset(c->ws.counters, 0, sizeof(unsigned) * 64));
+  OFH_HIP(hipMalloc(&c->ws.partials, sizeof(double) * kSlots * kMaxPartials));
   OFH_HIP(hipMalloc(&c->dev_scratch, 4096));
   OFH_HIP(hipMemset(c->dev_scratch, 0, 4096));
   OFH_HIP(hipHostMalloc(&c->host_pinned, 4096, hipHostMallocDefault));
@@ -220,7 +219,6 @@ int ofh_ctx_destroy(ofh_ctx c) {
   hipSetDevice(c->device);
   hipDeviceSynchronize();
   hipFree(c->ws.partials);
-  hipFree(c->ws.counters);
   hipFree(c->dev_scratch);
   hipHostFree(c->host_pinned);
   if (c->vws) hipFree(c->vws);
@@ -232,13 +230,6 @@ int ofh_ctx_destroy(ofh_ctx c) {
 int ofh_ctx_set_stream(ofh_ctx c, void* s) {
   OFH_CHECK(c, OFH_E_INVALID, "ofh_ctx_set_stream: null ctx");
   OFH_HIP(hipStreamSynchronize(c->stream));
-  if (s == nullptr) {
-    if (!c->own_stream) {
-      OFH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-      c->own_stream = true;
-    }
-    return OFH_OK;
-  }
   if (c->own_stream) OFH_HIP(hipStreamDestroy(c->stream));
   c->stream = (hipStream_t)s;
   c->own_stream = false;

@@ -101,35 +101,29 @@ __device__ __forceinline__ void block_sum_k(double (&v)[K], double* lds /*K*BLOC
   }
 }
 
-// Publish this block's K partials; returns true in every thread of the one
-// block that finished last.  The counter is reset by that block.
+// Cross-workgroup reductions never communicate inside a kernel: each block
+// writes its partial sums (one slot per quantity, one entry per block) and
+// the NEXT kernel on the stream, whose launch makes them visible, folds them
+// in block order.  Every consuming block folds the same numbers in the same
+// order, so all blocks agree bitwise on the result (and on any decision taken
+// from it) without fences or atomics.
 template <int K>
-__device__ __forceinline__ bool publish_partials(const double (&v)[K], double* partials,
-                                                 unsigned* counter, int bid, int nblocks,
-                                                 bool* lds_flag) {
+__device__ __forceinline__ void write_partials(const double (&v)[K], const int (&slot)[K],
+                                               double* partials, int bid) {
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) partials[k * kMaxPartials + bid] = v[k];
-    __threadfence();
-    unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    *lds_flag = (t == (unsigned)nblocks - 1);
+    for (int k = 0; k < K; ++k) partials[slot[k] * kMaxPartials + bid] = v[k];
   }
-  __syncthreads();
-  bool last = *lds_flag;
-  if (last) __threadfence();
-  return last;
 }
 
-// Fold the partials in block order (fixed per launch shape).
 template <int BLOCK, int K>
-__device__ __forceinline__ void fold_partials(double (&out)[K], const double* partials,
-                                              int nblocks, double* lds) {
+__device__ __forceinline__ void fold_partials(double (&out)[K], const int (&slot)[K],
+                                              const double* __restrict__ partials, int nparts,
+                                              double* lds) {
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < nblocks; i += BLOCK)
-      s += __hip_atomic_load(partials + k * kMaxPartials + i, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
+    for (int i = threadIdx.x; i < nparts; i += BLOCK) s += partials[slot[k] * kMaxPartials + i];
     out[k] = s;
   }
   block_sum_k<BLOCK, K>(out, lds);
@@ -138,9 +132,10 @@ __device__ __forceinline__ void fold_partials(double (&out)[K], const double* pa
 // ---------------------------------------------------------------------------
 // objects behind the opaque handles
 // ---------------------------------------------------------------------------
+constexpr int kSlots = 8;  // partial-sum slots of the workspace
+
 struct Workspace {
-  double* partials = nullptr;   // 4 * kMaxPartials doubles
-  unsigned* counters = nullptr; // 64 counters, zero between launches
+  double* partials = nullptr;   // kSlots * kMaxPartials doubles
 };
 
 }  // namespace ofh

@@ -2,9 +2,10 @@
 // dispatch on the GPU, plus the solve report shared by every solver.
 //
 // References: BiCGStab.h:87-162, GMRes.h:67-241, Prec_impl.h:226-255,
-// LinearSolver.cpp:194-244.  As in ofh_cg.hip, the scalars live on the device
-// and every kernel returns at once when the stop flag is set; the host
-// synchronises once per chunk (BiCGStab) or per restart cycle (GMRes).
+// LinearSolver.cpp:194-244.  As in ofh_cg.hip, scalars live on the device,
+// reductions are folded by the kernel that consumes them, and every kernel
+// returns at once when the stop flag is set; the host synchronises once per
+// chunk of iterations (BiCGStab) or once per restart cycle (GMRes).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,11 +26,9 @@ __global__ __launch_bounds__(kBlock) void k_true_res(int nblocks, const int* __r
                                                     const double* __restrict__ val,
                                                     const double* __restrict__ b,
                                                     const double* __restrict__ x,
-                                                    double* partials, unsigned* counter,
-                                                    double* out) {
+                                                    double* __restrict__ partials) {
   __shared__ double lds[kTile64];
   __shared__ double red[2 * kBlock / kWave];
-  __shared__ bool last;
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
   const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
@@ -41,15 +40,20 @@ __global__ __launch_bounds__(kBlock) void k_true_res(int nblocks, const int* __r
     v[1] = b[i] * b[i];
   }
   block_sum_k<kBlock, 2>(v, red);
-  if (publish_partials<2>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
-    double t[2];
-    fold_partials<kBlock, 2>(t, partials, gridDim.x, red);
-    if (threadIdx.x == 0) {
-      double nb = sqrt(t[1]);
-      if (nb == 0) nb = 1;
-      *out = sqrt(t[0]) / nb;
-      *counter = 0u;
-    }
+  const int slot[2] = {0, 1};
+  write_partials<2>(v, slot, partials, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_true_fold(int nparts, const double* __restrict__ partials,
+                                                     double* out) {
+  __shared__ double red[2 * kBlock / kWave];
+  double t[2];
+  const int slot[2] = {0, 1};
+  fold_partials<kBlock, 2>(t, slot, partials, nparts, red);
+  if (threadIdx.x == 0) {
+    double nb = sqrt(t[1]);
+    if (nb == 0) nb = 1;
+    *out = sqrt(t[0]) / nb;
   }
 }
 
@@ -57,8 +61,9 @@ int true_relres(ofh_csr A, const double* b, const double* x, double* out) {
   ofh_ctx ctx = A->ctx;
   double* d_out = ctx->dev_scratch + 16;
   hipLaunchKernelGGL(k_true_res, dim3(A->nblocks), dim3(kBlock), 0, ctx->stream, A->nblocks,
-                     A->row_blocks, A->row_ptr, A->col, A->val, b, x, ctx->ws.partials,
-                     ctx->ws.counters + 8, d_out);
+                     A->row_blocks, A->row_ptr, A->col, A->val, b, x, ctx->ws.partials);
+  hipLaunchKernelGGL(k_true_fold, dim3(1), dim3(kBlock), 0, ctx->stream, A->nblocks,
+                     ctx->ws.partials, d_out);
   OFH_HIP(hipGetLastError());
   OFH_HIP(hipMemcpyAsync(ctx->host_pinned, d_out, sizeof(double), hipMemcpyDeviceToHost,
                          ctx->stream));
@@ -81,11 +86,6 @@ int fill_info(ofh_csr A, const double* b, const double* x, const KrylovState& h,
   return true_relres(A, b, x, &info->true_relres);
 }
 
-// ---------------------------------------------------------------------------
-// element-wise helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ bool halted(const KrylovState* st) { return st->halt != 0; }
-
 // Prec::solve(b,x) (Prec_impl.h:226-255) for IDENT and DIAG.
 __global__ void k_prec_diag(int n, const double* __restrict__ b, const double* __restrict__ dinv,
                             double* __restrict__ x) {
@@ -96,7 +96,7 @@ __global__ void k_prec_diag(int n, const double* __restrict__ b, const double* _
 // ===========================================================================
 // BiCGStab
 // ===========================================================================
-// t = A x, r = b - t, rr = r; (b,b), (r,r)
+// r = b - A x, rr = r; partials (b,b) -> 0, (r,r) -> 1  (BiCGStab.h:101-103)
 __global__ __launch_bounds__(kBlock) void k_bi_init(int nblocks, const int* __restrict__ rb,
                                                    const int* __restrict__ rp,
                                                    const int* __restrict__ col,
@@ -104,11 +104,10 @@ __global__ __launch_bounds__(kBlock) void k_bi_init(int nblocks, const int* __re
                                                    const double* __restrict__ b,
                                                    const double* __restrict__ x,
                                                    double* __restrict__ r,
-                                                   double* __restrict__ rr, KrylovState* st,
-                                                   double* partials, unsigned* counter) {
+                                                   double* __restrict__ rr,
+                                                   double* __restrict__ partials) {
   __shared__ double lds[kTile64];
   __shared__ double red[2 * kBlock / kWave];
-  __shared__ bool last;
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
   const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
@@ -122,56 +121,78 @@ __global__ __launch_bounds__(kBlock) void k_bi_init(int nblocks, const int* __re
     v[1] = ri * ri;
   }
   block_sum_k<kBlock, 2>(v, red);
-  if (publish_partials<2>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
-    double t[2];
-    fold_partials<kBlock, 2>(t, partials, gridDim.x, red);
-    if (threadIdx.x == 0) {
-      double nrm = sqrt(t[0]);
-      if (nrm == 0.0) nrm = 1;  // BiCGStab.h:109-110
-      st->nrm = nrm;
-      st->res = sqrt(t[1]) / nrm;
-      st->res0 = st->res;
-      st->it = 0;
-      st->sconv = 0;
-      st->halt = 0;
-      if (st->res <= st->tol) {  // BiCGStab.h:111
-        st->done = 1;
-        st->halt = 1;
-        st->status = 0;
-      } else {
-        st->rho = t[1];  // (rr,r) of the first step: rr == r
-        if (st->rho == 0.0) {
-          st->done = 1;
-          st->halt = 1;
-          st->status = 2;
-        }
-      }
-      *counter = 0u;
-    }
-  }
+  const int slot[2] = {0, 1};
+  write_partials<2>(v, slot, partials, blockIdx.x);
 }
 
-// p = r (first step) or r + beta (p - omega v); pp = P p (BiCGStab.h:122-129)
+// Stop tests closing iteration it-1 and the direction of iteration it:
+//   it == 1: ||r0||/||b|| <= toler -> 0 (BiCGStab.h:111-116); rho_1 = (r0,r0)
+//   it >  1: ||r||/||b|| < toler -> it-1; omega == 0 -> 3 (BiCGStab.h:149-157);
+//            loop end -> -(max_it+1); rho_1 = (rr,r) == 0 -> 2 (:117-120);
+//            beta = (rho_1/rho_2) (alpha/omega) (:125)
+// then p = r + beta (p - omega v), pp = P p (:122-129).
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_bi_dir(int n, const double* __restrict__ r,
+__global__ __launch_bounds__(kBlock) void k_bi_dir(int n, int it, int nparts,
+                                                  const double* __restrict__ r,
                                                   const double* __restrict__ v,
                                                   double* __restrict__ p, double* __restrict__ pp,
                                                   const double* __restrict__ dinv,
-                                                  const KrylovState* __restrict__ st) {
+                                                  KrylovState* st,
+                                                  const double* __restrict__ partials) {
   if (st->done) return;
-  const bool first = st->it == 0;
-  const double beta = st->beta, omega = st->omega;
+  __shared__ double red[2 * kBlock / kWave];
+  double t[2];
+  const int slot[2] = {0, 1};
+  fold_partials<kBlock, 2>(t, slot, partials, nparts, red);
+  bool stop = false;
+  int status = 0;
+  double nrm = st->nrm, res, rho, beta = 0.0;
+  if (it == 1) {
+    nrm = sqrt(t[0]);
+    if (nrm == 0.0) nrm = 1;
+    res = sqrt(t[1]) / nrm;
+    rho = t[1];  // (rr, r) with rr == r
+    if (res <= st->tol) { stop = true; status = 0; }
+    else if (it > st->max_it) { stop = true; status = -(st->max_it + 1); }
+    else if (rho == 0.0) { stop = true; status = 2; }
+  } else {
+    res = sqrt(t[0]) / nrm;
+    rho = t[1];
+    if (res < st->tol) { stop = true; status = it - 1; }
+    else if (st->omega == 0.0) { stop = true; status = 3; }
+    else if (it > st->max_it || !(res == res)) { stop = true; status = -(st->max_it + 1); }
+    else if (rho == 0.0) { stop = true; status = 2; }
+    else beta = (rho / st->rho_buf[(it - 1) & 1]) * (st->alpha / st->omega);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->it = it - 1;
+    st->res = res;
+    if (it == 1) {
+      st->nrm = nrm;
+      st->res0 = res;
+    }
+    if (stop) {
+      st->done = 1;
+      st->halt = 1;
+      st->status = status;
+    } else {
+      st->rho_buf[it & 1] = rho;
+      st->beta = beta;
+    }
+  }
+  if (stop) return;
+  const double omega = st->omega;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const double pi = first ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
+    const double pi = it == 1 ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
     p[i] = pi;
     if (PREC == OFH_DIAG_PREC) pp[i] = pi * dinv[i];
     else if (PREC == OFH_IDENT_PREC) pp[i] = pi;
   }
 }
 
-// y = A x with one or two fused dot products u1.y (u2.y) -> scalars.
-// MODE 0: v = A pp, alpha = rho/(rr,v)          (BiCGStab.h:130-131)
-// MODE 1: t = A ss, omega = (t,s)/(t,t)          (BiCGStab.h:140-141)
+// y = A x with fused partial dots:
+// MODE 0: v = A pp, (rr,v) -> slot 2                 (BiCGStab.h:130-131)
+// MODE 1: t = A ss, (t,s) -> slot 4, (t,t) -> slot 5  (BiCGStab.h:140-141)
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_bi_spmv(int nblocks, const int* __restrict__ rb,
                                                    const int* __restrict__ rp,
@@ -180,12 +201,11 @@ __global__ __launch_bounds__(kBlock) void k_bi_spmv(int nblocks, const int* __re
                                                    const double* __restrict__ x,
                                                    double* __restrict__ y,
                                                    const double* __restrict__ u,
-                                                   KrylovState* st, double* partials,
-                                                   unsigned* counter) {
-  if (MODE == 0 ? st->done : halted(st)) return;
+                                                   const KrylovState* __restrict__ st,
+                                                   double* __restrict__ partials) {
+  if (MODE == 0 ? st->done : st->halt) return;
   __shared__ double lds[kTile64];
   __shared__ double red[2 * kBlock / kWave];
-  __shared__ bool last;
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
   const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
@@ -194,36 +214,31 @@ __global__ __launch_bounds__(kBlock) void k_bi_spmv(int nblocks, const int* __re
   if (i < r1) {
     y[i] = s;
     if (MODE == 0) {
-      v[0] = u[i] * s;  // (rr, v)
+      v[0] = u[i] * s;
     } else {
-      v[0] = s * u[i];  // (t, s)
-      v[1] = s * s;     // (t, t)
+      v[0] = s * u[i];
+      v[1] = s * s;
     }
   }
   block_sum_k<kBlock, 2>(v, red);
-  if (publish_partials<2>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
-    double t[2];
-    fold_partials<kBlock, 2>(t, partials, gridDim.x, red);
-    if (threadIdx.x == 0) {
-      if (MODE == 0) st->alpha = st->rho / t[0];
-      else st->omega = t[0] / t[1];
-      *counter = 0u;
-    }
-  }
+  const int slot[2] = {MODE == 0 ? 2 : 4, MODE == 0 ? 6 : 5};
+  write_partials<2>(v, slot, partials, blockIdx.x);
 }
 
-// s = r - alpha v, ||s||; early exit flag; ss = P s (BiCGStab.h:132-139)
+// alpha = rho_1/(rr,v); s = r - alpha v; (s,s) -> slot 3; ss = P s
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_bi_s(int n, const double* __restrict__ r,
+__global__ __launch_bounds__(kBlock) void k_bi_s(int n, int it, int nparts_rv,
+                                                const double* __restrict__ r,
                                                 const double* __restrict__ v,
                                                 double* __restrict__ s, double* __restrict__ ss,
-                                                const double* __restrict__ dinv,
-                                                KrylovState* st, double* partials,
-                                                unsigned* counter) {
+                                                const double* __restrict__ dinv, KrylovState* st,
+                                                double* __restrict__ partials) {
   if (st->done) return;
   __shared__ double red[kBlock / kWave];
-  __shared__ bool last;
-  const double alpha = st->alpha;
+  double rv[1];
+  const int sl[1] = {2};
+  fold_partials<kBlock, 1>(rv, sl, partials, nparts_rv, red);
+  const double alpha = st->rho_buf[it & 1] / rv[0];
   double acc[1] = {0.0};
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const double si = r[i] - alpha * v[i];
@@ -233,91 +248,67 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(int n, const double* __restrict
     else if (PREC == OFH_IDENT_PREC) ss[i] = si;
   }
   block_sum_k<kBlock, 1>(acc, red);
-  if (publish_partials<1>(acc, partials, counter, blockIdx.x, gridDim.x, &last)) {
-    double t[1];
-    fold_partials<kBlock, 1>(t, partials, gridDim.x, red);
-    if (threadIdx.x == 0) {
-      const double res = sqrt(t[0]) / st->nrm;
-      if (res < st->tol) {
-        st->sconv = 1;
-        st->halt = 1;
-        st->sconv_res = res;
-      }
-      *counter = 0u;
-    }
+  const int slot[1] = {3};
+  write_partials<1>(acc, slot, partials, blockIdx.x);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = alpha;
+}
+
+// ||s||/||b|| < toler: early exit of BiCGStab.h:132-138 (one block)
+__global__ __launch_bounds__(kBlock) void k_bi_scheck(int nparts, KrylovState* st,
+                                                     const double* __restrict__ partials) {
+  if (st->done) return;
+  __shared__ double red[kBlock / kWave];
+  double t[1];
+  const int slot[1] = {3};
+  fold_partials<kBlock, 1>(t, slot, partials, nparts, red);
+  if (threadIdx.x != 0) return;
+  const double res = sqrt(t[0]) / st->nrm;
+  if (res < st->tol) {
+    st->sconv = 1;
+    st->halt = 1;
+    st->sconv_res = res;
   }
 }
 
-// x += alpha pp + omega ss; r = s - omega t; ||r||, (rr,r); stop tests and the
-// next step's beta (BiCGStab.h:117-121, 142-158).
-__global__ __launch_bounds__(kBlock) void k_bi_upd(int n, double* __restrict__ x,
-                                                  double* __restrict__ r,
+// x += alpha pp + omega ss; r = s - omega t; (r,r) -> 0, (rr,r) -> 1
+__global__ __launch_bounds__(kBlock) void k_bi_upd(int n, int it, int nparts_t,
+                                                  double* __restrict__ x, double* __restrict__ r,
                                                   const double* __restrict__ rr,
                                                   const double* __restrict__ pp,
                                                   const double* __restrict__ s,
                                                   const double* __restrict__ ss,
-                                                  const double* __restrict__ t,
-                                                  KrylovState* st, double* partials,
-                                                  unsigned* counter) {
+                                                  const double* __restrict__ t, KrylovState* st,
+                                                  double* __restrict__ partials) {
   if (st->done) return;
   __shared__ double red[2 * kBlock / kWave];
-  __shared__ bool last;
-  const double alpha = st->alpha, omega = st->omega;
-  const bool sconv = st->sconv;
+  const double alpha = st->alpha;
+  if (st->sconv) {  // x += alpha pp; return it (BiCGStab.h:133-137)
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+      x[i] += alpha * pp[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->it = it;
+      st->res = st->sconv_res;
+      st->done = 1;
+      st->status = it;
+    }
+    return;
+  }
+  double ts[2];
+  const int sl[2] = {4, 5};
+  fold_partials<kBlock, 2>(ts, sl, partials, nparts_t, red);
+  const double omega = ts[0] / ts[1];
   double v[2] = {0.0, 0.0};
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    if (sconv) {
-      x[i] += alpha * pp[i];
-    } else {
-      x[i] += alpha * pp[i] + omega * ss[i];
-      const double ri = s[i] - omega * t[i];
-      r[i] = ri;
-      v[0] += ri * ri;
-      v[1] += rr[i] * ri;
-    }
+    x[i] += alpha * pp[i] + omega * ss[i];
+    const double ri = s[i] - omega * t[i];
+    r[i] = ri;
+    v[0] += ri * ri;
+    v[1] += rr[i] * ri;
   }
   block_sum_k<kBlock, 2>(v, red);
-  if (publish_partials<2>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
-    double tt[2];
-    fold_partials<kBlock, 2>(tt, partials, gridDim.x, red);
-    if (threadIdx.x == 0) {
-      const int it = st->it + 1;
-      st->it = it;
-      if (sconv) {
-        st->res = st->sconv_res;
-        st->done = 1;
-        st->halt = 1;
-        st->status = it;
-      } else {
-        const double res = sqrt(tt[0]) / st->nrm;
-        st->res = res;
-        if (res < st->tol) {
-          st->done = 1;
-          st->halt = 1;
-          st->status = it;
-        } else if (st->omega == 0.0) {
-          st->done = 1;
-          st->halt = 1;
-          st->status = 3;
-        } else if (it >= st->max_it || !(res == res)) {
-          st->done = 1;
-          st->halt = 1;
-          st->status = -(st->max_it + 1);
-        } else {
-          const double rho = tt[1];
-          if (rho == 0.0) {
-            st->done = 1;
-            st->halt = 1;
-            st->status = 2;
-          } else {
-            st->beta = (rho / st->rho) * (st->alpha / st->omega);
-            st->rho = rho;
-          }
-        }
-      }
-      *counter = 0u;
-    }
-  }
+  const int slot[2] = {0, 1};
+  write_partials<2>(v, slot, partials, blockIdx.x);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->omega = omega;
 }
 
 template <int PREC>
@@ -340,31 +331,47 @@ static int bicgstab_impl(ofh_csr A, const double* b, double* x, int max_it, doub
   h0.tol = toler;
   h0.max_it = max_it;
   OFH_HIP(hipMemcpyAsync(st, &h0, sizeof(h0), hipMemcpyHostToDevice, s));
-  unsigned* c0 = ctx->ws.counters;
   double* part = ctx->ws.partials;
   const int g = reduce_grid(n);
-  hipLaunchKernelGGL(k_bi_init, dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks, A->row_blocks,
-                     A->row_ptr, A->col, A->val, b, x, r, rr, st, part, c0);
+  const int nb = A->nblocks;
+  hipLaunchKernelGGL(k_bi_init, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr,
+                     A->col, A->val, b, x, r, rr, part);
+  auto dir = [&](int it) -> int {
+    hipLaunchKernelGGL(k_bi_dir<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, it == 1 ? nb : g, r, v,
+                       p, pp, A->dinv, st, part);
+    if (PREC == OFH_SSOR_PREC) return launch_ssor(A, p, pp, &st->done, s);
+    return OFH_OK;
+  };
+  if ((rc = dir(1))) return rc;
+  OFH_HIP(hipGetLastError());
   KrylovState h{};
-  int enq = 0, chunk = 4;
-  for (;;) {
-    for (int c = 0; c < chunk && enq < max_it; ++c, ++enq) {
-      hipLaunchKernelGGL(k_bi_dir<PREC>, dim3(g), dim3(kBlock), 0, s, n, r, v, p, pp, A->dinv, st);
-      if (PREC == OFH_SSOR_PREC && (rc = launch_ssor(A, p, pp, &st->done, s))) return rc;
-      hipLaunchKernelGGL(k_bi_spmv<0>, dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
-                         A->row_blocks, A->row_ptr, A->col, A->val, pp, v, rr, st, part, c0 + 1);
-      hipLaunchKernelGGL(k_bi_s<PREC>, dim3(g), dim3(kBlock), 0, s, n, r, v, sv, ss, A->dinv, st,
-                         part, c0 + 2);
-      if (PREC == OFH_SSOR_PREC && (rc = launch_ssor(A, sv, ss, &st->sconv, s))) return rc;
-      hipLaunchKernelGGL(k_bi_spmv<1>, dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
-                         A->row_blocks, A->row_ptr, A->col, A->val, ss, t, sv, st, part, c0 + 3);
-      hipLaunchKernelGGL(k_bi_upd, dim3(g), dim3(kBlock), 0, s, n, x, r, rr, pp, sv, ss, t, st,
-                         part, c0 + 4);
+  if ((rc = read_state(ctx, &h))) return rc;
+  int enq = 0, it_prev = 0;
+  double res_prev = h.res;
+  while (!h.done && enq < max_it) {
+    const int chunk = next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq);
+    it_prev = h.it;
+    res_prev = h.res;
+    for (int c = 0; c < chunk; ++c) {
+      const int it = ++enq;
+      hipLaunchKernelGGL(k_bi_spmv<0>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
+                         A->row_ptr, A->col, A->val, pp, v, rr, st, part);
+      hipLaunchKernelGGL(k_bi_s<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, nb, r, v, sv, ss,
+                         A->dinv, st, part);
+      hipLaunchKernelGGL(k_bi_scheck, dim3(1), dim3(kBlock), 0, s, g, st, part);
+      if (PREC == OFH_SSOR_PREC && (rc = launch_ssor(A, sv, ss, &st->halt, s))) return rc;
+      hipLaunchKernelGGL(k_bi_spmv<1>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
+                         A->row_ptr, A->col, A->val, ss, t, sv, st, part);
+      hipLaunchKernelGGL(k_bi_upd, dim3(g), dim3(kBlock), 0, s, n, it, nb, x, r, rr, pp, sv, ss,
+                         t, st, part);
+      if ((rc = dir(it + 1))) return rc;
     }
     OFH_HIP(hipGetLastError());
     if ((rc = read_state(ctx, &h))) return rc;
-    if (h.done || enq >= max_it) break;
-    chunk = next_chunk(h.it, h.res, h.res0, toler, max_it - enq);
+  }
+  if (!h.done) {
+    h.done = 1;
+    h.status = -(max_it + 1);
   }
   return fill_info(A, b, x, h, t0, info);
 }
@@ -372,9 +379,8 @@ static int bicgstab_impl(ofh_csr A, const double* b, double* x, int max_it, doub
 // ===========================================================================
 // GMRes(m)
 // ===========================================================================
-// r = P (b - A x); (r,r), (b,b).  INIT: GMRes.h:164-179 (returns it=1 when
-// already converged); else the restart residual of GMRes.h:226-232.
-template <int PREC, bool INIT>
+// r = P (b - A x); (r,r) -> 0, (b,b) -> 1     (GMRes.h:164-167, 226-229)
+template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_gm_resid(int nblocks, const int* __restrict__ rb,
                                                     const int* __restrict__ rp,
                                                     const int* __restrict__ col,
@@ -383,12 +389,11 @@ __global__ __launch_bounds__(kBlock) void k_gm_resid(int nblocks, const int* __r
                                                     const double* __restrict__ x,
                                                     double* __restrict__ r,
                                                     const double* __restrict__ dinv,
-                                                    KrylovState* st, double* partials,
-                                                    unsigned* counter) {
-  if (!INIT && st->done) return;
+                                                    const KrylovState* __restrict__ st,
+                                                    double* __restrict__ partials) {
+  if (st->done) return;
   __shared__ double lds[kTile64];
   __shared__ double red[2 * kBlock / kWave];
-  __shared__ bool last;
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
   const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
@@ -396,94 +401,17 @@ __global__ __launch_bounds__(kBlock) void k_gm_resid(int nblocks, const int* __r
   double v[2] = {0.0, 0.0};
   if (i < r1) {
     double ri = b[i] - s;
-    if (PREC == OFH_DIAG_PREC) ri *= dinv[i];
+    if (PREC == OFH_DIAG_PREC) ri *= dinv[i];  // P.solve(r) in place: r_i *= pivot_i
     r[i] = ri;
     v[0] = ri * ri;
     v[1] = b[i] * b[i];
   }
   block_sum_k<kBlock, 2>(v, red);
-  if (publish_partials<2>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
-    double t[2];
-    fold_partials<kBlock, 2>(t, partials, gridDim.x, red);
-    if (threadIdx.x == 0 && PREC != OFH_SSOR_PREC) {
-      if (INIT) {
-        double nb = sqrt(t[1]);
-        if (nb == 0.0) nb = 1;
-        st->nrm = nb;
-        st->it = 1;
-        st->done = 0;
-        st->sconv = 0;
-        st->halt = 0;
-      }
-      st->h_norm = sqrt(t[0]);  // beta
-      const double res = st->h_norm / st->nrm;
-      st->res = res;
-      if (INIT) {
-        st->res0 = res;
-        if (res <= st->tol) {
-          st->done = 1;
-          st->halt = 1;
-          st->status = 1;
-        }
-      } else if (res < st->tol) {
-        st->done = 1;
-        st->halt = 1;
-        st->status = st->it;
-      }
-    }
-    if (threadIdx.x == 0) *counter = 0u;
-  }
+  const int slot[2] = {0, 1};
+  write_partials<2>(v, slot, partials, blockIdx.x);
 }
 
-// SSOR variant of the residual bookkeeping: r was written by the sweeps.
-template <bool INIT>
-__global__ __launch_bounds__(kBlock) void k_gm_resid_norm(int n, const double* __restrict__ r,
-                                                         const double* __restrict__ b,
-                                                         KrylovState* st, double* partials,
-                                                         unsigned* counter) {
-  if (!INIT && st->done) return;
-  __shared__ double red[2 * kBlock / kWave];
-  __shared__ bool last;
-  double v[2] = {0.0, 0.0};
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    v[0] += r[i] * r[i];
-    v[1] += b[i] * b[i];
-  }
-  block_sum_k<kBlock, 2>(v, red);
-  if (publish_partials<2>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
-    double t[2];
-    fold_partials<kBlock, 2>(t, partials, gridDim.x, red);
-    if (threadIdx.x == 0) {
-      if (INIT) {
-        double nb = sqrt(t[1]);
-        if (nb == 0.0) nb = 1;
-        st->nrm = nb;
-        st->it = 1;
-        st->done = 0;
-        st->sconv = 0;
-        st->halt = 0;
-      }
-      st->h_norm = sqrt(t[0]);
-      const double res = st->h_norm / st->nrm;
-      st->res = res;
-      if (INIT) {
-        st->res0 = res;
-        if (res <= st->tol) {
-          st->done = 1;
-          st->halt = 1;
-          st->status = 1;
-        }
-      } else if (res < st->tol) {
-        st->done = 1;
-        st->halt = 1;
-        st->status = st->it;
-      }
-      *counter = 0u;
-    }
-  }
-}
-
-// plain b - A x into w (SSOR paths precondition it with the sweeps)
+// plain b - A x into w (the SSOR path preconditions it with the sweeps)
 __global__ __launch_bounds__(kBlock) void k_resid_plain(int nblocks, const int* __restrict__ rb,
                                                        const int* __restrict__ rp,
                                                        const int* __restrict__ col,
@@ -491,15 +419,64 @@ __global__ __launch_bounds__(kBlock) void k_resid_plain(int nblocks, const int* 
                                                        const double* __restrict__ b,
                                                        const double* __restrict__ x,
                                                        double* __restrict__ w,
-                                                       const KrylovState* __restrict__ st,
-                                                       int check) {
-  if (check && st->done) return;
+                                                       const KrylovState* __restrict__ st) {
+  if (st->done) return;
   __shared__ double lds[kTile64];
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
   const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
   const int i = r0 + threadIdx.x;
   if (i < r1) w[i] = b[i] - s;
+}
+
+// SSOR: (r,r) -> 0, (b,b) -> 1 of the preconditioned residual
+__global__ __launch_bounds__(kBlock) void k_gm_rnorm(int n, const double* __restrict__ r,
+                                                    const double* __restrict__ b,
+                                                    const KrylovState* __restrict__ st,
+                                                    double* __restrict__ partials) {
+  if (st->done) return;
+  __shared__ double red[2 * kBlock / kWave];
+  double v[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    v[0] += r[i] * r[i];
+    v[1] += b[i] * b[i];
+  }
+  block_sum_k<kBlock, 2>(v, red);
+  const int slot[2] = {0, 1};
+  write_partials<2>(v, slot, partials, blockIdx.x);
+}
+
+// beta = ||r||; stop tests: INIT (GMRes.h:169-179, returns it = 1) or the
+// restart test (GMRes.h:230-238).  One block.
+__global__ __launch_bounds__(kBlock) void k_gm_rcheck(int init, int nparts, KrylovState* st,
+                                                     const double* __restrict__ partials) {
+  if (st->done) return;
+  __shared__ double red[2 * kBlock / kWave];
+  double t[2];
+  const int slot[2] = {0, 1};
+  fold_partials<kBlock, 2>(t, slot, partials, nparts, red);
+  if (threadIdx.x != 0) return;
+  if (init) {
+    double nb = sqrt(t[1]);
+    if (nb == 0.0) nb = 1;
+    st->nrm = nb;
+    st->it = 1;
+  }
+  st->h_norm = sqrt(t[0]);
+  const double res = st->h_norm / st->nrm;
+  st->res = res;
+  if (init) {
+    st->res0 = res;
+    if (res <= st->tol) {
+      st->done = 1;
+      st->halt = 1;
+      st->status = 1;
+    }
+  } else if (res < st->tol) {
+    st->done = 1;
+    st->halt = 1;
+    st->status = st->it;
+  }
 }
 
 // v0 = (1/beta) r; s = 0; s[0] = beta (GMRes.h:187-189)
@@ -522,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_gm_av(int nblocks, const int* __rest
                                                  double* __restrict__ w,
                                                  const double* __restrict__ dinv,
                                                  const KrylovState* __restrict__ st) {
-  if (halted(st)) return;
+  if (st->halt) return;
   __shared__ double lds[kTile64];
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
@@ -531,19 +508,26 @@ __global__ __launch_bounds__(kBlock) void k_gm_av(int nblocks, const int* __rest
   if (i < r1) w[i] = PREC == OFH_DIAG_PREC ? s * dinv[i] : s;
 }
 
-// Modified Gram-Schmidt step k (GMRes.h:193-197), fused with the previous
-// step's update:  w -= H(k-1,i) v_{k-1};  H(k,i) = (w, v_k).
-// k == i+1 (v_k == null): the final update and H(i+1,i) = ||w||.
-__global__ __launch_bounds__(kBlock) void k_gm_mgs(int n, int k, int i, int m1,
+// Modified Gram-Schmidt step k of column i (GMRes.h:193-197):
+//   k > 0: H(k-1,i) = fold of the previous step; w -= H(k-1,i) v_{k-1}
+//   partial (w, v_k) (k <= i) or (w,w) (k == i+1) -> slot k&1
+__global__ __launch_bounds__(kBlock) void k_gm_mgs(int n, int k, int i, int m1, int nparts,
                                                   double* __restrict__ w,
                                                   const double* __restrict__ vprev,
                                                   const double* __restrict__ vk,
-                                                  double* __restrict__ H, KrylovState* st,
-                                                  double* partials, unsigned* counter) {
-  if (halted(st)) return;
+                                                  double* __restrict__ H,
+                                                  const KrylovState* __restrict__ st,
+                                                  double* __restrict__ partials) {
+  if (st->halt) return;
   __shared__ double red[kBlock / kWave];
-  __shared__ bool last;
-  const double h = k > 0 ? H[(k - 1) * m1 + i] : 0.0;
+  double h = 0.0;
+  if (k > 0) {
+    double t[1];
+    const int sl[1] = {(k - 1) & 1};
+    fold_partials<kBlock, 1>(t, sl, partials, nparts, red);
+    h = t[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) H[(k - 1) * m1 + i] = h;
+  }
   double v[1] = {0.0};
   for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
     double wj = w[j];
@@ -554,24 +538,27 @@ __global__ __launch_bounds__(kBlock) void k_gm_mgs(int n, int k, int i, int m1,
     v[0] += vk ? wj * vk[j] : wj * wj;
   }
   block_sum_k<kBlock, 1>(v, red);
-  if (publish_partials<1>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
-    double t[1];
-    fold_partials<kBlock, 1>(t, partials, gridDim.x, red);
-    if (threadIdx.x == 0) {
-      H[k * m1 + i] = vk ? t[0] : sqrt(t[0]);
-      *counter = 0u;
-    }
-  }
+  const int slot[1] = {k & 1};
+  write_partials<1>(v, slot, partials, blockIdx.x);
 }
 
-// v_{i+1} = (1/H(i+1,i)) w  (GMRes.h:199)
-__global__ void k_gm_scale(int n, int i, int m1, const double* __restrict__ w,
-                           double* __restrict__ vnext, const double* __restrict__ H,
-                           const KrylovState* __restrict__ st) {
-  if (halted(st)) return;
-  const double sc = 1.0 / H[(i + 1) * m1 + i];
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) vnext[j] = sc * w[j];
+// H(i+1,i) = ||w||; v_{i+1} = (1/H(i+1,i)) w  (GMRes.h:198-199)
+__global__ __launch_bounds__(kBlock) void k_gm_scale(int n, int i, int m1, int nparts,
+                                                    const double* __restrict__ w,
+                                                    double* __restrict__ vnext,
+                                                    double* __restrict__ H,
+                                                    const KrylovState* __restrict__ st,
+                                                    const double* __restrict__ partials) {
+  if (st->halt) return;
+  __shared__ double red[kBlock / kWave];
+  double t[1];
+  const int sl[1] = {(i + 1) & 1};
+  fold_partials<kBlock, 1>(t, sl, partials, nparts, red);
+  const double h = sqrt(t[0]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) H[(i + 1) * m1 + i] = h;
+  const double sc = 1.0 / h;
+  for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+    vnext[j] = sc * w[j];
 }
 
 __device__ void gm_rotate(double& dx, double& dy, double cs, double sn) {
@@ -580,10 +567,10 @@ __device__ void gm_rotate(double& dx, double& dy, double cs, double sn) {
   dx = temp;
 }
 
-// Givens rotations and the residual estimate (GMRes.h:200-210)
+// Givens rotations and the residual estimate (GMRes.h:200-210); one thread.
 __global__ void k_gm_givens(int i, int m1, double* H, double* sv, double* cs, double* sn,
                             KrylovState* st) {
-  if (halted(st) || threadIdx.x != 0) return;
+  if (st->halt || threadIdx.x != 0) return;
   for (int k = 0; k < i; k++) gm_rotate(H[k * m1 + i], H[(k + 1) * m1 + i], cs[k], sn[k]);
   const double dx = H[i * m1 + i], dy = H[(i + 1) * m1 + i];
   double c, s;
@@ -606,8 +593,8 @@ __global__ void k_gm_givens(int i, int m1, double* H, double* sv, double* cs, do
   const double res = fabs(sv[i + 1]) / st->nrm;
   st->res = res;
   if (res < st->tol) {
-    st->sconv = 1;
-    st->halt = 1;  // converged inside the cycle: update(x, i) follows
+    st->sconv = 1;  // converged inside the cycle: update(x, i) follows
+    st->halt = 1;
     st->cycle_i = i;
     st->status = st->it;
   } else {
@@ -646,7 +633,10 @@ __global__ void k_gm_xupd(int n, int m, const double* __restrict__ V, const doub
 
 __global__ void k_gm_finish_cycle(KrylovState* st) {
   if (threadIdx.x != 0 || st->done) return;
-  if (st->sconv) st->done = 1;
+  if (st->sconv) {
+    st->done = 1;
+    st->halt = 1;
+  }
 }
 
 template <int PREC>
@@ -678,27 +668,22 @@ static int gmres_impl(ofh_csr A, const double* b, double* x, int m, int max_it, 
   h0.tol = toler;
   h0.max_it = max_it;
   OFH_HIP(hipMemcpyAsync(st, &h0, sizeof(h0), hipMemcpyHostToDevice, s));
-  unsigned* c0 = ctx->ws.counters;
   double* part = ctx->ws.partials;
   const int g = reduce_grid(n);
+  const int nb = A->nblocks;
   const int ge = (n + 255) / 256;
   auto residual = [&](bool init) -> int {
     if (PREC == OFH_SSOR_PREC) {
-      hipLaunchKernelGGL(k_resid_plain, dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
-                         A->row_blocks, A->row_ptr, A->col, A->val, b, x, tmp, st, (int)!init);
-      int e = launch_ssor(A, tmp, r, init ? nullptr : &st->done, s);
+      hipLaunchKernelGGL(k_resid_plain, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
+                         A->row_ptr, A->col, A->val, b, x, tmp, st);
+      int e = launch_ssor(A, tmp, r, &st->done, s);
       if (e) return e;
-      if (init)
-        hipLaunchKernelGGL(k_gm_resid_norm<true>, dim3(g), dim3(kBlock), 0, s, n, r, b, st, part, c0);
-      else
-        hipLaunchKernelGGL(k_gm_resid_norm<false>, dim3(g), dim3(kBlock), 0, s, n, r, b, st, part, c0);
-    } else if (init) {
-      hipLaunchKernelGGL((k_gm_resid<PREC, true>), dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
-                         A->row_blocks, A->row_ptr, A->col, A->val, b, x, r, A->dinv, st, part, c0);
+      hipLaunchKernelGGL(k_gm_rnorm, dim3(g), dim3(kBlock), 0, s, n, r, b, st, part);
+      hipLaunchKernelGGL(k_gm_rcheck, dim3(1), dim3(kBlock), 0, s, (int)init, g, st, part);
     } else {
-      hipLaunchKernelGGL((k_gm_resid<PREC, false>), dim3(A->nblocks), dim3(kBlock), 0, s,
-                         A->nblocks, A->row_blocks, A->row_ptr, A->col, A->val, b, x, r, A->dinv,
-                         st, part, c0);
+      hipLaunchKernelGGL(k_gm_resid<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
+                         A->row_ptr, A->col, A->val, b, x, r, A->dinv, st, part);
+      hipLaunchKernelGGL(k_gm_rcheck, dim3(1), dim3(kBlock), 0, s, (int)init, nb, st, part);
     }
     return OFH_OK;
   };
@@ -712,22 +697,21 @@ static int gmres_impl(ofh_csr A, const double* b, double* x, int m, int max_it, 
     for (int i = 0; i < m && it <= max_it; i++, it++) {
       double* vi = V + (size_t)i * n;
       if (PREC == OFH_SSOR_PREC) {
-        hipLaunchKernelGGL(k_gm_av<OFH_IDENT_PREC>, dim3(A->nblocks), dim3(kBlock), 0, s,
-                           A->nblocks, A->row_blocks, A->row_ptr, A->col, A->val, vi, tmp,
-                           A->dinv, st);
+        hipLaunchKernelGGL(k_gm_av<OFH_IDENT_PREC>, dim3(nb), dim3(kBlock), 0, s, nb,
+                           A->row_blocks, A->row_ptr, A->col, A->val, vi, tmp, A->dinv, st);
         if ((rc = launch_ssor(A, tmp, w, &st->halt, s))) return rc;
       } else {
-        hipLaunchKernelGGL(k_gm_av<PREC>, dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
-                           A->row_blocks, A->row_ptr, A->col, A->val, vi, w, A->dinv, st);
+        hipLaunchKernelGGL(k_gm_av<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
+                           A->row_ptr, A->col, A->val, vi, w, A->dinv, st);
       }
       for (int k = 0; k <= i + 1; ++k) {
         const double* vprev = k > 0 ? V + (size_t)(k - 1) * n : nullptr;
         const double* vk = k <= i ? V + (size_t)k * n : nullptr;
-        hipLaunchKernelGGL(k_gm_mgs, dim3(g), dim3(kBlock), 0, s, n, k, i, m1, w, vprev, vk, H,
-                           st, part, c0 + 5);
+        hipLaunchKernelGGL(k_gm_mgs, dim3(g), dim3(kBlock), 0, s, n, k, i, m1, g, w, vprev, vk,
+                           H, st, part);
       }
-      hipLaunchKernelGGL(k_gm_scale, dim3(ge), dim3(256), 0, s, n, i, m1, w,
-                         V + (size_t)(i + 1) * n, H, st);
+      hipLaunchKernelGGL(k_gm_scale, dim3(g), dim3(kBlock), 0, s, n, i, m1, g, w,
+                         V + (size_t)(i + 1) * n, H, st, part);
       hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(64), 0, s, i, m1, H, sv, cs, sn, st);
     }
     hipLaunchKernelGGL(k_gm_backsolve, dim3(1), dim3(64), 0, s, m, m1, H, sv, y, st);

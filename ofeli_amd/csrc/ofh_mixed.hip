@@ -25,35 +25,23 @@ namespace ofh {
 
 constexpr double kDelta = 0.1;  // recursive-residual drop that triggers a replacement
 
-struct MixedState {
-  double rho, rho_1, alpha, beta, nrm, res_rec, res_true, rmax;
-  double tol, res0, pad[6];
-  int it, done, status, max_it;
-  int pause, halt, replacements, pad1;
-};
-static_assert(sizeof(MixedState) <= 512, "state block");
-
-inline MixedState* mstate_ptr(ofh_ctx ctx) {
-  return reinterpret_cast<MixedState*>(reinterpret_cast<char*>(ctx->dev_scratch) + 1024);
-}
-
 template <int PREC>
-__device__ __forceinline__ float zf(int i, const float* __restrict__ r, const float* __restrict__ d) {
-  return PREC == OFH_DIAG_PREC ? r[i] * d[i] : r[i];
+__device__ __forceinline__ float zf(int i, float ri, const float* __restrict__ d) {
+  return PREC == OFH_DIAG_PREC ? ri * d[i] : ri;
 }
 
-// FP64 residual r = b - A x64 -> r32; (b,b) | (r,r) in FP64, (r32,z32), (r32,r32).
-// INIT: set up the state; otherwise a residual replacement.
-template <int PREC, bool INIT>
+// FP64 residual r = b - A x64 -> r32 (rounded), x32 = 0; partials
+// (b,b) -> 0, (r,r) -> 1 in FP64, (r32,z32) -> 2, (r32,r32) -> 3.
+template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_mx_resid(
     int nblocks, const int* __restrict__ rb, const int* __restrict__ rp,
     const int* __restrict__ col, const double* __restrict__ val, const double* __restrict__ b,
     const double* __restrict__ x, float* __restrict__ r32, float* __restrict__ x32,
-    const float* __restrict__ dinv32, MixedState* st, double* partials, unsigned* counter) {
-  if (!INIT && st->done) return;
+    const float* __restrict__ dinv32, const KrylovState* __restrict__ st,
+    double* __restrict__ partials) {
+  if (st->done) return;
   __shared__ double lds[kTile64];
   __shared__ double red[4 * kBlock / kWave];
-  __shared__ bool last;
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
   const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
@@ -64,73 +52,111 @@ __global__ __launch_bounds__(kBlock) void k_mx_resid(
     const float rf = (float)ri;
     r32[i] = rf;
     x32[i] = 0.0f;
-    const float z = PREC == OFH_DIAG_PREC ? rf * dinv32[i] : rf;
+    const float z = zf<PREC>(i, rf, dinv32);
     v[0] = bi * bi;
     v[1] = ri * ri;
     v[2] = (double)rf * (double)z;
     v[3] = (double)rf * (double)rf;
   }
   block_sum_k<kBlock, 4>(v, red);
-  if (publish_partials<4>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
+  const int slot[4] = {0, 1, 2, 3};
+  write_partials<4>(v, slot, partials, blockIdx.x);
+}
+
+// Decision and direction after `k` inner iterations.
+//   PHASE 0 (after k_mx_upd): recursive residual; pause for a replacement
+//           when it dropped by kDelta, claims convergence or k hits max_it;
+//           else beta = rho_k/rho_{k-1}, p = z + beta p.
+//   PHASE 1 (after the initial k_mx_resid): ||b||, stop test of the guess, p = z.
+//   PHASE 2 (after a replacement): true residual stop test; else keep p:
+//           beta = (r,z)_new / rho_{k-1}, p = z + beta p.
+template <int PREC, int PHASE>
+__global__ __launch_bounds__(kBlock) void k_mx_dir(int n, int k, int nparts,
+                                                  const float* __restrict__ r32,
+                                                  const float* __restrict__ dinv32,
+                                                  float* __restrict__ p32, KrylovState* st,
+                                                  const double* __restrict__ partials) {
+  if (PHASE == 0 ? st->halt : st->done) return;
+  __shared__ double red[4 * kBlock / kWave];
+  double nrm = st->nrm, res, rz;
+  bool stop = false, pause = false;
+  int status = 0;
+  if (PHASE == 0) {
+    double t[2];
+    const int slot[2] = {4, 5};
+    fold_partials<kBlock, 2>(t, slot, partials, nparts, red);
+    res = sqrt(t[0]) / nrm;
+    rz = t[1];
+    pause = res <= kDelta * st->rmax || res <= st->tol || k >= st->max_it || !(res == res);
+  } else {
     double t[4];
-    fold_partials<kBlock, 4>(t, partials, gridDim.x, red);
-    if (threadIdx.x == 0) {
-      if (INIT) {
-        double nrm = sqrt(t[0]);
-        if (nrm == 0) nrm = 1;
+    const int slot[4] = {0, 1, 2, 3};
+    fold_partials<kBlock, 4>(t, slot, partials, nparts, red);
+    if (PHASE == 1) {
+      nrm = sqrt(t[0]);
+      if (nrm == 0) nrm = 1;
+    }
+    res = sqrt(t[1]) / nrm;
+    rz = t[2];
+    if (res <= st->tol) {
+      stop = true;
+      status = k;
+    } else if (k >= st->max_it || !(res == res)) {
+      stop = true;
+      status = -(st->max_it + 1);
+    }
+  }
+  const double beta = k > 0 ? rz / st->rho_buf[(k - 1) & 1] : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->it = k;
+    if (PHASE == 0) {
+      st->res = res;  // recursive estimate
+      if (pause) {
+        st->pause = 1;
+        st->halt = 1;
+      }
+    } else {
+      st->sconv_res = res;  // true FP64 residual
+      st->res = res;
+      st->rmax = res;
+      if (PHASE == 1) {
         st->nrm = nrm;
-        st->it = 0;
-        st->replacements = 0;
+        st->res0 = res;
       } else {
         st->replacements += 1;
       }
-      const double res = sqrt(t[1]) / st->nrm;
-      st->res_true = res;
-      st->res_rec = sqrt(t[3]) / st->nrm;
-      if (INIT) st->res0 = res;
-      st->rmax = res;
       st->pause = 0;
-      if (res <= st->tol) {
+      st->halt = stop ? 1 : 0;
+      if (stop) {
         st->done = 1;
-        st->status = st->it;
-      } else if (!INIT && (st->it >= st->max_it || !(res == res))) {
-        st->done = 1;
-        st->status = -(st->max_it + 1);
-      } else {
-        if (!INIT) st->beta = t[2] / st->rho_1;  // keep the direction p
-        st->rho = t[2];
+        st->status = status;
       }
-      st->halt = st->done;
-      *counter = 0u;
+    }
+    if (!stop && !pause) {
+      st->rho_buf[k & 1] = rz;
+      st->beta = beta;
     }
   }
-}
-
-template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_mx_dir(int n, const float* __restrict__ r32,
-                                                  const float* __restrict__ dinv32,
-                                                  float* __restrict__ p32,
-                                                  const MixedState* __restrict__ st) {
-  if (st->halt) return;
-  const bool first = st->it == 0;
-  const float beta = (float)st->beta;
+  if (stop || pause) return;
+  const float bf = (float)beta;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const float z = zf<PREC>(i, r32, dinv32);
-    p32[i] = first ? z : z + beta * p32[i];
+    const float z = zf<PREC>(i, r32[i], dinv32);
+    p32[i] = k == 0 ? z : z + bf * p32[i];
   }
 }
 
+// q32 = A32 p32; (p,q) -> slot 6 (FP64 accumulation)
 __global__ __launch_bounds__(kBlock) void k_mx_spmv(int nblocks, const int* __restrict__ rb,
                                                    const int* __restrict__ rp,
                                                    const int* __restrict__ col,
                                                    const float* __restrict__ val32,
                                                    const float* __restrict__ p32,
-                                                   float* __restrict__ q32, MixedState* st,
-                                                   double* partials, unsigned* counter) {
+                                                   float* __restrict__ q32,
+                                                   const KrylovState* __restrict__ st,
+                                                   double* __restrict__ partials) {
   if (st->halt) return;
   __shared__ float lds[kTile32];
   __shared__ double red[kBlock / kWave];
-  __shared__ bool last;
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
   const float s = csr_stream_rowsum<float, kTile32>(r0, r1, rp, col, val32, p32, lds, 0.0f);
@@ -141,61 +167,43 @@ __global__ __launch_bounds__(kBlock) void k_mx_spmv(int nblocks, const int* __re
     v[0] = (double)p32[i] * (double)s;
   }
   block_sum_k<kBlock, 1>(v, red);
-  if (publish_partials<1>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
-    double t[1];
-    fold_partials<kBlock, 1>(t, partials, gridDim.x, red);
-    if (threadIdx.x == 0) {
-      st->alpha = st->rho / t[0];
-      *counter = 0u;
-    }
-  }
+  const int slot[1] = {6};
+  write_partials<1>(v, slot, partials, blockIdx.x);
 }
 
+// alpha = rho/(p,q); x32 += alpha p; r32 -= alpha q; (r,r) -> 4, (r,z) -> 5
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_mx_upd(int n, float* __restrict__ x32,
-                                                  float* __restrict__ r32,
+__global__ __launch_bounds__(kBlock) void k_mx_upd(int n, int k, int nparts_pq,
+                                                  float* __restrict__ x32, float* __restrict__ r32,
                                                   const float* __restrict__ p32,
                                                   const float* __restrict__ q32,
                                                   const float* __restrict__ dinv32,
-                                                  MixedState* st, double* partials,
-                                                  unsigned* counter) {
+                                                  KrylovState* st, double* __restrict__ partials) {
   if (st->halt) return;
   __shared__ double red[2 * kBlock / kWave];
-  __shared__ bool last;
-  const float alpha = (float)st->alpha;
+  double pq[1];
+  const int sl[1] = {6};
+  fold_partials<kBlock, 1>(pq, sl, partials, nparts_pq, red);
+  const double alpha = st->rho_buf[(k - 1) & 1] / pq[0];
+  const float af = (float)alpha;
   double v[2] = {0.0, 0.0};
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    x32[i] += alpha * p32[i];
-    const float ri = r32[i] - alpha * q32[i];
+    x32[i] += af * p32[i];
+    const float ri = r32[i] - af * q32[i];
     r32[i] = ri;
-    const float z = PREC == OFH_DIAG_PREC ? ri * dinv32[i] : ri;
+    const float z = zf<PREC>(i, ri, dinv32);
     v[0] += (double)ri * (double)ri;
     v[1] += (double)ri * (double)z;
   }
   block_sum_k<kBlock, 2>(v, red);
-  if (publish_partials<2>(v, partials, counter, blockIdx.x, gridDim.x, &last)) {
-    double t[2];
-    fold_partials<kBlock, 2>(t, partials, gridDim.x, red);
-    if (threadIdx.x == 0) {
-      const int it = st->it + 1;
-      st->it = it;
-      const double res = sqrt(t[0]) / st->nrm;
-      st->res_rec = res;
-      st->rho_1 = st->rho;
-      st->beta = t[1] / st->rho;
-      st->rho = t[1];
-      if (res <= kDelta * st->rmax || res <= st->tol || it >= st->max_it || !(res == res)) {
-        st->pause = 1;
-        st->halt = 1;
-      }
-      *counter = 0u;
-    }
-  }
+  const int slot[2] = {4, 5};
+  write_partials<2>(v, slot, partials, blockIdx.x);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = alpha;
 }
 
 // x64 += x32 (the correction accumulated since the last replacement)
 __global__ void k_mx_accum(int n, double* __restrict__ x, const float* __restrict__ x32,
-                           const MixedState* __restrict__ st) {
+                           const KrylovState* __restrict__ st) {
   if (st->done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] += (double)x32[i];
@@ -217,55 +225,57 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   float* p32 = r32 + n;
   float* q32 = p32 + n;
   float* x32 = q32 + n;
-  MixedState* st = mstate_ptr(ctx);
-  MixedState h0{};
+  KrylovState* st = state_ptr(ctx);
+  KrylovState h0{};
   h0.tol = toler;
   h0.max_it = max_it;
   OFH_HIP(hipMemcpyAsync(st, &h0, sizeof(h0), hipMemcpyHostToDevice, s));
-  unsigned* c0 = ctx->ws.counters + 16;
   double* part = ctx->ws.partials;
   const int g = reduce_grid(n);
+  const int nb = A->nblocks;
   const int ge = (n + 255) / 256;
-  hipLaunchKernelGGL((k_mx_resid<PREC, true>), dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
-                     A->row_blocks, A->row_ptr, A->col, A->val, b, x, r32, x32, A->dinv32, st,
-                     part, c0);
+  hipLaunchKernelGGL(k_mx_resid<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
+                     A->row_ptr, A->col, A->val, b, x, r32, x32, A->dinv32, st, part);
+  hipLaunchKernelGGL((k_mx_dir<PREC, 1>), dim3(g), dim3(kBlock), 0, s, n, 0, nb, r32, A->dinv32,
+                     p32, st, part);
   OFH_HIP(hipGetLastError());
-  MixedState h{};
-  if ((rc = read_dev(ctx, st, &h))) return rc;
-  double last_true = h.res_true;
-  int stagnant = 0, fallback = 0;
+  KrylovState h{};
+  if ((rc = read_state(ctx, &h))) return rc;
+  double last_true = h.res;
+  int stagnant = 0, fallback = 0, it_prev = 0;
+  double res_prev = h.res;
   while (!h.done) {
-    // iterations expected until the next replacement
-    int chunk = 32;
-    if (h.it > 0 && h.res_rec < h.res0) {
-      double rate = std::pow(h.res_rec / h.res0, 1.0 / h.it);
-      double target = std::max(kDelta * h.rmax, toler);
-      if (rate < 1.0 && target < h.res_rec)
-        chunk = (int)std::ceil(std::log(target / h.res_rec) / std::log(rate) * 1.05) + 2;
-    }
-    chunk = std::max(4, std::min(chunk, 1024));
-    chunk = std::max(1, std::min(chunk, max_it - h.it));
+    int k = h.it;
+    const double target = std::max(kDelta * h.rmax, toler);
+    const int chunk = next_chunk(k, h.res, it_prev, res_prev, target, max_it - k);
+    it_prev = k;
+    res_prev = h.res;
     for (int c = 0; c < chunk; ++c) {
-      hipLaunchKernelGGL(k_mx_dir<PREC>, dim3(g), dim3(kBlock), 0, s, n, r32, A->dinv32, p32, st);
-      hipLaunchKernelGGL(k_mx_spmv, dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
-                         A->row_blocks, A->row_ptr, A->col, A->val32, p32, q32, st, part, c0 + 1);
-      hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, s, n, x32, r32, p32, q32,
-                         A->dinv32, st, part, c0 + 2);
+      ++k;
+      hipLaunchKernelGGL(k_mx_spmv, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr,
+                         A->col, A->val32, p32, q32, st, part);
+      hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, s, n, k, nb, x32, r32, p32,
+                         q32, A->dinv32, st, part);
+      hipLaunchKernelGGL((k_mx_dir<PREC, 0>), dim3(g), dim3(kBlock), 0, s, n, k, g, r32,
+                         A->dinv32, p32, st, part);
     }
     OFH_HIP(hipGetLastError());
-    if ((rc = read_dev(ctx, st, &h))) return rc;
-    if (!h.pause && h.it < max_it) continue;
-    // residual replacement in FP64
+    if ((rc = read_state(ctx, &h))) return rc;
+    if (!h.pause) continue;
+    // residual replacement in FP64 after h.it inner iterations
     hipLaunchKernelGGL(k_mx_accum, dim3(ge), dim3(256), 0, s, n, x, x32, st);
-    hipLaunchKernelGGL((k_mx_resid<PREC, false>), dim3(A->nblocks), dim3(kBlock), 0, s,
-                       A->nblocks, A->row_blocks, A->row_ptr, A->col, A->val, b, x, r32, x32,
-                       A->dinv32, st, part, c0 + 3);
+    hipLaunchKernelGGL(k_mx_resid<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
+                       A->row_ptr, A->col, A->val, b, x, r32, x32, A->dinv32, st, part);
+    hipLaunchKernelGGL((k_mx_dir<PREC, 2>), dim3(g), dim3(kBlock), 0, s, n, h.it, nb, r32,
+                       A->dinv32, p32, st, part);
     OFH_HIP(hipGetLastError());
-    if ((rc = read_dev(ctx, st, &h))) return rc;
+    if ((rc = read_state(ctx, &h))) return rc;
+    it_prev = h.it;
+    res_prev = h.res;
     if (h.done) break;
     // stagnation: the true residual must keep falling
-    stagnant = (h.res_true > 0.5 * last_true) ? stagnant + 1 : 0;
-    last_true = std::min(last_true, h.res_true);
+    stagnant = (h.res > 0.5 * last_true) ? stagnant + 1 : 0;
+    last_true = std::min(last_true, h.res);
     if (stagnant >= 3) {
       fallback = 1;
       break;
@@ -288,9 +298,9 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   OFH_HIP(hipStreamSynchronize(s));
   const double t1 = now_seconds();
   if (info) {
-    info->nb_it = h.done ? h.status : -(max_it + 1);
+    info->nb_it = h.status;
     info->iterations = h.it;
-    info->relres = h.res_true;
+    info->relres = h.res;
     info->outer = h.replacements;
     info->fallback = 0;
     info->seconds = t1 - t0;

@@ -11,13 +11,16 @@
 
 namespace ofh {
 
-// Scalars of one Krylov solve, resident on the device.  Kernels read them at
-// launch and the last block of each reduction writes them.
+// Scalars of one Krylov solve, resident on the device.  Block 0 of the kernel
+// that folds a reduction writes them; fields read by the blocks of a kernel
+// are never written by that same kernel (rho is double-buffered by the
+// iteration parity, which the host passes as a launch argument).
 struct KrylovState {
-  double rho, rho_1, alpha, beta, omega, nrm, res, tol;
-  double res0, sconv_res, h_norm, aux;
+  double rho_buf[2], alpha, beta, omega, nrm, res, tol;
+  double res0, sconv_res, h_norm, rmax;
   int it, done, status, max_it;
-  int sconv, cycle_i, halt, pad1;  // halt = done || sconv
+  int sconv, cycle_i, halt, pause;  // halt = done || sconv || pause
+  int replacements, pad[3];
 };
 static_assert(sizeof(KrylovState) <= 512, "state block");
 
@@ -36,18 +39,19 @@ inline int read_dev(ofh_ctx ctx, const S* d, S* h) {
 
 inline int read_state(ofh_ctx ctx, KrylovState* h) { return read_dev(ctx, state_ptr(ctx), h); }
 
-// Iterations to enqueue before the next host check: the observed geometric
-// residual reduction predicts the remaining count; clamp to [4, 1024].
-inline int next_chunk(int it, double res, double res0, double tol, int remaining) {
-  int c = 32;
-  if (it > 0 && res0 > 0 && res > 0 && res < res0 && tol > 0 && res > tol) {
-    double rate = std::pow(res / res0, 1.0 / it);
-    if (rate < 1.0) {
-      double pred = std::log(tol / res) / std::log(rate);
-      c = (int)std::ceil(pred * 1.05) + 2;
-    }
+// Iterations to enqueue before the next host check.  The residual reduction
+// observed since the previous check predicts the remaining count; a chunk is
+// at least 4 and at most as long as the iterations already done (>= 16), so a
+// poor prediction costs at most a doubling of the no-op launches.
+inline int next_chunk(int it, double res, int it_prev, double res_prev, double target,
+                      int remaining) {
+  int c = 8;
+  if (it > it_prev && res_prev > 0 && res > 0 && res < res_prev && target > 0 && res > target) {
+    const double rate = std::pow(res / res_prev, 1.0 / (it - it_prev));
+    const double pred = std::log(target / res) / std::log(rate);
+    c = (int)std::ceil(pred * 1.05) + 2;
   }
-  c = std::max(4, std::min(c, 1024));
+  c = std::max(4, std::min(c, std::max(16, it)));
   return std::max(1, std::min(c, remaining));
 }
 

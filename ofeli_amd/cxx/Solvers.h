@@ -1,0 +1,211 @@
+// Solvers.h -- Prec, CG, BiCGStab, GMRes and LinearSolver of the drop-in
+// layer (counterparts: include/solvers/Prec.h, CG.h:99-213,
+// BiCGStab.h:87-216, GMRes.h:148-298, LinearSolver.h:68-273 and
+// src/solvers/LinearSolver.cpp:63-244).  The iterations run on the GPU
+// (ofh_cg, ofh_bicgstab, ofh_gmres); the functions keep the reference's
+// signatures and return conventions: iteration count, 0 when the initial
+// guess already satisfies the test, -(max_it+1) without convergence,
+// 2 / 3 for BiCGStab breakdowns.
+#pragma once
+
+#include <cmath>
+
+#include "SpMatrix.h"
+#include "Vect.h"
+
+namespace OFELI {
+
+// Prec<T_> (Prec.h:73-176): IDENT, DIAG and SSOR on the GPU.  DILU/ILU are
+// not part of this path and are rejected.
+template <class T_>
+class Prec {
+ public:
+  Prec() = default;
+  explicit Prec(int type) : _type(type) {}
+  Prec(const SpMatrix<T_>& A, int type) : _type(type) { setMatrix(A); }
+  void setType(int type) { _type = type; }
+  void setMatrix(const SpMatrix<T_>& A) {
+    if (_type == -1)
+      throw OFELIException("In Prec::setMatrix(SpMatrix<T_>): Choose preconditioner before "
+                           "setting matrix !");
+    if (_type == DILU_PREC || _type == ILU_PREC)
+      throw OFELIException("In Prec::setMatrix: DILU_PREC/ILU_PREC are not provided by the GPU path");
+    _a = &A;
+  }
+  int type() const { return _type; }
+  const SpMatrix<T_>* matrix() const { return _a; }
+  // Prec::solve(b,x) (Prec_impl.h:226-255)
+  void solve(const Vect<T_>& b, Vect<T_>& x) const {
+    const size_t n = _a->size();
+    DeviceBuffer db, dx;
+    db.upload(b.data(), n);
+    dx.resize(n);
+    hip_check(ofh_prec_apply(_a->device(), _type, db.get(), dx.get()), "Prec::solve");
+    x.setSize(n);
+    dx.download(x.data(), n);
+  }
+  void solve(Vect<T_>& x) const {
+    Vect<T_> b(x);
+    solve(b, x);
+  }
+
+ private:
+  int _type = -1;
+  const SpMatrix<T_>* _a = nullptr;
+};
+
+namespace detail {
+enum Kind { KCG, KBICGSTAB, KGMRES };
+
+template <class T_>
+int run_solver(Kind kind, const SpMatrix<T_>& A, int prec, const Vect<T_>& b, Vect<T_>& x,
+               size_t m, int max_it, real_t toler, ofh_solve_info* out = nullptr) {
+  if (Verbosity > 2) {
+    const char* nm = kind == KCG ? "CG" : kind == KBICGSTAB ? "BiCGStab" : "GMRes";
+    std::printf("Running preconditioned %s method ...\n", nm);
+  }
+  const size_t n = A.size();
+  if (x.size() != n) x.setSize(n);
+  DeviceBuffer db, dx;
+  db.upload(b.data(), n);
+  dx.upload(x.data(), n);
+  ofh_solve_info info{};
+  ofh_csr dA = A.device();
+  int rc;
+  if (kind == KCG) rc = ofh_cg(dA, prec, db.get(), dx.get(), max_it, toler, &info);
+  else if (kind == KBICGSTAB) rc = ofh_bicgstab(dA, prec, db.get(), dx.get(), max_it, toler, &info);
+  else rc = ofh_gmres(dA, prec, db.get(), dx.get(), (int)m, max_it, toler, &info);
+  hip_check(rc, kind == KCG ? "CG" : kind == KBICGSTAB ? "BiCGStab" : "GMRes");
+  dx.download(x.data(), n);
+  if (out) *out = info;
+  if (Verbosity > 2) {
+    if (info.nb_it >= 0) std::printf("Convergence after %d iterations.\n", info.iterations);
+    else std::printf("No Convergence after %d iterations.\n", info.iterations);
+  }
+  return info.nb_it;
+}
+}  // namespace detail
+
+// CG(A, P, b, x, max_it, toler) and CG(A, prec, b, x, max_it, toler)
+template <class T_>
+int CG(const SpMatrix<T_>& A, const Prec<T_>& P, const Vect<T_>& b, Vect<T_>& x, int max_it,
+       real_t toler) {
+  return detail::run_solver(detail::KCG, A, P.type(), b, x, 0, max_it, toler);
+}
+template <class T_>
+int CG(const SpMatrix<T_>& A, int prec, const Vect<T_>& b, Vect<T_>& x, int max_it,
+       real_t toler) {
+  Prec<T_> p(A, prec);
+  return CG(A, p, b, x, max_it, toler);
+}
+template <class T_>
+int BiCGStab(const SpMatrix<T_>& A, const Prec<T_>& P, const Vect<T_>& b, Vect<T_>& x,
+             int max_it, real_t toler) {
+  return detail::run_solver(detail::KBICGSTAB, A, P.type(), b, x, 0, max_it, toler);
+}
+template <class T_>
+int BiCGStab(const SpMatrix<T_>& A, int prec, const Vect<T_>& b, Vect<T_>& x, int max_it,
+             real_t toler) {
+  return BiCGStab(A, Prec<T_>(A, prec), b, x, max_it, toler);
+}
+template <class T_>
+int GMRes(const SpMatrix<T_>& A, const Prec<T_>& P, const Vect<T_>& b, Vect<T_>& x, size_t m,
+          int max_it, real_t toler) {
+  return detail::run_solver(detail::KGMRES, A, P.type(), b, x, m, max_it, toler);
+}
+template <class T_>
+int GMRes(const SpMatrix<T_>& A, int prec, const Vect<T_>& b, Vect<T_>& x, size_t m, int max_it,
+          real_t toler) {
+  return GMRes(A, Prec<T_>(A, prec), b, x, m, max_it, toler);
+}
+
+// LinearSolver (LinearSolver.h:68-273): non-owning A, b, x; defaults
+// max_it = 1000, toler = sqrt(DBL_EPSILON) (LinearSolver.cpp:63-81).
+class LinearSolver {
+ public:
+  LinearSolver() = default;
+  LinearSolver(int max_it, real_t tolerance) : _max_it(max_it), _toler(tolerance) {}
+  LinearSolver(SpMatrix<real_t>& A, const Vect<real_t>& b, Vect<real_t>& x)
+      : _s(CG_SOLVER), _x(&x), _b(&b), _A(&A) {}
+
+  void setMaxIter(int m) { _max_it = m; }
+  void setTolerance(real_t tol) { _toler = tol; }
+  void setSolution(Vect<real_t>& x) { _x = &x; }
+  void setRHS(Vect<real_t>& b) { _b = &b; }
+  void setMatrix(SpMatrix<real_t>& A) { _A = &A; }
+  void set(SpMatrix<real_t>& A, const Vect<real_t>& b, Vect<real_t>& x) {
+    _A = &A;
+    _b = &b;
+    _x = &x;
+  }
+  void setSolver(Iteration s, Preconditioner p = DIAG_PREC) {
+    _s = s;
+    _p = p;
+  }
+  Iteration getSolver() const { return _s; }
+  Preconditioner getPreconditioner() const { return _p; }
+  int getNbIter() const { return _nb_it; }
+  const ofh_solve_info& getInfo() const { return _info; }
+
+  int solve(SpMatrix<real_t>& A, const Vect<real_t>& b, Vect<real_t>& x, Iteration s,
+            Preconditioner p = DIAG_PREC) {
+    set(A, b, x);
+    setSolver(s, p);
+    return solve();
+  }
+  int solve(Iteration s, Preconditioner p = DIAG_PREC) {
+    setSolver(s, p);
+    return solve();
+  }
+  // LinearSolver::solve (LinearSolver.cpp:194-244): returns 0, or the number
+  // of iterations performed when the method did not converge
+  int solve() {
+    _nb_it = 0;
+    if (!_A) throw OFELIException("In LinearSolver::solve(): No matrix has been defined.");
+    if (_A->isDiagonal()) {  // LinearSolver.cpp:200-210
+      for (size_t i = 1; i <= _A->getNbRows(); ++i) (*_x)(i) = (*_b)(i) / (*_A)(i, i);
+      return 0;
+    }
+    switch (_s) {
+      case CG_SOLVER:
+        _nb_it = detail::run_solver(detail::KCG, *_A, _p, *_b, *_x, 0, _max_it, _toler, &_info);
+        break;
+      case BICG_STAB_SOLVER:
+        _nb_it = detail::run_solver(detail::KBICGSTAB, *_A, _p, *_b, *_x, 0, _max_it, _toler,
+                                    &_info);
+        break;
+      case GMRES_SOLVER: {
+        // the reference passes m = b.size()/5 (LinearSolver.cpp:237-239),
+        // which is infeasible for large systems: capped at 50
+        size_t m = std::max<size_t>(1, std::min<size_t>(_b->size() / 5, 50));
+        _nb_it = detail::run_solver(detail::KGMRES, *_A, _p, *_b, *_x, m, _max_it, _toler,
+                                    &_info);
+        break;
+      }
+      default:
+        throw OFELIException("In LinearSolver::solve(): solver not provided by the GPU path "
+                             "(CG, BiCGStab and GMRes are)");
+    }
+    return _nb_it < 0 ? -_nb_it : 0;
+  }
+
+ private:
+  int _max_it = 1000, _nb_it = 0;
+  Iteration _s = DIRECT_SOLVER;
+  Preconditioner _p = DIAG_PREC;
+  real_t _toler = 1.4901161193847656e-08;
+  Vect<real_t>* _x = nullptr;
+  const Vect<real_t>* _b = nullptr;
+  SpMatrix<real_t>* _A = nullptr;
+  ofh_solve_info _info{};
+};
+
+template <class T_>
+int SpMatrix<T_>::solve(const Vect<T_>& b, Vect<T_>& x, bool) {
+  if (_solver == DIRECT_SOLVER) throw OFELIException("In SpMatrix::solve(...): No solver provided.");
+  LinearSolver ls(_max_it, _toler);
+  ls.set(*this, b, x);
+  return ls.solve(_solver, _prec);
+}
+
+}  // namespace OFELI

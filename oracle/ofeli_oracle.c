@@ -396,7 +396,8 @@ static void elas_moduli(double E, double nu, int mode, double* lambda, double* G
  *   kind ORC_ELAS2DQ4:    Elas2DQ4 Deviator + Dilatation (Elas2DQ4.cpp:186-221);
  *                         prm = {nu, mode}: mode 0 = reference parity
  *                         (plane strain, eMat = 2*Dev + Dil: both routines add
- *                         the cumulative eA0), mode 1 = plane stress, Dev + Dil.
+ *                         the cumulative eA0), mode 1 = plane stress, Dev + Dil,
+ *                         mode 2 = plane strain, Dev + Dil.
  * x holds the element's node coordinates (dim per node). */
 int orc_element(int kind, const double* x, double coef, const double* prm,
                 double* emat, double* ea1) {
@@ -460,7 +461,7 @@ int orc_element(int kind, const double* x, double coef, const double* prm,
       }
     }
     for (int k = 0; k < 64; ++k) em[k] += ea0[k];
-    if (mode == 1) memset(ea0, 0, sizeof ea0); /* textbook: separate terms */
+    if (mode != 0) memset(ea0, 0, sizeof ea0); /* textbook modes: separate terms */
     /* Dilatation */
     for (int k = 0; k < 4; ++k) {
       double c = lambda * w[k] * 1.0;

@@ -1,0 +1,89 @@
+"""The C++ drop-in layer (ofeli_amd/cxx, OFELI namespace) on the GPU: the
+OFELI-style programs of tests/cpp/dropin_demo.cpp against the oracle."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+EXE = os.path.join(HERE, "cpp", "dropin_demo")
+
+
+@pytest.fixture(scope="module")
+def cases():
+    if not os.path.exists(EXE):
+        pytest.fail("tests/cpp/dropin_demo not built (run tests/cpp/build.sh)")
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return {c["case"]: c for c in json.loads(r.stdout)}
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - b) / np.linalg.norm(b)
+
+
+def test_laplace2dt3_program(cases):
+    c = cases["laplace2dt3"]
+    m = O.mesh2d(16, 16)
+    ue = np.exp(m.coord[:, 0]) * np.sin(m.coord[:, 1]) + np.exp(m.coord[:, 1]) * np.cos(m.coord[:, 0])
+    dof, neq = O.number_equations(m.code)
+    rp, ci = O.node_graph(m.elem, 1, dof, neq)
+    a, b = O.assemble(O.LAPLACE2DT3, m, dof, neq, rp, ci, bc=ue)
+    ret, v, _ = O.cg(rp, ci, a, b, max_it=2000, toler=1e-12)
+    u = O.insert_bc(m.code, v, ue)
+    assert c["ret"] == 0 and c["it"] == ret
+    assert rel(c["u"], u) < 1e-11
+    assert c["maxerr"] < 1e-3
+
+
+def test_linear_solver_program(cases):
+    c = cases["linearsolver"]
+    n = 50
+    A = 2 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1)
+    x = np.linalg.solve(A, np.full(n, 1.0 / n))
+    assert c["ret"] == 0 and c["it_cg"] > 0 and c["res"] < 1e-11
+    assert rel(c["x"], x) < 1e-10
+    # GMRES(25) reaches the tolerance, as the oracle does (GMRES(10) stagnates
+    # on this problem in both)
+    assert c["it_bicgstab"] > 0 and c["it_gmres"] > 0 and c["diff23"] < 1e-9
+    a = cases["access"]
+    assert a["a33"] == 5.0 and a["a34"] == -1.0 and a["a310"] == 0.0
+
+
+def test_dc2dt3_backward_euler_program(cases):
+    c = cases["dc2dt3_be"]
+    m = O.mesh2d(12, 12)
+    f = 1.0 + m.coord[:, 0]
+    k = 1.0 + 0.5 * np.sin(0.1 * np.arange(1, m.nb_elements + 1))
+    dof, neq = O.number_equations(m.code)
+    rp, ci = O.node_graph(m.elem, 1, dof, neq)
+    u = np.zeros(m.nb_nodes)
+    for _ in range(5):
+        a, b = O.assemble(O.DC2DT3_BE, m, dof, neq, rp, ci, f=f, coef=k, prm=[0.01, 1.0, 1.0],
+                          u_prev=u)
+        x0 = u[m.code[:, 0] == 0]
+        ret, v, _ = O.cg(rp, ci, a, b, x0=x0, max_it=1000, toler=1e-12)
+        u = O.insert_bc(m.code, v)
+    assert rel(c["u"], u) < 1e-10
+
+
+def test_elas2dq4_program(cases):
+    c = cases["elas2dq4"]
+    m = O.mesh2d(8, 2, quad=True, nb_dof=2, codes=(1, -1, 0, 0), box=(0.0, 4.0, 0.0, 1.0))
+    m.code[:, 1] = m.code[:, 0]
+    E = 1.0 + 0.1 * (np.arange(1, m.nb_elements + 1) % 3)
+    dof, neq = O.number_equations(m.code)
+    rp, ci = O.node_graph(m.elem, 2, dof, neq)
+    a, b = O.assemble(O.ELAS2DQ4, m, dof, neq, rp, ci, coef=E, prm=[0.3, 0],
+                      bc=np.zeros(2 * m.nb_nodes))
+    tr = np.tile([0.0, -0.01], (m.sides.shape[0], 1))
+    b = O.traction_rhs(m, m.sides, tr, dof, b)
+    ret, v, _ = O.cg(rp, ci, a, b, prec=O.SSOR, max_it=1000, toler=1e-12)
+    u = O.insert_bc(m.code, v)
+    assert c["ret"] == 0 and abs(c["it"] - ret) <= 1
+    assert rel(c["u"], u) < 1e-10
