@@ -85,7 +85,28 @@ def main():
     spmv_ms = ev0.elapsed_time(ev1) / args.spmv_reps
     b_spmv64 = 12 * nnz + 4 * (n + 1) + 16 * n
     spmv_gbs = b_spmv64 / (spmv_ms * 1e-3) / 1e9
-    log("[bench] spmv64 %.4f ms  %.0f GB/s" % (spmv_ms, spmv_gbs))
+    log("[bench] spmv64 %.4f ms  %.0f GB/s (32-bit columns)" % (spmv_ms, spmv_gbs))
+    A.compress_columns(True)
+    ev0.record()
+    for _ in range(args.spmv_reps):
+        oa.spmv(A, x, y)
+    ev1.record()
+    torch.cuda.synchronize()
+    spmv_c16_ms = ev0.elapsed_time(ev1) / args.spmv_reps
+    A.compress_columns(False)
+    log("[bench] spmv64 %.4f ms (16-bit column offsets)" % spmv_c16_ms)
+    # streaming reference: a device-to-device copy of 1 GiB
+    big = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+    big2 = torch.empty_like(big)
+    big2.copy_(big)
+    ev0.record()
+    for _ in range(5):
+        big2.copy_(big)
+    ev1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 5 * 2 * big.numel() * 8 / (ev0.elapsed_time(ev1) * 1e-3) / 1e9
+    del big, big2
+    log("[bench] device copy %.0f GB/s (read+write)" % copy_gbs)
     xf, yf = x.float(), y.float()
     oa.spmv_f32(A, xf, yf)
     ev0.record()
@@ -158,9 +179,13 @@ def main():
         "mixed_true_relres": i32.true_relres, "assembly_ms": round(asm_ms, 3),
         "plan_setup_s": round(setup_s, 3), "cg64_us_per_iter": round(cg_iter_us, 2),
         "cg64_iter_GBs": round(b_cg64_iter / (cg_iter_us * 1e-6) / 1e9, 1),
-        "spmv32_ms": round(spmv32_ms, 4),
+        "spmv32_ms": round(spmv32_ms, 4), "spmv64_col16_ms": round(spmv_c16_ms, 4),
+        "device_copy_GBs": round(copy_gbs, 1),
         "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
+                     # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE runs recorded in
+                     # profiles/spmv_cg_poisson3d_96cube.md (96^3 system only)
+                     "traffic": 171.4e6 if args.n == 96 else None,
                      "kernel": "k_spmv<double> (CSR-stream FP64)",
                      "bytes_per_launch": b_spmv64, "avg_ms": round(spmv_ms, 4)},
         "cpu_baseline": cpu,

@@ -102,6 +102,13 @@ int ofh_csr_set_values_device(ofh_csr A, const double* d_val);
 int ofh_csr_get_values(ofh_csr A, double* h_val);
 int ofh_csr_values_device(ofh_csr A, double** d_val);
 
+/* Column-index storage of the SpMV stream (no reference counterpart: an
+ * internal format choice).  enable = 1 streams 16-bit offsets from the
+ * first row of each 256-row block instead of 32-bit columns when every
+ * column fits (otherwise the call keeps 32-bit columns); 0 (the default)
+ * streams 32-bit columns.  Results are identical either way. */
+int ofh_csr_compress_columns(ofh_csr A, int enable);
+
 /* y = A x: SpMatrix::Mult (SpMatrix_impl.h:728-734). */
 int ofh_spmv_f64(ofh_csr A, const double* d_x, double* d_y);
 /* y += a A x: SpMatrix::MultAdd (SpMatrix_impl.h:737-757). */

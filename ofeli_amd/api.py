@@ -109,6 +109,10 @@ class CsrMatrix:
         check(lib().ofh_csr_info(self.h, C.byref(n_), C.byref(nnz), C.byref(mr)))
         self.n, self.nnz, self.max_row_nnz = n_.value, nnz.value, mr.value
 
+    def compress_columns(self, enable=True):
+        """16-bit block-relative column offsets in the SpMV stream (default on)."""
+        check(lib().ofh_csr_compress_columns(self.h, int(bool(enable))))
+
     def pattern(self):
         rp = np.zeros(self.n + 1, np.int32)
         cl = np.zeros(max(self.nnz, 1), np.int32)

@@ -458,6 +458,7 @@ int ofh_asm_create(ofh_ctx ctx, int dim, int nb_nodes, const double* h_coord, in
   ofh_number_equations(nb_nodes, nb_dof, h_code, P->h_dof.data(), &neq);
   P->nb_eq = neq;
   OFH_CHECK(neq > 0, OFH_E_INVALID, "ofh_asm_create: every DOF is imposed");
+  OFH_CHECK(neq <= kMaxRows, OFH_E_INVALID, "ofh_asm_create: too many equations");
   std::vector<int> rnode(neq), rcomp(neq);
   for (int64_t t = 0; t < (int64_t)nb_nodes * nb_dof; ++t)
     if (P->h_dof[t]) {

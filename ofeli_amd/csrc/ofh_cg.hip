@@ -42,7 +42,7 @@ __device__ __forceinline__ double zval(int i, double ri, const double* __restric
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_cg_init(int nblocks, const int* __restrict__ rb,
                                                    const int* __restrict__ rp,
-                                                   const int* __restrict__ col,
+                                                   ColIndex ci,
                                                    const double* __restrict__ val,
                                                    const double* __restrict__ b,
                                                    const double* __restrict__ x,
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int nblocks, const int* __re
   __shared__ double red[3 * kBlock / kWave];
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
-  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
+  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, ci, val, x, lds, 0.0);
   double v[3] = {0.0, 0.0, 0.0};
   const int i = r0 + threadIdx.x;  // a long-row block holds one row, summed in thread 0
   if (i < r1) {
@@ -81,6 +81,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int n, int it, int nparts0, i
                                                   const double* __restrict__ partials) {
   if (st->done) return;
   __shared__ double red[3 * kBlock / kWave];
+  // issue this thread's loads first: they are in flight while the partials fold
+  const int base = blockIdx.x * (kBlock * kVec) + threadIdx.x;
+  double zv[kVec], pv[kVec];
+#pragma unroll
+  for (int j = 0; j < kVec; ++j) {
+    const int i = base + j * kBlock;
+    if (i < n) {
+      zv[j] = PREC == OFH_SSOR_PREC ? z[i] : r[i];
+      if (PREC == OFH_DIAG_PREC) zv[j] *= dinv[i];  // z = D^-1 r (Prec_impl.h:245-247)
+      pv[j] = p[i];
+    }
+  }
   double t[3];
   double nrm;
   if (PREC == OFH_SSOR_PREC) {
@@ -135,16 +147,17 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int n, int it, int nparts0, i
     }
   }
   if (stop) return;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const double zi = zval<PREC>(i, r[i], dinv, z);
-    p[i] = it == 0 ? zi : zi + beta * p[i];
+#pragma unroll
+  for (int j = 0; j < kVec; ++j) {
+    const int i = base + j * kBlock;
+    if (i < n) p[i] = it == 0 ? zv[j] : zv[j] + beta * pv[j];
   }
 }
 
 // q = A p; partial (p,q) -> slot 3 (CG.h:134-135)
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(int nblocks, const int* __restrict__ rb,
                                                    const int* __restrict__ rp,
-                                                   const int* __restrict__ col,
+                                                   ColIndex ci,
                                                    const double* __restrict__ val,
                                                    const double* __restrict__ p,
                                                    double* __restrict__ q,
@@ -155,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(int nblocks, const int* __re
   __shared__ double red[kBlock / kWave];
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
-  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, p, lds, 0.0);
+  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, ci, val, p, lds, 0.0);
   const int i = r0 + threadIdx.x;
   double v[1] = {0.0};
   if (i < r1) {
@@ -178,17 +191,36 @@ __global__ __launch_bounds__(kBlock) void k_cg_upd(int n, int it, int nparts_pq,
                                                   KrylovState* st, double* __restrict__ partials) {
   if (st->done) return;
   __shared__ double red[2 * kBlock / kWave];
+  // this thread's loads go out before the fold of the (p,q) partials
+  const int base = blockIdx.x * (kBlock * kVec) + threadIdx.x;
+  double xv[kVec], pv[kVec], rv[kVec], qv[kVec], dv[kVec];
+#pragma unroll
+  for (int j = 0; j < kVec; ++j) {
+    const int i = base + j * kBlock;
+    if (i < n) {
+      xv[j] = x[i];
+      pv[j] = p[i];
+      rv[j] = r[i];
+      qv[j] = q[i];
+      if (PREC == OFH_DIAG_PREC) dv[j] = dinv[i];
+    }
+  }
   double pq[1];
   const int sl[1] = {3};
   fold_partials<kBlock, 1>(pq, sl, partials, nparts_pq, red);
   const double alpha = st->rho_buf[(it - 1) & 1] / pq[0];
   double v[2] = {0.0, 0.0};
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    v[0] += ri * ri;
-    if (PREC != OFH_SSOR_PREC) v[1] += ri * zval<PREC>(i, ri, dinv, nullptr);
+#pragma unroll
+  for (int j = 0; j < kVec; ++j) {
+    const int i = base + j * kBlock;
+    if (i < n) {
+      x[i] = xv[j] + alpha * pv[j];
+      const double ri = rv[j] - alpha * qv[j];
+      r[i] = ri;
+      v[0] += ri * ri;
+      if (PREC == OFH_DIAG_PREC) v[1] += ri * (ri * dv[j]);
+      else if (PREC == OFH_IDENT_PREC) v[1] += ri * ri;
+    }
   }
   block_sum_k<kBlock, 2>(v, red);
   const int slot[2] = {0, 1};
@@ -273,7 +305,7 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   const int g = reduce_grid(n);
   const int nb = A->nblocks;
   hipLaunchKernelGGL(k_cg_init<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                     A->row_ptr, A->col, A->val, b, x, r, A->dinv, part);
+                     A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r, A->dinv, part);
   auto ssor_z = [&](int it, bool init) -> int {
     hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(kBlock), 0, s, it, init ? nb : g, (int)init, st,
                        part);
@@ -297,7 +329,7 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
     for (int c = 0; c < chunk; ++c) {
       const int it = ++enq;
       hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr,
-                         A->col, A->val, p, q, st, part);
+                         ColIndex{A->col, A->col16}, A->val, p, q, st, part);
       hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, nb, x, r, p, q,
                          A->dinv, st, part);
       if (PREC == OFH_SSOR_PREC && (rc = ssor_z(it, false))) return rc;

@@ -112,9 +112,12 @@ __global__ void k_axpy(int n, double a, const double* __restrict__ x, double* __
   if (i < n) y[i] += a * x[i];
 }
 
-int reduce_grid(int n) {  // fixed function of n: deterministic partial layout
-  int g = (n + kBlock * 4 - 1) / (kBlock * 4);
-  return std::max(1, std::min(g, 1024));
+// Grid of the element-wise solver kernels: exactly kVec entries per thread
+// (the load-first kernels rely on it), a fixed function of n so the partial
+// sums always have the same layout.
+int reduce_grid(int n) {
+  int g = (n + kBlock * kVec - 1) / (kBlock * kVec);
+  return std::max(1, g);
 }
 
 int launch_dot(ofh_ctx ctx, int n, const double* x, const double* y, double* d_out,
@@ -144,6 +147,39 @@ int ensure_vws(ofh_ctx ctx, size_t bytes, void** out) {
 
 double now_seconds() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+__global__ void k_make_rel16(const int* __restrict__ rb, const int* __restrict__ rp,
+                             const int* __restrict__ col, int16_t* __restrict__ rel, int* bad) {
+  const int r0 = rb[blockIdx.x], r1 = rb[blockIdx.x + 1];
+  for (int k = rp[r0] + threadIdx.x; k < rp[r1]; k += blockDim.x) {
+    const int d = col[k] - r0;
+    if (d < -32768 || d > 32767) {
+      *bad = 1;
+      rel[k] = 0;
+    } else {
+      rel[k] = (int16_t)d;
+    }
+  }
+}
+
+int csr_build_rel16(ofh_csr A) {
+  ofh_ctx ctx = A->ctx;
+  if (A->col16) return OFH_OK;
+  OFH_HIP(hipMalloc(&A->col16, sizeof(int16_t) * std::max<int64_t>(A->nnz, 1)));
+  int* bad = (int*)ctx->dev_scratch + 4;
+  OFH_HIP(hipMemsetAsync(bad, 0, sizeof(int), ctx->stream));
+  hipLaunchKernelGGL(k_make_rel16, dim3(A->nblocks), dim3(256), 0, ctx->stream, A->row_blocks,
+                     A->row_ptr, A->col, A->col16, bad);
+  OFH_HIP(hipGetLastError());
+  int h = 0;
+  OFH_HIP(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  OFH_HIP(hipStreamSynchronize(ctx->stream));
+  if (h) {  // some column is too far from its block: keep the int32 form only
+    OFH_HIP(hipFree(A->col16));
+    A->col16 = nullptr;
+  }
+  return OFH_OK;
 }
 
 int csr_ensure_val32(ofh_csr A) {
@@ -274,7 +310,7 @@ int ofh_memcpy_d2d(ofh_ctx c, void* d, const void* s, size_t bytes) {
 int ofh_csr_create(ofh_ctx c, int n, int64_t nnz, const int32_t* rp, const int32_t* col,
                    const double* val, ofh_csr* out) {
   OFH_CHECK(c && rp && col && out, OFH_E_INVALID, "ofh_csr_create: null argument");
-  OFH_CHECK(n > 0 && nnz >= 0 && nnz < (int64_t)0x7fffffff, OFH_E_INVALID,
+  OFH_CHECK(n > 0 && n <= kMaxRows && nnz >= 0 && nnz < (int64_t)0x7fffffff, OFH_E_INVALID,
             "ofh_csr_create: bad size");
   OFH_CHECK(rp[0] == 0 && rp[n] == nnz, OFH_E_INVALID,
             "ofh_csr_create: row_ptr must start at 0 and end at nnz");
@@ -314,11 +350,21 @@ int ofh_csr_create(ofh_ctx c, int n, int64_t nnz, const int32_t* rp, const int32
   return OFH_OK;
 }
 
+int ofh_csr_compress_columns(ofh_csr A, int enable) {
+  OFH_CHECK(A, OFH_E_INVALID, "null csr");
+  if (enable) return csr_build_rel16(A);
+  OFH_HIP(hipStreamSynchronize(A->ctx->stream));
+  if (A->col16) OFH_HIP(hipFree(A->col16));
+  A->col16 = nullptr;
+  return OFH_OK;
+}
+
 int ofh_csr_destroy(ofh_csr A) {
   if (!A) return OFH_OK;
   hipDeviceSynchronize();  // the context may already be gone: do not touch it
   hipFree(A->row_ptr);
   hipFree(A->col);
+  hipFree(A->col16);
   hipFree(A->val);
   hipFree(A->val32);
   hipFree(A->row_blocks);

@@ -45,6 +45,8 @@ constexpr int kMaxPartials = 1 << 16; // per-reduction partial slots (max workgr
 constexpr int kTile64 = 4096;         // LDS product tile of the CSR-stream SpMV (FP64)
 constexpr int kTile32 = 4096;         // same, FP32
 constexpr int kNumXcd = 8;            // MI355X: 8 XCDs, workgroups dealt round-robin
+constexpr int kVec = 4;               // vector entries per thread in element-wise solver kernels
+constexpr int kMaxRows = kMaxPartials * kBlock * kVec;  // largest n the reductions cover
 
 // Workgroups are dispatched round-robin over the XCDs; remap the hardware
 // block id so that each XCD walks one contiguous range of row blocks, which
@@ -158,6 +160,7 @@ struct ofh_csr_s {
   int64_t nnz = 0;
   int* row_ptr = nullptr;   // device, n+1
   int* col = nullptr;       // device, nnz (0-based)
+  int16_t* col16 = nullptr; // device, nnz: col - first row of its SpMV block (null if out of range)
   double* val = nullptr;    // device, nnz
   float* val32 = nullptr;   // device, nnz (lazy FP32 mirror)
   int* row_blocks = nullptr;  // device, nblocks+1 (CSR-stream row partition)
@@ -183,6 +186,8 @@ int ensure_vws(ofh_ctx ctx, size_t bytes, void** out);
 int reduce_grid(int n);
 int launch_dot(ofh_ctx ctx, int n, const double* x, const double* y, double* d_out, bool sqrt_out);
 int csr_ensure_val32(ofh_csr A);
+// 16-bit block-relative column offsets when every column fits (see ColIndex).
+int csr_build_rel16(ofh_csr A);
 int csr_ensure_dinv(ofh_csr A);
 int csr_ensure_levels(ofh_csr A);
 

@@ -22,7 +22,7 @@ namespace ofh {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_true_res(int nblocks, const int* __restrict__ rb,
                                                     const int* __restrict__ rp,
-                                                    const int* __restrict__ col,
+                                                    ColIndex ci,
                                                     const double* __restrict__ val,
                                                     const double* __restrict__ b,
                                                     const double* __restrict__ x,
@@ -31,7 +31,7 @@ __global__ __launch_bounds__(kBlock) void k_true_res(int nblocks, const int* __r
   __shared__ double red[2 * kBlock / kWave];
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
-  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
+  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, ci, val, x, lds, 0.0);
   const int i = r0 + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (i < r1) {
@@ -61,7 +61,7 @@ int true_relres(ofh_csr A, const double* b, const double* x, double* out) {
   ofh_ctx ctx = A->ctx;
   double* d_out = ctx->dev_scratch + 16;
   hipLaunchKernelGGL(k_true_res, dim3(A->nblocks), dim3(kBlock), 0, ctx->stream, A->nblocks,
-                     A->row_blocks, A->row_ptr, A->col, A->val, b, x, ctx->ws.partials);
+                     A->row_blocks, A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, ctx->ws.partials);
   hipLaunchKernelGGL(k_true_fold, dim3(1), dim3(kBlock), 0, ctx->stream, A->nblocks,
                      ctx->ws.partials, d_out);
   OFH_HIP(hipGetLastError());
@@ -99,7 +99,7 @@ __global__ void k_prec_diag(int n, const double* __restrict__ b, const double* _
 // r = b - A x, rr = r; partials (b,b) -> 0, (r,r) -> 1  (BiCGStab.h:101-103)
 __global__ __launch_bounds__(kBlock) void k_bi_init(int nblocks, const int* __restrict__ rb,
                                                    const int* __restrict__ rp,
-                                                   const int* __restrict__ col,
+                                                   ColIndex ci,
                                                    const double* __restrict__ val,
                                                    const double* __restrict__ b,
                                                    const double* __restrict__ x,
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_init(int nblocks, const int* __re
   __shared__ double red[2 * kBlock / kWave];
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
-  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
+  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, ci, val, x, lds, 0.0);
   const int i = r0 + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (i < r1) {
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_dir(int n, int it, int nparts,
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_bi_spmv(int nblocks, const int* __restrict__ rb,
                                                    const int* __restrict__ rp,
-                                                   const int* __restrict__ col,
+                                                   ColIndex ci,
                                                    const double* __restrict__ val,
                                                    const double* __restrict__ x,
                                                    double* __restrict__ y,
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_spmv(int nblocks, const int* __re
   __shared__ double red[2 * kBlock / kWave];
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
-  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
+  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, ci, val, x, lds, 0.0);
   const int i = r0 + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (i < r1) {
@@ -335,7 +335,7 @@ static int bicgstab_impl(ofh_csr A, const double* b, double* x, int max_it, doub
   const int g = reduce_grid(n);
   const int nb = A->nblocks;
   hipLaunchKernelGGL(k_bi_init, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr,
-                     A->col, A->val, b, x, r, rr, part);
+                     ColIndex{A->col, A->col16}, A->val, b, x, r, rr, part);
   auto dir = [&](int it) -> int {
     hipLaunchKernelGGL(k_bi_dir<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, it == 1 ? nb : g, r, v,
                        p, pp, A->dinv, st, part);
@@ -355,13 +355,13 @@ static int bicgstab_impl(ofh_csr A, const double* b, double* x, int max_it, doub
     for (int c = 0; c < chunk; ++c) {
       const int it = ++enq;
       hipLaunchKernelGGL(k_bi_spmv<0>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                         A->row_ptr, A->col, A->val, pp, v, rr, st, part);
+                         A->row_ptr, ColIndex{A->col, A->col16}, A->val, pp, v, rr, st, part);
       hipLaunchKernelGGL(k_bi_s<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, nb, r, v, sv, ss,
                          A->dinv, st, part);
       hipLaunchKernelGGL(k_bi_scheck, dim3(1), dim3(kBlock), 0, s, g, st, part);
       if (PREC == OFH_SSOR_PREC && (rc = launch_ssor(A, sv, ss, &st->halt, s))) return rc;
       hipLaunchKernelGGL(k_bi_spmv<1>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                         A->row_ptr, A->col, A->val, ss, t, sv, st, part);
+                         A->row_ptr, ColIndex{A->col, A->col16}, A->val, ss, t, sv, st, part);
       hipLaunchKernelGGL(k_bi_upd, dim3(g), dim3(kBlock), 0, s, n, it, nb, x, r, rr, pp, sv, ss,
                          t, st, part);
       if ((rc = dir(it + 1))) return rc;
@@ -383,7 +383,7 @@ static int bicgstab_impl(ofh_csr A, const double* b, double* x, int max_it, doub
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_gm_resid(int nblocks, const int* __restrict__ rb,
                                                     const int* __restrict__ rp,
-                                                    const int* __restrict__ col,
+                                                    ColIndex ci,
                                                     const double* __restrict__ val,
                                                     const double* __restrict__ b,
                                                     const double* __restrict__ x,
@@ -396,7 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_gm_resid(int nblocks, const int* __r
   __shared__ double red[2 * kBlock / kWave];
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
-  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
+  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, ci, val, x, lds, 0.0);
   const int i = r0 + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (i < r1) {
@@ -414,7 +414,7 @@ __global__ __launch_bounds__(kBlock) void k_gm_resid(int nblocks, const int* __r
 // plain b - A x into w (the SSOR path preconditions it with the sweeps)
 __global__ __launch_bounds__(kBlock) void k_resid_plain(int nblocks, const int* __restrict__ rb,
                                                        const int* __restrict__ rp,
-                                                       const int* __restrict__ col,
+                                                       ColIndex ci,
                                                        const double* __restrict__ val,
                                                        const double* __restrict__ b,
                                                        const double* __restrict__ x,
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(kBlock) void k_resid_plain(int nblocks, const int* 
   __shared__ double lds[kTile64];
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
-  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
+  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, ci, val, x, lds, 0.0);
   const int i = r0 + threadIdx.x;
   if (i < r1) w[i] = b[i] - s;
 }
@@ -493,7 +493,7 @@ __global__ void k_gm_v0(int n, int m, const double* __restrict__ r, double* __re
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_gm_av(int nblocks, const int* __restrict__ rb,
                                                  const int* __restrict__ rp,
-                                                 const int* __restrict__ col,
+                                                 ColIndex ci,
                                                  const double* __restrict__ val,
                                                  const double* __restrict__ vi,
                                                  double* __restrict__ w,
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_gm_av(int nblocks, const int* __rest
   __shared__ double lds[kTile64];
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
-  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, vi, lds, 0.0);
+  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, ci, val, vi, lds, 0.0);
   const int i = r0 + threadIdx.x;
   if (i < r1) w[i] = PREC == OFH_DIAG_PREC ? s * dinv[i] : s;
 }
@@ -675,14 +675,14 @@ static int gmres_impl(ofh_csr A, const double* b, double* x, int m, int max_it, 
   auto residual = [&](bool init) -> int {
     if (PREC == OFH_SSOR_PREC) {
       hipLaunchKernelGGL(k_resid_plain, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                         A->row_ptr, A->col, A->val, b, x, tmp, st);
+                         A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, tmp, st);
       int e = launch_ssor(A, tmp, r, &st->done, s);
       if (e) return e;
       hipLaunchKernelGGL(k_gm_rnorm, dim3(g), dim3(kBlock), 0, s, n, r, b, st, part);
       hipLaunchKernelGGL(k_gm_rcheck, dim3(1), dim3(kBlock), 0, s, (int)init, g, st, part);
     } else {
       hipLaunchKernelGGL(k_gm_resid<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                         A->row_ptr, A->col, A->val, b, x, r, A->dinv, st, part);
+                         A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r, A->dinv, st, part);
       hipLaunchKernelGGL(k_gm_rcheck, dim3(1), dim3(kBlock), 0, s, (int)init, nb, st, part);
     }
     return OFH_OK;
@@ -698,11 +698,11 @@ static int gmres_impl(ofh_csr A, const double* b, double* x, int m, int max_it, 
       double* vi = V + (size_t)i * n;
       if (PREC == OFH_SSOR_PREC) {
         hipLaunchKernelGGL(k_gm_av<OFH_IDENT_PREC>, dim3(nb), dim3(kBlock), 0, s, nb,
-                           A->row_blocks, A->row_ptr, A->col, A->val, vi, tmp, A->dinv, st);
+                           A->row_blocks, A->row_ptr, ColIndex{A->col, A->col16}, A->val, vi, tmp, A->dinv, st);
         if ((rc = launch_ssor(A, tmp, w, &st->halt, s))) return rc;
       } else {
         hipLaunchKernelGGL(k_gm_av<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                           A->row_ptr, A->col, A->val, vi, w, A->dinv, st);
+                           A->row_ptr, ColIndex{A->col, A->col16}, A->val, vi, w, A->dinv, st);
       }
       for (int k = 0; k <= i + 1; ++k) {
         const double* vprev = k > 0 ? V + (size_t)(k - 1) * n : nullptr;

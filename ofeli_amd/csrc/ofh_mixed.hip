@@ -35,7 +35,7 @@ __device__ __forceinline__ float zf(int i, float ri, const float* __restrict__ d
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_mx_resid(
     int nblocks, const int* __restrict__ rb, const int* __restrict__ rp,
-    const int* __restrict__ col, const double* __restrict__ val, const double* __restrict__ b,
+    ColIndex ci, const double* __restrict__ val, const double* __restrict__ b,
     const double* __restrict__ x, float* __restrict__ r32, float* __restrict__ x32,
     const float* __restrict__ dinv32, const KrylovState* __restrict__ st,
     double* __restrict__ partials) {
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_resid(
   __shared__ double red[4 * kBlock / kWave];
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
-  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, col, val, x, lds, 0.0);
+  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, ci, val, x, lds, 0.0);
   const int i = r0 + threadIdx.x;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < r1) {
@@ -78,6 +78,16 @@ __global__ __launch_bounds__(kBlock) void k_mx_dir(int n, int k, int nparts,
                                                   const double* __restrict__ partials) {
   if (PHASE == 0 ? st->halt : st->done) return;
   __shared__ double red[4 * kBlock / kWave];
+  const int base = blockIdx.x * (kBlock * kVec) + threadIdx.x;
+  float zv[kVec], pv[kVec];
+#pragma unroll
+  for (int j = 0; j < kVec; ++j) {  // loads in flight during the fold
+    const int i = base + j * kBlock;
+    if (i < n) {
+      zv[j] = zf<PREC>(i, r32[i], dinv32);
+      pv[j] = p32[i];
+    }
+  }
   double nrm = st->nrm, res, rz;
   bool stop = false, pause = false;
   int status = 0;
@@ -139,16 +149,17 @@ __global__ __launch_bounds__(kBlock) void k_mx_dir(int n, int k, int nparts,
   }
   if (stop || pause) return;
   const float bf = (float)beta;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const float z = zf<PREC>(i, r32[i], dinv32);
-    p32[i] = k == 0 ? z : z + bf * p32[i];
+#pragma unroll
+  for (int j = 0; j < kVec; ++j) {
+    const int i = base + j * kBlock;
+    if (i < n) p32[i] = k == 0 ? zv[j] : zv[j] + bf * pv[j];
   }
 }
 
 // q32 = A32 p32; (p,q) -> slot 6 (FP64 accumulation)
 __global__ __launch_bounds__(kBlock) void k_mx_spmv(int nblocks, const int* __restrict__ rb,
                                                    const int* __restrict__ rp,
-                                                   const int* __restrict__ col,
+                                                   ColIndex ci,
                                                    const float* __restrict__ val32,
                                                    const float* __restrict__ p32,
                                                    float* __restrict__ q32,
@@ -159,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_spmv(int nblocks, const int* __re
   __shared__ double red[kBlock / kWave];
   const int blk = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[blk], r1 = rb[blk + 1];
-  const float s = csr_stream_rowsum<float, kTile32>(r0, r1, rp, col, val32, p32, lds, 0.0f);
+  const float s = csr_stream_rowsum<float, kTile32>(r0, r1, rp, ci, val32, p32, lds, 0.0f);
   const int i = r0 + threadIdx.x;
   double v[1] = {0.0};
   if (i < r1) {
@@ -181,19 +192,36 @@ __global__ __launch_bounds__(kBlock) void k_mx_upd(int n, int k, int nparts_pq,
                                                   KrylovState* st, double* __restrict__ partials) {
   if (st->halt) return;
   __shared__ double red[2 * kBlock / kWave];
+  const int base = blockIdx.x * (kBlock * kVec) + threadIdx.x;
+  float xv[kVec], pv[kVec], rv[kVec], qv[kVec], dv[kVec];
+#pragma unroll
+  for (int j = 0; j < kVec; ++j) {  // loads in flight during the fold
+    const int i = base + j * kBlock;
+    if (i < n) {
+      xv[j] = x32[i];
+      pv[j] = p32[i];
+      rv[j] = r32[i];
+      qv[j] = q32[i];
+      dv[j] = PREC == OFH_DIAG_PREC ? dinv32[i] : 1.0f;
+    }
+  }
   double pq[1];
   const int sl[1] = {6};
   fold_partials<kBlock, 1>(pq, sl, partials, nparts_pq, red);
   const double alpha = st->rho_buf[(k - 1) & 1] / pq[0];
   const float af = (float)alpha;
   double v[2] = {0.0, 0.0};
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    x32[i] += af * p32[i];
-    const float ri = r32[i] - af * q32[i];
-    r32[i] = ri;
-    const float z = zf<PREC>(i, ri, dinv32);
-    v[0] += (double)ri * (double)ri;
-    v[1] += (double)ri * (double)z;
+#pragma unroll
+  for (int j = 0; j < kVec; ++j) {
+    const int i = base + j * kBlock;
+    if (i < n) {
+      x32[i] = xv[j] + af * pv[j];
+      const float ri = rv[j] - af * qv[j];
+      r32[i] = ri;
+      const float z = PREC == OFH_DIAG_PREC ? ri * dv[j] : ri;
+      v[0] += (double)ri * (double)ri;
+      v[1] += (double)ri * (double)z;
+    }
   }
   block_sum_k<kBlock, 2>(v, red);
   const int slot[2] = {4, 5};
@@ -235,7 +263,7 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   const int nb = A->nblocks;
   const int ge = (n + 255) / 256;
   hipLaunchKernelGGL(k_mx_resid<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                     A->row_ptr, A->col, A->val, b, x, r32, x32, A->dinv32, st, part);
+                     A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r32, x32, A->dinv32, st, part);
   hipLaunchKernelGGL((k_mx_dir<PREC, 1>), dim3(g), dim3(kBlock), 0, s, n, 0, nb, r32, A->dinv32,
                      p32, st, part);
   OFH_HIP(hipGetLastError());
@@ -253,7 +281,7 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     for (int c = 0; c < chunk; ++c) {
       ++k;
       hipLaunchKernelGGL(k_mx_spmv, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr,
-                         A->col, A->val32, p32, q32, st, part);
+                         ColIndex{A->col, A->col16}, A->val32, p32, q32, st, part);
       hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, s, n, k, nb, x32, r32, p32,
                          q32, A->dinv32, st, part);
       hipLaunchKernelGGL((k_mx_dir<PREC, 0>), dim3(g), dim3(kBlock), 0, s, n, k, g, r32,
@@ -265,7 +293,7 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     // residual replacement in FP64 after h.it inner iterations
     hipLaunchKernelGGL(k_mx_accum, dim3(ge), dim3(256), 0, s, n, x, x32, st);
     hipLaunchKernelGGL(k_mx_resid<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                       A->row_ptr, A->col, A->val, b, x, r32, x32, A->dinv32, st, part);
+                       A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r32, x32, A->dinv32, st, part);
     hipLaunchKernelGGL((k_mx_dir<PREC, 2>), dim3(g), dim3(kBlock), 0, s, n, h.it, nb, r32,
                        A->dinv32, p32, st, part);
     OFH_HIP(hipGetLastError());

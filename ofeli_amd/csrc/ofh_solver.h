@@ -49,7 +49,7 @@ inline int next_chunk(int it, double res, int it_prev, double res_prev, double t
   if (it > it_prev && res_prev > 0 && res > 0 && res < res_prev && target > 0 && res > target) {
     const double rate = std::pow(res / res_prev, 1.0 / (it - it_prev));
     const double pred = std::log(target / res) / std::log(rate);
-    c = (int)std::ceil(pred * 1.05) + 2;
+    c = (int)std::ceil(pred);
   }
   c = std::max(4, std::min(c, std::max(16, it)));
   return std::max(1, std::min(c, remaining));

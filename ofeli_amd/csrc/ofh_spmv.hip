@@ -21,7 +21,7 @@ namespace ofh {
 template <typename T, int TILE, int MODE>
 __global__ __launch_bounds__(kBlock) void k_spmv(int nblocks, const int* __restrict__ rb,
                                                  const int* __restrict__ rp,
-                                                 const int* __restrict__ col,
+                                                 ColIndex ci,
                                                  const T* __restrict__ val,
                                                  const T* __restrict__ x, T* __restrict__ y,
                                                  T alpha) {
@@ -31,14 +31,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nblocks, const int* __restr
   const int r = r0 + threadIdx.x;
   T init = T(0);
   if (MODE != 0 && r < r1) init = y[r];
-  T s = csr_stream_rowsum<T, TILE, MODE == 2>(r0, r1, rp, col, val, x, lds, init, alpha);
+  T s = csr_stream_rowsum<T, TILE, MODE == 2>(r0, r1, rp, ci, val, x, lds, init, alpha);
   if (r < r1) y[r] = s;
 }
 
 template <typename T, int TILE, int MODE>
 static int launch(ofh_csr A, const T* val, const T* x, T* y, T alpha, hipStream_t s) {
   hipLaunchKernelGGL((k_spmv<T, TILE, MODE>), dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
-                     A->row_blocks, A->row_ptr, A->col, val, x, y, alpha);
+                     A->row_blocks, A->row_ptr, ColIndex{A->col, A->col16}, val, x, y, alpha);
   OFH_HIP(hipGetLastError());
   return OFH_OK;
 }

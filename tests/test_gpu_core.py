@@ -59,6 +59,23 @@ def test_spmv_bitexact(ctx):
     assert np.array_equal(host(y), O.spmv(rp, ci, a, x))
 
 
+def test_spmv_column_compression_is_exact(ctx):
+    """16-bit block-relative columns and 32-bit columns give the same bits."""
+    import ofeli_amd as oa
+    rp, ci, a, b = dc2d_system(60)
+    A = gpu_csr(ctx, rp, ci, a)
+    x = np.random.default_rng(9).standard_normal(A.n)
+    ref = O.spmv(rp, ci, a, x)
+    for enable in (False, True, False):
+        A.compress_columns(enable)
+        y = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+        oa.spmv(A, dev(x), y)
+        assert np.array_equal(host(y), ref)
+        xs = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+        info = oa.cg(A, dev(b), xs, max_it=3000, toler=1e-10)
+        assert info.nb_it > 0
+
+
 def test_spmv_add_bitexact(ctx):
     """MultAdd(x,y) and MultAdd(a,x,y) keep the reference's term order."""
     import ofeli_amd as oa
