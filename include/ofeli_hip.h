@@ -151,6 +151,33 @@ int ofh_linear_solve(ofh_csr A, int solver, int prec, const double* d_b, double*
 int ofh_cg_mixed(ofh_csr A, int prec, const double* d_b, double* d_x, int max_it,
                  double toler, ofh_solve_info* info);
 
+/* ------------------------------------------------------- batched systems */
+/* Many independent systems in one persistent launch (no reference routine:
+ * the reference solves such a set with one LinearSolver::solve per system,
+ * LinearSolver.cpp:194-244 -> CG.h:99-157).  A is block-diagonal: system s
+ * owns rows [h_sys_rows[s], h_sys_rows[s+1]) and its columns stay inside
+ * that range (checked).  Typically A is the plan matrix of several disjoint
+ * meshes assembled together. */
+typedef struct ofh_batch_s* ofh_batch;
+int ofh_batch_create(ofh_csr A, int nsys, const int32_t* h_sys_rows, ofh_batch* out);
+int ofh_batch_destroy(ofh_batch B);
+/* CG + DIAG_PREC/IDENT_PREC on every system (x holds the initial guesses).
+ * mixed = 0: FP64, the reference iteration per system; mixed = 1: FP32
+ * inner iteration with FP64 residual replacement, stopping on the true FP64
+ * residual.  Per-system results (reference convention in h_status) may be
+ * NULL; total->iterations is the largest count. */
+int ofh_batch_cg(ofh_batch B, int prec, int mixed, const double* d_b, double* d_x, int max_it,
+                 double toler, int32_t* h_iters, int32_t* h_status, double* h_relres,
+                 ofh_solve_info* total);
+/* Launch shape: workgroup size {256,512,1024} x threads per row {1,2,4,8}. */
+int ofh_batch_set_config(ofh_batch B, int block_threads, int threads_per_row);
+int ofh_batch_get_config(ofh_batch B, int* block_threads, int* threads_per_row);
+/* Times every launch shape on this batch (x is restored) and keeps the
+ * fastest for later ofh_batch_cg calls. */
+int ofh_batch_autotune(ofh_batch B, int prec, int mixed, const double* d_b, double* d_x,
+                       int max_it, double toler, int* best_block, int* best_tpr,
+                       double* best_ms);
+
 /* ------------------------------------------------------ meshes (host side) */
 /* Structured rectangle, Mesh::Mesh(xmin,xmax,ymin,ymax,nx,ny,cx0,cxN,cy0,cyN,
  * opt,nb_dof) (Mesh.cpp:547-676): h_coord (nn x 2), h_code (nn x nb_dof; the

@@ -43,6 +43,13 @@ SIGNATURES = {
     "ofh_gmres": (i32, [vp, i32, vp, vp, i32, i32, f64, vp]),
     "ofh_linear_solve": (i32, [vp, i32, i32, vp, vp, i32, f64, vp]),
     "ofh_cg_mixed": (i32, [vp, i32, vp, vp, i32, f64, vp]),
+    "ofh_batch_create": (i32, [vp, i32, i32p, C.POINTER(vp)]),
+    "ofh_batch_destroy": (i32, [vp]),
+    "ofh_batch_cg": (i32, [vp, i32, i32, vp, vp, i32, f64, i32p, i32p, f64p, vp]),
+    "ofh_batch_set_config": (i32, [vp, i32, i32]),
+    "ofh_batch_get_config": (i32, [vp, C.POINTER(i32), C.POINTER(i32)]),
+    "ofh_batch_autotune": (i32, [vp, i32, i32, vp, vp, i32, f64, C.POINTER(i32), C.POINTER(i32),
+                                 f64p]),
     "ofh_mesh2d_structured": (i32, [f64, f64, f64, f64, i32, i32, i32, i32, i32, i32, i32, i32,
                                     f64p, i32p, i32p]),
     "ofh_mesh3d_kuhn": (i32, [f64] * 6 + [i32] * 9 + [f64p, i32p, i32p]),

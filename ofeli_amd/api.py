@@ -313,3 +313,73 @@ class AssemblyPlan:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Independent systems = the diagonal blocks of one CSR, solved together
+    by the persistent batched CG (ofh_batch_*)."""
+
+    def __init__(self, A, sys_rows):
+        sr = np.ascontiguousarray(sys_rows, np.int32)
+        self.A = A
+        self.nsys = len(sr) - 1
+        self.sys_rows = sr
+        h = C.c_void_p()
+        check(lib().ofh_batch_create(A.h, self.nsys, sr.ctypes.data_as(C.POINTER(C.c_int32)),
+                                     C.byref(h)))
+        self.h = h
+        A.ctx._children.add(self)
+
+    def cg(self, b, x, prec=DIAG_PREC, mixed=False, max_it=1000, toler=1e-8):
+        it = np.zeros(self.nsys, np.int32)
+        st = np.zeros(self.nsys, np.int32)
+        rr = np.zeros(self.nsys)
+        info = SolveInfo()
+        check(lib().ofh_batch_cg(self.h, int(prec), int(bool(mixed)), _f64(b, self.A.n),
+                                 _f64(x, self.A.n), int(max_it), float(toler),
+                                 it.ctypes.data_as(C.POINTER(C.c_int32)),
+                                 st.ctypes.data_as(C.POINTER(C.c_int32)),
+                                 rr.ctypes.data_as(C.POINTER(C.c_double)), C.byref(info)))
+        return it, st, rr, info
+
+    def set_config(self, block, tpr):
+        check(lib().ofh_batch_set_config(self.h, int(block), int(tpr)))
+
+    def config(self):
+        b, t = C.c_int(), C.c_int()
+        check(lib().ofh_batch_get_config(self.h, C.byref(b), C.byref(t)))
+        return b.value, t.value
+
+    def autotune(self, b, x, prec=DIAG_PREC, mixed=False, max_it=1000, toler=1e-8):
+        bb, bt, ms = C.c_int(), C.c_int(), C.c_double()
+        check(lib().ofh_batch_autotune(self.h, int(prec), int(bool(mixed)), _f64(b, self.A.n),
+                                       _f64(x, self.A.n), int(max_it), float(toler),
+                                       C.byref(bb), C.byref(bt), C.byref(ms)))
+        return bb.value, bt.value, ms.value
+
+    def close(self):
+        if self.h:
+            lib().ofh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def concat_meshes(meshes):
+    """Disjoint union of (coord, code, elem) meshes, node ids renumbered; the
+    free DOFs of mesh s stay contiguous, so its equations are rows
+    sys_rows[s]..sys_rows[s+1] of the plan matrix."""
+    coords, codes, elems, sys_rows = [], [], [], [0]
+    off = 0
+    for coord, code, elem in meshes:
+        coords.append(coord)
+        codes.append(code)
+        elems.append(elem + off)
+        off += coord.shape[0]
+        sys_rows.append(sys_rows[-1] + int((code == 0).sum()))
+    return (np.concatenate(coords), np.concatenate(codes), np.concatenate(elems),
+            np.array(sys_rows, np.int32))

@@ -78,3 +78,23 @@ def config1(n=96):
     rng = SplitMix64(SEED_BASE + 1)
     g = fourier_field(rng, centroids(coord, elem))
     return dict(coord=coord, code=code, elem=elem, f=np.ones(coord.shape[0]), coef=np.exp(g))
+
+
+def config2(nsys=512, nmin=16, nmax=96):
+    """Batched steady heat (DC2DT3): nsys unit squares, n x n cells with n
+    uniform in [nmin, nmax], per-element lognormal k (sigma 0.75), bump
+    source per system, Dirichlet 0.  Returns the concatenated mesh, the
+    system row ranges, k and f."""
+    from . import api
+    rng = SplitMix64(SEED_BASE + 2)
+    sizes = nmin + (rng.next_u64(nsys) % np.uint64(nmax - nmin + 1)).astype(np.int64)
+    meshes, ks, fs = [], [], []
+    for n in sizes:
+        coord, code, elem = api.mesh2d(int(n), int(n))
+        meshes.append((coord, code, elem))
+        ks.append(np.exp(0.75 * rng.normal(elem.shape[0])))
+        fs.append(gaussian_bumps(rng, coord))
+    coord, code, elem, sys_rows = api.concat_meshes(meshes)
+    return dict(coord=coord, code=code, elem=elem, sys_rows=sys_rows, sizes=sizes,
+                coef=np.concatenate(ks), f=np.concatenate(fs), meshes=meshes,
+                ks=ks, fs=fs)

@@ -1,0 +1,102 @@
+"""Batched persistent CG (ofh_batch_*) against per-system oracle solves."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _t():
+    import torch
+    return torch
+
+
+def small_batch(ctx, sizes=(5, 9, 12, 7, 16, 6), seed=0):
+    import ofeli_amd as oa
+    rng = np.random.default_rng(seed)
+    meshes, ks, fs = [], [], []
+    for n in sizes:
+        meshes.append(oa.api.mesh2d(n, n))
+        ks.append(np.exp(0.75 * rng.standard_normal(2 * n * n)))
+        fs.append(1.0 + rng.random((n + 1) ** 2))
+    coord, code, elem, sys_rows = oa.concat_meshes(meshes)
+    P = oa.AssemblyPlan(ctx, coord, elem, code)
+    b = _t().zeros(P.nb_eq, dtype=_t().float64, device="cuda")
+    P.build(oa.DC2DT3, coef=_t().as_tensor(np.concatenate(ks), device="cuda"),
+            f=_t().as_tensor(np.concatenate(fs), device="cuda"), matrix=True, b=b)
+    return P, b, sys_rows, sizes, ks, fs
+
+
+def oracle_system(n, k, f):
+    m = O.mesh2d(n, n)
+    dof, neq = O.number_equations(m.code)
+    rp, ci = O.node_graph(m.elem, 1, dof, neq)
+    a, b = O.assemble(O.DC2DT3, m, dof, neq, rp, ci, f=f, coef=k)
+    return rp, ci, a, b
+
+
+def test_batch_blocks_equal_per_system_assembly(ctx):
+    P, b, sys_rows, sizes, ks, fs = small_batch(ctx)
+    rp, col = P.A.pattern()
+    val = P.A.values()
+    bh = b.cpu().numpy()
+    for s, n in enumerate(sizes):
+        orp, oci, oa_, ob = oracle_system(n, ks[s], fs[s])
+        r0, r1 = sys_rows[s], sys_rows[s + 1]
+        k0 = rp[r0]
+        assert np.array_equal(rp[r0:r1 + 1] - k0, orp)
+        assert np.array_equal(col[rp[r0]:rp[r1]] - r0, oci - 1)
+        assert np.array_equal(val[rp[r0]:rp[r1]], oa_)
+        assert np.array_equal(bh[r0:r1], ob)
+
+
+@pytest.mark.parametrize("block,tpr", [(256, 1), (512, 4), (1024, 8)])
+def test_batch_fp64_matches_oracle(ctx, block, tpr):
+    import ofeli_amd as oa
+    P, b, sys_rows, sizes, ks, fs = small_batch(ctx)
+    B = oa.Batch(P.A, sys_rows)
+    B.set_config(block, tpr)
+    x = _t().zeros(P.nb_eq, dtype=_t().float64, device="cuda")
+    it, st, rr, info = B.cg(b, x, max_it=1000, toler=1e-10)
+    xh = x.cpu().numpy()
+    for s, n in enumerate(sizes):
+        orp, oci, oa_, ob = oracle_system(n, ks[s], fs[s])
+        ret, xo, _ = O.cg(orp, oci, oa_, ob, max_it=1000, toler=1e-10)
+        assert ret > 0 and abs(st[s] - ret) <= 1 and it[s] == st[s]
+        r0, r1 = sys_rows[s], sys_rows[s + 1]
+        assert np.linalg.norm(xh[r0:r1] - xo) / np.linalg.norm(xo) < 1e-9
+    assert info.nb_it > 0
+
+
+def test_batch_mixed_fp64_verified(ctx):
+    import ofeli_amd as oa
+    P, b, sys_rows, sizes, ks, fs = small_batch(ctx, sizes=(8, 20, 33, 12, 25))
+    B = oa.Batch(P.A, sys_rows)
+    x = _t().zeros(P.nb_eq, dtype=_t().float64, device="cuda")
+    it, st, rr, info = B.cg(b, x, mixed=True, max_it=2000, toler=1e-8)
+    assert (st > 0).all() and (rr <= 1e-8).all()
+    xh = x.cpu().numpy()
+    for s, n in enumerate(sizes):
+        orp, oci, oa_, ob = oracle_system(n, ks[s], fs[s])
+        ret, xo, _ = O.cg(orp, oci, oa_, ob, max_it=2000, toler=1e-12)
+        r0, r1 = sys_rows[s], sys_rows[s + 1]
+        xs = xh[r0:r1]
+        true = np.linalg.norm(ob - O.spmv(orp, oci, oa_, xs)) / np.linalg.norm(ob)
+        assert true <= 1e-8
+        assert np.linalg.norm(xs - xo) / np.linalg.norm(xo) < 1e-6
+
+
+def test_batch_autotune_and_not_block_diagonal(ctx):
+    import ofeli_amd as oa
+    P, b, sys_rows, sizes, ks, fs = small_batch(ctx)
+    B = oa.Batch(P.A, sys_rows)
+    x = _t().zeros(P.nb_eq, dtype=_t().float64, device="cuda")
+    blk, tpr, ms = B.autotune(b, x, max_it=1000, toler=1e-10)
+    assert blk in (256, 512, 1024) and tpr in (1, 2, 4, 8) and ms > 0
+    assert B.config() == (blk, tpr)
+    assert float(x.abs().max()) == 0.0  # the guesses are restored
+    bad = sys_rows.copy()
+    bad[1] += 1  # a row whose columns cross into the previous system
+    with pytest.raises(oa.OfhError):
+        oa.Batch(P.A, bad)
