@@ -8,7 +8,12 @@ solved to rtol 1e-8 with CG + DIAG_PREC.  One step = one solve from x0 = 0.
 FP64 residual replacement, FP64-verified); the plain FP64 CG time, the
 speedup and the SpMV roofline are reported beside it.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 96]
+`--config 2` benches the batched workload instead: 512 uneven 2D steady
+heat systems (DC2DT3, n x n cells with n uniform in [16, 96], 1,826,484 rows
+in all) solved in one persistent launch (FP64 and mixed), against a loop of
+per-system FP64 CG solves.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 96] [--config 1|2]
 """
 import argparse
 import json
@@ -39,11 +44,15 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", type=int, default=1, choices=(1, 2))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see the sharding design in DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "(the sharding design is written up beside the code); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
+    if args.config == 2:
+        return bench_batched(args)
 
     import torch
     import ofeli_amd as oa
@@ -188,6 +197,145 @@ def main():
                      "traffic": 171.4e6 if args.n == 96 else None,
                      "kernel": "k_spmv<double> (CSR-stream FP64)",
                      "bytes_per_launch": b_spmv64, "avg_ms": round(spmv_ms, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_batched(args):
+    """512 uneven DC2DT3 systems: persistent batched CG (FP64 and mixed,
+    autotuned launch shape, graph replay) vs a loop of per-system ofh_cg."""
+    import torch
+    import ofeli_amd as oa
+    from ofeli_amd import workloads as W
+
+    t0 = time.time()
+    d = W.config2()
+    log("[bench2] config2 generated in %.1fs" % (time.time() - t0))
+    ctx = oa.Context(0)
+    P = oa.AssemblyPlan(ctx, d["coord"], d["elem"], d["code"])
+    A = P.A
+    n, nnz = A.n, A.nnz
+    sys_rows = d["sys_rows"]
+    nsys = len(sys_rows) - 1
+    b = torch.zeros(n, dtype=torch.float64, device="cuda")
+    P.build(oa.DC2DT3, coef=torch.as_tensor(d["coef"], device="cuda"),
+            f=torch.as_tensor(d["f"], device="cuda"), matrix=True, b=b)
+    torch.cuda.synchronize()
+    log("[bench2] %d systems, n=%d nnz=%d" % (nsys, n, nnz))
+    B = oa.Batch(A, sys_rows)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    kw = dict(max_it=20000, toler=args.toler)
+    shape64 = B.autotune(b, x, mixed=False, **kw)
+    shape32 = B.autotune(b, x, mixed=True, **kw)
+    log("[bench2] autotune fp64 %s mixed %s" % (shape64, shape32))
+
+    def run(mixed):
+        B.set_config(*(shape32 if mixed else shape64)[:2])
+        xs = torch.zeros(n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        it, st, rr, info = B.cg(b, xs, mixed=mixed, **kw)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, it, st, rr, xs
+
+    for _ in range(args.warmup):
+        run(False)
+        run(True)
+    t64, t32 = [], []
+    for k in range(args.steps):
+        r64 = run(False)
+        r32 = run(True)
+        t64.append(r64[0])
+        t32.append(r32[0])
+        log("[bench2] step %d: batched fp64 %.2f ms | mixed %.2f ms" % (k, r64[0] * 1e3, r32[0] * 1e3))
+    _, it64, st64, rr64, x64 = r64
+    _, it32, st32, rr32, x32 = r32
+
+    # baseline: one ofh_cg per system (the reference's loop of LinearSolver
+    # solves, each on the GPU)
+    rp, col = A.pattern()
+    val = A.values()
+    mats = []
+    for s in range(nsys):
+        r0, r1 = int(sys_rows[s]), int(sys_rows[s + 1])
+        k0, k1 = int(rp[r0]), int(rp[r1])
+        mats.append(oa.CsrMatrix(ctx, r1 - r0, rp[r0:r1 + 1] - k0, col[k0:k1] - r0, val[k0:k1]))
+
+    def loop():
+        xs = torch.zeros(n, dtype=torch.float64, device="cuda")
+        its = []
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for s in range(nsys):
+            r0, r1 = int(sys_rows[s]), int(sys_rows[s + 1])
+            info = oa.cg(mats[s], b[r0:r1], xs[r0:r1], prec=oa.DIAG_PREC, **kw)
+            its.append(info.iterations)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, np.array(its), xs
+
+    loop()
+    tl = []
+    for _ in range(max(1, min(args.steps, 2))):
+        tl_s, it_loop, x_loop = loop()
+        tl.append(tl_s)
+    ms_loop = 1e3 * float(np.median(tl))
+    err64 = float(torch.linalg.norm(x64 - x_loop) / torch.linalg.norm(x_loop))
+    err32 = float(torch.linalg.norm(x32 - x_loop) / torch.linalg.norm(x_loop))
+    ms64 = 1e3 * float(np.median(t64))
+    ms32 = 1e3 * float(np.median(t32))
+    sizes = np.diff(sys_rows)
+    nnz_s = np.array([rp[sys_rows[s + 1]] - rp[sys_rows[s]] for s in range(nsys)])
+    # algorithmic bytes of the FP64 batched launch: per system and iteration
+    # one FP64 SpMV (12 B/nnz + 4 B/row pointer + x, y) + the CG vector streams
+    bytes64 = float(np.sum(it64.astype(np.float64) * (12 * nnz_s + 4 * sizes + 16 * sizes + 88 * sizes)))
+    eff_gbs = bytes64 / (ms64 * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.lib(fast=True)
+        sample = list(range(nsys))  # the whole batch: a few seconds on one core
+        t = time.perf_counter()
+        work = 0.0
+        for s in sample:
+            r0, r1 = int(sys_rows[s]), int(sys_rows[s + 1])
+            k0, k1 = int(rp[r0]), int(rp[r1])
+            O.cg((rp[r0:r1 + 1] - k0).astype(np.int64), (col[k0:k1] - r0).astype(np.int64) + 1,
+                 val[k0:k1], b[r0:r1].cpu().numpy(), prec=O.DIAG, max_it=20000, toler=args.toler,
+                 fast=True)
+            work += float(it_loop[s]) * (nnz_s[s] + 10 * sizes[s])
+        cpu_s = time.perf_counter() - t
+        total = float(np.sum(it_loop * (nnz_s + 10 * sizes)))
+        cpu = {"value": round(cpu_s * total / work * 1e3, 1),
+               "unit": "ms per batch of FP64 CG+DIAG solves (extrapolated)", "cores": 1,
+               "kind": "port",
+               "sample": "%d of the %d systems solved one after another by the oracle -Ofast "
+                         "on 1 host core in %.2f s" % (len(sample), nsys, cpu_s)}
+        log("[bench2] cpu sample %.2fs -> %.0f ms" % (cpu_s, cpu["value"]))
+
+    out = {
+        "metric": METRIC, "value": round(ms32, 3),
+        "unit": "ms per batch of 512 solves (batched mixed CG, rtol %g)" % args.toler,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms32, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "512 DC2DT3 steady heat systems, n x n cells, n in [16,96]",
+                   "rows": n, "nnz": nnz, "systems": nsys, "toler": args.toler,
+                   "precond": "DIAG_PREC"},
+        "batched_fp64_ms": round(ms64, 3), "loop_fp64_ms": round(ms_loop, 3),
+        "speedup_batched_fp64_vs_loop": round(ms_loop / ms64, 2),
+        "speedup_batched_mixed_vs_loop": round(ms_loop / ms32, 2),
+        "shape_fp64": list(shape64[:2]), "shape_mixed": list(shape32[:2]),
+        "max_iterations_fp64": int(it64.max()), "max_iterations_mixed": int(it32.max()),
+        "all_converged": bool((st64 >= 0).all() and (st32 >= 0).all()),
+        "max_true_relres_mixed": float(rr32.max()),
+        "batched_fp64_err_vs_loop": err64, "batched_mixed_err_vs_loop": err32,
+        "roofline": {"bound": "hbm", "achieved": round(eff_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(eff_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_batch_cg64 (persistent batched CG, FP64)",
+                     "note": "effective: each system's vectors stay in L2/MALL during its solve",
+                     "bytes_per_launch": bytes64, "avg_ms": round(ms64, 3)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -169,6 +169,10 @@ int ofh_batch_destroy(ofh_batch B);
 int ofh_batch_cg(ofh_batch B, int prec, int mixed, const double* d_b, double* d_x, int max_it,
                  double toler, int32_t* h_iters, int32_t* h_status, double* h_relres,
                  ofh_solve_info* total);
+/* ofh_batch_cg replays its launch (queue reset + persistent kernel) from a
+ * HIP graph captured for the current arguments (enable = 1, the default), or
+ * launches it directly (enable = 0).  Results are identical either way. */
+int ofh_batch_use_graph(ofh_batch B, int enable);
 /* Launch shape: workgroup size {256,512,1024} x threads per row {1,2,4,8}. */
 int ofh_batch_set_config(ofh_batch B, int block_threads, int threads_per_row);
 int ofh_batch_get_config(ofh_batch B, int* block_threads, int* threads_per_row);

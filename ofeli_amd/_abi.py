@@ -46,6 +46,7 @@ SIGNATURES = {
     "ofh_batch_create": (i32, [vp, i32, i32p, C.POINTER(vp)]),
     "ofh_batch_destroy": (i32, [vp]),
     "ofh_batch_cg": (i32, [vp, i32, i32, vp, vp, i32, f64, i32p, i32p, f64p, vp]),
+    "ofh_batch_use_graph": (i32, [vp, i32]),
     "ofh_batch_set_config": (i32, [vp, i32, i32]),
     "ofh_batch_get_config": (i32, [vp, C.POINTER(i32), C.POINTER(i32)]),
     "ofh_batch_autotune": (i32, [vp, i32, i32, vp, vp, i32, f64, C.POINTER(i32), C.POINTER(i32),

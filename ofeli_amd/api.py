@@ -110,7 +110,7 @@ class CsrMatrix:
         self.n, self.nnz, self.max_row_nnz = n_.value, nnz.value, mr.value
 
     def compress_columns(self, enable=True):
-        """16-bit block-relative column offsets in the SpMV stream (default on)."""
+        """16-bit block-relative column offsets in the SpMV stream (default off)."""
         check(lib().ofh_csr_compress_columns(self.h, int(bool(enable))))
 
     def pattern(self):
@@ -344,6 +344,9 @@ class Batch:
 
     def set_config(self, block, tpr):
         check(lib().ofh_batch_set_config(self.h, int(block), int(tpr)))
+
+    def use_graph(self, enable=True):
+        check(lib().ofh_batch_use_graph(self.h, int(bool(enable))))
 
     def config(self):
         b, t = C.c_int(), C.c_int()

@@ -28,6 +28,23 @@
 #include "ofh_internal.h"
 #include "ofh_solver.h"
 
+// What a captured batched launch depends on; a change of any field recaptures.
+struct BatchGraphKey {
+  const void* fn;
+  const double* b;
+  double* x;
+  int max_it;
+  double tol;
+  const void* dinv;
+  const void* val32;
+  const void* work;
+  int grid;
+  bool operator==(const BatchGraphKey& o) const {
+    return fn == o.fn && b == o.b && x == o.x && max_it == o.max_it && tol == o.tol &&
+           dinv == o.dinv && val32 == o.val32 && work == o.work && grid == o.grid;
+  }
+};
+
 struct ofh_batch_s {
   ofh_csr A = nullptr;
   int nsys = 0;
@@ -41,6 +58,10 @@ struct ofh_batch_s {
   size_t work_bytes = 0;
   int block = 256, tpr = 1;  // launch shape
   bool tuned = false;
+  bool use_graph = true;         // replay the launch from a captured graph
+  hipGraphExec_t gexec = nullptr;
+  hipStream_t cap = nullptr;     // capture-only stream
+  BatchGraphKey gkey{};
   std::vector<int> h_rows;
 };
 
@@ -319,10 +340,9 @@ static LaunchFn pick_fn(int prec, bool mixed, int block, int tpr) {
 }
 
 static int batch_run(ofh_batch B, int prec, int mixed, const double* b, double* x, int max_it,
-                     double toler, int block, int tpr) {
+                     double toler, int block, int tpr, hipStream_t s, bool graph) {
   ofh_csr M = B->A;
   ofh_ctx ctx = M->ctx;
-  hipStream_t s = ctx->stream;
   int rc;
   if (prec == OFH_DIAG_PREC && (rc = csr_ensure_dinv(M))) return rc;
   if (mixed && (rc = csr_ensure_val32(M))) return rc;
@@ -358,9 +378,36 @@ static int batch_run(ofh_batch B, int prec, int mixed, const double* b, double* 
   // empty, so all of them reach the exit
   const int per_cu = block >= 1024 ? 2 : (block >= 512 ? 4 : 8);
   const int grid = std::max(1, std::min(B->nsys, ctx->num_cu * per_cu));
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, B->counter, 0);
-  fn(a, grid, s);
-  OFH_HIP(hipGetLastError());
+  if (!graph) {
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, B->counter, 0);
+    fn(a, grid, s);
+    OFH_HIP(hipGetLastError());
+    return OFH_OK;
+  }
+  // The queue reset and the persistent launch are captured once per set of
+  // arguments into a graph and replayed; the reset is a kernel, not a memset
+  // node.
+  const BatchGraphKey key{(const void*)fn, b, x, max_it, toler, M->dinv, M->val32, B->work, grid};
+  if (!B->gexec || !(key == B->gkey)) {
+    if (B->gexec) {
+      OFH_HIP(hipGraphExecDestroy(B->gexec));
+      B->gexec = nullptr;
+    }
+    if (!B->cap) OFH_HIP(hipStreamCreateWithFlags(&B->cap, hipStreamNonBlocking));
+    OFH_HIP(hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, B->cap, B->counter, 0);
+    fn(a, grid, B->cap);
+    hipError_t le = hipGetLastError();
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(B->cap, &g);
+    OFH_HIP(le);
+    OFH_HIP(ce);
+    hipError_t ie = hipGraphInstantiate(&B->gexec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    OFH_HIP(ie);
+    B->gkey = key;
+  }
+  OFH_HIP(hipGraphLaunch(B->gexec, s));
   return OFH_OK;
 }
 
@@ -413,6 +460,8 @@ int ofh_batch_destroy(ofh_batch B) {
   hipDeviceSynchronize();
   void* ptrs[] = {B->sys_rows, B->order, B->counter, B->iters, B->status, B->relres, B->work};
   for (void* p : ptrs) hipFree(p);
+  if (B->gexec) hipGraphExecDestroy(B->gexec);
+  if (B->cap) hipStreamDestroy(B->cap);
   delete B;
   return OFH_OK;
 }
@@ -424,6 +473,12 @@ int ofh_batch_set_config(ofh_batch B, int block_threads, int threads_per_row) {
   B->block = block_threads;
   B->tpr = threads_per_row;
   B->tuned = true;
+  return OFH_OK;
+}
+
+int ofh_batch_use_graph(ofh_batch B, int enable) {
+  OFH_CHECK(B, OFH_E_INVALID, "null batch");
+  B->use_graph = enable != 0;
   return OFH_OK;
 }
 
@@ -442,7 +497,7 @@ int ofh_batch_cg(ofh_batch B, int prec, int mixed, const double* d_b, double* d_
             "ofh_batch_cg: IDENT_PREC or DIAG_PREC");
   hipStream_t s = B->A->ctx->stream;
   const double t0 = now_seconds();
-  int rc = batch_run(B, prec, mixed, d_b, d_x, max_it, toler, B->block, B->tpr);
+  int rc = batch_run(B, prec, mixed, d_b, d_x, max_it, toler, B->block, B->tpr, s, B->use_graph);
   if (rc) return rc;
   std::vector<int> it(B->nsys), st(B->nsys);
   std::vector<double> rr(B->nsys);
@@ -493,7 +548,7 @@ int ofh_batch_autotune(ofh_batch B, int prec, int mixed, const double* d_b, doub
       for (int rep = 0; rep < 2; ++rep) {  // first run warms caches and code
         OFH_HIP(hipMemcpyAsync(d_x, x0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         OFH_HIP(hipEventRecord(e0, s));
-        int rc = batch_run(B, prec, mixed, d_b, d_x, max_it, toler, blocks[ib], tprs[it]);
+        int rc = batch_run(B, prec, mixed, d_b, d_x, max_it, toler, blocks[ib], tprs[it], s, false);
         if (rc) { hipFree(x0); return rc; }
         OFH_HIP(hipEventRecord(e1, s));
         OFH_HIP(hipEventSynchronize(e1));

@@ -100,3 +100,35 @@ def test_batch_autotune_and_not_block_diagonal(ctx):
     bad[1] += 1  # a row whose columns cross into the previous system
     with pytest.raises(oa.OfhError):
         oa.Batch(P.A, bad)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batch_graph_replays_equal_eager(ctx, mixed):
+    """The captured launch replayed several times, with eager work and
+    synchronisations in between, gives bitwise the eager launch's result."""
+    import ofeli_amd as oa
+    torch = _t()
+    P, b, sys_rows, sizes, ks, fs = small_batch(ctx, sizes=(6, 14, 9, 21))
+    B = oa.Batch(P.A, sys_rows)
+    B.use_graph(False)
+    x_eager = torch.zeros(P.nb_eq, dtype=torch.float64, device="cuda")
+    it_e, st_e, rr_e, _ = B.cg(b, x_eager, mixed=mixed, max_it=1500, toler=1e-9)
+    B.use_graph(True)
+    x = torch.zeros_like(x_eager)
+    for rep in range(4):
+        x.zero_()
+        it, st, rr, _ = B.cg(b, x, mixed=mixed, max_it=1500, toler=1e-9)
+        torch.cuda.synchronize()
+        assert torch.equal(x, x_eager), rep
+        assert np.array_equal(it, it_e) and np.array_equal(st, st_e) and np.array_equal(rr, rr_e)
+        y = torch.zeros_like(x)
+        oa.spmv(P.A, x, y)  # eager work between replays
+        torch.cuda.synchronize()
+    # a second right-hand side recaptures, the first is still reproduced after
+    b2 = 2.0 * b
+    x2 = torch.zeros_like(x)
+    B.cg(b2, x2, mixed=mixed, max_it=1500, toler=1e-9)
+    x.zero_()
+    B.cg(b, x, mixed=mixed, max_it=1500, toler=1e-9)
+    assert torch.equal(x, x_eager)
+    assert float(torch.linalg.norm(x2 - 2.0 * x_eager) / torch.linalg.norm(x2)) < 1e-7

@@ -324,3 +324,28 @@ def test_bicgstab_breakdown_code():
     a = np.array([1.0, -1.0])
     ret, x, _ = O.bicgstab(rp, ci, a, np.array([1.0, 0.0]), prec=O.IDENT, max_it=5, toler=1e-12)
     assert ret in (2, 3) or ret < 0
+
+
+def test_traction_rhs_is_edge_integral():
+    """Constant traction per side: each end node of a side receives half the
+    side length times the traction (2-point Gauss is exact for it); the total
+    equals traction x loaded length; clamped DOFs receive nothing."""
+    m = O.mesh2d(6, 3, quad=True, nb_dof=2, codes=(1, -2, 0, 0), box=(0.0, 4.0, 0.0, 1.0))
+    m.code[:, 1] = m.code[:, 0]
+    dof, neq = O.number_equations(m.code)
+    right = m.sides[m.side_code == 2]  # negative codes make sides (Mesh.cpp:617-669)
+    assert right.shape[0] == 3 and np.allclose(m.coord[right - 1, 0], 4.0)
+    t = np.tile([0.25, -1.5], (3, 1))
+    b = O.traction_rhs(m, right, t.ravel(), dof, np.zeros(neq))
+    assert np.isclose(b[0::2].sum(), 0.25 * 1.0) and np.isclose(b[1::2].sum(), -1.5 * 1.0)
+    ys = m.coord[:, 1]
+    for node in np.unique(right):
+        h = 1.0 / 3.0
+        share = h if 0.0 < ys[node - 1] < 1.0 else h / 2
+        assert np.isclose(b[dof[node - 1, 1] - 1], -1.5 * share)
+        assert np.isclose(b[dof[node - 1, 0] - 1], 0.25 * share)
+    # a traction on the clamped edge changes nothing
+    left = np.array([[1, 2], [2, 3], [3, 4]])  # nodes of x = 0
+    assert np.allclose(m.coord[left - 1, 0], 0.0)
+    b2 = O.traction_rhs(m, left, np.tile([1.0, 1.0], len(left)), dof, np.zeros(neq))
+    assert not b2.any()
