@@ -244,6 +244,47 @@ int ofh_asm_traction(ofh_asm P, int nb_sides, const int32_t* h_side_nodes,
  * solution and the Dirichlet values (d_bc may be NULL: zero). */
 int ofh_asm_insert_bc(ofh_asm P, const double* d_v, const double* d_bc, double* d_u);
 
+/* ------------------------------------------ learned initial guess (device) */
+/* No reference routine: the reference starts a solve from zero (Equa::run,
+ * Vect_impl.h:547) or from the previous step's solution (TimeStepping.cpp:207).
+ * For a sequence of solves A x_{k+1} = b_{k+1} with A symmetric, positive
+ * diagonal, a message-passing network on the graph of A predicts the start
+ * vector from the last two solutions x_n, x_p and the new b:
+ *   r = b - A x_n, z = D^-1 r, s = rms(z), z2 = D^-1 (r - A z)
+ *   F = [z/s, z2/s, (x_n - x_p)/s, ln a_ii - mean, 0 x 4]      (FP32)
+ *   H = relu(F W_enc + b_enc)                                    (8 -> 64)
+ *   per layer: Z = H W_ab (64 -> 128, BF16 matrix cores),
+ *              M_i = sum_{j != i} relu(Z_i[:64] + Z_j[64:] + e_ij w_e + b_m),
+ *              e_ij = a_ij / sqrt(a_ii a_jj),
+ *              H += relu([H, M] W_u + b_u) (128 -> 64, BF16 matrix cores)
+ *   x0 = x_n + s (H w_dec + b_dec).
+ * Training minimises L = (x0-x*)^T A (x0-x*) / (x_n-x*)^T A (x_n-x*), the
+ * A-norm error of the guess relative to the warm start's, with x* the
+ * solver's solution; Adam (0.9, 0.999, 1e-8) on FP32 master weights.  The
+ * decoder starts at zero, so an untrained network returns x_n exactly.
+ * Parameters are one FP32 vector: W_enc[8][64], b_enc[64], per layer
+ * W_ab[64][128], w_e[64], b_m[64], W_u[128][64], b_u[64]; w_dec[64], b_dec. */
+typedef struct ofh_mpnn_s* ofh_mpnn;
+int ofh_mpnn_create(ofh_csr A, int layers, int hidden, uint64_t seed, ofh_mpnn* out);
+int ofh_mpnn_destroy(ofh_mpnn M);
+int ofh_mpnn_num_params(ofh_mpnn M, int64_t* count);
+int ofh_mpnn_get_params(ofh_mpnn M, float* h_params);
+/* Sets the parameters and resets the optimiser state. */
+int ofh_mpnn_set_params(ofh_mpnn M, const float* h_params);
+/* ofh_mpnn_guess replays its kernels from a HIP graph captured per argument
+ * set (enable = 1, the default) or launches them directly. */
+int ofh_mpnn_use_graph(ofh_mpnn M, int enable);
+/* x0 = network(x_n, x_p, b); stream-ordered, no host synchronisation. */
+int ofh_mpnn_guess(ofh_mpnn M, const double* d_xn, const double* d_xp, const double* d_b,
+                   double* d_x0);
+/* Forward + loss + backward: d_x0 receives the guess, h_loss the loss and
+ * h_grad (may be NULL) the gradient of the parameter vector. */
+int ofh_mpnn_loss_grad(ofh_mpnn M, const double* d_xn, const double* d_xp, const double* d_b,
+                       const double* d_xstar, double* d_x0, double* h_loss, float* h_grad);
+/* One Adam step on one sample (h_loss may be NULL: then no host sync). */
+int ofh_mpnn_train_step(ofh_mpnn M, const double* d_xn, const double* d_xp, const double* d_b,
+                        const double* d_xstar, double* d_x0, float lr, double* h_loss);
+
 #ifdef __cplusplus
 }
 #endif

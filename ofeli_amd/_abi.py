@@ -8,6 +8,7 @@ f64 = C.c_double
 i32p = C.POINTER(C.c_int32)
 i64p = C.POINTER(C.c_int64)
 f64p = C.POINTER(C.c_double)
+f32p = C.POINTER(C.c_float)
 
 # name -> (restype, argtypes); keep in the order of include/ofeli_hip.h
 SIGNATURES = {
@@ -64,6 +65,15 @@ SIGNATURES = {
     "ofh_asm_build": (i32, [vp, i32, vp, vp, vp, vp, f64p, i32, i32, vp]),
     "ofh_asm_traction": (i32, [vp, i32, i32p, vp, vp]),
     "ofh_asm_insert_bc": (i32, [vp, vp, vp, vp]),
+    "ofh_mpnn_create": (i32, [vp, i32, i32, C.c_uint64, C.POINTER(vp)]),
+    "ofh_mpnn_destroy": (i32, [vp]),
+    "ofh_mpnn_num_params": (i32, [vp, C.POINTER(i64)]),
+    "ofh_mpnn_get_params": (i32, [vp, f32p]),
+    "ofh_mpnn_set_params": (i32, [vp, f32p]),
+    "ofh_mpnn_use_graph": (i32, [vp, i32]),
+    "ofh_mpnn_guess": (i32, [vp, vp, vp, vp, vp]),
+    "ofh_mpnn_loss_grad": (i32, [vp, vp, vp, vp, vp, vp, f64p, f32p]),
+    "ofh_mpnn_train_step": (i32, [vp, vp, vp, vp, vp, vp, C.c_float, f64p]),
 }
 
 

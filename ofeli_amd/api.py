@@ -372,6 +372,68 @@ class Batch:
             pass
 
 
+class LearnedGuess:
+    """Message-passing network predicting the start vector of the next solve
+    of a sequence A x_{k+1} = b_{k+1} (ofh_mpnn_*); A symmetric."""
+
+    def __init__(self, A, layers=4, hidden=64, seed=0):
+        self.A = A
+        self.layers = layers
+        h = C.c_void_p()
+        check(lib().ofh_mpnn_create(A.h, int(layers), int(hidden), C.c_uint64(seed), C.byref(h)))
+        self.h = h
+        cnt = C.c_int64()
+        check(lib().ofh_mpnn_num_params(self.h, C.byref(cnt)))
+        self.nparams = cnt.value
+        A.ctx._children.add(self)
+
+    def params(self):
+        p = np.zeros(self.nparams, np.float32)
+        check(lib().ofh_mpnn_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.nparams:
+            raise ValueError("expected %d parameters" % self.nparams)
+        check(lib().ofh_mpnn_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def use_graph(self, enable=True):
+        check(lib().ofh_mpnn_use_graph(self.h, int(bool(enable))))
+
+    def guess(self, x_n, x_p, b, x0):
+        n = self.A.n
+        check(lib().ofh_mpnn_guess(self.h, _f64(x_n, n), _f64(x_p, n), _f64(b, n), _f64(x0, n)))
+
+    def loss_grad(self, x_n, x_p, b, x_star, x0, want_grad=True):
+        n = self.A.n
+        loss = C.c_double()
+        g = np.zeros(self.nparams, np.float32) if want_grad else None
+        check(lib().ofh_mpnn_loss_grad(self.h, _f64(x_n, n), _f64(x_p, n), _f64(b, n),
+                                       _f64(x_star, n), _f64(x0, n), C.byref(loss),
+                                       g.ctypes.data_as(C.POINTER(C.c_float)) if want_grad else None))
+        return loss.value, g
+
+    def train_step(self, x_n, x_p, b, x_star, x0, lr=1e-3, want_loss=True):
+        n = self.A.n
+        loss = C.c_double()
+        check(lib().ofh_mpnn_train_step(self.h, _f64(x_n, n), _f64(x_p, n), _f64(b, n),
+                                        _f64(x_star, n), _f64(x0, n), float(lr),
+                                        C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def close(self):
+        if self.h:
+            lib().ofh_mpnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def concat_meshes(meshes):
     """Disjoint union of (coord, code, elem) meshes, node ids renumbered; the
     free DOFs of mesh s stay contiguous, so its equations are rows
