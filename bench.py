@@ -44,7 +44,12 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", type=int, default=1, choices=(1, 2))
+    ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3))
+    ap.add_argument("--n3", type=int, default=512, help="config 3 grid (n x n cells)")
+    ap.add_argument("--t-steps", type=int, default=2000, help="config 3 time steps")
+    ap.add_argument("--train-steps", type=int, default=200, help="config 3 steps used for training")
+    ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--lr", type=float, default=1e-3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -53,6 +58,8 @@ def main():
         sys.exit(2)
     if args.config == 2:
         return bench_batched(args)
+    if args.config == 3:
+        return bench_transient(args)
 
     import torch
     import ofeli_amd as oa
@@ -336,6 +343,145 @@ def bench_batched(args):
                      "kernel": "k_batch_cg64 (persistent batched CG, FP64)",
                      "note": "effective: each system's vectors stay in L2/MALL during its solve",
                      "bytes_per_launch": bytes64, "avg_ms": round(ms64, 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_transient(args):
+    """Transient heat (DC2DT3, backward Euler, 512^2 cells, 2000 steps of
+    1e-3): steps 1..200 with FP64 CG warm-started from u^n give the training
+    samples of the learned guess; steps 201..T are then run from the same
+    state with each start/solver variant."""
+    import torch
+    import ofeli_amd as oa
+    from ofeli_amd import workloads as W
+    from ofeli_amd.transient import HeatBE
+
+    d = W.config3(args.n3)
+    ctx = oa.Context(0)
+    t0 = time.time()
+    S = HeatBE(ctx, d["coord"], d["elem"], d["code"], d["coef"], d["dt"])
+    torch.cuda.synchronize()
+    n, nnz = S.A.n, S.A.nnz
+    log("[bench3] n=%d nnz=%d setup %.2fs" % (n, nnz, time.time() - t0))
+    T, K = args.t_steps, args.train_steps
+    xy = torch.as_tensor(d["coord"], device="cuda")
+    # every step's nodal source, resident in HBM before timing
+    F = torch.empty((T, xy.shape[0]), dtype=torch.float64, device="cuda")
+    for s_ in range(T):
+        F[s_] = W.source_field(xy, d["path"], (s_ + 1) * d["dt"])
+    torch.cuda.synchronize()
+    tol = args.toler
+
+    samples = []
+    tA = time.perf_counter()
+    itA = []
+    for s_ in range(K):
+        S.rhs(F[s_])
+        S.start("warm")
+        x_n, x_p, b = S.x.clone(), S.xp.clone(), S.b.clone()
+        info = S.solve(mixed=False, toler=tol)
+        itA.append(info.iterations)
+        samples.append((x_n, x_p, b, S.x0.clone()))
+        S.advance()
+    torch.cuda.synchronize()
+    tA = time.perf_counter() - tA
+    log("[bench3] steps 1..%d fp64 warm: %.2f ms/step, mean %.1f it" % (K, tA / K * 1e3, np.mean(itA)))
+    st = S.state()
+
+    M = oa.LearnedGuess(S.A, layers=4, seed=0)
+    x0buf = torch.empty_like(S.b)
+    rng = np.random.default_rng(0)
+    t0 = time.perf_counter()
+    hist = []
+    nsteps = args.epochs * len(samples)
+    k_ = 0
+    for ep in range(args.epochs):
+        tot = 0.0
+        for i in rng.permutation(len(samples)):
+            lr = args.lr * 0.5 * (1 + np.cos(np.pi * k_ / nsteps))  # cosine decay
+            tot += M.train_step(*samples[i], x0buf, lr=lr)
+            k_ += 1
+        hist.append(tot / len(samples))
+        log("[bench3] epoch %d: mean loss %.4f" % (ep, hist[-1]))
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del samples
+
+    def run(guess, mixed):
+        S.set_state(st)
+        torch.cuda.synchronize()
+        its, rr = [], []
+        t = time.perf_counter()
+        for s_ in range(K, T):
+            info = S.step(F[s_], guess=guess, mixed=mixed, toler=tol)
+            its.append(info.iterations)
+            rr.append(info.relres if not mixed else info.true_relres)
+        torch.cuda.synchronize()
+        dt_ = time.perf_counter() - t
+        return dt_ / (T - K) * 1e3, np.array(its), max(rr), S.x.clone()
+
+    variants = [("fp64_warm", "warm", False), ("fp64_extrap", "extrapolate", False),
+                ("fp64_learned", M, False), ("mixed_warm", "warm", True),
+                ("mixed_learned", M, True)]
+    res = {}
+    for name, g, mx in variants:
+        ms, its, rr, xf = run(g, mx)
+        res[name] = (ms, its, rr, xf)
+        log("[bench3] %-14s %.3f ms/step  mean it %.1f  max relres %.2e" % (name, ms, its.mean(), rr))
+    # learned-guess inference alone
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    S.set_state(st)
+    S.rhs(F[K])
+    M.guess(S.x, S.xp, S.b, S.x0)
+    ev0.record()
+    for _ in range(20):
+        M.guess(S.x, S.xp, S.b, S.x0)
+    ev1.record()
+    torch.cuda.synchronize()
+    infer_ms = ev0.elapsed_time(ev1) / 20
+    base = res["fp64_warm"]
+    xb = base[3]
+    errs = {k: float(torch.linalg.norm(v[3] - xb) / torch.linalg.norm(xb)) for k, v in res.items()}
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.lib(fast=True)
+        S.set_state(st)
+        S.rhs(F[K])
+        rp, col = S.A.pattern()
+        vals = S.A.values()
+        bh, xh = S.b.cpu().numpy(), S.x.cpu().numpy()
+        t = time.perf_counter()
+        ret, _, _ = O.cg(rp.astype(np.int64), col.astype(np.int64) + 1, vals, bh, x0=xh,
+                         prec=O.DIAG, max_it=20000, toler=tol, fast=True)
+        cpu_s = time.perf_counter() - t
+        cpu = {"value": round(cpu_s * 1e3, 2), "unit": "ms per FP64 CG+DIAG solve (warm start)",
+               "cores": 1, "kind": "port",
+               "sample": "the solve of step %d (%d iterations) by the oracle -Ofast on 1 host core"
+                         % (K + 1, ret)}
+        log("[bench3] cpu %.1f ms (%d it)" % (cpu_s * 1e3, ret))
+
+    ms_h = res["mixed_learned"][0]
+    out = {
+        "metric": METRIC, "value": round(ms_h, 4),
+        "unit": "ms per time step (RHS assembly + learned guess + mixed CG to rtol %g)" % tol,
+        "n_gpus": 1, "steps": T - K, "warmup": K, "ms_per_step": round(ms_h, 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64+bf16",
+        "data": "synthetic",
+        "config": {"workload": "transient DC2DT3 backward Euler %dx%d, dt 1e-3, moving source"
+                               % (args.n3, args.n3), "rows": n, "nnz": nnz, "time_steps": T,
+                   "train_steps": K, "toler": tol, "precond": "DIAG_PREC"},
+        "variants_ms_per_step": {k: round(v[0], 4) for k, v in res.items()},
+        "variants_mean_iterations": {k: round(float(v[1].mean()), 2) for k, v in res.items()},
+        "variants_max_relres": {k: float(v[2]) for k, v in res.items()},
+        "final_state_rel_err_vs_fp64_warm": errs,
+        "speedup_mixed_learned_vs_fp64_warm": round(base[0] / ms_h, 3),
+        "speedup_fp64_learned_vs_fp64_warm": round(base[0] / res["fp64_learned"][0], 3),
+        "train_seconds": round(train_s, 2), "train_loss_per_epoch": [round(h, 4) for h in hist],
+        "guess_ms": round(infer_ms, 4), "phaseA_ms_per_step": round(tA / K * 1e3, 4),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

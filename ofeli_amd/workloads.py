@@ -98,3 +98,33 @@ def config2(nsys=512, nmin=16, nmax=96):
     return dict(coord=coord, code=code, elem=elem, sys_rows=sys_rows, sizes=sizes,
                 coef=np.concatenate(ks), f=np.concatenate(fs), meshes=meshes,
                 ks=ks, fs=fs)
+
+
+def config3(n=512):
+    """Transient heat (DC2DT3, backward Euler): n x n cells, k_e = exp(g) with
+    g the 64-mode Gaussian field of correlation 0.1 at element centroids,
+    rho = cp = 1, dt = 1e-3, Dirichlet 0, u^0 = 0; source: a Gaussian bump
+    (sigma 0.03, unit integral) moving on a seeded Lissajous path."""
+    from . import api
+    coord, code, elem = api.mesh2d(n, n)
+    rng = SplitMix64(SEED_BASE + 3)
+    g = fourier_field(rng, centroids(coord, elem))
+    wx, wy = rng.uniform(2, 2 * np.pi * 0.5, 2 * np.pi * 1.5)
+    phi = float(rng.uniform(1, 0.0, 2 * np.pi)[0])
+    sigma = 0.03
+    path = dict(wx=float(wx), wy=float(wy), phi=phi, sigma=sigma, amp=1.0 / (2 * np.pi * sigma ** 2),
+                radius=0.3)
+    return dict(coord=coord, code=code, elem=elem, coef=np.exp(g), dt=1e-3, path=path)
+
+
+def source_centre(path, t):
+    return (0.5 + path["radius"] * np.sin(path["wx"] * t + path["phi"]),
+            0.5 + path["radius"] * np.sin(path["wy"] * t))
+
+
+def source_field(xy, path, t):
+    """Nodal source at time t; xy is a numpy array or a torch tensor."""
+    cx, cy = source_centre(path, t)
+    d2 = (xy[:, 0] - cx) ** 2 + (xy[:, 1] - cy) ** 2
+    e = -d2 / (2.0 * path["sigma"] ** 2)
+    return path["amp"] * (e.exp() if hasattr(e, "exp") else np.exp(e))
