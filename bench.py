@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--train-steps", type=int, default=200, help="config 3 steps used for training")
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--proj-m", type=int, default=16, help="config 3 projection basis size")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -350,9 +351,11 @@ def bench_batched(args):
 
 def bench_transient(args):
     """Transient heat (DC2DT3, backward Euler, 512^2 cells, 2000 steps of
-    1e-3): steps 1..200 with FP64 CG warm-started from u^n give the training
-    samples of the learned guess; steps 201..T are then run from the same
-    state with each start/solver variant."""
+    1e-3).  Steps 1..200 run FP64 CG warm-started from u^n (the reference's
+    TimeStepping start); their (inputs, solution) pairs train the learned
+    guesses.  Steps 201..T are then run from the same state with each start
+    vector and solver: warm, extrapolated, learned, projection onto previous
+    solutions, projection corrected by the network; FP64 and mixed CG."""
     import torch
     import ofeli_amd as oa
     from ofeli_amd import workloads as W
@@ -365,7 +368,7 @@ def bench_transient(args):
     torch.cuda.synchronize()
     n, nnz = S.A.n, S.A.nnz
     log("[bench3] n=%d nnz=%d setup %.2fs" % (n, nnz, time.time() - t0))
-    T, K = args.t_steps, args.train_steps
+    T, K, PM = args.t_steps, args.train_steps, args.proj_m
     xy = torch.as_tensor(d["coord"], device="cuda")
     # every step's nodal source, resident in HBM before timing
     F = torch.empty((T, xy.shape[0]), dtype=torch.float64, device="cuda")
@@ -374,42 +377,63 @@ def bench_transient(args):
     torch.cuda.synchronize()
     tol = args.toler
 
-    samples = []
+    # phase A: FP64 warm-started steps, training samples
+    proj = oa.ProjectionGuess(S.A, m=PM)
+    xg = torch.empty_like(S.b)
+    samples, samples_p, last = [], [], []
     tA = time.perf_counter()
     itA = []
     for s_ in range(K):
         S.rhs(F[s_])
         S.start("warm")
         x_n, x_p, b = S.x.clone(), S.xp.clone(), S.b.clone()
+        proj.guess(S.b, xg)
         info = S.solve(mixed=False, toler=tol)
         itA.append(info.iterations)
-        samples.append((x_n, x_p, b, S.x0.clone()))
+        xs = S.x0.clone()
+        samples.append((x_n, x_p, b, xs))
+        samples_p.append((xg.clone(), x_n, b, xs))
         S.advance()
+        proj.update(S.x)
+        last = (last + [xs])[-PM:]
     torch.cuda.synchronize()
     tA = time.perf_counter() - tA
     log("[bench3] steps 1..%d fp64 warm: %.2f ms/step, mean %.1f it" % (K, tA / K * 1e3, np.mean(itA)))
     st = S.state()
+    proj.close()
 
-    M = oa.LearnedGuess(S.A, layers=4, seed=0)
-    x0buf = torch.empty_like(S.b)
-    rng = np.random.default_rng(0)
-    t0 = time.perf_counter()
-    hist = []
-    nsteps = args.epochs * len(samples)
-    k_ = 0
-    for ep in range(args.epochs):
-        tot = 0.0
-        for i in rng.permutation(len(samples)):
-            lr = args.lr * 0.5 * (1 + np.cos(np.pi * k_ / nsteps))  # cosine decay
-            tot += M.train_step(*samples[i], x0buf, lr=lr)
-            k_ += 1
-        hist.append(tot / len(samples))
-        log("[bench3] epoch %d: mean loss %.4f" % (ep, hist[-1]))
-    torch.cuda.synchronize()
-    train_s = time.perf_counter() - t0
-    del samples
+    def train(net, data, tag):
+        x0buf = torch.empty_like(S.b)
+        rng = np.random.default_rng(0)
+        t0 = time.perf_counter()
+        hist = []
+        nsteps = args.epochs * len(data)
+        k_ = 0
+        for ep in range(args.epochs):
+            tot = 0.0
+            for i in rng.permutation(len(data)):
+                lr = args.lr * 0.5 * (1 + np.cos(np.pi * k_ / nsteps))  # cosine decay
+                tot += net.train_step(*data[i], x0buf, lr=lr)
+                k_ += 1
+            hist.append(round(tot / len(data), 5))
+        torch.cuda.synchronize()
+        log("[bench3] %s: loss per epoch %s (%.1fs)" % (tag, hist, time.perf_counter() - t0))
+        return hist, time.perf_counter() - t0
 
-    def run(guess, mixed):
+    net_x = oa.LearnedGuess(S.A, layers=4, seed=0, extrapolate=True)
+    hist_x, tr_x = train(net_x, samples, "net on extrapolation")
+    net_p = oa.LearnedGuess(S.A, layers=4, seed=1, extrapolate=False)
+    hist_p, tr_p = train(net_p, samples_p, "net on projection")
+    del samples, samples_p
+
+    def make_proj():
+        P = oa.ProjectionGuess(S.A, m=PM)
+        for x in last:
+            P.update(x)
+        return P
+
+    def run(make, mixed):
+        guess = make()
         S.set_state(st)
         torch.cuda.synchronize()
         its, rr = [], []
@@ -422,25 +446,35 @@ def bench_transient(args):
         dt_ = time.perf_counter() - t
         return dt_ / (T - K) * 1e3, np.array(its), max(rr), S.x.clone()
 
-    variants = [("fp64_warm", "warm", False), ("fp64_extrap", "extrapolate", False),
-                ("fp64_learned", M, False), ("mixed_warm", "warm", True),
-                ("mixed_learned", M, True)]
+    variants = [("fp64_warm", lambda: "warm", False),
+                ("fp64_extrap", lambda: "extrapolate", False),
+                ("fp64_learned", lambda: net_x, False),
+                ("fp64_proj", make_proj, False),
+                ("fp64_proj_learned", lambda: (make_proj(), net_p), False),
+                ("mixed_warm", lambda: "warm", True),
+                ("mixed_proj", make_proj, True),
+                ("mixed_proj_learned", lambda: (make_proj(), net_p), True)]
     res = {}
-    for name, g, mx in variants:
-        ms, its, rr, xf = run(g, mx)
+    for name, make, mx in variants:
+        ms, its, rr, xf = run(make, mx)
         res[name] = (ms, its, rr, xf)
-        log("[bench3] %-14s %.3f ms/step  mean it %.1f  max relres %.2e" % (name, ms, its.mean(), rr))
-    # learned-guess inference alone
+        log("[bench3] %-19s %.3f ms/step  mean it %.1f  max relres %.2e" % (name, ms, its.mean(), rr))
+    # cost of the start vectors alone
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     S.set_state(st)
     S.rhs(F[K])
-    M.guess(S.x, S.xp, S.b, S.x0)
-    ev0.record()
-    for _ in range(20):
-        M.guess(S.x, S.xp, S.b, S.x0)
-    ev1.record()
-    torch.cuda.synchronize()
-    infer_ms = ev0.elapsed_time(ev1) / 20
+    P = make_proj()
+    timings = {}
+    for tag, fn in (("learned", lambda: net_x.guess(S.x, S.xp, S.b, S.x0)),
+                    ("proj_guess", lambda: P.guess(S.b, S.x0)),
+                    ("proj_update", lambda: P.update(S.x))):
+        fn()
+        ev0.record()
+        for _ in range(20):
+            fn()
+        ev1.record()
+        torch.cuda.synchronize()
+        timings[tag] = round(ev0.elapsed_time(ev1) / 20, 4)
     base = res["fp64_warm"]
     xb = base[3]
     errs = {k: float(torch.linalg.norm(v[3] - xb) / torch.linalg.norm(xb)) for k, v in res.items()}
@@ -464,24 +498,27 @@ def bench_transient(args):
                          % (K + 1, ret)}
         log("[bench3] cpu %.1f ms (%d it)" % (cpu_s * 1e3, ret))
 
-    ms_h = res["mixed_learned"][0]
+    head = "mixed_proj_learned"
+    ms_h = res[head][0]
     out = {
         "metric": METRIC, "value": round(ms_h, 4),
-        "unit": "ms per time step (RHS assembly + learned guess + mixed CG to rtol %g)" % tol,
+        "unit": "ms per time step (RHS assembly + projected, network-corrected start + mixed CG "
+                "to rtol %g)" % tol,
         "n_gpus": 1, "steps": T - K, "warmup": K, "ms_per_step": round(ms_h, 4),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64+bf16",
         "data": "synthetic",
         "config": {"workload": "transient DC2DT3 backward Euler %dx%d, dt 1e-3, moving source"
                                % (args.n3, args.n3), "rows": n, "nnz": nnz, "time_steps": T,
-                   "train_steps": K, "toler": tol, "precond": "DIAG_PREC"},
+                   "train_steps": K, "toler": tol, "precond": "DIAG_PREC", "proj_vectors": PM},
+        "baseline": "fp64_warm", "headline": head,
         "variants_ms_per_step": {k: round(v[0], 4) for k, v in res.items()},
         "variants_mean_iterations": {k: round(float(v[1].mean()), 2) for k, v in res.items()},
         "variants_max_relres": {k: float(v[2]) for k, v in res.items()},
         "final_state_rel_err_vs_fp64_warm": errs,
-        "speedup_mixed_learned_vs_fp64_warm": round(base[0] / ms_h, 3),
-        "speedup_fp64_learned_vs_fp64_warm": round(base[0] / res["fp64_learned"][0], 3),
-        "train_seconds": round(train_s, 2), "train_loss_per_epoch": [round(h, 4) for h in hist],
-        "guess_ms": round(infer_ms, 4), "phaseA_ms_per_step": round(tA / K * 1e3, 4),
+        "speedup_vs_fp64_warm": {k: round(base[0] / v[0], 3) for k, v in res.items()},
+        "train_seconds": round(tr_x + tr_p, 2),
+        "train_loss_per_epoch": {"net_extrapolation": hist_x, "net_projection": hist_p},
+        "start_vector_ms": timings, "phaseA_ms_per_step": round(tA / K * 1e3, 4),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

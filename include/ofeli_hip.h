@@ -244,28 +244,55 @@ int ofh_asm_traction(ofh_asm P, int nb_sides, const int32_t* h_side_nodes,
  * solution and the Dirichlet values (d_bc may be NULL: zero). */
 int ofh_asm_insert_bc(ofh_asm P, const double* d_v, const double* d_bc, double* d_u);
 
+/* --------------------------------- start vectors from previous solutions */
+/* No reference routine (the reference starts from zero or from the previous
+ * solution, TimeStepping.cpp:207).  For solves A x_k = b_k with one SPD
+ * matrix, keeps an A-orthonormal basis V (V^T A V = I) of the last
+ * max_vectors new solution directions; the start vector x0 = V V^T b is the
+ * A-norm-closest vector of span(V) to A^-1 b (Fischer 1998).  After each
+ * solve, ofh_proj_update keeps the solution in a ring of the last
+ * max_vectors solutions; when V is full it is rebuilt from the last
+ * max_vectors/2 of them, otherwise the solution is A-orthogonalised against
+ * V (classical Gram-Schmidt, two passes), A-normalised and appended (a
+ * direction with A-norm below 1e-12 of the solution's is stored as zero).
+ * The span always holds the newest solution, so the guess is never worse
+ * than the warm start in the A-norm.  An append costs one SpMV and about
+ * 6 x max_vectors vector passes; a guess 1 + 2 x max_vectors passes.
+ * max_vectors: 8, 16 or 32. */
+typedef struct ofh_proj_s* ofh_proj;
+int ofh_proj_create(ofh_csr A, int max_vectors, ofh_proj* out);
+int ofh_proj_destroy(ofh_proj P);
+int ofh_proj_reset(ofh_proj P);
+int ofh_proj_size(ofh_proj P, int* count);
+int ofh_proj_guess(ofh_proj P, const double* d_b, double* d_x0);
+int ofh_proj_update(ofh_proj P, const double* d_x);
+
 /* ------------------------------------------ learned initial guess (device) */
 /* No reference routine: the reference starts a solve from zero (Equa::run,
  * Vect_impl.h:547) or from the previous step's solution (TimeStepping.cpp:207).
  * For a sequence of solves A x_{k+1} = b_{k+1} with A symmetric, positive
- * diagonal, a message-passing network on the graph of A predicts the start
- * vector from the last two solutions x_n, x_p and the new b:
- *   r = b - A x_n, z = D^-1 r, s = rms(z), z2 = D^-1 (r - A z)
+ * diagonal, a message-passing network on the graph of A predicts a search
+ * direction d from the base x_b and the new b; the start vector is the base
+ * plus the exact A-norm line-search step along d, so it is never worse than
+ * the base:
+ *   x_b = x_n (extrapolate = 0) or x_n + (x_n - x_p) (extrapolate = 1)
+ *   r = b - A x_b, z = D^-1 r, s = rms(z), z2 = D^-1 (r - A z)
  *   F = [z/s, z2/s, (x_n - x_p)/s, ln a_ii - mean, 0 x 4]      (FP32)
  *   H = relu(F W_enc + b_enc)                                    (8 -> 64)
  *   per layer: Z = H W_ab (64 -> 128, BF16 matrix cores),
  *              M_i = sum_{j != i} relu(Z_i[:64] + Z_j[64:] + e_ij w_e + b_m),
  *              e_ij = a_ij / sqrt(a_ii a_jj),
  *              H += relu([H, M] W_u + b_u) (128 -> 64, BF16 matrix cores)
- *   x0 = x_n + s (H w_dec + b_dec).
- * Training minimises L = (x0-x*)^T A (x0-x*) / (x_n-x*)^T A (x_n-x*), the
- * A-norm error of the guess relative to the warm start's, with x* the
- * solver's solution; Adam (0.9, 0.999, 1e-8) on FP32 master weights.  The
- * decoder starts at zero, so an untrained network returns x_n exactly.
+ *   d = s (H w_dec + b_dec),  x0 = x_b + alpha d,  alpha = d.r / d.Ad.
+ * Training minimises L = 1 - (d.g)^2 / (d.Ad E0) with g = A (x* - x_b),
+ * E0 = (x* - x_b).g and x* the solver's solution: the A-norm error left
+ * after the line search relative to the base's (scale-invariant in d); Adam
+ * (0.9, 0.999, 1e-8) on FP32 master weights.
  * Parameters are one FP32 vector: W_enc[8][64], b_enc[64], per layer
  * W_ab[64][128], w_e[64], b_m[64], W_u[128][64], b_u[64]; w_dec[64], b_dec. */
 typedef struct ofh_mpnn_s* ofh_mpnn;
-int ofh_mpnn_create(ofh_csr A, int layers, int hidden, uint64_t seed, ofh_mpnn* out);
+int ofh_mpnn_create(ofh_csr A, int layers, int hidden, int extrapolate, uint64_t seed,
+                    ofh_mpnn* out);
 int ofh_mpnn_destroy(ofh_mpnn M);
 int ofh_mpnn_num_params(ofh_mpnn M, int64_t* count);
 int ofh_mpnn_get_params(ofh_mpnn M, float* h_params);
@@ -274,10 +301,10 @@ int ofh_mpnn_set_params(ofh_mpnn M, const float* h_params);
 /* ofh_mpnn_guess replays its kernels from a HIP graph captured per argument
  * set (enable = 1, the default) or launches them directly. */
 int ofh_mpnn_use_graph(ofh_mpnn M, int enable);
-/* x0 = network(x_n, x_p, b); stream-ordered, no host synchronisation. */
+/* x0 = x_b + alpha d; stream-ordered, no host synchronisation. */
 int ofh_mpnn_guess(ofh_mpnn M, const double* d_xn, const double* d_xp, const double* d_b,
                    double* d_x0);
-/* Forward + loss + backward: d_x0 receives the guess, h_loss the loss and
+/* Forward + loss + backward: d_x0 receives the guess, h_loss the loss L and
  * h_grad (may be NULL) the gradient of the parameter vector. */
 int ofh_mpnn_loss_grad(ofh_mpnn M, const double* d_xn, const double* d_xp, const double* d_b,
                        const double* d_xstar, double* d_x0, double* h_loss, float* h_grad);

@@ -372,15 +372,54 @@ class Batch:
             pass
 
 
+class ProjectionGuess:
+    """Start vectors from the span of previous solutions (ofh_proj_*):
+    x0 = V V^T b over an A-orthonormal basis of the last m directions."""
+
+    def __init__(self, A, m=16):
+        self.A = A
+        h = C.c_void_p()
+        check(lib().ofh_proj_create(A.h, int(m), C.byref(h)))
+        self.h = h
+        self.m = m
+        A.ctx._children.add(self)
+
+    def guess(self, b, x0):
+        check(lib().ofh_proj_guess(self.h, _f64(b, self.A.n), _f64(x0, self.A.n)))
+
+    def update(self, x):
+        check(lib().ofh_proj_update(self.h, _f64(x, self.A.n)))
+
+    def reset(self):
+        check(lib().ofh_proj_reset(self.h))
+
+    def size(self):
+        c = C.c_int()
+        check(lib().ofh_proj_size(self.h, C.byref(c)))
+        return c.value
+
+    def close(self):
+        if self.h:
+            lib().ofh_proj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class LearnedGuess:
     """Message-passing network predicting the start vector of the next solve
     of a sequence A x_{k+1} = b_{k+1} (ofh_mpnn_*); A symmetric."""
 
-    def __init__(self, A, layers=4, hidden=64, seed=0):
+    def __init__(self, A, layers=4, hidden=64, seed=0, extrapolate=False):
         self.A = A
         self.layers = layers
         h = C.c_void_p()
-        check(lib().ofh_mpnn_create(A.h, int(layers), int(hidden), C.c_uint64(seed), C.byref(h)))
+        check(lib().ofh_mpnn_create(A.h, int(layers), int(hidden), int(bool(extrapolate)),
+                                    C.c_uint64(seed), C.byref(h)))
         self.h = h
         cnt = C.c_int64()
         check(lib().ofh_mpnn_num_params(self.h, C.byref(cnt)))

@@ -1,16 +1,21 @@
 // ofh_cg.hip -- preconditioned CG on the GPU (CG.h:99-157 replacement).
 //
-// Launch sequence (host enqueues, the device decides):
-//   k_cg_init         r = b - A x; partial (b,b) (r,r) (r,z)
-//   k_cg_dir(0)       fold; stop test of the initial guess (CG.h:116); p = z
+// Launch sequence, IDENT/DIAG (host enqueues, the device decides), two
+// kernels per iteration:
+//   k_cg_init           r = b - A x, z = P r; partial (b,b) (r,r) (r,z)
+//   k_cg_dirspmv(0)     fold; stop test of the initial guess (CG.h:116);
+//                       p = z and q = A p; partial (p,q)
 //   per iteration it = 1, 2, ...:
-//     k_cg_spmv(it)   q = A p; partial (p,q)                    (CSR-stream)
-//     k_cg_upd(it)    fold (p,q): alpha = rho/(p,q); x += alpha p;
-//                     r -= alpha q; partial (r,r) (r,z) with z = P r
-//     k_cg_dir(it)    fold: ||r||/||b|| <= toler -> stop (CG.h:144); else
-//                     beta = rho_it/rho_{it-1}, p = z + beta p (CG.h:127-133)
-// SSOR_PREC: k_cg_upd stores no (r,z); k_cg_check folds (r,r) and takes the
-// stop decision, the level-scheduled sweeps compute z, k_cg_rz writes (r,z).
+//     k_cg_upd(it)      fold (p,q): alpha = rho/(p,q); x += alpha p;
+//                       r -= alpha q; z = P r; partial (r,r) (r,z)
+//     k_cg_dirspmv(it)  fold: ||r||/||b|| <= toler -> stop (CG.h:144); else
+//                       beta = rho_it/rho_{it-1}; q = A p with the new
+//                       direction p = z + beta p (CG.h:127-135) formed at the
+//                       SpMV's gather (ping-pong p buffers); partial (p,q)
+// Larger systems and SSOR_PREC use three kernels per iteration: k_cg_spmv
+// (q = A p), k_cg_upd and k_cg_dir (fold, stop test, p = z + beta p); with
+// SSOR, k_cg_check folds (r,r) for the stop decision before the
+// level-scheduled sweeps compute z and k_cg_rz writes (r,z).
 // A reduction is folded by every block of the kernel that needs it (no
 // intra-kernel cross-block traffic); all blocks fold the same partials in the
 // same order and take the same decision; block 0 records it.  Once the stop
@@ -48,6 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int nblocks, const int* __re
                                                    const double* __restrict__ x,
                                                    double* __restrict__ r,
                                                    const double* __restrict__ dinv,
+                                                   double* __restrict__ z,
                                                    double* __restrict__ partials) {
   __shared__ double lds[kTile64];
   __shared__ double red[3 * kBlock / kWave];
@@ -61,7 +67,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int nblocks, const int* __re
     r[i] = ri;
     v[0] = bi * bi;
     v[1] = ri * ri;
-    if (PREC != OFH_SSOR_PREC) v[2] = ri * zval<PREC>(i, ri, dinv, nullptr);
+    if (PREC != OFH_SSOR_PREC) {
+      const double zi = zval<PREC>(i, ri, dinv, nullptr);
+      if (PREC == OFH_DIAG_PREC && z) z[i] = zi;
+      v[2] = ri * zi;
+    }
   }
   block_sum_k<kBlock, 3>(v, red);
   const int slot[3] = {0, 1, 2};
@@ -154,6 +164,95 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int n, int it, int nparts0, i
   }
 }
 
+// The new direction p_c = z_c + beta p_c (CG.h:131-133; p_c = z_c for the
+// first direction), formed where the SpMV gathers it.
+struct GatherDir {
+  const double* __restrict__ z;
+  const double* __restrict__ p;
+  double beta;
+  int first;
+  __device__ __forceinline__ double operator()(int c) const {
+    return first ? z[c] : z[c] + beta * p[c];
+  }
+};
+
+// IDENT/DIAG, after `it` iterations: fold (r,r), (r,z) (it > 0: slots 0, 1
+// of k_cg_upd; it == 0: slots 0..2 of k_cg_init), stop test, beta; then
+// q = A p_new with p_new = z + beta p_old; p_new of the block's rows is
+// stored; partial (p,q) -> slot 3.  z is D^-1 r (DIAG) or r (IDENT).
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_cg_dirspmv(int nblocks, const int* __restrict__ rb,
+                                                      const int* __restrict__ rp, ColIndex ci,
+                                                      const double* __restrict__ val, int it,
+                                                      int nparts0, int nparts,
+                                                      const double* __restrict__ z,
+                                                      const double* __restrict__ p_old,
+                                                      double* __restrict__ p_new,
+                                                      double* __restrict__ q, KrylovState* st,
+                                                      double* __restrict__ partials) {
+  if (st->done) return;
+  __shared__ double lds[kTile64];
+  __shared__ double red[3 * kBlock / kWave];
+  const int blk = xcd_block(blockIdx.x, nblocks);
+  const int r0 = rb[blk], r1 = rb[blk + 1];
+  double t[3];
+  double nrm;
+  if (it == 0) {
+    const int slot[3] = {0, 1, 2};
+    fold_partials<kBlock, 3>(t, slot, partials, nparts0, red);
+    nrm = sqrt(t[0]);
+    if (nrm == 0) nrm = 1;  // CG.h:113-114
+  } else {
+    const int slot[2] = {0, 1};
+    double u[2];
+    fold_partials<kBlock, 2>(u, slot, partials, nparts, red);
+    t[1] = u[0];
+    t[2] = u[1];
+    nrm = st->nrm;
+  }
+  bool stop = false;
+  int status = 0;
+  const double res = sqrt(t[1]) / nrm;
+  if (res <= st->tol) {
+    stop = true;
+    status = it;  // 0 when the initial guess passes (CG.h:116-123)
+  } else if (it >= st->max_it || !(res == res)) {
+    stop = true;  // the loop counter ends at max_it+1 (CG.h:154-156)
+    status = -(st->max_it + 1);
+  }
+  const double rho = t[2];
+  const double beta = it > 0 ? rho / st->rho_buf[(it - 1) & 1] : 0.0;  // CG.h:131
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->it = it;
+    st->res = res;
+    if (it == 0) {
+      st->nrm = nrm;
+      st->res0 = res;
+    }
+    if (stop) {
+      st->done = 1;
+      st->status = status;
+    } else {
+      st->rho_buf[it & 1] = rho;
+      st->beta = beta;
+    }
+  }
+  if (stop) return;
+  const GatherDir g{z, p_old, beta, it == 0};
+  const double sq = csr_stream_rowsum_g<double, kTile64>(r0, r1, rp, ci, val, g, lds, 0.0);
+  const int i = r0 + threadIdx.x;
+  double v[1] = {0.0};
+  if (i < r1) {
+    const double pi = g(i);
+    p_new[i] = pi;
+    q[i] = sq;
+    v[0] = pi * sq;
+  }
+  block_sum_k<kBlock, 1>(v, red);
+  const int slot[1] = {3};
+  write_partials<1>(v, slot, partials, blockIdx.x);
+}
+
 // q = A p; partial (p,q) -> slot 3 (CG.h:134-135)
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(int nblocks, const int* __restrict__ rb,
                                                    const int* __restrict__ rp,
@@ -188,7 +287,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_upd(int n, int it, int nparts_pq,
                                                   const double* __restrict__ p,
                                                   const double* __restrict__ q,
                                                   const double* __restrict__ dinv,
-                                                  KrylovState* st, double* __restrict__ partials) {
+                                                  double* __restrict__ z, KrylovState* st,
+                                                  double* __restrict__ partials) {
   if (st->done) return;
   __shared__ double red[2 * kBlock / kWave];
   // this thread's loads go out before the fold of the (p,q) partials
@@ -218,8 +318,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_upd(int n, int it, int nparts_pq,
       const double ri = rv[j] - alpha * qv[j];
       r[i] = ri;
       v[0] += ri * ri;
-      if (PREC == OFH_DIAG_PREC) v[1] += ri * (ri * dv[j]);
-      else if (PREC == OFH_IDENT_PREC) v[1] += ri * ri;
+      if (PREC == OFH_DIAG_PREC) {
+        const double zi = ri * dv[j];  // Prec_impl.h:245-247
+        if (z) z[i] = zi;
+        v[1] += ri * zi;
+      } else if (PREC == OFH_IDENT_PREC) {
+        v[1] += ri * ri;
+      }
     }
   }
   block_sum_k<kBlock, 2>(v, red);
@@ -290,12 +395,13 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   if (PREC == OFH_DIAG_PREC && (rc = csr_ensure_dinv(A))) return rc;
   if (PREC == OFH_SSOR_PREC && (rc = csr_ensure_levels(A))) return rc;
   void* wsp;
-  const int nv = PREC == OFH_SSOR_PREC ? 4 : 3;
+  const int nv = PREC == OFH_IDENT_PREC ? 4 : 5;
   if ((rc = ensure_vws(ctx, sizeof(double) * (size_t)n * nv, &wsp))) return rc;
   double* r = (double*)wsp;
   double* p = r + n;
   double* q = p + n;
-  double* z = PREC == OFH_SSOR_PREC ? q + n : nullptr;
+  double* p1 = q + n;  // second direction buffer (IDENT/DIAG)
+  double* z = PREC == OFH_IDENT_PREC ? r : p1 + n;  // SSOR: p1 + n is the sweep output
   KrylovState* st = state_ptr(ctx);
   KrylovState h0{};
   h0.tol = toler;
@@ -305,7 +411,45 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   const int g = reduce_grid(n);
   const int nb = A->nblocks;
   hipLaunchKernelGGL(k_cg_init<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                     A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r, A->dinv, part);
+                     A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r, A->dinv,
+                     PREC == OFH_DIAG_PREC ? z : nullptr, part);
+  // The two-kernel form gathers z and the old direction at every nonzero;
+  // it is faster only while those vectors stay L2-resident (measured on
+  // MI355X: -5% per iteration at 261k rows, +26% at 857k rows).
+  constexpr int kFuseRows = 3 << 17;
+  if (PREC != OFH_SSOR_PREC && n <= kFuseRows) {
+    double* pb[2] = {p, p1};
+    const ColIndex ci{A->col, A->col16};
+    auto dirspmv = [&](int it) {
+      hipLaunchKernelGGL(k_cg_dirspmv<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
+                         A->row_ptr, ci, A->val, it, nb, g, z, pb[it & 1], pb[(it + 1) & 1], q,
+                         st, part);
+    };
+    dirspmv(0);
+    OFH_HIP(hipGetLastError());
+    KrylovState h{};
+    if ((rc = read_state(ctx, &h))) return rc;
+    int enq = 0, it_prev = 0;
+    double res_prev = h.res;
+    while (!h.done && enq < max_it) {
+      const int chunk = next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq);
+      it_prev = h.it;
+      res_prev = h.res;
+      for (int c = 0; c < chunk; ++c) {
+        const int it = ++enq;
+        hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, nb, x, r,
+                           pb[it & 1], q, A->dinv, PREC == OFH_DIAG_PREC ? z : nullptr, st, part);
+        dirspmv(it);
+      }
+      OFH_HIP(hipGetLastError());
+      if ((rc = read_state(ctx, &h))) return rc;
+    }
+    if (!h.done) {  // max_it == 0 with a non-converged guess
+      h.done = 1;
+      h.status = -(max_it + 1);
+    }
+    return fill_info(A, b, x, h, t0, info);
+  }
   auto ssor_z = [&](int it, bool init) -> int {
     hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(kBlock), 0, s, it, init ? nb : g, (int)init, st,
                        part);
@@ -331,7 +475,7 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
       hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr,
                          ColIndex{A->col, A->col16}, A->val, p, q, st, part);
       hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, nb, x, r, p, q,
-                         A->dinv, st, part);
+                         A->dinv, nullptr, st, part);
       if (PREC == OFH_SSOR_PREC && (rc = ssor_z(it, false))) return rc;
       hipLaunchKernelGGL(k_cg_dir<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, nb, g, r, A->dinv,
                          z, p, st, part);

@@ -66,12 +66,19 @@ __device__ __forceinline__ short f2bf(float f) {
 // ---------------------------------------------------------------------------
 // features
 // ---------------------------------------------------------------------------
-// r = b - A x_n, z = D^-1 r; partial sums of z^2 per workgroup.
+// The base x_b = x_n + ext (x_n - x_p) (ext = 0: x_n exactly).
+__device__ __forceinline__ double xbase(const double* __restrict__ xn,
+                                        const double* __restrict__ xp, double ext, int i) {
+  return xn[i] + ext * (xn[i] - xp[i]);
+}
+
+// r = b - A x_b, z = D^-1 r; partial sums of z^2 per workgroup.
 __global__ __launch_bounds__(256) void k_mp_resid(int n, const int* __restrict__ rp,
                                                   const int* __restrict__ col,
                                                   const double* __restrict__ val,
                                                   const double* __restrict__ dinv,
                                                   const double* __restrict__ xn,
+                                                  const double* __restrict__ xp, double ext,
                                                   const double* __restrict__ b, double* r,
                                                   double* z, double* part) {
   __shared__ double red[4];
@@ -79,7 +86,7 @@ __global__ __launch_bounds__(256) void k_mp_resid(int n, const int* __restrict__
   double zz = 0.0;
   if (i < n) {
     double s = 0.0;
-    for (int k = rp[i]; k < rp[i + 1]; ++k) s += val[k] * xn[col[k]];
+    for (int k = rp[i]; k < rp[i + 1]; ++k) s += val[k] * xbase(xn, xp, ext, col[k]);
     const double ri = b[i] - s;
     const double zi = dinv[i] * ri;
     r[i] = ri;
@@ -416,39 +423,33 @@ __device__ __forceinline__ float wave_sum_f(float v) {
   return v;
 }
 
-// x0 = x_n + s (H w_dec + b_dec); one wavefront per row.
+// d = s (H w_dec + b_dec), the learned direction; one wavefront per row.
 __global__ __launch_bounds__(256) void k_mp_dec(int n, const float* __restrict__ H,
                                                 const float* __restrict__ P, int off,
-                                                const double* __restrict__ xn,
-                                                const double* __restrict__ scal, double* x0) {
+                                                const double* __restrict__ scal, double* dv) {
   const int c = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= n) return;
   const float d = wave_sum_f(H[(size_t)i * kH + c] * P[off + c]) + P[off + kH];
-  if (c == 0) x0[i] = xn[i] + scal[0] * (double)d;
+  if (c == 0) dv[i] = scal[0] * (double)d;
 }
 
-// q = A (x0 - x*), qn = A (x_n - x*); partials of e.q and en.qn.
-__global__ __launch_bounds__(256) void k_mp_loss(int n, const int* __restrict__ rp,
-                                                 const int* __restrict__ col,
-                                                 const double* __restrict__ val,
-                                                 const double* __restrict__ x0,
-                                                 const double* __restrict__ xn,
-                                                 const double* __restrict__ xs, double* q,
-                                                 double* part, int nparts) {
+// q = A d; partials d.q -> slot 0, d.r -> slot 1 (line search).
+__global__ __launch_bounds__(256) void k_mp_ls(int n, const int* __restrict__ rp,
+                                               const int* __restrict__ col,
+                                               const double* __restrict__ val,
+                                               const double* __restrict__ dv,
+                                               const double* __restrict__ r, double* q,
+                                               double* part, int nparts) {
   __shared__ double red[8];
   const int i = blockIdx.x * 256 + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (i < n) {
-    double a = 0.0, an = 0.0;
-    for (int k = rp[i]; k < rp[i + 1]; ++k) {
-      const int j = col[k];
-      a += val[k] * (x0[j] - xs[j]);
-      an += val[k] * (xn[j] - xs[j]);
-    }
+    double a = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) a += val[k] * dv[col[k]];
     q[i] = a;
-    v[0] = (x0[i] - xs[i]) * a;
-    v[1] = (xn[i] - xs[i]) * an;
+    v[0] = dv[i] * a;
+    v[1] = dv[i] * r[i];
   }
   block_sum_k<256, 2>(v, red);
   if (threadIdx.x == 0) {
@@ -457,34 +458,99 @@ __global__ __launch_bounds__(256) void k_mp_loss(int n, const int* __restrict__ 
   }
 }
 
-// Folds the loss partials (every workgroup), dL/dx0 = 2 q / E0, decoder
-// backward: dH = (s dL/dx0) w_dec^T and partials of the decoder gradient.
+// alpha = d.r / d.Ad (0 if d.Ad <= 0), x0 = x_b + alpha d.
+__global__ __launch_bounds__(256) void k_mp_apply(int n, const double* __restrict__ part,
+                                                  int nparts, const double* __restrict__ xn,
+                                                  const double* __restrict__ xp, double ext,
+                                                  const double* __restrict__ dv, double* scal,
+                                                  double* x0) {
+  __shared__ double red[8];
+  double v[2] = {0.0, 0.0};
+  for (int p = threadIdx.x; p < nparts; p += 256) {
+    v[0] += part[p];
+    v[1] += part[nparts + p];
+  }
+  block_sum_k<256, 2>(v, red);
+  const double dAd = v[0], dr = v[1];
+  const double alpha = dAd > 0.0 ? dr / dAd : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[4] = alpha;
+    scal[5] = dAd;
+  }
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) x0[i] = xbase(xn, xp, ext, i) + alpha * dv[i];
+}
+
+// Training: g = A (x* - x_b); partials (x* - x_b).g -> slot 2 (E0), d.g -> slot 3.
+__global__ __launch_bounds__(256) void k_mp_loss(int n, const int* __restrict__ rp,
+                                                 const int* __restrict__ col,
+                                                 const double* __restrict__ val,
+                                                 const double* __restrict__ xn,
+                                                 const double* __restrict__ xp, double ext,
+                                                 const double* __restrict__ xs,
+                                                 const double* __restrict__ dv, double* gv,
+                                                 double* part, int nparts) {
+  __shared__ double red[8];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < n) {
+    double a = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      const int j = col[k];
+      a += val[k] * (xs[j] - xbase(xn, xp, ext, j));
+    }
+    gv[i] = a;
+    v[0] = (xs[i] - xbase(xn, xp, ext, i)) * a;
+    v[1] = dv[i] * a;
+  }
+  block_sum_k<256, 2>(v, red);
+  if (threadIdx.x == 0) {
+    part[2 * nparts + blockIdx.x] = v[0];
+    part[3 * nparts + blockIdx.x] = v[1];
+  }
+}
+
+// Folds d.Ad, E0, d.g (every workgroup); L = 1 - (d.g)^2 / (d.Ad E0) and
+// dL/dd = -2 (d.g) / (d.Ad E0) (g - (d.g / d.Ad) A d); decoder backward:
+// dH = (s dL/dd) w_dec^T and partials of the decoder gradient.
 __global__ __launch_bounds__(256) void k_mp_dec_bwd(int n, int n_pad, const float* __restrict__ H,
                                                     const float* __restrict__ P, int off,
                                                     const double* __restrict__ q,
+                                                    const double* __restrict__ gv,
                                                     const double* __restrict__ lpart, int nparts,
                                                     double* scal, float* dH, float* part) {
-  __shared__ double dred[8];
+  __shared__ double dred[12];
   __shared__ float red[kH + 1];
-  double v[2] = {0.0, 0.0};
+  double v[3] = {0.0, 0.0, 0.0};
   for (int p = threadIdx.x; p < nparts; p += 256) {
     v[0] += lpart[p];
-    v[1] += lpart[nparts + p];
+    v[1] += lpart[2 * nparts + p];
+    v[2] += lpart[3 * nparts + p];
   }
-  block_sum_k<256, 2>(v, dred);
-  const double E = v[0], E0 = v[1];
-  const double gscale = E0 > 0.0 ? 2.0 * scal[0] / E0 : 0.0;
+  block_sum_k<256, 3>(v, dred);
+  const double dAd = v[0], E0 = v[1], dg = v[2];
+  double L, c1 = 0.0, c2 = 0.0;
+  if (!(E0 > 0.0)) {
+    L = 0.0;  // the base is already the solution
+  } else if (!(dAd > 0.0)) {
+    L = 1.0;  // no direction
+  } else {
+    L = 1.0 - dg * dg / (dAd * E0);
+    c1 = -2.0 * dg / (dAd * E0);
+    c2 = dg / dAd;
+  }
+  const double sc = scal[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    scal[1] = E;
-    scal[2] = E0;
-    scal[3] = E0 > 0.0 ? E / E0 : 0.0;
+    scal[1] = E0;
+    scal[2] = dg;
+    scal[3] = L;
   }
   const int c = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const float w = P[off + c];
   float gw = 0.f, gb = 0.f;
   for (int i = blockIdx.x * 4 + wave; i < n_pad; i += gridDim.x * 4) {
     float dd = 0.f;
-    if (i < n) dd = (float)(gscale * q[i]);
+    if (i < n) dd = (float)(sc * c1 * (gv[i] - c2 * q[i]));
     dH[(size_t)i * kH + c] = dd * w;
     gw += H[(size_t)i * kH + c] * dd;
     gb += dd;
@@ -562,6 +628,7 @@ __global__ __launch_bounds__(256) void k_mp_to_bf16(int L, const float* __restri
 struct ofh_mpnn_s {
   ofh_csr A = nullptr;
   int L = 0;
+  double ext = 0.0;  // base x_b = x_n + ext (x_n - x_p)
   int n = 0, n_pad = 0;
   int64_t np = 0;
   float* params = nullptr;
@@ -583,9 +650,10 @@ struct ofh_mpnn_s {
   float* dZ = nullptr;      // n_pad x 128
   double* r = nullptr;      // n
   double* z = nullptr;      // n
-  double* q = nullptr;      // n
-  double* scal = nullptr;   // [s, E, E0, loss]
-  double* dpart = nullptr;  // 2 x nparts (row reductions)
+  double* q = nullptr;      // n: A d
+  double* dv = nullptr;     // n: the learned direction d
+  double* scal = nullptr;   // [s, E0, d.g, loss, alpha, d.Ad]
+  double* dpart = nullptr;  // 4 x nparts (row reductions)
   float* part = nullptr;    // kGradGrid x (128*64 + 128) (node reductions)
   double* h_pinned = nullptr;
   int nparts = 0;
@@ -601,7 +669,7 @@ namespace mp {
 static void free_all(ofh_mpnn M) {
   void* ptrs[] = {M->params, M->grads, M->adam_m, M->adam_v, M->wbf, M->e, M->ld, M->F,
                   M->Hs, M->Zs, M->Ms, M->Us, M->dH0, M->dH1, M->T, M->dZ, M->r, M->z, M->q,
-                  M->scal, M->dpart, M->part};
+                  M->dv, M->scal, M->dpart, M->part};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (M->h_pinned) hipHostFree(M->h_pinned);
@@ -625,7 +693,7 @@ static int forward(ofh_mpnn M, const double* xn, const double* xp, const double*
   const int n = M->n, n_pad = M->n_pad;
   const int g1 = (n + 255) / 256, gp = (n_pad + 255) / 256;
   hipLaunchKernelGGL(k_mp_resid, dim3(g1), dim3(256), 0, s, n, A->row_ptr, A->col, A->val, A->dinv,
-                     xn, b, M->r, M->z, M->dpart);
+                     xn, xp, M->ext, b, M->r, M->z, M->dpart);
   hipLaunchKernelGGL(k_mp_feat, dim3(gp), dim3(256), 0, s, n, n_pad, A->row_ptr, A->col, A->val,
                      A->dinv, M->r, M->z, xn, xp, M->ld, M->dpart, g1, M->F, M->scal);
   hipLaunchKernelGGL(k_mp_enc, dim3((n_pad * kH + 255) / 256), dim3(256), 0, s, n, n_pad, M->F,
@@ -661,7 +729,11 @@ static int forward(ofh_mpnn M, const double* xn, const double* xp, const double*
     hipLaunchKernelGGL((k_mp_rowgemm<kH2, kH, false, true, false>), dim3(gg), dim3(256), 0, s, u);
   }
   hipLaunchKernelGGL(k_mp_dec, dim3((n + 3) / 4), dim3(256), 0, s, n, Hl(M, M->L), M->params,
-                     off_dec(M->L), xn, M->scal, x0);
+                     off_dec(M->L), M->scal, M->dv);
+  hipLaunchKernelGGL(k_mp_ls, dim3(g1), dim3(256), 0, s, n, A->row_ptr, A->col, A->val, M->dv,
+                     M->r, M->q, M->dpart, g1);
+  hipLaunchKernelGGL(k_mp_apply, dim3(g1), dim3(256), 0, s, n, M->dpart, g1, xn, xp, M->ext,
+                     M->dv, M->scal, x0);
   OFH_HIP(hipGetLastError());
   return OFH_OK;
 }
@@ -672,18 +744,19 @@ static int fold(const float* part, int nparts, int width, float* out, hipStream_
 }
 
 // Loss and gradient into M->grads (forward must have run with training = true).
-static int backward(ofh_mpnn M, const double* xn, const double* xs, const double* x0,
+static int backward(ofh_mpnn M, const double* xn, const double* xp, const double* xs,
                     hipStream_t s) {
   ofh_csr A = M->A;
   const int n = M->n, n_pad = M->n_pad, L = M->L;
   const int g1 = (n + 255) / 256;
   const int gg = gemm_grid(n_pad);
-  hipLaunchKernelGGL(k_mp_loss, dim3(g1), dim3(256), 0, s, n, A->row_ptr, A->col, A->val, x0, xn,
-                     xs, M->q, M->dpart, g1);
+  // g = A (x* - x_b) goes to r (free after the forward pass)
+  hipLaunchKernelGGL(k_mp_loss, dim3(g1), dim3(256), 0, s, n, A->row_ptr, A->col, A->val, xn, xp,
+                     M->ext, xs, M->dv, M->r, M->dpart, g1);
   float* dH = M->dH0;
   float* dHn = M->dH1;
   hipLaunchKernelGGL(k_mp_dec_bwd, dim3(kGradGrid), dim3(256), 0, s, n, n_pad, Hl(M, L), M->params,
-                     off_dec(L), M->q, M->dpart, g1, M->scal, dH, M->part);
+                     off_dec(L), M->q, M->r, M->dpart, g1, M->scal, dH, M->part);
   fold(M->part, kGradGrid, kH + 1, M->grads + off_dec(L), s);
   for (int l = L - 1; l >= 0; --l) {
     // T = (dH [U > 0]) Wu^T  (n x 128)
@@ -779,10 +852,12 @@ using namespace ofh::mp;
 
 extern "C" {
 
-int ofh_mpnn_create(ofh_csr A, int layers, int hidden, uint64_t seed, ofh_mpnn* out) {
+int ofh_mpnn_create(ofh_csr A, int layers, int hidden, int extrapolate, uint64_t seed,
+                    ofh_mpnn* out) {
   OFH_CHECK(A && out, OFH_E_INVALID, "ofh_mpnn_create: bad argument");
   OFH_CHECK(hidden == kH, OFH_E_UNSUPPORTED, "ofh_mpnn_create: hidden width must be 64");
   OFH_CHECK(layers >= 1 && layers <= kMaxLayers, OFH_E_INVALID, "ofh_mpnn_create: layers in 1..8");
+  OFH_CHECK(extrapolate == 0 || extrapolate == 1, OFH_E_INVALID, "ofh_mpnn_create: extrapolate is 0 or 1");
   OFH_CHECK(A->n > 0 && A->n <= kMaxRows, OFH_E_INVALID, "ofh_mpnn_create: matrix size");
   const int n = A->n;
   // host copy of the matrix: symmetry, diagonal, edge weights
@@ -818,6 +893,7 @@ int ofh_mpnn_create(ofh_csr A, int layers, int hidden, uint64_t seed, ofh_mpnn* 
   ofh_mpnn M = new ofh_mpnn_s();
   M->A = A;
   M->L = layers;
+  M->ext = extrapolate;
   M->n = n;
   M->n_pad = (n + 15) / 16 * 16;
   M->np = n_params(layers);
@@ -847,8 +923,9 @@ int ofh_mpnn_create(ofh_csr A, int layers, int hidden, uint64_t seed, ofh_mpnn* 
   A_((void**)&M->r, sizeof(double) * n);
   A_((void**)&M->z, sizeof(double) * n);
   A_((void**)&M->q, sizeof(double) * n);
+  A_((void**)&M->dv, sizeof(double) * n);
   A_((void**)&M->scal, sizeof(double) * 8);
-  A_((void**)&M->dpart, sizeof(double) * 2 * M->nparts);
+  A_((void**)&M->dpart, sizeof(double) * 4 * M->nparts);
   A_((void**)&M->part, sizeof(float) * kGradGrid * (kH2 * kH + kH2));
   if (he == hipSuccess) he = hipHostMalloc((void**)&M->h_pinned, 64, hipHostMallocDefault);
   if (he != hipSuccess) {
@@ -856,8 +933,8 @@ int ofh_mpnn_create(ofh_csr A, int layers, int hidden, uint64_t seed, ofh_mpnn* 
     delete M;
     return fail(OFH_E_NOMEM, std::string("ofh_mpnn_create: ") + hipGetErrorString(he));
   }
-  // initial parameters: Glorot-uniform weights, zero biases, zero decoder
-  // (the untrained network returns the warm start x_n)
+  // initial parameters: Glorot-uniform weights (decoder included), zero
+  // biases; the line search keeps even the untrained guess no worse than x_b
   std::vector<float> p(M->np, 0.0f);
   SplitMix rng{seed};
   auto glorot = [&](int off, int fin, int fout) {
@@ -870,6 +947,7 @@ int ofh_mpnn_create(ofh_csr A, int layers, int hidden, uint64_t seed, ofh_mpnn* 
     for (int c = 0; c < kH; ++c) p[off_we(l) + c] = (float)rng.uniform(-1.0, 1.0);
     glorot(off_Wu(l), kH2, kH);
   }
+  glorot(off_dec(layers), kH, 1);
   hipStream_t s = A->ctx->stream;
   int ok = OFH_OK;
   auto H_ = [&](hipError_t x) {
@@ -972,7 +1050,7 @@ int ofh_mpnn_loss_grad(ofh_mpnn M, const double* d_xn, const double* d_xp, const
   hipStream_t s = M->A->ctx->stream;
   int rc = forward(M, d_xn, d_xp, d_b, d_x0, true, s);
   if (rc) return rc;
-  if ((rc = backward(M, d_xn, d_xstar, d_x0, s))) return rc;
+  if ((rc = backward(M, d_xn, d_xp, d_xstar, s))) return rc;
   OFH_HIP(hipMemcpyAsync(M->h_pinned, M->scal, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
   if (h_grad) OFH_HIP(hipMemcpyAsync(h_grad, M->grads, sizeof(float) * M->np, hipMemcpyDeviceToHost, s));
   OFH_HIP(hipStreamSynchronize(s));
@@ -987,7 +1065,7 @@ int ofh_mpnn_train_step(ofh_mpnn M, const double* d_xn, const double* d_xp, cons
   hipStream_t s = M->A->ctx->stream;
   int rc = forward(M, d_xn, d_xp, d_b, d_x0, true, s);
   if (rc) return rc;
-  if ((rc = backward(M, d_xn, d_xstar, d_x0, s))) return rc;
+  if ((rc = backward(M, d_xn, d_xp, d_xstar, s))) return rc;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   M->adam_t += 1;
   const float bc1 = 1.f - std::pow(b1, (float)M->adam_t), bc2 = 1.f - std::pow(b2, (float)M->adam_t);

@@ -29,15 +29,24 @@ __device__ __forceinline__ int col_at(const ColIndex& ci, int r0, int k) {
   return ci.rel ? r0 + (int)ci.rel[k] : ci.col[k];  // uniform branch, unswitched
 }
 
+// Gather of the multiplied vector: x[c].
+template <typename T>
+struct GatherVec {
+  const T* __restrict__ x;
+  __device__ __forceinline__ T operator()(int c) const { return x[c]; }
+};
+
 // Row sum of row r0+threadIdx.x over the rows [r0,r1) of one workgroup.
 // The sum starts at `init` and adds the row's products left to right.
 // Rows whose run exceeds TILE nonzeros come as a single-row block; that
 // row's sum is returned in thread 0.  Must be called by all kBlock threads.
-template <typename T, int TILE, bool SCALED = false>
-__device__ __forceinline__ T csr_stream_rowsum(int r0, int r1, const int* __restrict__ rp,
-                                               ColIndex ci, const T* __restrict__ val,
-                                               const T* __restrict__ x, T* lds, T init,
-                                               T scale = T(1)) {
+// X is the gather functor of the multiplied vector (a plain vector, or a
+// vector formed on the fly from others).
+template <typename T, int TILE, bool SCALED = false, class X = GatherVec<T>>
+__device__ __forceinline__ T csr_stream_rowsum_g(int r0, int r1, const int* __restrict__ rp,
+                                                 ColIndex ci, const T* __restrict__ val,
+                                                 const X& x, T* lds, T init,
+                                                 T scale = T(1)) {
   const int k0 = rp[r0], k1 = rp[r1];
   const int t = threadIdx.x;
   if (k1 - k0 <= TILE) {
@@ -47,14 +56,14 @@ __device__ __forceinline__ T csr_stream_rowsum(int r0, int r1, const int* __rest
       int c0 = col_at(ci, r0, k), c1 = col_at(ci, r0, k + kBlock),
           c2 = col_at(ci, r0, k + 2 * kBlock), c3 = col_at(ci, r0, k + 3 * kBlock);
       T a0 = val[k], a1 = val[k + kBlock], a2 = val[k + 2 * kBlock], a3 = val[k + 3 * kBlock];
-      T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+      T x0 = x(c0), x1 = x(c1), x2 = x(c2), x3 = x(c3);
       lds[k - k0] = nz_product<T, SCALED>(a0, x0, scale);
       lds[k + kBlock - k0] = nz_product<T, SCALED>(a1, x1, scale);
       lds[k + 2 * kBlock - k0] = nz_product<T, SCALED>(a2, x2, scale);
       lds[k + 3 * kBlock - k0] = nz_product<T, SCALED>(a3, x3, scale);
     }
     for (; k < k1; k += kBlock)
-      lds[k - k0] = nz_product<T, SCALED>(val[k], x[col_at(ci, r0, k)], scale);
+      lds[k - k0] = nz_product<T, SCALED>(val[k], x(col_at(ci, r0, k)), scale);
     __syncthreads();
     T s = init;
     const int r = r0 + t;
@@ -70,12 +79,21 @@ __device__ __forceinline__ T csr_stream_rowsum(int r0, int r1, const int* __rest
     const int c1 = min(c0 + TILE, k1);
     __syncthreads();
     for (int k = c0 + t; k < c1; k += kBlock)
-      lds[k - c0] = nz_product<T, SCALED>(val[k], x[col_at(ci, r0, k)], scale);
+      lds[k - c0] = nz_product<T, SCALED>(val[k], x(col_at(ci, r0, k)), scale);
     __syncthreads();
     if (t == 0)
       for (int j = 0; j < c1 - c0; ++j) s += lds[j];
   }
   return s;
+}
+
+template <typename T, int TILE, bool SCALED = false>
+__device__ __forceinline__ T csr_stream_rowsum(int r0, int r1, const int* __restrict__ rp,
+                                               ColIndex ci, const T* __restrict__ val,
+                                               const T* __restrict__ x, T* lds, T init,
+                                               T scale = T(1)) {
+  return csr_stream_rowsum_g<T, TILE, SCALED>(r0, r1, rp, ci, val, GatherVec<T>{x}, lds, init,
+                                              scale);
 }
 
 }  // namespace ofh

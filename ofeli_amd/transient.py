@@ -39,10 +39,23 @@ class HeatBE:
                      b=self.b)
 
     def start(self, guess=None):
+        """Start vector of the next solve in x0: "warm" (u^n), "extrapolate"
+        (2 u^n - u^{n-1}), a ProjectionGuess (span of previous solutions), a
+        LearnedGuess, or (ProjectionGuess, LearnedGuess): the network
+        corrects the projection (its base x_b is the projected vector, its
+        "previous" iterate u^n)."""
         if guess is None or guess == "warm":
             self.x0.copy_(self.x)
         elif guess == "extrapolate":
             self.x0.copy_(self.x).mul_(2.0).sub_(self.xp)
+        elif isinstance(guess, api.ProjectionGuess):
+            guess.guess(self.b, self.x0)
+        elif isinstance(guess, tuple):
+            proj, net = guess
+            if not hasattr(self, "xg"):
+                self.xg = self.x0.clone()
+            proj.guess(self.b, self.xg)
+            net.guess(self.xg, self.x, self.b, self.x0)
         else:
             guess.guess(self.x, self.xp, self.b, self.x0)
 
@@ -50,17 +63,20 @@ class HeatBE:
         fn = api.cg_mixed if mixed else api.cg
         return fn(self.A, self.b, self.x0, prec=prec, max_it=max_it, toler=toler)
 
-    def advance(self):
+    def advance(self, guess=None):
         self.xp.copy_(self.x)
         self.x.copy_(self.x0)
         self.P.insert_bc(self.x, None, self.u)
         self.step_count += 1
+        proj = guess[0] if isinstance(guess, tuple) else guess
+        if isinstance(proj, api.ProjectionGuess):
+            proj.update(self.x)
 
     def step(self, f, guess=None, mixed=False, toler=1e-8):
         self.rhs(f)
         self.start(guess)
         info = self.solve(mixed=mixed, toler=toler)
-        self.advance()
+        self.advance(guess)
         return info
 
     def state(self):

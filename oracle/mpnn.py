@@ -12,16 +12,22 @@ rounded to bfloat16 (round to nearest even) exactly where the GPU feeds its
 matrix cores, so the GPU result is reproduced up to FP32 accumulation order.
 
 Definition (n rows, A symmetric CSR with 0-based columns):
-  r  = b - A x_n,  z = D^-1 r,  s = rms(z) (1 if 0),  z2 = D^-1 (r - A z)
+  x_b = x_n + ext (x_n - x_p)      (ext = 0: warm base, 1: linear extrapolation)
+  r  = b - A x_b,  z = D^-1 r,  s = rms(z) (1 if 0),  z2 = D^-1 (r - A z)
   F  = [z/s, z2/s, (x_n - x_p)/s, ln a_ii - mean(ln a_ii), 0, 0, 0, 0]
   H  = relu(F W_enc + b_enc)
   per layer:  Z = H W_ab                    (64 -> 128)
               M_i = sum_{j in N(i), j != i} relu(Z_i[:64] + Z_j[64:] + e_ij w_e + b_m)
               U = relu([H, M] W_u + b_u)    (128 -> 64);  H = H + U
-  delta = H w_dec + b_dec,   x0 = x_n + s * delta
-with e_ij = a_ij / sqrt(a_ii a_jj).  Training loss (x* = the solver's
-solution): L = (x0-x*)^T A (x0-x*) / (x_n-x*)^T A (x_n-x*), the A-norm error
-of the guess relative to the warm start's; dL/dx0 = 2 A (x0 - x*) / E0.
+  d = s (H w_dec + b_dec)          (the learned search direction)
+  x0 = x_b + alpha d,  alpha = d^T r / d^T A d  (0 if d^T A d <= 0)
+with e_ij = a_ij / sqrt(a_ii a_jj).  alpha is the exact line search in the
+A-norm, so x0 is never worse than x_b.  Training loss (x* = the solver's
+solution, e_b = x* - x_b, g = A e_b, E0 = e_b^T g):
+  L = 1 - (d^T g)^2 / (d^T A d E0),
+the A-norm error left after the line search relative to the base's (one
+minus the squared A-cosine between d and e_b); it is invariant to the scale
+of d.  dL/dd = -2 (d^T g) / (d^T A d E0) (g - (d^T g / d^T A d) A d).
 """
 import numpy as np
 import scipy.sparse as sp
@@ -109,8 +115,12 @@ class Graph:
                                 shape=(self.n, len(self.ej)))
 
 
-def features(g, x_n, x_p, b):
-    r = b - g.A @ x_n
+def base(x_n, x_p, ext=0):
+    return x_n + float(ext) * (x_n - x_p)
+
+
+def features(g, x_n, x_p, b, ext=0):
+    r = b - g.A @ base(x_n, x_p, ext)
     z = g.dinv * r
     s = float(np.sqrt(np.dot(z, z) / g.n))
     if not s > 0:
@@ -124,11 +134,11 @@ def features(g, x_n, x_p, b):
     return F.astype(np.float32).astype(np.float64), s
 
 
-def forward(g, theta, layers, x_n, x_p, b, bf=False):
+def forward(g, theta, layers, x_n, x_p, b, bf=False, ext=0):
     """Returns x0 and the cache of the backward pass."""
     P = unpack(theta, layers)
     rnd = bf16 if bf else (lambda a: a)
-    F, s = features(g, x_n, x_p, b)
+    F, s = features(g, x_n, x_p, b, ext)
     H = np.maximum(F @ P["enc_W"] + P["enc_b"], 0.0)
     cache = {"F": F, "s": s, "H": [H], "Z": [], "M": [], "U": [], "P": P}
     for l in range(layers):
@@ -141,27 +151,38 @@ def forward(g, theta, layers, x_n, x_p, b, bf=False):
         cache["M"].append(M)
         cache["U"].append(U)
         cache["H"].append(H)
-    delta = H @ P["dec_w"] + P["dec_b"]
-    return x_n + s * delta, cache
+    d = s * (H @ P["dec_w"] + P["dec_b"])
+    xb = base(x_n, x_p, ext)
+    r = b - g.A @ xb
+    Ad = g.A @ d
+    dAd = float(d @ Ad)
+    alpha = float(d @ r) / dAd if dAd > 0 else 0.0
+    cache.update(d=d, Ad=Ad, dAd=dAd, xb=xb)
+    return xb + alpha * d, cache
 
 
-def loss(g, x0, x_n, x_star):
-    e = x0 - x_star
-    en = x_n - x_star
-    q = g.A @ e
-    E0 = float(en @ (g.A @ en))
-    if not E0 > 0:
-        return 0.0, np.zeros_like(x0)
-    return float(e @ q) / E0, 2.0 * q / E0
+def loss(g, cache, x_star):
+    """L and dL/dd of the line-searched direction (see the module text)."""
+    d, Ad, dAd = cache["d"], cache["Ad"], cache["dAd"]
+    eb = x_star - cache["xb"]
+    gv = g.A @ eb
+    E0 = float(eb @ gv)
+    if not E0 > 0:  # the base is already the solution
+        return 0.0, np.zeros_like(d)
+    if not dAd > 0:  # no direction: nothing removed
+        return 1.0, np.zeros_like(d)
+    dg = float(d @ gv)
+    L = 1.0 - dg * dg / (dAd * E0)
+    return L, -2.0 * dg / (dAd * E0) * (gv - (dg / dAd) * Ad)
 
 
-def backward(g, cache, layers, gx, bf=False):
-    """Gradient of the flat parameter vector given dL/dx0 = gx."""
+def backward(g, cache, layers, gd, bf=False):
+    """Gradient of the flat parameter vector given dL/dd = gd."""
     P = cache["P"]
     rnd = bf16 if bf else (lambda a: a)
     s = cache["s"]
     G = {}
-    dd = s * gx
+    dd = s * gd
     Hl = cache["H"][-1]
     G["dec_w"] = Hl.T @ dd
     G["dec_b"] = dd.sum()
@@ -188,10 +209,10 @@ def backward(g, cache, layers, gx, bf=False):
     return pack(G, layers)
 
 
-def loss_and_grad(g, theta, layers, x_n, x_p, b, x_star, bf=False):
-    x0, cache = forward(g, theta, layers, x_n, x_p, b, bf)
-    L, gx = loss(g, x0, x_n, x_star)
-    return L, backward(g, cache, layers, gx, bf), x0
+def loss_and_grad(g, theta, layers, x_n, x_p, b, x_star, bf=False, ext=0):
+    x0, cache = forward(g, theta, layers, x_n, x_p, b, bf, ext)
+    L, gd = loss(g, cache, x_star)
+    return L, backward(g, cache, layers, gd, bf), x0
 
 
 def init_params(layers, seed=0, h=HID):

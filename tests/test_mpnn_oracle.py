@@ -57,10 +57,26 @@ def test_untrained_decoder_returns_warm_start():
     P = N.unpack(th, 2)
     P["dec_w"][:] = 0.0
     th = N.pack(dict(P, dec_b=0.0), 2)
-    x0, _ = N.forward(g, th, 2, x_n, x_p, b)
+    x0, cache = N.forward(g, th, 2, x_n, x_p, b)
     assert np.array_equal(x0, x_n)
-    L, _ = N.loss(g, x0, x_n, xs)
-    assert L == 1.0
+    L, gd = N.loss(g, cache, xs)
+    assert L == 1.0 and not gd.any()
+
+
+@pytest.mark.parametrize("ext", [0, 1])
+def test_loss_is_the_a_norm_error_ratio_after_the_line_search(ext):
+    """With the exact solution as x*, L equals |x0 - x*|_A^2 / |x_b - x*|_A^2,
+    and the line search never makes the base worse."""
+    g, b, xs, x_n, x_p = small_system()
+    A = g.A.toarray()
+    for seed in range(3):
+        th = N.init_params(2, seed=seed)
+        x0, cache = N.forward(g, th, 2, x_n, x_p, b, ext=ext)
+        L, _ = N.loss(g, cache, xs)
+        xb = N.base(x_n, x_p, ext)
+        e0, eb = x0 - xs, xb - xs
+        ratio = (e0 @ A @ e0) / (eb @ A @ eb)
+        assert abs(L - ratio) <= 1e-9 and 0 <= L <= 1
 
 
 def test_scale_equivariance():
@@ -84,16 +100,16 @@ def test_features_definition():
     assert abs(F[:, 3].mean()) < 1e-6 and not F[:, 4:].any()
 
 
-@pytest.mark.parametrize("layers", [1, 2])
-def test_gradient_matches_finite_differences(layers):
+@pytest.mark.parametrize("layers,ext", [(1, 0), (2, 0), (2, 1)])
+def test_gradient_matches_finite_differences(layers, ext):
     g, b, xs, x_n, x_p = small_system()
     th = N.init_params(layers, seed=5)
-    L, grad, _ = N.loss_and_grad(g, th, layers, x_n, x_p, b, xs)
+    L, grad, _ = N.loss_and_grad(g, th, layers, x_n, x_p, b, xs, ext=ext)
     assert 0 < L
 
     def f(t):
-        x0, _ = N.forward(g, t, layers, x_n, x_p, b)
-        return N.loss(g, x0, x_n, xs)[0]
+        _, cache = N.forward(g, t, layers, x_n, x_p, b, ext=ext)
+        return N.loss(g, cache, xs)[0]
 
     rng = np.random.default_rng(0)
     h = 1e-6
