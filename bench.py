@@ -44,7 +44,13 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3))
+    ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3, 4))
+    ap.add_argument("--nx4", type=int, default=2048, help="config 4 cells along the beam")
+    ap.add_argument("--ny4", type=int, default=256, help="config 4 cells across the beam")
+    ap.add_argument("--cases", type=int, default=256, help="config 4 load cases")
+    ap.add_argument("--base-cases", type=int, default=8,
+                    help="config 4 cases solved by the per-case baselines")
+    ap.add_argument("--train-cases", type=int, default=32)
     ap.add_argument("--n3", type=int, default=512, help="config 3 grid (n x n cells)")
     ap.add_argument("--t-steps", type=int, default=2000, help="config 3 time steps")
     ap.add_argument("--train-steps", type=int, default=200, help="config 3 steps used for training")
@@ -61,6 +67,8 @@ def main():
         return bench_batched(args)
     if args.config == 3:
         return bench_transient(args)
+    if args.config == 4:
+        return bench_cantilever(args)
 
     import torch
     import ofeli_amd as oa
@@ -426,8 +434,8 @@ def bench_transient(args):
     hist_p, tr_p = train(net_p, samples_p, "net on projection")
     del samples, samples_p
 
-    def make_proj():
-        P = oa.ProjectionGuess(S.A, m=PM)
+    def make_proj(minres=False):
+        P = oa.ProjectionGuess(S.A, m=PM, minres=minres)
         for x in last:
             P.update(x)
         return P
@@ -450,6 +458,7 @@ def bench_transient(args):
                 ("fp64_extrap", lambda: "extrapolate", False),
                 ("fp64_learned", lambda: net_x, False),
                 ("fp64_proj", make_proj, False),
+                ("fp64_proj_mr", lambda: make_proj(True), False),
                 ("fp64_proj_learned", lambda: (make_proj(), net_p), False),
                 ("mixed_warm", lambda: "warm", True),
                 ("mixed_proj", make_proj, True),
@@ -519,6 +528,195 @@ def bench_transient(args):
         "train_seconds": round(tr_x + tr_p, 2),
         "train_loss_per_epoch": {"net_extrapolation": hist_x, "net_projection": hist_p},
         "start_vector_ms": timings, "phaseA_ms_per_step": round(tA / K * 1e3, 4),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_cantilever(args):
+    """Cantilever (Elas2DQ4 plane stress, 2048 x 256 quads, 1,052,672 free
+    DOFs) under 256 right-edge load cases that combine 8 basis tractions.
+    Per-case FP64 CG from zero (the reference solves each case by itself) is
+    timed on the first cases; the whole set is then solved with the start
+    vector projected on the previous solutions, FP64 and mixed, and with the
+    learned direction on top (trained on the first cases).  The exact
+    level-scheduled SSOR is timed per application."""
+    import torch
+    import ofeli_amd as oa
+    from ofeli_amd import workloads as W
+
+    t0 = time.time()
+    d = W.config4(args.nx4, args.ny4, ncases=args.cases)
+    log("[bench4] workload %.1fs" % (time.time() - t0))
+    ctx = oa.Context(0)
+    t0 = time.time()
+    P = oa.AssemblyPlan(ctx, d["coord"], d["elem"], d["code"])
+    A = P.A
+    n, nnz = A.n, A.nnz
+    torch.cuda.synchronize()
+    log("[bench4] plan n=%d nnz=%d %.2fs" % (n, nnz, time.time() - t0))
+    E = torch.as_tensor(d["coef"], device="cuda")
+    b0 = torch.zeros(n, dtype=torch.float64, device="cuda")
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    P.build(oa.ELAS2DQ4, coef=E, prm=d["prm"], matrix=True, b=b0)
+    ev0.record()
+    P.build(oa.ELAS2DQ4, coef=E, prm=d["prm"], matrix=True, b=b0)
+    ev1.record()
+    torch.cuda.synchronize()
+    asm_ms = ev0.elapsed_time(ev1)
+    B = torch.zeros((len(d["basis"]), n), dtype=torch.float64, device="cuda")
+    for k, tr in enumerate(d["basis"]):
+        P.traction(d["sides"], torch.as_tensor(tr, device="cuda"), B[k])
+    Bc = torch.as_tensor(d["alpha"], device="cuda") @ B  # every case's RHS, resident in HBM
+    torch.cuda.synchronize()
+    ncases = Bc.shape[0]
+    tol = args.toler
+    kw = dict(max_it=200000, toler=tol)
+
+    # exact SGS (SSOR_PREC, omega = 1): per application and per CG iteration
+    r = Bc[0].clone()
+    z = torch.zeros_like(r)
+    oa.ssor_solve(A, r, z)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(3):
+        oa.ssor_solve(A, r, z)
+    torch.cuda.synchronize()
+    ssor_ms = (time.perf_counter() - t) / 3 * 1e3
+    xs_ = torch.zeros_like(r)
+    t = time.perf_counter()
+    inf = oa.cg(A, Bc[0], xs_, prec=oa.SSOR_PREC, max_it=20, toler=1e-30)
+    torch.cuda.synchronize()
+    cg_ssor_it_ms = (time.perf_counter() - t) * 1e3 / max(inf.iterations, 1)
+    log("[bench4] SSOR application %.2f ms, CG+SSOR %.2f ms/iteration" % (ssor_ms, cg_ssor_it_ms))
+
+    def per_case(fn, cases):
+        its, ms, xs = [], [], []
+        for c in cases:
+            x = torch.zeros(n, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            info = fn(A, Bc[c], x, prec=oa.DIAG_PREC, **kw)
+            torch.cuda.synchronize()
+            ms.append((time.perf_counter() - t) * 1e3)
+            its.append(info.iterations)
+            xs.append(x)
+            log("[bench4]   case %d: %d it, %.1f ms%s" % (c, its[-1], ms[-1],
+                                                       " (FP64 fallback)" if info.fallback else ""))
+        return np.array(ms), np.array(its), xs
+
+    nb = min(args.base_cases, ncases)
+    ms64, it64, x64 = per_case(oa.cg, range(nb))
+    log("[bench4] fp64 zero start: %.1f ms/case, %.0f it" % (ms64.mean(), it64.mean()))
+    ms32, it32, x32 = per_case(oa.cg_mixed, range(nb))
+    log("[bench4] mixed zero start: %.1f ms/case, %.0f it" % (ms32.mean(), it32.mean()))
+
+    def all_cases(mixed, net=None, collect=0, tag=""):
+        proj = oa.ProjectionGuess(A, m=16, minres=True)
+        x0 = torch.empty(n, dtype=torch.float64, device="cuda")
+        xg = torch.empty_like(x0)
+        its, samples, sols = [], [], []
+        fn = oa.cg_mixed if mixed else oa.cg
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for c in range(ncases):
+            b = Bc[c]
+            if net is None:
+                proj.guess(b, x0)
+            else:
+                proj.guess(b, xg)
+                net.guess(xg, xg, b, x0)
+            if c < collect:
+                g_ = x0.clone()
+            # a start that passes toler ends the case; any other case is
+            # solved to toler/100 and joins the projection basis
+            info = oa.solve_with_projection(A, b, x0, proj, fn=fn, toler=tol, tighten=1e-2,
+                                            max_it=kw["max_it"], guess=False)
+            its.append(info.iterations if info.nb_it != 0 else 0)
+            if c < collect:
+                samples.append((g_, g_, b, x0.clone()))
+            if c < nb:
+                sols.append(x0.clone())
+            if c % 32 == 31 or c < 2:
+                log("[bench4]   %s case %d: %d it (%.1f s)" % (tag, c, its[-1], time.perf_counter() - t))
+        torch.cuda.synchronize()
+        dt_ = time.perf_counter() - t
+        proj.close()
+        return dt_ * 1e3 / ncases, np.array(its), samples, sols
+
+    res = {}
+    ms, its, samples, sols = all_cases(False, collect=args.train_cases, tag="fp64 proj")
+    res["fp64_proj"] = (ms, its, sols)
+    log("[bench4] fp64 projection: %.2f ms/case, it first %s, rest mean %.1f"
+        % (ms, its[:10].tolist(), its[10:].mean()))
+    ms, its, _, sols = all_cases(True, tag="mixed proj")
+    res["mixed_proj"] = (ms, its, sols)
+    log("[bench4] mixed projection: %.2f ms/case, rest mean %.1f" % (ms, its[10:].mean()))
+    net = oa.LearnedGuess(A, layers=4, seed=0)
+    x0buf = torch.empty(n, dtype=torch.float64, device="cuda")
+    rng = np.random.default_rng(0)
+    t = time.perf_counter()
+    hist = []
+    for ep in range(args.epochs):
+        tot = 0.0
+        for i in rng.permutation(len(samples)):
+            tot += net.train_step(*samples[i], x0buf, lr=args.lr)
+        hist.append(round(tot / max(len(samples), 1), 5))
+    train_s = time.perf_counter() - t
+    log("[bench4] training on %d cases: %s (%.1fs)" % (len(samples), hist, train_s))
+    ms, its, _, sols = all_cases(True, net=net, tag="mixed proj+net")
+    res["mixed_proj_learned"] = (ms, its, sols)
+    log("[bench4] mixed projection+learned: %.2f ms/case, rest mean %.1f" % (ms, its[10:].mean()))
+
+    def err(xa, xb):
+        return max(float(torch.linalg.norm(a - b_) / torch.linalg.norm(b_)) for a, b_ in zip(xa, xb))
+
+    errs = {"mixed_zero": err(x32, x64)}
+    for k, v in res.items():
+        errs[k] = err(v[2], x64)
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.lib(fast=True)
+        rp, col = A.pattern()
+        vals = A.values()
+        iters = args.cpu_iters
+        t = time.perf_counter()
+        O.cg(rp.astype(np.int64), col.astype(np.int64) + 1, vals, Bc[0].cpu().numpy(),
+             prec=O.DIAG, max_it=iters, toler=1e-300, fast=True)
+        cpu_s = time.perf_counter() - t
+        per_it = cpu_s / iters
+        cpu = {"value": round(per_it * it64.mean() * 1e3, 1),
+               "unit": "ms per FP64 CG+DIAG solve from zero (extrapolated)", "cores": 1,
+               "kind": "port",
+               "sample": "%d CG+DIAG iterations of case 0 by the oracle -Ofast on 1 host core, "
+                         "%.1f ms/iteration x %.0f GPU iterations" % (iters, per_it * 1e3, it64.mean())}
+        log("[bench4] cpu %.1f ms/it" % (per_it * 1e3))
+
+    head = "mixed_proj_learned"
+    base_ms = float(ms64.mean())
+    out = {
+        "metric": METRIC, "value": round(res[head][0], 3),
+        "unit": "ms per load case (projected + learned start, mixed CG+DIAG to rtol %g), "
+                "mean over %d cases" % (tol, ncases),
+        "n_gpus": 1, "steps": ncases, "warmup": 0, "ms_per_step": round(res[head][0], 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64+bf16",
+        "data": "synthetic",
+        "config": {"workload": "Elas2DQ4 cantilever %dx%d quads, plane stress, %d load cases"
+                               % (args.nx4, args.ny4, ncases), "rows": n, "nnz": nnz,
+                   "toler": tol, "precond": "DIAG_PREC", "train_cases": args.train_cases},
+        "baseline": "fp64 CG+DIAG from zero per case (first %d cases)" % nb,
+        "fp64_zero_ms_per_case": round(base_ms, 2), "fp64_zero_iterations": float(it64.mean()),
+        "mixed_zero_ms_per_case": round(float(ms32.mean()), 2),
+        "mixed_zero_iterations": float(it32.mean()),
+        "variants_ms_per_case": {k: round(v[0], 3) for k, v in res.items()},
+        "variants_iterations_first10": {k: v[1][:10].tolist() for k, v in res.items()},
+        "variants_mean_iterations_after10": {k: round(float(v[1][10:].mean()), 2) for k, v in res.items()},
+        "speedup_vs_fp64_zero": {k: round(base_ms / v[0], 2) for k, v in res.items()},
+        "max_rel_err_vs_fp64_zero_first_cases": errs,
+        "ssor_ms_per_application": round(ssor_ms, 2), "cg_ssor_ms_per_iteration": round(cg_ssor_it_ms, 2),
+        "assembly_ms": round(asm_ms, 3), "train_seconds": round(train_s, 2), "train_loss": hist,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -247,20 +247,22 @@ int ofh_asm_insert_bc(ofh_asm P, const double* d_v, const double* d_bc, double* 
 /* --------------------------------- start vectors from previous solutions */
 /* No reference routine (the reference starts from zero or from the previous
  * solution, TimeStepping.cpp:207).  For solves A x_k = b_k with one SPD
- * matrix, keeps an A-orthonormal basis V (V^T A V = I) of the last
- * max_vectors new solution directions; the start vector x0 = V V^T b is the
- * A-norm-closest vector of span(V) to A^-1 b (Fischer 1998).  After each
- * solve, ofh_proj_update keeps the solution in a ring of the last
- * max_vectors solutions; when V is full it is rebuilt from the last
- * max_vectors/2 of them, otherwise the solution is A-orthogonalised against
- * V (classical Gram-Schmidt, two passes), A-normalised and appended (a
- * direction with A-norm below 1e-12 of the solution's is stored as zero).
- * The span always holds the newest solution, so the guess is never worse
- * than the warm start in the A-norm.  An append costs one SpMV and about
- * 6 x max_vectors vector passes; a guess 1 + 2 x max_vectors passes.
+ * matrix, keeps a basis V of the span of recent solutions and returns the
+ * start vector of that span closest to A^-1 b (Fischer 1998):
+ *   mode 0: V A-orthonormal, x0 = V V^T b minimises the A-norm error;
+ *   mode 1: AV orthonormal, x0 = V (AV)^T b minimises ||b - A x0||_2, the
+ *           quantity of the solvers' stop test.
+ * ofh_proj_update keeps the solution in a ring of the last max_vectors
+ * solutions; when V is full it is rebuilt from the last max_vectors/2 of
+ * them, otherwise the solution is orthogonalised against V (classical
+ * Gram-Schmidt, two passes), normalised and appended.  A direction below
+ * 1e-6 of the solution's norm is stored as zero (it only carries residual
+ * noise).  The span always holds the newest solution, so the guess is never
+ * worse than the warm start in the mode's norm.  An append costs one SpMV and
+ * about 6 x max_vectors vector passes; a guess 1 + 2 x max_vectors passes.
  * max_vectors: 8, 16 or 32. */
 typedef struct ofh_proj_s* ofh_proj;
-int ofh_proj_create(ofh_csr A, int max_vectors, ofh_proj* out);
+int ofh_proj_create(ofh_csr A, int max_vectors, int mode, ofh_proj* out);
 int ofh_proj_destroy(ofh_proj P);
 int ofh_proj_reset(ofh_proj P);
 int ofh_proj_size(ofh_proj P, int* count);
@@ -270,8 +272,9 @@ int ofh_proj_update(ofh_proj P, const double* d_x);
 /* ------------------------------------------ learned initial guess (device) */
 /* No reference routine: the reference starts a solve from zero (Equa::run,
  * Vect_impl.h:547) or from the previous step's solution (TimeStepping.cpp:207).
- * For a sequence of solves A x_{k+1} = b_{k+1} with A symmetric, positive
- * diagonal, a message-passing network on the graph of A predicts a search
+ * For a sequence of solves A x_{k+1} = b_{k+1} with A symmetric (pattern
+ * exactly, values to 1e-12 relative), positive diagonal, a message-passing
+ * network on the graph of A predicts a search
  * direction d from the base x_b and the new b; the start vector is the base
  * plus the exact A-norm line-search step along d, so it is never worse than
  * the base:
@@ -281,7 +284,7 @@ int ofh_proj_update(ofh_proj P, const double* d_x);
  *   H = relu(F W_enc + b_enc)                                    (8 -> 64)
  *   per layer: Z = H W_ab (64 -> 128, BF16 matrix cores),
  *              M_i = sum_{j != i} relu(Z_i[:64] + Z_j[64:] + e_ij w_e + b_m),
- *              e_ij = a_ij / sqrt(a_ii a_jj),
+ *              e_ij = (a_ij + a_ji) / (2 sqrt(a_ii a_jj)),
  *              H += relu([H, M] W_u + b_u) (128 -> 64, BF16 matrix cores)
  *   d = s (H w_dec + b_dec),  x0 = x_b + alpha d,  alpha = d.r / d.Ad.
  * Training minimises L = 1 - (d.g)^2 / (d.Ad E0) with g = A (x* - x_b),

@@ -62,5 +62,5 @@ def check(rc):
 
 
 from .api import (Context, CsrMatrix, AssemblyPlan, Batch, LearnedGuess, ProjectionGuess,  # noqa: E402
-                  concat_meshes, cg, bicgstab, gmres, linear_solve,  # noqa: E402
+                  concat_meshes, solve_with_projection, cg, bicgstab, gmres, linear_solve,  # noqa: E402
                   cg_mixed, spmv, spmv_add, spmv_f32, prec_apply, ssor_solve, dot, norm2)

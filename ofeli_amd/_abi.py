@@ -65,7 +65,7 @@ SIGNATURES = {
     "ofh_asm_build": (i32, [vp, i32, vp, vp, vp, vp, f64p, i32, i32, vp]),
     "ofh_asm_traction": (i32, [vp, i32, i32p, vp, vp]),
     "ofh_asm_insert_bc": (i32, [vp, vp, vp, vp]),
-    "ofh_proj_create": (i32, [vp, i32, C.POINTER(vp)]),
+    "ofh_proj_create": (i32, [vp, i32, i32, C.POINTER(vp)]),
     "ofh_proj_destroy": (i32, [vp]),
     "ofh_proj_reset": (i32, [vp]),
     "ofh_proj_size": (i32, [vp, C.POINTER(i32)]),

@@ -373,13 +373,14 @@ class Batch:
 
 
 class ProjectionGuess:
-    """Start vectors from the span of previous solutions (ofh_proj_*):
-    x0 = V V^T b over an A-orthonormal basis of the last m directions."""
+    """Start vectors from the span of previous solutions (ofh_proj_*): the
+    A-norm-closest vector (minres=False) or the one of least residual
+    (minres=True)."""
 
-    def __init__(self, A, m=16):
+    def __init__(self, A, m=16, minres=False):
         self.A = A
         h = C.c_void_p()
-        check(lib().ofh_proj_create(A.h, int(m), C.byref(h)))
+        check(lib().ofh_proj_create(A.h, int(m), int(bool(minres)), C.byref(h)))
         self.h = h
         self.m = m
         A.ctx._children.add(self)
@@ -408,6 +409,26 @@ class ProjectionGuess:
             self.close()
         except Exception:
             pass
+
+
+def solve_with_projection(A, b, x, proj, fn=None, prec=DIAG_PREC, toler=1e-8, tighten=1e-2,
+                          max_it=200000, guess=True):
+    """One solve of a sequence sharing A: start from the projection of b on
+    the previous solutions (x holds it on return of proj.guess); if that start
+    already meets toler the solve ends at once; otherwise the system is solved
+    to toler*tighten and the solution joins the projection basis (stored
+    solutions carry residuals far below toler, so later combinations of them
+    pass the test without iterations).  Returns the SolveInfo of the last
+    solver call."""
+    fn = fn or cg
+    if guess:
+        proj.guess(b, x)
+    info = fn(A, b, x, prec=prec, max_it=0, toler=toler)
+    if info.nb_it == 0:
+        return info
+    info = fn(A, b, x, prec=prec, max_it=max_it, toler=toler * tighten)
+    proj.update(x)
+    return info
 
 
 class LearnedGuess:

@@ -11,7 +11,11 @@
 // A replacement happens whenever the recursive residual has dropped by
 // kDelta since the last one, or claims convergence; the solve stops only when
 // the true FP64 residual ||b - A x64|| / ||b|| <= toler.  If the true residual
-// stops improving, the FP64 CG finishes from the current x64.
+// stops halving over three replacements, or twice in a row a cycle is slow
+// (its true-residual reduction per inner iteration is below a fifth of the
+// best cycle's, or the recursive residual needed over 3x the first cycle's
+// length: FP32 no longer resolves the slow modes and the host forced the
+// replacement), the FP64 CG finishes from the current x64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -270,7 +274,8 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   KrylovState h{};
   if ((rc = read_state(ctx, &h))) return rc;
   double last_true = h.res;
-  int stagnant = 0, fallback = 0, it_prev = 0;
+  int stagnant = 0, slow = 0, fallback = 0, it_prev = 0, it_repl = 0, first_cycle = 0;
+  double best_rate = 0.0;
   double res_prev = h.res;
   while (!h.done) {
     int k = h.it;
@@ -289,22 +294,39 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     }
     OFH_HIP(hipGetLastError());
     if ((rc = read_state(ctx, &h))) return rc;
-    if (!h.pause) continue;
-    // residual replacement in FP64 after h.it inner iterations
+    if (h.done) break;
+    // A cycle far longer than the first one means FP32 can no longer lower
+    // the recursive residual: replace it now (the direction p_{k+1} and
+    // rho_k are already in place, so the replacement acts as iteration k+1).
+    const int cyc_now = h.it - it_repl;
+    const int cap = first_cycle > 0 ? std::max(200, 3 * first_cycle) : 2000;
+    const bool forced = !h.pause && cyc_now >= cap;
+    if (!h.pause && !forced) continue;
+    if (h.pause && first_cycle == 0) first_cycle = std::max(1, cyc_now);
+    const int kr = forced ? h.it + 1 : h.it;
+    // residual replacement in FP64 after kr inner iterations
     hipLaunchKernelGGL(k_mx_accum, dim3(ge), dim3(256), 0, s, n, x, x32, st);
     hipLaunchKernelGGL(k_mx_resid<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
                        A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r32, x32, A->dinv32, st, part);
-    hipLaunchKernelGGL((k_mx_dir<PREC, 2>), dim3(g), dim3(kBlock), 0, s, n, h.it, nb, r32,
+    hipLaunchKernelGGL((k_mx_dir<PREC, 2>), dim3(g), dim3(kBlock), 0, s, n, kr, nb, r32,
                        A->dinv32, p32, st, part);
     OFH_HIP(hipGetLastError());
     if ((rc = read_state(ctx, &h))) return rc;
     it_prev = h.it;
     res_prev = h.res;
     if (h.done) break;
-    // stagnation: the true residual must keep falling
+    // stagnation: the true residual must keep falling ...
     stagnant = (h.res > 0.5 * last_true) ? stagnant + 1 : 0;
+    // ... at a rate per inner iteration that does not collapse: when FP32
+    // can no longer resolve the slow modes (very ill-conditioned systems)
+    // the cycles grow long while the true residual barely moves
+    const int cyc = std::max(1, h.it - it_repl);
+    const double rate = (h.res > 0 && last_true > 0) ? std::log(last_true / h.res) / cyc : 0.0;
+    best_rate = std::max(best_rate, rate);
+    slow = (forced || rate < 0.2 * best_rate) ? slow + 1 : 0;
+    it_repl = h.it;
     last_true = std::min(last_true, h.res);
-    if (stagnant >= 3) {
+    if (stagnant >= 3 || slow >= 2) {
       fallback = 1;
       break;
     }

@@ -370,7 +370,7 @@ __global__ __launch_bounds__(256) void k_mp_edge_fwd(int n, int n_pad, const int
 
 // Backward of the aggregation.  dM = T[:, 64:].  With s_ij = dM_i [pre_ij > 0]:
 // dZ_v[c] = sum_u s_vu,  dZ_v[64+c] = sum_u s_uv (pre_uv recomputed from row v,
-// e_uv = e_vu since A is symmetric), and per-workgroup partials of
+// e_uv = e_vu by construction), and per-workgroup partials of
 // sum s_ij e_ij (we) and sum s_ij (bm).
 __global__ __launch_bounds__(256) void k_mp_edge_bwd(int n, int n_pad, const int* __restrict__ rp,
                                                      const int* __restrict__ col,
@@ -873,17 +873,20 @@ int ofh_mpnn_create(ofh_csr A, int layers, int hidden, int extrapolate, uint64_t
   for (int i = 0; i < n; ++i)
     OFH_CHECK(diag[i] > 0.0, OFH_E_INVALID,
               "ofh_mpnn_create: the diagonal must be positive (row " + std::to_string(i + 1) + ")");
+  // symmetric pattern, values symmetric up to rounding (1e-12 relative);
+  // the edge weights use the symmetric part so that e_ij = e_ji exactly
+  std::vector<float> e(A->nnz), ld(n);
   for (int i = 0; i < n; ++i)
     for (int k = rp[i]; k < rp[i + 1]; ++k) {
       const int j = col[k];
       const int* p = std::lower_bound(col.data() + rp[j], col.data() + rp[j + 1], i);
-      OFH_CHECK(p != col.data() + rp[j + 1] && *p == i && val[p - col.data()] == val[k],
-                OFH_E_INVALID, "ofh_mpnn_create: the matrix must be symmetric");
+      OFH_CHECK(p != col.data() + rp[j + 1] && *p == i, OFH_E_INVALID,
+                "ofh_mpnn_create: the matrix pattern must be symmetric");
+      const double aji = val[p - col.data()];
+      OFH_CHECK(std::fabs(aji - val[k]) <= 1e-12 * std::sqrt(diag[i] * diag[j]), OFH_E_INVALID,
+                "ofh_mpnn_create: the matrix must be symmetric");
+      e[k] = (float)((val[k] + aji) / (2.0 * std::sqrt(diag[i] * diag[j])));
     }
-  std::vector<float> e(A->nnz), ld(n);
-  for (int i = 0; i < n; ++i)
-    for (int k = rp[i]; k < rp[i + 1]; ++k)
-      e[k] = (float)(val[k] / std::sqrt(diag[i] * diag[col[k]]));
   double mean = 0.0;
   for (int i = 0; i < n; ++i) mean += std::log(diag[i]);
   mean /= n;

@@ -10,13 +10,18 @@
 // is then the vector of span(V) closest to A^-1 b in the A-norm (Fischer,
 // "Projection techniques for iterative solution of Ax = b with successive
 // right-hand sides", Comput. Methods Appl. Mech. Engrg. 163 (1998) 193-204).
-// After a solve, the solution x enters a ring of the last M solutions.  If
-// V is full it is rebuilt from the last M/2 solutions (newest included);
-// otherwise x is A-orthogonalised against V (classical Gram-Schmidt, two
-// passes), A-normalised and appended; a direction whose A-norm is below
-// 1e-12 of x's is stored as zero (nothing new).  The span therefore always
-// contains the newest solution: the guess is never worse than the warm
-// start in the A-norm.
+// After a solve, the solution x is A-orthogonalised against V (classical
+// Gram-Schmidt, two passes).  A direction whose norm is below 1e-6 of x's
+// is dropped: x is already in the span up to the previous solves' residual
+// noise, and Gram-Schmidt could not keep such a direction orthogonal.
+// Otherwise the direction is A-normalised and appended, and x joins a ring of
+// the last M accepted solutions; when V is full, it is rebuilt from the last
+// M/2 - 1 of them and x.  The span therefore always contains the newest
+// solution (up to that noise): the guess is never worse than the warm start
+// in the A-norm.
+// Minimum-residual mode: the same with the Euclidean inner product of AV
+// (W = AV orthonormal): x0 = V W^T b minimises ||b - A x0||_2 over span(V),
+// the quantity of the solvers' stop test (Fischer's second variant).
 //
 // Device: the basis is M x n FP64 (M in {8, 16, 32}; unused slots are zero
 // vectors).  Every coefficient set comes from per-workgroup partial dot
@@ -32,6 +37,7 @@
 struct ofh_proj_s {
   ofh_csr A = nullptr;
   int M = 0;       // basis slots
+  int mode = 0;    // 0: A-norm (Galerkin), 1: minimum residual
   int count = 0;   // directions stored so far (<= M)
   int head = 0;    // slot of the next direction
   int n = 0;
@@ -45,6 +51,8 @@ struct ofh_proj_s {
   double* Ad = nullptr;    // n
   double* part = nullptr;  // (M + 1) x nparts partial sums
   double* part2 = nullptr; // 2 x nparts
+  int* flag = nullptr;     // device: direction accepted
+  int* h_flag = nullptr;   // pinned host copy
 };
 
 namespace ofh {
@@ -112,10 +120,11 @@ __global__ __launch_bounds__(kBlock) void k_pj_guess(int n, const double* __rest
   }
 }
 
-// One Gram-Schmidt pass: c = AV^T src folded from the partials;
-// d = src - V c, Ad -= AV c.  Partials: pass 1 stores x.(Ax) (the values
-// before the update) in part2[0]; pass 2 stores d.(Ad) in part2[1].
-template <int M, int PASS>
+// One Gram-Schmidt pass: c folded from the partials; d = src - V c,
+// Ad -= AV c.  Partials (squared norms in the mode's inner product): pass 1
+// stores that of x (the values before the update) in part2[0], pass 2 that
+// of d in part2[1]: x.Ax and d.Ad (MR = 0) or Ax.Ax and Ad.Ad (MR = 1).
+template <int M, int PASS, int MR>
 __global__ __launch_bounds__(kBlock) void k_pj_orth(int n, const double* __restrict__ part,
                                                     int nparts, const double* __restrict__ V,
                                                     const double* __restrict__ AV,
@@ -141,20 +150,24 @@ __global__ __launch_bounds__(kBlock) void k_pj_orth(int n, const double* __restr
       const double di = si - sv, adi = ai - sa;
       d[i] = di;
       Ad[i] = adi;
-      v[0] += PASS == 1 ? si * ai : di * adi;
+      if (MR)
+        v[0] += PASS == 1 ? ai * ai : adi * adi;
+      else
+        v[0] += PASS == 1 ? si * ai : di * adi;
     }
   }
   block_sum_k<kBlock, 1>(v, red);
   if (threadIdx.x == 0) part2[(PASS - 1) * nparts + blockIdx.x] = v[0];
 }
 
-// Folds x.Ax and d.Ad; stores d/|d|_A and Ad/|d|_A in the slot, or zeros
-// when |d|_A^2 <= 1e-24 x.Ax.
+// Folds |x|^2 and |d|^2 (mode norm).  A direction with |d|^2 > 1e-12 |x|^2
+// is accepted: d/|d| and Ad/|d| go to the slot (when Vs is not null) and
+// block 0 sets *accepted; otherwise nothing is written but *accepted = 0.
 __global__ __launch_bounds__(kBlock) void k_pj_insert(int n, const double* __restrict__ part2,
                                                       int nparts, const double* __restrict__ d,
                                                       const double* __restrict__ Ad,
                                                       double* __restrict__ Vs,
-                                                      double* __restrict__ AVs) {
+                                                      double* __restrict__ AVs, int* accepted) {
   __shared__ double red[2 * kBlock / kWave];
   double t[2] = {0.0, 0.0};
   for (int p = threadIdx.x; p < nparts; p += kBlock) {
@@ -163,7 +176,10 @@ __global__ __launch_bounds__(kBlock) void k_pj_insert(int n, const double* __res
   }
   block_sum_k<kBlock, 2>(t, red);
   const double xax = t[0], dad = t[1];
-  const double scale = (dad > 1e-24 * xax && dad > 0.0) ? 1.0 / sqrt(dad) : 0.0;
+  const bool ok = dad > 1e-12 * xax && dad > 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *accepted = ok ? 1 : 0;
+  if (!ok || !Vs) return;
+  const double scale = 1.0 / sqrt(dad);
   const int base = blockIdx.x * (kBlock * kVecP) + threadIdx.x;
 #pragma unroll
   for (int j = 0; j < kVecP; ++j) {
@@ -183,30 +199,49 @@ __global__ void k_pj_zero(size_t count, double* p) {
 template <int M>
 static int guess_m(ofh_proj P, const double* b, double* x0, hipStream_t s) {
   const int g = P->nparts;
-  hipLaunchKernelGGL(k_pj_mdot<M>, dim3(g), dim3(kBlock), 0, s, P->n, b, P->V, P->part, g);
+  hipLaunchKernelGGL(k_pj_mdot<M>, dim3(g), dim3(kBlock), 0, s, P->n, b,
+                     P->mode ? P->AV : P->V, P->part, g);
   hipLaunchKernelGGL(k_pj_guess<M>, dim3(g), dim3(kBlock), 0, s, P->n, P->part, g, P->V, x0);
   OFH_HIP(hipGetLastError());
   return OFH_OK;
 }
 
+// Orthogonalises x against the basis and appends it to slot `head` if its
+// new direction is not negligible (dry: only decides).  *accepted on return.
 template <int M>
-static int insert_m(ofh_proj P, const double* x, hipStream_t s) {
+static int insert_m(ofh_proj P, const double* x, hipStream_t s, bool dry, bool* accepted) {
   const int g = P->nparts, n = P->n;
   int rc = launch_spmv64(P->A, x, P->Ad, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_pj_mdot<M>, dim3(g), dim3(kBlock), 0, s, n, x, P->AV, P->part, g);
-  hipLaunchKernelGGL((k_pj_orth<M, 1>), dim3(g), dim3(kBlock), 0, s, n, P->part, g, P->V, P->AV,
-                     x, P->d, P->Ad, P->part2);
-  hipLaunchKernelGGL(k_pj_mdot<M>, dim3(g), dim3(kBlock), 0, s, n, P->d, P->AV, P->part, g);
-  hipLaunchKernelGGL((k_pj_orth<M, 2>), dim3(g), dim3(kBlock), 0, s, n, P->part, g, P->V, P->AV,
-                     P->d, P->d, P->Ad, P->part2);
-  double* Vs = P->V + (size_t)P->head * n;
-  double* AVs = P->AV + (size_t)P->head * n;
+  // coefficients: A-norm (AV)^T x, or Euclidean (AV)^T (A x)
+  hipLaunchKernelGGL(k_pj_mdot<M>, dim3(g), dim3(kBlock), 0, s, n, P->mode ? P->Ad : x, P->AV,
+                     P->part, g);
+  if (P->mode)
+    hipLaunchKernelGGL((k_pj_orth<M, 1, 1>), dim3(g), dim3(kBlock), 0, s, n, P->part, g, P->V,
+                       P->AV, x, P->d, P->Ad, P->part2);
+  else
+    hipLaunchKernelGGL((k_pj_orth<M, 1, 0>), dim3(g), dim3(kBlock), 0, s, n, P->part, g, P->V,
+                       P->AV, x, P->d, P->Ad, P->part2);
+  hipLaunchKernelGGL(k_pj_mdot<M>, dim3(g), dim3(kBlock), 0, s, n, P->mode ? P->Ad : P->d, P->AV,
+                     P->part, g);
+  if (P->mode)
+    hipLaunchKernelGGL((k_pj_orth<M, 2, 1>), dim3(g), dim3(kBlock), 0, s, n, P->part, g, P->V,
+                       P->AV, P->d, P->d, P->Ad, P->part2);
+  else
+    hipLaunchKernelGGL((k_pj_orth<M, 2, 0>), dim3(g), dim3(kBlock), 0, s, n, P->part, g, P->V,
+                       P->AV, P->d, P->d, P->Ad, P->part2);
+  double* Vs = dry ? nullptr : P->V + (size_t)P->head * n;
+  double* AVs = dry ? nullptr : P->AV + (size_t)P->head * n;
   hipLaunchKernelGGL(k_pj_insert, dim3(g), dim3(kBlock), 0, s, n, P->part2, g, P->d, P->Ad, Vs,
-                     AVs);
+                     AVs, P->flag);
   OFH_HIP(hipGetLastError());
-  P->head = P->head + 1;
-  P->count = P->count + 1;
+  OFH_HIP(hipMemcpyAsync(P->h_flag, P->flag, sizeof(int), hipMemcpyDeviceToHost, s));
+  OFH_HIP(hipStreamSynchronize(s));
+  *accepted = *P->h_flag != 0;
+  if (*accepted && !dry) {
+    P->head = P->head + 1;
+    P->count = P->count + 1;
+  }
   return OFH_OK;
 }
 
@@ -228,18 +263,31 @@ static int zero_basis(ofh_proj P, hipStream_t s) {
 template <int M>
 static int update_m(ofh_proj P, const double* x, hipStream_t s) {
   const int n = P->n;
-  double* slot = P->X + (size_t)P->xhead * n;
-  hipLaunchKernelGGL(k_pj_copy, dim3((n + 255) / 256), dim3(256), 0, s, n, x, slot);
-  P->xhead = (P->xhead + 1) % M;
-  P->xcount = std::min(P->xcount + 1, M);
-  if (P->count < M) return insert_m<M>(P, slot, s);
-  // full: rebuild from the last M/2 solutions, oldest first
-  int rc = zero_basis(P, s);
-  if (rc) return rc;
-  const int R = std::min(M / 2, P->xcount);
-  for (int j = R; j >= 1; --j) {
-    const int k = ((P->xhead - j) % M + M) % M;
-    if ((rc = insert_m<M>(P, P->X + (size_t)k * n, s))) return rc;
+  bool acc = false;
+  int rc;
+  if (P->count < M) {
+    if ((rc = insert_m<M>(P, x, s, false, &acc))) return rc;
+  } else {
+    // full: if x brings a new direction, rebuild from the last M/2 - 1
+    // accepted solutions (oldest first) and x
+    if ((rc = insert_m<M>(P, x, s, true, &acc))) return rc;
+    if (acc) {
+      if ((rc = zero_basis(P, s))) return rc;
+      const int R = std::min(M / 2 - 1, P->xcount);
+      for (int j = R; j >= 1; --j) {
+        const int k = ((P->xhead - j) % M + M) % M;
+        bool a2 = false;
+        if ((rc = insert_m<M>(P, P->X + (size_t)k * n, s, false, &a2))) return rc;
+      }
+      if ((rc = insert_m<M>(P, x, s, false, &acc))) return rc;
+    }
+  }
+  if (acc) {  // remember the solution for a later rebuild
+    hipLaunchKernelGGL(k_pj_copy, dim3((n + 255) / 256), dim3(256), 0, s, n, x,
+                       P->X + (size_t)P->xhead * n);
+    OFH_HIP(hipGetLastError());
+    P->xhead = (P->xhead + 1) % M;
+    P->xcount = std::min(P->xcount + 1, M);
   }
   return OFH_OK;
 }
@@ -252,14 +300,16 @@ using namespace ofh::pj;
 
 extern "C" {
 
-int ofh_proj_create(ofh_csr A, int max_vectors, ofh_proj* out) {
+int ofh_proj_create(ofh_csr A, int max_vectors, int mode, ofh_proj* out) {
   OFH_CHECK(A && out, OFH_E_INVALID, "ofh_proj_create: bad argument");
+  OFH_CHECK(mode == 0 || mode == 1, OFH_E_INVALID, "ofh_proj_create: mode is 0 or 1");
   OFH_CHECK(max_vectors == 8 || max_vectors == 16 || max_vectors == 32, OFH_E_INVALID,
             "ofh_proj_create: max_vectors is 8, 16 or 32");
   OFH_CHECK(A->n > 0 && A->n <= kMaxRows, OFH_E_INVALID, "ofh_proj_create: matrix size");
   ofh_proj P = new ofh_proj_s();
   P->A = A;
   P->M = max_vectors;
+  P->mode = mode;
   P->n = A->n;
   P->nparts = (A->n + kBlock * kVecP - 1) / (kBlock * kVecP);
   const size_t n = A->n;
@@ -270,10 +320,13 @@ int ofh_proj_create(ofh_csr A, int max_vectors, ofh_proj* out) {
   if (e == hipSuccess) e = hipMalloc(&P->Ad, sizeof(double) * n);
   if (e == hipSuccess) e = hipMalloc(&P->part, sizeof(double) * (P->M + 1) * P->nparts);
   if (e == hipSuccess) e = hipMalloc(&P->part2, sizeof(double) * 2 * P->nparts);
+  if (e == hipSuccess) e = hipMalloc(&P->flag, sizeof(int));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&P->h_flag, sizeof(int), hipHostMallocDefault);
   if (e != hipSuccess) {
-    void* ptrs[] = {P->V, P->AV, P->X, P->d, P->Ad, P->part, P->part2};
+    void* ptrs[] = {P->V, P->AV, P->X, P->d, P->Ad, P->part, P->part2, P->flag};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
+    if (P->h_flag) (void)hipHostFree(P->h_flag);
     delete P;
     return fail(OFH_E_NOMEM, std::string("ofh_proj_create: ") + hipGetErrorString(e));
   }
@@ -284,9 +337,10 @@ int ofh_proj_create(ofh_csr A, int max_vectors, ofh_proj* out) {
 int ofh_proj_destroy(ofh_proj P) {
   if (!P) return OFH_OK;
   (void)hipDeviceSynchronize();
-  void* ptrs[] = {P->V, P->AV, P->X, P->d, P->Ad, P->part, P->part2};
+  void* ptrs[] = {P->V, P->AV, P->X, P->d, P->Ad, P->part, P->part2, P->flag};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (P->h_flag) (void)hipHostFree(P->h_flag);
   delete P;
   return OFH_OK;
 }

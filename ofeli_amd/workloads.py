@@ -128,3 +128,33 @@ def source_field(xy, path, t):
     d2 = (xy[:, 0] - cx) ** 2 + (xy[:, 1] - cy) ** 2
     e = -d2 / (2.0 * path["sigma"] ** 2)
     return path["amp"] * (e.exp() if hasattr(e, "exp") else np.exp(e))
+
+
+def config4(nx=2048, ny=256, ncases=256, nbasis=8):
+    """Cantilever (Elas2DQ4, plane stress, nu = 0.3) on [0,4] x [0,1], nx x ny
+    quadrilaterals, clamped at x = 0 (both DOFs); E_e lognormal with mean 1
+    and sigma 0.5 from the 64-mode Gaussian field (correlation 0.1) at the
+    element centroids.  Right-edge tractions: 8 basis loads (x and y
+    components of Legendre profiles P0..P3 in y, constant per side) and
+    ncases seeded N(0,1) combinations of them."""
+    from . import api
+    coord, code, elem = api.mesh2d(nx, ny, quad=True, nb_dof=2, box=(0.0, 4.0, 0.0, 1.0),
+                                   codes=(1, 0, 0, 0))
+    code[:, 1] = code[:, 0]  # clamp both DOFs of the left edge
+    rng = SplitMix64(SEED_BASE + 4)
+    g = fourier_field(rng, centroids(coord, elem))
+    E = np.exp(0.5 * g - 0.125)
+    # right-edge sides (node ids 1-based, nodes x-outer / y-inner)
+    first = nx * (ny + 1) + 1
+    sides = np.array([[first + j, first + j + 1] for j in range(ny)], np.int32)
+    ymid = 0.5 * (coord[sides[:, 0] - 1, 1] + coord[sides[:, 1] - 1, 1])
+    t = 2.0 * ymid - 1.0
+    leg = [np.ones_like(t), t, 0.5 * (3 * t ** 2 - 1), 0.5 * (5 * t ** 3 - 3 * t)]
+    basis = []
+    for k in range(nbasis):
+        tr = np.zeros((ny, 2))
+        tr[:, 0 if k < 4 else 1] = leg[k % 4]
+        basis.append(tr)
+    alpha = rng.normal(ncases * nbasis).reshape(ncases, nbasis)
+    return dict(coord=coord, code=code, elem=elem, coef=E, sides=sides, basis=basis,
+                alpha=alpha, prm=(0.3, 1.0))

@@ -21,7 +21,7 @@ Definition (n rows, A symmetric CSR with 0-based columns):
               U = relu([H, M] W_u + b_u)    (128 -> 64);  H = H + U
   d = s (H w_dec + b_dec)          (the learned search direction)
   x0 = x_b + alpha d,  alpha = d^T r / d^T A d  (0 if d^T A d <= 0)
-with e_ij = a_ij / sqrt(a_ii a_jj).  alpha is the exact line search in the
+with e_ij = (a_ij + a_ji) / (2 sqrt(a_ii a_jj)).  alpha is the exact line search in the
 A-norm, so x0 is never worse than x_b.  Training loss (x* = the solver's
 solution, e_b = x* - x_b, g = A e_b, E0 = e_b^T g):
   L = 1 - (d^T g)^2 / (d^T A d E0),
@@ -105,8 +105,12 @@ class Graph:
         self.dinv = 1.0 / diag
         off = self.col != rows
         self.ei, self.ej = rows[off], self.col[off]
-        # e_ij in FP32 as the GPU stores it
-        self.e = (self.val[off] / np.sqrt(diag[self.ei] * diag[self.ej])).astype(np.float32).astype(np.float64)
+        # e_ij of the symmetric part, in FP32 as the GPU stores it
+        vt = self.A.T.tocsr()
+        vt.sort_indices()
+        valt = vt.data  # a_ji in the (sorted, symmetric) pattern of A
+        self.e = ((self.val[off] + valt[off]) / (2.0 * np.sqrt(diag[self.ei] * diag[self.ej]))
+                  ).astype(np.float32).astype(np.float64)
         ld = np.log(diag)
         self.ld = (ld - ld.sum() / self.n).astype(np.float32).astype(np.float64)
         self.S = sp.csr_matrix((np.ones(len(self.ei)), (self.ei, np.arange(len(self.ei)))),

@@ -9,7 +9,8 @@ from tests.gpu_systems import be_system
 pytestmark = pytest.mark.gpu
 
 
-def test_projection_matches_oracle_through_a_rebuild(ctx):
+@pytest.mark.parametrize("minres", [False, True])
+def test_projection_matches_oracle_through_a_rebuild(ctx, minres):
     import torch
     import ofeli_amd as oa
     P, g, b, xs, x_n, x_p = be_system(ctx, 20, seed=11)
@@ -17,8 +18,8 @@ def test_projection_matches_oracle_through_a_rebuild(ctx):
     rp, col = A.pattern()
     Ah = sp.csr_matrix((A.values(), col, rp), shape=(A.n, A.n))
     rng = np.random.default_rng(0)
-    G = oa.ProjectionGuess(A, m=8)
-    O = PJ.Projection(Ah, 8)
+    G = oa.ProjectionGuess(A, m=8, minres=minres)
+    O = PJ.Projection(Ah, 8, minres=minres)
     x0 = torch.empty_like(b)
     counts = []
     for k in range(13):
@@ -38,6 +39,12 @@ def test_projection_matches_oracle_through_a_rebuild(ctx):
         assert G.size() == O.count
         counts.append(O.count)
     assert counts[7] == 8 and counts[8] == 4  # the ninth update rebuilt the basis
+    # a combination of stored solutions is dropped: nothing changes
+    xa = torch.zeros_like(b)
+    G.guess(bk, x0)
+    y = x0.clone()
+    G.update(1.5 * y)
+    assert G.size() == O.count and O.update(1.5 * y.cpu().numpy()) is False
 
 
 def test_projection_start_is_no_worse_than_warm_in_a_norm(ctx):

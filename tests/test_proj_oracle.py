@@ -50,14 +50,14 @@ def test_guess_is_never_worse_than_the_last_solution_in_a_norm():
         x = xs
 
 
-def test_full_basis_is_rebuilt_from_the_last_half_and_repeats_store_zero():
+def test_full_basis_is_rebuilt_and_repeats_are_dropped():
     A, rng = spd(seed=2)
     Ad = A.toarray()
     P = PJ.Projection(A, 8)
     xs = [np.linalg.solve(Ad, rng.standard_normal(A.shape[0])) for _ in range(11)]
     for x in xs:
-        P.update(x)
-    # update 9 found V full and rebuilt it from solutions 6..9; 10, 11 added
+        assert P.update(x)
+    # update 9 found V full and rebuilt it from solutions 6..8 and 9; 10, 11 added
     assert P.count == 6 and P.head == 6
     W = np.array(xs[5:]).T
     b = rng.standard_normal(A.shape[0])
@@ -66,5 +66,26 @@ def test_full_basis_is_rebuilt_from_the_last_half_and_repeats_store_zero():
     G = P.V[:6] @ Ad @ P.V[:6].T
     assert np.allclose(G, np.eye(6), atol=1e-10)
     v_before = P.V.copy()
-    P.update(xs[-1])  # nothing new: a zero direction fills slot 6
-    assert not P.V[6].any() and np.array_equal(P.V[:6], v_before[:6]) and P.count == 7
+    # nothing new: a solution in the span (or a combination) is dropped
+    assert not P.update(xs[-1]) and not P.update(0.3 * xs[-2] - 2.0 * xs[-4])
+    assert np.array_equal(P.V, v_before) and P.count == 6 and len(P.X) == 8
+
+
+def test_minres_guess_is_least_squares_over_stored_solutions():
+    A, rng = spd(seed=3)
+    Ad = A.toarray()
+    P = PJ.Projection(A, 8, minres=True)
+    xs = []
+    for k in range(6):
+        x = np.linalg.solve(Ad, rng.standard_normal(A.shape[0]))
+        P.update(x)
+        xs.append(x)
+    W = np.array(xs).T
+    b = rng.standard_normal(A.shape[0])
+    y, *_ = np.linalg.lstsq(Ad @ W, b, rcond=None)
+    want = W @ y
+    assert np.allclose(P.guess(b), want, atol=1e-10 * np.abs(want).max())
+    AV = P.AV[:6]
+    assert np.allclose(AV @ AV.T, np.eye(6), atol=1e-12)
+    # its residual is no larger than that of the last solution
+    assert np.linalg.norm(b - Ad @ P.guess(b)) <= np.linalg.norm(b - Ad @ xs[-1]) * (1 + 1e-12)
