@@ -386,7 +386,7 @@ def bench_transient(args):
     tol = args.toler
 
     # phase A: FP64 warm-started steps, training samples
-    proj = oa.ProjectionGuess(S.A, m=PM)
+    proj = oa.ProjectionGuess(S.A, m=PM, minres=True)
     xg = torch.empty_like(S.b)
     samples, samples_p, last = [], [], []
     tA = time.perf_counter()
@@ -459,10 +459,10 @@ def bench_transient(args):
                 ("fp64_learned", lambda: net_x, False),
                 ("fp64_proj", make_proj, False),
                 ("fp64_proj_mr", lambda: make_proj(True), False),
-                ("fp64_proj_learned", lambda: (make_proj(), net_p), False),
+                ("fp64_proj_mr_learned", lambda: (make_proj(True), net_p), False),
                 ("mixed_warm", lambda: "warm", True),
-                ("mixed_proj", make_proj, True),
-                ("mixed_proj_learned", lambda: (make_proj(), net_p), True)]
+                ("mixed_proj_mr", lambda: make_proj(True), True),
+                ("mixed_proj_mr_learned", lambda: (make_proj(True), net_p), True)]
     res = {}
     for name, make, mx in variants:
         ms, its, rr, xf = run(make, mx)
@@ -472,7 +472,7 @@ def bench_transient(args):
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     S.set_state(st)
     S.rhs(F[K])
-    P = make_proj()
+    P = make_proj(True)
     timings = {}
     for tag, fn in (("learned", lambda: net_x.guess(S.x, S.xp, S.b, S.x0)),
                     ("proj_guess", lambda: P.guess(S.b, S.x0)),
@@ -507,7 +507,7 @@ def bench_transient(args):
                          % (K + 1, ret)}
         log("[bench3] cpu %.1f ms (%d it)" % (cpu_s * 1e3, ret))
 
-    head = "mixed_proj_learned"
+    head = "mixed_proj_mr_learned"
     ms_h = res[head][0]
     out = {
         "metric": METRIC, "value": round(ms_h, 4),

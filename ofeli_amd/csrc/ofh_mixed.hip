@@ -16,6 +16,9 @@
 // best cycle's, or the recursive residual needed over 3x the first cycle's
 // length: FP32 no longer resolves the slow modes and the host forced the
 // replacement), the FP64 CG finishes from the current x64.
+// Per inner iteration, L2-resident systems (n <= 393,216) use two kernels,
+// k_mx_upd and k_mx_dirspmv (the direction update formed at the SpMV's
+// gather); larger ones three: k_mx_spmv, k_mx_upd, k_mx_dir.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,7 +44,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_resid(
     int nblocks, const int* __restrict__ rb, const int* __restrict__ rp,
     ColIndex ci, const double* __restrict__ val, const double* __restrict__ b,
     const double* __restrict__ x, float* __restrict__ r32, float* __restrict__ x32,
-    const float* __restrict__ dinv32, const KrylovState* __restrict__ st,
+    const float* __restrict__ dinv32, float* __restrict__ z32, const KrylovState* __restrict__ st,
     double* __restrict__ partials) {
   if (st->done) return;
   __shared__ double lds[kTile64];
@@ -57,6 +60,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_resid(
     r32[i] = rf;
     x32[i] = 0.0f;
     const float z = zf<PREC>(i, rf, dinv32);
+    if (PREC == OFH_DIAG_PREC && z32) z32[i] = z;
     v[0] = bi * bi;
     v[1] = ri * ri;
     v[2] = (double)rf * (double)z;
@@ -186,6 +190,112 @@ __global__ __launch_bounds__(kBlock) void k_mx_spmv(int nblocks, const int* __re
   write_partials<1>(v, slot, partials, blockIdx.x);
 }
 
+// The new FP32 direction p_c = z_c + beta p_c (p_c = z_c when k == 0),
+// formed where the SpMV gathers it.
+struct GatherDirF {
+  const float* __restrict__ z;
+  const float* __restrict__ p;
+  float beta;
+  int first;
+  __device__ __forceinline__ float operator()(int c) const {
+    return first ? z[c] : z[c] + beta * p[c];
+  }
+};
+
+// k_mx_dir<PREC, PHASE> fused with the next SpMV, for L2-resident systems:
+// the same decision (and, on pause or stop, the same early return), then
+// q32 = A32 p_new with p_new = z + beta p_old formed at the gather; p_new of
+// the block's rows is stored; (p,q) -> slot 6.  z32 holds D^-1 r (DIAG) or
+// r (IDENT); p_old and p_new are ping-pong buffers.
+template <int PREC, int PHASE>
+__global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* __restrict__ rb,
+                                                      const int* __restrict__ rp, ColIndex ci,
+                                                      const float* __restrict__ val32, int k,
+                                                      int nparts, const float* __restrict__ z32,
+                                                      const float* __restrict__ p_old,
+                                                      float* __restrict__ p_new,
+                                                      float* __restrict__ q32, KrylovState* st,
+                                                      double* __restrict__ partials) {
+  if (PHASE == 0 ? st->halt : st->done) return;
+  __shared__ float lds[kTile32];
+  __shared__ double red[4 * kBlock / kWave];
+  const int blk = xcd_block(blockIdx.x, nblocks);
+  const int r0 = rb[blk], r1 = rb[blk + 1];
+  double nrm = st->nrm, res, rz;
+  bool stop = false, pause = false;
+  int status = 0;
+  if (PHASE == 0) {
+    double t[2];
+    const int slot[2] = {4, 5};
+    fold_partials<kBlock, 2>(t, slot, partials, nparts, red);
+    res = sqrt(t[0]) / nrm;
+    rz = t[1];
+    pause = res <= kDelta * st->rmax || res <= st->tol || k >= st->max_it || !(res == res);
+  } else {
+    double t[4];
+    const int slot[4] = {0, 1, 2, 3};
+    fold_partials<kBlock, 4>(t, slot, partials, nparts, red);
+    if (PHASE == 1) {
+      nrm = sqrt(t[0]);
+      if (nrm == 0) nrm = 1;
+    }
+    res = sqrt(t[1]) / nrm;
+    rz = t[2];
+    if (res <= st->tol) {
+      stop = true;
+      status = k;
+    } else if (k >= st->max_it || !(res == res)) {
+      stop = true;
+      status = -(st->max_it + 1);
+    }
+  }
+  const double beta = k > 0 ? rz / st->rho_buf[(k - 1) & 1] : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->it = k;
+    if (PHASE == 0) {
+      st->res = res;
+      if (pause) {
+        st->pause = 1;
+        st->halt = 1;
+      }
+    } else {
+      st->sconv_res = res;
+      st->res = res;
+      st->rmax = res;
+      if (PHASE == 1) {
+        st->nrm = nrm;
+        st->res0 = res;
+      } else {
+        st->replacements += 1;
+      }
+      st->pause = 0;
+      st->halt = stop ? 1 : 0;
+      if (stop) {
+        st->done = 1;
+        st->status = status;
+      }
+    }
+    if (!stop && !pause) {
+      st->rho_buf[k & 1] = rz;
+      st->beta = beta;
+    }
+  }
+  if (stop || pause) return;
+  const GatherDirF g{z32, p_old, (float)beta, k == 0};
+  const float sq = csr_stream_rowsum_g<float, kTile32>(r0, r1, rp, ci, val32, g, lds, 0.0f);
+  const int i = r0 + threadIdx.x;
+  double v[1] = {0.0};
+  if (i < r1) {
+    const float pi = g(i);
+    p_new[i] = pi;
+    q32[i] = sq;
+    v[0] = (double)pi * (double)sq;
+  }
+  block_sum_k<kBlock, 1>(v, red);
+  const int slot[1] = {6};
+  write_partials<1>(v, slot, partials, blockIdx.x);
+}
+
 // alpha = rho/(p,q); x32 += alpha p; r32 -= alpha q; (r,r) -> 4, (r,z) -> 5
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_mx_upd(int n, int k, int nparts_pq,
@@ -193,7 +303,8 @@ __global__ __launch_bounds__(kBlock) void k_mx_upd(int n, int k, int nparts_pq,
                                                   const float* __restrict__ p32,
                                                   const float* __restrict__ q32,
                                                   const float* __restrict__ dinv32,
-                                                  KrylovState* st, double* __restrict__ partials) {
+                                                  float* __restrict__ z32, KrylovState* st,
+                                                  double* __restrict__ partials) {
   if (st->halt) return;
   __shared__ double red[2 * kBlock / kWave];
   const int base = blockIdx.x * (kBlock * kVec) + threadIdx.x;
@@ -223,6 +334,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_upd(int n, int k, int nparts_pq,
       const float ri = rv[j] - af * qv[j];
       r32[i] = ri;
       const float z = PREC == OFH_DIAG_PREC ? ri * dv[j] : ri;
+      if (PREC == OFH_DIAG_PREC && z32) z32[i] = z;
       v[0] += (double)ri * (double)ri;
       v[1] += (double)ri * (double)z;
     }
@@ -251,12 +363,35 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   int rc;
   if (PREC == OFH_DIAG_PREC && (rc = csr_ensure_dinv(A))) return rc;
   if ((rc = csr_ensure_val32(A))) return rc;
+  // L2-resident systems use two kernels per inner iteration (k_mx_upd,
+  // k_mx_dirspmv), larger ones three (k_mx_spmv, k_mx_upd, k_mx_dir)
+  constexpr int kFuseRows = 3 << 17;
+  const bool fused = n <= kFuseRows;
   void* wsp;
-  if ((rc = ensure_vws(ctx, sizeof(float) * (size_t)n * 4, &wsp))) return rc;
+  if ((rc = ensure_vws(ctx, sizeof(float) * (size_t)n * 6, &wsp))) return rc;
   float* r32 = (float*)wsp;
   float* p32 = r32 + n;
   float* q32 = p32 + n;
   float* x32 = q32 + n;
+  float* p32b = x32 + n;                                     // second direction buffer
+  float* z32 = PREC == OFH_DIAG_PREC ? p32b + n : r32;       // D^-1 r (fused form)
+  float* pb[2] = {p32, p32b};
+  float* zw = fused && PREC == OFH_DIAG_PREC ? z32 : nullptr;  // z written by resid/upd
+  const ColIndex ci{A->col, A->col16};
+  auto dirspmv = [&](int phase, int k, int nparts) {
+    if (phase == 0)
+      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 0>), dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
+                         A->row_blocks, A->row_ptr, ci, A->val32, k, nparts, z32, pb[k & 1],
+                         pb[(k + 1) & 1], q32, state_ptr(ctx), ctx->ws.partials);
+    else if (phase == 1)
+      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 1>), dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
+                         A->row_blocks, A->row_ptr, ci, A->val32, k, nparts, z32, pb[k & 1],
+                         pb[(k + 1) & 1], q32, state_ptr(ctx), ctx->ws.partials);
+    else
+      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 2>), dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
+                         A->row_blocks, A->row_ptr, ci, A->val32, k, nparts, z32, pb[k & 1],
+                         pb[(k + 1) & 1], q32, state_ptr(ctx), ctx->ws.partials);
+  };
   KrylovState* st = state_ptr(ctx);
   KrylovState h0{};
   h0.tol = toler;
@@ -267,9 +402,13 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   const int nb = A->nblocks;
   const int ge = (n + 255) / 256;
   hipLaunchKernelGGL(k_mx_resid<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                     A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r32, x32, A->dinv32, st, part);
-  hipLaunchKernelGGL((k_mx_dir<PREC, 1>), dim3(g), dim3(kBlock), 0, s, n, 0, nb, r32, A->dinv32,
-                     p32, st, part);
+                     A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r32, x32, A->dinv32, zw,
+                     st, part);
+  if (fused)
+    dirspmv(1, 0, nb);
+  else
+    hipLaunchKernelGGL((k_mx_dir<PREC, 1>), dim3(g), dim3(kBlock), 0, s, n, 0, nb, r32, A->dinv32,
+                       p32, st, part);
   OFH_HIP(hipGetLastError());
   KrylovState h{};
   if ((rc = read_state(ctx, &h))) return rc;
@@ -285,10 +424,16 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     res_prev = h.res;
     for (int c = 0; c < chunk; ++c) {
       ++k;
+      if (fused) {
+        hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, s, n, k, nb, x32, r32,
+                           pb[k & 1], q32, A->dinv32, zw, st, part);
+        dirspmv(0, k, g);
+        continue;
+      }
       hipLaunchKernelGGL(k_mx_spmv, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr,
                          ColIndex{A->col, A->col16}, A->val32, p32, q32, st, part);
       hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, s, n, k, nb, x32, r32, p32,
-                         q32, A->dinv32, st, part);
+                         q32, A->dinv32, nullptr, st, part);
       hipLaunchKernelGGL((k_mx_dir<PREC, 0>), dim3(g), dim3(kBlock), 0, s, n, k, g, r32,
                          A->dinv32, p32, st, part);
     }
@@ -307,9 +452,13 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     // residual replacement in FP64 after kr inner iterations
     hipLaunchKernelGGL(k_mx_accum, dim3(ge), dim3(256), 0, s, n, x, x32, st);
     hipLaunchKernelGGL(k_mx_resid<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                       A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r32, x32, A->dinv32, st, part);
-    hipLaunchKernelGGL((k_mx_dir<PREC, 2>), dim3(g), dim3(kBlock), 0, s, n, kr, nb, r32,
-                       A->dinv32, p32, st, part);
+                       A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r32, x32, A->dinv32,
+                       zw, st, part);
+    if (fused)
+      dirspmv(2, kr, nb);
+    else
+      hipLaunchKernelGGL((k_mx_dir<PREC, 2>), dim3(g), dim3(kBlock), 0, s, n, kr, nb, r32,
+                         A->dinv32, p32, st, part);
     OFH_HIP(hipGetLastError());
     if ((rc = read_state(ctx, &h))) return rc;
     it_prev = h.it;

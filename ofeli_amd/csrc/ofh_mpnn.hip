@@ -41,7 +41,8 @@ constexpr int kH = 64;        // hidden channels
 constexpr int kH2 = 128;
 constexpr int kFin = 8;       // input features (4 used, 4 zero)
 constexpr int kLayer = kH * kH2 + kH + kH + kH2 * kH + kH;  // parameters per layer
-constexpr int kGradGrid = 256;  // workgroups of the node-reduction kernels
+constexpr int kNodeGrid = 2048;  // workgroups of the node-reduction kernels (8 waves per SIMD)
+constexpr int kGemmGrid = 512;   // workgroups of the weight-gradient products
 constexpr int kMaxLayers = 8;
 
 // parameter offsets in the flat FP32 vector
@@ -275,6 +276,7 @@ __global__ __launch_bounds__(256) void k_mp_wgrad(WgradArgs a) {
   const int ntiles = a.n_pad / 16;
   const int nw = gridDim.x * 4, w = blockIdx.x * 4 + wave;
   const int t0 = (int)((int64_t)ntiles * w / nw), t1 = (int)((int64_t)ntiles * (w + 1) / nw);
+#pragma unroll 2
   for (int tile = t0; tile < t1; ++tile) {
     bf16x4 av[MB], bv[JB];
 #pragma unroll
@@ -333,14 +335,22 @@ __global__ __launch_bounds__(256) void k_mp_wgrad(WgradArgs a) {
   for (int i = threadIdx.x; i < W; i += 256) out[i] = red[i];
 }
 
-// out[i] = sum_p part[p][i] in p order; i < width.
+// out[i] = sum_p part[p][i], i < width: a workgroup per 64 columns, four
+// groups of 64 threads summing every fourth partial, then the four group
+// sums added in group order (fixed order: bitwise reproducible).
 __global__ __launch_bounds__(256) void k_mp_fold(const float* __restrict__ part, int nparts,
                                                  int width, float* __restrict__ out) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= width) return;
+  __shared__ float red[4][64];
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
   float s = 0.f;
-  for (int p = 0; p < nparts; ++p) s += part[(size_t)p * width + i];
-  out[i] = s;
+  if (c < width) {
+#pragma unroll 8
+    for (int p = g; p < nparts; p += 4) s += part[(size_t)p * width + c];
+  }
+  red[g][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (g == 0 && c < width)
+    out[c] = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
 }
 
 // ---------------------------------------------------------------------------
@@ -359,10 +369,18 @@ __global__ __launch_bounds__(256) void k_mp_edge_fwd(int n, int n_pad, const int
   float acc = 0.f;
   if (i < n) {
     const float za = Z[(size_t)i * kH2 + c], w = we[c], bb = bm[c];
-    for (int k = rp[i]; k < rp[i + 1]; ++k) {
-      const int j = col[k];
-      if (j == i) continue;
-      acc += fmaxf(za + Z[(size_t)j * kH2 + kH + c] + e[k] * w + bb, 0.f);
+    // the row's columns and edge weights are loaded lane-parallel, 64 at a
+    // time, and broadcast: the gathers of Z no longer wait on index loads
+    for (int k0 = rp[i], k1 = rp[i + 1]; k0 < k1; k0 += 64) {
+      const int m = min(64, k1 - k0);
+      const int jl = c < m ? col[k0 + c] : i;
+      const float el = c < m ? e[k0 + c] : 0.f;
+#pragma unroll 4
+      for (int t = 0; t < m; ++t) {
+        const int j = __shfl(jl, t, 64);
+        const float et = __shfl(el, t, 64);
+        if (j != i) acc += fmaxf(za + Z[(size_t)j * kH2 + kH + c] + et * w + bb, 0.f);
+      }
     }
   }
   M[(size_t)i * kH + c] = acc;
@@ -389,16 +407,24 @@ __global__ __launch_bounds__(256) void k_mp_edge_bwd(int n, int n_pad, const int
     if (v < n) {
       const float za = Z[(size_t)v * kH2 + c], zc = Z[(size_t)v * kH2 + kH + c];
       const float dm = T[(size_t)v * kH2 + kH + c];
-      for (int k = rp[v]; k < rp[v + 1]; ++k) {
-        const int u = col[k];
-        if (u == v) continue;
-        const float ek = e[k];
-        if (za + Z[(size_t)u * kH2 + kH + c] + ek * w + bb > 0.f) {
-          dza += dm;
-          gwe += dm * ek;
-          gbm += dm;
+      for (int k0 = rp[v], k1 = rp[v + 1]; k0 < k1; k0 += 64) {
+        const int m = min(64, k1 - k0);
+        const int ul = c < m ? col[k0 + c] : v;
+        const float el = c < m ? e[k0 + c] : 0.f;
+#pragma unroll 4
+        for (int t = 0; t < m; ++t) {
+          const int u = __shfl(ul, t, 64);
+          const float ek = __shfl(el, t, 64);
+          if (u == v) continue;
+          const float zuc = Z[(size_t)u * kH2 + kH + c], zua = Z[(size_t)u * kH2 + c];
+          const float tu = T[(size_t)u * kH2 + kH + c];
+          if (za + zuc + ek * w + bb > 0.f) {
+            dza += dm;
+            gwe += dm * ek;
+            gbm += dm;
+          }
+          if (zua + zc + ek * w + bb > 0.f) dzc += tu;
         }
-        if (Z[(size_t)u * kH2 + c] + zc + ek * w + bb > 0.f) dzc += T[(size_t)u * kH2 + kH + c];
       }
     }
     dZ[(size_t)v * kH2 + c] = dza;
@@ -654,7 +680,7 @@ struct ofh_mpnn_s {
   double* dv = nullptr;     // n: the learned direction d
   double* scal = nullptr;   // [s, E0, d.g, loss, alpha, d.Ad]
   double* dpart = nullptr;  // 4 x nparts (row reductions)
-  float* part = nullptr;    // kGradGrid x (128*64 + 128) (node reductions)
+  float* part = nullptr;    // per-workgroup partials of the node reductions
   double* h_pinned = nullptr;
   int nparts = 0;
   hipStream_t cap = nullptr;  // capture-only stream for the inference graph
@@ -739,7 +765,7 @@ static int forward(ofh_mpnn M, const double* xn, const double* xp, const double*
 }
 
 static int fold(const float* part, int nparts, int width, float* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_mp_fold, dim3((width + 255) / 256), dim3(256), 0, s, part, nparts, width, out);
+  hipLaunchKernelGGL(k_mp_fold, dim3((width + 63) / 64), dim3(256), 0, s, part, nparts, width, out);
   return OFH_OK;
 }
 
@@ -755,9 +781,9 @@ static int backward(ofh_mpnn M, const double* xn, const double* xp, const double
                      M->ext, xs, M->dv, M->r, M->dpart, g1);
   float* dH = M->dH0;
   float* dHn = M->dH1;
-  hipLaunchKernelGGL(k_mp_dec_bwd, dim3(kGradGrid), dim3(256), 0, s, n, n_pad, Hl(M, L), M->params,
+  hipLaunchKernelGGL(k_mp_dec_bwd, dim3(kNodeGrid), dim3(256), 0, s, n, n_pad, Hl(M, L), M->params,
                      off_dec(L), M->q, M->r, M->dpart, g1, M->scal, dH, M->part);
-  fold(M->part, kGradGrid, kH + 1, M->grads + off_dec(L), s);
+  fold(M->part, kNodeGrid, kH + 1, M->grads + off_dec(L), s);
   for (int l = L - 1; l >= 0; --l) {
     // T = (dH [U > 0]) Wu^T  (n x 128)
     RowGemmArgs t{};
@@ -784,14 +810,14 @@ static int backward(ofh_mpnn M, const double* xn, const double* xp, const double
     wu.Mg = Ul(M, l);
     wu.ldm = kH;
     wu.part = M->part;
-    hipLaunchKernelGGL((k_mp_wgrad<kH2, kH, true, true>), dim3(kGradGrid), dim3(256), 0, s, wu);
+    hipLaunchKernelGGL((k_mp_wgrad<kH2, kH, true, true>), dim3(kGemmGrid), dim3(256), 0, s, wu);
     // Wu (128 x 64) and bu (64) are adjacent in the parameter vector
-    fold(M->part, kGradGrid, kH2 * kH + kH, M->grads + off_Wu(l), s);
+    fold(M->part, kGemmGrid, kH2 * kH + kH, M->grads + off_Wu(l), s);
     // aggregation backward -> dZ, we/bm gradients (adjacent)
-    hipLaunchKernelGGL(k_mp_edge_bwd, dim3(kGradGrid), dim3(256), 0, s, n, n_pad, A->row_ptr,
+    hipLaunchKernelGGL(k_mp_edge_bwd, dim3(kNodeGrid), dim3(256), 0, s, n, n_pad, A->row_ptr,
                        A->col, M->e, Zl(M, l), M->params + off_we(l), M->params + off_bm(l), M->T,
                        M->dZ, M->part);
-    fold(M->part, kGradGrid, 2 * kH, M->grads + off_we(l), s);
+    fold(M->part, kNodeGrid, 2 * kH, M->grads + off_we(l), s);
     // dWab = H^T dZ
     WgradArgs wa{};
     wa.n_pad = n_pad;
@@ -801,8 +827,8 @@ static int backward(ofh_mpnn M, const double* xn, const double* xp, const double
     wa.G = M->dZ;
     wa.ldg = kH2;
     wa.part = M->part;
-    hipLaunchKernelGGL((k_mp_wgrad<kH, kH2, false, false>), dim3(kGradGrid), dim3(256), 0, s, wa);
-    fold(M->part, kGradGrid, kH * kH2, M->grads + off_Wab(l), s);
+    hipLaunchKernelGGL((k_mp_wgrad<kH, kH2, false, false>), dim3(kGemmGrid), dim3(256), 0, s, wa);
+    fold(M->part, kGemmGrid, kH * kH2, M->grads + off_Wab(l), s);
     // dH_prev = dH + T[:, :64] + dZ Wab^T
     RowGemmArgs h{};
     h.n = n;
@@ -819,9 +845,9 @@ static int backward(ofh_mpnn M, const double* xn, const double* xp, const double
     hipLaunchKernelGGL((k_mp_rowgemm<kH2, kH, true, false, false>), dim3(gg), dim3(256), 0, s, h);
     std::swap(dH, dHn);
   }
-  hipLaunchKernelGGL(k_mp_enc_bwd, dim3(kGradGrid), dim3(256), 0, s, n_pad, M->F, Hl(M, 0), dH,
+  hipLaunchKernelGGL(k_mp_enc_bwd, dim3(kNodeGrid), dim3(256), 0, s, n_pad, M->F, Hl(M, 0), dH,
                      M->part);
-  fold(M->part, kGradGrid, kFin * kH + kH, M->grads + off_enc_W(), s);
+  fold(M->part, kNodeGrid, kFin * kH + kH, M->grads + off_enc_W(), s);
   OFH_HIP(hipGetLastError());
   return OFH_OK;
 }
@@ -929,7 +955,8 @@ int ofh_mpnn_create(ofh_csr A, int layers, int hidden, int extrapolate, uint64_t
   A_((void**)&M->dv, sizeof(double) * n);
   A_((void**)&M->scal, sizeof(double) * 8);
   A_((void**)&M->dpart, sizeof(double) * 4 * M->nparts);
-  A_((void**)&M->part, sizeof(float) * kGradGrid * (kH2 * kH + kH2));
+  A_((void**)&M->part, sizeof(float) * std::max<size_t>((size_t)kGemmGrid * (kH2 * kH + kH2),
+                                                         (size_t)kNodeGrid * (kFin * kH + kH)));
   if (he == hipSuccess) he = hipHostMalloc((void**)&M->h_pinned, 64, hipHostMallocDefault);
   if (he != hipSuccess) {
     free_all(M);
