@@ -81,6 +81,11 @@ int ofh_ctx_destroy(ofh_ctx ctx);
  * default stream); the context's own stream is released. */
 int ofh_ctx_set_stream(ofh_ctx ctx, void* hip_stream);
 int ofh_ctx_synchronize(ofh_ctx ctx);
+/* Solvers replay chunks of iterations from HIP graphs captured per argument
+ * set and cached in the context (enable = 1), or launch every kernel
+ * directly (enable = 0, the default: measured faster on MI355X).  Results
+ * are identical either way. */
+int ofh_ctx_use_graphs(ofh_ctx ctx, int enable);
 int ofh_malloc(ofh_ctx ctx, size_t bytes, void** d_ptr);
 int ofh_free(ofh_ctx ctx, void* d_ptr);
 int ofh_memcpy_h2d(ofh_ctx ctx, void* d_dst, const void* h_src, size_t bytes);

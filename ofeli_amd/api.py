@@ -70,6 +70,10 @@ class Context:
     def synchronize(self):
         check(lib().ofh_ctx_synchronize(self.h))
 
+    def use_graphs(self, enable=True):
+        """Replay chunks of solver iterations from captured graphs (default off)."""
+        check(lib().ofh_ctx_use_graphs(self.h, int(bool(enable))))
+
     def close(self):
         if self.h:
             for ch in list(self._children):

@@ -83,13 +83,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int nblocks, const int* __re
 //   it >  0: folds (r,r) (slot 0) and (r,z) (slot 1 or, SSOR, slot 2)
 // SSOR: the stop decision was taken by k_cg_check before the sweeps.
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_cg_dir(int n, int it, int nparts0, int nparts,
+__global__ __launch_bounds__(kBlock) void k_cg_dir(int n, int it_arg, int nparts0, int nparts,
                                                   const double* __restrict__ r,
                                                   const double* __restrict__ dinv,
                                                   const double* __restrict__ z,
                                                   double* __restrict__ p, KrylovState* st,
                                                   const double* __restrict__ partials) {
   if (st->done) return;
+  const int it = it_arg >= 0 ? it_arg : st->kcur;
   __shared__ double red[3 * kBlock / kWave];
   // issue this thread's loads first: they are in flight while the partials fold
   const int base = blockIdx.x * (kBlock * kVec) + threadIdx.x;
@@ -179,18 +180,23 @@ struct GatherDir {
 // IDENT/DIAG, after `it` iterations: fold (r,r), (r,z) (it > 0: slots 0, 1
 // of k_cg_upd; it == 0: slots 0..2 of k_cg_init), stop test, beta; then
 // q = A p_new with p_new = z + beta p_old; p_new of the block's rows is
-// stored; partial (p,q) -> slot 3.  z is D^-1 r (DIAG) or r (IDENT).
+// stored; partial (p,q) -> slot 3.  z is D^-1 r (DIAG) or r (IDENT); the
+// directions alternate between p0 and p1 with the parity of `it`
+// (it_arg < 0: the iteration index comes from the state).
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_cg_dirspmv(int nblocks, const int* __restrict__ rb,
                                                       const int* __restrict__ rp, ColIndex ci,
-                                                      const double* __restrict__ val, int it,
-                                                      int nparts0, int nparts,
+                                                      const double* __restrict__ val,
+                                                      int it_arg, int nparts0, int nparts,
                                                       const double* __restrict__ z,
-                                                      const double* __restrict__ p_old,
-                                                      double* __restrict__ p_new,
+                                                      double* __restrict__ p0,
+                                                      double* __restrict__ p1,
                                                       double* __restrict__ q, KrylovState* st,
                                                       double* __restrict__ partials) {
   if (st->done) return;
+  const int it = it_arg >= 0 ? it_arg : st->kcur;
+  const double* __restrict__ p_old = (it & 1) ? p1 : p0;
+  double* __restrict__ p_new = (it & 1) ? p0 : p1;
   __shared__ double lds[kTile64];
   __shared__ double red[3 * kBlock / kWave];
   const int blk = xcd_block(blockIdx.x, nblocks);
@@ -282,14 +288,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(int nblocks, const int* __re
 // alpha = rho/(p,q); x += alpha p; r -= alpha q; partials (r,r) -> 0 and
 // (r,z) -> 1 (CG.h:135-138)
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_cg_upd(int n, int it, int nparts_pq,
+__global__ __launch_bounds__(kBlock) void k_cg_upd(int n, int it_arg, int nparts_pq,
                                                   double* __restrict__ x, double* __restrict__ r,
-                                                  const double* __restrict__ p,
+                                                  const double* __restrict__ p0,
+                                                  const double* __restrict__ p1,
                                                   const double* __restrict__ q,
                                                   const double* __restrict__ dinv,
                                                   double* __restrict__ z, KrylovState* st,
                                                   double* __restrict__ partials) {
   if (st->done) return;
+  // it_arg < 0: the iteration after the one the last decision recorded
+  const int it = it_arg >= 0 ? it_arg : st->it + 1;
+  const double* __restrict__ p = (it & 1) ? p1 : p0;
   __shared__ double red[2 * kBlock / kWave];
   // this thread's loads go out before the fold of the (p,q) partials
   const int base = blockIdx.x * (kBlock * kVec) + threadIdx.x;
@@ -330,7 +340,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_upd(int n, int it, int nparts_pq,
   block_sum_k<kBlock, 2>(v, red);
   const int slot[2] = {0, 1};
   write_partials<2>(v, slot, partials, blockIdx.x);
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = alpha;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->alpha = alpha;
+    st->kcur = it;
+  }
 }
 
 // SSOR: stop decision on (r,r) before the sweeps (one block).
@@ -418,15 +431,30 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   // MI355X: -5% per iteration at 261k rows, +26% at 857k rows).
   constexpr int kFuseRows = 3 << 17;
   if (PREC != OFH_SSOR_PREC && n <= kFuseRows) {
-    double* pb[2] = {p, p1};
     const ColIndex ci{A->col, A->col16};
-    auto dirspmv = [&](int it) {
-      hipLaunchKernelGGL(k_cg_dirspmv<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                         A->row_ptr, ci, A->val, it, nb, g, z, pb[it & 1], pb[(it + 1) & 1], q,
-                         st, part);
+    double* zw = PREC == OFH_DIAG_PREC ? z : nullptr;
+    // one iteration: k_cg_upd(it), k_cg_dirspmv(it); it = -1 takes the
+    // index from the state (captured chunks)
+    auto iteration = [&](int it, hipStream_t st_) {
+      hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, it, nb, x, r, p, p1, q,
+                         A->dinv, zw, st, part);
+      hipLaunchKernelGGL(k_cg_dirspmv<PREC>, dim3(nb), dim3(kBlock), 0, st_, nb, A->row_blocks,
+                         A->row_ptr, ci, A->val, it, nb, g, z, p, p1, q, st, part);
     };
-    dirspmv(0);
+    hipLaunchKernelGGL(k_cg_dirspmv<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
+                       A->row_ptr, ci, A->val, 0, nb, g, z, p, p1, q, st, part);
     OFH_HIP(hipGetLastError());
+    hipGraphExec_t chunk_graph = nullptr;
+    if (ctx->use_graphs) {
+      GraphKey key;
+      csr_graph_key(A, 16 * 1 + PREC, x, wsp, A->val, A->dinv, &key);
+      rc = ctx_graph(ctx, key, [&](hipStream_t cs) {
+        for (int c = 0; c < kGraphIters; ++c) iteration(-1, cs);
+        OFH_HIP(hipGetLastError());
+        return OFH_OK;
+      }, &chunk_graph);
+      if (rc) return rc;
+    }
     KrylovState h{};
     if ((rc = read_state(ctx, &h))) return rc;
     int enq = 0, it_prev = 0;
@@ -435,12 +463,11 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
       const int chunk = next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq);
       it_prev = h.it;
       res_prev = h.res;
-      for (int c = 0; c < chunk; ++c) {
-        const int it = ++enq;
-        hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, nb, x, r,
-                           pb[it & 1], q, A->dinv, PREC == OFH_DIAG_PREC ? z : nullptr, st, part);
-        dirspmv(it);
-      }
+      // whole captured chunks, then the remainder launched directly
+      const int full = chunk_graph ? chunk / kGraphIters : 0;
+      for (int c = 0; c < full; ++c) OFH_HIP(hipGraphLaunch(chunk_graph, s));
+      enq += full * kGraphIters;
+      for (int c = full * kGraphIters; c < chunk; ++c) iteration(++enq, s);
       OFH_HIP(hipGetLastError());
       if ((rc = read_state(ctx, &h))) return rc;
     }
@@ -466,20 +493,44 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   if ((rc = read_state(ctx, &h))) return rc;
   int enq = 0, it_prev = 0;
   double res_prev = h.res;
+  // one iteration: k_cg_spmv, k_cg_upd(it), [SSOR sweeps], k_cg_dir(it)
+  auto iteration = [&](int it, hipStream_t st_) -> int {
+    hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, st_, nb, A->row_blocks, A->row_ptr,
+                       ColIndex{A->col, A->col16}, A->val, p, q, st, part);
+    hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, it, nb, x, r, p, p, q,
+                       A->dinv, nullptr, st, part);
+    if (PREC == OFH_SSOR_PREC) {
+      const int e = ssor_z(it, false);
+      if (e) return e;
+    }
+    hipLaunchKernelGGL(k_cg_dir<PREC>, dim3(g), dim3(kBlock), 0, st_, n, it, nb, g, r, A->dinv,
+                       z, p, st, part);
+    return OFH_OK;
+  };
+  hipGraphExec_t chunk_graph = nullptr;
+  if (ctx->use_graphs && PREC != OFH_SSOR_PREC) {
+    GraphKey key;
+    csr_graph_key(A, 16 * 2 + PREC, x, wsp, A->val, A->dinv, &key);
+    rc = ctx_graph(ctx, key, [&](hipStream_t cs) {
+      for (int c = 0; c < kGraphIters; ++c) {
+        const int e = iteration(-1, cs);
+        if (e) return e;
+      }
+      OFH_HIP(hipGetLastError());
+      return OFH_OK;
+    }, &chunk_graph);
+    if (rc) return rc;
+  }
   while (!h.done && enq < max_it) {
     const int chunk = next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq);
     it_prev = h.it;
     res_prev = h.res;
-    for (int c = 0; c < chunk; ++c) {
-      const int it = ++enq;
-      hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr,
-                         ColIndex{A->col, A->col16}, A->val, p, q, st, part);
-      hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, nb, x, r, p, q,
-                         A->dinv, nullptr, st, part);
-      if (PREC == OFH_SSOR_PREC && (rc = ssor_z(it, false))) return rc;
-      hipLaunchKernelGGL(k_cg_dir<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, nb, g, r, A->dinv,
-                         z, p, st, part);
-    }
+    // whole captured chunks, then the remainder launched directly
+    const int full = chunk_graph ? chunk / kGraphIters : 0;
+    for (int c = 0; c < full; ++c) OFH_HIP(hipGraphLaunch(chunk_graph, s));
+    enq += full * kGraphIters;
+    for (int c = full * kGraphIters; c < chunk; ++c)
+      if ((rc = iteration(++enq, s))) return rc;
     OFH_HIP(hipGetLastError());
     if ((rc = read_state(ctx, &h))) return rc;
   }

@@ -130,6 +130,36 @@ int launch_dot(ofh_ctx ctx, int n, const double* x, const double* y, double* d_o
   return OFH_OK;
 }
 
+int ctx_graph(ofh_ctx ctx, const GraphKey& key, const std::function<int(hipStream_t)>& record,
+              hipGraphExec_t* out) {
+  for (auto& e : ctx->graphs)
+    if (e.first == key) {
+      *out = e.second;
+      return OFH_OK;
+    }
+  if (!ctx->cap) OFH_HIP(hipStreamCreateWithFlags(&ctx->cap, hipStreamNonBlocking));
+  OFH_HIP(hipStreamBeginCapture(ctx->cap, hipStreamCaptureModeThreadLocal));
+  const int rc = record(ctx->cap);
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(ctx->cap, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  OFH_HIP(ce);
+  hipGraphExec_t ge = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  OFH_HIP(ie);
+  if (ctx->graphs.size() >= 16) {
+    (void)hipGraphExecDestroy(ctx->graphs.front().second);
+    ctx->graphs.erase(ctx->graphs.begin());
+  }
+  ctx->graphs.emplace_back(key, ge);
+  *out = ge;
+  return OFH_OK;
+}
+
 int ensure_vws(ofh_ctx ctx, size_t bytes, void** out) {
   if (ctx->vws_bytes < bytes) {
     if (ctx->vws) {
@@ -258,8 +288,16 @@ int ofh_ctx_destroy(ofh_ctx c) {
   hipFree(c->dev_scratch);
   hipHostFree(c->host_pinned);
   if (c->vws) hipFree(c->vws);
+  for (auto& e : c->graphs) (void)hipGraphExecDestroy(e.second);
+  if (c->cap) (void)hipStreamDestroy(c->cap);
   if (c->own_stream) hipStreamDestroy(c->stream);
   delete c;
+  return OFH_OK;
+}
+
+int ofh_ctx_use_graphs(ofh_ctx c, int enable) {
+  OFH_CHECK(c, OFH_E_INVALID, "ofh_ctx_use_graphs: null ctx");
+  c->use_graphs = enable != 0;
   return OFH_OK;
 }
 

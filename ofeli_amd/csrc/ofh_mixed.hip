@@ -79,12 +79,13 @@ __global__ __launch_bounds__(kBlock) void k_mx_resid(
 //   PHASE 2 (after a replacement): true residual stop test; else keep p:
 //           beta = (r,z)_new / rho_{k-1}, p = z + beta p.
 template <int PREC, int PHASE>
-__global__ __launch_bounds__(kBlock) void k_mx_dir(int n, int k, int nparts,
+__global__ __launch_bounds__(kBlock) void k_mx_dir(int n, int k_arg, int nparts,
                                                   const float* __restrict__ r32,
                                                   const float* __restrict__ dinv32,
                                                   float* __restrict__ p32, KrylovState* st,
                                                   const double* __restrict__ partials) {
   if (PHASE == 0 ? st->halt : st->done) return;
+  const int k = k_arg >= 0 ? k_arg : st->kcur;  // k_arg < 0: index from the state
   __shared__ double red[4 * kBlock / kWave];
   const int base = blockIdx.x * (kBlock * kVec) + threadIdx.x;
   float zv[kVec], pv[kVec];
@@ -210,13 +211,17 @@ struct GatherDirF {
 template <int PREC, int PHASE>
 __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* __restrict__ rb,
                                                       const int* __restrict__ rp, ColIndex ci,
-                                                      const float* __restrict__ val32, int k,
-                                                      int nparts, const float* __restrict__ z32,
-                                                      const float* __restrict__ p_old,
-                                                      float* __restrict__ p_new,
+                                                      const float* __restrict__ val32,
+                                                      int k_arg, int nparts,
+                                                      const float* __restrict__ z32,
+                                                      float* __restrict__ p0,
+                                                      float* __restrict__ p1,
                                                       float* __restrict__ q32, KrylovState* st,
                                                       double* __restrict__ partials) {
   if (PHASE == 0 ? st->halt : st->done) return;
+  const int k = k_arg >= 0 ? k_arg : st->kcur;  // k_arg < 0: index from the state
+  const float* __restrict__ p_old = (k & 1) ? p1 : p0;
+  float* __restrict__ p_new = (k & 1) ? p0 : p1;
   __shared__ float lds[kTile32];
   __shared__ double red[4 * kBlock / kWave];
   const int blk = xcd_block(blockIdx.x, nblocks);
@@ -298,14 +303,18 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* _
 
 // alpha = rho/(p,q); x32 += alpha p; r32 -= alpha q; (r,r) -> 4, (r,z) -> 5
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_mx_upd(int n, int k, int nparts_pq,
+__global__ __launch_bounds__(kBlock) void k_mx_upd(int n, int k_arg, int nparts_pq,
                                                   float* __restrict__ x32, float* __restrict__ r32,
-                                                  const float* __restrict__ p32,
+                                                  const float* __restrict__ p0,
+                                                  const float* __restrict__ p1,
                                                   const float* __restrict__ q32,
                                                   const float* __restrict__ dinv32,
                                                   float* __restrict__ z32, KrylovState* st,
                                                   double* __restrict__ partials) {
   if (st->halt) return;
+  // k_arg < 0: the iteration after the one the last decision recorded
+  const int k = k_arg >= 0 ? k_arg : st->it + 1;
+  const float* __restrict__ p32 = (k & 1) ? p1 : p0;
   __shared__ double red[2 * kBlock / kWave];
   const int base = blockIdx.x * (kBlock * kVec) + threadIdx.x;
   float xv[kVec], pv[kVec], rv[kVec], qv[kVec], dv[kVec];
@@ -342,7 +351,10 @@ __global__ __launch_bounds__(kBlock) void k_mx_upd(int n, int k, int nparts_pq,
   block_sum_k<kBlock, 2>(v, red);
   const int slot[2] = {4, 5};
   write_partials<2>(v, slot, partials, blockIdx.x);
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = alpha;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->alpha = alpha;
+    st->kcur = k;
+  }
 }
 
 // x64 += x32 (the correction accumulated since the last replacement)
@@ -378,19 +390,19 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   float* pb[2] = {p32, p32b};
   float* zw = fused && PREC == OFH_DIAG_PREC ? z32 : nullptr;  // z written by resid/upd
   const ColIndex ci{A->col, A->col16};
-  auto dirspmv = [&](int phase, int k, int nparts) {
+  auto dirspmv = [&](int phase, int k, int nparts, hipStream_t st_) {
     if (phase == 0)
-      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 0>), dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
-                         A->row_blocks, A->row_ptr, ci, A->val32, k, nparts, z32, pb[k & 1],
-                         pb[(k + 1) & 1], q32, state_ptr(ctx), ctx->ws.partials);
+      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 0>), dim3(A->nblocks), dim3(kBlock), 0, st_,
+                         A->nblocks, A->row_blocks, A->row_ptr, ci, A->val32, k, nparts, z32,
+                         pb[0], pb[1], q32, state_ptr(ctx), ctx->ws.partials);
     else if (phase == 1)
-      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 1>), dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
-                         A->row_blocks, A->row_ptr, ci, A->val32, k, nparts, z32, pb[k & 1],
-                         pb[(k + 1) & 1], q32, state_ptr(ctx), ctx->ws.partials);
+      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 1>), dim3(A->nblocks), dim3(kBlock), 0, st_,
+                         A->nblocks, A->row_blocks, A->row_ptr, ci, A->val32, k, nparts, z32,
+                         pb[0], pb[1], q32, state_ptr(ctx), ctx->ws.partials);
     else
-      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 2>), dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
-                         A->row_blocks, A->row_ptr, ci, A->val32, k, nparts, z32, pb[k & 1],
-                         pb[(k + 1) & 1], q32, state_ptr(ctx), ctx->ws.partials);
+      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 2>), dim3(A->nblocks), dim3(kBlock), 0, st_,
+                         A->nblocks, A->row_blocks, A->row_ptr, ci, A->val32, k, nparts, z32,
+                         pb[0], pb[1], q32, state_ptr(ctx), ctx->ws.partials);
   };
   KrylovState* st = state_ptr(ctx);
   KrylovState h0{};
@@ -405,11 +417,37 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
                      A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r32, x32, A->dinv32, zw,
                      st, part);
   if (fused)
-    dirspmv(1, 0, nb);
+    dirspmv(1, 0, nb, s);
   else
     hipLaunchKernelGGL((k_mx_dir<PREC, 1>), dim3(g), dim3(kBlock), 0, s, n, 0, nb, r32, A->dinv32,
                        p32, st, part);
   OFH_HIP(hipGetLastError());
+  // one inner iteration; k = -1 takes the index from the state (captured chunks)
+  auto iteration = [&](int k, hipStream_t st_) {
+    if (fused) {
+      hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, k, nb, x32, r32, pb[0],
+                         pb[1], q32, A->dinv32, zw, st, part);
+      dirspmv(0, k, g, st_);
+      return;
+    }
+    hipLaunchKernelGGL(k_mx_spmv, dim3(nb), dim3(kBlock), 0, st_, nb, A->row_blocks, A->row_ptr,
+                       ColIndex{A->col, A->col16}, A->val32, p32, q32, st, part);
+    hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, k, nb, x32, r32, p32,
+                       p32, q32, A->dinv32, nullptr, st, part);
+    hipLaunchKernelGGL((k_mx_dir<PREC, 0>), dim3(g), dim3(kBlock), 0, st_, n, k, g, r32,
+                       A->dinv32, p32, st, part);
+  };
+  hipGraphExec_t chunk_graph = nullptr;
+  if (ctx->use_graphs) {
+    GraphKey key;
+    csr_graph_key(A, 16 * (fused ? 3 : 4) + PREC, x, wsp, A->val32, A->dinv32, &key);
+    rc = ctx_graph(ctx, key, [&](hipStream_t cs) {
+      for (int c = 0; c < kGraphIters; ++c) iteration(-1, cs);
+      OFH_HIP(hipGetLastError());
+      return OFH_OK;
+    }, &chunk_graph);
+    if (rc) return rc;
+  }
   KrylovState h{};
   if ((rc = read_state(ctx, &h))) return rc;
   double last_true = h.res;
@@ -422,21 +460,11 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     const int chunk = next_chunk(k, h.res, it_prev, res_prev, target, max_it - k);
     it_prev = k;
     res_prev = h.res;
-    for (int c = 0; c < chunk; ++c) {
-      ++k;
-      if (fused) {
-        hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, s, n, k, nb, x32, r32,
-                           pb[k & 1], q32, A->dinv32, zw, st, part);
-        dirspmv(0, k, g);
-        continue;
-      }
-      hipLaunchKernelGGL(k_mx_spmv, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr,
-                         ColIndex{A->col, A->col16}, A->val32, p32, q32, st, part);
-      hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, s, n, k, nb, x32, r32, p32,
-                         q32, A->dinv32, nullptr, st, part);
-      hipLaunchKernelGGL((k_mx_dir<PREC, 0>), dim3(g), dim3(kBlock), 0, s, n, k, g, r32,
-                         A->dinv32, p32, st, part);
-    }
+    // whole captured chunks, then the remainder launched directly
+    const int full = chunk_graph ? chunk / kGraphIters : 0;
+    for (int c = 0; c < full; ++c) OFH_HIP(hipGraphLaunch(chunk_graph, s));
+    k += full * kGraphIters;
+    for (int c = full * kGraphIters; c < chunk; ++c) iteration(++k, s);
     OFH_HIP(hipGetLastError());
     if ((rc = read_state(ctx, &h))) return rc;
     if (h.done) break;
@@ -455,7 +483,7 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
                        A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r32, x32, A->dinv32,
                        zw, st, part);
     if (fused)
-      dirspmv(2, kr, nb);
+      dirspmv(2, kr, nb, s);
     else
       hipLaunchKernelGGL((k_mx_dir<PREC, 2>), dim3(g), dim3(kBlock), 0, s, n, kr, nb, r32,
                          A->dinv32, p32, st, part);

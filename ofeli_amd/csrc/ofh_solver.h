@@ -20,7 +20,10 @@ struct KrylovState {
   double res0, sconv_res, h_norm, rmax;
   int it, done, status, max_it;
   int sconv, cycle_i, halt, pause;  // halt = done || sconv || pause
-  int replacements, pad[3];
+  int replacements;
+  int kcur;  // iteration index of the current update kernel, for kernels
+             // launched with it = -1 (captured chunks of iterations)
+  int pad[2];
 };
 static_assert(sizeof(KrylovState) <= 512, "state block");
 
