@@ -520,31 +520,21 @@ int ofh_asm_create(ofh_ctx ctx, int dim, int nb_nodes, const double* h_coord, in
   A->n = neq;
   A->nnz = nnz;
   OFH_HIP(hipMalloc(&A->row_ptr, sizeof(int) * (neq + 1)));
-  OFH_HIP(hipMalloc(&A->col, sizeof(int) * std::max<int64_t>(nnz, 1)));
-  OFH_HIP(hipMalloc(&A->val, sizeof(double) * std::max<int64_t>(nnz, 1)));
+  OFH_HIP(hipMalloc(&A->col, sizeof(int) * (std::max<int64_t>(nnz, 1) + kPad)));
+  OFH_HIP(hipMalloc(&A->val, sizeof(double) * (std::max<int64_t>(nnz, 1) + kPad)));
+  OFH_HIP(hipMemsetAsync(A->col + nnz, 0, sizeof(int) * kPad, ctx->stream));
+  OFH_HIP(hipMemsetAsync(A->val + nnz, 0, sizeof(double) * kPad, ctx->stream));
   OFH_HIP(hipMemsetAsync(A->val, 0, sizeof(double) * std::max<int64_t>(nnz, 1), s));
   OFH_HIP(hipMemcpyAsync(A->row_ptr, rp32.data(), sizeof(int) * (neq + 1), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_row_fill, dim3((neq + 255) / 256), dim3(256), 0, s, neq, nen, nb_dof,
                      P->elem, P->dof, P->adj_ptr, P->adj_el, P->row_node, A->row_ptr, A->col);
-  // row blocks of the CSR-stream SpMV
+  // row blocks of the CSR-stream SpMV (both tiles)
   {
-    std::vector<int> rb;
-    rb.push_back(0);
-    int r = 0, maxr = 0;
+    int maxr = 0;
     for (int i = 0; i < neq; ++i) maxr = std::max(maxr, rp32[i + 1] - rp32[i]);
-    while (r < neq) {
-      const int start = r;
-      const int64_t base = rp32[r];
-      while (r < neq && r - start < kBlock && (int64_t)rp32[r + 1] - base <= kTile64) ++r;
-      if (r == start) ++r;
-      rb.push_back(r);
-    }
-    A->nblocks = (int)rb.size() - 1;
     A->max_row_nnz = maxr;
-    OFH_CHECK(A->nblocks <= kMaxPartials, OFH_E_INVALID, "ofh_asm_create: matrix too large");
-    OFH_HIP(hipMalloc(&A->row_blocks, sizeof(int) * rb.size()));
-    OFH_HIP(hipMemcpyAsync(A->row_blocks, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice, s));
-    OFH_HIP(hipStreamSynchronize(s));
+    const int rc = csr_set_row_blocks(A, rp32.data());
+    if (rc) return rc;
   }
   P->A = A;
 

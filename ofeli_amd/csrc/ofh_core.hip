@@ -160,6 +160,23 @@ int ctx_graph(ofh_ctx ctx, const GraphKey& key, const std::function<int(hipStrea
   return OFH_OK;
 }
 
+int csr_set_row_blocks(ofh_csr A, const int* rp) {
+  const std::vector<int> rb = build_row_blocks(rp, A->n, kTile64);
+  const std::vector<int> rb32 = build_row_blocks(rp, A->n, kTile32);
+  A->nblocks = (int)rb.size() - 1;
+  A->nblocks32 = (int)rb32.size() - 1;
+  OFH_CHECK(A->nblocks <= kMaxPartials && A->nblocks32 <= kMaxPartials, OFH_E_INVALID,
+            "matrix too large for the reduction slots");
+  OFH_HIP(hipMalloc(&A->row_blocks, sizeof(int) * rb.size()));
+  OFH_HIP(hipMalloc(&A->row_blocks32, sizeof(int) * rb32.size()));
+  OFH_HIP(hipMemcpyAsync(A->row_blocks, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice,
+                         A->ctx->stream));
+  OFH_HIP(hipMemcpyAsync(A->row_blocks32, rb32.data(), sizeof(int) * rb32.size(),
+                         hipMemcpyHostToDevice, A->ctx->stream));
+  OFH_HIP(hipStreamSynchronize(A->ctx->stream));
+  return OFH_OK;
+}
+
 int ensure_vws(ofh_ctx ctx, size_t bytes, void** out) {
   if (ctx->vws_bytes < bytes) {
     if (ctx->vws) {
@@ -196,7 +213,8 @@ __global__ void k_make_rel16(const int* __restrict__ rb, const int* __restrict__
 int csr_build_rel16(ofh_csr A) {
   ofh_ctx ctx = A->ctx;
   if (A->col16) return OFH_OK;
-  OFH_HIP(hipMalloc(&A->col16, sizeof(int16_t) * std::max<int64_t>(A->nnz, 1)));
+  OFH_HIP(hipMalloc(&A->col16, sizeof(int16_t) * (std::max<int64_t>(A->nnz, 1) + kPad)));
+  OFH_HIP(hipMemsetAsync(A->col16 + A->nnz, 0, sizeof(int16_t) * kPad, A->ctx->stream));
   int* bad = (int*)ctx->dev_scratch + 4;
   OFH_HIP(hipMemsetAsync(bad, 0, sizeof(int), ctx->stream));
   hipLaunchKernelGGL(k_make_rel16, dim3(A->nblocks), dim3(256), 0, ctx->stream, A->row_blocks,
@@ -214,7 +232,10 @@ int csr_build_rel16(ofh_csr A) {
 
 int csr_ensure_val32(ofh_csr A) {
   if (A->val32_valid) return OFH_OK;
-  if (!A->val32) OFH_HIP(hipMalloc(&A->val32, sizeof(float) * std::max<int64_t>(A->nnz, 1)));
+  if (!A->val32) {
+    OFH_HIP(hipMalloc(&A->val32, sizeof(float) * (std::max<int64_t>(A->nnz, 1) + kPad)));
+    OFH_HIP(hipMemsetAsync(A->val32 + A->nnz, 0, sizeof(float) * kPad, A->ctx->stream));
+  }
   int g = (int)std::min<int64_t>((A->nnz + 255) / 256, 65535);
   hipLaunchKernelGGL(k_to_f32, dim3(std::max(g, 1)), dim3(256), 0, A->ctx->stream, A->nnz,
                      A->val, A->val32);
@@ -369,20 +390,20 @@ int ofh_csr_create(ofh_ctx c, int n, int64_t nnz, const int32_t* rp, const int32
   A->max_row_nnz = maxr;
   OFH_HIP(hipSetDevice(c->device));
   OFH_HIP(hipMalloc(&A->row_ptr, sizeof(int) * (n + 1)));
-  OFH_HIP(hipMalloc(&A->col, sizeof(int) * std::max<int64_t>(nnz, 1)));
-  OFH_HIP(hipMalloc(&A->val, sizeof(double) * std::max<int64_t>(nnz, 1)));
+  OFH_HIP(hipMalloc(&A->col, sizeof(int) * (std::max<int64_t>(nnz, 1) + kPad)));
+  OFH_HIP(hipMalloc(&A->val, sizeof(double) * (std::max<int64_t>(nnz, 1) + kPad)));
+  OFH_HIP(hipMemsetAsync(A->col + nnz, 0, sizeof(int) * kPad, c->stream));
+  OFH_HIP(hipMemsetAsync(A->val + nnz, 0, sizeof(double) * kPad, c->stream));
   OFH_HIP(hipMemcpyAsync(A->row_ptr, rp, sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
   OFH_HIP(hipMemcpyAsync(A->col, col, sizeof(int) * nnz, hipMemcpyHostToDevice, c->stream));
   if (val)
     OFH_HIP(hipMemcpyAsync(A->val, val, sizeof(double) * nnz, hipMemcpyHostToDevice, c->stream));
   else
     OFH_HIP(hipMemsetAsync(A->val, 0, sizeof(double) * std::max<int64_t>(nnz, 1), c->stream));
-  std::vector<int> rb = build_row_blocks(rp, n, kTile64);
-  A->nblocks = (int)rb.size() - 1;
-  OFH_CHECK(A->nblocks <= kMaxPartials, OFH_E_INVALID, "ofh_csr_create: matrix too large");
-  OFH_HIP(hipMalloc(&A->row_blocks, sizeof(int) * rb.size()));
-  OFH_HIP(hipMemcpyAsync(A->row_blocks, rb.data(), sizeof(int) * rb.size(),
-                         hipMemcpyHostToDevice, c->stream));
+  {
+    const int rc = csr_set_row_blocks(A, rp);
+    if (rc) return rc;
+  }
   OFH_HIP(hipStreamSynchronize(c->stream));
   *out = A;
   return OFH_OK;
@@ -406,6 +427,7 @@ int ofh_csr_destroy(ofh_csr A) {
   hipFree(A->val);
   hipFree(A->val32);
   hipFree(A->row_blocks);
+  if (A->row_blocks32) hipFree(A->row_blocks32);
   hipFree(A->dinv);
   hipFree(A->dinv32);
   hipFree(A->diag_pos);

@@ -389,20 +389,21 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   float* z32 = PREC == OFH_DIAG_PREC ? p32b + n : r32;       // D^-1 r (fused form)
   float* pb[2] = {p32, p32b};
   float* zw = fused && PREC == OFH_DIAG_PREC ? z32 : nullptr;  // z written by resid/upd
-  const ColIndex ci{A->col, A->col16};
+  const ColIndex ci{A->col, nullptr};  // FP32 partition: 32-bit columns
   auto dirspmv = [&](int phase, int k, int nparts, hipStream_t st_) {
+    const int nbf = A->nblocks32;
     if (phase == 0)
-      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 0>), dim3(A->nblocks), dim3(kBlock), 0, st_,
-                         A->nblocks, A->row_blocks, A->row_ptr, ci, A->val32, k, nparts, z32,
-                         pb[0], pb[1], q32, state_ptr(ctx), ctx->ws.partials);
+      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 0>), dim3(nbf), dim3(kBlock), 0, st_, nbf,
+                         A->row_blocks32, A->row_ptr, ci, A->val32, k, nparts, z32, pb[0], pb[1],
+                         q32, state_ptr(ctx), ctx->ws.partials);
     else if (phase == 1)
-      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 1>), dim3(A->nblocks), dim3(kBlock), 0, st_,
-                         A->nblocks, A->row_blocks, A->row_ptr, ci, A->val32, k, nparts, z32,
-                         pb[0], pb[1], q32, state_ptr(ctx), ctx->ws.partials);
+      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 1>), dim3(nbf), dim3(kBlock), 0, st_, nbf,
+                         A->row_blocks32, A->row_ptr, ci, A->val32, k, nparts, z32, pb[0], pb[1],
+                         q32, state_ptr(ctx), ctx->ws.partials);
     else
-      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 2>), dim3(A->nblocks), dim3(kBlock), 0, st_,
-                         A->nblocks, A->row_blocks, A->row_ptr, ci, A->val32, k, nparts, z32,
-                         pb[0], pb[1], q32, state_ptr(ctx), ctx->ws.partials);
+      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 2>), dim3(nbf), dim3(kBlock), 0, st_, nbf,
+                         A->row_blocks32, A->row_ptr, ci, A->val32, k, nparts, z32, pb[0], pb[1],
+                         q32, state_ptr(ctx), ctx->ws.partials);
   };
   KrylovState* st = state_ptr(ctx);
   KrylovState h0{};
@@ -411,7 +412,9 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   OFH_HIP(hipMemcpyAsync(st, &h0, sizeof(h0), hipMemcpyHostToDevice, s));
   double* part = ctx->ws.partials;
   const int g = reduce_grid(n);
-  const int nb = A->nblocks;
+  const int nb = A->nblocks;      // FP64 partition (residual replacement)
+  const int nb32 = A->nblocks32;  // FP32 partition (inner SpMV)
+  const ColIndex ci32{A->col, nullptr};  // 16-bit offsets follow the FP64 partition
   const int ge = (n + 255) / 256;
   hipLaunchKernelGGL(k_mx_resid<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
                      A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r32, x32, A->dinv32, zw,
@@ -425,14 +428,14 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   // one inner iteration; k = -1 takes the index from the state (captured chunks)
   auto iteration = [&](int k, hipStream_t st_) {
     if (fused) {
-      hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, k, nb, x32, r32, pb[0],
-                         pb[1], q32, A->dinv32, zw, st, part);
+      hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, k, nb32, x32, r32,
+                         pb[0], pb[1], q32, A->dinv32, zw, st, part);
       dirspmv(0, k, g, st_);
       return;
     }
-    hipLaunchKernelGGL(k_mx_spmv, dim3(nb), dim3(kBlock), 0, st_, nb, A->row_blocks, A->row_ptr,
-                       ColIndex{A->col, A->col16}, A->val32, p32, q32, st, part);
-    hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, k, nb, x32, r32, p32,
+    hipLaunchKernelGGL(k_mx_spmv, dim3(nb32), dim3(kBlock), 0, st_, nb32, A->row_blocks32,
+                       A->row_ptr, ci32, A->val32, p32, q32, st, part);
+    hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, k, nb32, x32, r32, p32,
                        p32, q32, A->dinv32, nullptr, st, part);
     hipLaunchKernelGGL((k_mx_dir<PREC, 0>), dim3(g), dim3(kBlock), 0, st_, n, k, g, r32,
                        A->dinv32, p32, st, part);

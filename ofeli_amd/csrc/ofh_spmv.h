@@ -51,7 +51,8 @@ __device__ __forceinline__ T csr_stream_rowsum_g(int r0, int r1, const int* __re
   const int t = threadIdx.x;
   if (k1 - k0 <= TILE) {
     int k = k0 + t;
-    // 4 independent val/col loads in flight per thread
+    // 4 independent val/col loads in flight per thread (measured faster on
+    // MI355X than 16-byte pair loads: 35 vs 47 us on the 96^3 system)
     for (; k + 3 * kBlock < k1; k += 4 * kBlock) {
       int c0 = col_at(ci, r0, k), c1 = col_at(ci, r0, k + kBlock),
           c2 = col_at(ci, r0, k + 2 * kBlock), c3 = col_at(ci, r0, k + 3 * kBlock);

@@ -37,8 +37,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nblocks, const int* __restr
 
 template <typename T, int TILE, int MODE>
 static int launch(ofh_csr A, const T* val, const T* x, T* y, T alpha, hipStream_t s) {
-  hipLaunchKernelGGL((k_spmv<T, TILE, MODE>), dim3(A->nblocks), dim3(kBlock), 0, s, A->nblocks,
-                     A->row_blocks, A->row_ptr, ColIndex{A->col, A->col16}, val, x, y, alpha);
+  // FP32 runs on its own partition (larger tile) with 32-bit columns: the
+  // 16-bit offsets are relative to the FP64 partition's blocks
+  const bool f32 = sizeof(T) == 4;
+  const int nb = f32 ? A->nblocks32 : A->nblocks;
+  hipLaunchKernelGGL((k_spmv<T, TILE, MODE>), dim3(nb), dim3(kBlock), 0, s, nb,
+                     f32 ? A->row_blocks32 : A->row_blocks, A->row_ptr,
+                     ColIndex{A->col, f32 ? nullptr : A->col16}, val, x, y, alpha);
   OFH_HIP(hipGetLastError());
   return OFH_OK;
 }
