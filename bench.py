@@ -302,10 +302,25 @@ def bench_batched(args):
     ms32 = 1e3 * float(np.median(t32))
     sizes = np.diff(sys_rows)
     nnz_s = np.array([rp[sys_rows[s + 1]] - rp[sys_rows[s]] for s in range(nsys)])
-    # algorithmic bytes of the FP64 batched launch: per system and iteration
-    # one FP64 SpMV (12 B/nnz + 4 B/row pointer + x, y) + the CG vector streams
-    bytes64 = float(np.sum(it64.astype(np.float64) * (12 * nnz_s + 4 * sizes + 16 * sizes + 88 * sizes)))
-    eff_gbs = bytes64 / (ms64 * 1e-3) / 1e9
+    # algorithmic bytes per system and iteration.  Streamed form: one SpMV
+    # (12 B/nnz + 4 B/row pointer + x, y) + the CG vector streams.  Resident
+    # form: the matrix (12 B/nnz FP64, 8 B/nnz FP32) + x read and written
+    # (FP64) + dinv; the direction and the row table stay in LDS, r in
+    # registers.  Mixed counts every iteration as an FP32 one.
+    def it_bytes(shape, mixed):
+        if shape[1] == 0:
+            return 8 * nnz_s + 4 * sizes if mixed else 12 * nnz_s + 24 * sizes
+        return 8 * nnz_s + 48 * sizes if mixed else 12 * nnz_s + 108 * sizes
+    bytes64 = float(np.sum(it64.astype(np.float64) * it_bytes(shape64, False)))
+    bytes32 = float(np.sum(it32.astype(np.float64) * it_bytes(shape32, True)))
+    eff_gbs = bytes32 / (ms32 * 1e-3) / 1e9
+    eff64_gbs = bytes64 / (ms64 * 1e-3) / 1e9
+
+    def kname(shape, mixed):
+        if shape[1] == 0:
+            return "k_batch_cgmx_res (resident batched mixed CG)" if mixed else \
+                "k_batch_cg64_res (resident batched CG, FP64)"
+        return ("k_batch_cgmx<%d,%d>" if mixed else "k_batch_cg64<%d,%d>") % tuple(shape[:2])
 
     cpu = None
     if not args.no_cpu:
@@ -349,9 +364,14 @@ def bench_batched(args):
         "batched_fp64_err_vs_loop": err64, "batched_mixed_err_vs_loop": err32,
         "roofline": {"bound": "hbm", "achieved": round(eff_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(eff_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_batch_cg64 (persistent batched CG, FP64)",
-                     "note": "effective: each system's vectors stay in L2/MALL during its solve",
-                     "bytes_per_launch": bytes64, "avg_ms": round(ms64, 3)},
+                     "kernel": kname(shape32, True),
+                     "note": "effective: algorithmic bytes / time; each system's matrix is "
+                             "re-read from L2/MALL in every iteration of its solve",
+                     "bytes_per_launch": bytes32, "avg_ms": round(ms32, 3)},
+        "roofline_fp64": {"achieved": round(eff64_gbs, 1), "unit": "GB/s",
+                          "frac": round(eff64_gbs / HBM_PEAK_GBS, 4),
+                          "kernel": kname(shape64, False), "bytes_per_launch": bytes64,
+                          "avg_ms": round(ms64, 3)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

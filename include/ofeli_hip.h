@@ -178,11 +178,16 @@ int ofh_batch_cg(ofh_batch B, int prec, int mixed, const double* d_b, double* d_
  * HIP graph captured for the current arguments (enable = 1, the default), or
  * launches it directly (enable = 0).  Results are identical either way. */
 int ofh_batch_use_graph(ofh_batch B, int enable);
-/* Launch shape: workgroup size {256,512,1024} x threads per row {1,2,4,8}. */
+/* Launch shape: workgroup size {256,512,1024} x threads per row {1,2,4,8},
+ * or (1024, 0): the resident form, one workgroup per system with the
+ * direction and the row table in LDS and r in registers (every system at
+ * most 12288 rows and 2^24 nonzeros, every row at most 255 entries).  Both
+ * forms give bitwise the same iterates at workgroup size 1024. */
 int ofh_batch_set_config(ofh_batch B, int block_threads, int threads_per_row);
 int ofh_batch_get_config(ofh_batch B, int* block_threads, int* threads_per_row);
-/* Times every launch shape on this batch (x is restored) and keeps the
- * fastest for later ofh_batch_cg calls. */
+/* Times every launch shape on this batch (x is restored; the resident form
+ * only when every system fits) and keeps the fastest for later ofh_batch_cg
+ * calls. */
 int ofh_batch_autotune(ofh_batch B, int prec, int mixed, const double* d_b, double* d_x,
                        int max_it, double toler, int* best_block, int* best_tpr,
                        double* best_ms);

@@ -56,7 +56,9 @@ struct ofh_batch_s {
   double* relres = nullptr; // device, nsys
   void* work = nullptr;     // device work vectors
   size_t work_bytes = 0;
-  int block = 256, tpr = 1;  // launch shape
+  int block = 256, tpr = 1;  // launch shape (tpr = 0: resident form)
+  int max_rows = 0;          // largest system
+  bool resident_ok = false;  // every system fits the resident form
   bool tuned = false;
   bool use_graph = true;         // replay the launch from a captured graph
   hipGraphExec_t gexec = nullptr;
@@ -309,6 +311,376 @@ __global__ __launch_bounds__(BLOCK) void k_batch_cgmx(BatchArgs A) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Resident form: one 1024-thread workgroup per system of at most
+// 1024 x kResRpt rows.  Slot j of thread t is the system's local row
+// t + 1024 j; the thread keeps the slots' r and q in registers, the direction
+// p lives in LDS, where the SpMV gathers it, and only the matrix (and, FP64,
+// x) is streamed from L2 / MALL in each iteration.  Global data is read and
+// written through per-system buffer descriptors (scalar registers) with
+// 32-bit offsets: slot j's offset is t*size + j*1024*size, the second term a
+// scalar, so no per-slot address registers are held, and a read outside the
+// system's range returns 0.  Each slot's row start and length are packed
+// into LDS once per system.  The SpMV walks groups of slots in "row
+// positions": one round loads positions s..s+SU-1 of every row of the group
+// at once (interior rows of P1 meshes have 7 entries: one round).
+// Same arithmetic per system as the streamed form (the reference's
+// sequential row sums, CG.h:99-157), dot products as block trees.
+// ---------------------------------------------------------------------------
+constexpr int kResBlock = 1024;
+constexpr int kResRpt = 12;  // rows per thread: systems up to 12,288 rows (96 KB of FP64 p)
+// slots per group and row positions per round of loads, FP64 / FP32 SpMV
+constexpr int kResGrp64 = 2, kResSu64 = 8;
+constexpr int kResGrp32 = 2, kResSu32 = 7;
+// each slot's row is packed in LDS as (start - system start) << 8 | length
+constexpr int kResMaxLen = 255, kResMaxNnz = 1 << 24;
+
+using Rsrc = __amdgpu_buffer_rsrc_t;
+
+// descriptor of count elements at p (raw buffer, gfx9 data format word)
+template <typename T>
+__device__ __forceinline__ Rsrc make_rsrc(const T* p, int count) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(p), (short)0, count * (int)sizeof(T),
+                                           0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bld(Rsrc r, int voff, int soff);
+template <>
+__device__ __forceinline__ double bld<double>(Rsrc r, int voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ float bld<float>(Rsrc r, int voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ int bld<int>(Rsrc r, int voff, int soff) {
+  return __builtin_bit_cast(int, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bst(double v, Rsrc r, int voff, int soff) {
+  typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+
+// one system's descriptors (uniform)
+struct ResSys {
+  int a, n_s, k_base;
+  Rsrc rp, col, val, b, x, dinv;
+};
+
+template <typename TV>
+__device__ __forceinline__ ResSys res_sys(const BatchArgs& A, int a, int n_s, const TV* val,
+                                          const void* dinv, int dinv_size) {
+  ResSys S;
+  S.a = a;
+  S.n_s = n_s;
+  S.k_base = __builtin_amdgcn_readfirstlane(A.rp[a]);
+  const int nnz_s = __builtin_amdgcn_readfirstlane(A.rp[a + n_s]) - S.k_base;
+  S.rp = make_rsrc(A.rp + a, n_s + 1);
+  S.col = make_rsrc(A.col + S.k_base, nnz_s);
+  S.val = make_rsrc(val + S.k_base, nnz_s);
+  S.b = make_rsrc(A.b + a, n_s);
+  S.x = make_rsrc(A.x + a, n_s);
+  S.dinv = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<char*>((const char*)dinv) + (size_t)a * dinv_size, (short)0, n_s * dinv_size,
+      0x00020000);
+  return S;
+}
+
+// packed row starts and lengths of this thread's slots (read back only by
+// the same thread, so no barrier is needed)
+__device__ __forceinline__ void res_rows(const ResSys& S, unsigned* rinfo) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < kResRpt; ++j) {
+    unsigned ri = 0;
+    if (t < S.n_s - kResBlock * j) {
+      const int so = kResBlock * j * 4;
+      const int e0 = bld<int>(S.rp, t * 4, so), e1 = bld<int>(S.rp, t * 4 + 4, so);
+      ri = ((unsigned)(e0 - S.k_base) << 8) | (unsigned)(e1 - e0);
+    }
+    rinfo[kResBlock * j + t] = ri;
+  }
+}
+
+template <int K>
+__device__ __forceinline__ void res_sum(double (&v)[K], double* red) {
+  block_sum_k<kResBlock, K>(v, red);
+}
+
+// q_j = sum_k a_k x[c_k] for the slots of this thread; x is the system's
+// vector, local index c - a, read from LDS (xl) or through a descriptor (xg).
+// GRP slots at a time, SU row positions per round of loads.
+template <int GRP, int SU, typename T, typename TV, bool LDSX>
+__device__ __forceinline__ void res_spmv(const ResSys& S, const unsigned* __restrict__ rinfo,
+                                         const T* __restrict__ xl, T (&q)[kResRpt]) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < kResRpt; ++j) q[j] = T(0);
+#pragma unroll
+  for (int g = 0; g < kResRpt; g += GRP) {
+    if (S.n_s <= kResBlock * g) break;  // uniform: no rows left in this group
+    int k0[GRP], len[GRP], mx = 0;
+#pragma unroll
+    for (int j = 0; j < GRP; ++j) {
+      const unsigned ri = rinfo[kResBlock * (g + j) + t];  // 0 for a slot beyond the system
+      k0[j] = (int)(ri >> 8);
+      len[j] = (int)(ri & 255u);
+      mx = max(mx, len[j]);
+    }
+    // every load is issued (past the row's end it reads other entries of the
+    // system, or 0 past the system), the sum takes only the row's own
+    // entries, in row order
+    for (int s = 0; s < mx; s += SU) {
+      TV v[SU][GRP];
+      int c[SU][GRP];
+#pragma unroll
+      for (int u = 0; u < SU; ++u)
+#pragma unroll
+        for (int j = 0; j < GRP; ++j) {
+          v[u][j] = bld<TV>(S.val, (k0[j] + s + u) * (int)sizeof(TV), 0);
+          c[u][j] = bld<int>(S.col, (k0[j] + s + u) * 4, 0);
+        }
+#pragma unroll
+      for (int u = 0; u < SU; ++u)
+#pragma unroll
+        for (int j = 0; j < GRP; ++j) {
+          const bool on = s + u < len[j];
+          const int ci = on ? c[u][j] - S.a : 0;
+          const T xv = LDSX ? xl[ci] : (T)bld<double>(S.x, ci * 8, 0);
+          const T prod = (T)v[u][j] * xv;
+          q[g + j] = on ? q[g + j] + prod : q[g + j];
+        }
+    }
+  }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(kResBlock) void k_batch_cg64_res(BatchArgs A) {
+  __shared__ double p[kResBlock * kResRpt];
+  __shared__ unsigned rinfo[kResBlock * kResRpt];
+  __shared__ double red[3 * kResBlock / kWave];
+  __shared__ int s_w;
+  const int t = threadIdx.x;
+  for (;;) {
+    if (t == 0) s_w = atomicAdd(A.counter, 1);
+    __syncthreads();
+    // wave-uniform: the system's descriptors stay in scalar registers
+    const int w = __builtin_amdgcn_readfirstlane(s_w);
+    if (w >= A.nsys) break;
+    const int sys = __builtin_amdgcn_readfirstlane(A.order[w]);
+    const int a = __builtin_amdgcn_readfirstlane(A.sys_rows[sys]);
+    const int n_s = __builtin_amdgcn_readfirstlane(A.sys_rows[sys + 1]) - a;
+    const ResSys S = res_sys<double>(A, a, n_s, A.val, A.dinv, 8);
+    res_rows(S, rinfo);
+    const auto own = [&](int j) { return t < n_s - kResBlock * j; };
+    const auto dinv = [&](int j) { return bld<double>(S.dinv, t * 8, kResBlock * j * 8); };
+    double r[kResRpt], q[kResRpt];
+    // r = b - A x (CG.h:115), x gathered through its descriptor
+    res_spmv<kResGrp64, kResSu64, double, double, false>(S, rinfo, nullptr, q);
+    double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < kResRpt; ++j) {
+      r[j] = 0.0;
+      if (own(j)) {
+        const double bi = bld<double>(S.b, t * 8, kResBlock * j * 8), ri = bi - q[j];
+        r[j] = ri;
+        v[0] += bi * bi;
+        v[1] += ri * ri;
+        v[2] += ri * (PREC == OFH_DIAG_PREC ? ri * dinv(j) : ri);
+      }
+    }
+    res_sum<3>(v, red);
+    double nrm = sqrt(v[0]);
+    if (nrm == 0) nrm = 1;
+    double res = sqrt(v[1]) / nrm, rho = v[2], rho_1 = 1.0;
+    int it = 0, st;
+    if (res <= A.tol) {
+      st = 0;
+    } else {
+      st = -(A.max_it + 1);
+      for (it = 1; it <= A.max_it; ++it) {
+        const double beta = it == 1 ? 0.0 : rho / rho_1;
+        // the reductions end in barriers: nobody reads the old p any more
+#pragma unroll
+        for (int j = 0; j < kResRpt; ++j) {
+          if (own(j)) {
+            const double zi = PREC == OFH_DIAG_PREC ? r[j] * dinv(j) : r[j];
+            double& pj = p[kResBlock * j + t];
+            pj = it == 1 ? zi : zi + beta * pj;
+          }
+        }
+        __syncthreads();
+        res_spmv<kResGrp64, kResSu64, double, double, true>(S, rinfo, p, q);
+        double pq[1] = {0.0};
+#pragma unroll
+        for (int j = 0; j < kResRpt; ++j)
+          if (own(j)) pq[0] += p[kResBlock * j + t] * q[j];
+        res_sum<1>(pq, red);
+        const double alpha = rho / pq[0];
+        double u[2] = {0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < kResRpt; ++j) {
+          if (own(j)) {
+            const int so = kResBlock * j * 8;
+            bst(bld<double>(S.x, t * 8, so) + alpha * p[kResBlock * j + t], S.x, t * 8, so);
+            const double ri = r[j] - alpha * q[j];
+            r[j] = ri;
+            u[0] += ri * ri;
+            u[1] += ri * (PREC == OFH_DIAG_PREC ? ri * dinv(j) : ri);
+          }
+        }
+        res_sum<2>(u, red);
+        res = sqrt(u[0]) / nrm;
+        if (res <= A.tol) {
+          st = it;
+          break;
+        }
+        if (!(res == res)) break;
+        rho_1 = rho;
+        rho = u[1];
+      }
+      if (st < 0) it = A.max_it;
+    }
+    if (t == 0) {
+      A.iters[sys] = it;
+      A.status[sys] = st;
+      A.relres[sys] = res;
+    }
+    __syncthreads();
+  }
+}
+
+// Mixed resident form: FP32 p in LDS, FP32 r, correction and q in registers,
+// FP64 residual replacement with x64 read from and written to global memory.
+template <int PREC>
+__global__ __launch_bounds__(kResBlock) void k_batch_cgmx_res(BatchArgs A) {
+  __shared__ float p[kResBlock * kResRpt];
+  __shared__ unsigned rinfo[kResBlock * kResRpt];
+  __shared__ double red[3 * kResBlock / kWave];
+  __shared__ int s_w;
+  const int t = threadIdx.x;
+  for (;;) {
+    if (t == 0) s_w = atomicAdd(A.counter, 1);
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(s_w);
+    if (w >= A.nsys) break;
+    const int sys = __builtin_amdgcn_readfirstlane(A.order[w]);
+    const int a = __builtin_amdgcn_readfirstlane(A.sys_rows[sys]);
+    const int n_s = __builtin_amdgcn_readfirstlane(A.sys_rows[sys + 1]) - a;
+    const ResSys S = res_sys<float>(A, a, n_s, A.val32, A.dinv32, 4);
+    const ResSys S64 = res_sys<double>(A, a, n_s, A.val, A.dinv32, 4);
+    res_rows(S, rinfo);
+    const auto own = [&](int j) { return t < n_s - kResBlock * j; };
+    const auto dinv = [&](int j) { return bld<float>(S.dinv, t * 4, kResBlock * j * 4); };
+    float r[kResRpt], x32[kResRpt], q[kResRpt];
+    double nrm = 0, res = 0, rho = 0, rho_1 = 1.0, rmax = 0;
+    int it = 0, st = 0;
+    bool first = true;
+    for (;;) {  // residual replacement (the first pass sets up the system)
+      double v[3] = {0.0, 0.0, 0.0};
+      {
+        double rq[kResRpt];  // FP64 row sums of A x64
+        res_spmv<kResGrp64, kResSu64, double, double, false>(S64, rinfo, nullptr, rq);
+#pragma unroll
+        for (int j = 0; j < kResRpt; ++j) {
+          r[j] = 0.0f;
+          x32[j] = 0.0f;
+          if (own(j)) {
+            const double bi = bld<double>(S.b, t * 8, kResBlock * j * 8), ri = bi - rq[j];
+            const float rf = (float)ri;
+            r[j] = rf;
+            v[0] += bi * bi;
+            v[1] += ri * ri;
+            const float zf = PREC == OFH_DIAG_PREC ? rf * dinv(j) : rf;
+            v[2] += (double)rf * (double)zf;
+          }
+        }
+      }
+      res_sum<3>(v, red);
+      if (first) {
+        nrm = sqrt(v[0]);
+        if (nrm == 0) nrm = 1;
+      }
+      res = sqrt(v[1]) / nrm;
+      rmax = res;
+      bool done = false;
+      if (res <= A.tol) {
+        st = it;
+        done = true;
+      } else if (it >= A.max_it || !(res == res)) {
+        st = -(A.max_it + 1);
+        done = true;
+      }
+      if (done) break;
+      const double beta = first ? 0.0 : v[2] / rho_1;  // keep the direction p
+      rho = v[2];
+#pragma unroll
+      for (int j = 0; j < kResRpt; ++j) {
+        if (own(j)) {
+          const float zi = PREC == OFH_DIAG_PREC ? r[j] * dinv(j) : r[j];
+          float& pj = p[kResBlock * j + t];
+          pj = first ? zi : zi + (float)beta * pj;
+        }
+      }
+      first = false;
+      for (;;) {  // FP32 inner iterations until the recursive residual drops by kBatchDelta
+        __syncthreads();
+        ++it;
+        res_spmv<kResGrp32, kResSu32, float, float, true>(S, rinfo, p, q);
+        double pq[1] = {0.0};
+#pragma unroll
+        for (int j = 0; j < kResRpt; ++j)
+          if (own(j)) pq[0] += (double)p[kResBlock * j + t] * (double)q[j];
+        res_sum<1>(pq, red);
+        const float alpha = (float)(rho / pq[0]);
+        double u[2] = {0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < kResRpt; ++j) {
+          if (own(j)) {
+            x32[j] += alpha * p[kResBlock * j + t];
+            const float ri = r[j] - alpha * q[j];
+            r[j] = ri;
+            const float zi = PREC == OFH_DIAG_PREC ? ri * dinv(j) : ri;
+            u[0] += (double)ri * (double)ri;
+            u[1] += (double)ri * (double)zi;
+          }
+        }
+        res_sum<2>(u, red);
+        const double rec = sqrt(u[0]) / nrm;
+        rho_1 = rho;
+        if (rec <= kBatchDelta * rmax || rec <= A.tol || it >= A.max_it || !(rec == rec)) break;
+        const double beta2 = u[1] / rho;
+        rho = u[1];
+#pragma unroll
+        for (int j = 0; j < kResRpt; ++j) {
+          if (own(j)) {
+            const float zi = PREC == OFH_DIAG_PREC ? r[j] * dinv(j) : r[j];
+            float& pj = p[kResBlock * j + t];
+            pj = zi + (float)beta2 * pj;
+          }
+        }
+      }
+      // x64 += correction (own rows); the barrier makes the new x64 visible to
+      // the whole workgroup before the next pass gathers it
+#pragma unroll
+      for (int j = 0; j < kResRpt; ++j) {
+        if (own(j)) {
+          const int so = kResBlock * j * 8;
+          bst(bld<double>(S.x, t * 8, so) + (double)x32[j], S.x, t * 8, so);
+        }
+      }
+      __syncthreads();
+    }
+    if (t == 0) {
+      A.iters[sys] = it;
+      A.status[sys] = st;
+      A.relres[sys] = res;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void k_fill_int(int* p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
@@ -324,7 +696,17 @@ static void launch_batch(const BatchArgs& a, int grid, hipStream_t s) {
 }
 
 template <int PREC, bool MIXED>
+static void launch_res(const BatchArgs& a, int grid, hipStream_t s) {
+  if (MIXED)
+    hipLaunchKernelGGL((k_batch_cgmx_res<PREC>), dim3(grid), dim3(kResBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_batch_cg64_res<PREC>), dim3(grid), dim3(kResBlock), 0, s, a);
+}
+
+// tpr = 0 selects the resident form (1024 threads)
+template <int PREC, bool MIXED>
 static LaunchFn pick(int block, int tpr) {
+  if (block == kResBlock && tpr == 0) return &launch_res<PREC, MIXED>;
 #define OFH_CASE(B, T) \
   if (block == B && tpr == T) return &launch_batch<B, T, PREC, MIXED>;
   OFH_CASE(256, 1) OFH_CASE(256, 2) OFH_CASE(256, 4) OFH_CASE(256, 8)
@@ -354,6 +736,9 @@ static int batch_run(ofh_batch B, int prec, int mixed, const double* b, double* 
   }
   LaunchFn fn = pick_fn(prec, mixed != 0, block, tpr);
   OFH_CHECK(fn, OFH_E_INVALID, "batched CG: unsupported launch shape");
+  OFH_CHECK(tpr != 0 || B->resident_ok, OFH_E_INVALID,
+            "batched CG: the resident form needs systems of at most 12288 rows, 2^24 "
+            "nonzeros and rows of at most 255 entries");
   BatchArgs a{};
   a.nsys = B->nsys;
   a.sys_rows = B->sys_rows;
@@ -376,7 +761,8 @@ static int batch_run(ofh_batch B, int prec, int mixed, const double* b, double* 
   a.relres = B->relres;
   // persistent grid: every workgroup keeps pulling systems until the queue is
   // empty, so all of them reach the exit
-  const int per_cu = block >= 1024 ? 2 : (block >= 512 ? 4 : 8);
+  // a resident workgroup holds 128 VGPRs x 16 waves: one per CU
+  const int per_cu = tpr == 0 ? 1 : block >= 1024 ? 2 : (block >= 512 ? 4 : 8);
   const int grid = std::max(1, std::min(B->nsys, ctx->num_cu * per_cu));
   if (!graph) {
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, B->counter, 0);
@@ -441,6 +827,14 @@ int ofh_batch_create(ofh_csr A, int nsys, const int32_t* h_sys_rows, ofh_batch* 
   B->A = A;
   B->nsys = nsys;
   B->h_rows.assign(h_sys_rows, h_sys_rows + nsys + 1);
+  int max_len = 0, max_nnz = 0;
+  for (int k = 0; k < nsys; ++k) {
+    B->max_rows = std::max(B->max_rows, h_sys_rows[k + 1] - h_sys_rows[k]);
+    max_nnz = std::max(max_nnz, rp[h_sys_rows[k + 1]] - rp[h_sys_rows[k]]);
+  }
+  for (int i = 0; i < A->n; ++i) max_len = std::max(max_len, rp[i + 1] - rp[i]);
+  B->resident_ok = B->max_rows <= kResBlock * kResRpt && max_len <= kResMaxLen && max_nnz < kResMaxNnz;
+
   hipStream_t s = A->ctx->stream;
   OFH_HIP(hipMalloc(&B->sys_rows, sizeof(int) * (nsys + 1)));
   OFH_HIP(hipMalloc(&B->order, sizeof(int) * nsys));
@@ -450,6 +844,7 @@ int ofh_batch_create(ofh_csr A, int nsys, const int32_t* h_sys_rows, ofh_batch* 
   OFH_HIP(hipMalloc(&B->relres, sizeof(double) * nsys));
   OFH_HIP(hipMemcpyAsync(B->sys_rows, h_sys_rows, sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, s));
   OFH_HIP(hipMemcpyAsync(B->order, order.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, s));
+
   OFH_HIP(hipStreamSynchronize(s));
   *out = B;
   return OFH_OK;
@@ -469,7 +864,11 @@ int ofh_batch_destroy(ofh_batch B) {
 int ofh_batch_set_config(ofh_batch B, int block_threads, int threads_per_row) {
   OFH_CHECK(B, OFH_E_INVALID, "null batch");
   OFH_CHECK(pick_fn(OFH_DIAG_PREC, false, block_threads, threads_per_row), OFH_E_INVALID,
-            "ofh_batch_set_config: block in {256,512,1024}, threads per row in {1,2,4,8}");
+            "ofh_batch_set_config: block in {256,512,1024}, threads per row in {1,2,4,8}, "
+            "or (1024, 0) for the resident form");
+  OFH_CHECK(threads_per_row != 0 || B->resident_ok, OFH_E_INVALID,
+            "ofh_batch_set_config: the resident form needs systems of at most 12288 rows, "
+            "2^24 nonzeros and rows of at most 255 entries");
   B->block = block_threads;
   B->tpr = threads_per_row;
   B->tuned = true;
@@ -539,16 +938,21 @@ int ofh_batch_autotune(ofh_batch B, int prec, int mixed, const double* d_b, doub
   hipEvent_t e0, e1;
   OFH_HIP(hipEventCreate(&e0));
   OFH_HIP(hipEventCreate(&e1));
-  const int blocks[3] = {256, 512, 1024}, tprs[4] = {1, 2, 4, 8};
+  // candidate shapes: 3 block sizes x 4 threads per row, then the resident form
+  // when every system fits on chip
+  std::vector<std::pair<int, int>> shapes;
+  for (int blk : {256, 512, 1024})
+    for (int t : {1, 2, 4, 8}) shapes.emplace_back(blk, t);
+  if (B->resident_ok) shapes.emplace_back(kResBlock, 0);
   float best = 1e30f;
   int bb = B->block, bt = B->tpr;
-  for (int ib = 0; ib < 3; ++ib)
-    for (int it = 0; it < 4; ++it) {
+  for (const auto& sh : shapes) {
+    {
       float ms_min = 1e30f;
       for (int rep = 0; rep < 2; ++rep) {  // first run warms caches and code
         OFH_HIP(hipMemcpyAsync(d_x, x0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         OFH_HIP(hipEventRecord(e0, s));
-        int rc = batch_run(B, prec, mixed, d_b, d_x, max_it, toler, blocks[ib], tprs[it], s, false);
+        int rc = batch_run(B, prec, mixed, d_b, d_x, max_it, toler, sh.first, sh.second, s, false);
         if (rc) { hipFree(x0); return rc; }
         OFH_HIP(hipEventRecord(e1, s));
         OFH_HIP(hipEventSynchronize(e1));
@@ -558,10 +962,11 @@ int ofh_batch_autotune(ofh_batch B, int prec, int mixed, const double* d_b, doub
       }
       if (ms_min < best) {
         best = ms_min;
-        bb = blocks[ib];
-        bt = tprs[it];
+        bb = sh.first;
+        bt = sh.second;
       }
     }
+  }
   OFH_HIP(hipMemcpyAsync(d_x, x0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   OFH_HIP(hipStreamSynchronize(s));
   hipEventDestroy(e0);

@@ -51,7 +51,7 @@ def test_batch_blocks_equal_per_system_assembly(ctx):
         assert np.array_equal(bh[r0:r1], ob)
 
 
-@pytest.mark.parametrize("block,tpr", [(256, 1), (512, 4), (1024, 8)])
+@pytest.mark.parametrize("block,tpr", [(256, 1), (512, 4), (1024, 8), (1024, 0)])
 def test_batch_fp64_matches_oracle(ctx, block, tpr):
     import ofeli_amd as oa
     P, b, sys_rows, sizes, ks, fs = small_batch(ctx)
@@ -93,7 +93,8 @@ def test_batch_autotune_and_not_block_diagonal(ctx):
     B = oa.Batch(P.A, sys_rows)
     x = _t().zeros(P.nb_eq, dtype=_t().float64, device="cuda")
     blk, tpr, ms = B.autotune(b, x, max_it=1000, toler=1e-10)
-    assert blk in (256, 512, 1024) and tpr in (1, 2, 4, 8) and ms > 0
+    assert (blk, tpr) == (1024, 0) or (blk in (256, 512, 1024) and tpr in (1, 2, 4, 8))
+    assert ms > 0
     assert B.config() == (blk, tpr)
     assert float(x.abs().max()) == 0.0  # the guesses are restored
     bad = sys_rows.copy()
@@ -132,3 +133,51 @@ def test_batch_graph_replays_equal_eager(ctx, mixed):
     B.cg(b, x, mixed=mixed, max_it=1500, toler=1e-9)
     assert torch.equal(x, x_eager)
     assert float(torch.linalg.norm(x2 - 2.0 * x_eager) / torch.linalg.norm(x2)) < 1e-7
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batch_resident_equals_streamed(ctx, mixed):
+    """The resident form (direction in LDS, r and x in registers / global)
+    sums every row and every dot product in the streamed 1024-thread form's
+    order, so both give bitwise the same iterates.  Systems span one to
+    twelve row slots per thread (an 111 x 111 cell square has 110^2 = 12,100
+    equations, the limit is 12,288)."""
+    import ofeli_amd as oa
+    torch = _t()
+    sizes = (111, 70, 33, 5, 12, 45)
+    P, b, sys_rows, _, ks, fs = small_batch(ctx, sizes=sizes, seed=3)
+    B = oa.Batch(P.A, sys_rows)
+    B.use_graph(False)
+    out = {}
+    for tpr in (1, 0):
+        B.set_config(1024, tpr)
+        x = torch.zeros(P.nb_eq, dtype=torch.float64, device="cuda")
+        it, st, rr, info = B.cg(b, x, mixed=mixed, max_it=4000, toler=1e-9)
+        out[tpr] = (x, it, st, rr)
+    x1, it1, st1, rr1 = out[1]
+    x0, it0, st0, rr0 = out[0]
+    assert (st0 > 0).all() and (rr0 <= 1e-9).all()
+    assert np.array_equal(it0, it1) and np.array_equal(st0, st1) and np.array_equal(rr0, rr1)
+    assert torch.equal(x0, x1)
+    # true residual per system
+    y = torch.zeros_like(x0)
+    oa.spmv(P.A, x0, y)
+    r = (b - y).cpu().numpy()
+    bh = b.cpu().numpy()
+    for s in range(len(sizes)):
+        r0, r1 = sys_rows[s], sys_rows[s + 1]
+        assert np.linalg.norm(r[r0:r1]) / np.linalg.norm(bh[r0:r1]) <= (1.2e-9 if not mixed else 1e-9)
+
+
+def test_batch_resident_size_limit(ctx):
+    """Systems above 12,288 rows cannot use the resident form: set_config
+    refuses it and the autotuner does not offer it."""
+    import ofeli_amd as oa
+    torch = _t()
+    P, b, sys_rows, sizes, ks, fs = small_batch(ctx, sizes=(112, 6))  # 111^2 = 12,321 rows
+    B = oa.Batch(P.A, sys_rows)
+    with pytest.raises(oa.OfhError):
+        B.set_config(1024, 0)
+    x = torch.zeros(P.nb_eq, dtype=torch.float64, device="cuda")
+    blk, tpr, ms = B.autotune(b, x, max_it=3000, toler=1e-8)
+    assert tpr in (1, 2, 4, 8)
