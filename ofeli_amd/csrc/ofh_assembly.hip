@@ -655,6 +655,7 @@ int ofh_asm_build(ofh_asm P, int kind, const double* d_coef, const double* d_f,
                        A->nnz, P->map_ptr, P->map, P->ke, A->val);
     A->val32_valid = false;
     A->dinv_valid = false;
+    ++A->version;
   }
   if (d_b)
     hipLaunchKernelGGL(k_gather_rhs, dim3((P->nb_eq + 255) / 256), dim3(256), 0, s, P->nb_eq,

@@ -59,6 +59,16 @@ struct ofh_batch_s {
   int block = 256, tpr = 1;  // launch shape (tpr = 0: resident form)
   int max_rows = 0;          // largest system
   bool resident_ok = false;  // every system fits the resident form
+  // resident form: ELL copy of the matrix, one width per system, 64-row
+  // chunks stored position-major
+  int* ell_e0 = nullptr;      // device, nsys + 1: first entry of each system
+  int* ell_w = nullptr;       // device, nsys: width (longest row) of each system
+  int* ell_col = nullptr;     // device: local column x 8 (bytes); padding -> kResPad x 8
+  int* ell_src = nullptr;     // device: CSR entry of each slot, -1 for padding
+  double* ell_val = nullptr;  // padding -0.0
+  float* ell_val32 = nullptr;
+  int64_t ell_n = 0;
+  uint64_t ell_version = ~0ull;  // CSR values version gathered into ell_val*
   bool tuned = false;
   bool use_graph = true;         // replay the launch from a captured graph
   hipGraphExec_t gexec = nullptr;
@@ -108,6 +118,11 @@ struct BatchArgs {
   const int* col;
   const double* val;
   const float* val32;
+  const int* ell_e0;        // resident form: first ELL entry of each system
+  const int* ell_w;         // ELL width of each system
+  const int* ell_col;       // byte offset of the local column
+  const double* ell_val;
+  const float* ell_val32;
   const double* dinv;
   const float* dinv32;
   const double* b;
@@ -316,24 +331,30 @@ __global__ __launch_bounds__(BLOCK) void k_batch_cgmx(BatchArgs A) {
 // 1024 x kResRpt rows.  Slot j of thread t is the system's local row
 // t + 1024 j; the thread keeps the slots' r and q in registers, the direction
 // p lives in LDS, where the SpMV gathers it, and only the matrix (and, FP64,
-// x) is streamed from L2 / MALL in each iteration.  Global data is read and
-// written through per-system buffer descriptors (scalar registers) with
-// 32-bit offsets: slot j's offset is t*size + j*1024*size, the second term a
-// scalar, so no per-slot address registers are held, and a read outside the
-// system's range returns 0.  Each slot's row start and length are packed
-// into LDS once per system.  The SpMV walks groups of slots in "row
-// positions": one round loads positions s..s+SU-1 of every row of the group
-// at once (interior rows of P1 meshes have 7 entries: one round).
-// Same arithmetic per system as the streamed form (the reference's
-// sequential row sums, CG.h:99-157), dot products as block trees.
+// x) is streamed from L2 / MALL in each iteration.
+//
+// The matrix is read from an ELL copy built at batch creation with one width
+// W per system (its longest row): local rows 64c..64c+63 form chunk c (one
+// wave's rows of one slot), stored position-major, entry s of row `lane` at
+// e0 + (c W + s) 64 + lane, so every load instruction reads 64 consecutive
+// entries, and the offset of a load is a per-slot scalar plus an immediate.
+// Columns are stored as byte offsets (local column x 8).  A padding entry has
+// the value -0.0 and points at a slot holding +0.0 (LDS word kResPad; past
+// the end of the system for the global vector, where the buffer read returns
+// 0): its product is -0.0, and q + (-0.0) == q exactly, so the row sums equal
+// the sums over the row's own entries in column order -- the streamed form's
+// arithmetic (the reference's sequential row sums, CG.h:99-157) -- without a
+// compare or a select.  Global data goes through per-system buffer
+// descriptors (scalar registers).  Dot products are block trees whose LDS
+// buffers alternate, so each needs one barrier.
 // ---------------------------------------------------------------------------
 constexpr int kResBlock = 1024;
 constexpr int kResRpt = 12;  // rows per thread: systems up to 12,288 rows (96 KB of FP64 p)
-// slots per group and row positions per round of loads, FP64 / FP32 SpMV
-constexpr int kResGrp64 = 2, kResSu64 = 8;
-constexpr int kResGrp32 = 2, kResSu32 = 7;
-// each slot's row is packed in LDS as (start - system start) << 8 | length
-constexpr int kResMaxLen = 255, kResMaxNnz = 1 << 24;
+constexpr int kResChunk = 64;
+constexpr int kResPad = kResBlock * kResRpt;  // LDS word holding +0.0
+constexpr int kResSu = 8;  // row positions per round of loads
+constexpr int kResGrp = 2;  // slots per round of loads
+constexpr int kResRed = 3 * kResBlock / kWave;  // doubles per reduction buffer
 
 using Rsrc = __amdgpu_buffer_rsrc_t;
 
@@ -364,21 +385,21 @@ __device__ __forceinline__ void bst(double v, Rsrc r, int voff, int soff) {
 
 // one system's descriptors (uniform)
 struct ResSys {
-  int a, n_s, k_base;
-  Rsrc rp, col, val, b, x, dinv;
+  int a, n_s, w;
+  Rsrc col, val, b, x, dinv;
 };
 
 template <typename TV>
-__device__ __forceinline__ ResSys res_sys(const BatchArgs& A, int a, int n_s, const TV* val,
-                                          const void* dinv, int dinv_size) {
+__device__ __forceinline__ ResSys res_sys(const BatchArgs& A, int sys, int a, int n_s,
+                                          const TV* ell_val, const void* dinv, int dinv_size) {
   ResSys S;
   S.a = a;
   S.n_s = n_s;
-  S.k_base = __builtin_amdgcn_readfirstlane(A.rp[a]);
-  const int nnz_s = __builtin_amdgcn_readfirstlane(A.rp[a + n_s]) - S.k_base;
-  S.rp = make_rsrc(A.rp + a, n_s + 1);
-  S.col = make_rsrc(A.col + S.k_base, nnz_s);
-  S.val = make_rsrc(val + S.k_base, nnz_s);
+  S.w = __builtin_amdgcn_readfirstlane(A.ell_w[sys]);
+  const int e0 = __builtin_amdgcn_readfirstlane(A.ell_e0[sys]);
+  const int ne = __builtin_amdgcn_readfirstlane(A.ell_e0[sys + 1]) - e0;
+  S.col = make_rsrc(A.ell_col + e0, ne);
+  S.val = make_rsrc(ell_val + e0, ne);
   S.b = make_rsrc(A.b + a, n_s);
   S.x = make_rsrc(A.x + a, n_s);
   S.dinv = __builtin_amdgcn_make_buffer_rsrc(
@@ -387,69 +408,66 @@ __device__ __forceinline__ ResSys res_sys(const BatchArgs& A, int a, int n_s, co
   return S;
 }
 
-// packed row starts and lengths of this thread's slots (read back only by
-// the same thread, so no barrier is needed)
-__device__ __forceinline__ void res_rows(const ResSys& S, unsigned* rinfo) {
-  const int t = threadIdx.x;
+// block tree of K sums into one of two alternating LDS buffers: the buffer
+// written here was last read before the previous reduction's barrier
+template <int K>
+__device__ __forceinline__ void res_sum(double (&v)[K], double* red, int& parity) {
 #pragma unroll
-  for (int j = 0; j < kResRpt; ++j) {
-    unsigned ri = 0;
-    if (t < S.n_s - kResBlock * j) {
-      const int so = kResBlock * j * 4;
-      const int e0 = bld<int>(S.rp, t * 4, so), e1 = bld<int>(S.rp, t * 4 + 4, so);
-      ri = ((unsigned)(e0 - S.k_base) << 8) | (unsigned)(e1 - e0);
-    }
-    rinfo[kResBlock * j + t] = ri;
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  double* buf = red + parity * kResRed;
+  parity ^= 1;
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  if (l == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) buf[k * (kResBlock / kWave) + w] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < kResBlock / kWave; ++i) s += buf[k * (kResBlock / kWave) + i];
+    v[k] = s;
   }
 }
 
-template <int K>
-__device__ __forceinline__ void res_sum(double (&v)[K], double* red) {
-  block_sum_k<kResBlock, K>(v, red);
-}
-
-// q_j = sum_k a_k x[c_k] for the slots of this thread; x is the system's
-// vector, local index c - a, read from LDS (xl) or through a descriptor (xg).
-// GRP slots at a time, SU row positions per round of loads.
-template <int GRP, int SU, typename T, typename TV, bool LDSX>
-__device__ __forceinline__ void res_spmv(const ResSys& S, const unsigned* __restrict__ rinfo,
-                                         const T* __restrict__ xl, T (&q)[kResRpt]) {
-  const int t = threadIdx.x;
+// q_j = sum_k a_k x[c_k] for the slots of this thread.  x is the system's
+// vector: LDS words of stride 8 bytes at xl (LDSX) or the global vector S.x.
+template <typename T, typename TV, bool LDSX>
+__device__ __forceinline__ void res_spmv(const ResSys& S, const char* __restrict__ xl,
+                                         T (&q)[kResRpt]) {
+  const int lane = threadIdx.x % kResChunk;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kResChunk);
+  constexpr int kWaves = kResBlock / kResChunk;
 #pragma unroll
   for (int j = 0; j < kResRpt; ++j) q[j] = T(0);
 #pragma unroll
-  for (int g = 0; g < kResRpt; g += GRP) {
-    if (S.n_s <= kResBlock * g) break;  // uniform: no rows left in this group
-    int k0[GRP], len[GRP], mx = 0;
+  for (int g = 0; g < kResRpt; g += kResGrp) {
+    if (S.n_s <= kResBlock * g) break;  // uniform: no rows left
+    for (int s0 = 0; s0 < S.w; s0 += kResSu) {
+      TV v[kResGrp][kResSu];
+      int c[kResGrp][kResSu];
 #pragma unroll
-    for (int j = 0; j < GRP; ++j) {
-      const unsigned ri = rinfo[kResBlock * (g + j) + t];  // 0 for a slot beyond the system
-      k0[j] = (int)(ri >> 8);
-      len[j] = (int)(ri & 255u);
-      mx = max(mx, len[j]);
-    }
-    // every load is issued (past the row's end it reads other entries of the
-    // system, or 0 past the system), the sum takes only the row's own
-    // entries, in row order
-    for (int s = 0; s < mx; s += SU) {
-      TV v[SU][GRP];
-      int c[SU][GRP];
+      for (int j = 0; j < kResGrp; ++j) {
+        const int cl = wv + kWaves * (g + j);  // this wave's chunk for slot g + j
+        const bool live = kResChunk * cl < S.n_s;
+        const int base = (cl * S.w + s0) * kResChunk;  // entry offset, uniform
 #pragma unroll
-      for (int u = 0; u < SU; ++u)
-#pragma unroll
-        for (int j = 0; j < GRP; ++j) {
-          v[u][j] = bld<TV>(S.val, (k0[j] + s + u) * (int)sizeof(TV), 0);
-          c[u][j] = bld<int>(S.col, (k0[j] + s + u) * 4, 0);
+        for (int u = 0; u < kResSu; ++u) {
+          v[j][u] = TV(-0.0);  // a position past the width adds -0.0 as padding does
+          c[j][u] = kResPad * 8;
+          if (live && s0 + u < S.w) {
+            v[j][u] = bld<TV>(S.val, lane * (int)sizeof(TV), (base + u * kResChunk) * (int)sizeof(TV));
+            c[j][u] = bld<int>(S.col, lane * 4, (base + u * kResChunk) * 4);
+          }
         }
+      }
 #pragma unroll
-      for (int u = 0; u < SU; ++u)
+      for (int j = 0; j < kResGrp; ++j)
 #pragma unroll
-        for (int j = 0; j < GRP; ++j) {
-          const bool on = s + u < len[j];
-          const int ci = on ? c[u][j] - S.a : 0;
-          const T xv = LDSX ? xl[ci] : (T)bld<double>(S.x, ci * 8, 0);
-          const T prod = (T)v[u][j] * xv;
-          q[g + j] = on ? q[g + j] + prod : q[g + j];
+        for (int u = 0; u < kResSu; ++u) {
+          const T xv = LDSX ? *(const T*)(xl + c[j][u]) : (T)bld<double>(S.x, c[j][u], 0);
+          q[g + j] += (T)v[j][u] * xv;  // padding: -0.0 * +0.0 = -0.0
         }
     }
   }
@@ -457,11 +475,12 @@ __device__ __forceinline__ void res_spmv(const ResSys& S, const unsigned* __rest
 
 template <int PREC>
 __global__ __launch_bounds__(kResBlock) void k_batch_cg64_res(BatchArgs A) {
-  __shared__ double p[kResBlock * kResRpt];
-  __shared__ unsigned rinfo[kResBlock * kResRpt];
-  __shared__ double red[3 * kResBlock / kWave];
+  __shared__ double p[kResPad + 1];
+  __shared__ double red[2 * kResRed];
   __shared__ int s_w;
   const int t = threadIdx.x;
+  int parity = 0;
+  if (t == 0) p[kResPad] = 0.0;
   for (;;) {
     if (t == 0) s_w = atomicAdd(A.counter, 1);
     __syncthreads();
@@ -471,13 +490,12 @@ __global__ __launch_bounds__(kResBlock) void k_batch_cg64_res(BatchArgs A) {
     const int sys = __builtin_amdgcn_readfirstlane(A.order[w]);
     const int a = __builtin_amdgcn_readfirstlane(A.sys_rows[sys]);
     const int n_s = __builtin_amdgcn_readfirstlane(A.sys_rows[sys + 1]) - a;
-    const ResSys S = res_sys<double>(A, a, n_s, A.val, A.dinv, 8);
-    res_rows(S, rinfo);
+    const ResSys S = res_sys<double>(A, sys, a, n_s, A.ell_val, A.dinv, 8);
     const auto own = [&](int j) { return t < n_s - kResBlock * j; };
     const auto dinv = [&](int j) { return bld<double>(S.dinv, t * 8, kResBlock * j * 8); };
     double r[kResRpt], q[kResRpt];
     // r = b - A x (CG.h:115), x gathered through its descriptor
-    res_spmv<kResGrp64, kResSu64, double, double, false>(S, rinfo, nullptr, q);
+    res_spmv<double, double, false>(S, nullptr, q);
     double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < kResRpt; ++j) {
@@ -490,7 +508,7 @@ __global__ __launch_bounds__(kResBlock) void k_batch_cg64_res(BatchArgs A) {
         v[2] += ri * (PREC == OFH_DIAG_PREC ? ri * dinv(j) : ri);
       }
     }
-    res_sum<3>(v, red);
+    res_sum<3>(v, red, parity);
     double nrm = sqrt(v[0]);
     if (nrm == 0) nrm = 1;
     double res = sqrt(v[1]) / nrm, rho = v[2], rho_1 = 1.0;
@@ -501,7 +519,7 @@ __global__ __launch_bounds__(kResBlock) void k_batch_cg64_res(BatchArgs A) {
       st = -(A.max_it + 1);
       for (it = 1; it <= A.max_it; ++it) {
         const double beta = it == 1 ? 0.0 : rho / rho_1;
-        // the reductions end in barriers: nobody reads the old p any more
+        // every SpMV read of the old p came before the last reduction's barrier
 #pragma unroll
         for (int j = 0; j < kResRpt; ++j) {
           if (own(j)) {
@@ -511,12 +529,12 @@ __global__ __launch_bounds__(kResBlock) void k_batch_cg64_res(BatchArgs A) {
           }
         }
         __syncthreads();
-        res_spmv<kResGrp64, kResSu64, double, double, true>(S, rinfo, p, q);
+        res_spmv<double, double, true>(S, (const char*)p, q);
         double pq[1] = {0.0};
 #pragma unroll
         for (int j = 0; j < kResRpt; ++j)
           if (own(j)) pq[0] += p[kResBlock * j + t] * q[j];
-        res_sum<1>(pq, red);
+        res_sum<1>(pq, red, parity);
         const double alpha = rho / pq[0];
         double u[2] = {0.0, 0.0};
 #pragma unroll
@@ -530,7 +548,7 @@ __global__ __launch_bounds__(kResBlock) void k_batch_cg64_res(BatchArgs A) {
             u[1] += ri * (PREC == OFH_DIAG_PREC ? ri * dinv(j) : ri);
           }
         }
-        res_sum<2>(u, red);
+        res_sum<2>(u, red, parity);
         res = sqrt(u[0]) / nrm;
         if (res <= A.tol) {
           st = it;
@@ -551,15 +569,17 @@ __global__ __launch_bounds__(kResBlock) void k_batch_cg64_res(BatchArgs A) {
   }
 }
 
-// Mixed resident form: FP32 p in LDS, FP32 r, correction and q in registers,
-// FP64 residual replacement with x64 read from and written to global memory.
+// Mixed resident form: FP32 p in LDS (one word per 8 bytes, matching the
+// byte-offset columns), FP32 r, correction and q in registers, FP64 residual
+// replacement with x64 read from and written to global memory.
 template <int PREC>
 __global__ __launch_bounds__(kResBlock) void k_batch_cgmx_res(BatchArgs A) {
-  __shared__ float p[kResBlock * kResRpt];
-  __shared__ unsigned rinfo[kResBlock * kResRpt];
-  __shared__ double red[3 * kResBlock / kWave];
+  __shared__ float p2[2 * (kResPad + 1)];
+  __shared__ double red[2 * kResRed];
   __shared__ int s_w;
   const int t = threadIdx.x;
+  int parity = 0;
+  if (t == 0) p2[2 * kResPad] = 0.0f;
   for (;;) {
     if (t == 0) s_w = atomicAdd(A.counter, 1);
     __syncthreads();
@@ -568,11 +588,11 @@ __global__ __launch_bounds__(kResBlock) void k_batch_cgmx_res(BatchArgs A) {
     const int sys = __builtin_amdgcn_readfirstlane(A.order[w]);
     const int a = __builtin_amdgcn_readfirstlane(A.sys_rows[sys]);
     const int n_s = __builtin_amdgcn_readfirstlane(A.sys_rows[sys + 1]) - a;
-    const ResSys S = res_sys<float>(A, a, n_s, A.val32, A.dinv32, 4);
-    const ResSys S64 = res_sys<double>(A, a, n_s, A.val, A.dinv32, 4);
-    res_rows(S, rinfo);
+    const ResSys S = res_sys<float>(A, sys, a, n_s, A.ell_val32, A.dinv32, 4);
+    const ResSys S64 = res_sys<double>(A, sys, a, n_s, A.ell_val, A.dinv32, 4);
     const auto own = [&](int j) { return t < n_s - kResBlock * j; };
     const auto dinv = [&](int j) { return bld<float>(S.dinv, t * 4, kResBlock * j * 4); };
+    const auto pw = [&](int j) -> float& { return p2[2 * (kResBlock * j + t)]; };
     float r[kResRpt], x32[kResRpt], q[kResRpt];
     double nrm = 0, res = 0, rho = 0, rho_1 = 1.0, rmax = 0;
     int it = 0, st = 0;
@@ -581,7 +601,7 @@ __global__ __launch_bounds__(kResBlock) void k_batch_cgmx_res(BatchArgs A) {
       double v[3] = {0.0, 0.0, 0.0};
       {
         double rq[kResRpt];  // FP64 row sums of A x64
-        res_spmv<kResGrp64, kResSu64, double, double, false>(S64, rinfo, nullptr, rq);
+        res_spmv<double, double, false>(S64, nullptr, rq);
 #pragma unroll
         for (int j = 0; j < kResRpt; ++j) {
           r[j] = 0.0f;
@@ -597,7 +617,7 @@ __global__ __launch_bounds__(kResBlock) void k_batch_cgmx_res(BatchArgs A) {
           }
         }
       }
-      res_sum<3>(v, red);
+      res_sum<3>(v, red, parity);
       if (first) {
         nrm = sqrt(v[0]);
         if (nrm == 0) nrm = 1;
@@ -619,26 +639,25 @@ __global__ __launch_bounds__(kResBlock) void k_batch_cgmx_res(BatchArgs A) {
       for (int j = 0; j < kResRpt; ++j) {
         if (own(j)) {
           const float zi = PREC == OFH_DIAG_PREC ? r[j] * dinv(j) : r[j];
-          float& pj = p[kResBlock * j + t];
-          pj = first ? zi : zi + (float)beta * pj;
+          pw(j) = first ? zi : zi + (float)beta * pw(j);
         }
       }
       first = false;
       for (;;) {  // FP32 inner iterations until the recursive residual drops by kBatchDelta
         __syncthreads();
         ++it;
-        res_spmv<kResGrp32, kResSu32, float, float, true>(S, rinfo, p, q);
+        res_spmv<float, float, true>(S, (const char*)p2, q);
         double pq[1] = {0.0};
 #pragma unroll
         for (int j = 0; j < kResRpt; ++j)
-          if (own(j)) pq[0] += (double)p[kResBlock * j + t] * (double)q[j];
-        res_sum<1>(pq, red);
+          if (own(j)) pq[0] += (double)pw(j) * (double)q[j];
+        res_sum<1>(pq, red, parity);
         const float alpha = (float)(rho / pq[0]);
         double u[2] = {0.0, 0.0};
 #pragma unroll
         for (int j = 0; j < kResRpt; ++j) {
           if (own(j)) {
-            x32[j] += alpha * p[kResBlock * j + t];
+            x32[j] += alpha * pw(j);
             const float ri = r[j] - alpha * q[j];
             r[j] = ri;
             const float zi = PREC == OFH_DIAG_PREC ? ri * dinv(j) : ri;
@@ -646,7 +665,7 @@ __global__ __launch_bounds__(kResBlock) void k_batch_cgmx_res(BatchArgs A) {
             u[1] += (double)ri * (double)zi;
           }
         }
-        res_sum<2>(u, red);
+        res_sum<2>(u, red, parity);
         const double rec = sqrt(u[0]) / nrm;
         rho_1 = rho;
         if (rec <= kBatchDelta * rmax || rec <= A.tol || it >= A.max_it || !(rec == rec)) break;
@@ -656,8 +675,7 @@ __global__ __launch_bounds__(kResBlock) void k_batch_cgmx_res(BatchArgs A) {
         for (int j = 0; j < kResRpt; ++j) {
           if (own(j)) {
             const float zi = PREC == OFH_DIAG_PREC ? r[j] * dinv(j) : r[j];
-            float& pj = p[kResBlock * j + t];
-            pj = zi + (float)beta2 * pj;
+            pw(j) = zi + (float)beta2 * pw(j);
           }
         }
       }
@@ -678,6 +696,19 @@ __global__ __launch_bounds__(kResBlock) void k_batch_cgmx_res(BatchArgs A) {
       A.relres[sys] = res;
     }
     __syncthreads();
+  }
+}
+
+// ELL values from the CSR values (padding -0.0); the FP32 copy rounds
+// exactly as the CSR's FP32 mirror does
+__global__ void k_ell_gather(int64_t n, const int* __restrict__ src, const double* __restrict__ val,
+                             double* __restrict__ sv, float* __restrict__ sv32) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int k = src[i];
+    const double v = k >= 0 ? val[k] : -0.0;
+    sv[i] = v;
+    sv32[i] = (float)v;
   }
 }
 
@@ -737,8 +768,14 @@ static int batch_run(ofh_batch B, int prec, int mixed, const double* b, double* 
   LaunchFn fn = pick_fn(prec, mixed != 0, block, tpr);
   OFH_CHECK(fn, OFH_E_INVALID, "batched CG: unsupported launch shape");
   OFH_CHECK(tpr != 0 || B->resident_ok, OFH_E_INVALID,
-            "batched CG: the resident form needs systems of at most 12288 rows, 2^24 "
-            "nonzeros and rows of at most 255 entries");
+            "batched CG: the resident form needs systems of at most 12288 rows");
+  if (tpr == 0 && B->ell_version != M->version) {  // refresh the ELL values
+    const int g = (int)std::min<int64_t>((B->ell_n + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_ell_gather, dim3(std::max(g, 1)), dim3(256), 0, s, B->ell_n,
+                       B->ell_src, M->val, B->ell_val, B->ell_val32);
+    OFH_HIP(hipGetLastError());
+    B->ell_version = M->version;
+  }
   BatchArgs a{};
   a.nsys = B->nsys;
   a.sys_rows = B->sys_rows;
@@ -748,6 +785,11 @@ static int batch_run(ofh_batch B, int prec, int mixed, const double* b, double* 
   a.col = M->col;
   a.val = M->val;
   a.val32 = M->val32;
+  a.ell_e0 = B->ell_e0;
+  a.ell_w = B->ell_w;
+  a.ell_col = B->ell_col;
+  a.ell_val = B->ell_val;
+  a.ell_val32 = B->ell_val32;
   a.dinv = M->dinv;
   a.dinv32 = M->dinv32;
   a.b = b;
@@ -827,13 +869,43 @@ int ofh_batch_create(ofh_csr A, int nsys, const int32_t* h_sys_rows, ofh_batch* 
   B->A = A;
   B->nsys = nsys;
   B->h_rows.assign(h_sys_rows, h_sys_rows + nsys + 1);
-  int max_len = 0, max_nnz = 0;
-  for (int k = 0; k < nsys; ++k) {
+  for (int k = 0; k < nsys; ++k)
     B->max_rows = std::max(B->max_rows, h_sys_rows[k + 1] - h_sys_rows[k]);
-    max_nnz = std::max(max_nnz, rp[h_sys_rows[k + 1]] - rp[h_sys_rows[k]]);
+  // ELL layout of the resident form: system k has width w_k (its longest
+  // row); chunk c holds local rows 64c..64c+63, entry s of row `lane` at
+  // e0_k + (c w_k + s) 64 + lane; columns as byte offsets (x 8)
+  std::vector<int> h_e0, h_w, h_col, h_src;
+  if (B->max_rows <= kResBlock * kResRpt) {
+    h_e0.resize(nsys + 1);
+    h_w.resize(nsys);
+    int64_t e = 0;
+    for (int k = 0; k < nsys; ++k) {
+      const int a = h_sys_rows[k], n_s = h_sys_rows[k + 1] - a;
+      int w = 0;
+      for (int l = 0; l < n_s; ++l) w = std::max(w, rp[a + l + 1] - rp[a + l]);
+      h_e0[k] = (int)std::min<int64_t>(e, INT32_MAX);
+      h_w[k] = w;
+      e += (int64_t)((n_s + kResChunk - 1) / kResChunk) * kResChunk * w;
+    }
+    h_e0[nsys] = (int)std::min<int64_t>(e, INT32_MAX);
+    if (e < INT32_MAX) {
+      B->resident_ok = true;
+      B->ell_n = e;
+      h_col.assign(e, kResPad * 8);
+      h_src.assign(e, -1);
+      for (int k = 0; k < nsys; ++k) {
+        const int a = h_sys_rows[k], n_s = h_sys_rows[k + 1] - a;
+        for (int l = 0; l < n_s; ++l) {
+          const int64_t base = h_e0[k] + (int64_t)(l / kResChunk) * kResChunk * h_w[k] + l % kResChunk;
+          for (int q = rp[a + l]; q < rp[a + l + 1]; ++q) {
+            const int64_t at = base + (int64_t)kResChunk * (q - rp[a + l]);
+            h_col[at] = (col[q] - a) * 8;
+            h_src[at] = q;
+          }
+        }
+      }
+    }
   }
-  for (int i = 0; i < A->n; ++i) max_len = std::max(max_len, rp[i + 1] - rp[i]);
-  B->resident_ok = B->max_rows <= kResBlock * kResRpt && max_len <= kResMaxLen && max_nnz < kResMaxNnz;
 
   hipStream_t s = A->ctx->stream;
   OFH_HIP(hipMalloc(&B->sys_rows, sizeof(int) * (nsys + 1)));
@@ -844,6 +916,19 @@ int ofh_batch_create(ofh_csr A, int nsys, const int32_t* h_sys_rows, ofh_batch* 
   OFH_HIP(hipMalloc(&B->relres, sizeof(double) * nsys));
   OFH_HIP(hipMemcpyAsync(B->sys_rows, h_sys_rows, sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, s));
   OFH_HIP(hipMemcpyAsync(B->order, order.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, s));
+  if (B->resident_ok) {
+    const size_t ne = std::max<size_t>(B->ell_n, 1);
+    OFH_HIP(hipMalloc(&B->ell_e0, sizeof(int) * (nsys + 1)));
+    OFH_HIP(hipMalloc(&B->ell_w, sizeof(int) * nsys));
+    OFH_HIP(hipMalloc(&B->ell_col, sizeof(int) * ne));
+    OFH_HIP(hipMalloc(&B->ell_src, sizeof(int) * ne));
+    OFH_HIP(hipMalloc(&B->ell_val, sizeof(double) * ne));
+    OFH_HIP(hipMalloc(&B->ell_val32, sizeof(float) * ne));
+    OFH_HIP(hipMemcpyAsync(B->ell_e0, h_e0.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, s));
+    OFH_HIP(hipMemcpyAsync(B->ell_w, h_w.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, s));
+    OFH_HIP(hipMemcpyAsync(B->ell_col, h_col.data(), sizeof(int) * B->ell_n, hipMemcpyHostToDevice, s));
+    OFH_HIP(hipMemcpyAsync(B->ell_src, h_src.data(), sizeof(int) * B->ell_n, hipMemcpyHostToDevice, s));
+  }
 
   OFH_HIP(hipStreamSynchronize(s));
   *out = B;
@@ -853,7 +938,9 @@ int ofh_batch_create(ofh_csr A, int nsys, const int32_t* h_sys_rows, ofh_batch* 
 int ofh_batch_destroy(ofh_batch B) {
   if (!B) return OFH_OK;
   hipDeviceSynchronize();
-  void* ptrs[] = {B->sys_rows, B->order, B->counter, B->iters, B->status, B->relres, B->work};
+  void* ptrs[] = {B->sys_rows, B->order,  B->counter, B->iters,   B->status,  B->relres,
+                  B->work,     B->ell_e0, B->ell_w,   B->ell_col, B->ell_src, B->ell_val,
+                  B->ell_val32};
   for (void* p : ptrs) hipFree(p);
   if (B->gexec) hipGraphExecDestroy(B->gexec);
   if (B->cap) hipStreamDestroy(B->cap);
@@ -867,8 +954,7 @@ int ofh_batch_set_config(ofh_batch B, int block_threads, int threads_per_row) {
             "ofh_batch_set_config: block in {256,512,1024}, threads per row in {1,2,4,8}, "
             "or (1024, 0) for the resident form");
   OFH_CHECK(threads_per_row != 0 || B->resident_ok, OFH_E_INVALID,
-            "ofh_batch_set_config: the resident form needs systems of at most 12288 rows, "
-            "2^24 nonzeros and rows of at most 255 entries");
+            "ofh_batch_set_config: the resident form needs systems of at most 12288 rows");
   B->block = block_threads;
   B->tpr = threads_per_row;
   B->tuned = true;

@@ -462,6 +462,7 @@ int ofh_csr_get_pattern(ofh_csr A, int32_t* rp, int32_t* col) {
 static void invalidate_values(ofh_csr A) {
   A->val32_valid = false;
   A->dinv_valid = false;
+  ++A->version;
 }
 
 int ofh_csr_set_values(ofh_csr A, const double* h) {

@@ -212,6 +212,7 @@ struct ofh_csr_s {
   float* dinv32 = nullptr;
   int* diag_pos = nullptr;    // device, n (position of a_ii; lazy)
   bool dinv_valid = false, val32_valid = false;
+  uint64_t version = 0;  // bumped whenever the values may have changed
   // level schedule for the SSOR sweeps (lazy)
   int* lvl_rows_fwd = nullptr;
   int* lvl_ptr_fwd = nullptr;

@@ -181,3 +181,33 @@ def test_batch_resident_size_limit(ctx):
     x = torch.zeros(P.nb_eq, dtype=torch.float64, device="cuda")
     blk, tpr, ms = B.autotune(b, x, max_it=3000, toler=1e-8)
     assert tpr in (1, 2, 4, 8)
+
+
+def test_batch_resident_follows_new_values(ctx):
+    """The resident form reads an ELL copy of the matrix; re-assembling the
+    matrix with new coefficients after a solve refreshes that copy, so the
+    next resident solve again equals the streamed one bitwise."""
+    import ofeli_amd as oa
+    torch = _t()
+    sizes = (20, 9, 31)
+    P, b, sys_rows, _, ks, fs = small_batch(ctx, sizes=sizes, seed=5)
+    B = oa.Batch(P.A, sys_rows)
+    B.set_config(1024, 0)
+    x = torch.zeros(P.nb_eq, dtype=torch.float64, device="cuda")
+    B.cg(b, x, max_it=2000, toler=1e-9)
+    rng = np.random.default_rng(11)
+    k2 = np.concatenate([np.exp(0.5 * rng.standard_normal(2 * n * n)) for n in sizes])
+    f2 = np.concatenate([1.0 + rng.random((n + 1) ** 2) for n in sizes])
+    b.zero_()
+    P.build(oa.DC2DT3, coef=torch.as_tensor(k2, device="cuda"), f=torch.as_tensor(f2, device="cuda"),
+            matrix=True, b=b)
+    out = {}
+    for tpr in (0, 1):
+        B.set_config(1024, tpr)
+        for mixed in (False, True):
+            x = torch.zeros(P.nb_eq, dtype=torch.float64, device="cuda")
+            it, st, rr, _ = B.cg(b, x, mixed=mixed, max_it=2000, toler=1e-9)
+            out[(tpr, mixed)] = (x, it)
+    for mixed in (False, True):
+        assert torch.equal(out[(0, mixed)][0], out[(1, mixed)][0])
+        assert np.array_equal(out[(0, mixed)][1], out[(1, mixed)][1])
