@@ -69,6 +69,7 @@ struct ofh_batch_s {
   float* ell_val32 = nullptr;
   int64_t ell_n = 0;
   uint64_t ell_version = ~0ull;  // CSR values version gathered into ell_val*
+  std::vector<int> h_ell_e0, h_ell_w;  // host copies of the ELL offsets and widths
   bool tuned = false;
   bool use_graph = true;         // replay the launch from a captured graph
   hipGraphExec_t gexec = nullptr;
@@ -752,6 +753,55 @@ static LaunchFn pick_fn(int prec, bool mixed, int block, int tpr) {
   return mixed ? pick<OFH_IDENT_PREC, true>(block, tpr) : pick<OFH_IDENT_PREC, false>(block, tpr);
 }
 
+// The ELL copy of the resident form: columns (byte offsets of the local
+// column, padding -> kResPad) and the CSR entry of every slot; the values are
+// gathered from the CSR values by k_ell_gather whenever they may have changed.
+static int ell_build(ofh_batch B) {
+  if (B->ell_col) return OFH_OK;
+  // drop what an earlier, failed build left
+  for (void** p : {(void**)&B->ell_e0, (void**)&B->ell_w, (void**)&B->ell_src,
+                   (void**)&B->ell_val, (void**)&B->ell_val32}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  ofh_csr A = B->A;
+  const int nsys = B->nsys;
+  const std::vector<int>& sr = B->h_rows;
+  std::vector<int> rp(A->n + 1), col(A->nnz);
+  int rc = ofh_csr_get_pattern(A, rp.data(), col.data());
+  if (rc) return rc;
+  std::vector<int> h_col(B->ell_n, kResPad * 8), h_src(B->ell_n, -1);
+  for (int k = 0; k < nsys; ++k) {
+    const int a = sr[k], n_s = sr[k + 1] - a, w = B->h_ell_w[k];
+    for (int l = 0; l < n_s; ++l) {
+      const int64_t base = B->h_ell_e0[k] + (int64_t)(l / kResChunk) * kResChunk * w + l % kResChunk;
+      for (int q = rp[a + l]; q < rp[a + l + 1]; ++q) {
+        const int64_t at = base + (int64_t)kResChunk * (q - rp[a + l]);
+        h_col[at] = (col[q] - a) * 8;
+        h_src[at] = q;
+      }
+    }
+  }
+  hipStream_t s = A->ctx->stream;
+  const size_t ne = std::max<size_t>(B->ell_n, 1);
+  OFH_HIP(hipMalloc(&B->ell_e0, sizeof(int) * (nsys + 1)));
+  OFH_HIP(hipMalloc(&B->ell_w, sizeof(int) * nsys));
+  OFH_HIP(hipMalloc(&B->ell_src, sizeof(int) * ne));
+  OFH_HIP(hipMalloc(&B->ell_val, sizeof(double) * ne));
+  OFH_HIP(hipMalloc(&B->ell_val32, sizeof(float) * ne));
+  OFH_HIP(hipMemcpyAsync(B->ell_e0, B->h_ell_e0.data(), sizeof(int) * (nsys + 1),
+                         hipMemcpyHostToDevice, s));
+  OFH_HIP(hipMemcpyAsync(B->ell_w, B->h_ell_w.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, s));
+  OFH_HIP(hipMemcpyAsync(B->ell_src, h_src.data(), sizeof(int) * B->ell_n, hipMemcpyHostToDevice, s));
+  int* dcol = nullptr;
+  OFH_HIP(hipMalloc(&dcol, sizeof(int) * ne));
+  OFH_HIP(hipMemcpyAsync(dcol, h_col.data(), sizeof(int) * B->ell_n, hipMemcpyHostToDevice, s));
+  OFH_HIP(hipStreamSynchronize(s));  // the host vectors are released on return
+  B->ell_col = dcol;  // set last: marks the copy complete
+  B->ell_version = ~0ull;
+  return OFH_OK;
+}
+
 static int batch_run(ofh_batch B, int prec, int mixed, const double* b, double* x, int max_it,
                      double toler, int block, int tpr, hipStream_t s, bool graph) {
   ofh_csr M = B->A;
@@ -769,6 +819,7 @@ static int batch_run(ofh_batch B, int prec, int mixed, const double* b, double* 
   OFH_CHECK(fn, OFH_E_INVALID, "batched CG: unsupported launch shape");
   OFH_CHECK(tpr != 0 || B->resident_ok, OFH_E_INVALID,
             "batched CG: the resident form needs systems of at most 12288 rows");
+  if (tpr == 0 && (rc = ell_build(B))) return rc;
   if (tpr == 0 && B->ell_version != M->version) {  // refresh the ELL values
     const int g = (int)std::min<int64_t>((B->ell_n + 255) / 256, 65535);
     hipLaunchKernelGGL(k_ell_gather, dim3(std::max(g, 1)), dim3(256), 0, s, B->ell_n,
@@ -871,40 +922,24 @@ int ofh_batch_create(ofh_csr A, int nsys, const int32_t* h_sys_rows, ofh_batch* 
   B->h_rows.assign(h_sys_rows, h_sys_rows + nsys + 1);
   for (int k = 0; k < nsys; ++k)
     B->max_rows = std::max(B->max_rows, h_sys_rows[k + 1] - h_sys_rows[k]);
-  // ELL layout of the resident form: system k has width w_k (its longest
-  // row); chunk c holds local rows 64c..64c+63, entry s of row `lane` at
-  // e0_k + (c w_k + s) 64 + lane; columns as byte offsets (x 8)
-  std::vector<int> h_e0, h_w, h_col, h_src;
+  // ELL layout of the resident form (built on its first use, ell_build):
+  // system k has width w_k (its longest row); chunk c holds local rows
+  // 64c..64c+63, entry s of row `lane` at e0_k + (c w_k + s) 64 + lane
   if (B->max_rows <= kResBlock * kResRpt) {
-    h_e0.resize(nsys + 1);
-    h_w.resize(nsys);
+    B->h_ell_e0.resize(nsys + 1);
+    B->h_ell_w.resize(nsys);
     int64_t e = 0;
     for (int k = 0; k < nsys; ++k) {
       const int a = h_sys_rows[k], n_s = h_sys_rows[k + 1] - a;
       int w = 0;
       for (int l = 0; l < n_s; ++l) w = std::max(w, rp[a + l + 1] - rp[a + l]);
-      h_e0[k] = (int)std::min<int64_t>(e, INT32_MAX);
-      h_w[k] = w;
+      B->h_ell_e0[k] = (int)std::min<int64_t>(e, INT32_MAX);
+      B->h_ell_w[k] = w;
       e += (int64_t)((n_s + kResChunk - 1) / kResChunk) * kResChunk * w;
     }
-    h_e0[nsys] = (int)std::min<int64_t>(e, INT32_MAX);
-    if (e < INT32_MAX) {
-      B->resident_ok = true;
-      B->ell_n = e;
-      h_col.assign(e, kResPad * 8);
-      h_src.assign(e, -1);
-      for (int k = 0; k < nsys; ++k) {
-        const int a = h_sys_rows[k], n_s = h_sys_rows[k + 1] - a;
-        for (int l = 0; l < n_s; ++l) {
-          const int64_t base = h_e0[k] + (int64_t)(l / kResChunk) * kResChunk * h_w[k] + l % kResChunk;
-          for (int q = rp[a + l]; q < rp[a + l + 1]; ++q) {
-            const int64_t at = base + (int64_t)kResChunk * (q - rp[a + l]);
-            h_col[at] = (col[q] - a) * 8;
-            h_src[at] = q;
-          }
-        }
-      }
-    }
+    B->h_ell_e0[nsys] = (int)std::min<int64_t>(e, INT32_MAX);
+    B->resident_ok = e < INT32_MAX;
+    B->ell_n = e;
   }
 
   hipStream_t s = A->ctx->stream;
@@ -916,19 +951,6 @@ int ofh_batch_create(ofh_csr A, int nsys, const int32_t* h_sys_rows, ofh_batch* 
   OFH_HIP(hipMalloc(&B->relres, sizeof(double) * nsys));
   OFH_HIP(hipMemcpyAsync(B->sys_rows, h_sys_rows, sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, s));
   OFH_HIP(hipMemcpyAsync(B->order, order.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, s));
-  if (B->resident_ok) {
-    const size_t ne = std::max<size_t>(B->ell_n, 1);
-    OFH_HIP(hipMalloc(&B->ell_e0, sizeof(int) * (nsys + 1)));
-    OFH_HIP(hipMalloc(&B->ell_w, sizeof(int) * nsys));
-    OFH_HIP(hipMalloc(&B->ell_col, sizeof(int) * ne));
-    OFH_HIP(hipMalloc(&B->ell_src, sizeof(int) * ne));
-    OFH_HIP(hipMalloc(&B->ell_val, sizeof(double) * ne));
-    OFH_HIP(hipMalloc(&B->ell_val32, sizeof(float) * ne));
-    OFH_HIP(hipMemcpyAsync(B->ell_e0, h_e0.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, s));
-    OFH_HIP(hipMemcpyAsync(B->ell_w, h_w.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, s));
-    OFH_HIP(hipMemcpyAsync(B->ell_col, h_col.data(), sizeof(int) * B->ell_n, hipMemcpyHostToDevice, s));
-    OFH_HIP(hipMemcpyAsync(B->ell_src, h_src.data(), sizeof(int) * B->ell_n, hipMemcpyHostToDevice, s));
-  }
 
   OFH_HIP(hipStreamSynchronize(s));
   *out = B;
