@@ -180,8 +180,8 @@ int ofh_batch_cg(ofh_batch B, int prec, int mixed, const double* d_b, double* d_
 int ofh_batch_use_graph(ofh_batch B, int enable);
 /* Launch shape: workgroup size {256,512,1024} x threads per row {1,2,4,8},
  * or (1024, 0): the resident form, one workgroup per system with the
- * direction in LDS, r in registers and the matrix read from a sliced-ELL
- * copy made at ofh_batch_create (every system at most 12288 rows).  Both
+ * direction in LDS, r in registers and the matrix read from an ELL copy
+ * made on its first use (every system at most 12288 rows).  Both
  * forms give bitwise the same iterates at workgroup size 1024. */
 int ofh_batch_set_config(ofh_batch B, int block_threads, int threads_per_row);
 int ofh_batch_get_config(ofh_batch B, int* block_threads, int* threads_per_row);
