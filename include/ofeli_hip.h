@@ -302,7 +302,11 @@ int ofh_proj_update(ofh_proj P, const double* d_x);
  * after the line search relative to the base's (scale-invariant in d); Adam
  * (0.9, 0.999, 1e-8) on FP32 master weights.
  * Parameters are one FP32 vector: W_enc[8][64], b_enc[64], per layer
- * W_ab[64][128], w_e[64], b_m[64], W_u[128][64], b_u[64]; w_dec[64], b_dec. */
+ * W_ab[64][128], w_e[64], b_m[64], W_u[128][64], b_u[64]; w_dec[64], b_dec.
+ * Objective 1 (ofh_mpnn_set_objective) aligns the start with CG's stop test
+ * on ||r||: alpha = Ad.r / Ad.Ad (minimum residual along d) and
+ * L = 1 - (Ad.r)^2 / (Ad.Ad r.r), the squared residual norm left relative to
+ * the base's; it needs no solution (d_xstar may be NULL). */
 typedef struct ofh_mpnn_s* ofh_mpnn;
 int ofh_mpnn_create(ofh_csr A, int layers, int hidden, int extrapolate, uint64_t seed,
                     ofh_mpnn* out);
@@ -314,6 +318,8 @@ int ofh_mpnn_set_params(ofh_mpnn M, const float* h_params);
 /* ofh_mpnn_guess replays its kernels from a HIP graph captured per argument
  * set (enable = 1, the default) or launches them directly. */
 int ofh_mpnn_use_graph(ofh_mpnn M, int enable);
+/* objective: 0 A-norm error (the default), 1 residual norm. */
+int ofh_mpnn_set_objective(ofh_mpnn M, int objective);
 /* x0 = x_b + alpha d; stream-ordered, no host synchronisation. */
 int ofh_mpnn_guess(ofh_mpnn M, const double* d_xn, const double* d_xp, const double* d_b,
                    double* d_x0);

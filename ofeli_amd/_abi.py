@@ -78,6 +78,7 @@ SIGNATURES = {
     "ofh_mpnn_get_params": (i32, [vp, f32p]),
     "ofh_mpnn_set_params": (i32, [vp, f32p]),
     "ofh_mpnn_use_graph": (i32, [vp, i32]),
+    "ofh_mpnn_set_objective": (i32, [vp, i32]),
     "ofh_mpnn_guess": (i32, [vp, vp, vp, vp, vp]),
     "ofh_mpnn_loss_grad": (i32, [vp, vp, vp, vp, vp, vp, f64p, f32p]),
     "ofh_mpnn_train_step": (i32, [vp, vp, vp, vp, vp, vp, C.c_float, f64p]),

@@ -437,15 +437,21 @@ def solve_with_projection(A, b, x, proj, fn=None, prec=DIAG_PREC, toler=1e-8, ti
 
 class LearnedGuess:
     """Message-passing network predicting the start vector of the next solve
-    of a sequence A x_{k+1} = b_{k+1} (ofh_mpnn_*); A symmetric."""
+    of a sequence A x_{k+1} = b_{k+1} (ofh_mpnn_*); A symmetric.
+    objective "energy": A-norm line search and loss (needs the solution x*);
+    "residual": minimum-residual line search and loss (x* not used)."""
 
-    def __init__(self, A, layers=4, hidden=64, seed=0, extrapolate=False):
+    def __init__(self, A, layers=4, hidden=64, seed=0, extrapolate=False, objective="energy"):
+        if objective not in ("energy", "residual"):
+            raise ValueError("objective is 'energy' or 'residual'")
         self.A = A
         self.layers = layers
         h = C.c_void_p()
         check(lib().ofh_mpnn_create(A.h, int(layers), int(hidden), int(bool(extrapolate)),
                                     C.c_uint64(seed), C.byref(h)))
         self.h = h
+        self.objective = objective
+        check(lib().ofh_mpnn_set_objective(self.h, int(objective == "residual")))
         cnt = C.c_int64()
         check(lib().ofh_mpnn_num_params(self.h, C.byref(cnt)))
         self.nparams = cnt.value
@@ -474,7 +480,8 @@ class LearnedGuess:
         loss = C.c_double()
         g = np.zeros(self.nparams, np.float32) if want_grad else None
         check(lib().ofh_mpnn_loss_grad(self.h, _f64(x_n, n), _f64(x_p, n), _f64(b, n),
-                                       _f64(x_star, n), _f64(x0, n), C.byref(loss),
+                                       None if x_star is None else _f64(x_star, n), _f64(x0, n),
+                                       C.byref(loss),
                                        g.ctypes.data_as(C.POINTER(C.c_float)) if want_grad else None))
         return loss.value, g
 
@@ -482,7 +489,8 @@ class LearnedGuess:
         n = self.A.n
         loss = C.c_double()
         check(lib().ofh_mpnn_train_step(self.h, _f64(x_n, n), _f64(x_p, n), _f64(b, n),
-                                        _f64(x_star, n), _f64(x0, n), float(lr),
+                                        None if x_star is None else _f64(x_star, n), _f64(x0, n),
+                                        float(lr),
                                         C.byref(loss) if want_loss else None))
         return loss.value if want_loss else None
 

@@ -460,31 +460,36 @@ __global__ __launch_bounds__(256) void k_mp_dec(int n, const float* __restrict__
   if (c == 0) dv[i] = scal[0] * (double)d;
 }
 
-// q = A d; partials d.q -> slot 0, d.r -> slot 1 (line search).
+// q = A d; line-search partials.  A-norm objective (obj 0): d.q -> slot 0,
+// d.r -> slot 1; minimum residual (obj 1): q.q -> 0, q.r -> 1, r.r -> 2.
 __global__ __launch_bounds__(256) void k_mp_ls(int n, const int* __restrict__ rp,
                                                const int* __restrict__ col,
                                                const double* __restrict__ val,
                                                const double* __restrict__ dv,
                                                const double* __restrict__ r, double* q,
-                                               double* part, int nparts) {
-  __shared__ double red[8];
+                                               double* part, int nparts, int obj) {
+  __shared__ double red[12];
   const int i = blockIdx.x * 256 + threadIdx.x;
-  double v[2] = {0.0, 0.0};
+  double v[3] = {0.0, 0.0, 0.0};
   if (i < n) {
     double a = 0.0;
     for (int k = rp[i]; k < rp[i + 1]; ++k) a += val[k] * dv[col[k]];
     q[i] = a;
-    v[0] = dv[i] * a;
-    v[1] = dv[i] * r[i];
+    const double w = obj ? a : dv[i];
+    v[0] = w * a;
+    v[1] = w * r[i];
+    v[2] = r[i] * r[i];
   }
-  block_sum_k<256, 2>(v, red);
+  block_sum_k<256, 3>(v, red);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = v[0];
     part[nparts + blockIdx.x] = v[1];
+    if (obj) part[2 * nparts + blockIdx.x] = v[2];
   }
 }
 
-// alpha = d.r / d.Ad (0 if d.Ad <= 0), x0 = x_b + alpha d.
+// alpha = slot 1 / slot 0 (0 if slot 0 <= 0): d.r / d.Ad (A-norm) or
+// Ad.r / Ad.Ad (minimum residual); x0 = x_b + alpha d.
 __global__ __launch_bounds__(256) void k_mp_apply(int n, const double* __restrict__ part,
                                                   int nparts, const double* __restrict__ xn,
                                                   const double* __restrict__ xp, double ext,
@@ -536,6 +541,52 @@ __global__ __launch_bounds__(256) void k_mp_loss(int n, const int* __restrict__ 
   }
 }
 
+// Training, minimum-residual objective: folds q.q, q.r, r.r of k_mp_ls
+// (every workgroup); L = 1 - (q.r)^2 / (q.q r.r) and dL/dd = A w with
+// w = -2 (q.r) / (q.q r.r) r + 2 (q.r)^2 / ((q.q)^2 r.r) q, formed at the
+// gather (A symmetric), into gv.
+__global__ __launch_bounds__(256) void k_mp_loss_res(int n, const int* __restrict__ rp,
+                                                     const int* __restrict__ col,
+                                                     const double* __restrict__ val,
+                                                     const double* __restrict__ r,
+                                                     const double* __restrict__ q,
+                                                     const double* __restrict__ part, int nparts,
+                                                     double* gv, double* scal) {
+  __shared__ double red[12];
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int p = threadIdx.x; p < nparts; p += 256) {
+    v[0] += part[p];
+    v[1] += part[nparts + p];
+    v[2] += part[2 * nparts + p];
+  }
+  block_sum_k<256, 3>(v, red);
+  const double qq = v[0], c = v[1], R = v[2];
+  double L, a1 = 0.0, a2 = 0.0;
+  if (!(R > 0.0)) {
+    L = 0.0;  // the base solves the system
+  } else if (!(qq > 0.0)) {
+    L = 1.0;  // no direction
+  } else {
+    L = 1.0 - c * c / (qq * R);
+    a1 = -2.0 * c / (qq * R);
+    a2 = 2.0 * c * c / (qq * qq * R);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[1] = R;
+    scal[2] = c;
+    scal[3] = L;
+  }
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    double a = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      const int j = col[k];
+      a += val[k] * (a1 * r[j] + a2 * q[j]);
+    }
+    gv[i] = a;
+  }
+}
+
 // Folds d.Ad, E0, d.g (every workgroup); L = 1 - (d.g)^2 / (d.Ad E0) and
 // dL/dd = -2 (d.g) / (d.Ad E0) (g - (d.g / d.Ad) A d); decoder backward:
 // dH = (s dL/dd) w_dec^T and partials of the decoder gradient.
@@ -544,33 +595,38 @@ __global__ __launch_bounds__(256) void k_mp_dec_bwd(int n, int n_pad, const floa
                                                     const double* __restrict__ q,
                                                     const double* __restrict__ gv,
                                                     const double* __restrict__ lpart, int nparts,
-                                                    double* scal, float* dH, float* part) {
+                                                    double* scal, float* dH, float* part,
+                                                    int obj) {
   __shared__ double dred[12];
   __shared__ float red[kH + 1];
-  double v[3] = {0.0, 0.0, 0.0};
-  for (int p = threadIdx.x; p < nparts; p += 256) {
-    v[0] += lpart[p];
-    v[1] += lpart[2 * nparts + p];
-    v[2] += lpart[3 * nparts + p];
-  }
-  block_sum_k<256, 3>(v, dred);
-  const double dAd = v[0], E0 = v[1], dg = v[2];
-  double L, c1 = 0.0, c2 = 0.0;
-  if (!(E0 > 0.0)) {
-    L = 0.0;  // the base is already the solution
-  } else if (!(dAd > 0.0)) {
-    L = 1.0;  // no direction
-  } else {
-    L = 1.0 - dg * dg / (dAd * E0);
-    c1 = -2.0 * dg / (dAd * E0);
-    c2 = dg / dAd;
+  double c1 = 1.0, c2 = 0.0;  // obj 1: gv is dL/dd already (k_mp_loss_res)
+  if (!obj) {
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int p = threadIdx.x; p < nparts; p += 256) {
+      v[0] += lpart[p];
+      v[1] += lpart[2 * nparts + p];
+      v[2] += lpart[3 * nparts + p];
+    }
+    block_sum_k<256, 3>(v, dred);
+    const double dAd = v[0], E0 = v[1], dg = v[2];
+    double L;
+    c1 = 0.0;
+    if (!(E0 > 0.0)) {
+      L = 0.0;  // the base is already the solution
+    } else if (!(dAd > 0.0)) {
+      L = 1.0;  // no direction
+    } else {
+      L = 1.0 - dg * dg / (dAd * E0);
+      c1 = -2.0 * dg / (dAd * E0);
+      c2 = dg / dAd;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      scal[1] = E0;
+      scal[2] = dg;
+      scal[3] = L;
+    }
   }
   const double sc = scal[0];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    scal[1] = E0;
-    scal[2] = dg;
-    scal[3] = L;
-  }
   const int c = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const float w = P[off + c];
   float gw = 0.f, gb = 0.f;
@@ -678,7 +734,8 @@ struct ofh_mpnn_s {
   double* z = nullptr;      // n
   double* q = nullptr;      // n: A d
   double* dv = nullptr;     // n: the learned direction d
-  double* scal = nullptr;   // [s, E0, d.g, loss, alpha, d.Ad]
+  double* scal = nullptr;   // [s, E0 | r.r, d.g | q.r, loss, alpha, slot 0 of the line search]
+  int obj = 0;              // objective: 0 A-norm error, 1 residual norm
   double* dpart = nullptr;  // 4 x nparts (row reductions)
   float* part = nullptr;    // per-workgroup partials of the node reductions
   double* h_pinned = nullptr;
@@ -757,7 +814,7 @@ static int forward(ofh_mpnn M, const double* xn, const double* xp, const double*
   hipLaunchKernelGGL(k_mp_dec, dim3((n + 3) / 4), dim3(256), 0, s, n, Hl(M, M->L), M->params,
                      off_dec(M->L), M->scal, M->dv);
   hipLaunchKernelGGL(k_mp_ls, dim3(g1), dim3(256), 0, s, n, A->row_ptr, A->col, A->val, M->dv,
-                     M->r, M->q, M->dpart, g1);
+                     M->r, M->q, M->dpart, g1, M->obj);
   hipLaunchKernelGGL(k_mp_apply, dim3(g1), dim3(256), 0, s, n, M->dpart, g1, xn, xp, M->ext,
                      M->dv, M->scal, x0);
   OFH_HIP(hipGetLastError());
@@ -776,13 +833,22 @@ static int backward(ofh_mpnn M, const double* xn, const double* xp, const double
   const int n = M->n, n_pad = M->n_pad, L = M->L;
   const int g1 = (n + 255) / 256;
   const int gg = gemm_grid(n_pad);
-  // g = A (x* - x_b) goes to r (free after the forward pass)
-  hipLaunchKernelGGL(k_mp_loss, dim3(g1), dim3(256), 0, s, n, A->row_ptr, A->col, A->val, xn, xp,
-                     M->ext, xs, M->dv, M->r, M->dpart, g1);
+  double* gv;
+  if (M->obj == 1) {
+    // dL/dd = A w into z (free after the forward pass); r is still b - A x_b
+    gv = M->z;
+    hipLaunchKernelGGL(k_mp_loss_res, dim3(g1), dim3(256), 0, s, n, A->row_ptr, A->col, A->val,
+                       M->r, M->q, M->dpart, g1, gv, M->scal);
+  } else {
+    // g = A (x* - x_b) goes to r (free after the forward pass)
+    gv = M->r;
+    hipLaunchKernelGGL(k_mp_loss, dim3(g1), dim3(256), 0, s, n, A->row_ptr, A->col, A->val, xn,
+                       xp, M->ext, xs, M->dv, gv, M->dpart, g1);
+  }
   float* dH = M->dH0;
   float* dHn = M->dH1;
   hipLaunchKernelGGL(k_mp_dec_bwd, dim3(kNodeGrid), dim3(256), 0, s, n, n_pad, Hl(M, L), M->params,
-                     off_dec(L), M->q, M->r, M->dpart, g1, M->scal, dH, M->part);
+                     off_dec(L), M->q, gv, M->dpart, g1, M->scal, dH, M->part, M->obj);
   fold(M->part, kNodeGrid, kH + 1, M->grads + off_dec(L), s);
   for (int l = L - 1; l >= 0; --l) {
     // T = (dH [U > 0]) Wu^T  (n x 128)
@@ -1073,9 +1139,20 @@ int ofh_mpnn_guess(ofh_mpnn M, const double* d_xn, const double* d_xp, const dou
   return OFH_OK;
 }
 
+int ofh_mpnn_set_objective(ofh_mpnn M, int objective) {
+  OFH_CHECK(M && (objective == 0 || objective == 1), OFH_E_INVALID,
+            "ofh_mpnn_set_objective: 0 (A-norm error) or 1 (residual norm)");
+  if (objective != M->obj && M->gexec) {  // the captured inference holds the old one
+    OFH_HIP(hipGraphExecDestroy(M->gexec));
+    M->gexec = nullptr;
+  }
+  M->obj = objective;
+  return OFH_OK;
+}
+
 int ofh_mpnn_loss_grad(ofh_mpnn M, const double* d_xn, const double* d_xp, const double* d_b,
                        const double* d_xstar, double* d_x0, double* h_loss, float* h_grad) {
-  OFH_CHECK(M && d_xn && d_xp && d_b && d_xstar && d_x0, OFH_E_INVALID,
+  OFH_CHECK(M && d_xn && d_xp && d_b && (d_xstar || M->obj == 1) && d_x0, OFH_E_INVALID,
             "ofh_mpnn_loss_grad: bad argument");
   hipStream_t s = M->A->ctx->stream;
   int rc = forward(M, d_xn, d_xp, d_b, d_x0, true, s);
@@ -1090,7 +1167,8 @@ int ofh_mpnn_loss_grad(ofh_mpnn M, const double* d_xn, const double* d_xp, const
 
 int ofh_mpnn_train_step(ofh_mpnn M, const double* d_xn, const double* d_xp, const double* d_b,
                         const double* d_xstar, double* d_x0, float lr, double* h_loss) {
-  OFH_CHECK(M && d_xn && d_xp && d_b && d_xstar && d_x0 && lr > 0.f, OFH_E_INVALID,
+  OFH_CHECK(M && d_xn && d_xp && d_b && (d_xstar || M->obj == 1) && d_x0 && lr > 0.f,
+            OFH_E_INVALID,
             "ofh_mpnn_train_step: bad argument");
   hipStream_t s = M->A->ctx->stream;
   int rc = forward(M, d_xn, d_xp, d_b, d_x0, true, s);

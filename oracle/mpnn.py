@@ -28,6 +28,13 @@ solution, e_b = x* - x_b, g = A e_b, E0 = e_b^T g):
 the A-norm error left after the line search relative to the base's (one
 minus the squared A-cosine between d and e_b); it is invariant to the scale
 of d.  dL/dd = -2 (d^T g) / (d^T A d E0) (g - (d^T g / d^T A d) A d).
+
+Minimum-residual objective (obj = 1), aligned with CG's stop test on ||r||:
+  x0 = x_b + alpha d,  alpha = (A d)^T r / (A d)^T A d    (0 if A d = 0)
+  L = 1 - c^2 / (q R),  c = (A d)^T r, q = (A d)^T A d, R = r^T r,
+the squared residual norm left after the line search relative to the base's;
+it needs no reference solution.  With u = A d,
+  dL/du = -2c/(qR) r + 2c^2/(q^2 R) u,   dL/dd = A dL/du   (A symmetric).
 """
 import numpy as np
 import scipy.sparse as sp
@@ -138,7 +145,7 @@ def features(g, x_n, x_p, b, ext=0):
     return F.astype(np.float32).astype(np.float64), s
 
 
-def forward(g, theta, layers, x_n, x_p, b, bf=False, ext=0):
+def forward(g, theta, layers, x_n, x_p, b, bf=False, ext=0, obj=0):
     """Returns x0 and the cache of the backward pass."""
     P = unpack(theta, layers)
     rnd = bf16 if bf else (lambda a: a)
@@ -160,9 +167,28 @@ def forward(g, theta, layers, x_n, x_p, b, bf=False, ext=0):
     r = b - g.A @ xb
     Ad = g.A @ d
     dAd = float(d @ Ad)
-    alpha = float(d @ r) / dAd if dAd > 0 else 0.0
-    cache.update(d=d, Ad=Ad, dAd=dAd, xb=xb)
+    if obj == 1:
+        q = float(Ad @ Ad)
+        alpha = float(Ad @ r) / q if q > 0 else 0.0
+    else:
+        alpha = float(d @ r) / dAd if dAd > 0 else 0.0
+    cache.update(d=d, Ad=Ad, dAd=dAd, xb=xb, r=r)
     return xb + alpha * d, cache
+
+
+def loss_residual(g, cache):
+    """Minimum-residual objective: L and dL/dd (see the module text)."""
+    Ad, r = cache["Ad"], cache["r"]
+    R = float(r @ r)
+    q = float(Ad @ Ad)
+    if not R > 0:  # the base solves the system
+        return 0.0, np.zeros_like(Ad)
+    if not q > 0:  # no direction: nothing removed
+        return 1.0, np.zeros_like(Ad)
+    c = float(Ad @ r)
+    L = 1.0 - c * c / (q * R)
+    w = (-2.0 * c / (q * R)) * r + (2.0 * c * c / (q * q * R)) * Ad
+    return L, g.A @ w
 
 
 def loss(g, cache, x_star):
@@ -213,9 +239,9 @@ def backward(g, cache, layers, gd, bf=False):
     return pack(G, layers)
 
 
-def loss_and_grad(g, theta, layers, x_n, x_p, b, x_star, bf=False, ext=0):
-    x0, cache = forward(g, theta, layers, x_n, x_p, b, bf, ext)
-    L, gd = loss(g, cache, x_star)
+def loss_and_grad(g, theta, layers, x_n, x_p, b, x_star, bf=False, ext=0, obj=0):
+    x0, cache = forward(g, theta, layers, x_n, x_p, b, bf, ext, obj)
+    L, gd = loss_residual(g, cache) if obj == 1 else loss(g, cache, x_star)
     return L, backward(g, cache, layers, gd, bf), x0
 
 

@@ -136,3 +136,48 @@ def test_bf16_forward_close_to_fp64():
     d = x0 - x_n
     assert np.linalg.norm(x0b - x0) <= 0.05 * np.linalg.norm(d)
     assert not np.array_equal(x0b, x0)
+
+
+@pytest.mark.parametrize("ext", [0, 1])
+def test_residual_loss_is_the_residual_ratio_after_the_line_search(ext):
+    """Minimum-residual objective: L equals |b - A x0|^2 / |b - A x_b|^2, and
+    the step minimises |b - A (x_b + alpha d)| over alpha (checked against a
+    dense scan of alpha)."""
+    g, b, xs, x_n, x_p = small_system()
+    A = g.A.toarray()
+    for seed in range(3):
+        th = N.init_params(2, seed=seed)
+        x0, cache = N.forward(g, th, 2, x_n, x_p, b, ext=ext, obj=1)
+        L, _ = N.loss_residual(g, cache)
+        xb = N.base(x_n, x_p, ext)
+        r0, rb = b - A @ x0, b - A @ xb
+        assert abs(L - (r0 @ r0) / (rb @ rb)) <= 1e-9 and 0 <= L <= 1
+        d = cache["d"]
+        alpha = (x0 - xb) @ d / (d @ d)
+        scan = [np.linalg.norm(b - A @ (xb + a * d)) for a in alpha * np.linspace(0.9, 1.1, 21)]
+        assert np.linalg.norm(r0) <= min(scan) * (1 + 1e-12)
+
+
+@pytest.mark.parametrize("layers,ext", [(1, 0), (2, 1)])
+def test_residual_gradient_matches_finite_differences(layers, ext):
+    g, b, xs, x_n, x_p = small_system()
+    th = N.init_params(layers, seed=6)
+    L, grad, _ = N.loss_and_grad(g, th, layers, x_n, x_p, b, None, ext=ext, obj=1)
+    assert 0 < L < 1
+
+    def f(t):
+        _, cache = N.forward(g, t, layers, x_n, x_p, b, ext=ext, obj=1)
+        return N.loss_residual(g, cache)[0]
+
+    rng = np.random.default_rng(1)
+    h = 1e-6
+    for _ in range(4):
+        v = rng.standard_normal(th.size)
+        fd = (f(th + h * v) - f(th - h * v)) / (2 * h)
+        assert abs(fd - grad @ v) <= 1e-6 * max(1.0, abs(fd)), (fd, grad @ v)
+    o = N.offsets(layers)
+    for i in [o[k] + 3 for k in o if k != "dec_b"] + [o["dec_b"]]:
+        e = np.zeros_like(th)
+        e[i] = h
+        fd = (f(th + e) - f(th - e)) / (2 * h)
+        assert abs(fd - grad[i]) <= 1e-6 * max(1e-2, abs(fd)), (i, fd, grad[i])
