@@ -35,7 +35,7 @@ extern "C" {
 #define OFH_E_NEGDET (-4)      /* Triang3/Tetra4/Quad4 negative determinant  */
 #define OFH_E_PATTERN (-5)     /* entry outside the sparse pattern / empty row */
 #define OFH_E_NOMEM (-6)
-#define OFH_E_UNSUPPORTED (-7) /* e.g. DILU_PREC, not provided by this path  */
+#define OFH_E_UNSUPPORTED (-7) /* e.g. DIRECT_SOLVER, not provided by this path */
 #define OFH_E_ZERODET (-8)     /* null determinant                           */
 
 /* Iteration and Preconditioner enums: same values as Matrix.h:75-93. */
@@ -129,7 +129,13 @@ int ofh_norm2_f64(ofh_ctx ctx, int n, const double* d_x, double* h_out);
 int ofh_axpy_f64(ofh_ctx ctx, int n, double a, const double* d_x, double* d_y);
 
 /* ------------------------------------------------------------ Prec / SSOR */
-/* Prec<T_>::solve(b,x) (Prec_impl.h:226-255) for IDENT, DIAG and SSOR. */
+/* Prec<T_>::solve(b,x) (Prec_impl.h:226-255) for IDENT, DIAG, DILU, ILU and
+ * SSOR.  DILU_PREC and ILU_PREC are the reference's D-ILU
+ * (SpMatrix::DILUFactorize / DILUSolve, SpMatrix_impl.h:834-900; its ILU
+ * routines call these): pivots 1/D_l with D_l = a_ll - sum_{i<l} a_li a_il / D_i
+ * (factorized on the GPU by level-scheduled rows, again whenever the
+ * values change), applied as M = (D+L) D^-1 (D+U) by exact triangular
+ * sweeps.  ofh_cg, ofh_bicgstab, ofh_gmres and ofh_linear_solve accept them. */
 int ofh_prec_apply(ofh_csr A, int prec, const double* d_b, double* d_x);
 /* SpMatrix::SSORSolve (SpMatrix_impl.h:913-942): symmetric Gauss-Seidel. */
 int ofh_ssor_solve(ofh_csr A, const double* d_b, double* d_x);

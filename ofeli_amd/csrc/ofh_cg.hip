@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int nblocks, const int* __re
     r[i] = ri;
     v[0] = bi * bi;
     v[1] = ri * ri;
-    if (PREC != OFH_SSOR_PREC) {
+    if (!sweep_prec(PREC)) {
       const double zi = zval<PREC>(i, ri, dinv, nullptr);
       if (PREC == OFH_DIAG_PREC && z) z[i] = zi;
       v[2] = ri * zi;
@@ -99,14 +99,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int n, int it_arg, int nparts
   for (int j = 0; j < kVec; ++j) {
     const int i = base + j * kBlock;
     if (i < n) {
-      zv[j] = PREC == OFH_SSOR_PREC ? z[i] : r[i];
+      zv[j] = sweep_prec(PREC) ? z[i] : r[i];
       if (PREC == OFH_DIAG_PREC) zv[j] *= dinv[i];  // z = D^-1 r (Prec_impl.h:245-247)
       pv[j] = p[i];
     }
   }
   double t[3];
   double nrm;
-  if (PREC == OFH_SSOR_PREC) {
+  if (sweep_prec(PREC)) {
     const int slot[1] = {2};
     double u[1];
     fold_partials<kBlock, 1>(u, slot, partials, nparts, red);  // written by k_cg_rz
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int n, int it_arg, int nparts
   bool stop = false;
   int status = 0;
   double res = 0;
-  if (PREC != OFH_SSOR_PREC) {
+  if (!sweep_prec(PREC)) {
     res = sqrt(t[1]) / nrm;
     if (res <= st->tol) {
       stop = true;
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int n, int it_arg, int nparts
   const double beta = it > 0 ? rho / st->rho_buf[(it - 1) & 1] : 0.0;  // CG.h:131
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->it = it;
-    if (PREC != OFH_SSOR_PREC) {
+    if (!sweep_prec(PREC)) {
       st->res = res;
       if (it == 0) {
         st->nrm = nrm;
@@ -406,7 +406,7 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   const double t0 = now_seconds();
   int rc;
   if (PREC == OFH_DIAG_PREC && (rc = csr_ensure_dinv(A))) return rc;
-  if (PREC == OFH_SSOR_PREC && (rc = csr_ensure_levels(A))) return rc;
+  if (sweep_prec(PREC) && (rc = csr_ensure_prec(A, PREC))) return rc;
   void* wsp;
   const int nv = PREC == OFH_IDENT_PREC ? 4 : 5;
   if ((rc = ensure_vws(ctx, sizeof(double) * (size_t)n * nv, &wsp))) return rc;
@@ -430,7 +430,7 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   // it is faster only while those vectors stay L2-resident (measured on
   // MI355X: -5% per iteration at 261k rows, +26% at 857k rows).
   constexpr int kFuseRows = 3 << 17;
-  if (PREC != OFH_SSOR_PREC && n <= kFuseRows) {
+  if (!sweep_prec(PREC) && n <= kFuseRows) {
     const ColIndex ci{A->col, A->col16};
     double* zw = PREC == OFH_DIAG_PREC ? z : nullptr;
     // one iteration: k_cg_upd(it), k_cg_dirspmv(it); it = -1 takes the
@@ -480,12 +480,12 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   auto ssor_z = [&](int it, bool init) -> int {
     hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(kBlock), 0, s, it, init ? nb : g, (int)init, st,
                        part);
-    int e = launch_ssor(A, r, z, &st->done, s);
+    int e = launch_prec(A, PREC, r, z, &st->done, s);
     if (e) return e;
     hipLaunchKernelGGL(k_cg_rz, dim3(g), dim3(kBlock), 0, s, n, r, z, st, part);
     return OFH_OK;
   };
-  if (PREC == OFH_SSOR_PREC && (rc = ssor_z(0, true))) return rc;
+  if (sweep_prec(PREC) && (rc = ssor_z(0, true))) return rc;
   hipLaunchKernelGGL(k_cg_dir<PREC>, dim3(g), dim3(kBlock), 0, s, n, 0, nb, g, r, A->dinv, z, p,
                      st, part);
   OFH_HIP(hipGetLastError());
@@ -499,7 +499,7 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
                        ColIndex{A->col, A->col16}, A->val, p, q, st, part);
     hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, it, nb, x, r, p, p, q,
                        A->dinv, nullptr, st, part);
-    if (PREC == OFH_SSOR_PREC) {
+    if (sweep_prec(PREC)) {
       const int e = ssor_z(it, false);
       if (e) return e;
     }
@@ -508,7 +508,7 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
     return OFH_OK;
   };
   hipGraphExec_t chunk_graph = nullptr;
-  if (ctx->use_graphs && PREC != OFH_SSOR_PREC) {
+  if (ctx->use_graphs && !sweep_prec(PREC)) {
     GraphKey key;
     csr_graph_key(A, 16 * 2 + PREC, x, wsp, A->val, A->dinv, &key);
     rc = ctx_graph(ctx, key, [&](hipStream_t cs) {
@@ -549,9 +549,8 @@ int cg_dispatch(ofh_csr A, int prec, const double* b, double* x, int max_it, dou
     case OFH_IDENT_PREC: return cg_impl<OFH_IDENT_PREC>(A, b, x, max_it, toler, info);
     case OFH_DIAG_PREC: return cg_impl<OFH_DIAG_PREC>(A, b, x, max_it, toler, info);
     case OFH_SSOR_PREC: return cg_impl<OFH_SSOR_PREC>(A, b, x, max_it, toler, info);
-    case OFH_DILU_PREC:
-    case OFH_ILU_PREC:
-      return fail(OFH_E_UNSUPPORTED, "DILU_PREC/ILU_PREC are not provided by this path");
+    case OFH_DILU_PREC:  // the reference's ILU is its DILU
+    case OFH_ILU_PREC: return cg_impl<OFH_DILU_PREC>(A, b, x, max_it, toler, info);
     default: return fail(OFH_E_INVALID, "unknown preconditioner");
   }
 }

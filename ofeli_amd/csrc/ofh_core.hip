@@ -436,6 +436,8 @@ int ofh_csr_destroy(ofh_csr A) {
   hipFree(A->lvl_rows_bwd);
   hipFree(A->lvl_ptr_bwd);
   hipFree(A->ssor_tmp);
+  hipFree(A->dilu_tpos);
+  hipFree(A->dilu_inv);
   delete[] A->h_lvl_ptr_fwd;
   delete[] A->h_lvl_ptr_bwd;
   delete A;

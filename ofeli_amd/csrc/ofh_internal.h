@@ -221,7 +221,10 @@ struct ofh_csr_s {
   int nlev_fwd = 0, nlev_bwd = 0, max_lev_width = 0;
   int* h_lvl_ptr_fwd = nullptr;  // host copies of the level pointers
   int* h_lvl_ptr_bwd = nullptr;
-  double* ssor_tmp = nullptr;    // forward-sweep result of the SSOR application
+  double* ssor_tmp = nullptr;    // forward-sweep result of the SSOR / DILU application
+  int* dilu_tpos = nullptr;      // nnz: position of a_ji for each lower entry a_ij (-1: none)
+  double* dilu_inv = nullptr;    // n: 1/D of the DILU factorization
+  uint64_t dilu_version = ~0ull; // values version dilu_inv was computed from
 };
 
 namespace ofh {
@@ -246,6 +249,15 @@ int launch_dot(ofh_ctx ctx, int n, const double* x, const double* y, double* d_o
 int csr_ensure_val32(ofh_csr A);
 // 16-bit block-relative column offsets when every column fits (see ColIndex).
 int csr_build_rel16(ofh_csr A);
+// Preconditioners applied by level-scheduled triangular sweeps (SSOR_PREC,
+// DILU_PREC and ILU_PREC, the latter two the same D-ILU as in the
+// reference); csr_ensure_prec builds what the first application needs.
+constexpr bool sweep_prec(int p) {
+  return p == OFH_SSOR_PREC || p == OFH_DILU_PREC || p == OFH_ILU_PREC;
+}
+int csr_ensure_prec(ofh_csr A, int prec);
+int launch_prec(ofh_csr A, int prec, const double* r, double* z, const int* d_done,
+                hipStream_t s);
 int csr_ensure_dinv(ofh_csr A);
 int csr_ensure_levels(ofh_csr A);
 // Both CSR-stream row partitions (FP64 and FP32 tiles) from the host row_ptr.

@@ -320,7 +320,7 @@ static int bicgstab_impl(ofh_csr A, const double* b, double* x, int max_it, doub
   const double t0 = now_seconds();
   int rc;
   if (PREC == OFH_DIAG_PREC && (rc = csr_ensure_dinv(A))) return rc;
-  if (PREC == OFH_SSOR_PREC && (rc = csr_ensure_levels(A))) return rc;
+  if (sweep_prec(PREC) && (rc = csr_ensure_prec(A, PREC))) return rc;
   void* wsp;
   if ((rc = ensure_vws(ctx, sizeof(double) * (size_t)n * 8, &wsp))) return rc;
   double* r = (double*)wsp;
@@ -339,7 +339,7 @@ static int bicgstab_impl(ofh_csr A, const double* b, double* x, int max_it, doub
   auto dir = [&](int it) -> int {
     hipLaunchKernelGGL(k_bi_dir<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, it == 1 ? nb : g, r, v,
                        p, pp, A->dinv, st, part);
-    if (PREC == OFH_SSOR_PREC) return launch_ssor(A, p, pp, &st->done, s);
+    if (sweep_prec(PREC)) return launch_prec(A, PREC, p, pp, &st->done, s);
     return OFH_OK;
   };
   if ((rc = dir(1))) return rc;
@@ -359,7 +359,7 @@ static int bicgstab_impl(ofh_csr A, const double* b, double* x, int max_it, doub
       hipLaunchKernelGGL(k_bi_s<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, nb, r, v, sv, ss,
                          A->dinv, st, part);
       hipLaunchKernelGGL(k_bi_scheck, dim3(1), dim3(kBlock), 0, s, g, st, part);
-      if (PREC == OFH_SSOR_PREC && (rc = launch_ssor(A, sv, ss, &st->halt, s))) return rc;
+      if (sweep_prec(PREC) && (rc = launch_prec(A, PREC, sv, ss, &st->halt, s))) return rc;
       hipLaunchKernelGGL(k_bi_spmv<1>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
                          A->row_ptr, ColIndex{A->col, A->col16}, A->val, ss, t, sv, st, part);
       hipLaunchKernelGGL(k_bi_upd, dim3(g), dim3(kBlock), 0, s, n, it, nb, x, r, rr, pp, sv, ss,
@@ -649,7 +649,7 @@ static int gmres_impl(ofh_csr A, const double* b, double* x, int m, int max_it, 
   const double t0 = now_seconds();
   int rc;
   if (PREC == OFH_DIAG_PREC && (rc = csr_ensure_dinv(A))) return rc;
-  if (PREC == OFH_SSOR_PREC && (rc = csr_ensure_levels(A))) return rc;
+  if (sweep_prec(PREC) && (rc = csr_ensure_prec(A, PREC))) return rc;
   void* wsp;
   const size_t nd = (size_t)n * (m1 + 3) + (size_t)m1 * m1 + 4 * (size_t)m1;
   if ((rc = ensure_vws(ctx, sizeof(double) * nd, &wsp))) return rc;
@@ -673,10 +673,10 @@ static int gmres_impl(ofh_csr A, const double* b, double* x, int m, int max_it, 
   const int nb = A->nblocks;
   const int ge = (n + 255) / 256;
   auto residual = [&](bool init) -> int {
-    if (PREC == OFH_SSOR_PREC) {
+    if (sweep_prec(PREC)) {
       hipLaunchKernelGGL(k_resid_plain, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
                          A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, tmp, st);
-      int e = launch_ssor(A, tmp, r, &st->done, s);
+      int e = launch_prec(A, PREC, tmp, r, &st->done, s);
       if (e) return e;
       hipLaunchKernelGGL(k_gm_rnorm, dim3(g), dim3(kBlock), 0, s, n, r, b, st, part);
       hipLaunchKernelGGL(k_gm_rcheck, dim3(1), dim3(kBlock), 0, s, (int)init, g, st, part);
@@ -696,10 +696,10 @@ static int gmres_impl(ofh_csr A, const double* b, double* x, int m, int max_it, 
                        sv, st);
     for (int i = 0; i < m && it <= max_it; i++, it++) {
       double* vi = V + (size_t)i * n;
-      if (PREC == OFH_SSOR_PREC) {
+      if (sweep_prec(PREC)) {
         hipLaunchKernelGGL(k_gm_av<OFH_IDENT_PREC>, dim3(nb), dim3(kBlock), 0, s, nb,
                            A->row_blocks, A->row_ptr, ColIndex{A->col, A->col16}, A->val, vi, tmp, A->dinv, st);
-        if ((rc = launch_ssor(A, tmp, w, &st->halt, s))) return rc;
+        if ((rc = launch_prec(A, PREC, tmp, w, &st->halt, s))) return rc;
       } else {
         hipLaunchKernelGGL(k_gm_av<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
                            A->row_ptr, ColIndex{A->col, A->col16}, A->val, vi, w, A->dinv, st);
@@ -741,9 +741,8 @@ int ofh_bicgstab(ofh_csr A, int prec, const double* b, double* x, int max_it, do
     case OFH_IDENT_PREC: return bicgstab_impl<OFH_IDENT_PREC>(A, b, x, max_it, toler, info);
     case OFH_DIAG_PREC: return bicgstab_impl<OFH_DIAG_PREC>(A, b, x, max_it, toler, info);
     case OFH_SSOR_PREC: return bicgstab_impl<OFH_SSOR_PREC>(A, b, x, max_it, toler, info);
-    case OFH_DILU_PREC:
-    case OFH_ILU_PREC:
-      return fail(OFH_E_UNSUPPORTED, "DILU_PREC/ILU_PREC are not provided by this path");
+    case OFH_DILU_PREC:  // the reference's ILU is its DILU
+    case OFH_ILU_PREC: return bicgstab_impl<OFH_DILU_PREC>(A, b, x, max_it, toler, info);
     default: return fail(OFH_E_INVALID, "unknown preconditioner");
   }
 }
@@ -755,9 +754,8 @@ int ofh_gmres(ofh_csr A, int prec, const double* b, double* x, int m, int max_it
     case OFH_IDENT_PREC: return gmres_impl<OFH_IDENT_PREC>(A, b, x, m, max_it, toler, info);
     case OFH_DIAG_PREC: return gmres_impl<OFH_DIAG_PREC>(A, b, x, m, max_it, toler, info);
     case OFH_SSOR_PREC: return gmres_impl<OFH_SSOR_PREC>(A, b, x, m, max_it, toler, info);
-    case OFH_DILU_PREC:
-    case OFH_ILU_PREC:
-      return fail(OFH_E_UNSUPPORTED, "DILU_PREC/ILU_PREC are not provided by this path");
+    case OFH_DILU_PREC:  // the reference's ILU is its DILU
+    case OFH_ILU_PREC: return gmres_impl<OFH_DILU_PREC>(A, b, x, m, max_it, toler, info);
     default: return fail(OFH_E_INVALID, "unknown preconditioner");
   }
 }
@@ -781,7 +779,8 @@ int ofh_prec_apply(ofh_csr A, int prec, const double* b, double* x) {
       return launch_ssor(A, b, x, nullptr, A->ctx->stream);
     case OFH_DILU_PREC:
     case OFH_ILU_PREC:
-      return fail(OFH_E_UNSUPPORTED, "DILU_PREC/ILU_PREC are not provided by this path");
+      OFH_CHECK(b != x, OFH_E_INVALID, "ofh_prec_apply: DILU/ILU needs distinct b and x");
+      return launch_prec(A, prec, b, x, nullptr, A->ctx->stream);
     default: return fail(OFH_E_INVALID, "unknown preconditioner");
   }
   OFH_HIP(hipGetLastError());

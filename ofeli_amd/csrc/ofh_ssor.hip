@@ -1,15 +1,24 @@
-// ofh_ssor.hip -- SSOR_PREC on the GPU: SpMatrix::SSORSolve (SpMatrix_impl.h:913-942).
+// ofh_ssor.hip -- preconditioners applied by triangular sweeps on the GPU:
+// SSOR_PREC (SpMatrix::SSORSolve, SpMatrix_impl.h:913-942) and
+// DILU_PREC / ILU_PREC (SpMatrix::DILUFactorize / DILUSolve,
+// SpMatrix_impl.h:834-900; the reference's ILU routines are its DILU ones).
 //
-// The reference applies symmetric Gauss-Seidel (omega = 1):
+// SSOR, symmetric Gauss-Seidel (omega = 1):
 //   forward   z_i = (b_i - sum_{j<i} a_ij z_j) / a_ii        i = 1..n
 //   backward  x_i = z_i - (sum_{j>i} a_ij x_j) / a_ii        i = n..1
-// i.e. M = (D+L) D^-1 (D+U).  Both sweeps are executed exactly, row by row,
-// in dependency order: rows are grouped into level sets (a row's level is one
-// more than the deepest row it reads), and every row of a level is computed
-// by one thread with the reference's left-to-right sums, so the result is
-// bitwise the reference's.  Runs of narrow levels are executed by a single
-// persistent 1024-thread workgroup with one barrier per level (no launch per
-// level); wide levels get a grid of their own.
+// D-ILU: pivot_i = 1/D_i with D_l = a_ll - sum_{i<l} (a_li pivot_i) a_il over
+// the i whose entries a_li and a_il are both present, subtracted in
+// increasing i; then
+//   forward   z_i = pivot_i (b_i - sum_{j<i} a_ij z_j)
+//   backward  x_i = z_i - pivot_i sum_{j>i} a_ij x_j
+// i.e. M = (D+L) D^-1 (D+U) in both cases.  Every sweep (the factorization
+// too: row l reads the pivots of the rows of its lower part) is executed
+// exactly, row by row, in dependency order: rows are grouped into level sets
+// (a row's level is one more than the deepest row it reads), and every row
+// of a level is computed by one thread with the reference's left-to-right
+// sums, so the result is bitwise the reference's.  Runs of narrow levels are
+// executed by a single persistent 1024-thread workgroup with one barrier per
+// level (no launch per level); wide levels get a grid of their own.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,58 +31,63 @@ namespace ofh {
 constexpr int kSgsWg = 1024;       // threads of the single-workgroup sweep
 constexpr int kWideLevel = 8192;   // rows per level above which a grid is launched
 
-// one row of the forward sweep
-__device__ __forceinline__ void sgs_fwd_row(int i, const int* __restrict__ rp,
-                                            const int* __restrict__ col,
-                                            const double* __restrict__ val,
-                                            const int* __restrict__ dpos,
-                                            const double* __restrict__ b, double* z) {
-  double s = 0;
-  const int d = dpos[i];
-  for (int k = rp[i]; k < d; ++k) s += val[k] * z[col[k]];
-  z[i] = (b[i] - s) / val[d];
-}
+enum SweepOp { kSgsFwd = 0, kSgsBwd = 1, kDiluFact = 2, kDiluFwd = 3, kDiluBwd = 4 };
+constexpr bool op_forward(int op) { return op != kSgsBwd && op != kDiluBwd; }
 
-__device__ __forceinline__ void sgs_bwd_row(int i, const int* __restrict__ rp,
-                                            const int* __restrict__ col,
-                                            const double* __restrict__ val,
-                                            const int* __restrict__ dpos,
-                                            const double* __restrict__ z, double* x) {
-  double s = 0;
-  const int d = dpos[i];
-  for (int k = d + 1; k < rp[i + 1]; ++k) s += val[k] * x[col[k]];
-  x[i] = z[i] - s / val[d];
-}
+struct SweepArgs {
+  const int* __restrict__ rp;
+  const int* __restrict__ col;
+  const double* __restrict__ val;
+  const int* __restrict__ dpos;    // position of a_ii
+  const int* __restrict__ tpos;    // DILU factorization: position of a_ji, -1 if absent
+  const double* __restrict__ inv;  // DILU solves: 1/D
+  const double* in;
+  double* out;                     // the factorization writes 1/D here
+  const int* done;                 // stop flag of the calling solver (may be null)
+};
 
-template <bool FWD>
-__global__ __launch_bounds__(kSgsWg) void k_sgs_levels_wg(
-    int l0, int l1, const int* __restrict__ lptr, const int* __restrict__ lrows,
-    const int* __restrict__ rp, const int* __restrict__ col, const double* __restrict__ val,
-    const int* __restrict__ dpos, const double* __restrict__ in, double* out,
-    const int* __restrict__ done) {
-  if (done && *done) return;
-  for (int l = l0; l < l1; ++l) {
-    const int a = lptr[l], e = lptr[l + 1];
-    for (int t = a + threadIdx.x; t < e; t += kSgsWg) {
-      const int i = lrows[t];
-      if (FWD) sgs_fwd_row(i, rp, col, val, dpos, in, out);
-      else sgs_bwd_row(i, rp, col, val, dpos, in, out);
+// one row of a sweep (the reference's operation order)
+template <int OP>
+__device__ __forceinline__ void sweep_row(int i, const SweepArgs& a) {
+  const int d = a.dpos[i];
+  double s = 0;
+  if (OP == kDiluFact) {
+    double p = a.val[d];
+    for (int k = a.rp[i]; k < d; ++k) {
+      const int t = a.tpos[k];
+      if (t >= 0) p -= a.val[k] * a.out[a.col[k]] * a.val[t];
     }
+    a.out[i] = 1.0 / p;
+  } else if (op_forward(OP)) {
+    for (int k = a.rp[i]; k < d; ++k) s += a.val[k] * a.out[a.col[k]];
+    a.out[i] = OP == kSgsFwd ? (a.in[i] - s) / a.val[d] : a.inv[i] * (a.in[i] - s);
+  } else {
+    for (int k = d + 1; k < a.rp[i + 1]; ++k) s += a.val[k] * a.out[a.col[k]];
+    a.out[i] = OP == kSgsBwd ? a.in[i] - s / a.val[d] : a.in[i] - a.inv[i] * s;
+  }
+}
+
+template <int OP>
+__global__ __launch_bounds__(kSgsWg) void k_sweep_levels_wg(int l0, int l1,
+                                                           const int* __restrict__ lptr,
+                                                           const int* __restrict__ lrows,
+                                                           SweepArgs a) {
+  if (a.done && *a.done) return;
+  for (int l = l0; l < l1; ++l) {
+    const int b = lptr[l], e = lptr[l + 1];
+    for (int t = b + threadIdx.x; t < e; t += kSgsWg) sweep_row<OP>(lrows[t], a);
     __syncthreads();
   }
 }
 
-template <bool FWD>
-__global__ __launch_bounds__(256) void k_sgs_level_grid(
-    int a, int e, const int* __restrict__ lrows, const int* __restrict__ rp,
-    const int* __restrict__ col, const double* __restrict__ val, const int* __restrict__ dpos,
-    const double* __restrict__ in, double* out, const int* __restrict__ done) {
-  if (done && *done) return;
-  const int t = a + blockIdx.x * 256 + threadIdx.x;
+template <int OP>
+__global__ __launch_bounds__(256) void k_sweep_level_grid(int b, int e,
+                                                         const int* __restrict__ lrows,
+                                                         SweepArgs a) {
+  if (a.done && *a.done) return;
+  const int t = b + blockIdx.x * 256 + threadIdx.x;
   if (t >= e) return;
-  const int i = lrows[t];
-  if (FWD) sgs_fwd_row(i, rp, col, val, dpos, in, out);
-  else sgs_bwd_row(i, rp, col, val, dpos, in, out);
+  sweep_row<OP>(lrows[t], a);
 }
 
 // Level sets of the lower (forward) and upper (backward) triangles, computed
@@ -141,24 +155,28 @@ int csr_ensure_levels(ofh_csr A) {
   return OFH_OK;
 }
 
-template <bool FWD>
-static int sweep(ofh_csr A, const double* in, double* out, const int* done, hipStream_t s) {
+template <int OP>
+static int sweep(ofh_csr A, SweepArgs a, hipStream_t s) {
+  constexpr bool FWD = op_forward(OP);
   const int nl = FWD ? A->nlev_fwd : A->nlev_bwd;
   const int* hp = FWD ? A->h_lvl_ptr_fwd : A->h_lvl_ptr_bwd;
   const int* dp = FWD ? A->lvl_ptr_fwd : A->lvl_ptr_bwd;
   const int* dr = FWD ? A->lvl_rows_fwd : A->lvl_rows_bwd;
+  a.rp = A->row_ptr;
+  a.col = A->col;
+  a.val = A->val;
+  a.dpos = A->diag_pos;
   int l = 0;
   while (l < nl) {
     if (hp[l + 1] - hp[l] > kWideLevel) {
-      const int a = hp[l], e = hp[l + 1];
-      hipLaunchKernelGGL(k_sgs_level_grid<FWD>, dim3((e - a + 255) / 256), dim3(256), 0, s, a, e,
-                         dr, A->row_ptr, A->col, A->val, A->diag_pos, in, out, done);
+      const int b = hp[l], e = hp[l + 1];
+      hipLaunchKernelGGL(k_sweep_level_grid<OP>, dim3((e - b + 255) / 256), dim3(256), 0, s, b, e,
+                         dr, a);
       ++l;
     } else {
       int l1 = l;
       while (l1 < nl && hp[l1 + 1] - hp[l1] <= kWideLevel) ++l1;
-      hipLaunchKernelGGL(k_sgs_levels_wg<FWD>, dim3(1), dim3(kSgsWg), 0, s, l, l1, dp, dr,
-                         A->row_ptr, A->col, A->val, A->diag_pos, in, out, done);
+      hipLaunchKernelGGL(k_sweep_levels_wg<OP>, dim3(1), dim3(kSgsWg), 0, s, l, l1, dp, dr, a);
       l = l1;
     }
   }
@@ -166,18 +184,87 @@ static int sweep(ofh_csr A, const double* in, double* out, const int* done, hipS
   return OFH_OK;
 }
 
+static int ensure_tmp(ofh_csr A) {
+  if (!A->ssor_tmp) OFH_HIP(hipMalloc(&A->ssor_tmp, sizeof(double) * std::max(A->n, 1)));
+  return OFH_OK;
+}
+
 // z = M^-1 r with M = (D+L) D^-1 (D+U); the forward result goes to a
-// scratch vector placed after the four solver vectors.
+// per-matrix scratch vector.
 int launch_ssor(ofh_csr A, const double* r, double* z, const int* d_done, hipStream_t s) {
   int rc = csr_ensure_levels(A);
+  if (rc || (rc = ensure_tmp(A))) return rc;
+  SweepArgs a{};
+  a.done = d_done;
+  a.in = r;
+  a.out = A->ssor_tmp;
+  if ((rc = sweep<kSgsFwd>(A, a, s))) return rc;
+  a.in = A->ssor_tmp;
+  a.out = z;
+  return sweep<kSgsBwd>(A, a, s);
+}
+
+// The D-ILU pivots, recomputed whenever the values may have changed.  The
+// map from each lower entry a_ij to its transposed entry a_ji (the entries
+// whose product enters D) depends on the pattern only and is built once.
+static int csr_ensure_dilu(ofh_csr A) {
+  int rc = csr_ensure_levels(A);  // diagonal positions, zero pivots, level sets
   if (rc) return rc;
-  double* tmp = A->ssor_tmp;
-  if (!tmp) {
-    OFH_HIP(hipMalloc(&A->ssor_tmp, sizeof(double) * A->n));
-    tmp = A->ssor_tmp;
+  hipStream_t s = A->ctx->stream;
+  if (!A->dilu_tpos) {
+    const int n = A->n;
+    std::vector<int> rp(n + 1), col(A->nnz), dpos(n), tpos(A->nnz, -1);
+    OFH_HIP(hipMemcpyAsync(rp.data(), A->row_ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, s));
+    OFH_HIP(hipMemcpyAsync(col.data(), A->col, sizeof(int) * A->nnz, hipMemcpyDeviceToHost, s));
+    OFH_HIP(hipMemcpyAsync(dpos.data(), A->diag_pos, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    OFH_HIP(hipStreamSynchronize(s));
+    for (int l = 0; l < n; ++l)
+      for (int k = rp[l]; k < dpos[l]; ++k) {  // a_li, i < l: find a_il in row i's upper part
+        const int i = col[k];
+        const int* b = col.data() + dpos[i] + 1;
+        const int* e = col.data() + rp[i + 1];
+        const int* f = std::lower_bound(b, e, l);
+        if (f != e && *f == l) tpos[k] = (int)(f - col.data());
+      }
+    int* d_tpos = nullptr;
+    OFH_HIP(hipMalloc(&d_tpos, sizeof(int) * std::max<int64_t>(A->nnz, 1)));
+    OFH_HIP(hipMemcpyAsync(d_tpos, tpos.data(), sizeof(int) * A->nnz, hipMemcpyHostToDevice, s));
+    if (!A->dilu_inv) OFH_HIP(hipMalloc(&A->dilu_inv, sizeof(double) * std::max(n, 1)));
+    OFH_HIP(hipStreamSynchronize(s));  // tpos leaves scope
+    A->dilu_tpos = d_tpos;
+    A->dilu_version = ~0ull;
   }
-  if ((rc = sweep<true>(A, r, tmp, d_done, s))) return rc;
-  return sweep<false>(A, tmp, z, d_done, s);
+  if (A->dilu_version != A->version) {
+    SweepArgs a{};
+    a.tpos = A->dilu_tpos;
+    a.out = A->dilu_inv;
+    if ((rc = sweep<kDiluFact>(A, a, s))) return rc;
+    A->dilu_version = A->version;
+  }
+  return OFH_OK;
+}
+
+int csr_ensure_prec(ofh_csr A, int prec) {
+  if (prec == OFH_SSOR_PREC) return csr_ensure_levels(A);
+  if (prec == OFH_DILU_PREC || prec == OFH_ILU_PREC) return csr_ensure_dilu(A);
+  return OFH_OK;
+}
+
+// z = M^-1 r for a sweep preconditioner
+int launch_prec(ofh_csr A, int prec, const double* r, double* z, const int* d_done,
+                hipStream_t s) {
+  if (prec == OFH_SSOR_PREC) return launch_ssor(A, r, z, d_done, s);
+  int rc = csr_ensure_dilu(A);
+  if (rc || (rc = ensure_tmp(A))) return rc;
+  SweepArgs a{};
+  a.done = d_done;
+  a.inv = A->dilu_inv;
+  a.in = r;
+  a.out = A->ssor_tmp;
+  if ((rc = sweep<kDiluFwd>(A, a, s))) return rc;
+  a.in = A->ssor_tmp;
+  a.out = z;
+  return sweep<kDiluBwd>(A, a, s);
 }
 
 }  // namespace ofh

@@ -32,8 +32,8 @@ class Equa {
   }
   virtual ~Equa() = default;
 
-  // Equa::setSolver (Equa.cpp:467-485)
-  void setSolver(Iteration s, Preconditioner p = DIAG_PREC) {
+  // Equa::setSolver (Equa.cpp:467-485; default preconditioner Equa.h:318-319)
+  void setSolver(Iteration s, Preconditioner p = IDENT_PREC) {
     if (s == DIRECT_SOLVER)
       throw OFELIException("In Equa::setSolver: the GPU path solves iteratively");
     _ls.setSolver(s, p);
@@ -161,21 +161,23 @@ class Equa {
 // Laplace2DT3 (Laplace2DT3.cpp:64-143): -Delta u = f with P1 triangles
 class Laplace2DT3 : public Equa {
  public:
-  Laplace2DT3(Mesh& ms, Vect<real_t>& u) : Equa(ms, u, OFH_LAPLACE2DT3, CG_SOLVER, DIAG_PREC) {}
+  // the reference's default is DIRECT_SOLVER (Laplace2DT3.cpp:69); this path
+  // solves iteratively, with the preconditioner of the other classes
+  Laplace2DT3(Mesh& ms, Vect<real_t>& u) : Equa(ms, u, OFH_LAPLACE2DT3, CG_SOLVER, DILU_PREC) {}
 };
 
 // Laplace3DT4 (Laplace3DT4.cpp:62-129): -Delta u = f with P1 tetrahedra;
 // with setElementCoefficient(k) it assembles DC3DT4's Diffusion(k_e).
 class Laplace3DT4 : public Equa {
  public:
-  Laplace3DT4(Mesh& ms, Vect<real_t>& u) : Equa(ms, u, OFH_LAPLACE3DT4, CG_SOLVER, DIAG_PREC) {}
+  Laplace3DT4(Mesh& ms, Vect<real_t>& u) : Equa(ms, u, OFH_LAPLACE3DT4, CG_SOLVER, DILU_PREC) {}  // Laplace3DT4.cpp:70
 };
 
 // DC2DT3 (DC2DT3.cpp:64-253): steady diffusion with per-element k, or the
 // backward-Euler transient of Equa_Therm::build (Equa_Therm.h:162-218).
 class DC2DT3 : public Equa {
  public:
-  DC2DT3(Mesh& ms, Vect<real_t>& u) : Equa(ms, u, OFH_DC2DT3, CG_SOLVER, DIAG_PREC) {}
+  DC2DT3(Mesh& ms, Vect<real_t>& u) : Equa(ms, u, OFH_DC2DT3, CG_SOLVER, DILU_PREC) {}  // DC2DT3.cpp:72
   void setTransient(TimeScheme s) {
     if (s != BACKWARD_EULER && s != NONE)
       throw OFELIException("DC2DT3::setTransient: the GPU path provides BACKWARD_EULER");
@@ -215,7 +217,7 @@ class DC2DT3 : public Equa {
 // PlaneStress()/PlaneStrain() take effect with eMat = Deviator + Dilatation.
 class Elas2DQ4 : public Equa {
  public:
-  Elas2DQ4(Mesh& ms, Vect<real_t>& u) : Equa(ms, u, OFH_ELAS2DQ4, CG_SOLVER, SSOR_PREC) {
+  Elas2DQ4(Mesh& ms, Vect<real_t>& u) : Equa(ms, u, OFH_ELAS2DQ4, CG_SOLVER, DILU_PREC) {  // Elas2DQ4.cpp:61
     setYoung(1000.0);  // material/Generic.md: E = 1000, nu = 0.2
   }
   void PlaneStress() { _stress = true; }

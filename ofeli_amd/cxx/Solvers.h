@@ -15,8 +15,8 @@
 
 namespace OFELI {
 
-// Prec<T_> (Prec.h:73-176): IDENT, DIAG and SSOR on the GPU.  DILU/ILU are
-// not part of this path and are rejected.
+// Prec<T_> (Prec.h:73-176): IDENT, DIAG, DILU/ILU (the reference's ILU is
+// its DILU) and SSOR on the GPU.
 template <class T_>
 class Prec {
  public:
@@ -28,8 +28,6 @@ class Prec {
     if (_type == -1)
       throw OFELIException("In Prec::setMatrix(SpMatrix<T_>): Choose preconditioner before "
                            "setting matrix !");
-    if (_type == DILU_PREC || _type == ILU_PREC)
-      throw OFELIException("In Prec::setMatrix: DILU_PREC/ILU_PREC are not provided by the GPU path");
     _a = &A;
   }
   int type() const { return _type; }

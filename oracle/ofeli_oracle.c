@@ -660,8 +660,79 @@ double orc_dot(oi n, const double* u, const double* v) {
 }
 
 /* Prec::setMatrix + inv_diag, Prec_impl.h:164-195, 306-316. */
+/* 1-based position of the diagonal entry of each row (0: none, or a zero
+ * diagonal value -> ORC_E_ZERODIAG), as SpMatrix::DILUFactorize's id[]. */
+static int diag_ids(oi n, const oi* row_ptr, const oi* col_ind, const double* a, oi* id) {
+  oi k = 0;
+  for (oi i = 0; i < n; i++) {
+    id[i] = 0;
+    for (oi j = row_ptr[i]; j < row_ptr[i + 1]; j++, k++)
+      if (col_ind[j] == i + 1) {
+        id[i] = k + 1;
+        if (a[k] == 0.0) return ORC_E_ZERODIAG;
+      }
+    if (id[i] == 0) return ORC_E_ZERODIAG;
+  }
+  return ORC_OK;
+}
+
+/* SpMatrix::DILUFactorize, SpMatrix_impl.h:834-870 (ILUFactorize is the same
+ * routine, :873-878): with pivot_i = a_ii, for i = 1..n in turn,
+ *   pivot_i <- 1 / pivot_i, then for every upper entry a_il of row i whose
+ *   transposed entry a_li lies in the lower part of row l:
+ *   pivot_l <- pivot_l - (a_li pivot_i) a_il,
+ * so pivot_i = 1/D_i with D_l = a_ll - sum_{i<l} a_li a_il / D_i (D-ILU). */
+int orc_dilu_factorize(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
+                       double* pivot) {
+  oi* id = (oi*)malloc(sizeof(oi) * (n > 0 ? n : 1));
+  if (!id) return ORC_E_NOMEM;
+  int rc = diag_ids(n, row_ptr, col_ind, a, id);
+  if (rc != ORC_OK) { free(id); return rc; }
+  for (oi i = 0; i < n; i++) pivot[i] = a[id[i] - 1];
+  for (oi i = 0; i < n; i++) {
+    pivot[i] = 1.0 / pivot[i];
+    for (oi j = id[i]; j < row_ptr[i + 1]; j++) {
+      int found = 0;
+      double c = 0.0;
+      const oi l = col_ind[j] - 1;
+      for (oi k = row_ptr[l]; k < id[l] - 1; k++)
+        if (col_ind[k] == i + 1) found = 1, c = a[k];
+      if (found) pivot[l] -= c * pivot[i] * a[j];
+    }
+  }
+  free(id);
+  return ORC_OK;
+}
+
+/* SpMatrix::DILUSolve, SpMatrix_impl.h:881-900: forward
+ * z_i = pivot_i (b_i - sum_{j<=i} a_ij z_j) (the diagonal term meets z_i = 0),
+ * then backward x_i = z_i - pivot_i sum_{j>i} a_ij x_j, i = n..1; that is
+ * x = M^-1 b with M = (D+L) D^-1 (D+U). */
+int orc_dilu_solve(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
+                   const double* pivot, const double* b, double* x) {
+  oi* id = (oi*)malloc(sizeof(oi) * (n > 0 ? n : 1));
+  double* z = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+  if (!id || !z) { free(id); free(z); return ORC_E_NOMEM; }
+  int rc = diag_ids(n, row_ptr, col_ind, a, id);
+  if (rc != ORC_OK) { free(id); free(z); return rc; }
+  for (oi i = 0; i < n; i++) {
+    double s = 0;
+    for (oi j = row_ptr[i]; j < id[i]; j++) s += a[j] * z[col_ind[j] - 1];
+    z[i] = pivot[i] * (b[i] - s);
+  }
+  for (oi i = n - 1; i >= 0; i--) {
+    double s = 0;
+    for (oi j = id[i]; j < row_ptr[i + 1]; j++) s += a[j] * x[col_ind[j] - 1];
+    x[i] = z[i] - pivot[i] * s;
+  }
+  free(id);
+  free(z);
+  return ORC_OK;
+}
+
 int orc_prec_setup(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
                    int prec, double* pivot) {
+  if (prec == ORC_DILU || prec == ORC_ILU) return orc_dilu_factorize(n, row_ptr, col_ind, a, pivot);
   if (prec == ORC_DIAG) {
     for (oi i = 1; i <= n; i++) {
       oi p = col_pos(row_ptr, col_ind, i, i);
@@ -718,6 +789,8 @@ int orc_prec_apply(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
     for (oi i = 0; i < n; i++) x[i] = b[i] * pivot[i];
   } else if (prec == ORC_SSOR) {
     return orc_ssor_solve(n, row_ptr, col_ind, a, b, x);
+  } else if (prec == ORC_DILU || prec == ORC_ILU) {
+    return orc_dilu_solve(n, row_ptr, col_ind, a, pivot, b, x);
   } else {
     return ORC_E_ARG;
   }

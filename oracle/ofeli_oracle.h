@@ -35,7 +35,7 @@ enum {
 
 /* Preconditioners, numbered as the reference's Preconditioner enum
  * (Matrix.h:84-93). */
-enum { ORC_IDENT = 1, ORC_DIAG = 2, ORC_SSOR = 5 };
+enum { ORC_IDENT = 1, ORC_DIAG = 2, ORC_DILU = 3, ORC_ILU = 4, ORC_SSOR = 5 };
 
 /* Error codes (the reference throws OFELIException in these cases). */
 enum {
@@ -96,6 +96,11 @@ int orc_prec_setup(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
                    int prec, double* pivot);
 int orc_prec_apply(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
                    int prec, const double* pivot, const double* b, double* x);
+/* DILU_PREC / ILU_PREC (the reference's ILU routines are its DILU ones). */
+int orc_dilu_factorize(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
+                       double* pivot);
+int orc_dilu_solve(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
+                   const double* pivot, const double* b, double* x);
 int orc_ssor_solve(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
                    const double* b, double* x);
 int orc_cg(oi n, const oi* row_ptr, const oi* col_ind, const double* a, int prec,

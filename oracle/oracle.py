@@ -15,7 +15,7 @@ LIB_STRICT = os.path.join(HERE, "liborc.so")       # -O2 -ffp-contract=off
 LIB_FAST = os.path.join(HERE, "liborc_fast.so")    # the reference's -Ofast
 
 LAPLACE2DT3, DC2DT3, DC2DT3_BE, LAPLACE3DT4, ELAS2DQ4 = 1, 2, 3, 4, 5
-IDENT, DIAG, SSOR = 1, 2, 5
+IDENT, DIAG, DILU, ILU, SSOR = 1, 2, 3, 4, 5
 
 _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -66,6 +66,8 @@ def lib(fast=False):
     L.orc_dot.argtypes = [_oi, _f64p, _f64p]
     L.orc_dot.restype = C.c_double
     L.orc_ssor_solve.argtypes = [_oi, _i64p, _i64p, _f64p, _f64p, _f64p]
+    L.orc_dilu_factorize.argtypes = [_oi, _i64p, _i64p, _f64p, _f64p]
+    L.orc_dilu_solve.argtypes = [_oi, _i64p, _i64p, _f64p, _f64p, _f64p, _f64p]
     for fn in ("orc_cg", "orc_bicgstab"):
         getattr(L, fn).argtypes = [_oi, _i64p, _i64p, _f64p, C.c_int, _f64p, _f64p,
                                    C.c_int, C.c_double, C.POINTER(C.c_double)]
@@ -235,6 +237,24 @@ def ssor_solve(row_ptr, col_ind, a, b):
     x = np.zeros(n)
     rc = lib().orc_ssor_solve(n, row_ptr, col_ind, _f64(a), _f64(b), x)
     if rc != 0:
+        raise RuntimeError("zero pivot")
+    return x
+
+
+def dilu_factorize(row_ptr, col_ind, a):
+    """pivot = 1/D of SpMatrix::DILUFactorize (DILU_PREC and ILU_PREC)."""
+    n = row_ptr.shape[0] - 1
+    pivot = np.zeros(n)
+    if lib().orc_dilu_factorize(n, row_ptr, col_ind, _f64(a), pivot) != 0:
+        raise RuntimeError("zero pivot")
+    return pivot
+
+
+def dilu_solve(row_ptr, col_ind, a, pivot, b):
+    """x = M^-1 b of SpMatrix::DILUSolve, M = (D+L) D^-1 (D+U)."""
+    n = row_ptr.shape[0] - 1
+    x = np.zeros(n)
+    if lib().orc_dilu_solve(n, row_ptr, col_ind, _f64(a), _f64(pivot), _f64(b), x) != 0:
         raise RuntimeError("zero pivot")
     return x
 

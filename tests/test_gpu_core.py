@@ -135,7 +135,43 @@ def test_prec_diag_and_ssor_bitexact(ctx):
     assert np.array_equal(host(z), O.ssor_solve(rp, ci, a, r))
 
 
-@pytest.mark.parametrize("prec", [1, 2, 5])
+def drop_upper(rp, ci, a, every=3):
+    """The same matrix without the first upper entry of every `every`-th row:
+    a structurally unsymmetric pattern (its lower partner has no transpose)."""
+    keep = np.ones(len(ci), bool)
+    for i in range(0, len(rp) - 1, every):
+        up = [k for k in range(rp[i], rp[i + 1]) if ci[k] > i + 1]
+        if up:
+            keep[up[0]] = False
+    nrp = np.concatenate([[0], np.cumsum([keep[rp[i]:rp[i + 1]].sum() for i in range(len(rp) - 1)])])
+    return nrp.astype(rp.dtype), ci[keep], a[keep]
+
+
+@pytest.mark.parametrize("unsym", [False, True])
+def test_prec_dilu_bitexact(ctx, unsym):
+    """DILU_PREC / ILU_PREC (the reference's ILU is its DILU): the GPU's
+    level-scheduled factorization and sweeps give bitwise the oracle's
+    SpMatrix::DILUSolve; new values refresh the pivots."""
+    import ofeli_amd as oa
+    rp, ci, a, b = dc2d_system(32)
+    if unsym:
+        rp, ci, a = drop_upper(rp, ci, a)
+    A = gpu_csr(ctx, rp, ci, a)
+    r = np.random.default_rng(5).standard_normal(A.n)
+    z = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+    z2 = _torch().zeros_like(z)
+    piv = O.dilu_factorize(rp, ci, a)
+    oa.prec_apply(A, oa.DILU_PREC, dev(r), z)
+    assert np.array_equal(host(z), O.dilu_solve(rp, ci, a, piv, r))
+    oa.prec_apply(A, oa.ILU_PREC, dev(r), z2)
+    assert _torch().equal(z, z2)
+    a2 = a * (1.0 + 0.1 * np.cos(np.arange(len(a))))
+    A.set_values(a2)
+    oa.prec_apply(A, oa.DILU_PREC, dev(r), z)
+    assert np.array_equal(host(z), O.dilu_solve(rp, ci, a2, O.dilu_factorize(rp, ci, a2), r))
+
+
+@pytest.mark.parametrize("prec", [1, 2, 3, 4, 5])
 def test_cg_matches_oracle(ctx, prec):
     """CG: same return convention, iteration count within 1, and the GPU and
     oracle solutions agree to relative L2 <= 1e-9 at rtol 1e-10."""
@@ -176,7 +212,7 @@ def test_cg_not_converged_convention(ctx):
     assert ret == -8 and info.nb_it == -8 and info.iterations == 7
 
 
-@pytest.mark.parametrize("prec", [1, 2, 5])
+@pytest.mark.parametrize("prec", [1, 2, 3, 5])
 def test_bicgstab_matches_oracle(ctx, prec):
     """BiCGStab's residual history is erratic, so rounding differences in the
     tree-summed dot products move the step that first crosses the tolerance by
@@ -200,7 +236,7 @@ def test_bicgstab_matches_oracle(ctx, prec):
     assert info.true_relres <= 1e-9
 
 
-@pytest.mark.parametrize("prec", [1, 2, 5])
+@pytest.mark.parametrize("prec", [1, 2, 3, 5])
 def test_gmres_matches_oracle(ctx, prec):
     import ofeli_amd as oa
     rp, ci, a, b = dc2d_system(24)
@@ -239,5 +275,6 @@ def test_zero_pivot_error(ctx):
     with pytest.raises(oa.OfhError) as e:
         oa.cg(A, dev(b), x)
     assert e.value.code == -3
-    with pytest.raises(oa.OfhError):
+    with pytest.raises(oa.OfhError) as e:
         oa.cg(A, dev(b), x, prec=oa.DILU_PREC)
+    assert e.value.code == -3
