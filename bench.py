@@ -44,6 +44,8 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep-precs", action="store_true",
+                    help="config 1: also time CG with the sweep preconditioners (DILU, SSOR)")
     ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3, 4))
     ap.add_argument("--nx4", type=int, default=2048, help="config 4 cells along the beam")
     ap.add_argument("--ny4", type=int, default=256, help="config 4 cells across the beam")
@@ -215,7 +217,56 @@ def main():
                      "bytes_per_launch": b_spmv64, "avg_ms": round(spmv_ms, 4)},
         "cpu_baseline": cpu,
     }
+    if args.sweep_precs:
+        out["sweep_preconditioners"] = sweep_precs(args, oa, A, b, vals if cpu else None)
     print(json.dumps(out), flush=True)
+
+
+def sweep_precs(args, oa, A, b, vals):
+    """CG with the preconditioners applied by level-scheduled triangular
+    sweeps (DILU_PREC = ILU_PREC, SSOR_PREC) on the same system: the first
+    application (level sets, factorisation), one application, one solve,
+    and the oracle's CG iteration with the same preconditioner on one core."""
+    import torch
+    n = A.n
+    res = {}
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for name, pc in (("dilu", oa.DILU_PREC), ("ssor", oa.SSOR_PREC)):
+        z = torch.zeros(n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        oa.prec_apply(A, pc, b, z)
+        torch.cuda.synchronize()
+        first = time.perf_counter() - t
+        ev0.record()
+        for _ in range(3):
+            oa.prec_apply(A, pc, b, z)
+        ev1.record()
+        torch.cuda.synchronize()
+        apply_ms = ev0.elapsed_time(ev1) / 3
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        info = oa.cg(A, b, x, prec=pc, max_it=20000, toler=args.toler)
+        torch.cuda.synchronize()
+        solve_ms = (time.perf_counter() - t) * 1e3
+        res[name] = {"solve_ms": round(solve_ms, 2), "iterations": info.iterations,
+                     "true_relres": info.true_relres, "apply_ms": round(apply_ms, 3),
+                     "first_apply_ms": round(first * 1e3, 2)}
+        log("[bench] CG+%s: %.1f ms, %d it, application %.3f ms (first %.1f ms)"
+            % (name, solve_ms, info.iterations, apply_ms, first * 1e3))
+    if vals is not None:
+        from oracle import oracle as O
+        rp, cl = A.pattern()
+        bh = b.cpu().numpy()
+        t = time.perf_counter()
+        O.cg(rp.astype(np.int64), cl.astype(np.int64) + 1, vals, bh, prec=O.DILU, max_it=10,
+             toler=1e-300, fast=True)
+        per_it = (time.perf_counter() - t) / 10
+        res["cpu_dilu_ms_per_iteration"] = round(per_it * 1e3, 2)
+        res["cpu_dilu_ms_per_solve"] = round(per_it * res["dilu"]["iterations"] * 1e3, 1)
+        log("[bench] cpu CG+DILU %.1f ms/it" % (per_it * 1e3))
+    return res
 
 
 def bench_batched(args):

@@ -432,14 +432,11 @@ int ofh_csr_destroy(ofh_csr A) {
   hipFree(A->dinv32);
   hipFree(A->diag_pos);
   hipFree(A->lvl_rows_fwd);
-  hipFree(A->lvl_ptr_fwd);
   hipFree(A->lvl_rows_bwd);
-  hipFree(A->lvl_ptr_bwd);
+  hipFree(A->sweep_ctr);
   hipFree(A->ssor_tmp);
   hipFree(A->dilu_tpos);
   hipFree(A->dilu_inv);
-  delete[] A->h_lvl_ptr_fwd;
-  delete[] A->h_lvl_ptr_bwd;
   delete A;
   return OFH_OK;
 }

@@ -214,13 +214,10 @@ struct ofh_csr_s {
   bool dinv_valid = false, val32_valid = false;
   uint64_t version = 0;  // bumped whenever the values may have changed
   // level schedule for the SSOR sweeps (lazy)
-  int* lvl_rows_fwd = nullptr;
-  int* lvl_ptr_fwd = nullptr;
-  int* lvl_rows_bwd = nullptr;
-  int* lvl_ptr_bwd = nullptr;
-  int nlev_fwd = 0, nlev_bwd = 0, max_lev_width = 0;
-  int* h_lvl_ptr_fwd = nullptr;  // host copies of the level pointers
-  int* h_lvl_ptr_bwd = nullptr;
+  int* lvl_rows_fwd = nullptr;   // rows of the lower triangle listed by level (-1: padding)
+  int* lvl_rows_bwd = nullptr;   // ... of the upper triangle
+  int nlev_fwd = 0, nlev_bwd = 0, nlist_fwd = 0, nlist_bwd = 0;
+  int* sweep_ctr = nullptr;      // 2 ints: chunk ticket and finished workgroups of a sweep
   double* ssor_tmp = nullptr;    // forward-sweep result of the SSOR / DILU application
   int* dilu_tpos = nullptr;      // nnz: position of a_ji for each lower entry a_ij (-1: none)
   double* dilu_inv = nullptr;    // n: 1/D of the DILU factorization
