@@ -897,6 +897,171 @@ done:
   return ret;
 }
 
+/* SpMatrix::TMult, SpMatrix_impl.h:761-767, as BiCG.h:148 uses it: y = A^T x,
+ * accumulated row by row (y_j += a_ij x_i for i = 1..n) into a cleared y.
+ * The reference's loop indexes _col_ind[k-1] for entry k (the previous
+ * entry's 1-based column used as a 0-based index, and _col_ind[-1] for the
+ * first entry) and adds into y without clearing it; neither can be
+ * reproduced, so the product its callers need is computed. */
+void orc_spmv_t(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
+                const double* x, double* y) {
+  for (oi j = 0; j < n; j++) y[j] = 0.0;
+  for (oi i = 0; i < n; i++)
+    for (oi k = row_ptr[i]; k < row_ptr[i + 1]; k++) y[col_ind[k] - 1] += a[k] * x[i];
+}
+
+/* Prec::TransSolve for ILU_PREC, Prec_impl.h:269-284: x = M^-T b with
+ * M = (D+L) D^-1 (D+U) and pivot = 1/D, by the reference's column sweeps:
+ *   i = 1..n:  z_i = t_i, then t_j -= (pivot_i z_i) a_ij for the upper a_ij;
+ *   i = n..1:  x_i = pivot_i z_i, then z_j -= a_ij x_i for the lower a_ij,
+ * with t = b.  The reference indexes Prec's own _row_ptr/_col_ind, which are
+ * never filled (Prec.h:173); the matrix's arrays are what it means. */
+int orc_dilu_trans_solve(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
+                         const double* pivot, const double* b, double* x) {
+  oi* id = (oi*)malloc(sizeof(oi) * (n > 0 ? n : 1));
+  double* z = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+  double* t = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+  if (!id || !z || !t) { free(id); free(z); free(t); return ORC_E_NOMEM; }
+  int rc = diag_ids(n, row_ptr, col_ind, a, id);
+  if (rc != ORC_OK) { free(id); free(z); free(t); return rc; }
+  for (oi i = 0; i < n; i++) t[i] = b[i];
+  for (oi i = 0; i < n; i++) {
+    z[i] = t[i];
+    const double tmp = pivot[i] * z[i];
+    for (oi j = id[i]; j < row_ptr[i + 1]; j++) t[col_ind[j] - 1] -= tmp * a[j];
+  }
+  for (oi i = n - 1; i >= 0; i--) {
+    x[i] = pivot[i] * z[i];
+    for (oi j = row_ptr[i]; j < id[i] - 1; j++) z[col_ind[j] - 1] -= a[j] * x[i];
+  }
+  free(id);
+  free(z);
+  free(t);
+  return ORC_OK;
+}
+
+/* Prec::TransSolve(b,x), Prec_impl.h:259-296: x = b, then DIAG_PREC scales
+ * by the pivots and ILU_PREC solves with M^T; IDENT, DILU and SSOR have no
+ * case in the switch and leave x = b. */
+int orc_prec_trans_apply(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
+                         int prec, const double* pivot, const double* b, double* x) {
+  if (prec == ORC_DIAG) {
+    for (oi i = 0; i < n; i++) x[i] = b[i] * pivot[i];
+  } else if (prec == ORC_ILU) {
+    return orc_dilu_trans_solve(n, row_ptr, col_ind, a, pivot, b, x);
+  } else if (prec == ORC_IDENT || prec == ORC_DILU || prec == ORC_SSOR) {
+    for (oi i = 0; i < n; i++) x[i] = b[i];
+  } else {
+    return ORC_E_ARG;
+  }
+  return ORC_OK;
+}
+
+/* CGS, CGS.h:91-171, as written: the vector q is declared but never
+ * assigned, so it stays zero: from the second iteration u = r + beta*0 and
+ * p = u + beta*(0 + beta*p).  Returns 0 (converged at start), it
+ * (||r||/||b|| < toler after iteration it), 2 when (s,r) = 0 (whatever the
+ * iteration), or -max_it. */
+int orc_cgs(oi n, const oi* row_ptr, const oi* col_ind, const double* a, int prec,
+            const double* b, double* x, int max_it, double toler, double* res_out) {
+  ALLOCV(pivot); ALLOCV(r); ALLOCV(p); ALLOCV(y); ALLOCV(q); ALLOCV(z); ALLOCV(v);
+  ALLOCV(u); ALLOCV(w); ALLOCV(s); ALLOCV(tmp);
+  int ret, it;
+  double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0, res;
+  int rc = orc_prec_setup(n, row_ptr, col_ind, a, prec, pivot);
+  if (rc != ORC_OK) { ret = -1000000 + rc; goto done; }
+  double nrm = orc_norm2(n, b);
+  orc_spmv(n, row_ptr, col_ind, a, x, tmp);
+  for (oi i = 0; i < n; i++) s[i] = b[i] - tmp[i];
+  for (oi i = 0; i < n; i++) r[i] = s[i];
+  if (nrm == 0) nrm = 1;
+  if ((res = orc_norm2(n, r) / nrm) <= toler) { ret = 0; goto out; }
+  for (it = 1; it <= max_it; it++) {
+    rho_1 = orc_dot(n, s, r);
+    if (rho_1 == 0.0) { res = orc_norm2(n, r) / nrm; ret = 2; goto out; }
+    if (it == 1) {
+      for (oi i = 0; i < n; i++) u[i] = r[i];
+      for (oi i = 0; i < n; i++) p[i] = u[i];
+    } else {
+      beta = rho_1 / rho_2;
+      for (oi i = 0; i < n; i++) u[i] = r[i] + beta * q[i];
+      for (oi i = 0; i < n; i++) p[i] = u[i] + beta * (q[i] + beta * p[i]);
+    }
+    rc = orc_prec_apply(n, row_ptr, col_ind, a, prec, pivot, p, y);  /* y = p; P.solve(y) */
+    if (rc != ORC_OK) { ret = -1000000 + rc; goto done; }
+    orc_spmv(n, row_ptr, col_ind, a, y, v);
+    alpha = rho_1 / orc_dot(n, s, v);
+    for (oi i = 0; i < n; i++) tmp[i] = u[i] - alpha * v[i];
+    for (oi i = 0; i < n; i++) tmp[i] += u[i];
+    rc = orc_prec_apply(n, row_ptr, col_ind, a, prec, pivot, tmp, w);  /* P.solve(w) */
+    if (rc != ORC_OK) { ret = -1000000 + rc; goto done; }
+    for (oi i = 0; i < n; i++) x[i] += alpha * w[i];
+    orc_spmv(n, row_ptr, col_ind, a, w, z);
+    for (oi i = 0; i < n; i++) r[i] += -alpha * z[i];
+    rho_2 = rho_1;
+    if ((res = orc_norm2(n, r) / nrm) < toler) { ret = it; goto out; }
+  }
+  ret = -max_it;
+out:
+  if (res_out) *res_out = res;
+done:
+  free(pivot); free(r); free(p); free(y); free(q); free(z); free(v); free(u); free(w);
+  free(s); free(tmp);
+  return ret;
+}
+
+/* BiCG, BiCG.h:94-170, with the transposed products it means (orc_spmv_t,
+ * orc_prec_trans_apply): z = P r, zz = P^T rr, rho = (z,rr); p = z + beta p,
+ * pp = zz + beta pp; q = A p, qq = A^T pp; alpha = rho/(pp,q);
+ * x += alpha p, r -= alpha q, rr -= alpha qq.  The in-loop stop test divides
+ * by ||b|| itself (BiCG.h:158), the start test by ||b|| or 1.  Returns 0, it
+ * (converged, or rho = 0 at iteration it), or -(max_it+1). */
+int orc_bicg(oi n, const oi* row_ptr, const oi* col_ind, const double* a, int prec,
+             const double* b, double* x, int max_it, double toler, double* res_out) {
+  ALLOCV(pivot); ALLOCV(r); ALLOCV(z); ALLOCV(zz); ALLOCV(p); ALLOCV(pp); ALLOCV(q);
+  ALLOCV(qq); ALLOCV(rr);
+  int ret, it;
+  double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0, res;
+  int rc = orc_prec_setup(n, row_ptr, col_ind, a, prec, pivot);
+  if (rc != ORC_OK) { ret = -1000000 + rc; goto done; }
+  const double bn = orc_norm2(n, b);
+  double nrm = bn;
+  orc_spmv(n, row_ptr, col_ind, a, x, r);
+  for (oi i = 0; i < n; i++) r[i] = b[i] - r[i];
+  for (oi i = 0; i < n; i++) rr[i] = r[i];
+  if (nrm == 0.0) nrm = 1;
+  if ((res = orc_norm2(n, r) / nrm) <= toler) { ret = 0; goto out; }
+  for (it = 1; it <= max_it; it++) {
+    rc = orc_prec_apply(n, row_ptr, col_ind, a, prec, pivot, r, z);
+    if (rc == ORC_OK) rc = orc_prec_trans_apply(n, row_ptr, col_ind, a, prec, pivot, rr, zz);
+    if (rc != ORC_OK) { ret = -1000000 + rc; goto done; }
+    rho_1 = orc_dot(n, z, rr);
+    if (rho_1 == 0.0) { res = orc_norm2(n, r) / nrm; ret = it; goto out; }
+    if (it == 1) {
+      for (oi i = 0; i < n; i++) p[i] = z[i];
+      for (oi i = 0; i < n; i++) pp[i] = zz[i];
+    } else {
+      beta = rho_1 / rho_2;
+      for (oi i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+      for (oi i = 0; i < n; i++) pp[i] = zz[i] + beta * pp[i];
+    }
+    orc_spmv(n, row_ptr, col_ind, a, p, q);
+    orc_spmv_t(n, row_ptr, col_ind, a, pp, qq);
+    alpha = rho_1 / orc_dot(n, pp, q);
+    for (oi i = 0; i < n; i++) x[i] += alpha * p[i];
+    for (oi i = 0; i < n; i++) r[i] += -alpha * q[i];
+    for (oi i = 0; i < n; i++) rr[i] += -alpha * qq[i];
+    rho_2 = rho_1;
+    if ((res = orc_norm2(n, r) / bn) < toler) { ret = it; goto out; }
+  }
+  ret = -it;
+out:
+  if (res_out) *res_out = res;
+done:
+  free(pivot); free(r); free(z); free(zz); free(p); free(pp); free(q); free(qq); free(rr);
+  return ret;
+}
+
 /* Givens helpers, GMRes.h:67-115. */
 static void g_rotate(double* dx, double* dy, double cs, double sn) {
   double temp = cs * *dx + sn * *dy;

@@ -103,6 +103,18 @@ int orc_dilu_solve(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
                    const double* pivot, const double* b, double* x);
 int orc_ssor_solve(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
                    const double* b, double* x);
+/* y = A^T x (SpMatrix::TMult as its callers intend) */
+void orc_spmv_t(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
+                const double* x, double* y);
+/* Prec::TransSolve(b,x): DIAG scales, ILU solves M^T x = b, others copy */
+int orc_dilu_trans_solve(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
+                         const double* pivot, const double* b, double* x);
+int orc_prec_trans_apply(oi n, const oi* row_ptr, const oi* col_ind, const double* a,
+                         int prec, const double* pivot, const double* b, double* x);
+int orc_cgs(oi n, const oi* row_ptr, const oi* col_ind, const double* a, int prec,
+            const double* b, double* x, int max_it, double toler, double* res_out);
+int orc_bicg(oi n, const oi* row_ptr, const oi* col_ind, const double* a, int prec,
+             const double* b, double* x, int max_it, double toler, double* res_out);
 int orc_cg(oi n, const oi* row_ptr, const oi* col_ind, const double* a, int prec,
            const double* b, double* x, int max_it, double toler, double* res_out);
 int orc_bicgstab(oi n, const oi* row_ptr, const oi* col_ind, const double* a,

@@ -68,7 +68,11 @@ def lib(fast=False):
     L.orc_ssor_solve.argtypes = [_oi, _i64p, _i64p, _f64p, _f64p, _f64p]
     L.orc_dilu_factorize.argtypes = [_oi, _i64p, _i64p, _f64p, _f64p]
     L.orc_dilu_solve.argtypes = [_oi, _i64p, _i64p, _f64p, _f64p, _f64p, _f64p]
-    for fn in ("orc_cg", "orc_bicgstab"):
+    L.orc_prec_setup.argtypes = [_oi, _i64p, _i64p, _f64p, C.c_int, _f64p]
+    L.orc_spmv_t.argtypes = [_oi, _i64p, _i64p, _f64p, _f64p, _f64p]
+    L.orc_dilu_trans_solve.argtypes = [_oi, _i64p, _i64p, _f64p, _f64p, _f64p, _f64p]
+    L.orc_prec_trans_apply.argtypes = [_oi, _i64p, _i64p, _f64p, C.c_int, _f64p, _f64p, _f64p]
+    for fn in ("orc_cg", "orc_bicgstab", "orc_cgs", "orc_bicg"):
         getattr(L, fn).argtypes = [_oi, _i64p, _i64p, _f64p, C.c_int, _f64p, _f64p,
                                    C.c_int, C.c_double, C.POINTER(C.c_double)]
     L.orc_gmres.argtypes = [_oi, _i64p, _i64p, _f64p, C.c_int, _f64p, _f64p, C.c_int,
@@ -232,6 +236,28 @@ def spmv(row_ptr, col_ind, a, x):
     return y
 
 
+def spmv_t(row_ptr, col_ind, a, x):
+    """y = A^T x (SpMatrix::TMult as BiCG uses it)."""
+    n = row_ptr.shape[0] - 1
+    y = np.zeros(n)
+    lib().orc_spmv_t(n, row_ptr, col_ind, _f64(a), _f64(x), y)
+    return y
+
+
+def prec_trans_apply(row_ptr, col_ind, a, prec, b):
+    """x of Prec::TransSolve(b,x): DIAG scales, ILU solves M^T x = b, IDENT,
+    DILU and SSOR return b."""
+    n = row_ptr.shape[0] - 1
+    pivot = np.zeros(n)
+    if prec in (DIAG, DILU, ILU):
+        if lib().orc_prec_setup(n, row_ptr, col_ind, _f64(a), prec, pivot) != 0:
+            raise RuntimeError("zero pivot")
+    x = np.zeros(n)
+    if lib().orc_prec_trans_apply(n, row_ptr, col_ind, _f64(a), prec, pivot, _f64(b), x) != 0:
+        raise RuntimeError("preconditioner error")
+    return x
+
+
 def ssor_solve(row_ptr, col_ind, a, b):
     n = row_ptr.shape[0] - 1
     x = np.zeros(n)
@@ -279,6 +305,19 @@ def cg(row_ptr, col_ind, a, b, x0=None, prec=DIAG, max_it=1000, toler=1.49011611
 def bicgstab(row_ptr, col_ind, a, b, x0=None, prec=DIAG, max_it=1000,
              toler=1.4901161193847656e-08, fast=False):
     return _solve("orc_bicgstab", row_ptr, col_ind, a, b, x0, prec, max_it, toler, fast)
+
+
+def cgs(row_ptr, col_ind, a, b, x0=None, prec=DIAG, max_it=1000,
+        toler=1.4901161193847656e-08, fast=False):
+    """CGS.h:91-171 as written -> (ret, x, res); ret = it, 0, 2, or -max_it."""
+    return _solve("orc_cgs", row_ptr, col_ind, a, b, x0, prec, max_it, toler, fast)
+
+
+def bicg(row_ptr, col_ind, a, b, x0=None, prec=DIAG, max_it=1000,
+         toler=1.4901161193847656e-08, fast=False):
+    """BiCG.h:94-170 with true transposed products -> (ret, x, res);
+    ret = it, 0, or -(max_it+1)."""
+    return _solve("orc_bicg", row_ptr, col_ind, a, b, x0, prec, max_it, toler, fast)
 
 
 def gmres(row_ptr, col_ind, a, b, m, x0=None, prec=DIAG, max_it=1000,
