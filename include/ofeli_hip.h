@@ -35,7 +35,7 @@ extern "C" {
 #define OFH_E_NEGDET (-4)      /* Triang3/Tetra4/Quad4 negative determinant  */
 #define OFH_E_PATTERN (-5)     /* entry outside the sparse pattern / empty row */
 #define OFH_E_NOMEM (-6)
-#define OFH_E_UNSUPPORTED (-7) /* e.g. DIRECT_SOLVER, not provided by this path */
+#define OFH_E_UNSUPPORTED (-7) /* DIRECT_SOLVER, not provided by this path */
 #define OFH_E_ZERODET (-8)     /* null determinant                           */
 
 /* Iteration and Preconditioner enums: same values as Matrix.h:75-93. */
@@ -118,6 +118,12 @@ int ofh_csr_compress_columns(ofh_csr A, int enable);
 int ofh_spmv_f64(ofh_csr A, const double* d_x, double* d_y);
 /* y += a A x: SpMatrix::MultAdd (SpMatrix_impl.h:737-757). */
 int ofh_spmv_add_f64(ofh_csr A, double a, const double* d_x, double* d_y);
+/* y = A^T x: SpMatrix::TMult (SpMatrix_impl.h:761-767) as BiCG uses it.
+ * y is overwritten (the reference adds into y and indexes the previous
+ * entry's column, which is not reproduced).  A CSR copy of A^T is built on
+ * the first call and its values re-gathered after every value change; each
+ * y_j sums a_ij x_i in ascending i. */
+int ofh_spmv_t_f64(ofh_csr A, const double* d_x, double* d_y);
 /* FP32 SpMV on the FP32 mirror of the values (mixed-precision inner loop). */
 int ofh_spmv_f32(ofh_csr A, const float* d_x, float* d_y);
 
@@ -137,6 +143,11 @@ int ofh_axpy_f64(ofh_ctx ctx, int n, double a, const double* d_x, double* d_y);
  * values change), applied as M = (D+L) D^-1 (D+U) by exact triangular
  * sweeps.  ofh_cg, ofh_bicgstab, ofh_gmres and ofh_linear_solve accept them. */
 int ofh_prec_apply(ofh_csr A, int prec, const double* d_b, double* d_x);
+/* Prec::TransSolve(b,x) (Prec_impl.h:259-296): DIAG_PREC scales by 1/a_ii,
+ * ILU_PREC solves M^T x = b (two exact sweeps over A^T, bitwise the
+ * reference's column-oriented loops), IDENT/DILU/SSOR_PREC copy b (the
+ * reference's switch has no case for them). */
+int ofh_prec_trans_apply(ofh_csr A, int prec, const double* d_b, double* d_x);
 /* SpMatrix::SSORSolve (SpMatrix_impl.h:913-942): symmetric Gauss-Seidel. */
 int ofh_ssor_solve(ofh_csr A, const double* d_b, double* d_x);
 
@@ -150,6 +161,16 @@ int ofh_bicgstab(ofh_csr A, int prec, const double* d_b, double* d_x, int max_it
 /* GMRes(m) (GMRes.h:148-241 via :274-283). */
 int ofh_gmres(ofh_csr A, int prec, const double* d_b, double* d_x, int m, int max_it,
               double toler, ofh_solve_info* info);
+/* CGS (CGS.h:91-171 via :203-212), as written: its q is never assigned and
+ * stays 0, so it is not the textbook CGS and often does not converge.
+ * Returns 0, it, 2 when (s,r) = 0, or -max_it. */
+int ofh_cgs(ofh_csr A, int prec, const double* d_b, double* d_x, int max_it,
+            double toler, ofh_solve_info* info);
+/* BiCG (BiCG.h:94-170 via :202-211) with the true transposed products
+ * (ofh_spmv_t_f64, ofh_prec_trans_apply).  Returns 0, it (also when
+ * (z,rr) = 0), or -(max_it+1); the in-loop test divides by ||b|| itself. */
+int ofh_bicg(ofh_csr A, int prec, const double* d_b, double* d_x, int max_it,
+             double toler, ofh_solve_info* info);
 /* LinearSolver::solve (LinearSolver.cpp:194-244) dispatch; GMRES uses
  * m = min(n/5, 50) instead of the reference's n/5 (LinearSolver.cpp:238). */
 int ofh_linear_solve(ofh_csr A, int solver, int prec, const double* d_b, double* d_x,

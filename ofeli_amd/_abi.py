@@ -34,14 +34,18 @@ SIGNATURES = {
     "ofh_csr_compress_columns": (i32, [vp, i32]),
     "ofh_spmv_f64": (i32, [vp, vp, vp]),
     "ofh_spmv_add_f64": (i32, [vp, f64, vp, vp]),
+    "ofh_spmv_t_f64": (i32, [vp, vp, vp]),
     "ofh_spmv_f32": (i32, [vp, vp, vp]),
     "ofh_dot_f64": (i32, [vp, i32, vp, vp, f64p]),
     "ofh_norm2_f64": (i32, [vp, i32, vp, f64p]),
     "ofh_axpy_f64": (i32, [vp, i32, f64, vp, vp]),
     "ofh_prec_apply": (i32, [vp, i32, vp, vp]),
+    "ofh_prec_trans_apply": (i32, [vp, i32, vp, vp]),
     "ofh_ssor_solve": (i32, [vp, vp, vp]),
     "ofh_cg": (i32, [vp, i32, vp, vp, i32, f64, vp]),
     "ofh_bicgstab": (i32, [vp, i32, vp, vp, i32, f64, vp]),
+    "ofh_cgs": (i32, [vp, i32, vp, vp, i32, f64, vp]),
+    "ofh_bicg": (i32, [vp, i32, vp, vp, i32, f64, vp]),
     "ofh_gmres": (i32, [vp, i32, vp, vp, i32, i32, f64, vp]),
     "ofh_linear_solve": (i32, [vp, i32, i32, vp, vp, i32, f64, vp]),
     "ofh_cg_mixed": (i32, [vp, i32, vp, vp, i32, f64, vp]),

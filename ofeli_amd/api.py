@@ -160,6 +160,11 @@ def spmv_add(A, a, x, y):
     check(lib().ofh_spmv_add_f64(A.h, float(a), _f64(x, A.n), _f64(y, A.n)))
 
 
+def spmv_t(A, x, y):
+    """y = A^T x (SpMatrix::TMult as BiCG uses it; y is overwritten)."""
+    check(lib().ofh_spmv_t_f64(A.h, _f64(x, A.n), _f64(y, A.n)))
+
+
 def spmv_f32(A, x, y):
     check(lib().ofh_spmv_f32(A.h, _ptr(x), _ptr(y)))
 
@@ -180,6 +185,12 @@ def prec_apply(A, prec, b, x):
     check(lib().ofh_prec_apply(A.h, prec, _f64(b, A.n), _f64(x, A.n)))
 
 
+def prec_trans_apply(A, prec, b, x):
+    """x of Prec::TransSolve(b,x): DIAG scales, ILU solves M^T x = b,
+    IDENT/DILU/SSOR copy b."""
+    check(lib().ofh_prec_trans_apply(A.h, prec, _f64(b, A.n), _f64(x, A.n)))
+
+
 def ssor_solve(A, b, x):
     check(lib().ofh_ssor_solve(A.h, _f64(b, A.n), _f64(x, A.n)))
 
@@ -198,6 +209,16 @@ def cg(A, b, x, prec=DIAG_PREC, max_it=1000, toler=1.4901161193847656e-08):
 
 def bicgstab(A, b, x, prec=DIAG_PREC, max_it=1000, toler=1.4901161193847656e-08):
     return _solve("ofh_bicgstab", A, b, x, prec, int(max_it), float(toler))
+
+
+def cgs(A, b, x, prec=DIAG_PREC, max_it=1000, toler=1.4901161193847656e-08):
+    """CGS (CGS.h:91-171) as the reference writes it (q stays 0)."""
+    return _solve("ofh_cgs", A, b, x, prec, int(max_it), float(toler))
+
+
+def bicg(A, b, x, prec=DIAG_PREC, max_it=1000, toler=1.4901161193847656e-08):
+    """BiCG (BiCG.h:94-170) with the true transposed products."""
+    return _solve("ofh_bicg", A, b, x, prec, int(max_it), float(toler))
 
 
 def gmres(A, b, x, m, prec=DIAG_PREC, max_it=1000, toler=1.4901161193847656e-08):

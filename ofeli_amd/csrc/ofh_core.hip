@@ -272,6 +272,12 @@ int csr_ensure_dinv(ofh_csr A) {
   return OFH_OK;
 }
 
+void csr_values_changed(ofh_csr A) {
+  A->val32_valid = false;
+  A->dinv_valid = false;
+  ++A->version;
+}
+
 }  // namespace ofh
 
 using namespace ofh;
@@ -437,6 +443,8 @@ int ofh_csr_destroy(ofh_csr A) {
   hipFree(A->ssor_tmp);
   hipFree(A->dilu_tpos);
   hipFree(A->dilu_inv);
+  hipFree(A->tr_perm);
+  ofh_csr_destroy(A->tr);
   delete A;
   return OFH_OK;
 }
@@ -458,24 +466,18 @@ int ofh_csr_get_pattern(ofh_csr A, int32_t* rp, int32_t* col) {
   return OFH_OK;
 }
 
-static void invalidate_values(ofh_csr A) {
-  A->val32_valid = false;
-  A->dinv_valid = false;
-  ++A->version;
-}
-
 int ofh_csr_set_values(ofh_csr A, const double* h) {
   OFH_CHECK(A && h, OFH_E_INVALID, "null argument");
   OFH_HIP(hipMemcpyAsync(A->val, h, sizeof(double) * A->nnz, hipMemcpyHostToDevice, A->ctx->stream));
   OFH_HIP(hipStreamSynchronize(A->ctx->stream));
-  invalidate_values(A);
+  csr_values_changed(A);
   return OFH_OK;
 }
 
 int ofh_csr_set_values_device(ofh_csr A, const double* d) {
   OFH_CHECK(A && d, OFH_E_INVALID, "null argument");
   OFH_HIP(hipMemcpyAsync(A->val, d, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice, A->ctx->stream));
-  invalidate_values(A);
+  csr_values_changed(A);
   return OFH_OK;
 }
 
@@ -489,7 +491,7 @@ int ofh_csr_get_values(ofh_csr A, double* h) {
 int ofh_csr_values_device(ofh_csr A, double** d) {
   OFH_CHECK(A && d, OFH_E_INVALID, "null argument");
   *d = A->val;
-  invalidate_values(A);  // the caller may write through the pointer
+  csr_values_changed(A);  // the caller may write through the pointer
   return OFH_OK;
 }
 

@@ -222,6 +222,9 @@ struct ofh_csr_s {
   int* dilu_tpos = nullptr;      // nnz: position of a_ji for each lower entry a_ij (-1: none)
   double* dilu_inv = nullptr;    // n: 1/D of the DILU factorization
   uint64_t dilu_version = ~0ull; // values version dilu_inv was computed from
+  ofh_csr_s* tr = nullptr;       // transpose (BiCG, TMult, TransSolve; lazy)
+  int* tr_perm = nullptr;        // nnz: position in A of each entry of tr
+  uint64_t tr_version = ~0ull;   // values version tr->val was gathered from
 };
 
 namespace ofh {
@@ -255,6 +258,13 @@ constexpr bool sweep_prec(int p) {
 int csr_ensure_prec(ofh_csr A, int prec);
 int launch_prec(ofh_csr A, int prec, const double* r, double* z, const int* d_done,
                 hipStream_t s);
+// z = M^-T r for ILU_PREC (Prec::TransSolve): two sweeps over A's transpose
+// with A's D-ILU pivots.
+int launch_dilu_trans(ofh_csr A, const double* r, double* z, const int* d_done, hipStream_t s);
+// A->tr, the CSR transpose of A, with values current (built on first use).
+int csr_ensure_transpose(ofh_csr A);
+// Everything derived from the values (FP32 copy, pivots, version) is stale.
+void csr_values_changed(ofh_csr A);
 int csr_ensure_dinv(ofh_csr A);
 int csr_ensure_levels(ofh_csr A);
 // Both CSR-stream row partitions (FP64 and FP32 tiles) from the host row_ptr.

@@ -797,10 +797,10 @@ int ofh_linear_solve(ofh_csr A, int solver, int prec, const double* b, double* x
       int m = std::max(1, std::min(A->n / 5, 50));
       return ofh_gmres(A, prec, b, x, m, max_it, toler, info);
     }
+    case OFH_CGS_SOLVER: return ofh_cgs(A, prec, b, x, max_it, toler, info);
+    case OFH_BICG_SOLVER: return ofh_bicg(A, prec, b, x, max_it, toler, info);
     case OFH_DIRECT_SOLVER:
-    case OFH_CGS_SOLVER:
-    case OFH_BICG_SOLVER:
-      return fail(OFH_E_UNSUPPORTED, "solver not provided by this path (CG, BICG_STAB, GMRES are)");
+      return fail(OFH_E_UNSUPPORTED, "DIRECT_SOLVER is not provided by this path");
     default: return fail(OFH_E_INVALID, "unknown solver");
   }
 }

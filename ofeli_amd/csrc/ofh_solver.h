@@ -18,6 +18,7 @@ namespace ofh {
 struct KrylovState {
   double rho_buf[2], alpha, beta, omega, nrm, res, tol;
   double res0, sconv_res, h_norm, rmax;
+  double bnorm, rnorm;  // BiCG: ||b|| as is, ||r|| of the last check
   int it, done, status, max_it;
   int sconv, cycle_i, halt, pause;  // halt = done || sconv || pause
   int replacements;

@@ -43,8 +43,11 @@ constexpr int kDepBatch = 8;       // entries of a row polled together
 constexpr unsigned long long kPending = 0x7FF4000000005EEDull;  // signalling NaN
 constexpr int kSpinLimit = 1 << 20;  // polls before a row gives up (about a second)
 
-enum SweepOp { kSgsFwd = 0, kSgsBwd = 1, kDiluFact = 2, kDiluFwd = 3, kDiluBwd = 4 };
-constexpr bool op_forward(int op) { return op != kSgsBwd && op != kDiluBwd; }
+// kTDiluFwd / kTDiluBwd run on the transposed matrix (Prec::TransSolve).
+enum SweepOp {
+  kSgsFwd = 0, kSgsBwd = 1, kDiluFact = 2, kDiluFwd = 3, kDiluBwd = 4, kTDiluFwd = 5, kTDiluBwd = 6
+};
+constexpr bool op_forward(int op) { return op != kSgsBwd && op != kDiluBwd && op != kTDiluBwd; }
 
 struct SweepArgs {
   const int* __restrict__ rp;
@@ -77,26 +80,32 @@ __global__ __launch_bounds__(kSweepWg) void k_fill_pending(double* __restrict__ 
 
 // One row of a sweep, in the reference's operation order.  The entries it
 // reads are taken kDepBatch at a time: their published values are loaded
-// together and re-polled until none reads as pending, then summed in order.
+// together and re-polled until none reads as pending, then folded in order.
 // Rows of one wave never read each other (see csr_ensure_levels), so a lane
-// may simply wait.
+// may simply wait.  Per op, with v the published value of the entry's column
+// c and w the entry's value:
+//   kSgsFwd/kDiluFwd/kSgsBwd/kDiluBwd  s = 0, s += w v (ascending c)
+//   kDiluFact  s = a_ii, s -= (w v) a_ci      (a_ci: the transposed entry)
+//   kTDiluFwd  s = in_i, s -= (inv_c v) w     (Prec_impl.h:273-278)
+//   kTDiluBwd  s = in_i, s -= w v, descending c (Prec_impl.h:279-283)
 template <int OP>
 __device__ __forceinline__ void sweep_row(int i, const SweepArgs& a) {
   constexpr bool FWD = op_forward(OP);
+  constexpr bool DESC = OP == kTDiluBwd;
   const int d = a.dpos[i];
   const int kb = FWD ? a.rp[i] : d + 1;
   const int ke = FWD ? d : a.rp[i + 1];
-  double s = OP == kDiluFact ? a.val[d] : 0.0;
+  double s = OP == kDiluFact ? a.val[d] : (OP == kTDiluFwd || OP == kTDiluBwd) ? a.in[i] : 0.0;
   int spins = 0;
   bool ok = true;
-  for (int k0 = kb; k0 < ke && ok; k0 += kDepBatch) {
+  for (int b0 = 0; b0 < ke - kb && ok; b0 += kDepBatch) {
     int c[kDepBatch];
     double w[kDepBatch], wt[kDepBatch];
 #pragma unroll
     for (int j = 0; j < kDepBatch; ++j) {
-      const int k = k0 + j;
+      const int k = DESC ? ke - 1 - (b0 + j) : kb + b0 + j;
       c[j] = -1;
-      if (k < ke) {
+      if (b0 + j < ke - kb) {
         if (OP == kDiluFact) {
           const int t = a.tpos[k];
           if (t >= 0) {
@@ -105,6 +114,7 @@ __device__ __forceinline__ void sweep_row(int i, const SweepArgs& a) {
           }
         } else {
           c[j] = a.col[k];
+          if (OP == kTDiluFwd) wt[j] = a.inv[c[j]];
         }
         w[j] = a.val[k];
       }
@@ -129,6 +139,10 @@ __device__ __forceinline__ void sweep_row(int i, const SweepArgs& a) {
       const double v = __longlong_as_double(u[j]);
       if (OP == kDiluFact)
         s -= w[j] * v * wt[j];
+      else if (OP == kTDiluFwd)
+        s -= wt[j] * v * w[j];
+      else if (OP == kTDiluBwd)
+        s -= w[j] * v;
       else
         s += w[j] * v;
     }
@@ -142,8 +156,12 @@ __device__ __forceinline__ void sweep_row(int i, const SweepArgs& a) {
     r = a.inv[i] * (a.in[i] - s);
   else if (OP == kSgsBwd)
     r = a.in[i] - s / a.val[d];
-  else
+  else if (OP == kDiluBwd)
     r = a.in[i] - a.inv[i] * s;
+  else if (OP == kTDiluFwd)
+    r = s;
+  else
+    r = a.inv[i] * s;
   store_pub(a.out + i, ok ? r : __builtin_nan(""));
 }
 
@@ -350,6 +368,26 @@ int launch_prec(ofh_csr A, int prec, const double* r, double* z, const int* d_do
   a.in = A->ssor_tmp;
   a.out = z;
   return sweep<kDiluBwd>(A, a, s);
+}
+
+// Prec::TransSolve for ILU_PREC (Prec_impl.h:269-284): z = M^-T r, the
+// forward sweep over the lower triangle of A^T (t_j -= (pivot_i z_i) a_ij,
+// i ascending) and the backward sweep over its upper triangle (z_j -= a_ij
+// x_i, i descending, then x_j = pivot_j z_j), with A's pivots.
+int launch_dilu_trans(ofh_csr A, const double* r, double* z, const int* d_done, hipStream_t s) {
+  int rc = csr_ensure_dilu(A);
+  if (rc || (rc = csr_ensure_transpose(A))) return rc;
+  ofh_csr T = A->tr;
+  if ((rc = csr_ensure_levels(T)) || (rc = ensure_tmp(T))) return rc;
+  SweepArgs a{};
+  a.done = d_done;
+  a.inv = A->dilu_inv;
+  a.in = r;
+  a.out = T->ssor_tmp;
+  if ((rc = sweep<kTDiluFwd>(T, a, s))) return rc;
+  a.in = T->ssor_tmp;
+  a.out = z;
+  return sweep<kTDiluBwd>(T, a, s);
 }
 
 }  // namespace ofh

@@ -1,11 +1,12 @@
-// Solvers.h -- Prec, CG, BiCGStab, GMRes and LinearSolver of the drop-in
-// layer (counterparts: include/solvers/Prec.h, CG.h:99-213,
-// BiCGStab.h:87-216, GMRes.h:148-298, LinearSolver.h:68-273 and
-// src/solvers/LinearSolver.cpp:63-244).  The iterations run on the GPU
-// (ofh_cg, ofh_bicgstab, ofh_gmres); the functions keep the reference's
-// signatures and return conventions: iteration count, 0 when the initial
-// guess already satisfies the test, -(max_it+1) without convergence,
-// 2 / 3 for BiCGStab breakdowns.
+// Solvers.h -- Prec, CG, CGS, BiCG, BiCGStab, GMRes and LinearSolver of the
+// drop-in layer (counterparts: include/solvers/Prec.h, CG.h:99-213,
+// CGS.h:91-228, BiCG.h:94-224, BiCGStab.h:87-216, GMRes.h:148-298,
+// LinearSolver.h:68-273 and src/solvers/LinearSolver.cpp:63-244).  The
+// iterations run on the GPU (ofh_cg, ofh_cgs, ofh_bicg, ofh_bicgstab,
+// ofh_gmres); the functions keep the reference's signatures and return
+// conventions: iteration count, 0 when the initial guess already satisfies
+// the test, -(max_it+1) without convergence (CGS: -max_it), 2 / 3 for
+// BiCGStab breakdowns, 2 for CGS's (s,r) = 0.
 #pragma once
 
 #include <cmath>
@@ -46,6 +47,21 @@ class Prec {
     Vect<T_> b(x);
     solve(b, x);
   }
+  // Prec::TransSolve(b,x) (Prec_impl.h:259-296): DIAG scales, ILU solves
+  // with M^T, IDENT/DILU/SSOR return b
+  void TransSolve(const Vect<T_>& b, Vect<T_>& x) const {
+    const size_t n = _a->size();
+    DeviceBuffer db, dx;
+    db.upload(b.data(), n);
+    dx.resize(n);
+    hip_check(ofh_prec_trans_apply(_a->device(), _type, db.get(), dx.get()), "Prec::TransSolve");
+    x.setSize(n);
+    dx.download(x.data(), n);
+  }
+  void TransSolve(Vect<T_>& x) const {
+    Vect<T_> b(x);
+    TransSolve(b, x);
+  }
 
  private:
   int _type = -1;
@@ -53,15 +69,22 @@ class Prec {
 };
 
 namespace detail {
-enum Kind { KCG, KBICGSTAB, KGMRES };
+enum Kind { KCG, KBICGSTAB, KGMRES, KCGS, KBICG };
+
+inline const char* kind_name(Kind k) {
+  switch (k) {
+    case KCG: return "CG";
+    case KBICGSTAB: return "BiCGStab";
+    case KGMRES: return "GMRes";
+    case KCGS: return "CGS";
+    default: return "BiCG";
+  }
+}
 
 template <class T_>
 int run_solver(Kind kind, const SpMatrix<T_>& A, int prec, const Vect<T_>& b, Vect<T_>& x,
                size_t m, int max_it, real_t toler, ofh_solve_info* out = nullptr) {
-  if (Verbosity > 2) {
-    const char* nm = kind == KCG ? "CG" : kind == KBICGSTAB ? "BiCGStab" : "GMRes";
-    std::printf("Running preconditioned %s method ...\n", nm);
-  }
+  if (Verbosity > 2) std::printf("Running preconditioned %s method ...\n", kind_name(kind));
   const size_t n = A.size();
   if (x.size() != n) x.setSize(n);
   DeviceBuffer db, dx;
@@ -72,8 +95,10 @@ int run_solver(Kind kind, const SpMatrix<T_>& A, int prec, const Vect<T_>& b, Ve
   int rc;
   if (kind == KCG) rc = ofh_cg(dA, prec, db.get(), dx.get(), max_it, toler, &info);
   else if (kind == KBICGSTAB) rc = ofh_bicgstab(dA, prec, db.get(), dx.get(), max_it, toler, &info);
+  else if (kind == KCGS) rc = ofh_cgs(dA, prec, db.get(), dx.get(), max_it, toler, &info);
+  else if (kind == KBICG) rc = ofh_bicg(dA, prec, db.get(), dx.get(), max_it, toler, &info);
   else rc = ofh_gmres(dA, prec, db.get(), dx.get(), (int)m, max_it, toler, &info);
-  hip_check(rc, kind == KCG ? "CG" : kind == KBICGSTAB ? "BiCGStab" : "GMRes");
+  hip_check(rc, kind_name(kind));
   dx.download(x.data(), n);
   if (out) *out = info;
   if (Verbosity > 2) {
@@ -95,6 +120,28 @@ int CG(const SpMatrix<T_>& A, int prec, const Vect<T_>& b, Vect<T_>& x, int max_
        real_t toler) {
   Prec<T_> p(A, prec);
   return CG(A, p, b, x, max_it, toler);
+}
+// CGS (CGS.h:91-228) as the reference writes it: its q is never assigned
+template <class T_>
+int CGS(const SpMatrix<T_>& A, const Prec<T_>& P, const Vect<T_>& b, Vect<T_>& x, int max_it,
+        real_t toler) {
+  return detail::run_solver(detail::KCGS, A, P.type(), b, x, 0, max_it, toler);
+}
+template <class T_>
+int CGS(const SpMatrix<T_>& A, int prec, const Vect<T_>& b, Vect<T_>& x, int max_it,
+        real_t toler) {
+  return CGS(A, Prec<T_>(A, prec), b, x, max_it, toler);
+}
+// BiCG (BiCG.h:94-224) with the true transposed products
+template <class T_>
+int BiCG(const SpMatrix<T_>& A, const Prec<T_>& P, const Vect<T_>& b, Vect<T_>& x, int max_it,
+         real_t toler) {
+  return detail::run_solver(detail::KBICG, A, P.type(), b, x, 0, max_it, toler);
+}
+template <class T_>
+int BiCG(const SpMatrix<T_>& A, int prec, const Vect<T_>& b, Vect<T_>& x, int max_it,
+         real_t toler) {
+  return BiCG(A, Prec<T_>(A, prec), b, x, max_it, toler);
 }
 template <class T_>
 int BiCGStab(const SpMatrix<T_>& A, const Prec<T_>& P, const Vect<T_>& b, Vect<T_>& x,
@@ -168,6 +215,12 @@ class LinearSolver {
       case CG_SOLVER:
         _nb_it = detail::run_solver(detail::KCG, *_A, _p, *_b, *_x, 0, _max_it, _toler, &_info);
         break;
+      case CGS_SOLVER:
+        _nb_it = detail::run_solver(detail::KCGS, *_A, _p, *_b, *_x, 0, _max_it, _toler, &_info);
+        break;
+      case BICG_SOLVER:
+        _nb_it = detail::run_solver(detail::KBICG, *_A, _p, *_b, *_x, 0, _max_it, _toler, &_info);
+        break;
       case BICG_STAB_SOLVER:
         _nb_it = detail::run_solver(detail::KBICGSTAB, *_A, _p, *_b, *_x, 0, _max_it, _toler,
                                     &_info);
@@ -181,8 +234,8 @@ class LinearSolver {
         break;
       }
       default:
-        throw OFELIException("In LinearSolver::solve(): solver not provided by the GPU path "
-                             "(CG, BiCGStab and GMRes are)");
+        throw OFELIException("In LinearSolver::solve(): DIRECT_SOLVER is not provided by the "
+                             "GPU path");
     }
     return _nb_it < 0 ? -_nb_it : 0;
   }

@@ -150,6 +150,17 @@ class SpMatrix {
     Mult(x, y);
     return y;
   }
+  // SpMatrix::TMult (SpMatrix_impl.h:761-767): y = A^T x.  y is overwritten;
+  // the reference adds into y and indexes the previous entry's column.
+  void TMult(const Vect<T_>& x, Vect<T_>& y) const {
+    y.setSize(_nrows);
+    ofh_csr A = device();
+    DeviceBuffer dx, dy;
+    dx.upload(x.data(), _nrows);
+    dy.resize(_nrows);
+    hip_check(ofh_spmv_t_f64(A, dx.get(), dy.get()), "SpMatrix::TMult");
+    dy.download(y.data(), _nrows);
+  }
 
   // SpMatrix::setSolver / solve (SpMatrix_impl.h:956-978); unlike the
   // reference, the max_it and toler given here are honoured.

@@ -114,6 +114,38 @@ int main() {
     int ret = eq.run();
     std::printf("{\"case\": \"elas2dq4\", \"ret\": %d, \"it\": %d, ", ret, eq.lastSolve().nb_it);
     print_vec("u", u);
+    std::printf("},\n");
+  }
+  // 5. unsymmetric SpMatrix: BiCG, CGS, SpMatrix::TMult, Prec::TransSolve
+  {
+    const size_t n = 40;
+    std::vector<size_t> rp(1, 0), ci;
+    std::vector<real_t> a;
+    for (size_t i = 1; i <= n; ++i) {
+      if (i > 1) { ci.push_back(i - 1); a.push_back(-1.3); }
+      ci.push_back(i); a.push_back(2.5);
+      if (i < n) { ci.push_back(i + 1); a.push_back(-0.7); }
+      rp.push_back(ci.size());
+    }
+    SpMatrix<real_t> A(rp, ci, a);
+    Vect<real_t> b(n), x1(n), x2(n), v(n), t(n), z(n);
+    for (size_t i = 1; i <= n; ++i) {
+      b(i) = 1.0 + 0.1 * i;
+      v(i) = std::sin(0.3 * i);
+    }
+    int itb = BiCG(A, DIAG_PREC, b, x1, 1000, 1e-12);
+    int itc = CGS(A, IDENT_PREC, b, x2, 5, 1e-12);
+    A.TMult(v, t);
+    Prec<real_t> P(A, ILU_PREC);
+    P.TransSolve(v, z);
+    std::printf("{\"case\": \"unsym\", \"it_bicg\": %d, \"it_cgs\": %d, ", itb, itc);
+    print_vec("x_bicg", x1);
+    std::printf(", ");
+    print_vec("x_cgs", x2);
+    std::printf(", ");
+    print_vec("tmult", t);
+    std::printf(", ");
+    print_vec("transsolve", z);
     std::printf("}\n");
   }
   std::printf("]\n");

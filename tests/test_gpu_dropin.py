@@ -87,3 +87,28 @@ def test_elas2dq4_program(cases):
     u = O.insert_bc(m.code, v)
     assert c["ret"] == 0 and abs(c["it"] - ret) <= 1
     assert rel(c["u"], u) < 1e-10
+
+
+def test_unsymmetric_program(cases):
+    """BiCG and CGS through the OFELI free functions, SpMatrix::TMult and
+    Prec::TransSolve on a 40-point unsymmetric tridiagonal matrix."""
+    c = cases["unsym"]
+    n = 40
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j, v in ((i - 1, -1.3), (i, 2.5), (i + 1, -0.7)):
+            if 0 <= j < n:
+                rows.append(i), cols.append(j), vals.append(v)
+    rp = np.searchsorted(rows, np.arange(n + 1)).astype(np.int64)
+    ci = np.array(cols, np.int64) + 1
+    a = np.array(vals)
+    b = 1.0 + 0.1 * np.arange(1, n + 1)
+    v = np.sin(0.3 * np.arange(1, n + 1))
+    ret, x, _ = O.bicg(rp, ci, a, b, prec=O.DIAG, max_it=1000, toler=1e-12)
+    assert ret > 0 and abs(c["it_bicg"] - ret) <= 1
+    assert rel(c["x_bicg"], x) < 1e-9
+    ret, x, _ = O.cgs(rp, ci, a, b, prec=O.IDENT, max_it=5, toler=1e-12)
+    assert c["it_cgs"] == ret == -5
+    assert rel(c["x_cgs"], x) < 1e-10
+    assert np.array_equal(c["tmult"], O.spmv_t(rp, ci, a, v))
+    assert np.array_equal(c["transsolve"], O.prec_trans_apply(rp, ci, a, O.ILU, v))
