@@ -46,6 +46,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-precs", action="store_true",
                     help="config 1: also time CG with the sweep preconditioners (DILU, SSOR)")
+    ap.add_argument("--krylov", action="store_true",
+                    help="config 1: also time BiCG, BiCGStab and CGS (DIAG_PREC)")
     ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3, 4))
     ap.add_argument("--nx4", type=int, default=2048, help="config 4 cells along the beam")
     ap.add_argument("--ny4", type=int, default=256, help="config 4 cells across the beam")
@@ -219,7 +221,47 @@ def main():
     }
     if args.sweep_precs:
         out["sweep_preconditioners"] = sweep_precs(args, oa, A, b, vals if cpu else None)
+    if args.krylov:
+        out["other_krylov"] = other_krylov(args, oa, A, b, ms64, it64, vals if cpu else None)
     print(json.dumps(out), flush=True)
+
+
+def other_krylov(args, oa, A, b, cg_ms, cg_it, vals):
+    """BiCG (with its A^T product), BiCGStab and CGS on the same system with
+    DIAG_PREC.  The reference's CGS does not converge here (its q stays 0),
+    so it runs a fixed 200 iterations and is reported per iteration."""
+    import torch
+    n = A.n
+    res = {"cg": {"solve_ms": round(cg_ms, 2), "iterations": cg_it,
+                  "us_per_iteration": round(cg_ms * 1e3 / cg_it, 1)}}
+    for name, fn, max_it, toler in (("bicg", oa.bicg, 20000, args.toler),
+                                    ("bicgstab", oa.bicgstab, 20000, args.toler),
+                                    ("cgs_200_iterations", oa.cgs, 200, 1e-300)):
+        ts = []
+        for rep in range(3):
+            x = torch.zeros(n, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            info = fn(A, b, x, prec=oa.DIAG_PREC, max_it=max_it, toler=toler)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        ms = 1e3 * min(ts[1:])
+        res[name] = {"solve_ms": round(ms, 2), "iterations": info.iterations, "nb_it": info.nb_it,
+                     "true_relres": info.true_relres,
+                     "us_per_iteration": round(ms * 1e3 / max(info.iterations, 1), 1)}
+        log("[bench] %s: %.2f ms, %d it (%.1f us/it), true %.2e"
+            % (name, ms, info.iterations, ms * 1e3 / max(info.iterations, 1), info.true_relres))
+    if vals is not None:
+        from oracle import oracle as O
+        rp, cl = A.pattern()
+        bh = b.cpu().numpy()
+        t = time.perf_counter()
+        O.bicg(rp.astype(np.int64), cl.astype(np.int64) + 1, vals, bh, prec=O.DIAG, max_it=20,
+               toler=1e-300, fast=True)
+        per_it = (time.perf_counter() - t) / 20
+        res["cpu_bicg_ms_per_iteration"] = round(per_it * 1e3, 2)
+        log("[bench] cpu BiCG+DIAG %.1f ms/it" % (per_it * 1e3))
+    return res
 
 
 def sweep_precs(args, oa, A, b, vals):
