@@ -233,6 +233,32 @@ int ofh_mesh2d_structured(double xmin, double xmax, double ymin, double ymax, in
 int ofh_mesh3d_kuhn(double xmin, double xmax, double ymin, double ymax, double zmin,
                     double zmax, int nx, int ny, int nz, int cx0, int cxN, int cy0, int cyN,
                     int cz0, int czN, double* h_coord, int32_t* h_code, int32_t* h_elem);
+/* OFELI XML mesh files (.m).  ofh_mesh_read is Mesh::get(file)
+ * (Mesh.cpp:2639-2670) as XMLParser::read_mesh / read_mesh_data parse it
+ * (XMLParser.cpp:1175-1220, 2210-2293): coordinates are rounded to single
+ * precision (stof), node and side marks become nb_dof codes by DOFCode
+ * (MeshUtil.cpp:101-116; a negative mark yields its digits, positive).
+ * Shapes use the reference's ElementShape values (OFELI_Config.h:272-283:
+ * LINE 2, TRIANGLE 3, QUADRILATERAL 4, TETRAHEDRON 5, ...).  Labels are
+ * 1-based; codes are nb_nodes x nb_dof, side codes nb_sides x nb_dof.
+ * Any output pointer of ofh_mesh_get may be NULL. */
+typedef struct ofh_mesh_s* ofh_mesh;
+int ofh_mesh_read(const char* path, ofh_mesh* out);
+int ofh_mesh_info(ofh_mesh m, int* dim, int* nb_dof, int* nb_nodes, int* nb_elements,
+                  int* el_shape, int* nen, int* nb_sides, int* sd_shape, int* nsn,
+                  int* nb_materials);
+int ofh_mesh_get(ofh_mesh m, double* h_coord, int32_t* h_code, int32_t* h_elem,
+                 int32_t* h_elem_code, int32_t* h_side, int32_t* h_side_code);
+int ofh_mesh_material(ofh_mesh m, int i, int* code, char* name, int name_len);
+int ofh_mesh_destroy(ofh_mesh m);
+/* Mesh::put(file) (Mesh.cpp:2734-2800), byte for byte; the <Material>
+ * section is written when there are several materials or the only one is
+ * not named "Generic". */
+int ofh_mesh_write(const char* path, int dim, int nb_dof, int nb_nodes, const double* h_coord,
+                   const int32_t* h_code, int nb_elements, int el_shape, int nen,
+                   const int32_t* h_elem, const int32_t* h_elem_code, int nb_sides,
+                   int sd_shape, int nsn, const int32_t* h_side, const int32_t* h_side_code,
+                   int nb_materials, const int32_t* mat_code, const char* const* mat_name);
 /* DOF numbering with imposed-DOF elimination, Mesh::NumberEquations with
  * removeImposedDOF (Mesh.cpp:1499-1509): returns nb_eq, labels 1-based. */
 int ofh_number_equations(int nb_nodes, int nb_dof, const int32_t* h_code, int32_t* h_dof,

@@ -7,6 +7,7 @@ library work are ordered without extra synchronisation.
 """
 import atexit
 import ctypes as C
+import os
 import weakref
 
 import numpy as np
@@ -249,6 +250,72 @@ def mesh2d(nx, ny, quad=False, nb_dof=1, codes=(1, 1, 1, 1), box=(0.0, 1.0, 0.0,
                                       code.ctypes.data_as(C.POINTER(C.c_int32)),
                                       elem.ctypes.data_as(C.POINTER(C.c_int32))))
     return coord, code, elem
+
+
+class MeshFile:
+    """An OFELI XML mesh (.m) as read by Mesh::get(file): coord (nn x dim),
+    code (nn x nb_dof), elem (ne x nen, 1-based), elem_code, sides
+    (ns x nsn, 1-based), side_code (ns x nb_dof), shapes as ElementShape
+    values (LINE 2, TRIANGLE 3, QUADRILATERAL 4, TETRAHEDRON 5, ...) and
+    the (code, name) materials."""
+
+    def __init__(self, dim, nb_dof, coord, code, elem, elem_code, el_shape, sides, side_code,
+                 sd_shape, materials=()):
+        self.dim, self.nb_dof = int(dim), int(nb_dof)
+        self.coord, self.code, self.elem, self.elem_code = coord, code, elem, elem_code
+        self.sides, self.side_code = sides, side_code
+        self.el_shape, self.sd_shape = int(el_shape), int(sd_shape)
+        self.materials = list(materials)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def read_mesh(path):
+    """Mesh::get(file) for the OFELI XML format (C++ reader)."""
+    h = C.c_void_p()
+    check(lib().ofh_mesh_read(os.fsencode(path), C.byref(h)))
+    try:
+        v = [C.c_int() for _ in range(10)]
+        check(lib().ofh_mesh_info(h, *[C.byref(x) for x in v]))
+        dim, nb_dof, nn, ne, el_shape, nen, ns, sd_shape, nsn, nm = [x.value for x in v]
+        coord = np.zeros((nn, dim))
+        code = np.zeros((nn, nb_dof), np.int32)
+        elem = np.zeros((ne, nen), np.int32)
+        elem_code = np.zeros(ne, np.int32)
+        sides = np.zeros((ns, max(nsn, 1)), np.int32)
+        side_code = np.zeros((ns, nb_dof), np.int32)
+        ptr = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+        check(lib().ofh_mesh_get(h, ptr(coord, C.c_double), ptr(code, C.c_int32),
+                                 ptr(elem, C.c_int32), ptr(elem_code, C.c_int32),
+                                 ptr(sides, C.c_int32), ptr(side_code, C.c_int32)))
+        mats = []
+        for i in range(nm):
+            c, name = C.c_int(), C.create_string_buffer(256)
+            check(lib().ofh_mesh_material(h, i, C.byref(c), name, 256))
+            mats.append((c.value, name.value.decode()))
+    finally:
+        lib().ofh_mesh_destroy(h)
+    return MeshFile(dim, nb_dof, coord, code, elem, elem_code, el_shape, sides[:, :nsn], side_code,
+                    sd_shape, mats)
+
+
+def write_mesh(path, m):
+    """Mesh::put(file), byte for byte (C++ writer)."""
+    coord = np.ascontiguousarray(m.coord, np.float64)
+    code, elem, ec = _i32(m.code), _i32(m.elem), _i32(m.elem_code)
+    sides, sc = _i32(m.sides), _i32(m.side_code)
+    mc = _i32([c for c, _ in m.materials])
+    names = (C.c_char_p * max(len(m.materials), 1))(*[n.encode() for _, n in m.materials])
+    ptr = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+    nsn = sides.shape[1] if sides.ndim == 2 and sides.shape[0] else 0
+    check(lib().ofh_mesh_write(os.fsencode(path), m.dim, m.nb_dof, coord.shape[0],
+                               ptr(coord, C.c_double), ptr(code, C.c_int32), elem.shape[0],
+                               m.el_shape, elem.shape[1] if elem.ndim == 2 else 0,
+                               ptr(elem, C.c_int32), ptr(ec, C.c_int32), sides.shape[0],
+                               m.sd_shape, nsn, ptr(sides, C.c_int32), ptr(sc, C.c_int32),
+                               len(m.materials), ptr(mc, C.c_int32), names))
 
 
 def mesh3d_kuhn(nx, ny, nz, codes=(1, 1, 1, 1, 1, 1), box=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0)):

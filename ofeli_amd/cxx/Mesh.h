@@ -4,10 +4,12 @@
 // boundary codes (Mesh.cpp:547-676 for rectangles; Mesh.cpp:678-726 node
 // layout for boxes, with a positively oriented Kuhn tetrahedral split), DOF
 // numbering with imposed-DOF elimination (Mesh.cpp:1483-1527,
-// removeImposedDOF Mesh.h:420) and the boundary sides of rectangles.
+// removeImposedDOF Mesh.h:420), the boundary sides of rectangles, and the
+// OFELI XML mesh files (.m): Mesh(file) / get(file) and put(file) / save(file).
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "OFELI_Config.h"
@@ -71,6 +73,81 @@ class Mesh {
     NumberEquations();
   }
 
+  // Mesh(file) and Mesh::get(file) for the OFELI XML format (Mesh.cpp:88-100,
+  // 2639-2670): coordinates rounded to single precision as the reference
+  // reads them; equations numbered as after Mesh::get.
+  explicit Mesh(const std::string& file) { get(file); }
+  void get(const std::string& file) {
+    ofh_mesh h = nullptr;
+    hip_check(ofh_mesh_read(file.c_str(), &h), "Mesh::get");
+    int dim = 0, nb_dof = 0, nn = 0, ne = 0, es = 0, nen = 0, ns = 0, ss = 0, nsn = 0, nm = 0;
+    ofh_mesh_info(h, &dim, &nb_dof, &nn, &ne, &es, &nen, &ns, &ss, &nsn, &nm);
+    _dim = dim;
+    _nb_dof = nb_dof;
+    _nen = nen;
+    _el_shape = es;
+    _sd_shape = ss;
+    _coord.assign((size_t)nn * dim, 0.0);
+    _code.assign((size_t)nn * nb_dof, 0);
+    _elem.assign((size_t)ne * nen, 0);
+    _elem_code.assign((size_t)ne, 0);
+    _sides.assign((size_t)ns * nsn, 0);
+    _side_codes.assign((size_t)ns * nb_dof, 0);
+    ofh_mesh_get(h, _coord.data(), _code.data(), _elem.data(), _elem_code.data(), _sides.data(),
+                 _side_codes.data());
+    _materials.clear();
+    for (int i = 0; i < nm; ++i) {
+      int c = 0;
+      char name[256];
+      ofh_mesh_material(h, i, &c, name, (int)sizeof(name));
+      _materials.emplace_back(c, name);
+    }
+    ofh_mesh_destroy(h);
+    _side_code.resize(ns);
+    for (int s = 0; s < ns; ++s) _side_code[s] = _side_codes[(size_t)s * nb_dof];
+    _no_imposed_dof = false;
+    NumberEquations();
+    touch();
+  }
+  // Mesh::put (Mesh.cpp:2734-2800) and Mesh::save(file) for ".m" names
+  void put(const std::string& file) const {
+    std::vector<int32_t> ec = _elem_code, sc = _side_codes;
+    if (ec.size() != getNbElements()) ec.assign(getNbElements(), 1);
+    if (sc.size() != getNbSides() * _nb_dof) {
+      sc.assign(getNbSides() * _nb_dof, 0);
+      for (size_t s = 0; s < getNbSides(); ++s) sc[s * _nb_dof] = _side_code[s];
+    }
+    std::vector<int32_t> mc;
+    std::vector<const char*> mn;
+    for (auto& m : _materials) {
+      mc.push_back(m.first);
+      mn.push_back(m.second.c_str());
+    }
+    const int nsn = getNbSides() ? (int)(_sides.size() / getNbSides()) : 0;
+    hip_check(ofh_mesh_write(file.c_str(), _dim, _nb_dof, (int)getNbNodes(), _coord.data(),
+                             _code.data(), (int)getNbElements(), elementShape(), _nen,
+                             _elem.data(), ec.data(), (int)getNbSides(), sideShape(), nsn,
+                             _sides.data(), sc.data(), (int)mc.size(), mc.data(), mn.data()),
+              "Mesh::put");
+  }
+  void save(const std::string& file) const {
+    if (file.size() < 2 || file.compare(file.size() - 2, 2, ".m") != 0)
+      throw OFELIException("Mesh::save: only the OFELI format (.m) is provided");
+    put(file);
+  }
+  // ElementShape values (OFELI_Config.h:272-283)
+  int elementShape() const {
+    if (_el_shape) return _el_shape;
+    if (_nen == 2) return 2;
+    if (_dim == 2) return _nen == 4 ? 4 : 3;
+    return 5;
+  }
+  int sideShape() const { return _sd_shape ? _sd_shape : (_dim == 3 ? 3 : 2); }
+  int getElementCode(size_t e) const {
+    return e - 1 < _elem_code.size() ? _elem_code[e - 1] : 1;
+  }
+  const std::vector<std::pair<int, std::string>>& materials() const { return _materials; }
+
   size_t getDim() const { return _dim; }
   size_t getNbNodes() const { return _coord.size() / (_dim ? _dim : 1); }
   size_t getNbElements() const { return _nen ? _elem.size() / _nen : 0; }
@@ -124,6 +201,9 @@ class Mesh {
 
  private:
   int _dim = 0, _nb_dof = 1, _nen = 0, _nx = 0, _ny = 0;
+  int _el_shape = 0, _sd_shape = 0;  // from a mesh file (0: by dimension)
+  std::vector<int32_t> _elem_code, _side_codes;
+  std::vector<std::pair<int, std::string>> _materials;
   size_t _nb_eq = 0;
   bool _no_imposed_dof = false;
   uint64_t _rev = 0;

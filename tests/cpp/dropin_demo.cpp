@@ -20,7 +20,7 @@ static void print_vec(const char* name, const Vect<real_t>& v) {
   std::printf("]");
 }
 
-int main() {
+int main(int argc, char** argv) {
   std::printf("[\n");
   // 1. Laplace2DT3 with harmonic Dirichlet data (demos/laplace/laplace_demo1.cpp)
   {
@@ -146,6 +146,26 @@ int main() {
     print_vec("tmult", t);
     std::printf(", ");
     print_vec("transsolve", z);
+    std::printf("}%s\n", argc > 1 ? "," : "");
+  }
+  // 6. laplace_demo1 on a mesh file (argv[1]: the reference's laplace/square.m)
+  if (argc > 1) {
+    Mesh ms(argv[1]);
+    Vect<real_t> u(ms), bc(ms);
+    for (size_t n = 1; n <= ms.getNbNodes(); ++n) {
+      real_t x = ms.getCoord(n, 0), y = ms.getCoord(n, 1);
+      bc(n) = std::exp(x) * std::sin(y) + std::exp(y) * std::cos(x);
+    }
+    Laplace2DT3 eq(ms, u);
+    eq.setInput(EType::BOUNDARY_CONDITION, bc);
+    eq.setSolver(CG_SOLVER, DIAG_PREC);
+    eq.setLinearTolerance(1e-12);
+    int ret = eq.run();
+    if (argc > 2) ms.put(argv[2]);
+    std::printf("{\"case\": \"mesh_file\", \"ret\": %d, \"it\": %d, \"nodes\": %zu, "
+                "\"elements\": %zu, \"sides\": %zu, ", ret, eq.lastSolve().nb_it,
+                ms.getNbNodes(), ms.getNbElements(), ms.getNbSides());
+    print_vec("u", u);
     std::printf("}\n");
   }
   std::printf("]\n");

@@ -12,15 +12,19 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 EXE = os.path.join(HERE, "cpp", "dropin_demo")
+SQUARE = os.path.join(HERE, "golden", "meshes", "square.m")
 
 
 @pytest.fixture(scope="module")
-def cases():
+def cases(tmp_path_factory):
     if not os.path.exists(EXE):
         pytest.fail("tests/cpp/dropin_demo not built (run tests/cpp/build.sh)")
-    r = subprocess.run([EXE], capture_output=True, text=True, timeout=120)
+    out = str(tmp_path_factory.mktemp("dropin") / "square_put.m")
+    r = subprocess.run([EXE, SQUARE, out], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    return {c["case"]: c for c in json.loads(r.stdout)}
+    cs = {c["case"]: c for c in json.loads(r.stdout)}
+    cs["mesh_file"]["put"] = out
+    return cs
 
 
 def rel(a, b):
@@ -112,3 +116,25 @@ def test_unsymmetric_program(cases):
     assert rel(c["x_cgs"], x) < 1e-10
     assert np.array_equal(c["tmult"], O.spmv_t(rp, ci, a, v))
     assert np.array_equal(c["transsolve"], O.prec_trans_apply(rp, ci, a, O.ILU, v))
+
+
+def test_mesh_file_program(cases):
+    """laplace_demo1's pattern on the reference's own laplace/square.m read
+    through Mesh(file): the solution equals the oracle's on the oracle's
+    reading of the file, and Mesh::put writes what the oracle writes."""
+    from oracle import mesh_io as MIO
+    c = cases["mesh_file"]
+    r = MIO.read(SQUARE)
+    m = O.Mesh(r["coord"], r["code"], r["elem"].astype(np.int64), 1)
+    x, y = m.coord[:, 0], m.coord[:, 1]
+    ue = np.exp(x) * np.sin(y) + np.exp(y) * np.cos(x)
+    dof, neq = O.number_equations(m.code)
+    rp, ci = O.node_graph(m.elem, 1, dof, neq)
+    a, b = O.assemble(O.LAPLACE2DT3, m, dof, neq, rp, ci, bc=ue)
+    ret, v, _ = O.cg(rp, ci, a, b, max_it=2000, toler=1e-12)
+    u = O.insert_bc(m.code, v, ue)
+    assert (c["nodes"], c["elements"], c["sides"]) == (142, 242, 40)
+    assert c["ret"] == 0 and abs(c["it"] - ret) <= 1
+    assert rel(c["u"], u) < 1e-10
+    MIO.write(c["put"] + ".oracle", r)
+    assert open(c["put"], "rb").read() == open(c["put"] + ".oracle", "rb").read()
