@@ -48,6 +48,9 @@ def main():
                     help="config 1: also time CG with the sweep preconditioners (DILU, SSOR)")
     ap.add_argument("--krylov", action="store_true",
                     help="config 1: also time BiCG, BiCGStab and CGS (DIAG_PREC)")
+    ap.add_argument("--mesh-io", action="store_true",
+                    help="config 1: also time the .m mesh writer/reader on a 1024^2 P1 mesh and "
+                         "the GPU plan + assembly of the mesh read back")
     ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3, 4))
     ap.add_argument("--nx4", type=int, default=2048, help="config 4 cells along the beam")
     ap.add_argument("--ny4", type=int, default=256, help="config 4 cells across the beam")
@@ -223,7 +226,55 @@ def main():
         out["sweep_preconditioners"] = sweep_precs(args, oa, A, b, vals if cpu else None)
     if args.krylov:
         out["other_krylov"] = other_krylov(args, oa, A, b, ms64, it64, vals if cpu else None)
+    if args.mesh_io:
+        out["mesh_io"] = mesh_io_leg(args, oa, ctx)
     print(json.dumps(out), flush=True)
+
+
+def mesh_io_leg(args, oa, ctx, n=1024):
+    """OFELI .m files: Mesh::put and Mesh::get (C++ writer / reader) on a
+    structured n x n P1 mesh, then the GPU plan (graph + numbering) and a
+    Laplace2DT3 assembly of the mesh read back; the oracle's Python reader
+    on the same file as the CPU baseline."""
+    import tempfile
+    import torch
+    from ofeli_amd.api import MeshFile
+    coord, code, elem = oa.api.mesh2d(n, n)
+    m = MeshFile(2, 1, coord, code, elem, np.ones(len(elem), np.int32), 3,
+                 np.zeros((0, 2), np.int32), np.zeros((0, 1), np.int32), 2)
+    d = tempfile.mkdtemp()
+    path = os.path.join(d, "mesh.m")
+    t = time.perf_counter()
+    oa.write_mesh(path, m)
+    tw = time.perf_counter() - t
+    size = os.path.getsize(path)
+    t = time.perf_counter()
+    r = oa.read_mesh(path)
+    tr = time.perf_counter() - t
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    P = oa.AssemblyPlan(ctx, r.coord, r.elem, r.code)
+    b = torch.zeros(P.nb_eq, dtype=torch.float64, device="cuda")
+    P.build(oa.LAPLACE2DT3, f=torch.ones(len(r.coord), dtype=torch.float64, device="cuda"),
+            matrix=True, b=b)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t
+    res = {"nodes": len(r.coord), "elements": len(r.elem), "file_MB": round(size / 1e6, 1),
+           "write_s": round(tw, 3), "read_s": round(tr, 3),
+           "read_MB_per_s": round(size / 1e6 / tr, 1), "write_MB_per_s": round(size / 1e6 / tw, 1),
+           "plan_and_assembly_s": round(tp, 3), "rows": P.nb_eq}
+    log("[bench] mesh io: %.1f MB, write %.3f s, read %.3f s (%.0f MB/s), plan+assembly %.3f s"
+        % (size / 1e6, tw, tr, size / 1e6 / tr, tp))
+    if not args.no_cpu:
+        from oracle import mesh_io as MIO
+        t = time.perf_counter()
+        MIO.read(path)
+        to = time.perf_counter() - t
+        res["cpu_oracle_read_s"] = round(to, 2)
+        log("[bench] oracle (Python) read %.2f s" % to)
+    os.remove(path)
+    os.rmdir(d)
+    return res
 
 
 def other_krylov(args, oa, A, b, cg_ms, cg_it, vals):

@@ -20,6 +20,7 @@
 // code_1 < 0) as "%10zu", element rows "%8d" x (nodes + code), side rows
 // "%8d" x nodes and the mark sum_j code_j 10^(n-j) (size_t) as "%6zu".
 #include <cerrno>
+#include <charconv>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -74,7 +75,14 @@ bool next_token(const char*& p, const char* e, const char*& tb, const char*& te)
   return true;
 }
 
+// stoi / stof of one token: the longest numeric prefix, an error when there
+// is none or the value is out of range.  A token that from_chars reads whole
+// gets the same correctly rounded value as strtof; any other token (a leading
+// '+', a hexadecimal float, trailing characters) goes through the C library.
 bool to_int(const char* b, const char* e, int& v) {
+  const auto r = std::from_chars(b, e, v);
+  if (r.ec == std::errc() && r.ptr == e) return true;
+  if (r.ec == std::errc::result_out_of_range) return false;
   char buf[64];
   const size_t n = (size_t)(e - b);
   if (n == 0 || n >= sizeof(buf)) return false;
@@ -82,13 +90,20 @@ bool to_int(const char* b, const char* e, int& v) {
   buf[n] = 0;
   char* q;
   errno = 0;
-  const long x = std::strtol(buf, &q, 10);  // stoi: leading digits, range-checked
+  const long x = std::strtol(buf, &q, 10);
   if (q == buf || errno == ERANGE || x < INT32_MIN || x > INT32_MAX) return false;
   v = (int)x;
   return true;
 }
 
 bool to_float(const char* b, const char* e, double& v) {
+  float f;
+  const auto r = std::from_chars(b, e, f);
+  if (r.ec == std::errc() && r.ptr == e) {
+    v = (double)f;
+    return true;
+  }
+  if (r.ec == std::errc::result_out_of_range) return false;
   char buf[64];
   const size_t n = (size_t)(e - b);
   if (n == 0 || n >= sizeof(buf)) return false;
@@ -96,7 +111,7 @@ bool to_float(const char* b, const char* e, double& v) {
   buf[n] = 0;
   char* q;
   errno = 0;
-  const float x = std::strtof(buf, &q);  // stof: out of float range is an error
+  const float x = std::strtof(buf, &q);
   if (q == buf || errno == ERANGE) return false;
   v = (double)x;
   return true;
@@ -259,10 +274,23 @@ bool parse(const std::string& text, ofh_mesh_s& m, std::string& err) {
   return true;
 }
 
-void put_pow_mark(std::string& out, const char* fmt, size_t m) {
+// right-aligned fields as iostream's setw (the writer's hot loops)
+template <class T>
+void put_int(std::string& out, T v, int width) {
   char buf[32];
-  std::snprintf(buf, sizeof(buf), fmt, m);
-  out += buf;
+  const auto r = std::to_chars(buf, buf + sizeof(buf), v);
+  const int n = (int)(r.ptr - buf);
+  if (n < width) out.append((size_t)(width - n), ' ');
+  out.append(buf, (size_t)n);
+}
+
+// "%18.8e": to_chars gives printf's correctly rounded digits and exponent
+void put_sci(std::string& out, double v) {
+  char buf[48];
+  const auto r = std::to_chars(buf, buf + sizeof(buf), v, std::chars_format::scientific, 8);
+  const int n = (int)(r.ptr - buf);
+  if (n < 18) out.append((size_t)(18 - n), ' ');
+  out.append(buf, (size_t)n);
 }
 
 }  // namespace
@@ -363,14 +391,15 @@ int ofh_mesh_write(const char* path, int dim, int nb_dof, int nb_nodes, const do
     out += "   <Nodes>\n";
     for (int64_t i = 0; i < nb_nodes; ++i) {
       for (int d = 0; d < dim; ++d) {
-        std::snprintf(buf, sizeof(buf), "  %18.8e", coord[i * dim + d]);
-        out += buf;
+        out += "  ";
+        put_sci(out, coord[i * dim + d]);
       }
       const int32_t* c = code + i * nb_dof;
       size_t mk = 0;
       for (int j = 0; j < nb_dof; ++j) mk += (size_t)std::abs(c[j]) * pw[j];
       if (c[0] < 0) mk *= (size_t)-1;  // size_t m; m *= sign (Mesh.cpp:2762-2767)
-      put_pow_mark(out, "%10zu\n", mk);
+      put_int(out, mk, 10);
+      out += '\n';
     }
     out += "   </Nodes>\n";
   }
@@ -379,12 +408,9 @@ int ofh_mesh_write(const char* path, int dim, int nb_dof, int nb_nodes, const do
                   kShapeName[el_shape], nen);
     out += buf;
     for (int64_t e = 0; e < nb_elements; ++e) {
-      for (int k = 0; k < nen; ++k) {
-        std::snprintf(buf, sizeof(buf), "%8d", elem[e * nen + k]);
-        out += buf;
-      }
-      std::snprintf(buf, sizeof(buf), "%8d\n", elem_code[e]);
-      out += buf;
+      for (int k = 0; k < nen; ++k) put_int(out, elem[e * nen + k], 8);
+      put_int(out, elem_code[e], 8);
+      out += '\n';
     }
     out += "   </Elements>\n";
   }
@@ -393,13 +419,11 @@ int ofh_mesh_write(const char* path, int dim, int nb_dof, int nb_nodes, const do
                   nsn);
     out += buf;
     for (int64_t s = 0; s < nb_sides; ++s) {
-      for (int k = 0; k < nsn; ++k) {
-        std::snprintf(buf, sizeof(buf), "%8d", side[s * nsn + k]);
-        out += buf;
-      }
+      for (int k = 0; k < nsn; ++k) put_int(out, side[s * nsn + k], 8);
       size_t mk = 0;
       for (int j = 0; j < nb_dof; ++j) mk += (size_t)(int64_t)side_code[s * nb_dof + j] * pw[j];
-      put_pow_mark(out, "%6zu\n", mk);
+      put_int(out, mk, 6);
+      out += '\n';
     }
     out += "   </Sides>\n";
   }
