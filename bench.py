@@ -117,7 +117,7 @@ def main():
     spmv_ms = ev0.elapsed_time(ev1) / args.spmv_reps
     b_spmv64 = 12 * nnz + 4 * (n + 1) + 16 * n
     spmv_gbs = b_spmv64 / (spmv_ms * 1e-3) / 1e9
-    log("[bench] spmv64 %.4f ms  %.0f GB/s (32-bit columns)" % (spmv_ms, spmv_gbs))
+    log("[bench] spmv64 %.4f ms  %.0f GB/s (SELL-64)" % (spmv_ms, spmv_gbs))
     A.compress_columns(True)
     ev0.record()
     for _ in range(args.spmv_reps):
@@ -126,7 +126,7 @@ def main():
     torch.cuda.synchronize()
     spmv_c16_ms = ev0.elapsed_time(ev1) / args.spmv_reps
     A.compress_columns(False)
-    log("[bench] spmv64 %.4f ms (16-bit column offsets)" % spmv_c16_ms)
+    log("[bench] spmv64 %.4f ms (CSR-stream with 16-bit column offsets)" % spmv_c16_ms)
     # streaming reference: a device-to-device copy of 1 GiB
     big = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
     big2 = torch.empty_like(big)
@@ -217,8 +217,8 @@ def main():
                      "unit": "GB/s", "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
                      # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE runs recorded in
                      # profiles/spmv_cg_poisson3d_96cube.md (96^3 system only)
-                     "traffic": 171.4e6 if args.n == 96 else None,
-                     "kernel": "k_spmv<double> (CSR-stream FP64)",
+                     "traffic": 167.8e6 if args.n == 96 else None,
+                     "kernel": "k_spmv_sell<double, 2> (SELL-64 FP64)",
                      "bytes_per_launch": b_spmv64, "avg_ms": round(spmv_ms, 4)},
         "cpu_baseline": cpu,
     }

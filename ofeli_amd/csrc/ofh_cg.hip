@@ -285,6 +285,32 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(int nblocks, const int* __re
   write_partials<1>(v, slot, partials, blockIdx.x);
 }
 
+// q = A p from the SELL-64 copy, partials of (p,q) -> slot 3
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_sell(int n, int nblocks,
+                                                        const int* __restrict__ cptr,
+                                                        const int* __restrict__ clen,
+                                                        const uint32_t* __restrict__ col,
+                                                        const double* __restrict__ val,
+                                                        const double* __restrict__ p,
+                                                        double* __restrict__ q,
+                                                        const KrylovState* __restrict__ st,
+                                                        double* __restrict__ partials) {
+  if (st->done) return;
+  __shared__ double red[kBlock / kWave];
+  const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
+  double v[1] = {0.0};
+  if (row / kSellChunk * kSellChunk < n) {  // whole waves past the end only reduce
+    const double s = sell_rowsum<double, 2>(row, cptr, clen, col, val, x_rsrc(p, n));
+    if (row < n) {
+      q[row] = s;
+      v[0] = p[row] * s;
+    }
+  }
+  block_sum_k<kBlock, 1>(v, red);
+  const int slot[1] = {3};
+  write_partials<1>(v, slot, partials, blockIdx.x);
+}
+
 // alpha = rho/(p,q); x += alpha p; r -= alpha q; partials (r,r) -> 0 and
 // (r,z) -> 1 (CG.h:135-138)
 template <int PREC>
@@ -493,11 +519,20 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   if ((rc = read_state(ctx, &h))) return rc;
   int enq = 0, it_prev = 0;
   double res_prev = h.res;
+  // the SpMV from the SELL-64 copy when it is in use (one partial per 256 rows)
+  bool sell = false;
+  const int nbs = (n + kBlock - 1) / kBlock;
+  if (nbs <= kMaxPartials && (rc = csr_ensure_sell(A, false, &sell))) return rc;
+  const int npq = sell ? nbs : nb;
   // one iteration: k_cg_spmv, k_cg_upd(it), [SSOR sweeps], k_cg_dir(it)
   auto iteration = [&](int it, hipStream_t st_) -> int {
-    hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, st_, nb, A->row_blocks, A->row_ptr,
-                       ColIndex{A->col, A->col16}, A->val, p, q, st, part);
-    hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, it, nb, x, r, p, p, q,
+    if (sell)
+      hipLaunchKernelGGL(k_cg_spmv_sell, dim3(nbs), dim3(kBlock), 0, st_, n, nbs, A->sell_cptr,
+                         A->sell_clen, A->sell_col, A->sell_val, p, q, st, part);
+    else
+      hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, st_, nb, A->row_blocks,
+                         A->row_ptr, ColIndex{A->col, A->col16}, A->val, p, q, st, part);
+    hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, it, npq, x, r, p, p, q,
                        A->dinv, nullptr, st, part);
     if (sweep_prec(PREC)) {
       const int e = ssor_z(it, false);

@@ -445,6 +445,11 @@ int ofh_csr_destroy(ofh_csr A) {
   hipFree(A->dilu_inv);
   hipFree(A->tr_perm);
   ofh_csr_destroy(A->tr);
+  hipFree(A->sell_cptr);
+  hipFree(A->sell_clen);
+  hipFree(A->sell_col);
+  hipFree(A->sell_val);
+  hipFree(A->sell_val32);
   delete A;
   return OFH_OK;
 }

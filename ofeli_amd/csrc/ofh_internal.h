@@ -222,6 +222,17 @@ struct ofh_csr_s {
   int* dilu_tpos = nullptr;      // nnz: position of a_ji for each lower entry a_ij (-1: none)
   double* dilu_inv = nullptr;    // n: 1/D of the DILU factorization
   uint64_t dilu_version = ~0ull; // values version dilu_inv was computed from
+  // SELL-64 copy for the streaming SpMVs (lazy): rows in chunks of 64 (one
+  // wave), each chunk padded to its longest row and stored position-major
+  int sell_state = 0;            // 0: not built, 1: in use, -1: too much padding
+  int sell_nchunks = 0;
+  int64_t sell_total = 0;        // stored entries (padding included)
+  int* sell_cptr = nullptr;      // nchunks: first stored entry of each chunk
+  int* sell_clen = nullptr;      // nchunks: stored entries per row of the chunk
+  uint32_t* sell_col = nullptr;  // column index (padding: kSellPadCol)
+  double* sell_val = nullptr;    // padding: -0.0
+  float* sell_val32 = nullptr;
+  uint64_t sell_version = ~0ull, sell_version32 = ~0ull;
   ofh_csr_s* tr = nullptr;       // transpose (BiCG, TMult, TransSolve; lazy)
   int* tr_perm = nullptr;        // nnz: position in A of each entry of tr
   uint64_t tr_version = ~0ull;   // values version tr->val was gathered from
@@ -261,6 +272,10 @@ int launch_prec(ofh_csr A, int prec, const double* r, double* z, const int* d_do
 // z = M^-T r for ILU_PREC (Prec::TransSolve): two sweeps over A's transpose
 // with A's D-ILU pivots.
 int launch_dilu_trans(ofh_csr A, const double* r, double* z, const int* d_done, hipStream_t s);
+// The SELL-64 copy with current FP64 (fp32: FP32) values; *ok is false when
+// the pattern pads too much (or 16-bit columns are on) and the CSR-stream
+// kernels are to be used.
+int csr_ensure_sell(ofh_csr A, bool fp32, bool* ok);
 // A->tr, the CSR transpose of A, with values current (built on first use).
 int csr_ensure_transpose(ofh_csr A);
 // Everything derived from the values (FP32 copy, pivots, version) is stale.

@@ -165,6 +165,32 @@ __global__ __launch_bounds__(kBlock) void k_mx_dir(int n, int k_arg, int nparts,
   }
 }
 
+// q32 = A32 p32 from the SELL-64 copy; (p,q) -> slot 6 (FP64 accumulation)
+__global__ __launch_bounds__(kBlock) void k_mx_spmv_sell(int n, int nblocks,
+                                                        const int* __restrict__ cptr,
+                                                        const int* __restrict__ clen,
+                                                        const uint32_t* __restrict__ col,
+                                                        const float* __restrict__ val32,
+                                                        const float* __restrict__ p32,
+                                                        float* __restrict__ q32,
+                                                        const KrylovState* __restrict__ st,
+                                                        double* __restrict__ partials) {
+  if (st->halt) return;
+  __shared__ double red[kBlock / kWave];
+  const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
+  double v[1] = {0.0};
+  if (row / kSellChunk * kSellChunk < n) {  // whole waves past the end only reduce
+    const float s = sell_rowsum<float, 4>(row, cptr, clen, col, val32, x_rsrc(p32, n));
+    if (row < n) {
+      q32[row] = s;
+      v[0] = (double)p32[row] * (double)s;
+    }
+  }
+  block_sum_k<kBlock, 1>(v, red);
+  const int slot[1] = {6};
+  write_partials<1>(v, slot, partials, blockIdx.x);
+}
+
 // q32 = A32 p32; (p,q) -> slot 6 (FP64 accumulation)
 __global__ __launch_bounds__(kBlock) void k_mx_spmv(int nblocks, const int* __restrict__ rb,
                                                    const int* __restrict__ rp,
@@ -416,6 +442,10 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   const int nb32 = A->nblocks32;  // FP32 partition (inner SpMV)
   const ColIndex ci32{A->col, nullptr};  // 16-bit offsets follow the FP64 partition
   const int ge = (n + 255) / 256;
+  // the unfused inner SpMV from the SELL-64 copy when it is in use
+  bool sell = false;
+  const int nbs = (n + kBlock - 1) / kBlock;
+  if (!fused && nbs <= kMaxPartials && (rc = csr_ensure_sell(A, true, &sell))) return rc;
   hipLaunchKernelGGL(k_mx_resid<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
                      A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r32, x32, A->dinv32, zw,
                      st, part);
@@ -433,10 +463,14 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
       dirspmv(0, k, g, st_);
       return;
     }
-    hipLaunchKernelGGL(k_mx_spmv, dim3(nb32), dim3(kBlock), 0, st_, nb32, A->row_blocks32,
-                       A->row_ptr, ci32, A->val32, p32, q32, st, part);
-    hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, k, nb32, x32, r32, p32,
-                       p32, q32, A->dinv32, nullptr, st, part);
+    if (sell)
+      hipLaunchKernelGGL(k_mx_spmv_sell, dim3(nbs), dim3(kBlock), 0, st_, n, nbs, A->sell_cptr,
+                         A->sell_clen, A->sell_col, A->sell_val32, p32, q32, st, part);
+    else
+      hipLaunchKernelGGL(k_mx_spmv, dim3(nb32), dim3(kBlock), 0, st_, nb32, A->row_blocks32,
+                         A->row_ptr, ci32, A->val32, p32, q32, st, part);
+    hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, k, sell ? nbs : nb32, x32,
+                       r32, p32, p32, q32, A->dinv32, nullptr, st, part);
     hipLaunchKernelGGL((k_mx_dir<PREC, 0>), dim3(g), dim3(kBlock), 0, st_, n, k, g, r32,
                        A->dinv32, p32, st, part);
   };

@@ -88,6 +88,62 @@ __device__ __forceinline__ T csr_stream_rowsum_g(int r0, int r1, const int* __re
   return s;
 }
 
+// ---------------------------------------------------------------------------
+// SELL-64: one row per lane.  A chunk of 64 consecutive rows (one wave) is
+// stored position-major -- entry j of every row of the chunk in 64
+// consecutive slots -- and padded to its longest row with the value -0.0 and
+// a column whose byte offset lies past the end of x, which the buffer load
+// returns as +0.0: the padding adds -0.0, which leaves every sum unchanged.
+// Each lane adds its row's products left to right, the reference's order.
+// ---------------------------------------------------------------------------
+constexpr int kSellChunk = 64;
+constexpr uint32_t kSellPadCol = 0x3FFFFFFEu;  // x byte offset >= 2^31 for 4- and 8-byte x
+
+using XRsrc = __amdgpu_buffer_rsrc_t;
+
+template <typename T>
+__device__ __forceinline__ XRsrc x_rsrc(const T* x, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(x), (short)0, n * (int)sizeof(T),
+                                           0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ T x_load(XRsrc r, uint32_t c) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(c * 8u), 0, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(c * 4u), 0, 0));
+}
+
+// Sum of row `row` (chunk q = row / 64) of the SELL copy, from 0, with U
+// entries' loads in flight.
+template <typename T, int U>
+__device__ __forceinline__ T sell_rowsum(int row, const int* __restrict__ cptr,
+                                         const int* __restrict__ clen,
+                                         const uint32_t* __restrict__ col,
+                                         const T* __restrict__ val, XRsrc xr) {
+  const int q = row / kSellChunk;
+  const int base = cptr[q] + (row & (kSellChunk - 1));
+  const int len = clen[q];
+  T s = T(0);
+  int j = 0;
+  for (; j + U <= len; j += U) {
+    uint32_t c[U];
+    T a[U], v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      c[u] = col[base + (j + u) * kSellChunk];
+      a[u] = val[base + (j + u) * kSellChunk];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = x_load<T>(xr, c[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += a[u] * v[u];
+  }
+  for (; j < len; ++j) s += val[base + j * kSellChunk] * x_load<T>(xr, col[base + j * kSellChunk]);
+  return s;
+}
+
 template <typename T, int TILE, bool SCALED = false>
 __device__ __forceinline__ T csr_stream_rowsum(int r0, int r1, const int* __restrict__ rp,
                                                ColIndex ci, const T* __restrict__ val,
