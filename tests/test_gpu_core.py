@@ -59,6 +59,43 @@ def test_spmv_bitexact(ctx):
     assert np.array_equal(host(y), O.spmv(rp, ci, a, x))
 
 
+@pytest.mark.parametrize("shape", [(28, 28, 28), (400, 20)])
+def test_spmv_sell_bitexact_and_value_changes(ctx, shape):
+    """y = A x from the SELL-64 copy (ragged last chunk: 21,952 and 8,000 rows
+    are not multiples of 64 x 4) bitwise the oracle's row sums, also after the
+    values change (the copy is gathered again) and with non-finite x entries
+    (padding adds -0.0, never 0 * inf); FP32 bitwise the sequential float sums."""
+    import ofeli_amd as oa
+    torch = _torch()
+    rp, ci, a = grid_system(shape, 31)
+    A = gpu_csr(ctx, rp, ci, a)
+    rng = np.random.default_rng(32)
+    x = rng.standard_normal(A.n)
+    y = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+    oa.spmv(A, dev(x), y)
+    assert np.array_equal(host(y), O.spmv(rp, ci, a, x))
+    a2 = a * (1.0 + 0.5 * np.cos(np.arange(len(a))))
+    A.set_values(a2)
+    oa.spmv(A, dev(x), y)
+    assert np.array_equal(host(y), O.spmv(rp, ci, a2, x))
+    xs = x.copy()
+    xs[[5, 77]] = [np.inf, np.nan]
+    oa.spmv(A, dev(xs), y)
+    with np.errstate(invalid="ignore"):
+        assert np.array_equal(host(y), O.spmv(rp, ci, a2, xs), equal_nan=True)
+    x32 = x.astype(np.float32)
+    y32 = torch.zeros(A.n, dtype=torch.float32, device="cuda")
+    oa.spmv_f32(A, torch.as_tensor(x32, device="cuda"), y32)
+    a32 = a2.astype(np.float32)
+    ref = np.zeros(A.n, np.float32)
+    for i in range(A.n):  # sequential float32 row sums
+        s = np.float32(0.0)
+        for k in range(rp[i], rp[i + 1]):
+            s = np.float32(s + np.float32(a32[k] * x32[ci[k] - 1]))
+        ref[i] = s
+    assert np.array_equal(y32.cpu().numpy(), ref)
+
+
 def test_spmv_column_compression_is_exact(ctx):
     """16-bit block-relative columns and 32-bit columns give the same bits."""
     import ofeli_amd as oa
