@@ -32,6 +32,28 @@ namespace ofh {
 
 constexpr double kDelta = 0.1;  // recursive-residual drop that triggers a replacement
 
+// Inner iterations to enqueue before the host looks at the state again.  The
+// device pauses by itself at the replacement point and the launches queued
+// past it return at once, but each still costs its dispatch and starves the
+// GPU while the host issues it; a look costs one round trip.  So approach the
+// pause from below: 3/4 of the predicted distance (right after a replacement
+// the previous cycle's length, otherwise the residual's rate over the last
+// chunk), at least 2.
+inline int mixed_chunk(int it, double res, int it_prev, double res_prev, double target,
+                       int remaining, int cycle_start, int last_cycle) {
+  int c;
+  if (it == cycle_start && last_cycle > 0) {
+    c = last_cycle * 3 / 4;
+  } else if (it > it_prev && res_prev > 0 && res > 0 && res < res_prev && res > target) {
+    const double rate = std::pow(res / res_prev, 1.0 / (it - it_prev));
+    c = (int)(0.75 * std::log(target / res) / std::log(rate));
+  } else {
+    c = 8;
+  }
+  c = std::max(2, std::min(c, std::max(16, it)));
+  return std::max(1, std::min(c, remaining));
+}
+
 template <int PREC>
 __device__ __forceinline__ float zf(int i, float ri, const float* __restrict__ d) {
   return PREC == OFH_DIAG_PREC ? ri * d[i] : ri;
@@ -489,12 +511,14 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   if ((rc = read_state(ctx, &h))) return rc;
   double last_true = h.res;
   int stagnant = 0, slow = 0, fallback = 0, it_prev = 0, it_repl = 0, first_cycle = 0;
+  int last_cycle = 0;
   double best_rate = 0.0;
   double res_prev = h.res;
   while (!h.done) {
     int k = h.it;
     const double target = std::max(kDelta * h.rmax, toler);
-    const int chunk = next_chunk(k, h.res, it_prev, res_prev, target, max_it - k);
+    const int chunk =
+        mixed_chunk(k, h.res, it_prev, res_prev, target, max_it - k, it_repl, last_cycle);
     it_prev = k;
     res_prev = h.res;
     // whole captured chunks, then the remainder launched directly
@@ -514,6 +538,7 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     if (!h.pause && !forced) continue;
     if (h.pause && first_cycle == 0) first_cycle = std::max(1, cyc_now);
     const int kr = forced ? h.it + 1 : h.it;
+    last_cycle = kr - it_repl;
     // residual replacement in FP64 after kr inner iterations
     hipLaunchKernelGGL(k_mx_accum, dim3(ge), dim3(256), 0, s, n, x, x32, st);
     hipLaunchKernelGGL(k_mx_resid<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
