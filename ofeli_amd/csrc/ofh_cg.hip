@@ -285,6 +285,92 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(int nblocks, const int* __re
   write_partials<1>(v, slot, partials, blockIdx.x);
 }
 
+// The new direction at a gather of the SELL-64 copy (buffer loads: the
+// padding column reads z = p = +0.0, so it contributes -0.0 * +0.0).
+struct SellGatherDir {
+  XRsrc z, p;
+  double beta;
+  int first;
+  __device__ __forceinline__ double operator()(uint32_t c) const {
+    const double zc = x_load<double>(z, c);
+    return first ? zc : zc + beta * x_load<double>(p, c);
+  }
+};
+
+// k_cg_dirspmv with the product from the SELL-64 copy (one row per lane,
+// 256 rows per workgroup): the same stop test, beta, p_new = z + beta p_old
+// at the gathers, p_new of the block's rows stored, partial (p,q) -> slot 3.
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_cg_dirspmv_sell(
+    int n, int nblocks, const int* __restrict__ cptr, const int* __restrict__ clen,
+    const uint32_t* __restrict__ col, const double* __restrict__ val, int it_arg, int nparts0,
+    int nparts, const double* __restrict__ z, double* __restrict__ p0, double* __restrict__ p1,
+    double* __restrict__ q, KrylovState* st, double* __restrict__ partials) {
+  if (st->done) return;
+  const int it = it_arg >= 0 ? it_arg : st->kcur;
+  const double* __restrict__ p_old = (it & 1) ? p1 : p0;
+  double* __restrict__ p_new = (it & 1) ? p0 : p1;
+  __shared__ double red[3 * kBlock / kWave];
+  double t[3];
+  double nrm;
+  if (it == 0) {
+    const int slot[3] = {0, 1, 2};
+    fold_partials<kBlock, 3>(t, slot, partials, nparts0, red);
+    nrm = sqrt(t[0]);
+    if (nrm == 0) nrm = 1;  // CG.h:113-114
+  } else {
+    const int slot[2] = {0, 1};
+    double u[2];
+    fold_partials<kBlock, 2>(u, slot, partials, nparts, red);
+    t[1] = u[0];
+    t[2] = u[1];
+    nrm = st->nrm;
+  }
+  bool stop = false;
+  int status = 0;
+  const double res = sqrt(t[1]) / nrm;
+  if (res <= st->tol) {
+    stop = true;
+    status = it;
+  } else if (it >= st->max_it || !(res == res)) {
+    stop = true;
+    status = -(st->max_it + 1);
+  }
+  const double rho = t[2];
+  const double beta = it > 0 ? rho / st->rho_buf[(it - 1) & 1] : 0.0;  // CG.h:131
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->it = it;
+    st->res = res;
+    if (it == 0) {
+      st->nrm = nrm;
+      st->res0 = res;
+    }
+    if (stop) {
+      st->done = 1;
+      st->status = status;
+    } else {
+      st->rho_buf[it & 1] = rho;
+      st->beta = beta;
+    }
+  }
+  if (stop) return;
+  const SellGatherDir g{x_rsrc(z, n), x_rsrc(p_old, n), beta, it == 0};
+  const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
+  double v[1] = {0.0};
+  if (row / kSellChunk * kSellChunk < n) {  // whole waves past the end only reduce
+    const double sq = sell_rowsum_g<double, 2>(row, cptr, clen, col, val, g);
+    if (row < n) {
+      const double pi = it == 0 ? z[row] : z[row] + beta * p_old[row];
+      p_new[row] = pi;
+      q[row] = sq;
+      v[0] = pi * sq;
+    }
+  }
+  block_sum_k<kBlock, 1>(v, red);
+  const int slot[1] = {3};
+  write_partials<1>(v, slot, partials, blockIdx.x);
+}
+
 // q = A p from the SELL-64 copy, partials of (p,q) -> slot 3
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_sell(int n, int nblocks,
                                                         const int* __restrict__ cptr,
@@ -459,16 +545,28 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   if (!sweep_prec(PREC) && n <= kFuseRows) {
     const ColIndex ci{A->col, A->col16};
     double* zw = PREC == OFH_DIAG_PREC ? z : nullptr;
+    // the product from the SELL-64 copy when it is in use (one partial per 256 rows)
+    bool sell = false;
+    const int nbs = (n + kBlock - 1) / kBlock;
+    if (nbs <= kMaxPartials && (rc = csr_ensure_sell(A, false, &sell))) return rc;
+    const int npq = sell ? nbs : nb;
+    auto dirspmv = [&](int it, hipStream_t st_) {
+      if (sell)
+        hipLaunchKernelGGL(k_cg_dirspmv_sell<PREC>, dim3(nbs), dim3(kBlock), 0, st_, n, nbs,
+                           A->sell_cptr, A->sell_clen, A->sell_col, A->sell_val, it, nb, g, z, p,
+                           p1, q, st, part);
+      else
+        hipLaunchKernelGGL(k_cg_dirspmv<PREC>, dim3(nb), dim3(kBlock), 0, st_, nb, A->row_blocks,
+                           A->row_ptr, ci, A->val, it, nb, g, z, p, p1, q, st, part);
+    };
     // one iteration: k_cg_upd(it), k_cg_dirspmv(it); it = -1 takes the
     // index from the state (captured chunks)
     auto iteration = [&](int it, hipStream_t st_) {
-      hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, it, nb, x, r, p, p1, q,
+      hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, it, npq, x, r, p, p1, q,
                          A->dinv, zw, st, part);
-      hipLaunchKernelGGL(k_cg_dirspmv<PREC>, dim3(nb), dim3(kBlock), 0, st_, nb, A->row_blocks,
-                         A->row_ptr, ci, A->val, it, nb, g, z, p, p1, q, st, part);
+      dirspmv(it, st_);
     };
-    hipLaunchKernelGGL(k_cg_dirspmv<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                       A->row_ptr, ci, A->val, 0, nb, g, z, p, p1, q, st, part);
+    dirspmv(0, s);
     OFH_HIP(hipGetLastError());
     hipGraphExec_t chunk_graph = nullptr;
     if (ctx->use_graphs) {

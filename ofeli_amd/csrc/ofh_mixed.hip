@@ -251,12 +251,35 @@ struct GatherDirF {
   }
 };
 
+// The same at a gather of the SELL-64 copy (buffer loads: the padding column
+// reads z = p = +0.0, so it contributes -0.0 * +0.0).
+struct SellGatherDirF {
+  XRsrc z, p;
+  float beta;
+  int first;
+  __device__ __forceinline__ float operator()(uint32_t c) const {
+    const float zc = x_load<float>(z, c);
+    return first ? zc : zc + beta * x_load<float>(p, c);
+  }
+};
+
+// The SELL-64 copy's FP32 arrays (SELL kernels only).
+struct SellF32 {
+  int n;
+  const int* __restrict__ cptr;
+  const int* __restrict__ clen;
+  const uint32_t* __restrict__ col;
+  const float* __restrict__ val;
+};
+
 // k_mx_dir<PREC, PHASE> fused with the next SpMV, for L2-resident systems:
 // the same decision (and, on pause or stop, the same early return), then
 // q32 = A32 p_new with p_new = z + beta p_old formed at the gather; p_new of
 // the block's rows is stored; (p,q) -> slot 6.  z32 holds D^-1 r (DIAG) or
 // r (IDENT); p_old and p_new are ping-pong buffers.
-template <int PREC, int PHASE>
+// SELL: the product from the SELL-64 copy (256 rows per workgroup, one per
+// lane), else from the CSR-stream partition.
+template <int PREC, int PHASE, bool SELL>
 __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* __restrict__ rb,
                                                       const int* __restrict__ rp, ColIndex ci,
                                                       const float* __restrict__ val32,
@@ -265,15 +288,12 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* _
                                                       float* __restrict__ p0,
                                                       float* __restrict__ p1,
                                                       float* __restrict__ q32, KrylovState* st,
-                                                      double* __restrict__ partials) {
+                                                      double* __restrict__ partials, SellF32 sa) {
   if (PHASE == 0 ? st->halt : st->done) return;
   const int k = k_arg >= 0 ? k_arg : st->kcur;  // k_arg < 0: index from the state
   const float* __restrict__ p_old = (k & 1) ? p1 : p0;
   float* __restrict__ p_new = (k & 1) ? p0 : p1;
-  __shared__ float lds[kTile32];
   __shared__ double red[4 * kBlock / kWave];
-  const int blk = xcd_block(blockIdx.x, nblocks);
-  const int r0 = rb[blk], r1 = rb[blk + 1];
   double nrm = st->nrm, res, rz;
   bool stop = false, pause = false;
   int status = 0;
@@ -334,15 +354,34 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* _
     }
   }
   if (stop || pause) return;
-  const GatherDirF g{z32, p_old, (float)beta, k == 0};
-  const float sq = csr_stream_rowsum_g<float, kTile32>(r0, r1, rp, ci, val32, g, lds, 0.0f);
-  const int i = r0 + threadIdx.x;
   double v[1] = {0.0};
-  if (i < r1) {
-    const float pi = g(i);
-    p_new[i] = pi;
-    q32[i] = sq;
-    v[0] = (double)pi * (double)sq;
+  if constexpr (SELL) {
+    const int n = sa.n;
+    const float bf = (float)beta;
+    const SellGatherDirF g{x_rsrc(z32, n), x_rsrc(p_old, n), bf, k == 0};
+    const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
+    if (row / kSellChunk * kSellChunk < n) {  // whole waves past the end only reduce
+      const float sq = sell_rowsum_g<float, 4>(row, sa.cptr, sa.clen, sa.col, sa.val, g);
+      if (row < n) {
+        const float pi = k == 0 ? z32[row] : z32[row] + bf * p_old[row];
+        p_new[row] = pi;
+        q32[row] = sq;
+        v[0] = (double)pi * (double)sq;
+      }
+    }
+  } else {
+    __shared__ float lds[kTile32];
+    const int blk = xcd_block(blockIdx.x, nblocks);
+    const int r0 = rb[blk], r1 = rb[blk + 1];
+    const GatherDirF g{z32, p_old, (float)beta, k == 0};
+    const float sq = csr_stream_rowsum_g<float, kTile32>(r0, r1, rp, ci, val32, g, lds, 0.0f);
+    const int i = r0 + threadIdx.x;
+    if (i < r1) {
+      const float pi = g(i);
+      p_new[i] = pi;
+      q32[i] = sq;
+      v[0] = (double)pi * (double)sq;
+    }
   }
   block_sum_k<kBlock, 1>(v, red);
   const int slot[1] = {6};
@@ -438,20 +477,28 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   float* pb[2] = {p32, p32b};
   float* zw = fused && PREC == OFH_DIAG_PREC ? z32 : nullptr;  // z written by resid/upd
   const ColIndex ci{A->col, nullptr};  // FP32 partition: 32-bit columns
+  // the fused product from the SELL-64 copy when it is in use
+  bool fsell = false;
+  const int nbsf = (n + kBlock - 1) / kBlock;
+  if (fused && nbsf <= kMaxPartials && (rc = csr_ensure_sell(A, true, &fsell))) return rc;
+  const SellF32 sa{n, A->sell_cptr, A->sell_clen, A->sell_col, A->sell_val32};
+  const int npq_fused = fsell ? nbsf : A->nblocks32;  // partials of the fused product
   auto dirspmv = [&](int phase, int k, int nparts, hipStream_t st_) {
-    const int nbf = A->nblocks32;
-    if (phase == 0)
-      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 0>), dim3(nbf), dim3(kBlock), 0, st_, nbf,
-                         A->row_blocks32, A->row_ptr, ci, A->val32, k, nparts, z32, pb[0], pb[1],
-                         q32, state_ptr(ctx), ctx->ws.partials);
-    else if (phase == 1)
-      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 1>), dim3(nbf), dim3(kBlock), 0, st_, nbf,
-                         A->row_blocks32, A->row_ptr, ci, A->val32, k, nparts, z32, pb[0], pb[1],
-                         q32, state_ptr(ctx), ctx->ws.partials);
-    else
-      hipLaunchKernelGGL((k_mx_dirspmv<PREC, 2>), dim3(nbf), dim3(kBlock), 0, st_, nbf,
-                         A->row_blocks32, A->row_ptr, ci, A->val32, k, nparts, z32, pb[0], pb[1],
-                         q32, state_ptr(ctx), ctx->ws.partials);
+    const int nbf = fsell ? nbsf : A->nblocks32;
+#define OFH_MX_DIRSPMV(PH, SL)                                                                \
+  hipLaunchKernelGGL((k_mx_dirspmv<PREC, PH, SL>), dim3(nbf), dim3(kBlock), 0, st_, nbf,       \
+                     A->row_blocks32, A->row_ptr, ci, A->val32, k, nparts, z32, pb[0], pb[1], \
+                     q32, state_ptr(ctx), ctx->ws.partials, sa)
+    if (fsell) {
+      if (phase == 0) OFH_MX_DIRSPMV(0, true);
+      else if (phase == 1) OFH_MX_DIRSPMV(1, true);
+      else OFH_MX_DIRSPMV(2, true);
+    } else {
+      if (phase == 0) OFH_MX_DIRSPMV(0, false);
+      else if (phase == 1) OFH_MX_DIRSPMV(1, false);
+      else OFH_MX_DIRSPMV(2, false);
+    }
+#undef OFH_MX_DIRSPMV
   };
   KrylovState* st = state_ptr(ctx);
   KrylovState h0{};
@@ -480,7 +527,7 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   // one inner iteration; k = -1 takes the index from the state (captured chunks)
   auto iteration = [&](int k, hipStream_t st_) {
     if (fused) {
-      hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, k, nb32, x32, r32,
+      hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, k, npq_fused, x32, r32,
                          pb[0], pb[1], q32, A->dinv32, zw, st, part);
       dirspmv(0, k, g, st_);
       return;

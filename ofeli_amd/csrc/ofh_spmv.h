@@ -115,13 +115,21 @@ __device__ __forceinline__ T x_load(XRsrc r, uint32_t c) {
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(c * 4u), 0, 0));
 }
 
+// Gather of x[c] through a buffer descriptor (padding columns read +0.0).
+template <typename T>
+struct SellGather {
+  XRsrc x;
+  __device__ __forceinline__ T operator()(uint32_t c) const { return x_load<T>(x, c); }
+};
+
 // Sum of row `row` (chunk q = row / 64) of the SELL copy, from 0, with U
-// entries' loads in flight.
-template <typename T, int U>
-__device__ __forceinline__ T sell_rowsum(int row, const int* __restrict__ cptr,
-                                         const int* __restrict__ clen,
-                                         const uint32_t* __restrict__ col,
-                                         const T* __restrict__ val, XRsrc xr) {
+// entries' loads in flight; G gathers the multiplied vector and must return
+// +0.0 for the padding column.
+template <typename T, int U, class G>
+__device__ __forceinline__ T sell_rowsum_g(int row, const int* __restrict__ cptr,
+                                           const int* __restrict__ clen,
+                                           const uint32_t* __restrict__ col,
+                                           const T* __restrict__ val, const G& g) {
   const int q = row / kSellChunk;
   const int base = cptr[q] + (row & (kSellChunk - 1));
   const int len = clen[q];
@@ -136,12 +144,20 @@ __device__ __forceinline__ T sell_rowsum(int row, const int* __restrict__ cptr,
       a[u] = val[base + (j + u) * kSellChunk];
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) v[u] = x_load<T>(xr, c[u]);
+    for (int u = 0; u < U; ++u) v[u] = g(c[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) s += a[u] * v[u];
   }
-  for (; j < len; ++j) s += val[base + j * kSellChunk] * x_load<T>(xr, col[base + j * kSellChunk]);
+  for (; j < len; ++j) s += val[base + j * kSellChunk] * g(col[base + j * kSellChunk]);
   return s;
+}
+
+template <typename T, int U>
+__device__ __forceinline__ T sell_rowsum(int row, const int* __restrict__ cptr,
+                                         const int* __restrict__ clen,
+                                         const uint32_t* __restrict__ col,
+                                         const T* __restrict__ val, XRsrc xr) {
+  return sell_rowsum_g<T, U>(row, cptr, clen, col, val, SellGather<T>{xr});
 }
 
 template <typename T, int TILE, bool SCALED = false>
