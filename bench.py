@@ -278,16 +278,19 @@ def mesh_io_leg(args, oa, ctx, n=1024):
 
 
 def other_krylov(args, oa, A, b, cg_ms, cg_it, vals):
-    """BiCG (with its A^T product), BiCGStab and CGS on the same system with
-    DIAG_PREC.  The reference's CGS does not converge here (its q stays 0),
-    so it runs a fixed 200 iterations and is reported per iteration."""
+    """BiCG (with its A^T product), BiCGStab, CGS and GMRes(50) on the same
+    system with DIAG_PREC.  The reference's CGS does not converge here (its q
+    stays 0), so it runs a fixed 200 iterations and is reported per iteration."""
     import torch
     n = A.n
     res = {"cg": {"solve_ms": round(cg_ms, 2), "iterations": cg_it,
                   "us_per_iteration": round(cg_ms * 1e3 / cg_it, 1)}}
     for name, fn, max_it, toler in (("bicg", oa.bicg, 20000, args.toler),
                                     ("bicgstab", oa.bicgstab, 20000, args.toler),
-                                    ("cgs_200_iterations", oa.cgs, 200, 1e-300)):
+                                    ("cgs_200_iterations", oa.cgs, 200, 1e-300),
+                                    ("gmres_50", lambda A_, b_, x_, prec, max_it, toler:
+                                     oa.gmres(A_, b_, x_, 50, prec=prec, max_it=max_it,
+                                              toler=toler), 2000, args.toler)):
         ts = []
         for rep in range(3):
             x = torch.zeros(n, dtype=torch.float64, device="cuda")

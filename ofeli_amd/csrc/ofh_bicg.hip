@@ -87,6 +87,24 @@ int csr_ensure_transpose(ofh_csr A) {
 // ---------------------------------------------------------------------------
 // shared kernels
 // ---------------------------------------------------------------------------
+template <int SLOT, bool SELL>
+__global__ __launch_bounds__(kBlock) void k_kr_spmv(MatRows64 m, const double* __restrict__ x,
+                                                   double* __restrict__ y,
+                                                   const double* __restrict__ u,
+                                                   const KrylovState* __restrict__ st,
+                                                   double* __restrict__ partials);
+
+// y = A x with SLOT's partials, from the matrix's SELL-64 copy when it has one
+template <int SLOT>
+static void kr_spmv(const MatRows64& m, bool sell, const double* x, double* y, const double* u,
+                    const KrylovState* st, double* partials, hipStream_t s) {
+  if (sell)
+    hipLaunchKernelGGL((k_kr_spmv<SLOT, true>), dim3(m.nblocks), dim3(kBlock), 0, s, m, x, y, u,
+                       st, partials);
+  else
+    hipLaunchKernelGGL((k_kr_spmv<SLOT, false>), dim3(m.nblocks), dim3(kBlock), 0, s, m, x, y,
+                       u, st, partials);
+}
 // r = b - A x; r2 = r; partials (b,b) -> 0, (r,r) -> 1
 __global__ __launch_bounds__(kBlock) void k_kr_init(int nblocks, const int* __restrict__ rb,
                                                    const int* __restrict__ rp, ColIndex ci,
@@ -116,24 +134,19 @@ __global__ __launch_bounds__(kBlock) void k_kr_init(int nblocks, const int* __re
 
 // y = A x (skipped once the solver stopped); with SLOT >= 0 also the partial
 // sums of (u, y) into that slot.
-template <int SLOT>
-__global__ __launch_bounds__(kBlock) void k_kr_spmv(int nblocks, const int* __restrict__ rb,
-                                                   const int* __restrict__ rp, ColIndex ci,
-                                                   const double* __restrict__ val,
-                                                   const double* __restrict__ x,
+template <int SLOT, bool SELL>
+__global__ __launch_bounds__(kBlock) void k_kr_spmv(MatRows64 m, const double* __restrict__ x,
                                                    double* __restrict__ y,
                                                    const double* __restrict__ u,
                                                    const KrylovState* __restrict__ st,
                                                    double* __restrict__ partials) {
   if (st->done) return;
-  __shared__ double lds[kTile64];
   __shared__ double red[kBlock / kWave];
-  const int blk = xcd_block(blockIdx.x, nblocks);
-  const int r0 = rb[blk], r1 = rb[blk + 1];
-  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, ci, val, x, lds, 0.0);
-  const int i = r0 + threadIdx.x;
+  int i;
+  bool valid;
+  const double s = mat_rowsum64<SELL>(m, x, i, valid);
   double v[1] = {0.0};
-  if (i < r1) {
+  if (valid) {
     y[i] = s;
     if (SLOT >= 0) v[0] = u[i] * s;
   }
@@ -309,6 +322,9 @@ static int cgs_impl(ofh_csr A, const double* b, double* x, int max_it, double to
   const int g = reduce_grid(n);
   const int nb = A->nblocks;
   const ColIndex ci{A->col, A->col16};
+  MatRows64 mr;
+  bool sell = false;
+  if ((rc = mat_rows64(A, &mr, &sell))) return rc;
   hipLaunchKernelGGL(k_kr_init, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr, ci,
                      A->val, b, x, r, sv, part);
   KrylovState h{};
@@ -326,13 +342,11 @@ static int cgs_impl(ofh_csr A, const double* b, double* x, int max_it, double to
       hipLaunchKernelGGL(k_cgs_dir<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, it == 1 ? nb : g, r,
                          u, p, y, A->dinv, st, part);
       if (sweep_prec(PREC) && (rc = launch_prec(A, PREC, p, y, &st->done, s))) return rc;
-      hipLaunchKernelGGL(k_kr_spmv<2>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                         A->row_ptr, ci, A->val, y, v, sv, st, part);
-      hipLaunchKernelGGL(k_cgs_w<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, nb, u, v, w, A->dinv,
-                         st, part);
+      kr_spmv<2>(mr, sell, y, v, sv, st, part, s);
+      hipLaunchKernelGGL(k_cgs_w<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, mr.nblocks, u, v, w,
+                         A->dinv, st, part);
       if (sweep_prec(PREC) && (rc = launch_prec(A, PREC, w, wp, &st->done, s))) return rc;
-      hipLaunchKernelGGL(k_kr_spmv<-1>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                         A->row_ptr, ci, A->val, wp, z, (const double*)nullptr, st, part);
+      kr_spmv<-1>(mr, sell, wp, z, nullptr, st, part, s);
       hipLaunchKernelGGL(k_cgs_upd, dim3(g), dim3(kBlock), 0, s, n, x, r, sv, wp, z, st, part);
     }
     OFH_HIP(hipGetLastError());
@@ -499,8 +513,11 @@ static int bicg_impl(ofh_csr A, const double* b, double* x, int max_it, double t
   OFH_HIP(hipMemcpyAsync(st, &h0, sizeof(h0), hipMemcpyHostToDevice, s));
   double* part = ctx->ws.partials;
   const int g = reduce_grid(n);
-  const int nb = A->nblocks, nbt = T->nblocks;
-  const ColIndex ci{A->col, A->col16}, cit{T->col, T->col16};
+  const int nb = A->nblocks;
+  const ColIndex ci{A->col, A->col16};
+  MatRows64 mr, mrt;  // A and its transposed copy
+  bool sell = false, sellt = false;
+  if ((rc = mat_rows64(A, &mr, &sell)) || (rc = mat_rows64(T, &mrt, &sellt))) return rc;
   hipLaunchKernelGGL(k_kr_init, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr, ci,
                      A->val, b, x, r, rr, part);
   KrylovState h{};
@@ -524,12 +541,10 @@ static int bicg_impl(ofh_csr A, const double* b, double* x, int max_it, double t
       }
       hipLaunchKernelGGL(k_bcg_dir<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, g, z, zz, p, pp, st,
                          part);
-      hipLaunchKernelGGL(k_kr_spmv<3>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                         A->row_ptr, ci, A->val, p, q, pp, st, part);
-      hipLaunchKernelGGL(k_kr_spmv<-1>, dim3(nbt), dim3(kBlock), 0, s, nbt, T->row_blocks,
-                         T->row_ptr, cit, T->val, pp, qq, (const double*)nullptr, st, part);
-      hipLaunchKernelGGL(k_bcg_upd, dim3(g), dim3(kBlock), 0, s, n, it, nb, x, r, rr, p, q, qq, st,
-                         part);
+      kr_spmv<3>(mr, sell, p, q, pp, st, part, s);
+      kr_spmv<-1>(mrt, sellt, pp, qq, nullptr, st, part, s);
+      hipLaunchKernelGGL(k_bcg_upd, dim3(g), dim3(kBlock), 0, s, n, it, mr.nblocks, x, r, rr, p,
+                         q, qq, st, part);
     }
     OFH_HIP(hipGetLastError());
     if ((rc = read_state(ctx, &h))) return rc;

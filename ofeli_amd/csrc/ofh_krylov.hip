@@ -193,25 +193,19 @@ __global__ __launch_bounds__(kBlock) void k_bi_dir(int n, int it, int nparts,
 // y = A x with fused partial dots:
 // MODE 0: v = A pp, (rr,v) -> slot 2                 (BiCGStab.h:130-131)
 // MODE 1: t = A ss, (t,s) -> slot 4, (t,t) -> slot 5  (BiCGStab.h:140-141)
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_bi_spmv(int nblocks, const int* __restrict__ rb,
-                                                   const int* __restrict__ rp,
-                                                   ColIndex ci,
-                                                   const double* __restrict__ val,
-                                                   const double* __restrict__ x,
+template <int MODE, bool SELL>
+__global__ __launch_bounds__(kBlock) void k_bi_spmv(MatRows64 m, const double* __restrict__ x,
                                                    double* __restrict__ y,
                                                    const double* __restrict__ u,
                                                    const KrylovState* __restrict__ st,
                                                    double* __restrict__ partials) {
   if (MODE == 0 ? st->done : st->halt) return;
-  __shared__ double lds[kTile64];
   __shared__ double red[2 * kBlock / kWave];
-  const int blk = xcd_block(blockIdx.x, nblocks);
-  const int r0 = rb[blk], r1 = rb[blk + 1];
-  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, ci, val, x, lds, 0.0);
-  const int i = r0 + threadIdx.x;
+  int i;
+  bool valid;
+  const double s = mat_rowsum64<SELL>(m, x, i, valid);
   double v[2] = {0.0, 0.0};
-  if (i < r1) {
+  if (valid) {
     y[i] = s;
     if (MODE == 0) {
       v[0] = u[i] * s;
@@ -334,6 +328,23 @@ static int bicgstab_impl(ofh_csr A, const double* b, double* x, int max_it, doub
   double* part = ctx->ws.partials;
   const int g = reduce_grid(n);
   const int nb = A->nblocks;
+  MatRows64 mr;
+  bool sell = false;
+  if ((rc = mat_rows64(A, &mr, &sell))) return rc;
+  const int nbp = mr.nblocks;  // workgroups (= partials) of the iteration's products
+  auto spmv = [&](int mode, const double* xv, double* yv, const double* uv) {
+#define OFH_BI_SPMV(MD, SL)                                                                   \
+  hipLaunchKernelGGL((k_bi_spmv<MD, SL>), dim3(nbp), dim3(kBlock), 0, s, mr, xv, yv, uv, st, \
+                     part)
+    if (sell) {
+      if (mode == 0) OFH_BI_SPMV(0, true);
+      else OFH_BI_SPMV(1, true);
+    } else {
+      if (mode == 0) OFH_BI_SPMV(0, false);
+      else OFH_BI_SPMV(1, false);
+    }
+#undef OFH_BI_SPMV
+  };
   hipLaunchKernelGGL(k_bi_init, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr,
                      ColIndex{A->col, A->col16}, A->val, b, x, r, rr, part);
   auto dir = [&](int it) -> int {
@@ -354,15 +365,13 @@ static int bicgstab_impl(ofh_csr A, const double* b, double* x, int max_it, doub
     res_prev = h.res;
     for (int c = 0; c < chunk; ++c) {
       const int it = ++enq;
-      hipLaunchKernelGGL(k_bi_spmv<0>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                         A->row_ptr, ColIndex{A->col, A->col16}, A->val, pp, v, rr, st, part);
-      hipLaunchKernelGGL(k_bi_s<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, nb, r, v, sv, ss,
+      spmv(0, pp, v, rr);
+      hipLaunchKernelGGL(k_bi_s<PREC>, dim3(g), dim3(kBlock), 0, s, n, it, nbp, r, v, sv, ss,
                          A->dinv, st, part);
       hipLaunchKernelGGL(k_bi_scheck, dim3(1), dim3(kBlock), 0, s, g, st, part);
       if (sweep_prec(PREC) && (rc = launch_prec(A, PREC, sv, ss, &st->halt, s))) return rc;
-      hipLaunchKernelGGL(k_bi_spmv<1>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                         A->row_ptr, ColIndex{A->col, A->col16}, A->val, ss, t, sv, st, part);
-      hipLaunchKernelGGL(k_bi_upd, dim3(g), dim3(kBlock), 0, s, n, it, nb, x, r, rr, pp, sv, ss,
+      spmv(1, ss, t, sv);
+      hipLaunchKernelGGL(k_bi_upd, dim3(g), dim3(kBlock), 0, s, n, it, nbp, x, r, rr, pp, sv, ss,
                          t, st, part);
       if ((rc = dir(it + 1))) return rc;
     }
@@ -490,22 +499,16 @@ __global__ void k_gm_v0(int n, int m, const double* __restrict__ r, double* __re
 }
 
 // w = P (A v_i)   (GMRes.h:191-192)
-template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_gm_av(int nblocks, const int* __restrict__ rb,
-                                                 const int* __restrict__ rp,
-                                                 ColIndex ci,
-                                                 const double* __restrict__ val,
-                                                 const double* __restrict__ vi,
+template <int PREC, bool SELL>
+__global__ __launch_bounds__(kBlock) void k_gm_av(MatRows64 m, const double* __restrict__ vi,
                                                  double* __restrict__ w,
                                                  const double* __restrict__ dinv,
                                                  const KrylovState* __restrict__ st) {
   if (st->halt) return;
-  __shared__ double lds[kTile64];
-  const int blk = xcd_block(blockIdx.x, nblocks);
-  const int r0 = rb[blk], r1 = rb[blk + 1];
-  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, ci, val, vi, lds, 0.0);
-  const int i = r0 + threadIdx.x;
-  if (i < r1) w[i] = PREC == OFH_DIAG_PREC ? s * dinv[i] : s;
+  int i;
+  bool valid;
+  const double s = mat_rowsum64<SELL>(m, vi, i, valid);
+  if (valid) w[i] = PREC == OFH_DIAG_PREC ? s * dinv[i] : s;
 }
 
 // Modified Gram-Schmidt step k of column i (GMRes.h:193-197):
@@ -672,6 +675,23 @@ static int gmres_impl(ofh_csr A, const double* b, double* x, int m, int max_it, 
   const int g = reduce_grid(n);
   const int nb = A->nblocks;
   const int ge = (n + 255) / 256;
+  MatRows64 mr;
+  bool sell = false;
+  if ((rc = mat_rows64(A, &mr, &sell))) return rc;
+  // w = P (A v_i) for the plain and diagonal preconditioners, A v_i otherwise
+  auto av = [&](const double* vi, double* wv, bool diag) {
+#define OFH_GM_AV(P, SL)                                                                   \
+  hipLaunchKernelGGL((k_gm_av<P, SL>), dim3(mr.nblocks), dim3(kBlock), 0, s, mr, vi, wv, \
+                     A->dinv, st)
+    if (sell) {
+      if (diag) OFH_GM_AV(OFH_DIAG_PREC, true);
+      else OFH_GM_AV(OFH_IDENT_PREC, true);
+    } else {
+      if (diag) OFH_GM_AV(OFH_DIAG_PREC, false);
+      else OFH_GM_AV(OFH_IDENT_PREC, false);
+    }
+#undef OFH_GM_AV
+  };
   auto residual = [&](bool init) -> int {
     if (sweep_prec(PREC)) {
       hipLaunchKernelGGL(k_resid_plain, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
@@ -697,12 +717,10 @@ static int gmres_impl(ofh_csr A, const double* b, double* x, int m, int max_it, 
     for (int i = 0; i < m && it <= max_it; i++, it++) {
       double* vi = V + (size_t)i * n;
       if (sweep_prec(PREC)) {
-        hipLaunchKernelGGL(k_gm_av<OFH_IDENT_PREC>, dim3(nb), dim3(kBlock), 0, s, nb,
-                           A->row_blocks, A->row_ptr, ColIndex{A->col, A->col16}, A->val, vi, tmp, A->dinv, st);
+        av(vi, tmp, false);
         if ((rc = launch_prec(A, PREC, tmp, w, &st->halt, s))) return rc;
       } else {
-        hipLaunchKernelGGL(k_gm_av<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                           A->row_ptr, ColIndex{A->col, A->col16}, A->val, vi, w, A->dinv, st);
+        av(vi, w, PREC == OFH_DIAG_PREC);
       }
       for (int k = 0; k <= i + 1; ++k) {
         const double* vprev = k > 0 ? V + (size_t)(k - 1) * n : nullptr;

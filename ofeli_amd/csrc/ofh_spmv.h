@@ -169,4 +169,61 @@ __device__ __forceinline__ T csr_stream_rowsum(int r0, int r1, const int* __rest
                                               scale);
 }
 
+// The FP64 matrix as a product kernel reads it: the CSR-stream partition
+// (nblocks row blocks) and, when sell.cptr is set, its SELL-64 copy (256 rows
+// per workgroup, nblocks = ceil(n / 256)).
+struct Sell64 {
+  int n;
+  const int* __restrict__ cptr;
+  const int* __restrict__ clen;
+  const uint32_t* __restrict__ col;
+  const double* __restrict__ val;
+};
+struct MatRows64 {
+  int nblocks;
+  const int* __restrict__ rb;
+  const int* __restrict__ rp;
+  ColIndex ci;
+  const double* __restrict__ val;
+  Sell64 sell;
+};
+
+// (A x)_i for this lane's row i of the workgroup; `valid` is false for lanes
+// past the last row.  Every lane of the workgroup must call it (the CSR-stream
+// form synchronizes the workgroup).  Both forms give the CSR row sums bitwise.
+template <bool SELL>
+__device__ __forceinline__ double mat_rowsum64(const MatRows64& m, const double* __restrict__ x,
+                                               int& i, bool& valid) {
+  if constexpr (SELL) {
+    i = xcd_block(blockIdx.x, m.nblocks) * kBlock + threadIdx.x;
+    valid = i < m.sell.n;
+    if (i / kSellChunk * kSellChunk >= m.sell.n) return 0.0;  // a whole wave past the end
+    return sell_rowsum<double, 2>(i, m.sell.cptr, m.sell.clen, m.sell.col, m.sell.val,
+                                  x_rsrc(x, m.sell.n));
+  } else {
+    __shared__ double lds[kTile64];
+    const int blk = xcd_block(blockIdx.x, m.nblocks);
+    const int r0 = m.rb[blk], r1 = m.rb[blk + 1];
+    const double s = csr_stream_rowsum<double, kTile64>(r0, r1, m.rp, m.ci, m.val, x, lds, 0.0);
+    i = r0 + threadIdx.x;
+    valid = i < r1;
+    return s;
+  }
+}
+
+// The FP64 matrix for mat_rowsum64<*sell>: the SELL-64 copy when it is worth
+// using and one partial per workgroup fits the reduction slots.
+inline int mat_rows64(ofh_csr A, MatRows64* m, bool* sell) {
+  *sell = false;
+  const int nbs = (A->n + kBlock - 1) / kBlock;
+  if (nbs <= kMaxPartials) {
+    const int rc = csr_ensure_sell(A, false, sell);
+    if (rc) return rc;
+  }
+  *m = MatRows64{*sell ? nbs : A->nblocks, A->row_blocks, A->row_ptr,
+                 ColIndex{A->col, A->col16}, A->val,
+                 Sell64{A->n, A->sell_cptr, A->sell_clen, A->sell_col, A->sell_val}};
+  return OFH_OK;
+}
+
 }  // namespace ofh
