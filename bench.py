@@ -115,9 +115,18 @@ def main():
     ev1.record()
     torch.cuda.synchronize()
     spmv_ms = ev0.elapsed_time(ev1) / args.spmv_reps
-    b_spmv64 = 12 * nnz + 4 * (n + 1) + 16 * n
+    # bytes the kernel must move: its matrix copy once (SELL-64: values and
+    # 16- or 32-bit columns per stored entry, two ints per 64-row chunk), x
+    # once and y; the CSR form with 32-bit columns is reported beside it
+    sell = A.sell_info()
+    b_csr64 = 12 * nnz + 4 * (n + 1) + 16 * n
+    if sell["in_use"]:
+        b_spmv64 = (8 + sell["col_bytes"]) * sell["stored"] + 8 * ((n + 63) // 64) + 16 * n
+    else:
+        b_spmv64 = b_csr64
     spmv_gbs = b_spmv64 / (spmv_ms * 1e-3) / 1e9
-    log("[bench] spmv64 %.4f ms  %.0f GB/s (SELL-64)" % (spmv_ms, spmv_gbs))
+    log("[bench] spmv64 %.4f ms  %.0f GB/s (SELL-64, %d-byte columns; CSR bytes / time: %.0f GB/s)"
+        % (spmv_ms, spmv_gbs, sell["col_bytes"], b_csr64 / (spmv_ms * 1e-3) / 1e9))
     A.compress_columns(True)
     ev0.record()
     for _ in range(args.spmv_reps):
@@ -147,7 +156,8 @@ def main():
     ev1.record()
     torch.cuda.synchronize()
     spmv32_ms = ev0.elapsed_time(ev1) / args.spmv_reps
-    b_spmv32 = 8 * nnz + 4 * (n + 1) + 8 * n
+    b_spmv32 = ((4 + sell["col_bytes"]) * sell["stored"] + 8 * ((n + 63) // 64) + 8 * n
+                if sell["in_use"] else 8 * nnz + 4 * (n + 1) + 8 * n)
     log("[bench] spmv32 %.4f ms  %.0f GB/s" % (spmv32_ms, b_spmv32 / (spmv32_ms * 1e-3) / 1e9))
 
     def solve(fn):
@@ -217,9 +227,12 @@ def main():
                      "unit": "GB/s", "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
                      # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE runs recorded in
                      # profiles/spmv_cg_poisson3d_96cube.md (96^3 system only)
-                     "traffic": 167.8e6 if args.n == 96 else None,
-                     "kernel": "k_spmv_sell<double, 2> (SELL-64 FP64)",
-                     "bytes_per_launch": b_spmv64, "avg_ms": round(spmv_ms, 4)},
+                     "traffic": 142.3e6 if args.n == 96 and sell["col_bytes"] == 2 else None,
+                     "kernel": "k_spmv_sell<double, 2> (SELL-64 FP64, %d-byte columns)"
+                               % sell["col_bytes"],
+                     "bytes_per_launch": b_spmv64, "avg_ms": round(spmv_ms, 4),
+                     "csr32_bytes_per_launch": b_csr64,
+                     "csr32_equivalent_GBs": round(b_csr64 / (spmv_ms * 1e-3) / 1e9, 1)},
         "cpu_baseline": cpu,
     }
     if args.sweep_precs:

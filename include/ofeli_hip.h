@@ -114,6 +114,14 @@ int ofh_csr_values_device(ofh_csr A, double** d_val);
  * streams 32-bit columns.  Results are identical either way. */
 int ofh_csr_compress_columns(ofh_csr A, int enable);
 
+/* The SELL-64 copy that y = A x and the Krylov products stream (no reference
+ * counterpart: an internal format, reported for measurement).  Builds the
+ * copy if the matrix has none yet; *in_use = 0 when its padding would exceed
+ * a quarter of nnz (the CSR-stream kernels run), else 1 with *stored = the
+ * stored entries (nnz + padding) and *col_bytes = 2 (16-bit offsets from the
+ * chunk's first row) or 4 (32-bit columns). */
+int ofh_csr_sell_info(ofh_csr A, int* in_use, int64_t* stored, int* col_bytes);
+
 /* y = A x: SpMatrix::Mult (SpMatrix_impl.h:728-734). */
 int ofh_spmv_f64(ofh_csr A, const double* d_x, double* d_y);
 /* y += a A x: SpMatrix::MultAdd (SpMatrix_impl.h:737-757). */

@@ -118,6 +118,12 @@ class CsrMatrix:
         """16-bit block-relative column offsets in the SpMV stream (default off)."""
         check(lib().ofh_csr_compress_columns(self.h, int(bool(enable))))
 
+    def sell_info(self):
+        """The SELL-64 copy the products stream: {"in_use", "stored", "col_bytes"}."""
+        u, st, cb = C.c_int(), C.c_int64(), C.c_int()
+        check(lib().ofh_csr_sell_info(self.h, C.byref(u), C.byref(st), C.byref(cb)))
+        return {"in_use": bool(u.value), "stored": st.value, "col_bytes": cb.value}
+
     def pattern(self):
         rp = np.zeros(self.n + 1, np.int32)
         cl = np.zeros(max(self.nnz, 1), np.int32)

@@ -303,7 +303,8 @@ struct SellGatherDir {
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_cg_dirspmv_sell(
     int n, int nblocks, const int* __restrict__ cptr, const int* __restrict__ clen,
-    const uint32_t* __restrict__ col, const double* __restrict__ val, int it_arg, int nparts0,
+    const uint32_t* __restrict__ col, const int16_t* __restrict__ col16,
+    const double* __restrict__ val, int it_arg, int nparts0,
     int nparts, const double* __restrict__ z, double* __restrict__ p0, double* __restrict__ p1,
     double* __restrict__ q, KrylovState* st, double* __restrict__ partials) {
   if (st->done) return;
@@ -358,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_dirspmv_sell(
   const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
   double v[1] = {0.0};
   if (row / kSellChunk * kSellChunk < n) {  // whole waves past the end only reduce
-    const double sq = sell_rowsum_g<double, 2>(row, cptr, clen, col, val, g);
+    const double sq = sell_rowsum_g<double, 2>(row, cptr, clen, col, col16, val, g);
     if (row < n) {
       const double pi = it == 0 ? z[row] : z[row] + beta * p_old[row];
       p_new[row] = pi;
@@ -376,6 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_sell(int n, int nblocks,
                                                         const int* __restrict__ cptr,
                                                         const int* __restrict__ clen,
                                                         const uint32_t* __restrict__ col,
+                                                        const int16_t* __restrict__ col16,
                                                         const double* __restrict__ val,
                                                         const double* __restrict__ p,
                                                         double* __restrict__ q,
@@ -386,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_sell(int n, int nblocks,
   const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
   double v[1] = {0.0};
   if (row / kSellChunk * kSellChunk < n) {  // whole waves past the end only reduce
-    const double s = sell_rowsum<double, 2>(row, cptr, clen, col, val, x_rsrc(p, n));
+    const double s = sell_rowsum<double, 2>(row, cptr, clen, col, col16, val, x_rsrc(p, n));
     if (row < n) {
       q[row] = s;
       v[0] = p[row] * s;
@@ -553,7 +555,8 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
     auto dirspmv = [&](int it, hipStream_t st_) {
       if (sell)
         hipLaunchKernelGGL(k_cg_dirspmv_sell<PREC>, dim3(nbs), dim3(kBlock), 0, st_, n, nbs,
-                           A->sell_cptr, A->sell_clen, A->sell_col, A->sell_val, it, nb, g, z, p,
+                           A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val, it,
+                           nb, g, z, p,
                            p1, q, st, part);
       else
         hipLaunchKernelGGL(k_cg_dirspmv<PREC>, dim3(nb), dim3(kBlock), 0, st_, nb, A->row_blocks,
@@ -626,7 +629,7 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   auto iteration = [&](int it, hipStream_t st_) -> int {
     if (sell)
       hipLaunchKernelGGL(k_cg_spmv_sell, dim3(nbs), dim3(kBlock), 0, st_, n, nbs, A->sell_cptr,
-                         A->sell_clen, A->sell_col, A->sell_val, p, q, st, part);
+                         A->sell_clen, A->sell_col, A->sell_col16, A->sell_val, p, q, st, part);
     else
       hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, st_, nb, A->row_blocks,
                          A->row_ptr, ColIndex{A->col, A->col16}, A->val, p, q, st, part);

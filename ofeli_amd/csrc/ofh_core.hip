@@ -448,6 +448,7 @@ int ofh_csr_destroy(ofh_csr A) {
   hipFree(A->sell_cptr);
   hipFree(A->sell_clen);
   hipFree(A->sell_col);
+  hipFree(A->sell_col16);
   hipFree(A->sell_val);
   hipFree(A->sell_val32);
   delete A;

@@ -230,6 +230,8 @@ struct ofh_csr_s {
   int* sell_cptr = nullptr;      // nchunks: first stored entry of each chunk
   int* sell_clen = nullptr;      // nchunks: stored entries per row of the chunk
   uint32_t* sell_col = nullptr;  // column index (padding: kSellPadCol)
+  int16_t* sell_col16 = nullptr;  // column - first row of the chunk, when every one fits
+                                  // (padding: kSellPad16); null otherwise
   double* sell_val = nullptr;    // padding: -0.0
   float* sell_val32 = nullptr;
   uint64_t sell_version = ~0ull, sell_version32 = ~0ull;

@@ -192,6 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_spmv_sell(int n, int nblocks,
                                                         const int* __restrict__ cptr,
                                                         const int* __restrict__ clen,
                                                         const uint32_t* __restrict__ col,
+                                                        const int16_t* __restrict__ col16,
                                                         const float* __restrict__ val32,
                                                         const float* __restrict__ p32,
                                                         float* __restrict__ q32,
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_spmv_sell(int n, int nblocks,
   const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
   double v[1] = {0.0};
   if (row / kSellChunk * kSellChunk < n) {  // whole waves past the end only reduce
-    const float s = sell_rowsum<float, 4>(row, cptr, clen, col, val32, x_rsrc(p32, n));
+    const float s = sell_rowsum<float, 4>(row, cptr, clen, col, col16, val32, x_rsrc(p32, n));
     if (row < n) {
       q32[row] = s;
       v[0] = (double)p32[row] * (double)s;
@@ -269,6 +270,7 @@ struct SellF32 {
   const int* __restrict__ cptr;
   const int* __restrict__ clen;
   const uint32_t* __restrict__ col;
+  const int16_t* __restrict__ col16;
   const float* __restrict__ val;
 };
 
@@ -361,7 +363,8 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* _
     const SellGatherDirF g{x_rsrc(z32, n), x_rsrc(p_old, n), bf, k == 0};
     const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
     if (row / kSellChunk * kSellChunk < n) {  // whole waves past the end only reduce
-      const float sq = sell_rowsum_g<float, 4>(row, sa.cptr, sa.clen, sa.col, sa.val, g);
+      const float sq =
+          sell_rowsum_g<float, 4>(row, sa.cptr, sa.clen, sa.col, sa.col16, sa.val, g);
       if (row < n) {
         const float pi = k == 0 ? z32[row] : z32[row] + bf * p_old[row];
         p_new[row] = pi;
@@ -481,7 +484,7 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   bool fsell = false;
   const int nbsf = (n + kBlock - 1) / kBlock;
   if (fused && nbsf <= kMaxPartials && (rc = csr_ensure_sell(A, true, &fsell))) return rc;
-  const SellF32 sa{n, A->sell_cptr, A->sell_clen, A->sell_col, A->sell_val32};
+  const SellF32 sa{n, A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val32};
   const int npq_fused = fsell ? nbsf : A->nblocks32;  // partials of the fused product
   auto dirspmv = [&](int phase, int k, int nparts, hipStream_t st_) {
     const int nbf = fsell ? nbsf : A->nblocks32;
@@ -534,7 +537,8 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     }
     if (sell)
       hipLaunchKernelGGL(k_mx_spmv_sell, dim3(nbs), dim3(kBlock), 0, st_, n, nbs, A->sell_cptr,
-                         A->sell_clen, A->sell_col, A->sell_val32, p32, q32, st, part);
+                         A->sell_clen, A->sell_col, A->sell_col16, A->sell_val32, p32, q32, st,
+                         part);
     else
       hipLaunchKernelGGL(k_mx_spmv, dim3(nb32), dim3(kBlock), 0, st_, nb32, A->row_blocks32,
                          A->row_ptr, ci32, A->val32, p32, q32, st, part);

@@ -98,6 +98,9 @@ __device__ __forceinline__ T csr_stream_rowsum_g(int r0, int r1, const int* __re
 // ---------------------------------------------------------------------------
 constexpr int kSellChunk = 64;
 constexpr uint32_t kSellPadCol = 0x3FFFFFFEu;  // x byte offset >= 2^31 for 4- and 8-byte x
+// 16-bit form of the columns: column - first row of the chunk, in
+// [-32767, 32767]; the padding slots hold kSellPad16 and read as kSellPadCol.
+constexpr int kSellPad16 = -32768;
 
 using XRsrc = __amdgpu_buffer_rsrc_t;
 
@@ -122,14 +125,27 @@ struct SellGather {
   __device__ __forceinline__ T operator()(uint32_t c) const { return x_load<T>(x, c); }
 };
 
+// Column of stored entry e, from the 32-bit or the 16-bit array.
+struct SellCol32 {
+  const uint32_t* __restrict__ c;
+  __device__ __forceinline__ uint32_t operator()(int e) const { return c[e]; }
+};
+struct SellCol16 {
+  const int16_t* __restrict__ c;
+  int row0;  // first row of the chunk
+  __device__ __forceinline__ uint32_t operator()(int e) const {
+    const int o = c[e];
+    return o == kSellPad16 ? kSellPadCol : (uint32_t)(row0 + o);
+  }
+};
+
 // Sum of row `row` (chunk q = row / 64) of the SELL copy, from 0, with U
-// entries' loads in flight; G gathers the multiplied vector and must return
-// +0.0 for the padding column.
-template <typename T, int U, class G>
-__device__ __forceinline__ T sell_rowsum_g(int row, const int* __restrict__ cptr,
-                                           const int* __restrict__ clen,
-                                           const uint32_t* __restrict__ col,
-                                           const T* __restrict__ val, const G& g) {
+// entries' loads in flight; C reads the columns, G gathers the multiplied
+// vector and must return +0.0 for the padding column.
+template <typename T, int U, class C, class G>
+__device__ __forceinline__ T sell_rowsum_cg(int row, const int* __restrict__ cptr,
+                                            const int* __restrict__ clen, const C& col,
+                                            const T* __restrict__ val, const G& g) {
   const int q = row / kSellChunk;
   const int base = cptr[q] + (row & (kSellChunk - 1));
   const int len = clen[q];
@@ -140,7 +156,7 @@ __device__ __forceinline__ T sell_rowsum_g(int row, const int* __restrict__ cptr
     T a[U], v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      c[u] = col[base + (j + u) * kSellChunk];
+      c[u] = col(base + (j + u) * kSellChunk);
       a[u] = val[base + (j + u) * kSellChunk];
     }
 #pragma unroll
@@ -148,16 +164,31 @@ __device__ __forceinline__ T sell_rowsum_g(int row, const int* __restrict__ cptr
 #pragma unroll
     for (int u = 0; u < U; ++u) s += a[u] * v[u];
   }
-  for (; j < len; ++j) s += val[base + j * kSellChunk] * g(col[base + j * kSellChunk]);
+  for (; j < len; ++j) s += val[base + j * kSellChunk] * g(col(base + j * kSellChunk));
   return s;
+}
+
+// The same from the 16-bit columns when the copy has them (col16 non-null,
+// the same for every lane of the launch), else from the 32-bit ones.
+template <typename T, int U, class G>
+__device__ __forceinline__ T sell_rowsum_g(int row, const int* __restrict__ cptr,
+                                           const int* __restrict__ clen,
+                                           const uint32_t* __restrict__ col,
+                                           const int16_t* __restrict__ col16,
+                                           const T* __restrict__ val, const G& g) {
+  if (col16)
+    return sell_rowsum_cg<T, U>(row, cptr, clen, SellCol16{col16, row & ~(kSellChunk - 1)}, val,
+                                g);
+  return sell_rowsum_cg<T, U>(row, cptr, clen, SellCol32{col}, val, g);
 }
 
 template <typename T, int U>
 __device__ __forceinline__ T sell_rowsum(int row, const int* __restrict__ cptr,
                                          const int* __restrict__ clen,
                                          const uint32_t* __restrict__ col,
+                                         const int16_t* __restrict__ col16,
                                          const T* __restrict__ val, XRsrc xr) {
-  return sell_rowsum_g<T, U>(row, cptr, clen, col, val, SellGather<T>{xr});
+  return sell_rowsum_g<T, U>(row, cptr, clen, col, col16, val, SellGather<T>{xr});
 }
 
 template <typename T, int TILE, bool SCALED = false>
@@ -177,6 +208,7 @@ struct Sell64 {
   const int* __restrict__ cptr;
   const int* __restrict__ clen;
   const uint32_t* __restrict__ col;
+  const int16_t* __restrict__ col16;
   const double* __restrict__ val;
 };
 struct MatRows64 {
@@ -198,7 +230,7 @@ __device__ __forceinline__ double mat_rowsum64(const MatRows64& m, const double*
     i = xcd_block(blockIdx.x, m.nblocks) * kBlock + threadIdx.x;
     valid = i < m.sell.n;
     if (i / kSellChunk * kSellChunk >= m.sell.n) return 0.0;  // a whole wave past the end
-    return sell_rowsum<double, 2>(i, m.sell.cptr, m.sell.clen, m.sell.col, m.sell.val,
+    return sell_rowsum<double, 2>(i, m.sell.cptr, m.sell.clen, m.sell.col, m.sell.col16, m.sell.val,
                                   x_rsrc(x, m.sell.n));
   } else {
     __shared__ double lds[kTile64];
@@ -222,7 +254,8 @@ inline int mat_rows64(ofh_csr A, MatRows64* m, bool* sell) {
   }
   *m = MatRows64{*sell ? nbs : A->nblocks, A->row_blocks, A->row_ptr,
                  ColIndex{A->col, A->col16}, A->val,
-                 Sell64{A->n, A->sell_cptr, A->sell_clen, A->sell_col, A->sell_val}};
+                 Sell64{A->n, A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16,
+                        A->sell_val}};
   return OFH_OK;
 }
 

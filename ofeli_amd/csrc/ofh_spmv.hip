@@ -31,19 +31,37 @@ namespace ofh {
 // ---------------------------------------------------------------------------
 // SELL-64 copy
 // ---------------------------------------------------------------------------
-// lane = row of its chunk: columns of the row's stored slots
+// lane = row of its chunk: columns of the row's stored slots, as 32-bit
+// indices and as 16-bit offsets from the chunk's first row (*bad = 1 when one
+// does not fit)
 __global__ __launch_bounds__(kBlock) void k_sell_cols(int n, const int* __restrict__ rp,
                                                       const int* __restrict__ col,
                                                       const int* __restrict__ cptr,
                                                       const int* __restrict__ clen,
-                                                      uint32_t* __restrict__ scol) {
+                                                      uint32_t* __restrict__ scol,
+                                                      int16_t* __restrict__ scol16,
+                                                      int* __restrict__ bad) {
   const int row = blockIdx.x * kBlock + threadIdx.x;
   const int q = row / kSellChunk;
   if (q * kSellChunk >= n) return;
   const int base = cptr[q] + (row & (kSellChunk - 1)), len = clen[q];
   const int k0 = row < n ? rp[row] : 0, nr = row < n ? rp[row + 1] - k0 : 0;
-  for (int j = 0; j < len; ++j)
-    scol[base + j * kSellChunk] = j < nr ? (uint32_t)col[k0 + j] : kSellPadCol;
+  const int row0 = q * kSellChunk;
+  bool fits = true;
+  for (int j = 0; j < len; ++j) {
+    const int e = base + j * kSellChunk;
+    if (j < nr) {
+      const int c = col[k0 + j];
+      const int o = c - row0;
+      fits = fits && o >= -32767 && o <= 32767;
+      scol[e] = (uint32_t)c;
+      scol16[e] = (int16_t)(fits ? o : 0);
+    } else {
+      scol[e] = kSellPadCol;
+      scol16[e] = (int16_t)kSellPad16;
+    }
+  }
+  if (!fits) *bad = 1;  // every writer stores the same value
 }
 
 template <typename T>
@@ -85,24 +103,37 @@ int csr_ensure_sell(ofh_csr A, bool fp32, bool* ok) {
       A->sell_state = -1;
       return OFH_OK;
     }
-    int *d_cptr = nullptr, *d_clen = nullptr;
+    int *d_cptr = nullptr, *d_clen = nullptr, *d_bad = nullptr;
     uint32_t* d_col = nullptr;
+    int16_t* d_col16 = nullptr;
     double* d_val = nullptr;
+    const size_t tot = (size_t)std::max<int64_t>(total, 1);
     OFH_HIP(hipMalloc(&d_cptr, sizeof(int) * nch));
     OFH_HIP(hipMalloc(&d_clen, sizeof(int) * nch));
-    OFH_HIP(hipMalloc(&d_col, sizeof(uint32_t) * std::max<int64_t>(total, 1)));
-    OFH_HIP(hipMalloc(&d_val, sizeof(double) * std::max<int64_t>(total, 1)));
+    OFH_HIP(hipMalloc(&d_bad, sizeof(int)));
+    OFH_HIP(hipMalloc(&d_col, sizeof(uint32_t) * tot));
+    OFH_HIP(hipMalloc(&d_col16, sizeof(int16_t) * tot));
+    OFH_HIP(hipMalloc(&d_val, sizeof(double) * tot));
     OFH_HIP(hipMemcpyAsync(d_cptr, cptr.data(), sizeof(int) * nch, hipMemcpyHostToDevice, s));
     OFH_HIP(hipMemcpyAsync(d_clen, clen.data(), sizeof(int) * nch, hipMemcpyHostToDevice, s));
+    OFH_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_sell_cols, dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col, d_cptr,
-                       d_clen, d_col);
+                       d_clen, d_col, d_col16, d_bad);
     OFH_HIP(hipGetLastError());
+    int bad = 0;
+    OFH_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s));
     OFH_HIP(hipStreamSynchronize(s));  // cptr, clen leave scope
+    OFH_HIP(hipFree(d_bad));
+    if (bad) {  // some column lies too far from its chunk: 32-bit columns only
+      OFH_HIP(hipFree(d_col16));
+      d_col16 = nullptr;
+    }
     A->sell_nchunks = nch;
     A->sell_total = total;
     A->sell_cptr = d_cptr;
     A->sell_clen = d_clen;
     A->sell_col = d_col;
+    A->sell_col16 = d_col16;
     A->sell_val = d_val;
     A->sell_version = A->sell_version32 = ~0ull;
     A->sell_state = 1;
@@ -131,11 +162,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(int n, int nblocks,
                                                       const int* __restrict__ cptr,
                                                       const int* __restrict__ clen,
                                                       const uint32_t* __restrict__ col,
+                                                      const int16_t* __restrict__ col16,
                                                       const T* __restrict__ val, const T* x,
                                                       T* __restrict__ y) {
   const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
   if (row / kSellChunk * kSellChunk >= n) return;  // whole wave past the end
-  const T s = sell_rowsum<T, U>(row, cptr, clen, col, val, x_rsrc(x, n));
+  const T s = sell_rowsum<T, U>(row, cptr, clen, col, col16, val, x_rsrc(x, n));
   if (row < n) y[row] = s;
 }
 
@@ -169,6 +201,23 @@ static int launch(ofh_csr A, const T* val, const T* x, T* y, T alpha, hipStream_
   return OFH_OK;
 }
 
+}  // namespace ofh
+
+extern "C" int ofh_csr_sell_info(ofh_csr A, int* in_use, int64_t* stored, int* col_bytes) {
+  using namespace ofh;
+  OFH_CHECK(A && in_use && stored && col_bytes, OFH_E_INVALID, "ofh_csr_sell_info: null argument");
+  bool ok = false;
+  const int rc = csr_ensure_sell(A, false, &ok);
+  if (rc) return rc;
+  OFH_HIP(hipStreamSynchronize(A->ctx->stream));
+  *in_use = ok ? 1 : 0;
+  *stored = ok ? A->sell_total : 0;
+  *col_bytes = ok ? (A->sell_col16 ? 2 : 4) : 0;
+  return OFH_OK;
+}
+
+namespace ofh {
+
 int launch_spmv64(ofh_csr A, const double* x, double* y, hipStream_t s) {
   bool sell;
   int rc = csr_ensure_sell(A, false, &sell);
@@ -176,7 +225,7 @@ int launch_spmv64(ofh_csr A, const double* x, double* y, hipStream_t s) {
   if (sell) {
     const int nb = (A->n + kBlock - 1) / kBlock;
     hipLaunchKernelGGL((k_spmv_sell<double, 2>), dim3(nb), dim3(kBlock), 0, s, A->n, nb,
-                       A->sell_cptr, A->sell_clen, A->sell_col, A->sell_val, x, y);
+                       A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val, x, y);
     OFH_HIP(hipGetLastError());
     return OFH_OK;
   }
@@ -190,7 +239,8 @@ int launch_spmv32(ofh_csr A, const float* x, float* y, hipStream_t s) {
   if (sell) {
     const int nb = (A->n + kBlock - 1) / kBlock;
     hipLaunchKernelGGL((k_spmv_sell<float, 4>), dim3(nb), dim3(kBlock), 0, s, A->n, nb,
-                       A->sell_cptr, A->sell_clen, A->sell_col, A->sell_val32, x, y);
+                       A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val32, x,
+                       y);
     OFH_HIP(hipGetLastError());
     return OFH_OK;
   }

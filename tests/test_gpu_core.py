@@ -96,6 +96,61 @@ def test_spmv_sell_bitexact_and_value_changes(ctx, shape):
     assert np.array_equal(y32.cpu().numpy(), ref)
 
 
+def far_coupled_system(n=40000, far=35000):
+    """SPD system: a 1D Laplacian plus couplings i <-> i + far.  The far
+    columns lie more than 32,767 rows from their 64-row chunk, so the SELL-64
+    copy keeps 32-bit columns only."""
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        ent = {i: 4.0 + (i % 7) * 0.25}
+        if i > 0:
+            ent[i - 1] = -1.0
+        if i + 1 < n:
+            ent[i + 1] = -1.0
+        if i + far < n:
+            ent[i + far] = -0.5
+        if i - far >= 0:
+            ent[i - far] = -0.5
+        for j in sorted(ent):
+            rows.append(i)
+            cols.append(j)
+            vals.append(ent[j])
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, np.asarray(rows) + 1, 1)
+    return np.cumsum(rp), np.asarray(cols, np.int64) + 1, np.asarray(vals)
+
+
+def test_sell_with_32bit_columns_only(ctx):
+    """The SELL-64 products with columns too far for 16-bit offsets: y = A x
+    bitwise the oracle's row sums (FP64 and sequential FP32), and CG, mixed CG
+    and BiCGStab (their products read the same copy) solve to the tolerance."""
+    import ofeli_amd as oa
+    torch = _torch()
+    rp, ci, a = far_coupled_system()
+    A = gpu_csr(ctx, rp, ci, a)
+    rng = np.random.default_rng(41)
+    x = rng.standard_normal(A.n)
+    y = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+    oa.spmv(A, dev(x), y)
+    assert np.array_equal(host(y), O.spmv(rp, ci, a, x))
+    x32, a32 = x.astype(np.float32), a.astype(np.float32)
+    y32 = torch.zeros(A.n, dtype=torch.float32, device="cuda")
+    oa.spmv_f32(A, torch.as_tensor(x32, device="cuda"), y32)
+    ref = np.zeros(A.n, np.float32)
+    for i in range(A.n):  # sequential float32 row sums
+        s = np.float32(0.0)
+        for k in range(rp[i], rp[i + 1]):
+            s = np.float32(s + np.float32(a32[k] * x32[ci[k] - 1]))
+        ref[i] = s
+    assert np.array_equal(y32.cpu().numpy(), ref)
+    b = O.spmv(rp, ci, a, np.ones(A.n))  # solution 1
+    for fn in (oa.cg, oa.cg_mixed, oa.bicgstab):
+        xs = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+        info = fn(A, dev(b), xs, prec=oa.DIAG_PREC, max_it=2000, toler=1e-10)
+        assert info.true_relres <= 2e-10, fn  # recursive-residual stop tests
+        assert rel(host(xs), np.ones(A.n)) < 1e-8, fn
+
+
 def test_spmv_column_compression_is_exact(ctx):
     """16-bit block-relative columns and 32-bit columns give the same bits."""
     import ofeli_amd as oa
