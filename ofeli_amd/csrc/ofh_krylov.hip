@@ -20,21 +20,16 @@ namespace ofh {
 // ---------------------------------------------------------------------------
 // true residual ||b - A x|| / ||b|| (FP64, no temporary vector)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_true_res(int nblocks, const int* __restrict__ rb,
-                                                    const int* __restrict__ rp,
-                                                    ColIndex ci,
-                                                    const double* __restrict__ val,
-                                                    const double* __restrict__ b,
+template <bool SELL>
+__global__ __launch_bounds__(kBlock) void k_true_res(MatRows64 m, const double* __restrict__ b,
                                                     const double* __restrict__ x,
                                                     double* __restrict__ partials) {
-  __shared__ double lds[kTile64];
   __shared__ double red[2 * kBlock / kWave];
-  const int blk = xcd_block(blockIdx.x, nblocks);
-  const int r0 = rb[blk], r1 = rb[blk + 1];
-  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, ci, val, x, lds, 0.0);
-  const int i = r0 + threadIdx.x;
+  int i;
+  bool valid;
+  const double s = mat_rowsum64<SELL>(m, x, i, valid);
   double v[2] = {0.0, 0.0};
-  if (i < r1) {
+  if (valid) {
     const double d = b[i] - s;
     v[0] = d * d;
     v[1] = b[i] * b[i];
@@ -60,9 +55,17 @@ __global__ __launch_bounds__(kBlock) void k_true_fold(int nparts, const double* 
 int true_relres(ofh_csr A, const double* b, const double* x, double* out) {
   ofh_ctx ctx = A->ctx;
   double* d_out = ctx->dev_scratch + 16;
-  hipLaunchKernelGGL(k_true_res, dim3(A->nblocks), dim3(kBlock), 0, ctx->stream, A->nblocks,
-                     A->row_blocks, A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, ctx->ws.partials);
-  hipLaunchKernelGGL(k_true_fold, dim3(1), dim3(kBlock), 0, ctx->stream, A->nblocks,
+  MatRows64 mr;
+  bool sell = false;
+  int rc = mat_rows64(A, &mr, &sell);
+  if (rc) return rc;
+  if (sell)
+    hipLaunchKernelGGL(k_true_res<true>, dim3(mr.nblocks), dim3(kBlock), 0, ctx->stream, mr, b, x,
+                       ctx->ws.partials);
+  else
+    hipLaunchKernelGGL(k_true_res<false>, dim3(mr.nblocks), dim3(kBlock), 0, ctx->stream, mr, b,
+                       x, ctx->ws.partials);
+  hipLaunchKernelGGL(k_true_fold, dim3(1), dim3(kBlock), 0, ctx->stream, mr.nblocks,
                      ctx->ws.partials, d_out);
   OFH_HIP(hipGetLastError());
   OFH_HIP(hipMemcpyAsync(ctx->host_pinned, d_out, sizeof(double), hipMemcpyDeviceToHost,

@@ -61,22 +61,18 @@ __device__ __forceinline__ float zf(int i, float ri, const float* __restrict__ d
 
 // FP64 residual r = b - A x64 -> r32 (rounded), x32 = 0; partials
 // (b,b) -> 0, (r,r) -> 1 in FP64, (r32,z32) -> 2, (r32,r32) -> 3.
-template <int PREC>
+template <int PREC, bool SELL>
 __global__ __launch_bounds__(kBlock) void k_mx_resid(
-    int nblocks, const int* __restrict__ rb, const int* __restrict__ rp,
-    ColIndex ci, const double* __restrict__ val, const double* __restrict__ b,
-    const double* __restrict__ x, float* __restrict__ r32, float* __restrict__ x32,
-    const float* __restrict__ dinv32, float* __restrict__ z32, const KrylovState* __restrict__ st,
-    double* __restrict__ partials) {
+    MatRows64 m, const double* __restrict__ b, const double* __restrict__ x,
+    float* __restrict__ r32, float* __restrict__ x32, const float* __restrict__ dinv32,
+    float* __restrict__ z32, const KrylovState* __restrict__ st, double* __restrict__ partials) {
   if (st->done) return;
-  __shared__ double lds[kTile64];
   __shared__ double red[4 * kBlock / kWave];
-  const int blk = xcd_block(blockIdx.x, nblocks);
-  const int r0 = rb[blk], r1 = rb[blk + 1];
-  const double s = csr_stream_rowsum<double, kTile64>(r0, r1, rp, ci, val, x, lds, 0.0);
-  const int i = r0 + threadIdx.x;
+  int i;
+  bool valid;
+  const double s = mat_rowsum64<SELL>(m, x, i, valid);
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  if (i < r1) {
+  if (valid) {
     const double bi = b[i], ri = bi - s;
     const float rf = (float)ri;
     r32[i] = rf;
@@ -510,7 +506,6 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   OFH_HIP(hipMemcpyAsync(st, &h0, sizeof(h0), hipMemcpyHostToDevice, s));
   double* part = ctx->ws.partials;
   const int g = reduce_grid(n);
-  const int nb = A->nblocks;      // FP64 partition (residual replacement)
   const int nb32 = A->nblocks32;  // FP32 partition (inner SpMV)
   const ColIndex ci32{A->col, nullptr};  // 16-bit offsets follow the FP64 partition
   const int ge = (n + 255) / 256;
@@ -518,14 +513,25 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   bool sell = false;
   const int nbs = (n + kBlock - 1) / kBlock;
   if (!fused && nbs <= kMaxPartials && (rc = csr_ensure_sell(A, true, &sell))) return rc;
-  hipLaunchKernelGGL(k_mx_resid<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                     A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r32, x32, A->dinv32, zw,
-                     st, part);
+  // the FP64 residual b - A x, from the SELL-64 copy when it is in use
+  MatRows64 mr;
+  bool rsell = false;
+  if ((rc = mat_rows64(A, &mr, &rsell))) return rc;
+  const int nbr = mr.nblocks;  // its workgroups (= partials)
+  auto resid = [&]() {
+    if (rsell)
+      hipLaunchKernelGGL((k_mx_resid<PREC, true>), dim3(nbr), dim3(kBlock), 0, s, mr, b, x, r32,
+                         x32, A->dinv32, zw, st, part);
+    else
+      hipLaunchKernelGGL((k_mx_resid<PREC, false>), dim3(nbr), dim3(kBlock), 0, s, mr, b, x, r32,
+                         x32, A->dinv32, zw, st, part);
+  };
+  resid();
   if (fused)
-    dirspmv(1, 0, nb, s);
+    dirspmv(1, 0, nbr, s);
   else
-    hipLaunchKernelGGL((k_mx_dir<PREC, 1>), dim3(g), dim3(kBlock), 0, s, n, 0, nb, r32, A->dinv32,
-                       p32, st, part);
+    hipLaunchKernelGGL((k_mx_dir<PREC, 1>), dim3(g), dim3(kBlock), 0, s, n, 0, nbr, r32,
+                       A->dinv32, p32, st, part);
   OFH_HIP(hipGetLastError());
   // one inner iteration; k = -1 takes the index from the state (captured chunks)
   auto iteration = [&](int k, hipStream_t st_) {
@@ -592,13 +598,11 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     last_cycle = kr - it_repl;
     // residual replacement in FP64 after kr inner iterations
     hipLaunchKernelGGL(k_mx_accum, dim3(ge), dim3(256), 0, s, n, x, x32, st);
-    hipLaunchKernelGGL(k_mx_resid<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
-                       A->row_ptr, ColIndex{A->col, A->col16}, A->val, b, x, r32, x32, A->dinv32,
-                       zw, st, part);
+    resid();
     if (fused)
-      dirspmv(2, kr, nb, s);
+      dirspmv(2, kr, nbr, s);
     else
-      hipLaunchKernelGGL((k_mx_dir<PREC, 2>), dim3(g), dim3(kBlock), 0, s, n, kr, nb, r32,
+      hipLaunchKernelGGL((k_mx_dir<PREC, 2>), dim3(g), dim3(kBlock), 0, s, n, kr, nbr, r32,
                          A->dinv32, p32, st, part);
     OFH_HIP(hipGetLastError());
     if ((rc = read_state(ctx, &h))) return rc;
