@@ -276,8 +276,12 @@ struct SellF32 {
 // the block's rows is stored; (p,q) -> slot 6.  z32 holds D^-1 r (DIAG) or
 // r (IDENT); p_old and p_new are ping-pong buffers.
 // SELL: the product from the SELL-64 copy (256 rows per workgroup, one per
-// lane), else from the CSR-stream partition.
-template <int PREC, int PHASE, bool SELL>
+// lane), else from the CSR-stream partition.  QREC (SELL only): the product
+// by recurrence, q_new = A z + beta q_old (= A p_new in exact arithmetic, as
+// q_old = A p_old), so the gather reads z alone and p is never gathered; the
+// rounding drift of q against A p is bounded by the FP64 residual
+// replacements, which end every cycle.
+template <int PREC, int PHASE, bool SELL, bool QREC>
 __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* __restrict__ rb,
                                                       const int* __restrict__ rp, ColIndex ci,
                                                       const float* __restrict__ val32,
@@ -353,7 +357,23 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* _
   }
   if (stop || pause) return;
   double v[1] = {0.0};
-  if constexpr (SELL) {
+  if constexpr (SELL && QREC) {
+    const int n = sa.n;
+    const float bf = (float)beta;
+    const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
+    if (row / kSellChunk * kSellChunk < n) {  // whole waves past the end only reduce
+      const float az = sell_rowsum_g<float, 4>(row, sa.cptr, sa.clen, sa.col, sa.col16, sa.val,
+                                               SellGather<float>{x_rsrc(z32, n)});
+      if (row < n) {
+        const float zi = z32[row];
+        const float pi = k == 0 ? zi : zi + bf * p_old[row];
+        const float qi = k == 0 ? az : az + bf * q32[row];
+        p_new[row] = pi;
+        q32[row] = qi;
+        v[0] = (double)pi * (double)qi;
+      }
+    }
+  } else if constexpr (SELL) {
     const int n = sa.n;
     const float bf = (float)beta;
     const SellGatherDirF g{x_rsrc(z32, n), x_rsrc(p_old, n), bf, k == 0};
@@ -462,9 +482,16 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   if (PREC == OFH_DIAG_PREC && (rc = csr_ensure_dinv(A))) return rc;
   if ((rc = csr_ensure_val32(A))) return rc;
   // L2-resident systems use two kernels per inner iteration (k_mx_upd,
-  // k_mx_dirspmv), larger ones three (k_mx_spmv, k_mx_upd, k_mx_dir)
+  // k_mx_dirspmv: from the SELL-64 copy with q by recurrence, one gather;
+  // else the direction formed at the gathers), larger ones three (k_mx_spmv,
+  // k_mx_upd, k_mx_dir): with 256 rows per product workgroup, every one of
+  // them would fold the update's partials.
   constexpr int kFuseRows = 3 << 17;
   const bool fused = n <= kFuseRows;
+  bool fsell = false;
+  const int nbsf = (n + kBlock - 1) / kBlock;
+  if (fused && nbsf <= kMaxPartials && (rc = csr_ensure_sell(A, true, &fsell))) return rc;
+  const bool qrec = fsell;
   void* wsp;
   if ((rc = ensure_vws(ctx, sizeof(float) * (size_t)n * 6, &wsp))) return rc;
   float* r32 = (float*)wsp;
@@ -476,26 +503,26 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   float* pb[2] = {p32, p32b};
   float* zw = fused && PREC == OFH_DIAG_PREC ? z32 : nullptr;  // z written by resid/upd
   const ColIndex ci{A->col, nullptr};  // FP32 partition: 32-bit columns
-  // the fused product from the SELL-64 copy when it is in use
-  bool fsell = false;
-  const int nbsf = (n + kBlock - 1) / kBlock;
-  if (fused && nbsf <= kMaxPartials && (rc = csr_ensure_sell(A, true, &fsell))) return rc;
   const SellF32 sa{n, A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val32};
   const int npq_fused = fsell ? nbsf : A->nblocks32;  // partials of the fused product
   auto dirspmv = [&](int phase, int k, int nparts, hipStream_t st_) {
     const int nbf = fsell ? nbsf : A->nblocks32;
-#define OFH_MX_DIRSPMV(PH, SL)                                                                \
-  hipLaunchKernelGGL((k_mx_dirspmv<PREC, PH, SL>), dim3(nbf), dim3(kBlock), 0, st_, nbf,       \
+#define OFH_MX_DIRSPMV(PH, SL, QR)                                                             \
+  hipLaunchKernelGGL((k_mx_dirspmv<PREC, PH, SL, QR>), dim3(nbf), dim3(kBlock), 0, st_, nbf,    \
                      A->row_blocks32, A->row_ptr, ci, A->val32, k, nparts, z32, pb[0], pb[1], \
                      q32, state_ptr(ctx), ctx->ws.partials, sa)
-    if (fsell) {
-      if (phase == 0) OFH_MX_DIRSPMV(0, true);
-      else if (phase == 1) OFH_MX_DIRSPMV(1, true);
-      else OFH_MX_DIRSPMV(2, true);
+    if (qrec) {
+      if (phase == 0) OFH_MX_DIRSPMV(0, true, true);
+      else if (phase == 1) OFH_MX_DIRSPMV(1, true, true);
+      else OFH_MX_DIRSPMV(2, true, true);
+    } else if (fsell) {
+      if (phase == 0) OFH_MX_DIRSPMV(0, true, false);
+      else if (phase == 1) OFH_MX_DIRSPMV(1, true, false);
+      else OFH_MX_DIRSPMV(2, true, false);
     } else {
-      if (phase == 0) OFH_MX_DIRSPMV(0, false);
-      else if (phase == 1) OFH_MX_DIRSPMV(1, false);
-      else OFH_MX_DIRSPMV(2, false);
+      if (phase == 0) OFH_MX_DIRSPMV(0, false, false);
+      else if (phase == 1) OFH_MX_DIRSPMV(1, false, false);
+      else OFH_MX_DIRSPMV(2, false, false);
     }
 #undef OFH_MX_DIRSPMV
   };
