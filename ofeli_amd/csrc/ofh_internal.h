@@ -151,6 +151,36 @@ __device__ __forceinline__ void fold_partials(double (&out)[K], const int (&slot
   block_sum_k<BLOCK, K>(out, lds);
 }
 
+// fold_partials in two steps for nparts <= 7 * BLOCK: the loads first (in
+// flight while the caller starts other work), then the same sums in the same
+// order.
+template <int BLOCK, int K>
+struct PartialsPrefetch {
+  static constexpr int kMax = 7 * BLOCK;
+  double v[K][7];
+  __device__ __forceinline__ void load(const int (&slot)[K], const double* __restrict__ partials,
+                                       int nparts) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        const int i = threadIdx.x + j * BLOCK;
+        v[k][j] = i < nparts ? partials[slot[k] * kMaxPartials + i] : 0.0;
+      }
+  }
+  __device__ __forceinline__ void fold(double (&out)[K], int nparts, double* lds) const {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < 7; ++j)
+        if (threadIdx.x + j * BLOCK < nparts) s += v[k][j];
+      out[k] = s;
+    }
+    block_sum_k<BLOCK, K>(out, lds);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // objects behind the opaque handles
 // ---------------------------------------------------------------------------

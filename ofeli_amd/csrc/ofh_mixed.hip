@@ -16,9 +16,12 @@
 // best cycle's, or the recursive residual needed over 3x the first cycle's
 // length: FP32 no longer resolves the slow modes and the host forced the
 // replacement), the FP64 CG finishes from the current x64.
-// Per inner iteration, L2-resident systems (n <= 393,216) use two kernels,
-// k_mx_upd and k_mx_dirspmv (the direction update formed at the SpMV's
-// gather); larger ones three: k_mx_spmv, k_mx_upd, k_mx_dir.
+// Per inner iteration two kernels, k_mx_upd and k_mx_dirspmv, when the
+// product reads the SELL-64 copy: there q = A p is carried by recurrence,
+// q_new = A z + beta q_old, so the product gathers z alone and runs before
+// the direction's decision.  Without the SELL copy, L2-resident systems
+// (n <= 393,216) form the direction at the product's gathers and larger ones
+// use three kernels: k_mx_spmv, k_mx_upd, k_mx_dir.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -296,13 +299,30 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* _
   const float* __restrict__ p_old = (k & 1) ? p1 : p0;
   float* __restrict__ p_new = (k & 1) ? p0 : p1;
   __shared__ double red[4 * kBlock / kWave];
+  // QREC: A z does not depend on the decision below, so the product runs
+  // first, with the update's partials already loading (prefetch) when few
+  float az = 0.0f;
+  int qrow = 0;
+  const bool pre = QREC && PHASE == 0 && nparts <= PartialsPrefetch<kBlock, 2>::kMax;
+  PartialsPrefetch<kBlock, 2> pf;
+  if constexpr (SELL && QREC) {
+    const int slot[2] = {4, 5};
+    if (pre) pf.load(slot, partials, nparts);
+    qrow = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
+    if (qrow / kSellChunk * kSellChunk < sa.n)  // whole waves past the end skip it
+      az = sell_rowsum_g<float, 4>(qrow, sa.cptr, sa.clen, sa.col, sa.col16, sa.val,
+                                   SellGather<float>{x_rsrc(z32, sa.n)});
+  }
   double nrm = st->nrm, res, rz;
   bool stop = false, pause = false;
   int status = 0;
   if (PHASE == 0) {
     double t[2];
     const int slot[2] = {4, 5};
-    fold_partials<kBlock, 2>(t, slot, partials, nparts, red);
+    if (pre)
+      pf.fold(t, nparts, red);
+    else
+      fold_partials<kBlock, 2>(t, slot, partials, nparts, red);
     res = sqrt(t[0]) / nrm;
     rz = t[1];
     pause = res <= kDelta * st->rmax || res <= st->tol || k >= st->max_it || !(res == res);
@@ -358,20 +378,15 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* _
   if (stop || pause) return;
   double v[1] = {0.0};
   if constexpr (SELL && QREC) {
-    const int n = sa.n;
     const float bf = (float)beta;
-    const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
-    if (row / kSellChunk * kSellChunk < n) {  // whole waves past the end only reduce
-      const float az = sell_rowsum_g<float, 4>(row, sa.cptr, sa.clen, sa.col, sa.col16, sa.val,
-                                               SellGather<float>{x_rsrc(z32, n)});
-      if (row < n) {
-        const float zi = z32[row];
-        const float pi = k == 0 ? zi : zi + bf * p_old[row];
-        const float qi = k == 0 ? az : az + bf * q32[row];
-        p_new[row] = pi;
-        q32[row] = qi;
-        v[0] = (double)pi * (double)qi;
-      }
+    const int row = qrow;
+    if (row < sa.n) {
+      const float zi = z32[row];
+      const float pi = k == 0 ? zi : zi + bf * p_old[row];
+      const float qi = k == 0 ? az : az + bf * q32[row];
+      p_new[row] = pi;
+      q32[row] = qi;
+      v[0] = (double)pi * (double)qi;
     }
   } else if constexpr (SELL) {
     const int n = sa.n;
@@ -481,17 +496,17 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   int rc;
   if (PREC == OFH_DIAG_PREC && (rc = csr_ensure_dinv(A))) return rc;
   if ((rc = csr_ensure_val32(A))) return rc;
-  // L2-resident systems use two kernels per inner iteration (k_mx_upd,
-  // k_mx_dirspmv: from the SELL-64 copy with q by recurrence, one gather;
-  // else the direction formed at the gathers), larger ones three (k_mx_spmv,
-  // k_mx_upd, k_mx_dir): with 256 rows per product workgroup, every one of
-  // them would fold the update's partials.
+  // Two kernels per inner iteration (k_mx_upd, k_mx_dirspmv) when the
+  // product reads the SELL-64 copy (q by recurrence: one gather, the
+  // update's partials loading during the product) or the system is
+  // L2-resident (direction formed at the gathers); otherwise three
+  // (k_mx_spmv, k_mx_upd, k_mx_dir).
   constexpr int kFuseRows = 3 << 17;
-  const bool fused = n <= kFuseRows;
   bool fsell = false;
   const int nbsf = (n + kBlock - 1) / kBlock;
-  if (fused && nbsf <= kMaxPartials && (rc = csr_ensure_sell(A, true, &fsell))) return rc;
+  if (nbsf <= kMaxPartials && (rc = csr_ensure_sell(A, true, &fsell))) return rc;
   const bool qrec = fsell;
+  const bool fused = n <= kFuseRows || qrec;
   void* wsp;
   if ((rc = ensure_vws(ctx, sizeof(float) * (size_t)n * 6, &wsp))) return rc;
   float* r32 = (float*)wsp;
