@@ -382,7 +382,8 @@ def test_gmres_matches_oracle(ctx, prec):
     assert rel(host(x), xo) <= 1e-8
 
 
-def test_mixed_cg_fp64_verified(ctx):
+@pytest.mark.parametrize("prec", ["DIAG_PREC", "IDENT_PREC"])
+def test_mixed_cg_fp64_verified(ctx, prec):
     """Mixed precision: stops on the true FP64 residual; error vs the FP64
     oracle solution of the same tolerance stays at the solver-accuracy level."""
     import ofeli_amd as oa
@@ -390,11 +391,31 @@ def test_mixed_cg_fp64_verified(ctx):
     A = gpu_csr(ctx, rp, ci, a)
     ret, xo, _ = O.cg(rp, ci, a, b, max_it=10000, toler=1e-12)
     x = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
-    info = oa.cg_mixed(A, dev(b), x, max_it=10000, toler=1e-8)
+    info = oa.cg_mixed(A, dev(b), x, prec=getattr(oa, prec), max_it=10000, toler=1e-8)
     assert info.nb_it > 0 and info.fallback == 0
     assert info.true_relres <= 1e-8
     # ||x - x*|| / ||x*|| <= cond-bounded: generous 1e-5 on this small system
     assert rel(host(x), xo) <= 1e-5
+
+
+def test_mixed_cg_many_partials(ctx):
+    """Mixed CG on 2,000,000 rows: more update partials than the product
+    kernel prefetches (it folds them after the product instead); the solve
+    still ends on the true FP64 residual with the known solution."""
+    import scipy.sparse as sp
+    import ofeli_amd as oa
+    n = 2_000_000
+    d = 4.0 + (np.arange(n) % 7) * 0.25
+    M = sp.diags([-np.ones(n - 1), d, -np.ones(n - 1)], [-1, 0, 1], format="csr")
+    M.sort_indices()
+    rp, ci, a = M.indptr.astype(np.int64), M.indices.astype(np.int64) + 1, M.data
+    A = gpu_csr(ctx, rp, ci, a)
+    b = M @ np.ones(n)
+    x = _torch().zeros(n, dtype=_torch().float64, device="cuda")
+    info = oa.cg_mixed(A, dev(b), x, max_it=1000, toler=1e-10)
+    assert info.nb_it > 0 and info.fallback == 0
+    assert info.true_relres <= 1e-10
+    assert rel(host(x), np.ones(n)) < 1e-9
 
 
 def test_zero_pivot_error(ctx):
