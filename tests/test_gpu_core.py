@@ -69,6 +69,11 @@ def test_spmv_sell_bitexact_and_value_changes(ctx, shape):
     torch = _torch()
     rp, ci, a = grid_system(shape, 31)
     A = gpu_csr(ctx, rp, ci, a)
+    lens = np.diff(rp)
+    pad = np.zeros(-(-A.n // 64) * 64, np.int64)
+    pad[:A.n] = lens
+    stored = int(pad.reshape(-1, 64).max(axis=1).sum() * 64)
+    assert A.sell_info() == {"in_use": True, "stored": stored, "col_bytes": 2}
     rng = np.random.default_rng(32)
     x = rng.standard_normal(A.n)
     y = torch.zeros(A.n, dtype=torch.float64, device="cuda")
@@ -128,6 +133,7 @@ def test_sell_with_32bit_columns_only(ctx):
     torch = _torch()
     rp, ci, a = far_coupled_system()
     A = gpu_csr(ctx, rp, ci, a)
+    assert A.sell_info() == {"in_use": True, "stored": 130112, "col_bytes": 4}
     rng = np.random.default_rng(41)
     x = rng.standard_normal(A.n)
     y = torch.zeros(A.n, dtype=torch.float64, device="cuda")
