@@ -222,7 +222,9 @@ int ofh_batch_set_config(ofh_batch B, int block_threads, int threads_per_row);
 int ofh_batch_get_config(ofh_batch B, int* block_threads, int* threads_per_row);
 /* Times every launch shape on this batch (x is restored; the resident form
  * only when every system fits) and keeps the fastest for later ofh_batch_cg
- * calls. */
+ * calls.  The choice is cached for the process, keyed by prec, mixed and
+ * every system's rows and nonzeros: a later batch of the same shape takes
+ * it without timing (*best_ms then reports the cached time). */
 int ofh_batch_autotune(ofh_batch B, int prec, int mixed, const double* d_b, double* d_x,
                        int max_it, double toler, int* best_block, int* best_tpr,
                        double* best_ms);
@@ -255,8 +257,15 @@ int ofh_mesh_read(const char* path, ofh_mesh* out);
 int ofh_mesh_info(ofh_mesh m, int* dim, int* nb_dof, int* nb_nodes, int* nb_elements,
                   int* el_shape, int* nen, int* nb_sides, int* sd_shape, int* nsn,
                   int* nb_materials);
-int ofh_mesh_get(ofh_mesh m, double* h_coord, int32_t* h_code, int32_t* h_elem,
-                 int32_t* h_elem_code, int32_t* h_side, int32_t* h_side_code);
+/* Each non-NULL buffer comes with its capacity in elements; a buffer shorter
+ * than its array (sizes from ofh_mesh_info) fails with OFH_E_INVALID before
+ * anything is copied.  The reader rejects files whose records disagree in
+ * width (a second <Mesh>, a later <Elements>/<Sides> changing shape or
+ * nodes), so the sizes ofh_mesh_info reports are exact. */
+int ofh_mesh_get(ofh_mesh m, double* h_coord, int64_t coord_len, int32_t* h_code,
+                 int64_t code_len, int32_t* h_elem, int64_t elem_len, int32_t* h_elem_code,
+                 int64_t elem_code_len, int32_t* h_side, int64_t side_len,
+                 int32_t* h_side_code, int64_t side_code_len);
 int ofh_mesh_material(ofh_mesh m, int i, int* code, char* name, int name_len);
 int ofh_mesh_destroy(ofh_mesh m);
 /* Mesh::put(file) (Mesh.cpp:2734-2800), byte for byte; the <Material>

@@ -64,4 +64,5 @@ def check(rc):
 from .api import (Context, CsrMatrix, AssemblyPlan, Batch, LearnedGuess, ProjectionGuess,  # noqa: E402
                   concat_meshes, solve_with_projection, cg, bicgstab, gmres, linear_solve,  # noqa: E402
                   cgs, bicg, cg_mixed, spmv, spmv_add, spmv_t, spmv_f32, prec_apply,  # noqa: E402
-                  prec_trans_apply, ssor_solve, dot, norm2, MeshFile, read_mesh, write_mesh)
+                  prec_trans_apply, ssor_solve, dot, norm2, axpy, MeshFile, read_mesh,  # noqa: E402
+                  write_mesh)

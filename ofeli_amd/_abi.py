@@ -63,7 +63,7 @@ SIGNATURES = {
     "ofh_mesh3d_kuhn": (i32, [f64] * 6 + [i32] * 9 + [f64p, i32p, i32p]),
     "ofh_mesh_read": (i32, [C.c_char_p, C.POINTER(vp)]),
     "ofh_mesh_info": (i32, [vp] + [C.POINTER(i32)] * 10),
-    "ofh_mesh_get": (i32, [vp, f64p, i32p, i32p, i32p, i32p, i32p]),
+    "ofh_mesh_get": (i32, [vp, f64p, i64, i32p, i64, i32p, i64, i32p, i64, i32p, i64, i32p, i64]),
     "ofh_mesh_material": (i32, [vp, i32, C.POINTER(i32), C.c_char_p, i32]),
     "ofh_mesh_destroy": (i32, [vp]),
     "ofh_mesh_write": (i32, [C.c_char_p, i32, i32, i32, f64p, i32p, i32, i32, i32, i32p, i32p,

@@ -188,6 +188,11 @@ def norm2(ctx, x):
     return out.value
 
 
+def axpy(ctx, a, x, y):
+    """y = a x + y (Vect += a*x)."""
+    check(lib().ofh_axpy_f64(ctx.h, y.numel(), float(a), _f64(x, y.numel()), _f64(y)))
+
+
 def prec_apply(A, prec, b, x):
     check(lib().ofh_prec_apply(A.h, prec, _f64(b, A.n), _f64(x, A.n)))
 
@@ -293,9 +298,12 @@ def read_mesh(path):
         sides = np.zeros((ns, max(nsn, 1)), np.int32)
         side_code = np.zeros((ns, nb_dof), np.int32)
         ptr = lambda a, t: a.ctypes.data_as(C.POINTER(t))
-        check(lib().ofh_mesh_get(h, ptr(coord, C.c_double), ptr(code, C.c_int32),
-                                 ptr(elem, C.c_int32), ptr(elem_code, C.c_int32),
-                                 ptr(sides, C.c_int32), ptr(side_code, C.c_int32)))
+        check(lib().ofh_mesh_get(h, ptr(coord, C.c_double), coord.size,
+                                 ptr(code, C.c_int32), code.size,
+                                 ptr(elem, C.c_int32), elem.size,
+                                 ptr(elem_code, C.c_int32), elem_code.size,
+                                 ptr(sides, C.c_int32), sides.size,
+                                 ptr(side_code, C.c_int32), side_code.size))
         mats = []
         for i in range(nm):
             c, name = C.c_int(), C.create_string_buffer(256)

@@ -44,6 +44,8 @@ struct ofh_asm_s {
   int* err = nullptr;          // device error slots
   ofh_csr A = nullptr;
   std::vector<int> h_dof;
+  int* trac = nullptr;         // side tractions: side nodes + per-DOF contribution lists
+  size_t trac_cap = 0;         // ints held by trac (grown on demand)
 };
 
 namespace ofh {
@@ -235,6 +237,24 @@ __global__ __launch_bounds__(256) void k_element(int nb_el, ElemArgs a) {
             for (int dj = 0; dj < 2; ++dj)
               out[(2 * i + di) * 8 + 2 * j + dj] = q4_entry(i, di, j, dj, dx, dy, w, G, lambda, mode);
     }
+    if (a.f) {  // body force (fx, fy per node) at the Gauss points, before updateBC
+      double sh[4][4];
+      quad_values(a.g0, a.g1, sh);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        double fx = 0., fy = 0.;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          fx += sh[j][k] * a.f[nd[j] * 2];
+          fy += sh[j][k] * a.f[nd[j] * 2 + 1];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          erhs[2 * i] += w[k] * fx * sh[i][k];
+          erhs[2 * i + 1] += w[k] * fy * sh[i][k];
+        }
+      }
+    }
     if (a.bc) {  // updateBC, Equation_impl.h:160-176
       for (int in = 0; in < 8; ++in)
         for (int j = 0; j < 4; ++j)
@@ -363,26 +383,35 @@ __global__ void k_gather_rhs(int nb_eq, int nb_dof, int nee, const int* __restri
   b[r] = s;
 }
 
-// Elas2DQ4 side tractions, one thread, side order (Equa_Solid.h:194-202)
-__global__ void k_traction(int nb_sides, const int* __restrict__ sides,
-                           const double* __restrict__ coord, const double* __restrict__ tr,
-                           const int* __restrict__ dof, double* b, int* err) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  for (int s = 0; s < nb_sides; ++s) {
+// Elas2DQ4 side tractions (Equa_Solid.h:194-202, Elas2DQ4::BoundaryRHS
+// Elas2DQ4.cpp:256-263).  The reference adds side by side into b; here one
+// thread per touched equation adds that equation's contributions in side
+// order (the host lists them, stably sorted by equation), which is the same
+// sequence of additions for every entry of b.
+//   sides: nb_sides x 2 1-based nodes; lst_ptr/lst_dof: touched equations
+//   (0-based) and their contribution ranges; lst_sk: side * 2 + component.
+__global__ void k_traction(int nlist, const int* __restrict__ sides,
+                           const int* __restrict__ lst_ptr, const int* __restrict__ lst_dof,
+                           const int* __restrict__ lst_sk, const double* __restrict__ coord,
+                           const double* __restrict__ tr, double* __restrict__ b, int* err) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlist) return;
+  const int d = lst_dof[t];
+  double acc = b[d];
+  for (int q = lst_ptr[t]; q < lst_ptr[t + 1]; ++q) {
+    const int s = lst_sk[q] >> 1, k = lst_sk[q] & 1;
     const int n0 = sides[2 * s] - 1, n1 = sides[2 * s + 1] - 1;
     const double dlx = 0.5 * (coord[2 * n1] - coord[2 * n0]);
     const double dly = 0.5 * (coord[2 * n1 + 1] - coord[2 * n0 + 1]);
     const double det = sqrt(dlx * dlx + dly * dly + 0.0 * 0.0);  // Line2.cpp:85-86
-    if (det == 0.0) { err[2] = min(err[2], s); continue; }
+    if (det == 0.0) {  // the side is skipped and reported
+      atomicMin(&err[2], s);
+      continue;
+    }
     const double c = 0.5 * (2 * det);
-    const int nd[2] = {n0, n1};
-    for (int i = 0; i < 2; ++i)
-      for (int k = 0; k < 2; ++k) {
-        const double v = 0.0 + c * tr[2 * s + k];
-        const int d = dof[nd[i] * 2 + k];
-        if (d) b[d - 1] += v;
-      }
+    acc += 0.0 + c * tr[2 * s + k];
   }
+  b[d] = acc;
 }
 
 // Vect::insertBC: u_i = v_j for free DOFs (in order), bc_i otherwise
@@ -517,6 +546,7 @@ int ofh_asm_create(ofh_ctx ctx, int dim, int nb_nodes, const double* h_coord, in
   std::vector<int> rp32(rptr.begin(), rptr.end());
   ofh_csr A = new ofh_csr_s();
   A->ctx = ctx;
+  A->serial = next_csr_serial();
   A->n = neq;
   A->nnz = nnz;
   OFH_HIP(hipMalloc(&A->row_ptr, sizeof(int) * (neq + 1)));
@@ -567,7 +597,8 @@ int ofh_asm_destroy(ofh_asm P) {
   if (!P) return OFH_OK;
   hipDeviceSynchronize();
   void* ptrs[] = {P->coord, P->elem, P->dof, P->code, P->adj_ptr, P->adj_el, P->adj_loc,
-                  P->row_node, P->row_comp, P->map_ptr, P->map, P->ke, P->erhs, P->err};
+                  P->row_node, P->row_comp, P->map_ptr, P->map, P->ke, P->erhs, P->err,
+                  P->trac};
   for (void* p : ptrs) hipFree(p);
   ofh_csr_destroy(P->A);
   delete P;
@@ -604,8 +635,6 @@ int ofh_asm_build(ofh_asm P, int kind, const double* d_coef, const double* d_f,
             OFH_E_INVALID, "ofh_asm_build: element kind does not match the mesh");
   OFH_CHECK(kind == OFH_LAPLACE2DT3 || kind == OFH_LAPLACE3DT4 || d_coef, OFH_E_INVALID,
             "ofh_asm_build: this kind needs a per-element coefficient");
-  OFH_CHECK(kind != OFH_ELAS2DQ4 || d_f == nullptr, OFH_E_UNSUPPORTED,
-            "ofh_asm_build: Elas2DQ4 body forces are not provided (use tractions)");
   OFH_CHECK(kind != OFH_DC2DT3_BE || (h_prm && nprm >= 3), OFH_E_INVALID,
             "ofh_asm_build: DC2DT3_BE needs {dt, rho, cp}");
   OFH_CHECK(kind != OFH_ELAS2DQ4 || (h_prm && nprm >= 2), OFH_E_INVALID,
@@ -675,17 +704,61 @@ int ofh_asm_traction(ofh_asm P, int nb_sides, const int32_t* h_side_nodes,
               "ofh_asm_traction: side node out of range");
   if (nb_sides == 0) return OFH_OK;
   hipStream_t s = P->ctx->stream;
-  int* ds;
-  OFH_HIP(hipMalloc(&ds, sizeof(int) * 2 * nb_sides));
-  OFH_HIP(hipMemcpyAsync(ds, h_side_nodes, sizeof(int) * 2 * nb_sides, hipMemcpyHostToDevice, s));
+  // contributions (equation, side, component) in the reference's order of
+  // additions, stably grouped by equation: cnt[d] per equation, then the
+  // device buffer [sides | side*2+k per contribution | ranges | equations]
+  std::vector<int> cnt(P->nb_eq, 0);
+  for (int sd = 0; sd < nb_sides; ++sd)
+    for (int i = 0; i < 2; ++i)
+      for (int k = 0; k < 2; ++k) {
+        const int d = P->h_dof[(h_side_nodes[2 * sd + i] - 1) * 2 + k];
+        if (d) ++cnt[d - 1];
+      }
+  std::vector<int> start(P->nb_eq + 1, 0);
+  for (int d = 0; d < P->nb_eq; ++d) start[d + 1] = start[d] + cnt[d];
+  const int ncontrib = start[P->nb_eq];
+  std::vector<int> sk(ncontrib);
+  std::vector<int> cursor(start.begin(), start.end() - 1);
+  for (int sd = 0; sd < nb_sides; ++sd)
+    for (int i = 0; i < 2; ++i)
+      for (int k = 0; k < 2; ++k) {
+        const int d = P->h_dof[(h_side_nodes[2 * sd + i] - 1) * 2 + k];
+        if (d) sk[cursor[d - 1]++] = sd * 2 + k;
+      }
+  std::vector<int> ptr(1, 0), eqs;
+  for (int d = 0; d < P->nb_eq; ++d)
+    if (cnt[d]) {
+      eqs.push_back(d);
+      ptr.push_back(start[d + 1]);
+    }
+  const int nlist = (int)eqs.size();
+  std::vector<int> host(h_side_nodes, h_side_nodes + 2 * nb_sides);
+  host.insert(host.end(), sk.begin(), sk.end());
+  host.insert(host.end(), ptr.begin(), ptr.end());
+  host.insert(host.end(), eqs.begin(), eqs.end());
+  const size_t need = host.size();
+  if (P->trac_cap < need) {  // grown on demand, kept by the plan
+    OFH_HIP(hipStreamSynchronize(s));
+    if (P->trac) OFH_HIP(hipFree(P->trac));
+    P->trac = nullptr;
+    P->trac_cap = 0;
+    OFH_HIP(hipMalloc(&P->trac, sizeof(int) * need));
+    P->trac_cap = need;
+  }
+  OFH_HIP(hipMemcpyAsync(P->trac, host.data(), sizeof(int) * need, hipMemcpyHostToDevice, s));
+  OFH_HIP(hipStreamSynchronize(s));  // `host` goes out of scope on every return path
   int rc = reset_err(P);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_traction, dim3(1), dim3(64), 0, s, nb_sides, ds, P->coord, d_traction,
-                     P->dof, d_b, P->err);
-  OFH_HIP(hipGetLastError());
-  rc = check_err(P, "Line2");
-  hipFree(ds);
-  return rc;
+  if (nlist > 0) {
+    const int* d_sides = P->trac;
+    const int* d_sk = d_sides + 2 * nb_sides;
+    const int* d_ptr = d_sk + ncontrib;
+    const int* d_eq = d_ptr + ptr.size();
+    hipLaunchKernelGGL(k_traction, dim3((nlist + 255) / 256), dim3(256), 0, s, nlist, d_sides,
+                       d_ptr, d_eq, d_sk, P->coord, d_traction, d_b, P->err);
+    OFH_HIP(hipGetLastError());
+  }
+  return check_err(P, "Line2");
 }
 
 int ofh_asm_insert_bc(ofh_asm P, const double* d_v, const double* d_bc, double* d_u) {

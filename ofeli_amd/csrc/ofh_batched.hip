@@ -22,6 +22,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <numeric>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -76,11 +78,34 @@ struct ofh_batch_s {
   hipStream_t cap = nullptr;     // capture-only stream
   BatchGraphKey gkey{};
   std::vector<int> h_rows;
+  std::vector<int> h_nnz;  // nonzeros of each system (autotuner cache key)
 };
 
 namespace ofh {
 
 constexpr double kBatchDelta = 0.1;
+
+// Autotuner choices of this process, keyed by what the timing depends on:
+// the solver variant, the matrix size and every system's rows and nonzeros.
+struct TuneEntry {
+  int block, tpr;
+  double ms;
+};
+static std::mutex g_tune_mu;
+static std::map<std::string, TuneEntry> g_tune_cache;
+
+static std::string tune_key(const ofh_batch_s* B, int prec, int mixed) {
+  std::string k;
+  auto put = [&k](int64_t v) { k.append(reinterpret_cast<const char*>(&v), sizeof(v)); };
+  put(prec);
+  put(mixed);
+  put(B->A->n);
+  put(B->A->nnz);
+  put(B->nsys);
+  for (int v : B->h_rows) put(v);
+  for (int v : B->h_nnz) put(v);
+  return k;
+}
 
 template <int BLOCK>
 __device__ __forceinline__ double bsum(double v, double* red) {
@@ -479,6 +504,7 @@ __global__ __launch_bounds__(kResBlock) void k_batch_cg64_res(BatchArgs A) {
   __shared__ double p[kResPad + 1];
   __shared__ double red[2 * kResRed];
   __shared__ int s_w;
+  OFH_LDS_BUDGET(sizeof(double) * (kResPad + 1) + sizeof(double) * 2 * kResRed + sizeof(int));
   const int t = threadIdx.x;
   int parity = 0;
   if (t == 0) p[kResPad] = 0.0;
@@ -578,6 +604,7 @@ __global__ __launch_bounds__(kResBlock) void k_batch_cgmx_res(BatchArgs A) {
   __shared__ float p2[2 * (kResPad + 1)];
   __shared__ double red[2 * kResRed];
   __shared__ int s_w;
+  OFH_LDS_BUDGET(sizeof(float) * 2 * (kResPad + 1) + sizeof(double) * 2 * kResRed + sizeof(int));
   const int t = threadIdx.x;
   int parity = 0;
   if (t == 0) p2[2 * kResPad] = 0.0f;
@@ -920,6 +947,8 @@ int ofh_batch_create(ofh_csr A, int nsys, const int32_t* h_sys_rows, ofh_batch* 
   B->A = A;
   B->nsys = nsys;
   B->h_rows.assign(h_sys_rows, h_sys_rows + nsys + 1);
+  B->h_nnz.resize(nsys);
+  for (int k = 0; k < nsys; ++k) B->h_nnz[k] = rp[h_sys_rows[k + 1]] - rp[h_sys_rows[k]];
   for (int k = 0; k < nsys; ++k)
     B->max_rows = std::max(B->max_rows, h_sys_rows[k + 1] - h_sys_rows[k]);
   // ELL layout of the resident form (built on its first use, ell_build):
@@ -1040,12 +1069,36 @@ int ofh_batch_autotune(ofh_batch B, int prec, int mixed, const double* d_b, doub
   ofh_csr M = B->A;
   hipStream_t s = M->ctx->stream;
   const int n = M->n;
+  // a batch of the same shape (system sizes, nonzeros, solver variant) was
+  // tuned before in this process: reuse its choice without timing again
+  const std::string key = tune_key(B, prec, mixed);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      B->block = it->second.block;
+      B->tpr = it->second.tpr;
+      B->tuned = true;
+      if (best_block) *best_block = B->block;
+      if (best_tpr) *best_tpr = B->tpr;
+      if (best_ms) *best_ms = it->second.ms;
+      return OFH_OK;
+    }
+  }
   double* x0;
   OFH_HIP(hipMalloc(&x0, sizeof(double) * n));
   OFH_HIP(hipMemcpyAsync(x0, d_x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-  hipEvent_t e0, e1;
-  OFH_HIP(hipEventCreate(&e0));
-  OFH_HIP(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  // every return path below releases x0 and both events
+  auto release = [&]() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(x0);
+  };
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    release();
+    return fail(OFH_E_HIP, "ofh_batch_autotune: hipEventCreate failed");
+  }
   // candidate shapes: 3 block sizes x 4 threads per row, then the resident form
   // when every system fits on chip
   std::vector<std::pair<int, int>> shapes;
@@ -1058,14 +1111,26 @@ int ofh_batch_autotune(ofh_batch B, int prec, int mixed, const double* d_b, doub
     {
       float ms_min = 1e30f;
       for (int rep = 0; rep < 2; ++rep) {  // first run warms caches and code
-        OFH_HIP(hipMemcpyAsync(d_x, x0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        OFH_HIP(hipEventRecord(e0, s));
-        int rc = batch_run(B, prec, mixed, d_b, d_x, max_it, toler, sh.first, sh.second, s, false);
-        if (rc) { hipFree(x0); return rc; }
-        OFH_HIP(hipEventRecord(e1, s));
-        OFH_HIP(hipEventSynchronize(e1));
         float ms = 0;
-        OFH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        hipError_t he = hipMemcpyAsync(d_x, x0, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+        if (he == hipSuccess) he = hipEventRecord(e0, s);
+        if (he != hipSuccess) {
+          release();
+          return fail(OFH_E_HIP, std::string("ofh_batch_autotune: ") + hipGetErrorString(he));
+        }
+        const int rc =
+            batch_run(B, prec, mixed, d_b, d_x, max_it, toler, sh.first, sh.second, s, false);
+        if (rc) {
+          release();
+          return rc;
+        }
+        he = hipEventRecord(e1, s);
+        if (he == hipSuccess) he = hipEventSynchronize(e1);
+        if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+        if (he != hipSuccess) {
+          release();
+          return fail(OFH_E_HIP, std::string("ofh_batch_autotune: ") + hipGetErrorString(he));
+        }
         ms_min = std::min(ms_min, ms);
       }
       if (ms_min < best) {
@@ -1075,14 +1140,18 @@ int ofh_batch_autotune(ofh_batch B, int prec, int mixed, const double* d_b, doub
       }
     }
   }
-  OFH_HIP(hipMemcpyAsync(d_x, x0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-  OFH_HIP(hipStreamSynchronize(s));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipFree(x0);
+  hipError_t he = hipMemcpyAsync(d_x, x0, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  release();
+  if (he != hipSuccess)
+    return fail(OFH_E_HIP, std::string("ofh_batch_autotune: ") + hipGetErrorString(he));
   B->block = bb;
   B->tpr = bt;
   B->tuned = true;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = TuneEntry{bb, bt, (double)best};
+  }
   if (best_block) *best_block = bb;
   if (best_tpr) *best_tpr = bt;
   if (best_ms) *best_ms = best;

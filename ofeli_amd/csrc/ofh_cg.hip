@@ -574,7 +574,7 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
     hipGraphExec_t chunk_graph = nullptr;
     if (ctx->use_graphs) {
       GraphKey key;
-      csr_graph_key(A, 16 * 1 + PREC, x, wsp, A->val, A->dinv, &key);
+      csr_graph_key(A, 16 * 1 + PREC, (uintptr_t)sell, x, wsp, A->val, A->dinv, &key);
       rc = ctx_graph(ctx, key, [&](hipStream_t cs) {
         for (int c = 0; c < kGraphIters; ++c) iteration(-1, cs);
         OFH_HIP(hipGetLastError());
@@ -646,7 +646,7 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   hipGraphExec_t chunk_graph = nullptr;
   if (ctx->use_graphs && !sweep_prec(PREC)) {
     GraphKey key;
-    csr_graph_key(A, 16 * 2 + PREC, x, wsp, A->val, A->dinv, &key);
+    csr_graph_key(A, 16 * 2 + PREC, (uintptr_t)sell, x, wsp, A->val, A->dinv, &key);
     rc = ctx_graph(ctx, key, [&](hipStream_t cs) {
       for (int c = 0; c < kGraphIters; ++c) {
         const int e = iteration(-1, cs);

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -122,7 +123,8 @@ int reduce_grid(int n) {
 
 int launch_dot(ofh_ctx ctx, int n, const double* x, const double* y, double* d_out,
                bool sqrt_out) {
-  int g = reduce_grid(n);
+  // k_dot strides over the grid, so any n fits the kMaxPartials slots of slot 0
+  const int g = std::min(reduce_grid(n), kMaxPartials);
   hipLaunchKernelGGL(k_dot, dim3(g), dim3(kBlock), 0, ctx->stream, n, x, y, ctx->ws.partials);
   hipLaunchKernelGGL(k_fold_one, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->ws.partials, g,
                      d_out, (int)sqrt_out);
@@ -190,6 +192,11 @@ int ensure_vws(ofh_ctx ctx, size_t bytes, void** out) {
   }
   *out = ctx->vws;
   return OFH_OK;
+}
+
+uint64_t next_csr_serial() {
+  static std::atomic<uint64_t> next{1};
+  return next++;
 }
 
 double now_seconds() {
@@ -391,6 +398,7 @@ int ofh_csr_create(ofh_ctx c, int n, int64_t nnz, const int32_t* rp, const int32
   }
   ofh_csr A = new ofh_csr_s();
   A->ctx = c;
+  A->serial = next_csr_serial();
   A->n = n;
   A->nnz = nnz;
   A->max_row_nnz = maxr;
@@ -503,7 +511,8 @@ int ofh_csr_values_device(ofh_csr A, double** d) {
 
 // ---------------------------------------------------------------- BLAS-1
 int ofh_dot_f64(ofh_ctx c, int n, const double* x, const double* y, double* h_out) {
-  OFH_CHECK(c && x && y && h_out && n >= 0, OFH_E_INVALID, "ofh_dot_f64: bad argument");
+  OFH_CHECK(c && h_out && n >= 0 && (n == 0 || (x && y)), OFH_E_INVALID,
+            "ofh_dot_f64: bad argument");
   double* d_out = c->dev_scratch + 8;
   int rc = launch_dot(c, n, x, y, d_out, false);
   if (rc) return rc;
@@ -513,7 +522,7 @@ int ofh_dot_f64(ofh_ctx c, int n, const double* x, const double* y, double* h_ou
 }
 
 int ofh_norm2_f64(ofh_ctx c, int n, const double* x, double* h_out) {
-  OFH_CHECK(c && x && h_out && n >= 0, OFH_E_INVALID, "ofh_norm2_f64: bad argument");
+  OFH_CHECK(c && h_out && n >= 0 && (n == 0 || x), OFH_E_INVALID, "ofh_norm2_f64: bad argument");
   double* d_out = c->dev_scratch + 8;
   int rc = launch_dot(c, n, x, x, d_out, true);
   if (rc) return rc;
@@ -523,7 +532,7 @@ int ofh_norm2_f64(ofh_ctx c, int n, const double* x, double* h_out) {
 }
 
 int ofh_axpy_f64(ofh_ctx c, int n, double a, const double* x, double* y) {
-  OFH_CHECK(c && x && y && n >= 0, OFH_E_INVALID, "ofh_axpy_f64: bad argument");
+  OFH_CHECK(c && n >= 0 && (n == 0 || (x && y)), OFH_E_INVALID, "ofh_axpy_f64: bad argument");
   if (n == 0) return OFH_OK;
   hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, a, x, y);
   OFH_HIP(hipGetLastError());

@@ -111,6 +111,23 @@ __device__ __forceinline__ int quad_shape(const double (&x)[8], double g0, doubl
   return 0;
 }
 
+// Q4 shape function values at the 2x2 Gauss points, sh[node][gp] with
+// gp = 2*i + j for (s, t) = (g_i, g_j) (Quad4::setLocal, Quad4.cpp:127-130).
+__device__ __forceinline__ void quad_values(double g0, double g1, double (&sh)[4][4]) {
+  const double g[2] = {g0, g1};
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int gp = 2 * i + j;
+      const double s = g[i], t = g[j];
+      sh[0][gp] = 0.25 * (1. - s) * (1. - t);
+      sh[1][gp] = 0.25 * (1. + s) * (1. - t);
+      sh[2][gp] = 0.25 * (1. + s) * (1. + t);
+      sh[3][gp] = 0.25 * (1. - s) * (1. + t);
+    }
+}
+
 // One entry (row 2i+di, column 2j+dj) of the Elas2DQ4 element matrix.
 // eA0 accumulates the Deviator terms over the Gauss points, then (mode 0,
 // as the reference) keeps accumulating the Dilatation terms on top, and eMat

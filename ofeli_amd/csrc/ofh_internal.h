@@ -56,6 +56,16 @@ constexpr int kMaxRows = kMaxPartials * kBlock * kVec;  // largest n the reducti
 // CSR value/column arrays carry kPad zeroed trailing elements, so that
 // kernels may load nonzeros in aligned pairs or vectors past an odd end.
 constexpr int kPad = 2;
+// LDS a gfx950 workgroup may allocate (160 KiB).  A kernel whose static LDS
+// exceeds it does not launch (the queue reports an invalid allocation), so
+// every kernel whose __shared__ arrays are sized by constants or template
+// parameters checks its total at compile time with OFH_LDS_BUDGET.
+constexpr size_t kLdsBytes = 160 * 1024;
+#define OFH_LDS_BUDGET(bytes) \
+  static_assert((size_t)(bytes) <= ::ofh::kLdsBytes, "static LDS exceeds the 160 KiB per workgroup")
+// the CSR-stream kernels: one product tile plus at most four reduction rows
+OFH_LDS_BUDGET(sizeof(double) * kTile64 + 4 * sizeof(double) * kBlock / kWave);
+OFH_LDS_BUDGET(sizeof(float) * kTile32 + 4 * sizeof(double) * kBlock / kWave);
 
 // Workgroups are dispatched round-robin over the XCDs; remap the hardware
 // block id so that each XCD walks one contiguous range of row blocks, which
@@ -193,9 +203,10 @@ struct Workspace {
 // Identity of a captured launch sequence: which sequence and every pointer
 // and size it was recorded with.
 struct GraphKey {
-  uintptr_t v[12] = {};
+  static constexpr int kN = 16;
+  uintptr_t v[kN] = {};
   bool operator==(const GraphKey& o) const {
-    for (int i = 0; i < 12; ++i)
+    for (int i = 0; i < kN; ++i)
       if (v[i] != o.v[i]) return false;
     return true;
   }
@@ -226,6 +237,7 @@ struct ofh_ctx_s {
 
 struct ofh_csr_s {
   ofh_ctx ctx = nullptr;
+  uint64_t serial = 0;      // unique per matrix of the process (captured-graph keys)
   int n = 0;
   int64_t nnz = 0;
   int* row_ptr = nullptr;   // device, n+1
@@ -276,16 +288,20 @@ int ensure_vws(ofh_ctx ctx, size_t bytes, void** out);
 // only) for `key`, captured on first use and cached in the context.
 int ctx_graph(ofh_ctx ctx, const GraphKey& key, const std::function<int(hipStream_t)>& record,
               hipGraphExec_t* out);
-// Key of a captured solver chunk over A: the sequence id, the solution and
-// work buffers, the value and pivot arrays it reads and every CSR array.
-inline void csr_graph_key(ofh_csr A, uintptr_t kind, const void* x, const void* work,
-                          const void* val, const void* dinv, GraphKey* key) {
-  const uintptr_t v[12] = {kind, (uintptr_t)A, (uintptr_t)x, (uintptr_t)work,
-                           ((uintptr_t)(uint32_t)A->n << 32) | (uint32_t)A->nblocks,
-                           (uintptr_t)val, (uintptr_t)dinv, (uintptr_t)A->col16,
-                           (uintptr_t)A->col, (uintptr_t)A->row_blocks, (uintptr_t)A->row_ptr,
-                           (uintptr_t)kGraphIters};
-  for (int i = 0; i < 12; ++i) key->v[i] = v[i];
+// Key of a captured solver chunk over A: the sequence id and kernel variant,
+// the matrix's process-unique serial (a matrix created later at the same
+// address never matches), the solution and work buffers, the value and
+// pivot arrays it reads, every CSR array and the SELL-64 copy's arrays.
+inline void csr_graph_key(ofh_csr A, uintptr_t kind, uintptr_t variant, const void* x,
+                          const void* work, const void* val, const void* dinv, GraphKey* key) {
+  const uintptr_t v[GraphKey::kN] = {
+      kind, variant, (uintptr_t)A->serial, (uintptr_t)x, (uintptr_t)work,
+      ((uintptr_t)(uint32_t)A->n << 32) | (uint32_t)A->nblocks, (uintptr_t)val,
+      (uintptr_t)dinv, (uintptr_t)A->col16, (uintptr_t)A->col, (uintptr_t)A->row_blocks,
+      (uintptr_t)A->row_ptr, (uintptr_t)A->sell_cptr, (uintptr_t)A->sell_col16,
+      (uintptr_t)A->sell_col ^ ((uintptr_t)A->sell_val << 1) ^ (uintptr_t)A->sell_val32,
+      (uintptr_t)kGraphIters};
+  for (int i = 0; i < GraphKey::kN; ++i) key->v[i] = v[i];
 }
 int reduce_grid(int n);
 int launch_dot(ofh_ctx ctx, int n, const double* x, const double* y, double* d_out, bool sqrt_out);
@@ -325,4 +341,6 @@ int launch_ssor(ofh_csr A, const double* r, double* z, const int* d_done, hipStr
 // x <- b - A x style true residual ||b-Ax||/||b|| in FP64 (host result).
 int true_relres(ofh_csr A, const double* b, const double* x, double* out);
 double now_seconds();
+// A process-unique id for each new matrix (ofh_csr_s::serial).
+uint64_t next_csr_serial();
 }  // namespace ofh

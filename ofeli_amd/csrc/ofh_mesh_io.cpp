@@ -227,6 +227,9 @@ bool parse(const std::string& text, ofh_mesh_s& m, std::string& err) {
       continue;
     }
     if (name == "Mesh") {
+      // One mesh per file: a second <Mesh> could change dim or nb_dof under
+      // records already read with the first one's widths.
+      if (seen_mesh) return (err = "mesh file: more than one <Mesh> element", false);
       in_mesh = seen_mesh = true;
       for (auto& a : attributes(ne, gt)) {
         if (a.first == "dim") m.dim = std::atoi(a.second.c_str());
@@ -248,17 +251,24 @@ bool parse(const std::string& text, ofh_mesh_s& m, std::string& err) {
     else if (name == "Sides") s = kSides;
     else if (name == "Material") s = kMaterial;
     if (s == kElements || s == kSides) {
+      int& shape = s == kElements ? m.el_shape : m.sd_shape;
+      int& nodes = s == kElements ? m.nen : m.nsn;
+      const bool have = !(s == kElements ? m.elem_code : m.side_code).empty();
+      const int shape0 = shape, nodes0 = nodes;
       for (auto& a : attributes(ne, gt)) {
         if (a.first == "shape") {
           const int sh = shape_of(a.second);
           if (sh < 0) return (err = "mesh file: unknown shape " + a.second, false);
-          (s == kElements ? m.el_shape : m.sd_shape) = sh;
+          shape = sh;
         } else if (a.first == "nodes") {
-          (s == kElements ? m.nen : m.nsn) = std::atoi(a.second.c_str());
+          nodes = std::atoi(a.second.c_str());
         }
       }
-      const int nn = s == kElements ? m.nen : m.nsn;
-      if (nn < 1 || nn > 27) return (err = "mesh file: bad nodes attribute", false);
+      if (nodes < 1 || nodes > 27) return (err = "mesh file: bad nodes attribute", false);
+      // Records of one kind share one width: a later section may repeat the
+      // shape and node count, not change them.
+      if (have && (shape != shape0 || nodes != nodes0))
+        return (err = "mesh file: a later section changes the element or side shape", false);
     }
     if (s != kNone && !self_closing) {
       sec = s;
@@ -266,7 +276,18 @@ bool parse(const std::string& text, ofh_mesh_s& m, std::string& err) {
     }
   }
   if (!seen_mesh) return (err = "mesh file: no <Mesh> element", false);
+  // text after the last tag (a cut file) would otherwise be dropped silently
+  if (in_mesh || sec != kNone) return (err = "mesh file: unterminated <Mesh> or section", false);
   const int64_t nn = (int64_t)m.code.size() / m.nb_dof;
+  // Every array holds whole records of the widths ofh_mesh_info reports.
+  if ((int64_t)m.code.size() != nn * m.nb_dof || (int64_t)m.coord.size() != nn * m.dim ||
+      (int64_t)m.elem.size() != (int64_t)m.elem_code.size() * m.nen ||
+      (int64_t)m.side_code.size() % m.nb_dof != 0 ||
+      (int64_t)m.side.size() != (int64_t)(m.side_code.size() / m.nb_dof) * m.nsn ||
+      m.mat_code.size() != m.mat_name.size())
+    return (err = "mesh file: inconsistent record widths", false);
+  if (nn > INT32_MAX || (int64_t)m.elem_code.size() > INT32_MAX)
+    return (err = "mesh file: too many records", false);
   for (int32_t v : m.elem)
     if (v < 1 || v > nn) return (err = "mesh file: element node label out of range", false);
   for (int32_t v : m.side)
@@ -334,9 +355,19 @@ int ofh_mesh_info(ofh_mesh m, int* dim, int* nb_dof, int* nb_nodes, int* nb_elem
   return OFH_OK;
 }
 
-int ofh_mesh_get(ofh_mesh m, double* coord, int32_t* code, int32_t* elem, int32_t* elem_code,
-                 int32_t* side, int32_t* side_code) {
+int ofh_mesh_get(ofh_mesh m, double* coord, int64_t coord_len, int32_t* code, int64_t code_len,
+                 int32_t* elem, int64_t elem_len, int32_t* elem_code, int64_t elem_code_len,
+                 int32_t* side, int64_t side_len, int32_t* side_code, int64_t side_code_len) {
   if (!m) return OFH_E_INVALID;
+  // every requested buffer must hold its whole array: check all before copying any
+  auto fits = [](const void* dst, int64_t len, size_t need) {
+    return !dst || (len >= 0 && (uint64_t)len >= need);
+  };
+  if (!fits(coord, coord_len, m->coord.size()) || !fits(code, code_len, m->code.size()) ||
+      !fits(elem, elem_len, m->elem.size()) ||
+      !fits(elem_code, elem_code_len, m->elem_code.size()) ||
+      !fits(side, side_len, m->side.size()) || !fits(side_code, side_code_len, m->side_code.size()))
+    return OFH_E_INVALID;
   auto cp = [](auto* dst, const auto& v) {
     if (dst && !v.empty()) std::memcpy(dst, v.data(), sizeof(v[0]) * v.size());
   };

@@ -282,8 +282,8 @@ struct SellF32 {
 // lane), else from the CSR-stream partition.  QREC (SELL only): the product
 // by recurrence, q_new = A z + beta q_old (= A p_new in exact arithmetic, as
 // q_old = A p_old), so the gather reads z alone and p is never gathered; the
-// rounding drift of q against A p is bounded by the FP64 residual
-// replacements, which end every cycle.
+// rounding drift of q against A p lasts one cycle: the launch after each FP64
+// residual replacement (PHASE 2) uses the gathering form, q = A p_new.
 template <int PREC, int PHASE, bool SELL, bool QREC>
 __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* __restrict__ rb,
                                                       const int* __restrict__ rp, ColIndex ci,
@@ -527,9 +527,12 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
                      A->row_blocks32, A->row_ptr, ci, A->val32, k, nparts, z32, pb[0], pb[1], \
                      q32, state_ptr(ctx), ctx->ws.partials, sa)
     if (qrec) {
+      // After a residual replacement (phase 2) the product is formed from
+      // the gathered direction, q = A (z + beta p_old): this resets the drift
+      // of the recurrence-carried q against A p once per cycle.
       if (phase == 0) OFH_MX_DIRSPMV(0, true, true);
       else if (phase == 1) OFH_MX_DIRSPMV(1, true, true);
-      else OFH_MX_DIRSPMV(2, true, true);
+      else OFH_MX_DIRSPMV(2, true, false);
     } else if (fsell) {
       if (phase == 0) OFH_MX_DIRSPMV(0, true, false);
       else if (phase == 1) OFH_MX_DIRSPMV(1, true, false);
@@ -598,7 +601,9 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   hipGraphExec_t chunk_graph = nullptr;
   if (ctx->use_graphs) {
     GraphKey key;
-    csr_graph_key(A, 16 * (fused ? 3 : 4) + PREC, x, wsp, A->val32, A->dinv32, &key);
+    csr_graph_key(A, 16 * (fused ? 3 : 4) + PREC,
+                  (uintptr_t)fsell | (uintptr_t)qrec << 1 | (uintptr_t)sell << 2, x, wsp,
+                  A->val32, A->dinv32, &key);
     rc = ctx_graph(ctx, key, [&](hipStream_t cs) {
       for (int c = 0; c < kGraphIters; ++c) iteration(-1, cs);
       OFH_HIP(hipGetLastError());

@@ -263,6 +263,7 @@ __global__ __launch_bounds__(256) void k_mp_wgrad(WgradArgs a) {
   constexpr int MB = K / 16, JB = O / 16;
   constexpr int W = K * O + (COLSUM ? O : 0);
   __shared__ float red[K * O + O];
+  OFH_LDS_BUDGET(sizeof(float) * (K * O + O));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lg = lane >> 4;
   f32x4 acc[MB][JB];
@@ -653,6 +654,7 @@ __global__ __launch_bounds__(256) void k_mp_enc_bwd(int n_pad, const float* __re
                                                     const float* __restrict__ dH, float* part) {
   constexpr int W = kFin * kH + kH;
   __shared__ float red[W];
+  OFH_LDS_BUDGET(sizeof(float) * W);
   const int c = threadIdx.x & 63, wave = threadIdx.x >> 6;
   float gw[kFin] = {0, 0, 0, 0, 0, 0, 0, 0};
   float gb = 0.f;

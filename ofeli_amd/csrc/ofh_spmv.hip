@@ -179,6 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nblocks, const int* __restr
                                                  const T* __restrict__ x, T* __restrict__ y,
                                                  T alpha) {
   __shared__ T lds[TILE];
+  OFH_LDS_BUDGET(sizeof(T) * TILE);
   const int b = xcd_block(blockIdx.x, nblocks);
   const int r0 = rb[b], r1 = rb[b + 1];
   const int r = r0 + threadIdx.x;

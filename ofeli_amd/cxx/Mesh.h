@@ -93,8 +93,13 @@ class Mesh {
     _elem_code.assign((size_t)ne, 0);
     _sides.assign((size_t)ns * nsn, 0);
     _side_codes.assign((size_t)ns * nb_dof, 0);
-    ofh_mesh_get(h, _coord.data(), _code.data(), _elem.data(), _elem_code.data(), _sides.data(),
-                 _side_codes.data());
+    const int rc = ofh_mesh_get(h, _coord.data(), (int64_t)_coord.size(), _code.data(),
+                                (int64_t)_code.size(), _elem.data(), (int64_t)_elem.size(),
+                                _elem_code.data(), (int64_t)_elem_code.size(), _sides.data(),
+                                (int64_t)_sides.size(), _side_codes.data(),
+                                (int64_t)_side_codes.size());
+    if (rc != OFH_OK) ofh_mesh_destroy(h);
+    hip_check(rc, "Mesh::get");
     _materials.clear();
     for (int i = 0; i < nm; ++i) {
       int c = 0;

@@ -28,11 +28,12 @@ class SpMatrix {
  public:
   SpMatrix() = default;
 
-  // SpMatrix(mesh, dof=0, is_diagonal=0): pattern of NodeGraph on the GPU
+  // SpMatrix(mesh, dof=0, is_diagonal=0) (SpMatrix_impl.h:108-119): the
+  // NodeGraph pattern on the GPU; dof != 0 selects NodeGraphScal's pattern
+  // as setMesh does
   explicit SpMatrix(Mesh& mesh, size_t dof = 0, int is_diagonal = 0) {
-    (void)dof;
     _diag_only = is_diagonal != 0;
-    setMesh(mesh);
+    setMesh(mesh, dof);
   }
 
   // SpMatrix(row_ptr, col_ind) / (row_ptr, col_ind, a): col_ind 1-based
@@ -59,16 +60,21 @@ class SpMatrix {
 
   ~SpMatrix() { release(); }
 
-  // SpMatrix::setMesh (SpMatrix_impl.h:391-413)
-  void setMesh(Mesh& mesh) {
+  // SpMatrix::setMesh (SpMatrix_impl.h:391-413).  dof == 0: NodeGraph over
+  // the mesh's equations (imposed DOFs eliminated), nb_eq rows.  dof != 0:
+  // NodeGraphScal (GraphOfMatrix.cpp:532-546), one row per node numbered as
+  // the node (Matrix_impl.h:148-150), every element node pair coupled, no
+  // elimination: built as NodeGraph of one DOF per node with no imposed DOF.
+  void setMesh(Mesh& mesh, size_t dof = 0) {
     release();
     std::vector<int32_t> rp, col;
     const int dim = (int)mesh.getDim();
     ofh_asm plan = nullptr;
+    const std::vector<int32_t> free_nodes(dof ? mesh.getNbNodes() : 0, 0);
     hip_check(ofh_asm_create(hip_context(), dim, (int)mesh.getNbNodes(), mesh.coords().data(),
                              (int)mesh.getNbElements(), mesh.nbElementNodes(),
-                             mesh.elements().data(), mesh.nbDOFPerNode(), mesh.codes().data(),
-                             &plan),
+                             mesh.elements().data(), dof ? 1 : mesh.nbDOFPerNode(),
+                             dof ? free_nodes.data() : mesh.codes().data(), &plan),
               "SpMatrix::setMesh");
     _plan = plan;
     hip_check(ofh_asm_matrix(plan, &_csr), "SpMatrix::setMesh");

@@ -240,6 +240,52 @@ oi orc_node_graph(oi nb_el, int nen, int nb_dof, const oi* elem, const oi* dof,
   return nu;
 }
 
+/* NodeGraphScal, GraphOfMatrix.cpp:532-546: the overload SpMatrix::setMesh
+ * calls for dof != 0 (SpMatrix_impl.h:399-402).  Every element node pair
+ * (in, jn), both orders, as node numbers: one row per node, no equation
+ * numbering and no imposed-DOF elimination; the matrix has nb_nodes rows
+ * (Matrix_impl.h:148-150).  FinalizeGraph/StoreGraph as above. */
+oi orc_node_graph_scal(oi nb_el, int nen, const oi* elem, oi nb_nodes, oi* row_ptr,
+                       oi** col_ind) {
+  oi cap = nb_el * nen * nen * 2;
+  uint64_t* ij = (uint64_t*)malloc(sizeof(uint64_t) * (cap > 0 ? cap : 1));
+  if (!ij) return ORC_E_NOMEM;
+  oi np = 0;
+  for (oi e = 0; e < nb_el; ++e) {
+    const oi* en = elem + e * nen;
+    for (int in = 0; in < nen; ++in)
+      for (int jn = 0; jn < nen; ++jn) {
+        uint64_t a = (uint64_t)(en[in] - 1), b = (uint64_t)(en[jn] - 1);
+        ij[np++] = (a << 32) | b;
+        if (en[in] != en[jn]) ij[np++] = (b << 32) | a;
+      }
+  }
+  uint64_t* tmp = (uint64_t*)malloc(sizeof(uint64_t) * (np > 0 ? np : 1));
+  if (!tmp) { free(ij); return ORC_E_NOMEM; }
+  radix_sort_u64(ij, tmp, np, 64);
+  free(tmp);
+  oi nu = 0;
+  for (oi k = 0; k < np; ++k)
+    if (nu == 0 || ij[k] != ij[nu - 1]) ij[nu++] = ij[k];
+  oi* ci = (oi*)malloc(sizeof(oi) * (nu > 0 ? nu : 1));
+  oi nr = 0;
+  row_ptr[nr++] = 0;
+  for (oi k = 0; k < nu; ++k) {
+    ci[k] = (oi)(ij[k] & 0xFFFFFFFFu) + 1;
+    if (k > 0 && (ij[k] >> 32) > (ij[k - 1] >> 32)) {
+      /* a node in no element leaves an empty row, which StoreGraph cannot
+       * represent (it pushes row_ptr only where the row index changes) */
+      if ((oi)(ij[k] >> 32) != (oi)(ij[k - 1] >> 32) + 1) { free(ij); free(ci); return ORC_E_PATTERN; }
+      row_ptr[nr++] = k;
+    }
+  }
+  free(ij);
+  if (nr != nb_nodes) { free(ci); return ORC_E_PATTERN; }
+  row_ptr[nr] = nu;
+  *col_ind = ci;
+  return nu;
+}
+
 /* ======================================================================= */
 /* Shape functions                                                         */
 /* ======================================================================= */
@@ -546,6 +592,27 @@ int orc_assemble(int kind, int dim, const double* coord, oi nb_el, const oi* ele
         for (int i = 0; i < 4; ++i) {
           if (coef) erhs[i] += f[en[i] - 1] * 0.25 * vol; /* DC3DT4.cpp:207-211 */
           else erhs[i] += 0.25 * vol * f[en[i] - 1];     /* Laplace3DT4.cpp:125-129 */
+        }
+      } else if (kind == ORC_ELAS2DQ4) {
+        /* Body force at the 2x2 Gauss points, f = 2 values per node (fx, fy).
+         * Equa_Solid::build (Equa_Solid.h:188-189) calls Elas2DQ4::BodyRHS()
+         * (Elas2DQ4.cpp:224-237), whose loop reads _ebf(2j-1), _ebf(2j) for
+         * j = 0..3: the pair of node j-1, and for j = 0 two entries before the
+         * local vector (undefined behaviour).  Restated here is the intended
+         * sum, as the one-argument BodyRHS(f) (Elas2DQ4.cpp:240-253) writes it:
+         * node j's force with node j's shape function. */
+        double sh[16], dsh[32], w[4];
+        orc_quad4_atgauss(x, sh, dsh, w);
+        for (int k = 0; k < 4; ++k) {
+          double fx = 0., fy = 0.;
+          for (int j = 0; j < 4; ++j) {
+            fx += sh[4 * j + k] * f[(en[j] - 1) * 2];
+            fy += sh[4 * j + k] * f[(en[j] - 1) * 2 + 1];
+          }
+          for (int i = 0; i < 4; ++i) {
+            erhs[2 * i] += w[k] * fx * sh[4 * i + k];
+            erhs[2 * i + 1] += w[k] * fy * sh[4 * i + k];
+          }
         }
       }
     }

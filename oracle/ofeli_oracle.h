@@ -66,6 +66,8 @@ oi  orc_number_equations(oi nb_nodes, int nb_dof, const int* code, oi* dof);
 /* ---- sparse pattern (L3) ---- */
 oi  orc_node_graph(oi nb_el, int nen, int nb_dof, const oi* elem, const oi* dof,
                    oi nb_eq, oi* row_ptr, oi** col_ind);
+oi  orc_node_graph_scal(oi nb_el, int nen, const oi* elem, oi nb_nodes, oi* row_ptr,
+                        oi** col_ind);
 void orc_free(void* p);
 
 /* ---- shape functions (L2) ---- */

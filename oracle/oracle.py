@@ -50,6 +50,9 @@ def lib(fast=False):
     L.orc_node_graph.argtypes = [_oi, C.c_int, C.c_int, _i64p, _i64p, _oi, _i64p,
                                  C.POINTER(C.POINTER(_oi))]
     L.orc_node_graph.restype = _oi
+    L.orc_node_graph_scal.argtypes = [_oi, C.c_int, _i64p, _oi, _i64p,
+                                      C.POINTER(C.POINTER(_oi))]
+    L.orc_node_graph_scal.restype = _oi
     L.orc_free.argtypes = [C.c_void_p]
     L.orc_triang3.argtypes = [_f64p, C.POINTER(C.c_double), _f64p]
     L.orc_tetra4.argtypes = [_f64p, C.POINTER(C.c_double), _f64p]
@@ -158,6 +161,23 @@ def node_graph(elem, nb_dof, dof, nb_eq):
                            C.byref(cp))
     if nnz < 0:
         raise RuntimeError("orc_node_graph failed: %d" % nnz)
+    col = np.ctypeslib.as_array(cp, shape=(nnz,)).copy()
+    L.orc_free(cp)
+    return row_ptr, col
+
+
+def node_graph_scal(elem, nb_nodes):
+    """NodeGraphScal (GraphOfMatrix.cpp:532-546), the pattern of
+    SpMatrix(mesh, dof != 0): node numbers, no elimination; row_ptr 0-based,
+    col_ind 1-based."""
+    L = lib()
+    elem = np.ascontiguousarray(elem, np.int64)
+    row_ptr = np.zeros(nb_nodes + 1, np.int64)
+    cp = C.POINTER(_oi)()
+    nnz = L.orc_node_graph_scal(elem.shape[0], elem.shape[1], elem, nb_nodes, row_ptr,
+                                C.byref(cp))
+    if nnz < 0:
+        raise RuntimeError("orc_node_graph_scal failed: %d" % nnz)
     col = np.ctypeslib.as_array(cp, shape=(nnz,)).copy()
     L.orc_free(cp)
     return row_ptr, col

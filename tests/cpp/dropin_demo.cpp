@@ -116,6 +116,20 @@ int main(int argc, char** argv) {
     print_vec("u", u);
     std::printf("},\n");
   }
+  // 4b. SpMatrix(mesh, dof): dof != 0 gives NodeGraphScal's node pattern,
+  // dof == 0 NodeGraph's equation pattern (SpMatrix_impl.h:391-413)
+  {
+    Mesh ms(0., 1., 0., 1., 5, 4, 1, 1, 1, 1, TRIANGLE, 1);
+    SpMatrix<real_t> As(ms, 1), Ae(ms);
+    std::printf("{\"case\": \"graph_scal\", \"n\": %zu, \"n_eq\": %zu, \"row_ptr\": [",
+                As.getNbRows(), Ae.getNbRows());
+    for (size_t i = 1; i <= As.getNbRows() + 1; ++i)
+      std::printf("%s%zu", i > 1 ? "," : "", As.getRowPtr(i));
+    std::printf("], \"col_ind\": [");
+    for (size_t k = 1; k <= As.getLength(); ++k)
+      std::printf("%s%zu", k > 1 ? "," : "", As.getColInd(k));
+    std::printf("]},\n");
+  }
   // 5. unsymmetric SpMatrix: BiCG, CGS, SpMatrix::TMult, Prec::TransSolve
   {
     const size_t n = 40;

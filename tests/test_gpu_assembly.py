@@ -93,6 +93,45 @@ def test_elas2dq4_cantilever(ctx, mode):
     assert np.array_equal(host(gb), ref)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_elas2dq4_body_force(ctx, mode):
+    """Body force (fx, fy per node) at the 2x2 Gauss points, added before the
+    Dirichlet lifting (Equa_Solid.h:184-190), with nonzero imposed
+    displacements: right-hand side bitwise."""
+    m = O.mesh2d(10, 6, quad=True, nb_dof=2, codes=(1, 0, 2, 0), box=(0.0, 3.0, 0.0, 1.0))
+    m.code[:, 1] = m.code[:, 0]
+    rng = np.random.default_rng(9)
+    E = np.exp(0.5 * rng.standard_normal(m.nb_elements))
+    bf = rng.standard_normal(2 * m.nb_nodes)
+    bc = 1e-2 * rng.standard_normal(2 * m.nb_nodes)
+    check_plan(ctx, m, O.ELAS2DQ4, f=bf, coef=E, prm=[0.3, mode], bc=bc)
+
+
+def test_elas2dq4_tractions_on_two_edges(ctx):
+    """Sides listed out of geometric order over the right and top edges: the
+    corner node receives contributions from sides far apart in the list;
+    every entry of b adds them in side order, bitwise the oracle's."""
+    import ofeli_amd as oa
+    nx, ny = 9, 5
+    m = O.mesh2d(nx, ny, quad=True, nb_dof=2, codes=(1, 0, 0, 0), box=(0.0, 2.0, 0.0, 1.0))
+    m.code[:, 1] = m.code[:, 0]
+    rng = np.random.default_rng(10)
+    E = np.exp(0.3 * rng.standard_normal(m.nb_elements))
+    P, a, b, dof = check_plan(ctx, m, O.ELAS2DQ4, coef=E, prm=[0.3, 1])
+    right = [[nx * (ny + 1) + j + 1, nx * (ny + 1) + j + 2] for j in range(ny)]
+    top = [[i * (ny + 1) + ny + 1, (i + 1) * (ny + 1) + ny + 1] for i in range(nx)]
+    sides = np.array(right[::-1] + top[::2] + top[1::2], np.int32)
+    tr = rng.standard_normal((sides.shape[0], 2))
+    gb = dev(b)
+    P.traction(sides, dev(tr), gb)
+    ref = O.traction_rhs(m, sides.astype(np.int64), tr, dof, b)
+    assert np.array_equal(host(gb), ref)
+    # a second call with fewer sides reuses the plan's buffers
+    gb2 = dev(b)
+    P.traction(sides[:3], dev(tr[:3]), gb2)
+    assert np.array_equal(host(gb2), O.traction_rhs(m, sides[:3].astype(np.int64), tr[:3], dof, b))
+
+
 def test_insert_bc(ctx):
     m = O.mesh2d(7, 5)
     import ofeli_amd as oa

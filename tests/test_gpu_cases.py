@@ -36,7 +36,7 @@ def test_cantilever_load_cases_match_oracle(ctx):
         basis_orc.append(bo)
     G = oa.ProjectionGuess(A, m=16, minres=True)
     x0 = torch.empty_like(b0)
-    its = []
+    its, errs = [], []
     for c in range(d["alpha"].shape[0]):
         al = d["alpha"][c]
         b = sum(float(al[k]) * basis_gpu[k] for k in range(8))
@@ -46,7 +46,10 @@ def test_cantilever_load_cases_match_oracle(ctx):
         assert info.true_relres <= 1e-8
         ret, xo, _ = O.cg(rp, ci, a, bo, max_it=20000, toler=1e-12)
         assert ret > 0
-        assert np.linalg.norm(x0.cpu().numpy() - xo) <= 1e-5 * np.linalg.norm(xo), c
+        e = np.linalg.norm(x0.cpu().numpy() - xo) / np.linalg.norm(xo)
+        errs.append(e)
+        assert e <= 1e-7, (c, e)
+    print("load cases: max err vs oracle %.2e" % max(errs))
     # the first 8 cases span every case: the later ones need no iterations
     assert min(its[:8]) > 0 and max(its[8:]) == 0 and G.size() == 8, its
 
@@ -66,9 +69,17 @@ def test_mixed_cg_converges_on_an_ill_conditioned_system(ctx):
             f=torch.ones(coord.shape[0], dtype=torch.float64, device="cuda"), matrix=True, b=b)
     x64 = torch.zeros_like(b)
     i64 = oa.cg(P.A, b, x64, max_it=100000, toler=1e-8)
+    xt = torch.zeros_like(b)  # the judge: FP64 CG far below the tolerance
+    it_ = oa.cg(P.A, b, xt, max_it=200000, toler=1e-13)
     x = torch.zeros_like(b)
     info = oa.cg_mixed(P.A, b, x, max_it=100000, toler=1e-8)
-    assert i64.nb_it > 0
+    assert i64.nb_it > 0 and it_.nb_it > 0
     assert info.nb_it > 0 and info.true_relres <= 1e-8, info.as_dict()
     assert info.iterations < 6 * i64.iterations, (info.iterations, i64.iterations)
-    assert float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64)) < 1e-5
+    nt = torch.linalg.norm(xt)
+    e64 = float(torch.linalg.norm(x64 - xt) / nt)
+    emx = float(torch.linalg.norm(x - xt) / nt)
+    print("ill-conditioned: err FP64@1e-8 %.2e, mixed %.2e (fallback %d)"
+          % (e64, emx, info.fallback))
+    # the mixed solution is as accurate as the FP64 solve of the same tolerance
+    assert emx <= max(1e-7, 10 * e64)

@@ -138,3 +138,17 @@ def test_mesh_file_program(cases):
     assert rel(c["u"], u) < 1e-10
     MIO.write(c["put"] + ".oracle", r)
     assert open(c["put"], "rb").read() == open(c["put"] + ".oracle", "rb").read()
+
+
+def test_spmatrix_mesh_dof_pattern(cases):
+    """SpMatrix(mesh, dof != 0) has NodeGraphScal's pattern (one row per
+    node, no elimination; GraphOfMatrix.cpp:532-546 as SpMatrix_impl.h:399-402
+    selects it), SpMatrix(mesh) NodeGraph's over the equations."""
+    c = cases["graph_scal"]
+    m = O.mesh2d(5, 4)
+    rp, ci = O.node_graph_scal(m.elem, m.nb_nodes)
+    assert c["n"] == m.nb_nodes == 30
+    assert np.array_equal(np.array(c["row_ptr"]), rp)
+    assert np.array_equal(np.array(c["col_ind"]), ci)
+    dof, neq = O.number_equations(m.code)
+    assert c["n_eq"] == neq == 12
