@@ -163,6 +163,18 @@ int ofh_ssor_solve(ofh_csr A, const double* d_b, double* d_x);
 /* CG (CG.h:99-157 via CG.h:189-200): x holds the initial guess. */
 int ofh_cg(ofh_csr A, int prec, const double* d_b, double* d_x, int max_it,
            double toler, ofh_solve_info* info);
+/* CG that allocates nothing (no reference counterpart: for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The ofh_cg iteration for
+ * IDENT_PREC / DIAG_PREC (CG.h:99-157; DIAG pivots 1/a_ii found per call),
+ * results within rounding of ofh_cg's; its work vectors, partial sums and
+ * solver state live in a static device array of the library (16.8 MB per
+ * GPU, held until the process ends) instead of the context's workspace, and
+ * no SELL copy or other matrix companion is built.  At most 262,144 rows
+ * (OFH_E_INVALID beyond).  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU race (serialise them, e.g. one stream). */
+int ofh_cg_noalloc(ofh_csr A, int prec, const double* d_b, double* d_x, int max_it,
+                   double toler, ofh_solve_info* info);
 /* BiCGStab (BiCGStab.h:87-162 via :194-216). */
 int ofh_bicgstab(ofh_csr A, int prec, const double* d_b, double* d_x, int max_it,
                  double toler, ofh_solve_info* info);

@@ -44,6 +44,7 @@ SIGNATURES = {
     "ofh_prec_trans_apply": (i32, [vp, i32, vp, vp]),
     "ofh_ssor_solve": (i32, [vp, vp, vp]),
     "ofh_cg": (i32, [vp, i32, vp, vp, i32, f64, vp]),
+    "ofh_cg_noalloc": (i32, [vp, i32, vp, vp, i32, f64, vp]),
     "ofh_bicgstab": (i32, [vp, i32, vp, vp, i32, f64, vp]),
     "ofh_cgs": (i32, [vp, i32, vp, vp, i32, f64, vp]),
     "ofh_bicg": (i32, [vp, i32, vp, vp, i32, f64, vp]),

@@ -28,6 +28,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
 
 #include "ofh_internal.h"
 #include "ofh_solver.h"
@@ -691,9 +694,190 @@ int cg_dispatch(ofh_csr A, int prec, const double* b, double* x, int max_it, dou
   }
 }
 
+// ---------------------------------------------------------------------------
+// CG without allocation (ofh_cg_noalloc): the IDENT/DIAG two-kernel iteration
+// above over the CSR-stream partition, with every buffer beyond the caller's
+// matrix, b and x in the file-scope device array `scratch`:
+//   [partials: kSlots x kMaxPartials | state block | 6 vectors of kNaMaxRows]
+// The array exists once per GPU for the life of the process; host code uses
+// the address hipGetSymbolAddress returned for the current device at context
+// creation (noalloc_setup), never the array's name.  Every call on a GPU
+// shares that GPU's array: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr int kNaMaxRows = 1 << 18;  // 262,144 rows
+constexpr size_t kNaPartials = (size_t)kSlots * kMaxPartials;  // doubles
+constexpr size_t kNaStateDoubles = 128;  // KrylovState (<= 512 B), result, pivot flag
+constexpr size_t kNaDoubles = kNaPartials + kNaStateDoubles + 6 * (size_t)kNaMaxRows;
+static_assert(sizeof(KrylovState) <= 512, "state block");
+
+static __device__ __align__(256) double scratch[kNaDoubles];
+
+constexpr int kNaMaxDevices = 64;
+static double* g_na_scratch[kNaMaxDevices] = {};
+static std::mutex g_na_mu;
+
+int noalloc_setup(int device) {
+  OFH_CHECK(device >= 0 && device < kNaMaxDevices, OFH_E_INVALID,
+            "ofh_cg_noalloc: device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (g_na_scratch[device]) return OFH_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(OFH_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_na_scratch[device] = static_cast<double*>(ptr);
+  return OFH_OK;
+}
+
+// KrylovState of a new solve (a kernel, so that no host copy is needed)
+__global__ void k_na_state_init(KrylovState* st, double tol, int max_it, int* bad) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *st = KrylovState{};
+    st->tol = tol;
+    st->max_it = max_it;
+    *bad = 0x7fffffff;
+  }
+}
+
+// pivot_i = 1/a_ii (Prec_impl.h:306-316) with the diagonal found by a binary
+// search of the sorted row; a missing or zero diagonal is reported in *bad.
+__global__ void k_na_inv_diag(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                              const double* __restrict__ val, double* __restrict__ dinv,
+                              int* __restrict__ bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int lo = rp[i], hi = rp[i + 1] - 1, pos = -1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int c = col[mid];
+    if (c == i) {
+      pos = mid;
+      break;
+    }
+    if (c < i) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  const double a = pos < 0 ? 0.0 : val[pos];
+  if (a == 0.0) {
+    atomicMin(bad, i);
+    dinv[i] = 0.0;
+  } else {
+    dinv[i] = 1.0 / a;
+  }
+}
+
+template <int PREC>
+static int cg_noalloc_impl(ofh_csr A, double* sc, const double* b, double* x, int max_it,
+                           double toler, ofh_solve_info* info) {
+  ofh_ctx ctx = A->ctx;
+  hipStream_t s = ctx->stream;
+  const int n = A->n;
+  const double t0 = now_seconds();
+  double* part = sc;
+  KrylovState* st = reinterpret_cast<KrylovState*>(sc + kNaPartials);
+  double* d_out = sc + kNaPartials + 64;
+  int* bad = reinterpret_cast<int*>(sc + kNaPartials + 72);
+  double* r = sc + kNaPartials + kNaStateDoubles;
+  double* p0 = r + kNaMaxRows;
+  double* p1 = p0 + kNaMaxRows;
+  double* q = p1 + kNaMaxRows;
+  double* z = q + kNaMaxRows;
+  double* dinv = z + kNaMaxRows;
+  hipLaunchKernelGGL(k_na_state_init, dim3(1), dim3(64), 0, s, st, toler, max_it, bad);
+  if (PREC == OFH_DIAG_PREC) {
+    hipLaunchKernelGGL(k_na_inv_diag, dim3((n + 255) / 256), dim3(256), 0, s, n, A->row_ptr,
+                       A->col, A->val, dinv, bad);
+    int hbad = 0;
+    OFH_HIP(hipMemcpyAsync(ctx->host_pinned, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    OFH_HIP(hipStreamSynchronize(s));
+    std::memcpy(&hbad, ctx->host_pinned, sizeof(int));
+    if (hbad != 0x7fffffff)
+      return fail(OFH_E_ZERO_PIVOT,
+                  "In Prec::inv_diag(): null diagonal term: " + std::to_string(hbad + 1));
+  }
+  const int g = reduce_grid(n);
+  const int nb = A->nblocks;
+  const ColIndex ci{A->col, A->col16};
+  double* zw = PREC == OFH_DIAG_PREC ? z : nullptr;
+  const double* dv = PREC == OFH_DIAG_PREC ? dinv : nullptr;
+  hipLaunchKernelGGL(k_cg_init<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr,
+                     ci, A->val, b, x, r, dv, zw, part);
+  const double* zsrc = PREC == OFH_DIAG_PREC ? z : r;
+  auto dirspmv = [&](int it) {
+    hipLaunchKernelGGL(k_cg_dirspmv<PREC>, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks,
+                       A->row_ptr, ci, A->val, it, nb, g, zsrc, p0, p1, q, st, part);
+  };
+  dirspmv(0);
+  OFH_HIP(hipGetLastError());
+  auto read = [&](KrylovState* h) -> int {
+    OFH_HIP(hipMemcpyAsync(ctx->host_pinned, st, sizeof(KrylovState), hipMemcpyDeviceToHost, s));
+    OFH_HIP(hipStreamSynchronize(s));
+    std::memcpy(h, ctx->host_pinned, sizeof(KrylovState));
+    return OFH_OK;
+  };
+  KrylovState h{};
+  int rc;
+  if ((rc = read(&h))) return rc;
+  int enq = 0, it_prev = 0;
+  double res_prev = h.res;
+  while (!h.done && enq < max_it) {
+    const int chunk = next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq);
+    it_prev = h.it;
+    res_prev = h.res;
+    for (int c = 0; c < chunk; ++c) {
+      ++enq;
+      hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, s, n, enq, nb, x, r, p0, p1, q,
+                         dv, zw, st, part);
+      dirspmv(enq);
+    }
+    OFH_HIP(hipGetLastError());
+    if ((rc = read(&h))) return rc;
+  }
+  if (!h.done) {  // max_it == 0 with a non-converged guess
+    h.done = 1;
+    h.status = -(max_it + 1);
+  }
+  const double t1 = now_seconds();
+  if (!info) return OFH_OK;
+  info->nb_it = h.status;
+  info->iterations = h.it;
+  info->relres = h.res;
+  info->outer = 0;
+  info->fallback = 0;
+  if ((rc = true_relres_csr(A, b, x, part, d_out, &info->true_relres))) return rc;
+  info->seconds = t1 - t0;
+  return OFH_OK;
+}
+
+int cg_noalloc(ofh_csr A, int prec, const double* b, double* x, int max_it, double toler,
+               ofh_solve_info* info) {
+  OFH_CHECK(A && b && x, OFH_E_INVALID, "ofh_cg_noalloc: null argument");
+  OFH_CHECK(max_it >= 0, OFH_E_INVALID, "ofh_cg_noalloc: negative max_it");
+  OFH_CHECK(A->n <= kNaMaxRows, OFH_E_INVALID,
+            "ofh_cg_noalloc: " + std::to_string(A->n) + " rows exceed the scratch maximum of " +
+                std::to_string(kNaMaxRows));
+  const int dev = A->ctx->device;
+  double* sc = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    sc = dev >= 0 && dev < kNaMaxDevices ? g_na_scratch[dev] : nullptr;
+  }
+  OFH_CHECK(sc, OFH_E_INVALID, "ofh_cg_noalloc: scratch not set up for this device");
+  switch (prec) {
+    case OFH_IDENT_PREC: return cg_noalloc_impl<OFH_IDENT_PREC>(A, sc, b, x, max_it, toler, info);
+    case OFH_DIAG_PREC: return cg_noalloc_impl<OFH_DIAG_PREC>(A, sc, b, x, max_it, toler, info);
+    default: return fail(OFH_E_UNSUPPORTED, "ofh_cg_noalloc: IDENT_PREC or DIAG_PREC only");
+  }
+}
+
 }  // namespace ofh
 
 extern "C" int ofh_cg(ofh_csr A, int prec, const double* b, double* x, int max_it,
                       double toler, ofh_solve_info* info) {
   return ofh::cg_dispatch(A, prec, b, x, max_it, toler, info);
+}
+
+extern "C" int ofh_cg_noalloc(ofh_csr A, int prec, const double* b, double* x, int max_it,
+                              double toler, ofh_solve_info* info) {
+  return ofh::cg_noalloc(A, prec, b, x, max_it, toler, info);
 }

@@ -299,6 +299,10 @@ int ofh_ctx_create(int device, ofh_ctx* out) {
   OFH_HIP(hipGetDeviceCount(&count));
   OFH_CHECK(device >= 0 && device < count, OFH_E_INVALID, "ofh_ctx_create: bad device");
   OFH_HIP(hipSetDevice(device));
+  {
+    const int rc = noalloc_setup(device);
+    if (rc) return rc;
+  }
   ofh_ctx c = new ofh_ctx_s();
   c->device = device;
   hipDeviceProp_t prop;

@@ -340,6 +340,13 @@ int launch_spmv32(ofh_csr A, const float* x, float* y, hipStream_t s);
 int launch_ssor(ofh_csr A, const double* r, double* z, const int* d_done, hipStream_t s);
 // x <- b - A x style true residual ||b-Ax||/||b|| in FP64 (host result).
 int true_relres(ofh_csr A, const double* b, const double* x, double* out);
+// The same over the CSR-stream partition with the given partial slots and
+// result word; builds and allocates nothing.
+int true_relres_csr(ofh_csr A, const double* b, const double* x, double* partials, double* d_out,
+                    double* out);
+// Looks up, once per device and with that device current, the address of
+// the scratch array of ofh_cg_noalloc (called by ofh_ctx_create).
+int noalloc_setup(int device);
 double now_seconds();
 // A process-unique id for each new matrix (ofh_csr_s::serial).
 uint64_t next_csr_serial();

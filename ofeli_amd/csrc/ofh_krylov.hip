@@ -75,6 +75,25 @@ int true_relres(ofh_csr A, const double* b, const double* x, double* out) {
   return OFH_OK;
 }
 
+// The same from the CSR-stream partition only, with caller-provided partial
+// slots and result word (nothing is built or allocated).
+int true_relres_csr(ofh_csr A, const double* b, const double* x, double* partials, double* d_out,
+                    double* out) {
+  ofh_ctx ctx = A->ctx;
+  const MatRows64 mr{A->nblocks, A->row_blocks, A->row_ptr, ColIndex{A->col, A->col16}, A->val,
+                     Sell64{A->n, nullptr, nullptr, nullptr, nullptr, nullptr}};
+  hipLaunchKernelGGL(k_true_res<false>, dim3(mr.nblocks), dim3(kBlock), 0, ctx->stream, mr, b, x,
+                     partials);
+  hipLaunchKernelGGL(k_true_fold, dim3(1), dim3(kBlock), 0, ctx->stream, mr.nblocks, partials,
+                     d_out);
+  OFH_HIP(hipGetLastError());
+  OFH_HIP(hipMemcpyAsync(ctx->host_pinned, d_out, sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  OFH_HIP(hipStreamSynchronize(ctx->stream));
+  *out = *(double*)ctx->host_pinned;
+  return OFH_OK;
+}
+
 int fill_info(ofh_csr A, const double* b, const double* x, const KrylovState& h, double t0,
               ofh_solve_info* info) {
   OFH_HIP(hipStreamSynchronize(A->ctx->stream));
