@@ -42,6 +42,12 @@ def main():
     ap.add_argument("--n", type=int, default=96)
     ap.add_argument("--toler", type=float, default=1e-8)
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--cold-reps", type=int, default=20,
+                    help="SpMV launches timed after a 1 GiB cache-evicting write each")
+    ap.add_argument("--replica", type=int, default=128,
+                    help="config 1: n of the cube replica (n^3 cells) for the cold SpMV "
+                         "roofline beyond the Infinity Cache; 0 skips it")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="config 1: full oracle solves timed")
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-precs", action="store_true",
@@ -77,6 +83,161 @@ def main():
     if args.config == 4:
         return bench_cantilever(args)
 
+    return bench_poisson3d(args)
+
+
+# ---------------------------------------------------------------- config 1
+def b_spmv64(n, nnz):
+    """Algorithmic bytes of y = A x, FP64 CSR with int32 indices (the survey's
+    B_spmv64): values and columns once, row pointers, x once, y written."""
+    return 12 * nnz + 4 * (n + 1) + 16 * n
+
+
+def b_spmv32(n, nnz):
+    """B_spmv32: the same with FP32 values and vectors."""
+    return 8 * nnz + 4 * (n + 1) + 8 * n
+
+
+def kernel_bytes(n, nnz):
+    """Algorithmic bytes per launch of each kernel of the CG iterations
+    (FP64: B_spmv64 + 88 n per iteration; mixed: B_spmv32 + 44 n)."""
+    return {
+        # mixed, two kernels: product of z with q by recurrence (reads p_old,
+        # q_old, writes p_new, q beside the product), update (reads x, p, r, q,
+        # 1/a_ii; writes x, r, z)
+        "k_mx_dirspmv": b_spmv32(n, nnz) + 12 * n,
+        "k_mx_upd": 32 * n,
+        # FP64, three kernels: q = A p, update (reads x, p, r, q, 1/a_ii;
+        # writes x, r), direction (reads r, 1/a_ii, p; writes p)
+        "k_cg_spmv": b_spmv64(n, nnz),
+        "k_cg_upd": 56 * n,
+        "k_cg_dir": 32 * n,
+    }
+
+
+def working(ms):
+    """Launch durations of the launches that did work: the host enqueues
+    launches past the solver's stop points, which return at once."""
+    ms = np.asarray(ms)
+    if ms.size == 0:
+        return ms
+    return ms[ms >= 0.5 * np.median(ms)]
+
+
+def pmc_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` on an n-row system from the committed
+    rocprofv3 --pmc summaries (profiles/*.md lines "traffic <kernel> n=<rows>
+    FETCH_KiB=<f> WRITE_KiB=<w> fetch_scale=<a> write_scale=<b>"), or None."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path, encoding="utf-8"):
+            m = re.match(r"traffic (\S+) n=(\d+) FETCH_KiB=([\d.]+) WRITE_KiB=([\d.]+) "
+                         r"fetch_scale=([\d.]+) write_scale=([\d.]+)", line.strip())
+            if m and m.group(1) == kernel and int(m.group(2)) == n:
+                f, w, fs, ws = map(float, m.groups()[2:])
+                return {"bytes": round((f * fs + w * ws) * 1024), "source": os.path.relpath(path, ROOT)}
+    return None
+
+
+def spmv_timing(oa, ctx, A, reps_warm, reps_cold, fp32=False):
+    """Standalone SpMV durations from dispatch-attached events: `reps_warm`
+    back-to-back launches (matrix and vectors stay in the 256 MB Infinity
+    Cache when they fit), and `reps_cold` launches each preceded by a 1 GiB
+    streaming write that evicts the caches."""
+    import torch
+    x = torch.rand(A.n, dtype=torch.float64, device="cuda")
+    y = torch.zeros_like(x)
+    if fp32:
+        x, y = x.float(), y.float()
+    fn = oa.spmv_f32 if fp32 else oa.spmv
+    kind = oa.PROF_SPMV32 if fp32 else oa.PROF_SPMV64
+    for _ in range(3):
+        fn(A, x, y)
+    ctx.profile(True)
+    for _ in range(reps_warm):
+        fn(A, x, y)
+    warm = ctx.profile_read(kind)
+    cold = np.zeros(0)
+    if reps_cold:
+        flush = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        ctx.profile(True)
+        for k in range(reps_cold):
+            flush.fill_(float(k))
+            fn(A, x, y)
+        cold = ctx.profile_read(kind)
+        del flush
+    ctx.profile(False)
+    torch.cuda.synchronize()
+    return warm, cold
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model}
+
+
+def cpu_baseline_poisson(args, d, A, b, vals_gpu, it64):
+    """The oracle's CG+DIAG (the reference's algorithm in its operation
+    order) on the same system, one host thread pinned to one core: full
+    solves, median of `cpu_reps` (-Ofast, as the reference builds), one
+    strict -O2 -ffp-contract=off solve, and the element-loop assembly of the
+    values and right-hand side (the pattern is the GPU's, checked equal)."""
+    from oracle import oracle as O
+    rp, cl = A.pattern()
+    rp64 = rp.astype(np.int64)
+    ci64 = cl.astype(np.int64) + 1
+    bh = b.cpu().numpy()
+    old = os.sched_getaffinity(0)
+    core = min(old)
+    os.sched_setaffinity(0, {core})
+    try:
+        O.lib(fast=True)
+        O.lib(fast=False)
+        ts, rets = [], []
+        for _ in range(args.cpu_reps):
+            t = time.perf_counter()
+            ret, xo, res = O.cg(rp64, ci64, vals_gpu, bh, prec=O.DIAG, max_it=20000,
+                                toler=args.toler, fast=True)
+            ts.append(time.perf_counter() - t)
+            rets.append(ret)
+            log("[bench] cpu -Ofast solve %.2f s (%d it)" % (ts[-1], ret))
+        t = time.perf_counter()
+        ret_s, xs_, _ = O.cg(rp64, ci64, vals_gpu, bh, prec=O.DIAG, max_it=20000,
+                             toler=args.toler, fast=False)
+        strict_s = time.perf_counter() - t
+        log("[bench] cpu strict -O2 solve %.2f s (%d it)" % (strict_s, ret_s))
+        m = O.Mesh(d["coord"], d["code"], d["elem"].astype(np.int64), 1)
+        dof, neq = O.number_equations(m.code)
+        t = time.perf_counter()
+        a, bo = O.assemble(O.LAPLACE3DT4, m, dof, neq, rp64, ci64, f=d["f"], coef=d["coef"])
+        asm_s = time.perf_counter() - t
+        log("[bench] cpu assembly (values + RHS) %.2f s" % asm_s)
+    finally:
+        os.sched_setaffinity(0, old)
+    med = float(np.median(ts))
+    return {"value": round(med * 1e3, 1), "unit": "ms per FP64 CG+DIAG solve to rtol %g" % args.toler,
+            "cores": 1, "kind": "port",
+            "sample": "%d full solves of this system (%s iterations) by the oracle built -Ofast, "
+                      "one thread pinned to core %d; median" % (len(ts), sorted(set(rets)), core),
+            "solves_s": [round(t_, 3) for t_ in ts], "strict_O2_solve_s": round(strict_s, 3),
+            "strict_O2_iterations": ret_s, "assembly_values_rhs_s": round(asm_s, 3),
+            "assembly_bitwise_equal_gpu": bool(np.array_equal(a, vals_gpu) and np.array_equal(bo, bh)),
+            "gpu_fp64_iterations": it64, "host": cpu_info()}
+
+
+def bench_poisson3d(args):
+    """3D variable-k Poisson (config 1): GPU assembly, the SpMV roofline warm
+    and cold (96^3 and a 128^3 replica larger than the Infinity Cache), FP64
+    CG and mixed CG per solve, per-kernel durations of one profiled solve of
+    each, and the oracle on one host core."""
     import torch
     import ofeli_amd as oa
     from ofeli_amd import workloads as W
@@ -104,38 +265,14 @@ def main():
     asm_ms = ev0.elapsed_time(ev1)
     log("[bench] assembly %.3f ms" % asm_ms)
 
-    # SpMV roofline (dominant kernel of every CG iteration)
-    x = torch.rand(n, dtype=torch.float64, device="cuda")
-    y = torch.zeros_like(x)
-    for _ in range(5):
-        oa.spmv(A, x, y)
-    ev0.record()
-    for _ in range(args.spmv_reps):
-        oa.spmv(A, x, y)
-    ev1.record()
-    torch.cuda.synchronize()
-    spmv_ms = ev0.elapsed_time(ev1) / args.spmv_reps
-    # bytes the kernel must move: its matrix copy once (SELL-64: values and
-    # 16- or 32-bit columns per stored entry, two ints per 64-row chunk), x
-    # once and y; the CSR form with 32-bit columns is reported beside it
+    # standalone SpMV, warm and cold
     sell = A.sell_info()
-    b_csr64 = 12 * nnz + 4 * (n + 1) + 16 * n
-    if sell["in_use"]:
-        b_spmv64 = (8 + sell["col_bytes"]) * sell["stored"] + 8 * ((n + 63) // 64) + 16 * n
-    else:
-        b_spmv64 = b_csr64
-    spmv_gbs = b_spmv64 / (spmv_ms * 1e-3) / 1e9
-    log("[bench] spmv64 %.4f ms  %.0f GB/s (SELL-64, %d-byte columns; CSR bytes / time: %.0f GB/s)"
-        % (spmv_ms, spmv_gbs, sell["col_bytes"], b_csr64 / (spmv_ms * 1e-3) / 1e9))
-    A.compress_columns(True)
-    ev0.record()
-    for _ in range(args.spmv_reps):
-        oa.spmv(A, x, y)
-    ev1.record()
-    torch.cuda.synchronize()
-    spmv_c16_ms = ev0.elapsed_time(ev1) / args.spmv_reps
-    A.compress_columns(False)
-    log("[bench] spmv64 %.4f ms (CSR-stream with 16-bit column offsets)" % spmv_c16_ms)
+    warm64, cold64 = spmv_timing(oa, ctx, A, args.spmv_reps, args.cold_reps)
+    warm32, _ = spmv_timing(oa, ctx, A, args.spmv_reps, 0, fp32=True)
+    B64, B32 = b_spmv64(n, nnz), b_spmv32(n, nnz)
+    spmv_ms, spmv_cold_ms, spmv32_ms = warm64.mean(), cold64.mean(), warm32.mean()
+    log("[bench] spmv64 warm %.4f ms (%.0f GB/s)  cold %.4f ms (%.0f GB/s)  spmv32 warm %.4f ms"
+        % (spmv_ms, B64 / spmv_ms / 1e6, spmv_cold_ms, B64 / spmv_cold_ms / 1e6, spmv32_ms))
     # streaming reference: a device-to-device copy of 1 GiB
     big = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
     big2 = torch.empty_like(big)
@@ -148,17 +285,6 @@ def main():
     copy_gbs = 5 * 2 * big.numel() * 8 / (ev0.elapsed_time(ev1) * 1e-3) / 1e9
     del big, big2
     log("[bench] device copy %.0f GB/s (read+write)" % copy_gbs)
-    xf, yf = x.float(), y.float()
-    oa.spmv_f32(A, xf, yf)
-    ev0.record()
-    for _ in range(args.spmv_reps):
-        oa.spmv_f32(A, xf, yf)
-    ev1.record()
-    torch.cuda.synchronize()
-    spmv32_ms = ev0.elapsed_time(ev1) / args.spmv_reps
-    b_spmv32 = ((4 + sell["col_bytes"]) * sell["stored"] + 8 * ((n + 63) // 64) + 8 * n
-                if sell["in_use"] else 8 * nnz + 4 * (n + 1) + 8 * n)
-    log("[bench] spmv32 %.4f ms  %.0f GB/s" % (spmv32_ms, b_spmv32 / (spmv32_ms * 1e-3) / 1e9))
 
     def solve(fn):
         xs = torch.zeros(n, dtype=torch.float64, device="cuda")
@@ -169,8 +295,8 @@ def main():
         solve(oa.cg)
         solve(oa.cg_mixed)
     t64, t32 = [], []
+    torch.cuda.synchronize()
     for k in range(args.steps):
-        torch.cuda.synchronize()
         t = time.perf_counter()
         i64, x64 = solve(oa.cg)
         torch.cuda.synchronize()
@@ -186,28 +312,68 @@ def main():
     ms64 = 1e3 * float(np.median(t64))
     ms32 = 1e3 * float(np.median(t32))
     it64 = i64.iterations
-    cg_iter_us = ms64 * 1e3 / max(it64, 1)
-    b_cg64_iter = b_spmv64 + 88 * n
+
+    # per-kernel durations: one profiled solve of each (after the timed region)
+    kb = kernel_bytes(n, nnz)
+    kernels = {}
+    ctx.profile(True)
+    solve(oa.cg_mixed)
+    solve(oa.cg)
+    for name, kind in (("k_mx_dirspmv", oa.PROF_MX_DIRSPMV), ("k_mx_upd", oa.PROF_MX_UPD),
+                       ("k_cg_spmv", oa.PROF_CG_SPMV), ("k_cg_upd", oa.PROF_CG_UPD),
+                       ("k_cg_dir", oa.PROF_CG_DIR)):
+        ms = ctx.profile_read(kind)
+        w = working(ms)
+        if w.size == 0:
+            continue
+        avg = float(w.mean())
+        kernels[name] = {"launches": int(ms.size), "working_launches": int(w.size),
+                         "avg_us": round(avg * 1e3, 2), "bytes_per_launch": kb[name],
+                         "GBs": round(kb[name] / avg / 1e6, 1),
+                         "frac": round(kb[name] / avg / 1e6 / HBM_PEAK_GBS, 4)}
+        log("[bench] %-13s %4d/%4d launches  %.2f us  %.0f GB/s" % (
+            name, w.size, ms.size, avg * 1e3, kb[name] / avg / 1e6))
+    ctx.profile(False)
+    # share of the mixed step's kernel time per kernel
+    mx_total = sum(kernels[k]["avg_us"] * kernels[k]["working_launches"]
+                   for k in ("k_mx_dirspmv", "k_mx_upd") if k in kernels)
+
+    # replica larger than the 256 MB Infinity Cache
+    replica = None
+    if args.replica:
+        del P, A, b, x32, x64, coef, f
+        torch.cuda.empty_cache()
+        t0 = time.time()
+        dr = W.config1(args.replica)
+        Pr = oa.AssemblyPlan(ctx, dr["coord"], dr["elem"], dr["code"])
+        br = torch.zeros(Pr.nb_eq, dtype=torch.float64, device="cuda")
+        Pr.build(oa.LAPLACE3DT4, coef=torch.as_tensor(dr["coef"], device="cuda"),
+                 f=torch.as_tensor(dr["f"], device="cuda"), matrix=True, b=br)
+        torch.cuda.synchronize()
+        nr, nnzr = Pr.A.n, Pr.A.nnz
+        wr, cr = spmv_timing(oa, ctx, Pr.A, args.spmv_reps, args.cold_reps)
+        Br = b_spmv64(nr, nnzr)
+        replica = {"rows": nr, "nnz": nnzr, "bytes_per_launch": Br,
+                   "warm_ms": round(float(wr.mean()), 4), "cold_ms": round(float(cr.mean()), 4),
+                   "warm_GBs": round(Br / wr.mean() / 1e6, 1), "cold_GBs": round(Br / cr.mean() / 1e6, 1)}
+        log("[bench] replica %d^3: n=%d nnz=%d spmv64 warm %.4f ms (%.0f GB/s) cold %.4f ms (%.0f GB/s) (%.0fs)"
+            % (args.replica, nr, nnzr, wr.mean(), Br / wr.mean() / 1e6, cr.mean(),
+               Br / cr.mean() / 1e6, time.time() - t0))
+        Pr.close()
+        del Pr, br, dr
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as O
-        vals = A.values()
-        rp, cl = A.pattern()
-        rp64 = rp.astype(np.int64)
-        ci64 = cl.astype(np.int64) + 1
-        bh = b.cpu().numpy()
-        O.lib(fast=True)
-        t = time.perf_counter()
-        O.cg(rp64, ci64, vals, bh, prec=O.DIAG, max_it=args.cpu_iters, toler=1e-300, fast=True)
-        cpu_s = time.perf_counter() - t
-        per_it = cpu_s / args.cpu_iters
-        cpu = {"value": per_it * it64 * 1e3, "unit": "ms per FP64 CG+DIAG solve (extrapolated)",
-               "cores": 1, "kind": "port",
-               "sample": "%d CG+DIAG iterations of this system on 1 host core (oracle -Ofast), "
-                         "%.1f ms/iteration x %d GPU iterations" % (args.cpu_iters, per_it * 1e3, it64)}
-        log("[bench] cpu %.1f ms/it" % (per_it * 1e3))
+        # the 96^3 system again (the GPU copies were released for the replica)
+        Pc = oa.AssemblyPlan(ctx, d["coord"], d["elem"], d["code"])
+        bc_ = torch.zeros(Pc.nb_eq, dtype=torch.float64, device="cuda")
+        Pc.build(oa.LAPLACE3DT4, coef=torch.as_tensor(d["coef"], device="cuda"),
+                 f=torch.as_tensor(d["f"], device="cuda"), matrix=True, b=bc_)
+        cpu = cpu_baseline_poisson(args, d, Pc.A, bc_, Pc.A.values(), it64)
+        Pc.close()
 
+    dom = kernels.get("k_mx_dirspmv")
+    traffic = pmc_traffic("k_mx_dirspmv", n)
     out = {
         "metric": METRIC, "value": round(ms32, 3), "unit": "ms per solve (mixed FP32/FP64 CG, rtol 1e-8)",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms32, 3),
@@ -219,28 +385,51 @@ def main():
         "mixed_replacements": i32.outer, "speedup_mixed_vs_fp64": round(ms64 / ms32, 3),
         "mixed_err_vs_fp64": err, "fp64_true_relres": i64.true_relres,
         "mixed_true_relres": i32.true_relres, "assembly_ms": round(asm_ms, 3),
-        "plan_setup_s": round(setup_s, 3), "cg64_us_per_iter": round(cg_iter_us, 2),
-        "cg64_iter_GBs": round(b_cg64_iter / (cg_iter_us * 1e-6) / 1e9, 1),
-        "spmv32_ms": round(spmv32_ms, 4), "spmv64_col16_ms": round(spmv_c16_ms, 4),
+        "plan_setup_s": round(setup_s, 3), "cg64_us_per_iter": round(ms64 * 1e3 / max(it64, 1), 2),
         "device_copy_GBs": round(copy_gbs, 1),
-        "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
-                     # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE runs recorded in
-                     # profiles/spmv_cg_poisson3d_96cube.md (96^3 system only)
-                     "traffic": 142.3e6 if args.n == 96 and sell["col_bytes"] == 2 else None,
-                     "kernel": "k_spmv_sell<double, 2> (SELL-64 FP64, %d-byte columns)"
-                               % sell["col_bytes"],
-                     "bytes_per_launch": b_spmv64, "avg_ms": round(spmv_ms, 4),
-                     "csr32_bytes_per_launch": b_csr64,
-                     "csr32_equivalent_GBs": round(b_csr64 / (spmv_ms * 1e-3) / 1e9, 1)},
+        "kernels": kernels,
+        "mixed_kernel_share": {k: round(kernels[k]["avg_us"] * kernels[k]["working_launches"] / mx_total, 3)
+                               for k in ("k_mx_dirspmv", "k_mx_upd") if k in kernels and mx_total},
+        "roofline": None if dom is None else {
+            "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": dom["frac"], "traffic": None if traffic is None else traffic["bytes"],
+            "traffic_source": None if traffic is None else traffic["source"],
+            "kernel": "k_mx_dirspmv<2,0,true,true> (mixed CG: FP32 SELL-64 product of z, "
+                      "q by recurrence, direction, (p,q) partials)",
+            "bytes_per_launch": dom["bytes_per_launch"], "avg_ms": round(dom["avg_us"] / 1e3, 5),
+            "bytes_definition": "B_spmv32 + 12n = 8 nnz + 4(n+1) + 20n"},
+        "roofline_spmv64": {
+            "kernel": "k_spmv_sell<double,2> (SELL-64, %d-byte columns, %d stored entries)"
+                      % (sell["col_bytes"], sell["stored"]),
+            "bytes_per_launch": B64, "bytes_definition": "B_spmv64 = 12 nnz + 4(n+1) + 16n",
+            "warm_ms": round(float(spmv_ms), 5), "warm_GBs": round(B64 / spmv_ms / 1e6, 1),
+            "warm_frac": round(B64 / spmv_ms / 1e6 / HBM_PEAK_GBS, 4),
+            "cold_ms": round(float(spmv_cold_ms), 5), "cold_GBs": round(B64 / spmv_cold_ms / 1e6, 1),
+            "cold_frac": round(B64 / spmv_cold_ms / 1e6 / HBM_PEAK_GBS, 4),
+            "cold_protocol": "each launch after a 1 GiB streaming write",
+            "traffic": (pmc_traffic("k_spmv_sell<double,2>", n) or {}).get("bytes"),
+            "spmv32_warm_ms": round(float(spmv32_ms), 5),
+            "spmv32_warm_GBs": round(B32 / spmv32_ms / 1e6, 1)},
+        "roofline_cold": None if replica is None else dict(
+            replica, kernel="k_spmv_sell<double,2>", workload="%d^3 Kuhn replica" % args.replica,
+            bound="hbm", peak=HBM_PEAK_GBS, unit="GB/s",
+            frac=round(replica["cold_GBs"] / HBM_PEAK_GBS, 4),
+            bytes_definition="B_spmv64 = 12 nnz + 4(n+1) + 16n",
+            cold_protocol="each launch after a 1 GiB streaming write"),
         "cpu_baseline": cpu,
     }
-    if args.sweep_precs:
-        out["sweep_preconditioners"] = sweep_precs(args, oa, A, b, vals if cpu else None)
-    if args.krylov:
-        out["other_krylov"] = other_krylov(args, oa, A, b, ms64, it64, vals if cpu else None)
-    if args.mesh_io:
-        out["mesh_io"] = mesh_io_leg(args, oa, ctx)
+    if args.sweep_precs or args.krylov or args.mesh_io:
+        Pe = oa.AssemblyPlan(ctx, d["coord"], d["elem"], d["code"])
+        be = torch.zeros(Pe.nb_eq, dtype=torch.float64, device="cuda")
+        Pe.build(oa.LAPLACE3DT4, coef=torch.as_tensor(d["coef"], device="cuda"),
+                 f=torch.as_tensor(d["f"], device="cuda"), matrix=True, b=be)
+        vals = Pe.A.values() if not args.no_cpu else None
+        if args.sweep_precs:
+            out["sweep_preconditioners"] = sweep_precs(args, oa, Pe.A, be, vals)
+        if args.krylov:
+            out["other_krylov"] = other_krylov(args, oa, Pe.A, be, ms64, it64, vals)
+        if args.mesh_io:
+            out["mesh_io"] = mesh_io_leg(args, oa, ctx)
     print(json.dumps(out), flush=True)
 
 

@@ -86,6 +86,26 @@ int ofh_ctx_synchronize(ofh_ctx ctx);
  * directly (enable = 0, the default: measured faster on MI355X).  Results
  * are identical either way. */
 int ofh_ctx_use_graphs(ofh_ctx ctx, int enable);
+/* Per-kernel timing (no reference counterpart; the reference's Timer
+ * measures whole calls with clock(), Timer.h:93-129).  enable = 1 clears
+ * earlier records and, until disabled, gives each launch of the kernels
+ * below (made by ofh_cg, ofh_cg_mixed, ofh_spmv_f64 and ofh_spmv_f32) a
+ * start and a stop event attached to its own dispatch, so each pair times
+ * the kernel's execution; launches replayed from captured graphs are not
+ * timed.  ofh_ctx_profile_read copies the durations (ms) of up to cap
+ * recorded launches of one kind to h_ms and the number recorded to *count;
+ * launches the host enqueued past the solver's stop point are recorded too
+ * (they return at once). */
+#define OFH_PROF_MX_DIRSPMV 0 /* mixed CG: direction + FP32 product        */
+#define OFH_PROF_MX_UPD 1     /* mixed CG: x, r, z update + dot products   */
+#define OFH_PROF_CG_DIRSPMV 2 /* FP64 CG, two-kernel form: direction + A p */
+#define OFH_PROF_CG_UPD 3     /* FP64 CG: x, r, z update + dot products    */
+#define OFH_PROF_CG_SPMV 4    /* FP64 CG, three-kernel form: q = A p       */
+#define OFH_PROF_CG_DIR 5     /* FP64 CG, three-kernel form: direction     */
+#define OFH_PROF_SPMV64 6     /* ofh_spmv_f64 from the SELL-64 copy        */
+#define OFH_PROF_SPMV32 7     /* ofh_spmv_f32 from the SELL-64 copy        */
+int ofh_ctx_profile(ofh_ctx ctx, int enable);
+int ofh_ctx_profile_read(ofh_ctx ctx, int kind, double* h_ms, int cap, int* count);
 int ofh_malloc(ofh_ctx ctx, size_t bytes, void** d_ptr);
 int ofh_free(ofh_ctx ctx, void* d_ptr);
 int ofh_memcpy_h2d(ofh_ctx ctx, void* d_dst, const void* h_src, size_t bytes);

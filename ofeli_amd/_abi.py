@@ -45,6 +45,8 @@ SIGNATURES = {
     "ofh_ssor_solve": (i32, [vp, vp, vp]),
     "ofh_cg": (i32, [vp, i32, vp, vp, i32, f64, vp]),
     "ofh_cg_noalloc": (i32, [vp, i32, vp, vp, i32, f64, vp]),
+    "ofh_ctx_profile": (i32, [vp, i32]),
+    "ofh_ctx_profile_read": (i32, [vp, i32, f64p, i32, C.POINTER(i32)]),
     "ofh_bicgstab": (i32, [vp, i32, vp, vp, i32, f64, vp]),
     "ofh_cgs": (i32, [vp, i32, vp, vp, i32, f64, vp]),
     "ofh_bicg": (i32, [vp, i32, vp, vp, i32, f64, vp]),

@@ -71,6 +71,19 @@ class Context:
     def synchronize(self):
         check(lib().ofh_ctx_synchronize(self.h))
 
+    def profile(self, enable=True):
+        """Per-kernel timing of the solver loops on (clears earlier records) or off."""
+        check(lib().ofh_ctx_profile(self.h, int(bool(enable))))
+
+    def profile_read(self, kind):
+        """Durations (ms) of the recorded launches of one kind (OFH_PROF_*)."""
+        cnt = C.c_int()
+        check(lib().ofh_ctx_profile_read(self.h, int(kind), None, 0, C.byref(cnt)))
+        ms = np.zeros(max(cnt.value, 1))
+        check(lib().ofh_ctx_profile_read(self.h, int(kind), ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                         cnt.value, C.byref(cnt)))
+        return ms[: cnt.value]
+
     def use_graphs(self, enable=True):
         """Replay chunks of solver iterations from captured graphs (default off)."""
         check(lib().ofh_ctx_use_graphs(self.h, int(bool(enable))))

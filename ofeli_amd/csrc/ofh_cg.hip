@@ -418,7 +418,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_upd(int n, int it_arg, int nparts
   const int it = it_arg >= 0 ? it_arg : st->it + 1;
   const double* __restrict__ p = (it & 1) ? p1 : p0;
   __shared__ double red[2 * kBlock / kWave];
-  // this thread's loads go out before the fold of the (p,q) partials
+  // the (p,q) partials load first: their fold (and its barriers) runs while
+  // this thread's vector loads are in flight
+  const int sl[1] = {3};
+  PartialsPrefetch<kBlock, 1, 14> pf;
+  const bool pre = nparts_pq <= pf.kMax;
+  if (pre) pf.load(sl, partials, nparts_pq);
   const int base = blockIdx.x * (kBlock * kVec) + threadIdx.x;
   double xv[kVec], pv[kVec], rv[kVec], qv[kVec], dv[kVec];
 #pragma unroll
@@ -433,8 +438,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_upd(int n, int it_arg, int nparts
     }
   }
   double pq[1];
-  const int sl[1] = {3};
-  fold_partials<kBlock, 1>(pq, sl, partials, nparts_pq, red);
+  if (pre)
+    pf.fold(pq, nparts_pq, red);
+  else
+    fold_partials<kBlock, 1>(pq, sl, partials, nparts_pq, red);
   const double alpha = st->rho_buf[(it - 1) & 1] / pq[0];
   double v[2] = {0.0, 0.0};
 #pragma unroll
@@ -555,22 +562,21 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
     const int nbs = (n + kBlock - 1) / kBlock;
     if (nbs <= kMaxPartials && (rc = csr_ensure_sell(A, false, &sell))) return rc;
     const int npq = sell ? nbs : nb;
-    auto dirspmv = [&](int it, hipStream_t st_) {
+    auto dirspmv = [&](int it, hipStream_t st_, int pk = -1) {  // pk: profiling kind
       if (sell)
-        hipLaunchKernelGGL(k_cg_dirspmv_sell<PREC>, dim3(nbs), dim3(kBlock), 0, st_, n, nbs,
-                           A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val, it,
-                           nb, g, z, p,
-                           p1, q, st, part);
+        prof_kernel(ctx, pk, k_cg_dirspmv_sell<PREC>, dim3(nbs), dim3(kBlock), st_, n, nbs,
+                    A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val, it, nb,
+                    g, z, p, p1, q, st, part);
       else
-        hipLaunchKernelGGL(k_cg_dirspmv<PREC>, dim3(nb), dim3(kBlock), 0, st_, nb, A->row_blocks,
-                           A->row_ptr, ci, A->val, it, nb, g, z, p, p1, q, st, part);
+        prof_kernel(ctx, pk, k_cg_dirspmv<PREC>, dim3(nb), dim3(kBlock), st_, nb, A->row_blocks,
+                    A->row_ptr, ci, A->val, it, nb, g, z, p, p1, q, st, part);
     };
     // one iteration: k_cg_upd(it), k_cg_dirspmv(it); it = -1 takes the
     // index from the state (captured chunks)
     auto iteration = [&](int it, hipStream_t st_) {
-      hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, it, npq, x, r, p, p1, q,
-                         A->dinv, zw, st, part);
-      dirspmv(it, st_);
+      prof_kernel(ctx, OFH_PROF_CG_UPD, k_cg_upd<PREC>, dim3(g), dim3(kBlock), st_, n, it, npq, x,
+                  r, p, p1, q, A->dinv, zw, st, part);
+      dirspmv(it, st_, OFH_PROF_CG_DIRSPMV);
     };
     dirspmv(0, s);
     OFH_HIP(hipGetLastError());
@@ -578,11 +584,14 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
     if (ctx->use_graphs) {
       GraphKey key;
       csr_graph_key(A, 16 * 1 + PREC, (uintptr_t)sell, x, wsp, A->val, A->dinv, &key);
+      const bool prof = ctx->prof;
+      ctx->prof = false;  // no event records inside a captured graph
       rc = ctx_graph(ctx, key, [&](hipStream_t cs) {
         for (int c = 0; c < kGraphIters; ++c) iteration(-1, cs);
         OFH_HIP(hipGetLastError());
         return OFH_OK;
       }, &chunk_graph);
+      ctx->prof = prof;
       if (rc) return rc;
     }
     KrylovState h{};
@@ -631,25 +640,28 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   // one iteration: k_cg_spmv, k_cg_upd(it), [SSOR sweeps], k_cg_dir(it)
   auto iteration = [&](int it, hipStream_t st_) -> int {
     if (sell)
-      hipLaunchKernelGGL(k_cg_spmv_sell, dim3(nbs), dim3(kBlock), 0, st_, n, nbs, A->sell_cptr,
-                         A->sell_clen, A->sell_col, A->sell_col16, A->sell_val, p, q, st, part);
+      prof_kernel(ctx, OFH_PROF_CG_SPMV, k_cg_spmv_sell, dim3(nbs), dim3(kBlock), st_, n, nbs,
+                  A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val, p, q, st,
+                  part);
     else
-      hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kBlock), 0, st_, nb, A->row_blocks,
-                         A->row_ptr, ColIndex{A->col, A->col16}, A->val, p, q, st, part);
-    hipLaunchKernelGGL(k_cg_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, it, npq, x, r, p, p, q,
-                       A->dinv, nullptr, st, part);
+      prof_kernel(ctx, OFH_PROF_CG_SPMV, k_cg_spmv, dim3(nb), dim3(kBlock), st_, nb,
+                  A->row_blocks, A->row_ptr, ColIndex{A->col, A->col16}, A->val, p, q, st, part);
+    prof_kernel(ctx, OFH_PROF_CG_UPD, k_cg_upd<PREC>, dim3(g), dim3(kBlock), st_, n, it, npq, x,
+                r, p, p, q, A->dinv, (double*)nullptr, st, part);
     if (sweep_prec(PREC)) {
       const int e = ssor_z(it, false);
       if (e) return e;
     }
-    hipLaunchKernelGGL(k_cg_dir<PREC>, dim3(g), dim3(kBlock), 0, st_, n, it, nb, g, r, A->dinv,
-                       z, p, st, part);
+    prof_kernel(ctx, OFH_PROF_CG_DIR, k_cg_dir<PREC>, dim3(g), dim3(kBlock), st_, n, it, nb, g, r,
+                A->dinv, z, p, st, part);
     return OFH_OK;
   };
   hipGraphExec_t chunk_graph = nullptr;
   if (ctx->use_graphs && !sweep_prec(PREC)) {
     GraphKey key;
     csr_graph_key(A, 16 * 2 + PREC, (uintptr_t)sell, x, wsp, A->val, A->dinv, &key);
+    const bool prof = ctx->prof;
+    ctx->prof = false;  // no event records inside a captured graph
     rc = ctx_graph(ctx, key, [&](hipStream_t cs) {
       for (int c = 0; c < kGraphIters; ++c) {
         const int e = iteration(-1, cs);
@@ -658,6 +670,7 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
       OFH_HIP(hipGetLastError());
       return OFH_OK;
     }, &chunk_graph);
+    ctx->prof = prof;
     if (rc) return rc;
   }
   while (!h.done && enq < max_it) {

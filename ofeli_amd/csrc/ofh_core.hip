@@ -327,6 +327,11 @@ int ofh_ctx_destroy(ofh_ctx c) {
   hipHostFree(c->host_pinned);
   if (c->vws) hipFree(c->vws);
   for (auto& e : c->graphs) (void)hipGraphExecDestroy(e.second);
+  for (auto& r : c->prof_recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : c->prof_pool) (void)hipEventDestroy(e);
   if (c->cap) (void)hipStreamDestroy(c->cap);
   if (c->own_stream) hipStreamDestroy(c->stream);
   delete c;
@@ -336,6 +341,36 @@ int ofh_ctx_destroy(ofh_ctx c) {
 int ofh_ctx_use_graphs(ofh_ctx c, int enable) {
   OFH_CHECK(c, OFH_E_INVALID, "ofh_ctx_use_graphs: null ctx");
   c->use_graphs = enable != 0;
+  return OFH_OK;
+}
+
+int ofh_ctx_profile(ofh_ctx c, int enable) {
+  OFH_CHECK(c, OFH_E_INVALID, "ofh_ctx_profile: null ctx");
+  OFH_HIP(hipStreamSynchronize(c->stream));
+  for (auto& r : c->prof_recs) {
+    c->prof_pool.push_back(r.a);
+    c->prof_pool.push_back(r.b);
+  }
+  c->prof_recs.clear();
+  c->prof = enable != 0;
+  return OFH_OK;
+}
+
+int ofh_ctx_profile_read(ofh_ctx c, int kind, double* h_ms, int cap, int* count) {
+  OFH_CHECK(c && count && cap >= 0 && (cap == 0 || h_ms), OFH_E_INVALID,
+            "ofh_ctx_profile_read: bad argument");
+  OFH_HIP(hipStreamSynchronize(c->stream));
+  int k = 0;
+  for (const auto& r : c->prof_recs) {
+    if (r.kind != kind) continue;
+    if (k < cap) {
+      float ms = 0.f;
+      OFH_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      h_ms[k] = ms;
+    }
+    ++k;
+  }
+  *count = k;
   return OFH_OK;
 }
 

@@ -7,6 +7,7 @@
 // always produces bitwise identical results.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
@@ -161,19 +162,19 @@ __device__ __forceinline__ void fold_partials(double (&out)[K], const int (&slot
   block_sum_k<BLOCK, K>(out, lds);
 }
 
-// fold_partials in two steps for nparts <= 7 * BLOCK: the loads first (in
-// flight while the caller starts other work), then the same sums in the same
-// order.
-template <int BLOCK, int K>
+// fold_partials in two steps for nparts <= PER * BLOCK: the loads first (in
+// flight while the caller starts other work), then the sums, in the same
+// order in every workgroup: per thread over j, then the block tree.
+template <int BLOCK, int K, int PER = 7>
 struct PartialsPrefetch {
-  static constexpr int kMax = 7 * BLOCK;
-  double v[K][7];
+  static constexpr int kMax = PER * BLOCK;
+  double v[K][PER];
   __device__ __forceinline__ void load(const int (&slot)[K], const double* __restrict__ partials,
                                        int nparts) {
 #pragma unroll
     for (int k = 0; k < K; ++k)
 #pragma unroll
-      for (int j = 0; j < 7; ++j) {
+      for (int j = 0; j < PER; ++j) {
         const int i = threadIdx.x + j * BLOCK;
         v[k][j] = i < nparts ? partials[slot[k] * kMaxPartials + i] : 0.0;
       }
@@ -183,7 +184,7 @@ struct PartialsPrefetch {
     for (int k = 0; k < K; ++k) {
       double s = 0.0;
 #pragma unroll
-      for (int j = 0; j < 7; ++j)
+      for (int j = 0; j < PER; ++j)
         if (threadIdx.x + j * BLOCK < nparts) s += v[k][j];
       out[k] = s;
     }
@@ -215,6 +216,13 @@ struct GraphKey {
 // Iterations per captured chunk of a Krylov solve.
 constexpr int kGraphIters = 8;
 
+// One timed launch of a solver kernel (ofh_ctx_profile): events recorded on
+// the launching stream right before and right after it.
+struct ProfRec {
+  int kind;
+  hipEvent_t a, b;
+};
+
 }  // namespace ofh
 
 struct ofh_ctx_s {
@@ -233,6 +241,10 @@ struct ofh_ctx_s {
   bool use_graphs = false;
   hipStream_t cap = nullptr;  // capture-only stream
   std::vector<std::pair<ofh::GraphKey, hipGraphExec_t>> graphs;  // small cache, oldest first
+  // per-kernel timing of the solver loops (ofh_ctx_profile), off by default
+  bool prof = false;
+  std::vector<ofh::ProfRec> prof_recs;
+  std::vector<hipEvent_t> prof_pool;  // events of finished records, reused
 };
 
 struct ofh_csr_s {
@@ -284,6 +296,32 @@ struct ofh_csr_s {
 
 namespace ofh {
 int ensure_vws(ofh_ctx ctx, size_t bytes, void** out);
+// Launches `kernel` on s.  With profiling on and kind >= 0 (OFH_PROF_*) the
+// launch carries a start and a stop event in its own dispatch
+// (hipExtLaunchKernelGGL), so the recorded time is the kernel's execution,
+// and the pair is kept as one timed launch of `kind`.
+template <class F, class... Args>
+inline void prof_kernel(ofh_ctx ctx, int kind, F kernel, dim3 grid, dim3 block, hipStream_t s,
+                        Args... args) {
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  if (ctx->prof && kind >= 0) {
+    for (auto& e : ev) {
+      if (!ctx->prof_pool.empty()) {
+        e = ctx->prof_pool.back();
+        ctx->prof_pool.pop_back();
+      } else if (hipEventCreate(&e) != hipSuccess) {
+        e = nullptr;
+      }
+    }
+    if (!ev[0] || !ev[1]) {
+      for (auto& e : ev)
+        if (e) ctx->prof_pool.push_back(e);
+      ev[0] = ev[1] = nullptr;
+    }
+  }
+  hipExtLaunchKernelGGL(kernel, grid, block, 0, s, ev[0], ev[1], 0, args...);
+  if (ev[0]) ctx->prof_recs.push_back(ProfRec{kind, ev[0], ev[1]});
+}
 // The executable graph of `record` (launches on the given stream, kernels
 // only) for `key`, captured on first use and cached in the context.
 int ctx_graph(ofh_ctx ctx, const GraphKey& key, const std::function<int(hipStream_t)>& record,

@@ -437,10 +437,16 @@ __global__ __launch_bounds__(kBlock) void k_mx_upd(int n, int k_arg, int nparts_
   const int k = k_arg >= 0 ? k_arg : st->it + 1;
   const float* __restrict__ p32 = (k & 1) ? p1 : p0;
   __shared__ double red[2 * kBlock / kWave];
+  // the (p,q) partials load first: their fold (and its barriers) runs while
+  // this thread's vector loads are in flight
+  const int sl[1] = {6};
+  PartialsPrefetch<kBlock, 1, 14> pf;
+  const bool pre = nparts_pq <= pf.kMax;
+  if (pre) pf.load(sl, partials, nparts_pq);
   const int base = blockIdx.x * (kBlock * kVec) + threadIdx.x;
   float xv[kVec], pv[kVec], rv[kVec], qv[kVec], dv[kVec];
 #pragma unroll
-  for (int j = 0; j < kVec; ++j) {  // loads in flight during the fold
+  for (int j = 0; j < kVec; ++j) {
     const int i = base + j * kBlock;
     if (i < n) {
       xv[j] = x32[i];
@@ -451,8 +457,10 @@ __global__ __launch_bounds__(kBlock) void k_mx_upd(int n, int k_arg, int nparts_
     }
   }
   double pq[1];
-  const int sl[1] = {6};
-  fold_partials<kBlock, 1>(pq, sl, partials, nparts_pq, red);
+  if (pre)
+    pf.fold(pq, nparts_pq, red);
+  else
+    fold_partials<kBlock, 1>(pq, sl, partials, nparts_pq, red);
   const double alpha = st->rho_buf[(k - 1) & 1] / pq[0];
   const float af = (float)alpha;
   double v[2] = {0.0, 0.0};
@@ -520,12 +528,13 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   const ColIndex ci{A->col, nullptr};  // FP32 partition: 32-bit columns
   const SellF32 sa{n, A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val32};
   const int npq_fused = fsell ? nbsf : A->nblocks32;  // partials of the fused product
-  auto dirspmv = [&](int phase, int k, int nparts, hipStream_t st_) {
+  // pk: the profiling kind of this launch (-1: not timed)
+  auto dirspmv = [&](int phase, int k, int nparts, hipStream_t st_, int pk = -1) {
     const int nbf = fsell ? nbsf : A->nblocks32;
-#define OFH_MX_DIRSPMV(PH, SL, QR)                                                             \
-  hipLaunchKernelGGL((k_mx_dirspmv<PREC, PH, SL, QR>), dim3(nbf), dim3(kBlock), 0, st_, nbf,    \
-                     A->row_blocks32, A->row_ptr, ci, A->val32, k, nparts, z32, pb[0], pb[1], \
-                     q32, state_ptr(ctx), ctx->ws.partials, sa)
+#define OFH_MX_DIRSPMV(PH, SL, QR)                                                              \
+  prof_kernel(ctx, pk, k_mx_dirspmv<PREC, PH, SL, QR>, dim3(nbf), dim3(kBlock), st_, nbf,        \
+              A->row_blocks32, A->row_ptr, ci, A->val32, k, nparts, z32, pb[0], pb[1], q32,      \
+              state_ptr(ctx), ctx->ws.partials, sa)
     if (qrec) {
       // After a residual replacement (phase 2) the product is formed from
       // the gathered direction, q = A (z + beta p_old): this resets the drift
@@ -581,9 +590,9 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   // one inner iteration; k = -1 takes the index from the state (captured chunks)
   auto iteration = [&](int k, hipStream_t st_) {
     if (fused) {
-      hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, k, npq_fused, x32, r32,
-                         pb[0], pb[1], q32, A->dinv32, zw, st, part);
-      dirspmv(0, k, g, st_);
+      prof_kernel(ctx, OFH_PROF_MX_UPD, k_mx_upd<PREC>, dim3(g), dim3(kBlock), st_, n, k,
+                  npq_fused, x32, r32, pb[0], pb[1], q32, A->dinv32, zw, st, part);
+      dirspmv(0, k, g, st_, OFH_PROF_MX_DIRSPMV);
       return;
     }
     if (sell)
@@ -604,11 +613,14 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     csr_graph_key(A, 16 * (fused ? 3 : 4) + PREC,
                   (uintptr_t)fsell | (uintptr_t)qrec << 1 | (uintptr_t)sell << 2, x, wsp,
                   A->val32, A->dinv32, &key);
+    const bool prof = ctx->prof;
+    ctx->prof = false;  // no event records inside a captured graph
     rc = ctx_graph(ctx, key, [&](hipStream_t cs) {
       for (int c = 0; c < kGraphIters; ++c) iteration(-1, cs);
       OFH_HIP(hipGetLastError());
       return OFH_OK;
     }, &chunk_graph);
+    ctx->prof = prof;
     if (rc) return rc;
   }
   KrylovState h{};

@@ -225,8 +225,8 @@ int launch_spmv64(ofh_csr A, const double* x, double* y, hipStream_t s) {
   if (rc) return rc;
   if (sell) {
     const int nb = (A->n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL((k_spmv_sell<double, 2>), dim3(nb), dim3(kBlock), 0, s, A->n, nb,
-                       A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val, x, y);
+    prof_kernel(A->ctx, OFH_PROF_SPMV64, k_spmv_sell<double, 2>, dim3(nb), dim3(kBlock), s, A->n,
+                nb, A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val, x, y);
     OFH_HIP(hipGetLastError());
     return OFH_OK;
   }
@@ -239,9 +239,8 @@ int launch_spmv32(ofh_csr A, const float* x, float* y, hipStream_t s) {
   if (rc) return rc;
   if (sell) {
     const int nb = (A->n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL((k_spmv_sell<float, 4>), dim3(nb), dim3(kBlock), 0, s, A->n, nb,
-                       A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val32, x,
-                       y);
+    prof_kernel(A->ctx, OFH_PROF_SPMV32, k_spmv_sell<float, 4>, dim3(nb), dim3(kBlock), s, A->n,
+                nb, A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val32, x, y);
     OFH_HIP(hipGetLastError());
     return OFH_OK;
   }
