@@ -37,13 +37,11 @@ struct ofh_asm_s {
   int* adj_loc = nullptr;    // local index of the node in that element
   int* row_node = nullptr;   // nb_eq: node of each equation
   int* row_comp = nullptr;   // nb_eq: DOF component of each equation
-  int64_t* map_ptr = nullptr;  // nnz + 1
-  int* map = nullptr;          // element-matrix entry ids (e*nee*nee + kk)
-  double* ke = nullptr;        // nb_el x nee x nee
-  double* erhs = nullptr;      // nb_el x nee
   int* err = nullptr;          // device error slots
   ofh_csr A = nullptr;
   std::vector<int> h_dof;
+  uint8_t* lmap = nullptr;     // per (adjacency entry, DOF component): each local column's
+                               // position in the row (0xFF: eliminated DOF)
   int* trac = nullptr;         // side tractions: side nodes + per-DOF contribution lists
   size_t trac_cap = 0;         // ints held by trac (grown on demand)
 };
@@ -145,41 +143,37 @@ __device__ __forceinline__ int find_col(const int* __restrict__ col, int a, int 
   return -1;
 }
 
-// Gather map: for each nonzero of row r, the element-matrix entries that
-// Matrix::Assembly adds to it, in ascending element order.  FILL = false
-// counts them, FILL = true writes them.
-template <bool FILL>
-__global__ void k_map(int nb_eq, int nen, int nb_dof, int nee, const int* __restrict__ elem,
-                      const int* __restrict__ dof, const int* __restrict__ adj_ptr,
-                      const int* __restrict__ adj_el, const int* __restrict__ adj_loc,
-                      const int* __restrict__ row_node, const int* __restrict__ row_comp,
-                      const int* __restrict__ row_ptr, const int* __restrict__ col,
-                      int* __restrict__ cnt, const int64_t* __restrict__ map_ptr,
-                      int* __restrict__ map) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nb_eq) return;
-  const int node = row_node[r], k = row_comp[r];
-  const int ra = row_ptr[r], re = row_ptr[r + 1];
-  if (FILL)
-    for (int p = ra; p < re; ++p) cnt[p] = 0;
-  for (int t = adj_ptr[node]; t < adj_ptr[node + 1]; ++t) {
-    const int e = adj_el[t], i = adj_loc[t];
-    const int* en = elem + (size_t)e * nen;
-    for (int j = 0; j < nen; ++j)
-      for (int l = 0; l < nb_dof; ++l) {
-        const int d = dof[en[j] * nb_dof + l];
-        if (!d) continue;
-        const int p = find_col(col, ra, re, d - 1);
-        if (p < 0) continue;  // cannot happen: the pattern comes from the same adjacency
-        const int kk = (i * nb_dof + k) * nee + j * nb_dof + l;
-        if (FILL) map[map_ptr[p] + cnt[p]++] = e * nee * nee + kk;
-        else cnt[p]++;
-      }
+// Positions within its row of the columns an element adds to: for every
+// adjacency entry t of a node (element e, local node i) and DOF component k
+// of that node, lmap[(t nb_dof + k) nee + j nb_dof + l] = position of the
+// column of local DOF (j, l) in the row of DOF (node, k), or 0xFF when that
+// column (or the row) is an eliminated DOF.  Rows hold at most kMaxRowCand
+// (< 255) columns.
+__global__ void k_local_map(int nb_nodes, int nen, int nb_dof, int nee,
+                            const int* __restrict__ elem, const int* __restrict__ dof,
+                            const int* __restrict__ adj_ptr, const int* __restrict__ adj_el,
+                            const int* __restrict__ row_ptr, const int* __restrict__ col,
+                            uint8_t* __restrict__ lmap) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= nb_nodes) return;
+  for (int k = 0; k < nb_dof; ++k) {
+    const int dr = dof[node * nb_dof + k];
+    const int ra = dr ? row_ptr[dr - 1] : 0, re = dr ? row_ptr[dr] : 0;
+    for (int t = adj_ptr[node]; t < adj_ptr[node + 1]; ++t) {
+      const int* en = elem + (size_t)adj_el[t] * nen;
+      uint8_t* out = lmap + ((size_t)t * nb_dof + k) * nee;
+      for (int j = 0; j < nen; ++j)
+        for (int l = 0; l < nb_dof; ++l) {
+          const int d = dof[en[j] * nb_dof + l];
+          const int p = (dr && d) ? find_col(col, ra, re, d - 1) : -1;
+          out[j * nb_dof + l] = p < 0 ? (uint8_t)0xFF : (uint8_t)(p - ra);
+        }
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
-// element kernel: element matrix (when ke != null) and right-hand side
+// element routines and the per-equation assembly
 // ---------------------------------------------------------------------------
 struct ElemArgs {
   int kind, nen, nb_dof, nee, dim;
@@ -192,8 +186,6 @@ struct ElemArgs {
   const double* uprev;
   double p0, p1, p2;  // kind parameters
   double g0, g1, w0, w1;  // 2-point Gauss rule (Q4)
-  double* ke;
-  double* erhs;
   int* err;
 };
 
@@ -201,10 +193,17 @@ __device__ __forceinline__ void report(int* err, int e, int rc) {
   atomicMin(&err[rc == -1 ? 1 : 2], e);
 }
 
+// Row `in` of element e's matrix (em_row, NEE entries, when want_row) and
+// its right-hand-side entry (*rhs, when want_rhs), computed with the
+// floating-point operations, in the order, of the equation classes' element
+// routines: Laplace2DT3::LHS / BodyRHS, DC2DT3 Diffusion / LCapacity /
+// BodyRHS and the backward-Euler combination, Laplace3DT4 / DC3DT4,
+// Elas2DQ4 Deviator / Dilatation / body force, then updateBC
+// (Equation_impl.h:160-176) and the time term (Equa_Therm.h:203).
+// Returns 0, or -1 / -2 for a negative / null Jacobian.
 template <int KIND>
-__global__ __launch_bounds__(256) void k_element(int nb_el, ElemArgs a) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nb_el) return;
+__device__ __forceinline__ int elem_row(const ElemArgs& a, int e, int in, bool want_row,
+                                        bool want_rhs, double* em_row, double* rhs) {
   constexpr int NEN = (KIND == OFH_LAPLACE3DT4 || KIND == OFH_ELAS2DQ4) ? 4 : 3;
   constexpr int NB = KIND == OFH_ELAS2DQ4 ? 2 : 1;
   constexpr int NEE = NEN * NB;
@@ -217,170 +216,180 @@ __global__ __launch_bounds__(256) void k_element(int nb_el, ElemArgs a) {
   for (int i = 0; i < NEN; ++i)
 #pragma unroll
     for (int c = 0; c < DIM; ++c) x[DIM * i + c] = a.coord[(size_t)nd[i] * DIM + c];
-  double erhs[NEE];
-#pragma unroll
-  for (int q = 0; q < NEE; ++q) erhs[q] = 0.0;
+  const bool need_row = want_row || (want_rhs && a.bc);
+  double row[NEE];
+  double r = 0.0;
 
   if constexpr (KIND == OFH_ELAS2DQ4) {
     double dx[4][4], dy[4][4], w[4];
     const int rc = quad_shape(reinterpret_cast<const double(&)[8]>(x), a.g0, a.g1, a.w0, a.w1, dx,
                               dy, w);
-    if (rc) { report(a.err, e, rc); return; }
+    if (rc) return rc;
     double lambda, G;
     const int mode = (int)a.p1;
     elas_moduli(a.coef[e], a.p0, mode, lambda, G);
-    if (a.ke) {
-      double* out = a.ke + (size_t)e * 64;
-      for (int i = 0; i < 4; ++i)
-        for (int di = 0; di < 2; ++di)
-          for (int j = 0; j < 4; ++j)
-            for (int dj = 0; dj < 2; ++dj)
-              out[(2 * i + di) * 8 + 2 * j + dj] = q4_entry(i, di, j, dj, dx, dy, w, G, lambda, mode);
+    const int i = in / 2, di = in % 2;
+    if (need_row) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) row[2 * j + dj] = q4_entry(i, di, j, dj, dx, dy, w, G, lambda, mode);
     }
-    if (a.f) {  // body force (fx, fy per node) at the Gauss points, before updateBC
+    if (want_rhs && a.f) {  // body force (fx, fy per node) at the Gauss points
       double sh[4][4];
       quad_values(a.g0, a.g1, sh);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        double fx = 0., fy = 0.;
+        double fc = 0.;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          fx += sh[j][k] * a.f[nd[j] * 2];
-          fy += sh[j][k] * a.f[nd[j] * 2 + 1];
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          erhs[2 * i] += w[k] * fx * sh[i][k];
-          erhs[2 * i + 1] += w[k] * fy * sh[i][k];
-        }
+        for (int j = 0; j < 4; ++j) fc += sh[j][k] * a.f[nd[j] * 2 + di];
+        r += w[k] * fc * sh[i][k];
       }
     }
-    if (a.bc) {  // updateBC, Equation_impl.h:160-176
-      for (int in = 0; in < 8; ++in)
-        for (int j = 0; j < 4; ++j)
-          for (int l = 0; l < 2; ++l)
-            if (a.code[nd[j] * 2 + l] > 0)
-              erhs[in] -= q4_entry(in / 2, in % 2, j, l, dx, dy, w, G, lambda, mode) *
-                          a.bc[nd[j] * 2 + l];
+  } else if constexpr (KIND == OFH_LAPLACE3DT4) {
+    double d[12], meas;
+    const int rc = tet_shape(reinterpret_cast<const double(&)[12]>(x), meas, d);
+    if (rc) return rc;
+    const bool var = a.coef != nullptr;
+    const double c = var ? 1.0 * a.coef[e] * meas : 0.0;
+    if (need_row) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double dd = d[3 * in] * d[3 * j] + d[3 * in + 1] * d[3 * j + 1] + d[3 * in + 2] * d[3 * j + 2];
+        row[j] = var ? 0.0 + (0.0 + c * dd) : meas * dd;
+      }
+    }
+    if (want_rhs && a.f) {
+      const double fi = a.f[nd[in]];
+      r += var ? fi * 0.25 * meas : 0.25 * meas * fi;
     }
   } else {
-    double em[NEE * NEE];
-    double ea1[9];
-    double meas;  // area or volume
-    if constexpr (KIND == OFH_LAPLACE3DT4) {
-      double d[12];
-      const int rc = tet_shape(reinterpret_cast<const double(&)[12]>(x), meas, d);
-      if (rc) { report(a.err, e, rc); return; }
-      const bool var = a.coef != nullptr;
-      const double c = var ? 1.0 * a.coef[e] * meas : 0.0;
+    double d[6], meas;
+    const int rc = tri_shape(reinterpret_cast<const double(&)[6]>(x), meas, d);
+    if (rc) return rc;
+    const double k = KIND == OFH_LAPLACE2DT3 ? 0.0 : a.coef[e];
+    double ea0[3];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const double dd = d[2 * in] * d[2 * j] + d[2 * in + 1] * d[2 * j + 1] + 0.0 * 0.0;
+      ea0[j] = KIND == OFH_LAPLACE2DT3 ? 0.0 + meas * dd : 0.0 + 1.0 * k * meas * dd;
+    }
+    double ea1[3] = {0.0, 0.0, 0.0};
+    double dti = 0.0;
+    if constexpr (KIND == OFH_DC2DT3_BE) {
+      dti = 1. / a.p0;
+      const double c = kThird * meas * a.p1 * a.p2 * 1.0;
+      ea1[in] += c;  // lumped capacity: the diagonal entry only
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const double dd = d[3 * i] * d[3 * j] + d[3 * i + 1] * d[3 * j + 1] + d[3 * i + 2] * d[3 * j + 2];
-          em[4 * i + j] = var ? 0.0 + (0.0 + c * dd) : meas * dd;
-        }
-      if (a.f) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const double fi = a.f[nd[i]];
-          erhs[i] += var ? fi * 0.25 * meas : 0.25 * meas * fi;
-        }
-      }
+      for (int j = 0; j < 3; ++j) row[j] = dti * ea1[j] + ea0[j];
     } else {
-      double d[6];
-      const int rc = tri_shape(reinterpret_cast<const double(&)[6]>(x), meas, d);
-      if (rc) { report(a.err, e, rc); return; }
-      const double k = KIND == OFH_LAPLACE2DT3 ? 0.0 : a.coef[e];
-      double ea0[9];
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          const double dd = d[2 * i] * d[2 * j] + d[2 * i + 1] * d[2 * j + 1] + 0.0 * 0.0;
-          ea0[3 * i + j] = KIND == OFH_LAPLACE2DT3 ? 0.0 + meas * dd : 0.0 + 1.0 * k * meas * dd;
-        }
-      if constexpr (KIND == OFH_DC2DT3_BE) {
-        const double dti = 1. / a.p0;
-        const double c = kThird * meas * a.p1 * a.p2 * 1.0;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) ea1[q] = 0.0;
-        ea1[0] += c;
-        ea1[4] += c;
-        ea1[8] += c;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) em[q] = dti * ea1[q] + ea0[q];
+      for (int j = 0; j < 3; ++j) row[j] = ea0[j];
+    }
+    if (want_rhs && a.f) {
+      if (KIND == OFH_LAPLACE2DT3) {
+        r += a.f[nd[in]] * meas * kThird;
       } else {
-#pragma unroll
-        for (int q = 0; q < 9; ++q) em[q] = ea0[q];
-      }
-      if (a.f) {
-        if (KIND == OFH_LAPLACE2DT3) {
-#pragma unroll
-          for (int i = 0; i < 3; ++i) erhs[i] += a.f[nd[i]] * meas * kThird;
-        } else {
-          const double c = kThird * meas;
-#pragma unroll
-          for (int i = 0; i < 3; ++i) erhs[i] += c * a.f[nd[i]];
-        }
+        const double c = kThird * meas;
+        r += c * a.f[nd[in]];
       }
     }
-    if (a.ke) {
-      double* out = a.ke + (size_t)e * NEE * NEE;
+    if (want_rhs && a.bc) {  // updateBC, Equation_impl.h:160-176
 #pragma unroll
-      for (int q = 0; q < NEE * NEE; ++q) out[q] = em[q];
-    }
-    if (a.bc) {  // updateBC, Equation_impl.h:160-176
-#pragma unroll
-      for (int in = 0; in < NEE; ++in)
-#pragma unroll
-        for (int j = 0; j < NEN; ++j)
-          if (a.code[nd[j]] > 0) erhs[in] -= em[in * NEE + j] * a.bc[nd[j]];
+      for (int j = 0; j < 3; ++j)
+        if (a.code[nd[j]] > 0) r -= row[j] * a.bc[nd[j]];
     }
     if constexpr (KIND == OFH_DC2DT3_BE) {  // eRHS += dti*eA1*u^n, Equa_Therm.h:203
-      if (a.uprev) {
-        const double dti = 1. / a.p0;
-        double m1[9];
+      if (want_rhs && a.uprev) {
+        double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < 9; ++q) m1[q] = dti * ea1[q];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          double s = 0.0;
-#pragma unroll
-          for (int j = 0; j < 3; ++j) s += m1[3 * i + j] * a.uprev[nd[j]];
-          erhs[i] = erhs[i] + s;
-        }
+        for (int j = 0; j < 3; ++j) s += dti * ea1[j] * a.uprev[nd[j]];
+        r = r + s;
       }
     }
-  }
-  double* ro = a.erhs + (size_t)e * NEE;
+    if (want_row)
 #pragma unroll
-  for (int q = 0; q < NEE; ++q) ro[q] = erhs[q];
+      for (int j = 0; j < NEE; ++j) em_row[j] = row[j];
+    if (want_rhs) *rhs = r;
+    return 0;
+  }
+  if (want_rhs && a.bc) {  // updateBC, Equation_impl.h:160-176
+#pragma unroll
+    for (int j = 0; j < NEN; ++j)
+#pragma unroll
+      for (int l = 0; l < NB; ++l)
+        if (a.code[nd[j] * NB + l] > 0) r -= row[j * NB + l] * a.bc[nd[j] * NB + l];
+  }
+  if (want_row)
+#pragma unroll
+    for (int j = 0; j < NEE; ++j) em_row[j] = row[j];
+  if (want_rhs) *rhs = r;
+  return 0;
 }
 
-// a_p = 0 + sum of its element entries in ascending element order
-__global__ void k_gather_matrix(int64_t nnz, const int64_t* __restrict__ map_ptr,
-                                const int* __restrict__ map, const double* __restrict__ ke,
-                                double* __restrict__ val) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nnz) return;
-  double s = 0.0;
-  for (int64_t t = map_ptr[p]; t < map_ptr[p + 1]; ++t) s += ke[map[t]];
-  val[p] = s;
-}
-
-// b_r = 0 + sum over the elements of its node, ascending
-__global__ void k_gather_rhs(int nb_eq, int nb_dof, int nee, const int* __restrict__ row_node,
-                             const int* __restrict__ row_comp, const int* __restrict__ adj_ptr,
-                             const int* __restrict__ adj_el, const int* __restrict__ adj_loc,
-                             const double* __restrict__ erhs, double* __restrict__ b) {
+// The element loop of build() and Matrix::Assembly / Vect::Assembly
+// (Matrix_impl.h:172-201, Vect_impl.h:1522-1534) per equation: one thread
+// per row r walks the elements around its node in ascending element order,
+// computes its row of each element matrix and its right-hand-side entry,
+// and adds them to its own nonzeros and b_r.  Each nonzero and b_r is
+// 0 + its element contributions in ascending element order, the
+// reference's sums, with no element-matrix buffer and no atomics.
+constexpr int kAsmBlock = 128;  // threads (rows) per workgroup of k_asm_rows
+constexpr int kAsmAcc = 16;     // row nonzeros summed in LDS; longer rows add in place
+template <int KIND>
+__global__ __launch_bounds__(kAsmBlock) void k_asm_rows(int nb_eq, ElemArgs a,
+                                                        const int* __restrict__ adj_ptr,
+                                                        const int* __restrict__ adj_el,
+                                                        const int* __restrict__ adj_loc,
+                                                        const int* __restrict__ row_node,
+                                                        const int* __restrict__ row_comp,
+                                                        const uint8_t* __restrict__ lmap,
+                                                        const int* __restrict__ row_ptr,
+                                                        double* __restrict__ val,
+                                                        double* __restrict__ b) {
+  constexpr int NEN = (KIND == OFH_LAPLACE3DT4 || KIND == OFH_ELAS2DQ4) ? 4 : 3;
+  constexpr int NB = KIND == OFH_ELAS2DQ4 ? 2 : 1;
+  constexpr int NEE = NEN * NB;
+  __shared__ double acc_all[kAsmAcc * kAsmBlock];
+  OFH_LDS_BUDGET(sizeof(double) * kAsmAcc * kAsmBlock);
+  double* acc = acc_all + threadIdx.x;  // this row's sums, stride kAsmBlock
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nb_eq) return;
-  const int node = row_node[r], k = row_comp[r];
+  const int node = row_node[r], comp = row_comp[r];
+  const int ra = row_ptr[r], len = row_ptr[r + 1] - ra;
+  const bool lds = len <= kAsmAcc;
+  if (val) {  // Matrix::clear (Matrix_impl.h:164-169)
+    if (lds)
+      for (int q = 0; q < len; ++q) acc[q * kAsmBlock] = 0.0;
+    else
+      for (int q = 0; q < len; ++q) val[ra + q] = 0.0;
+  }
   double s = 0.0;
-  for (int t = adj_ptr[node]; t < adj_ptr[node + 1]; ++t)
-    s += erhs[(size_t)adj_el[t] * nee + adj_loc[t] * nb_dof + k];
-  b[r] = s;
+  for (int t = adj_ptr[node]; t < adj_ptr[node + 1]; ++t) {
+    const int e = adj_el[t], in = adj_loc[t] * NB + comp;
+    double row[NEE], rhs = 0.0;
+    const int rc = elem_row<KIND>(a, e, in, val != nullptr, b != nullptr, row, &rhs);
+    if (rc) {
+      report(a.err, e, rc);
+      continue;
+    }
+    if (val) {
+      const uint8_t* pos = lmap + ((size_t)t * NB + comp) * NEE;
+#pragma unroll
+      for (int j = 0; j < NEE; ++j) {
+        const int q = pos[j];
+        if (q == 0xFF) continue;
+        if (lds)
+          acc[q * kAsmBlock] += row[j];
+        else
+          val[ra + q] += row[j];
+      }
+    }
+    s += rhs;
+  }
+  if (val && lds)
+    for (int q = 0; q < len; ++q) val[ra + q] = acc[q * kAsmBlock];
+  if (b) b[r] = s;
 }
 
 // Elas2DQ4 side tractions (Equa_Solid.h:194-202, Elas2DQ4::BoundaryRHS
@@ -467,8 +476,8 @@ int ofh_asm_create(ofh_ctx ctx, int dim, int nb_nodes, const double* h_coord, in
   OFH_CHECK((dim == 2 || dim == 3) && nb_nodes > 0 && nb_el > 0 && (nen == 3 || nen == 4) &&
                 (nb_dof == 1 || nb_dof == 2),
             OFH_E_INVALID, "ofh_asm_create: unsupported mesh shape");
-  OFH_CHECK((int64_t)nb_el * nen * nen * nb_dof * nb_dof < (int64_t)0x7fffffff, OFH_E_INVALID,
-            "ofh_asm_create: element-matrix buffer exceeds 2^31 entries");
+  OFH_CHECK((int64_t)nb_el * nen < (int64_t)0x7fffffff, OFH_E_INVALID,
+            "ofh_asm_create: element-node table exceeds 2^31 entries");
   for (int64_t t = 0; t < (int64_t)nb_el * nen; ++t)
     OFH_CHECK(h_elem[t] >= 1 && h_elem[t] <= nb_nodes, OFH_E_INVALID,
               "ofh_asm_create: element node id out of range");
@@ -567,28 +576,14 @@ int ofh_asm_create(ofh_ctx ctx, int dim, int nb_nodes, const double* h_coord, in
     if (rc) return rc;
   }
   P->A = A;
-
-  // gather map
-  int* mcnt;
-  OFH_HIP(hipMalloc(&mcnt, sizeof(int) * std::max<int64_t>(nnz, 1)));
-  OFH_HIP(hipMemsetAsync(mcnt, 0, sizeof(int) * nnz, s));
-  const int nee = P->nee;
-  hipLaunchKernelGGL(k_map<false>, dim3((neq + 255) / 256), dim3(256), 0, s, neq, nen, nb_dof, nee,
-                     P->elem, P->dof, P->adj_ptr, P->adj_el, P->adj_loc, P->row_node, P->row_comp,
-                     A->row_ptr, A->col, mcnt, (const int64_t*)nullptr, (int*)nullptr);
-  std::vector<int64_t> mptr;
-  if ((rc = host_exclusive_scan(s, mcnt, (int)nnz, mptr))) return rc;
-  OFH_HIP(hipMalloc(&P->map_ptr, sizeof(int64_t) * (nnz + 1)));
-  OFH_HIP(hipMalloc(&P->map, sizeof(int) * std::max<int64_t>(mptr[nnz], 1)));
-  OFH_HIP(hipMemcpyAsync(P->map_ptr, mptr.data(), sizeof(int64_t) * (nnz + 1), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_map<true>, dim3((neq + 255) / 256), dim3(256), 0, s, neq, nen, nb_dof, nee,
-                     P->elem, P->dof, P->adj_ptr, P->adj_el, P->adj_loc, P->row_node, P->row_comp,
-                     A->row_ptr, A->col, mcnt, P->map_ptr, P->map);
+  // column positions of every element contribution within its row
+  OFH_HIP(hipMalloc(&P->lmap, std::max<size_t>((size_t)tot * nb_dof * P->nee, 1)));
+  hipLaunchKernelGGL(k_local_map, dim3((nb_nodes + 255) / 256), dim3(256), 0, s, nb_nodes, nen,
+                     nb_dof, P->nee, P->elem, P->dof, P->adj_ptr, P->adj_el, A->row_ptr, A->col,
+                     P->lmap);
   OFH_HIP(hipGetLastError());
+
   OFH_HIP(hipStreamSynchronize(s));
-  OFH_HIP(hipFree(mcnt));
-  OFH_HIP(hipMalloc(&P->ke, sizeof(double) * (size_t)nb_el * nee * nee));
-  OFH_HIP(hipMalloc(&P->erhs, sizeof(double) * (size_t)nb_el * nee));
   *out = P;
   return OFH_OK;
 }
@@ -597,8 +592,7 @@ int ofh_asm_destroy(ofh_asm P) {
   if (!P) return OFH_OK;
   hipDeviceSynchronize();
   void* ptrs[] = {P->coord, P->elem, P->dof, P->code, P->adj_ptr, P->adj_el, P->adj_loc,
-                  P->row_node, P->row_comp, P->map_ptr, P->map, P->ke, P->erhs, P->err,
-                  P->trac};
+                  P->row_node, P->row_comp, P->err, P->lmap, P->trac};
   for (void* p : ptrs) hipFree(p);
   ofh_csr_destroy(P->A);
   delete P;
@@ -664,34 +658,26 @@ int ofh_asm_build(ofh_asm P, int kind, const double* d_coef, const double* d_f,
   a.g1 = gx[1];
   a.w0 = gw[0];
   a.w1 = gw[1];
-  a.ke = want_matrix ? P->ke : nullptr;
-  a.erhs = P->erhs;
   a.err = P->err;
-  const dim3 grid((P->nb_el + 255) / 256), blk(256);
+  ofh_csr A = P->A;
+  double* val = want_matrix ? A->val : nullptr;
+  if (!val && !d_b) return OFH_OK;
+  const dim3 grid((P->nb_eq + kAsmBlock - 1) / kAsmBlock), blk(kAsmBlock);
+#define OFH_ASM_ROWS(K)                                                                       \
+  hipLaunchKernelGGL(k_asm_rows<K>, grid, blk, 0, s, P->nb_eq, a, P->adj_ptr, P->adj_el,      \
+                     P->adj_loc, P->row_node, P->row_comp, P->lmap, A->row_ptr, val, d_b)
   switch (kind) {
-    case OFH_LAPLACE2DT3: hipLaunchKernelGGL(k_element<OFH_LAPLACE2DT3>, grid, blk, 0, s, P->nb_el, a); break;
-    case OFH_DC2DT3: hipLaunchKernelGGL(k_element<OFH_DC2DT3>, grid, blk, 0, s, P->nb_el, a); break;
-    case OFH_DC2DT3_BE: hipLaunchKernelGGL(k_element<OFH_DC2DT3_BE>, grid, blk, 0, s, P->nb_el, a); break;
-    case OFH_LAPLACE3DT4: hipLaunchKernelGGL(k_element<OFH_LAPLACE3DT4>, grid, blk, 0, s, P->nb_el, a); break;
-    case OFH_ELAS2DQ4: hipLaunchKernelGGL(k_element<OFH_ELAS2DQ4>, grid, blk, 0, s, P->nb_el, a); break;
+    case OFH_LAPLACE2DT3: OFH_ASM_ROWS(OFH_LAPLACE2DT3); break;
+    case OFH_DC2DT3: OFH_ASM_ROWS(OFH_DC2DT3); break;
+    case OFH_DC2DT3_BE: OFH_ASM_ROWS(OFH_DC2DT3_BE); break;
+    case OFH_LAPLACE3DT4: OFH_ASM_ROWS(OFH_LAPLACE3DT4); break;
+    case OFH_ELAS2DQ4: OFH_ASM_ROWS(OFH_ELAS2DQ4); break;
     default: return fail(OFH_E_INVALID, "ofh_asm_build: unknown kind");
   }
+#undef OFH_ASM_ROWS
   OFH_HIP(hipGetLastError());
-  if ((rc = check_err(P, "element loop"))) return rc;
-  if (want_matrix) {
-    ofh_csr A = P->A;
-    hipLaunchKernelGGL(k_gather_matrix, dim3((unsigned)((A->nnz + 255) / 256)), dim3(256), 0, s,
-                       A->nnz, P->map_ptr, P->map, P->ke, A->val);
-    A->val32_valid = false;
-    A->dinv_valid = false;
-    ++A->version;
-  }
-  if (d_b)
-    hipLaunchKernelGGL(k_gather_rhs, dim3((P->nb_eq + 255) / 256), dim3(256), 0, s, P->nb_eq,
-                       P->nb_dof, P->nee, P->row_node, P->row_comp, P->adj_ptr, P->adj_el,
-                       P->adj_loc, P->erhs, d_b);
-  OFH_HIP(hipGetLastError());
-  return OFH_OK;
+  if (want_matrix) csr_values_changed(A);
+  return check_err(P, "element loop");
 }
 
 int ofh_asm_traction(ofh_asm P, int nb_sides, const int32_t* h_side_nodes,
