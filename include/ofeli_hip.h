@@ -219,7 +219,9 @@ int ofh_linear_solve(ofh_csr A, int solver, int prec, const double* d_b, double*
  * FP32 SpMV and vector updates, FP64 Krylov dot products, FP64 solution
  * accumulation and FP64 residual replacement b-Ax; stops on the true FP64
  * residual ||b-Ax||/||b|| <= toler; falls back to FP64 CG from the current
- * iterate if the FP32 iteration stagnates. prec: IDENT or DIAG. */
+ * iterate if the FP32 iteration stagnates.  prec: IDENT, DIAG, or SSOR /
+ * DILU / ILU (z = M^-1 r by the FP64 level-scheduled sweeps of the FP32
+ * residual, rounded back to FP32). */
 int ofh_cg_mixed(ofh_csr A, int prec, const double* d_b, double* d_x, int max_it,
                  double toler, ofh_solve_info* info);
 

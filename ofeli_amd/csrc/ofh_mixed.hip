@@ -57,8 +57,12 @@ inline int mixed_chunk(int it, double res, int it_prev, double res_prev, double 
   return std::max(1, std::min(c, remaining));
 }
 
+// z_i of the inner iteration: D^-1 r (DIAG, d = the FP32 pivots), r (IDENT),
+// or, for the sweep preconditioners (SSOR, DILU), the z the sweeps computed
+// (d = that z vector, see k_mx_rz).
 template <int PREC>
 __device__ __forceinline__ float zf(int i, float ri, const float* __restrict__ d) {
+  if (sweep_prec(PREC)) return d[i];
   return PREC == OFH_DIAG_PREC ? ri * d[i] : ri;
 }
 
@@ -80,11 +84,13 @@ __global__ __launch_bounds__(kBlock) void k_mx_resid(
     const float rf = (float)ri;
     r32[i] = rf;
     x32[i] = 0.0f;
-    const float z = zf<PREC>(i, rf, dinv32);
-    if (PREC == OFH_DIAG_PREC && z32) z32[i] = z;
     v[0] = bi * bi;
     v[1] = ri * ri;
-    v[2] = (double)rf * (double)z;
+    if (!sweep_prec(PREC)) {  // sweeps: (r,z) -> slot 2 comes from k_mx_rz
+      const float z = zf<PREC>(i, rf, dinv32);
+      if (PREC == OFH_DIAG_PREC && z32) z32[i] = z;
+      v[2] = (double)rf * (double)z;
+    }
     v[3] = (double)rf * (double)rf;
   }
   block_sum_k<kBlock, 4>(v, red);
@@ -453,7 +459,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_upd(int n, int k_arg, int nparts_
       pv[j] = p32[i];
       rv[j] = r32[i];
       qv[j] = q32[i];
-      dv[j] = PREC == OFH_DIAG_PREC ? dinv32[i] : 1.0f;
+      dv[j] = PREC == OFH_DIAG_PREC ? dinv32[i] : 1.0f;  // (unused by the sweeps)
     }
   }
   double pq[1];
@@ -471,10 +477,12 @@ __global__ __launch_bounds__(kBlock) void k_mx_upd(int n, int k_arg, int nparts_
       x32[i] = xv[j] + af * pv[j];
       const float ri = rv[j] - af * qv[j];
       r32[i] = ri;
-      const float z = PREC == OFH_DIAG_PREC ? ri * dv[j] : ri;
-      if (PREC == OFH_DIAG_PREC && z32) z32[i] = z;
       v[0] += (double)ri * (double)ri;
-      v[1] += (double)ri * (double)z;
+      if (!sweep_prec(PREC)) {  // sweeps: (r,z) -> slot 5 comes from k_mx_rz
+        const float z = PREC == OFH_DIAG_PREC ? ri * dv[j] : ri;
+        if (PREC == OFH_DIAG_PREC && z32) z32[i] = z;
+        v[1] += (double)ri * (double)z;
+      }
     }
   }
   block_sum_k<kBlock, 2>(v, red);
@@ -484,6 +492,56 @@ __global__ __launch_bounds__(kBlock) void k_mx_upd(int n, int k_arg, int nparts_
     st->alpha = alpha;
     st->kcur = k;
   }
+}
+
+// Sweep preconditioners, PHASE 0: the pause decision of k_mx_dir<.., 0> on
+// the recursive residual (slot 4), taken before the sweeps so that a pause
+// skips them; the same fold, the same numbers, the same state as k_mx_dir
+// records when it pauses (k_mx_dir then returns at once on halt).
+__global__ __launch_bounds__(kBlock) void k_mx_check(int k_arg, int nparts, KrylovState* st,
+                                                    const double* __restrict__ partials) {
+  if (st->halt) return;
+  const int k = k_arg >= 0 ? k_arg : st->kcur;
+  __shared__ double red[kBlock / kWave];
+  double t[1];
+  const int slot[1] = {4};
+  fold_partials<kBlock, 1>(t, slot, partials, nparts, red);
+  const double res = sqrt(t[0]) / st->nrm;
+  const bool pause = res <= kDelta * st->rmax || res <= st->tol || k >= st->max_it || !(res == res);
+  if (pause && threadIdx.x == 0) {
+    st->it = k;
+    st->res = res;
+    st->pause = 1;
+    st->halt = 1;
+  }
+}
+
+// r64 = r32 (the FP64 input of the sweeps), unless *skip
+__global__ void k_mx_to64(int n, const int* __restrict__ skip, const float* __restrict__ r32,
+                          double* __restrict__ r64) {
+  if (*skip) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r64[i] = (double)r32[i];
+}
+
+// z32 = z64 (the sweeps' result, rounded) and the partial (r32, z32) -> slot,
+// unless *skip
+__global__ __launch_bounds__(kBlock) void k_mx_rz(int n, const int* __restrict__ skip,
+                                                 const double* __restrict__ z64,
+                                                 const float* __restrict__ r32,
+                                                 float* __restrict__ z32, int slot_id,
+                                                 double* __restrict__ partials) {
+  if (*skip) return;
+  __shared__ double red[kBlock / kWave];
+  double v[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const float z = (float)z64[i];
+    z32[i] = z;
+    v[0] += (double)r32[i] * (double)z;
+  }
+  block_sum_k<kBlock, 1>(v, red);
+  const int slot[1] = {slot_id};
+  write_partials<1>(v, slot, partials, blockIdx.x);
 }
 
 // x64 += x32 (the correction accumulated since the last replacement)
@@ -503,6 +561,7 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   const double t0 = now_seconds();
   int rc;
   if (PREC == OFH_DIAG_PREC && (rc = csr_ensure_dinv(A))) return rc;
+  if (sweep_prec(PREC) && (rc = csr_ensure_prec(A, PREC))) return rc;
   if ((rc = csr_ensure_val32(A))) return rc;
   // Two kernels per inner iteration (k_mx_upd, k_mx_dirspmv) when the
   // product reads the SELL-64 copy (q by recurrence: one gather, the
@@ -510,19 +569,29 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   // L2-resident (direction formed at the gathers); otherwise three
   // (k_mx_spmv, k_mx_upd, k_mx_dir).
   constexpr int kFuseRows = 3 << 17;
+  // The sweep preconditioners (SSOR, DILU) take the three-kernel form with
+  // z = M^-1 r between the update and the direction: k_mx_check (pause
+  // decision), the FP64 level-scheduled sweeps of the FP32 residual, k_mx_rz.
   bool fsell = false;
   const int nbsf = (n + kBlock - 1) / kBlock;
-  if (nbsf <= kMaxPartials && (rc = csr_ensure_sell(A, true, &fsell))) return rc;
+  if (!sweep_prec(PREC) && nbsf <= kMaxPartials && (rc = csr_ensure_sell(A, true, &fsell)))
+    return rc;
   const bool qrec = fsell;
-  const bool fused = n <= kFuseRows || qrec;
+  const bool fused = !sweep_prec(PREC) && (n <= kFuseRows || qrec);
   void* wsp;
-  if ((rc = ensure_vws(ctx, sizeof(float) * (size_t)n * 6, &wsp))) return rc;
+  const size_t vbytes = sizeof(float) * (size_t)n * 6 + (sweep_prec(PREC) ? sizeof(double) * 2 * (size_t)n : 0);
+  if ((rc = ensure_vws(ctx, vbytes, &wsp))) return rc;
   float* r32 = (float*)wsp;
   float* p32 = r32 + n;
   float* q32 = p32 + n;
   float* x32 = q32 + n;
   float* p32b = x32 + n;                                     // second direction buffer
-  float* z32 = PREC == OFH_DIAG_PREC ? p32b + n : r32;       // D^-1 r (fused form)
+  // z: D^-1 r (fused DIAG form), the sweeps' result (SSOR, DILU), else r
+  float* z32 = (PREC == OFH_DIAG_PREC || sweep_prec(PREC)) ? p32b + n : r32;
+  double* r64 = sweep_prec(PREC) ? (double*)((char*)wsp + sizeof(float) * (size_t)n * 6) : nullptr;
+  double* z64 = r64 ? r64 + n : nullptr;
+  // what k_mx_dir / k_mx_resid read as `dinv32`: the pivots, or z for the sweeps
+  const float* dz32 = sweep_prec(PREC) ? z32 : A->dinv32;
   float* pb[2] = {p32, p32b};
   float* zw = fused && PREC == OFH_DIAG_PREC ? z32 : nullptr;  // z written by resid/upd
   const ColIndex ci{A->col, nullptr};  // FP32 partition: 32-bit columns
@@ -580,20 +649,30 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
       hipLaunchKernelGGL((k_mx_resid<PREC, false>), dim3(nbr), dim3(kBlock), 0, s, mr, b, x, r32,
                          x32, A->dinv32, zw, st, part);
   };
+  // z32 = M^-1 r32 by the FP64 sweeps and (r32, z32) -> slot (sweep precs)
+  auto sweep_z = [&](const int* skip, int nparts, int slot_id, hipStream_t st_) -> int {
+    hipLaunchKernelGGL(k_mx_to64, dim3(ge), dim3(256), 0, st_, n, skip, r32, r64);
+    const int e = launch_prec(A, PREC, r64, z64, skip, st_);
+    if (e) return e;
+    hipLaunchKernelGGL(k_mx_rz, dim3(nparts), dim3(kBlock), 0, st_, n, skip, z64, r32, z32,
+                       slot_id, part);
+    return OFH_OK;
+  };
   resid();
+  if (sweep_prec(PREC) && (rc = sweep_z(&st->done, nbr, 2, s))) return rc;
   if (fused)
     dirspmv(1, 0, nbr, s);
   else
     hipLaunchKernelGGL((k_mx_dir<PREC, 1>), dim3(g), dim3(kBlock), 0, s, n, 0, nbr, r32,
-                       A->dinv32, p32, st, part);
+                       dz32, p32, st, part);
   OFH_HIP(hipGetLastError());
   // one inner iteration; k = -1 takes the index from the state (captured chunks)
-  auto iteration = [&](int k, hipStream_t st_) {
+  auto iteration = [&](int k, hipStream_t st_) -> int {
     if (fused) {
       prof_kernel(ctx, OFH_PROF_MX_UPD, k_mx_upd<PREC>, dim3(g), dim3(kBlock), st_, n, k,
                   npq_fused, x32, r32, pb[0], pb[1], q32, A->dinv32, zw, st, part);
       dirspmv(0, k, g, st_, OFH_PROF_MX_DIRSPMV);
-      return;
+      return OFH_OK;
     }
     if (sell)
       hipLaunchKernelGGL(k_mx_spmv_sell, dim3(nbs), dim3(kBlock), 0, st_, n, nbs, A->sell_cptr,
@@ -604,11 +683,17 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
                          A->row_ptr, ci32, A->val32, p32, q32, st, part);
     hipLaunchKernelGGL(k_mx_upd<PREC>, dim3(g), dim3(kBlock), 0, st_, n, k, sell ? nbs : nb32, x32,
                        r32, p32, p32, q32, A->dinv32, nullptr, st, part);
+    if (sweep_prec(PREC)) {
+      hipLaunchKernelGGL(k_mx_check, dim3(1), dim3(kBlock), 0, st_, k, g, st, part);
+      const int e = sweep_z(&st->halt, g, 5, st_);
+      if (e) return e;
+    }
     hipLaunchKernelGGL((k_mx_dir<PREC, 0>), dim3(g), dim3(kBlock), 0, st_, n, k, g, r32,
-                       A->dinv32, p32, st, part);
+                       dz32, p32, st, part);
+    return OFH_OK;
   };
   hipGraphExec_t chunk_graph = nullptr;
-  if (ctx->use_graphs) {
+  if (ctx->use_graphs && !sweep_prec(PREC)) {
     GraphKey key;
     csr_graph_key(A, 16 * (fused ? 3 : 4) + PREC,
                   (uintptr_t)fsell | (uintptr_t)qrec << 1 | (uintptr_t)sell << 2, x, wsp,
@@ -616,7 +701,10 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     const bool prof = ctx->prof;
     ctx->prof = false;  // no event records inside a captured graph
     rc = ctx_graph(ctx, key, [&](hipStream_t cs) {
-      for (int c = 0; c < kGraphIters; ++c) iteration(-1, cs);
+      for (int c = 0; c < kGraphIters; ++c) {
+        const int e = iteration(-1, cs);
+        if (e) return e;
+      }
       OFH_HIP(hipGetLastError());
       return OFH_OK;
     }, &chunk_graph);
@@ -641,7 +729,8 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     const int full = chunk_graph ? chunk / kGraphIters : 0;
     for (int c = 0; c < full; ++c) OFH_HIP(hipGraphLaunch(chunk_graph, s));
     k += full * kGraphIters;
-    for (int c = full * kGraphIters; c < chunk; ++c) iteration(++k, s);
+    for (int c = full * kGraphIters; c < chunk; ++c)
+      if ((rc = iteration(++k, s))) return rc;
     OFH_HIP(hipGetLastError());
     if ((rc = read_state(ctx, &h))) return rc;
     if (h.done) break;
@@ -658,11 +747,12 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     // residual replacement in FP64 after kr inner iterations
     hipLaunchKernelGGL(k_mx_accum, dim3(ge), dim3(256), 0, s, n, x, x32, st);
     resid();
+    if (sweep_prec(PREC) && (rc = sweep_z(&st->done, nbr, 2, s))) return rc;
     if (fused)
       dirspmv(2, kr, nbr, s);
     else
       hipLaunchKernelGGL((k_mx_dir<PREC, 2>), dim3(g), dim3(kBlock), 0, s, n, kr, nbr, r32,
-                         A->dinv32, p32, st, part);
+                         dz32, p32, st, part);
     OFH_HIP(hipGetLastError());
     if ((rc = read_state(ctx, &h))) return rc;
     it_prev = h.it;
@@ -721,7 +811,9 @@ extern "C" int ofh_cg_mixed(ofh_csr A, int prec, const double* b, double* x, int
   switch (prec) {
     case OFH_IDENT_PREC: return mixed_impl<OFH_IDENT_PREC>(A, b, x, max_it, toler, info);
     case OFH_DIAG_PREC: return mixed_impl<OFH_DIAG_PREC>(A, b, x, max_it, toler, info);
-    default:
-      return fail(OFH_E_UNSUPPORTED, "ofh_cg_mixed: IDENT_PREC or DIAG_PREC only");
+    case OFH_SSOR_PREC: return mixed_impl<OFH_SSOR_PREC>(A, b, x, max_it, toler, info);
+    case OFH_DILU_PREC:  // the reference's ILU is its DILU
+    case OFH_ILU_PREC: return mixed_impl<OFH_DILU_PREC>(A, b, x, max_it, toler, info);
+    default: return fail(OFH_E_INVALID, "ofh_cg_mixed: unknown preconditioner");
   }
 }

@@ -83,3 +83,38 @@ def test_mixed_cg_converges_on_an_ill_conditioned_system(ctx):
           % (e64, emx, info.fallback))
     # the mixed solution is as accurate as the FP64 solve of the same tolerance
     assert emx <= max(1e-7, 10 * e64)
+
+
+@pytest.mark.parametrize("prec", ["DIAG_PREC", "SSOR_PREC"])
+def test_mixed_cg_on_a_small_cantilever(ctx, prec):
+    """Mixed CG on a Q4 plane-stress cantilever (48 x 12 quads): the FP32
+    inner iteration (q carried by recurrence, recomputed as A p after each
+    FP64 replacement) reaches the FP64 stop test without the FP64 fallback,
+    and its solution is within 1e-7 of the oracle's rtol-1e-12 solution.
+    (From 64 x 16 quads on, with DIAG_PREC, FP32 no longer resolves the
+    slowest modes and the solver finishes in FP64 CG: tested by the
+    ill-conditioned case above.)"""
+    import torch
+    import ofeli_amd as oa
+    from ofeli_amd import workloads as W
+    d = W.config4(48, 12, ncases=1)
+    P = oa.AssemblyPlan(ctx, d["coord"], d["elem"], d["code"])
+    b = torch.zeros(P.nb_eq, dtype=torch.float64, device="cuda")
+    P.build(oa.ELAS2DQ4, coef=torch.as_tensor(d["coef"], device="cuda"), prm=d["prm"], matrix=True,
+            b=b)
+    tr = sum(float(d["alpha"][0][k]) * torch.as_tensor(d["basis"][k], device="cuda").reshape(-1, 2)
+             for k in range(8))
+    P.traction(d["sides"], tr.contiguous(), b)
+    rp, col = P.A.pattern()
+    a = P.A.values()
+    ret, xo, _ = O.cg(rp.astype(np.int64), col.astype(np.int64) + 1, a, b.cpu().numpy(),
+                      prec=O.DIAG, max_it=100000, toler=1e-12)
+    assert ret > 0
+    x = torch.zeros_like(b)
+    info = oa.cg_mixed(P.A, b, x, prec=getattr(oa, prec), max_it=100000, toler=1e-8)
+    e = np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo)
+    print("cantilever mixed %s: %d inner it, %d repl, fallback %d, err %.2e"
+          % (prec, info.iterations, info.outer, info.fallback, e))
+    assert info.nb_it > 0 and info.true_relres <= 1e-8
+    assert info.fallback == 0
+    assert e <= 1e-7

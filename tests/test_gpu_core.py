@@ -388,7 +388,7 @@ def test_gmres_matches_oracle(ctx, prec):
     assert rel(host(x), xo) <= 1e-8
 
 
-@pytest.mark.parametrize("prec", ["DIAG_PREC", "IDENT_PREC"])
+@pytest.mark.parametrize("prec", ["DIAG_PREC", "IDENT_PREC", "SSOR_PREC", "DILU_PREC"])
 def test_mixed_cg_fp64_verified(ctx, prec):
     """Mixed precision: stops on the true FP64 residual; its error against
     the FP64 oracle solution stays at the solver-accuracy level (<= 1e-7)."""
@@ -400,6 +400,10 @@ def test_mixed_cg_fp64_verified(ctx, prec):
     info = oa.cg_mixed(A, dev(b), x, prec=getattr(oa, prec), max_it=10000, toler=1e-8)
     assert info.nb_it > 0 and info.fallback == 0
     assert info.true_relres <= 1e-8
+    # the inner FP32 iteration keeps the preconditioner's effect: about the
+    # FP64 iteration count of the same preconditioner (oracle, rtol 1e-8)
+    r8, _, _ = O.cg(rp, ci, a, b, prec=getattr(O, prec.split("_")[0]), max_it=10000, toler=1e-8)
+    assert r8 > 0 and info.iterations <= 1.3 * r8 + 10, (info.iterations, r8)
     # error against the oracle's rtol-1e-12 solution: FP64 CG at rtol 1e-8
     # sits at the same level, so 1e-7 leaves room only for rounding
     e = rel(host(x), xo)
