@@ -417,6 +417,10 @@ int ofh_mpnn_create(ofh_csr A, int layers, int hidden, int extrapolate, uint64_t
 int ofh_mpnn_destroy(ofh_mpnn M);
 int ofh_mpnn_num_params(ofh_mpnn M, int64_t* count);
 int ofh_mpnn_get_params(ofh_mpnn M, float* h_params);
+/* Re-reads the edge weights and diagonal features from the matrix's current
+ * values (same pattern), e.g. after a new assembly with other coefficients;
+ * the parameters are kept, so one network serves a family of matrices. */
+int ofh_mpnn_refresh(ofh_mpnn M);
 /* Sets the parameters and resets the optimiser state. */
 int ofh_mpnn_set_params(ofh_mpnn M, const float* h_params);
 /* ofh_mpnn_guess replays its kernels from a HIP graph captured per argument

@@ -89,6 +89,7 @@ SIGNATURES = {
     "ofh_proj_update": (i32, [vp, vp]),
     "ofh_mpnn_create": (i32, [vp, i32, i32, i32, C.c_uint64, C.POINTER(vp)]),
     "ofh_mpnn_destroy": (i32, [vp]),
+    "ofh_mpnn_refresh": (i32, [vp]),
     "ofh_mpnn_num_params": (i32, [vp, C.POINTER(i64)]),
     "ofh_mpnn_get_params": (i32, [vp, f32p]),
     "ofh_mpnn_set_params": (i32, [vp, f32p]),

@@ -572,6 +572,10 @@ class LearnedGuess:
         self.nparams = cnt.value
         A.ctx._children.add(self)
 
+    def refresh(self):
+        """Re-read the graph weights from A's current values (same pattern)."""
+        check(lib().ofh_mpnn_refresh(self.h))
+
     def params(self):
         p = np.zeros(self.nparams, np.float32)
         check(lib().ofh_mpnn_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))))

@@ -722,6 +722,7 @@ struct ofh_mpnn_s {
   int adam_t = 0;
   uint16_t* wbf = nullptr;  // per layer: Wab then Wu, BF16
   float* e = nullptr;       // nnz edge weights a_ij / sqrt(a_ii a_jj)
+  int* tpos = nullptr;      // nnz: position of a_ji
   float* ld = nullptr;      // n: ln a_ii - mean
   float* F = nullptr;       // n_pad x 8
   float* Hs = nullptr;      // (L+1) x n_pad x 64
@@ -752,7 +753,7 @@ namespace ofh {
 namespace mp {
 
 static void free_all(ofh_mpnn M) {
-  void* ptrs[] = {M->params, M->grads, M->adam_m, M->adam_v, M->wbf, M->e, M->ld, M->F,
+  void* ptrs[] = {M->params, M->grads, M->adam_m, M->adam_v, M->wbf, M->e, M->tpos, M->ld, M->F,
                   M->Hs, M->Zs, M->Ms, M->Us, M->dH0, M->dH1, M->T, M->dZ, M->r, M->z, M->q,
                   M->dv, M->scal, M->dpart, M->part};
   for (void* p : ptrs)
@@ -920,6 +921,88 @@ static int backward(ofh_mpnn M, const double* xn, const double* xp, const double
   return OFH_OK;
 }
 
+// The checks the network needs on A (host, once per pattern): positive
+// diagonal, symmetric pattern, values symmetric to 1e-12 relative; returns
+// the position of a_ji for every entry a_ij.
+static int graph_check(ofh_csr A, std::vector<int>& tpos) {
+  const int n = A->n;
+  std::vector<int> rp(n + 1), col(A->nnz);
+  std::vector<double> val(A->nnz);
+  int rc = ofh_csr_get_pattern(A, rp.data(), col.data());
+  if (rc) return rc;
+  if ((rc = ofh_csr_get_values(A, val.data()))) return rc;
+  std::vector<double> diag(n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+      if (col[k] == i) diag[i] = val[k];
+  for (int i = 0; i < n; ++i)
+    OFH_CHECK(diag[i] > 0.0, OFH_E_INVALID,
+              "ofh_mpnn: the diagonal must be positive (row " + std::to_string(i + 1) + ")");
+  tpos.assign(A->nnz, 0);
+  for (int i = 0; i < n; ++i)
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      const int j = col[k];
+      const int* p = std::lower_bound(col.data() + rp[j], col.data() + rp[j + 1], i);
+      OFH_CHECK(p != col.data() + rp[j + 1] && *p == i, OFH_E_INVALID,
+                "ofh_mpnn: the matrix pattern must be symmetric");
+      const double aji = val[p - col.data()];
+      OFH_CHECK(std::fabs(aji - val[k]) <= 1e-12 * std::sqrt(diag[i] * diag[j]), OFH_E_INVALID,
+                "ofh_mpnn: the matrix must be symmetric");
+      tpos[k] = (int)(p - col.data());
+    }
+  return OFH_OK;
+}
+
+// e_ij = (a_ij + a_ji) / (2 sqrt(a_ii a_jj)) (e_ij = e_ji exactly), one row
+// per thread, and the partial sums of ln a_ii per workgroup
+__global__ __launch_bounds__(256) void k_mp_edge_weights(int n, const int* __restrict__ rp,
+                                                         const int* __restrict__ col,
+                                                         const double* __restrict__ val,
+                                                         const int* __restrict__ tpos,
+                                                         const int* __restrict__ dpos,
+                                                         float* __restrict__ e,
+                                                         double* __restrict__ part) {
+  __shared__ double red[4];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double l = 0.0;
+  if (i < n) {
+    const double aii = val[dpos[i]];
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+      e[k] = (float)((val[k] + val[tpos[k]]) / (2.0 * sqrt(aii * val[dpos[col[k]]])));
+    l = log(aii);
+  }
+  l = block_sum<256>(l, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = l;
+}
+
+// ld_i = ln a_ii - mean_j ln a_jj (every workgroup folds the partials alike)
+__global__ __launch_bounds__(256) void k_mp_logdiag(int n, const double* __restrict__ val,
+                                                    const int* __restrict__ dpos,
+                                                    const double* __restrict__ part, int nparts,
+                                                    float* __restrict__ ld) {
+  __shared__ double red[4];
+  double t = 0.0;
+  for (int p = threadIdx.x; p < nparts; p += 256) t += part[p];
+  t = block_sum<256>(t, red);
+  const double mean = t / (double)n;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) ld[i] = (float)(log(val[dpos[i]]) - mean);
+}
+
+// The graph inputs of the network from A's current values, on the GPU.
+static int graph_weights(ofh_mpnn M, hipStream_t s) {
+  ofh_csr A = M->A;
+  int rc = csr_ensure_dinv(A);  // diagonal positions and 1/a_ii of the current values
+  if (rc) return rc;
+  const int nb = (A->n + 255) / 256;
+  hipLaunchKernelGGL(k_mp_edge_weights, dim3(nb), dim3(256), 0, s, A->n, A->row_ptr, A->col,
+                     A->val, M->tpos, A->diag_pos, M->e, M->dpart);
+  hipLaunchKernelGGL(k_mp_logdiag, dim3(nb), dim3(256), 0, s, A->n, A->val, A->diag_pos,
+                     M->dpart, nb, M->ld);
+  OFH_HIP(hipGetLastError());
+  return OFH_OK;
+}
+
 static int refresh_bf16(ofh_mpnn M, hipStream_t s) {
   const int tot = M->L * 2 * kH * kH2;
   hipLaunchKernelGGL(k_mp_to_bf16, dim3((tot + 255) / 256), dim3(256), 0, s, M->L, M->params, M->wbf);
@@ -954,39 +1037,9 @@ int ofh_mpnn_create(ofh_csr A, int layers, int hidden, int extrapolate, uint64_t
   OFH_CHECK(extrapolate == 0 || extrapolate == 1, OFH_E_INVALID, "ofh_mpnn_create: extrapolate is 0 or 1");
   OFH_CHECK(A->n > 0 && A->n <= kMaxRows, OFH_E_INVALID, "ofh_mpnn_create: matrix size");
   const int n = A->n;
-  // host copy of the matrix: symmetry, diagonal, edge weights
-  std::vector<int> rp(n + 1), col(A->nnz);
-  std::vector<double> val(A->nnz);
-  int rc = ofh_csr_get_pattern(A, rp.data(), col.data());
+  std::vector<int> tpos;
+  int rc = graph_check(A, tpos);
   if (rc) return rc;
-  if ((rc = ofh_csr_get_values(A, val.data()))) return rc;
-  std::vector<double> diag(n, 0.0);
-  for (int i = 0; i < n; ++i)
-    for (int k = rp[i]; k < rp[i + 1]; ++k)
-      if (col[k] == i) diag[i] = val[k];
-  for (int i = 0; i < n; ++i)
-    OFH_CHECK(diag[i] > 0.0, OFH_E_INVALID,
-              "ofh_mpnn_create: the diagonal must be positive (row " + std::to_string(i + 1) + ")");
-  // symmetric pattern, values symmetric up to rounding (1e-12 relative);
-  // the edge weights use the symmetric part so that e_ij = e_ji exactly
-  std::vector<float> e(A->nnz), ld(n);
-  for (int i = 0; i < n; ++i)
-    for (int k = rp[i]; k < rp[i + 1]; ++k) {
-      const int j = col[k];
-      const int* p = std::lower_bound(col.data() + rp[j], col.data() + rp[j + 1], i);
-      OFH_CHECK(p != col.data() + rp[j + 1] && *p == i, OFH_E_INVALID,
-                "ofh_mpnn_create: the matrix pattern must be symmetric");
-      const double aji = val[p - col.data()];
-      OFH_CHECK(std::fabs(aji - val[k]) <= 1e-12 * std::sqrt(diag[i] * diag[j]), OFH_E_INVALID,
-                "ofh_mpnn_create: the matrix must be symmetric");
-      e[k] = (float)((val[k] + aji) / (2.0 * std::sqrt(diag[i] * diag[j])));
-    }
-  double mean = 0.0;
-  for (int i = 0; i < n; ++i) mean += std::log(diag[i]);
-  mean /= n;
-  for (int i = 0; i < n; ++i) ld[i] = (float)(std::log(diag[i]) - mean);
-  if ((rc = csr_ensure_dinv(A))) return rc;
-
   ofh_mpnn M = new ofh_mpnn_s();
   M->A = A;
   M->L = layers;
@@ -1007,6 +1060,7 @@ int ofh_mpnn_create(ofh_csr A, int layers, int hidden, int extrapolate, uint64_t
   A_((void**)&M->adam_v, sizeof(float) * M->np);
   A_((void**)&M->wbf, sizeof(uint16_t) * layers * 2 * kH * kH2);
   A_((void**)&M->e, sizeof(float) * A->nnz);
+  A_((void**)&M->tpos, sizeof(int) * A->nnz);
   A_((void**)&M->ld, sizeof(float) * n);
   A_((void**)&M->F, sizeof(float) * np * kFin);
   A_((void**)&M->Hs, sizeof(float) * (layers + 1) * np * kH);
@@ -1052,8 +1106,8 @@ int ofh_mpnn_create(ofh_csr A, int layers, int hidden, int extrapolate, uint64_t
     if (x != hipSuccess && ok == OFH_OK) ok = fail(OFH_E_HIP, hipGetErrorString(x));
   };
   H_(hipMemcpyAsync(M->params, p.data(), sizeof(float) * M->np, hipMemcpyHostToDevice, s));
-  H_(hipMemcpyAsync(M->e, e.data(), sizeof(float) * A->nnz, hipMemcpyHostToDevice, s));
-  H_(hipMemcpyAsync(M->ld, ld.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+  H_(hipMemcpyAsync(M->tpos, tpos.data(), sizeof(int) * A->nnz, hipMemcpyHostToDevice, s));
+  if (ok == OFH_OK) ok = graph_weights(M, s);
   H_(hipMemsetAsync(M->adam_m, 0, sizeof(float) * M->np, s));
   H_(hipMemsetAsync(M->adam_v, 0, sizeof(float) * M->np, s));
   H_(hipMemsetAsync(M->grads, 0, sizeof(float) * M->np, s));
@@ -1074,6 +1128,11 @@ int ofh_mpnn_destroy(ofh_mpnn M) {
   free_all(M);
   delete M;
   return OFH_OK;
+}
+
+int ofh_mpnn_refresh(ofh_mpnn M) {
+  OFH_CHECK(M, OFH_E_INVALID, "ofh_mpnn_refresh: null network");
+  return graph_weights(M, M->A->ctx->stream);
 }
 
 int ofh_mpnn_num_params(ofh_mpnn M, int64_t* count) {

@@ -16,7 +16,15 @@ from . import DC2DT3_BE, DIAG_PREC
 
 
 class HeatBE:
-    def __init__(self, ctx, coord, elem, code, k, dt, rho=1.0, cp=1.0):
+    """judge_every > 0 lets plain projection judge the network: every
+    judge_every-th step with a (ProjectionGuess, LearnedGuess) start computes
+    both starts and keeps the network for the following steps only if its
+    start's residual is below judge_keep times the projection's; otherwise
+    the following steps use the projection alone (the network's inference
+    is skipped until the next judged step)."""
+
+    def __init__(self, ctx, coord, elem, code, k, dt, rho=1.0, cp=1.0, judge_every=0,
+                 judge_keep=0.5):
         import torch
         self.P = api.AssemblyPlan(ctx, coord, elem, code)
         self.A = self.P.A
@@ -32,6 +40,10 @@ class HeatBE:
         self.P.build(DC2DT3_BE, coef=self.k, f=z(coord.shape[0]), u_prev=self.u, prm=self.prm,
                      matrix=True, b=self.b)
         self.step_count = 0
+        self.judge_every, self.judge_keep = int(judge_every), float(judge_keep)
+        self.use_net = True
+        self.judged = 0      # judged steps
+        self.net_steps = 0   # steps started from the network's guess
 
     def rhs(self, f):
         """b = C/dt u^n + F(f) (Equa_Therm.h:197-205), on the GPU."""
@@ -54,10 +66,30 @@ class HeatBE:
             proj, net = guess
             if not hasattr(self, "xg"):
                 self.xg = self.x0.clone()
+                self.rq = self.x0.clone()
             proj.guess(self.b, self.xg)
-            net.guess(self.xg, self.x, self.b, self.x0)
+            judge = self.judge_every > 0 and self.step_count % self.judge_every == 0
+            if judge:
+                net.guess(self.xg, self.x, self.b, self.x0)
+                rn, rp = self._resnorm(self.x0), self._resnorm(self.xg)
+                self.use_net = rn < self.judge_keep * rp
+                self.judged += 1
+                if not self.use_net:
+                    self.x0.copy_(self.xg)
+            elif self.use_net or self.judge_every <= 0:
+                net.guess(self.xg, self.x, self.b, self.x0)
+            else:
+                self.x0.copy_(self.xg)
+            if self.use_net or self.judge_every <= 0:
+                self.net_steps += 1
         else:
             guess.guess(self.x, self.xp, self.b, self.x0)
+
+    def _resnorm(self, x):
+        """||b - A x||, on the GPU."""
+        import torch
+        api.spmv(self.A, x, self.rq)
+        return float(torch.linalg.norm(self.b - self.rq))
 
     def solve(self, mixed=False, toler=1e-8, max_it=20000, prec=DIAG_PREC):
         fn = api.cg_mixed if mixed else api.cg

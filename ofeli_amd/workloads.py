@@ -58,6 +58,19 @@ def fourier_field(rng, pts, modes=64, corr=0.1):
     return np.sqrt(2.0 / modes) * g
 
 
+def fourier_field_device(rng, pts, modes=64, corr=0.1):
+    """fourier_field with the same random modes (drawn from rng on the host)
+    evaluated by torch on the device of the tensor pts (float64)."""
+    import torch
+    dim = pts.shape[1]
+    w = torch.as_tensor(rng.normal(modes * dim).reshape(modes, dim) / corr, device=pts.device)
+    phi = torch.as_tensor(rng.uniform(modes, 0.0, 2.0 * np.pi), device=pts.device)
+    g = torch.zeros(pts.shape[0], dtype=torch.float64, device=pts.device)
+    for m in range(modes):
+        g += torch.cos(pts @ w[m] + phi[m])
+    return np.sqrt(2.0 / modes) * g
+
+
 def centroids(coord, elem):
     return coord[elem - 1].mean(axis=1)
 

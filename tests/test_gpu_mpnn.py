@@ -234,3 +234,84 @@ def test_trained_network_beats_its_base_on_held_out_steps(ctx):
     assert max(ratios) <= 1.0 + 1e-9
     assert np.mean(ratios) < 0.9
     assert np.mean(ratios) < np.mean(ratios_fresh) - 0.05
+
+
+def _kuhn_family(ctx, n, seeds):
+    """A 3D Kuhn P1 Poisson plan and k-fields of several seeds (same mesh)."""
+    import ofeli_amd as oa
+    from ofeli_amd import workloads as W
+    torch = _t()
+    coord, code, elem = oa.api.mesh3d_kuhn(n, n, n)
+    P = oa.AssemblyPlan(ctx, coord, elem, code)
+    pts = W.centroids(coord, elem)
+    ks = [torch.as_tensor(np.exp(W.fourier_field(W.SplitMix64(s), pts, corr=0.2)), device="cuda")
+          for s in seeds]
+    f = torch.ones(coord.shape[0], dtype=torch.float64, device="cuda")
+    return P, ks, f
+
+
+def test_refresh_equals_a_network_built_on_the_new_values(ctx):
+    """ofh_mpnn_refresh re-reads the graph weights from new values: the
+    guess then equals bitwise that of a network with the same parameters
+    created on the new values."""
+    import ofeli_amd as oa
+    torch = _t()
+    P, ks, f = _kuhn_family(ctx, 8, (11, 12))
+    b = torch.zeros(P.nb_eq, dtype=torch.float64, device="cuda")
+    zero = torch.zeros_like(b)
+    P.build(oa.LAPLACE3DT4, coef=ks[0], f=f, matrix=True, b=b)
+    M1 = oa.LearnedGuess(P.A, layers=2, seed=3)
+    P.build(oa.LAPLACE3DT4, coef=ks[1], f=f, matrix=True, b=b)
+    M1.refresh()
+    M2 = oa.LearnedGuess(P.A, layers=2, seed=3)
+    x1, x2 = torch.empty_like(b), torch.empty_like(b)
+    M1.guess(zero, zero, b, x1)
+    M2.guess(zero, zero, b, x2)
+    torch.cuda.synchronize()
+    assert torch.equal(x1, x2)
+
+
+def test_learned_start_on_a_held_out_coefficient_field(ctx):
+    """Steady solves with new coefficient fields: networks trained on the
+    solutions for 8 k-fields of a 14^3 Kuhn mesh (base x = 0, line search
+    along their direction) start the solve of a 9th, unseen k-field."""
+    import ofeli_amd as oa
+    torch = _t()
+    P, ks, f = _kuhn_family(ctx, 14, range(21, 30))
+    n = P.nb_eq
+    zero = torch.zeros(n, dtype=torch.float64, device="cuda")
+    data = []
+    for k in ks:
+        b = torch.zeros(n, dtype=torch.float64, device="cuda")
+        P.build(oa.LAPLACE3DT4, coef=k, f=f, matrix=True, b=b)
+        xs = torch.zeros_like(b)
+        assert oa.cg(P.A, b, xs, max_it=5000, toler=1e-11).nb_it > 0
+        data.append((k, b, xs))
+    out = {}
+    for obj in ("energy", "residual"):
+        M = oa.LearnedGuess(P.A, layers=4, seed=0, objective=obj)
+        x0 = torch.empty_like(zero)
+        rng = np.random.default_rng(0)
+        for ep in range(30):
+            for i in rng.permutation(8):
+                k, b, xs = data[i]
+                P.build(oa.LAPLACE3DT4, coef=k, f=f, matrix=True, b=b)
+                M.refresh()
+                M.train_step(zero, zero, b, xs, x0, lr=2e-3, want_loss=False)
+        k, b, xs = data[8]
+        P.build(oa.LAPLACE3DT4, coef=k, f=f, matrix=True, b=b)
+        M.refresh()
+        loss, _ = M.loss_grad(zero, zero, b, xs, x0, want_grad=False)
+        x_l = torch.empty_like(b)
+        M.guess(zero, zero, b, x_l)
+        r = torch.empty_like(b)
+        oa.spmv(P.A, x_l, r)
+        rel0 = float(torch.linalg.norm(b - r) / torch.linalg.norm(b))
+        i_l = oa.cg(P.A, b, x_l, max_it=5000, toler=1e-8)
+        out[obj] = (loss, rel0, i_l.nb_it)
+        M.close()
+    x_z = torch.zeros_like(b)
+    i_z = oa.cg(P.A, b, x_z, max_it=5000, toler=1e-8)
+    print("held-out k-field: zero start %d it; " % i_z.nb_it + "; ".join(
+        "%s-trained: loss %.3f, start relres %.3f, %d it" % (k, *v) for k, v in out.items()))
+    assert out["energy"][0] < 0.5 and out["residual"][1] < 1.0

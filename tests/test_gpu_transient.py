@@ -69,3 +69,25 @@ def test_heat_be_steps_match_oracle(ctx):
         e = np.linalg.norm(H.u.cpu().numpy() - u_free) / np.linalg.norm(u_free)
         assert e <= 1e-8, (s, e)
     assert H.step_count == STEPS
+
+
+def test_projection_judges_the_network(ctx):
+    """HeatBE(judge_every=k): every k-th step computes both the projected
+    start and the network's correction of it and keeps the network only if
+    its start's residual is below judge_keep times the projection's.  An
+    untrained network does not pass: after each judged step the projection
+    alone starts the solves, and every solve meets the stop test."""
+    import torch
+    import ofeli_amd as oa
+    from ofeli_amd.transient import HeatBE
+    coord, code, elem = oa.api.mesh2d(NX, NX)
+    k = np.exp(0.5 * np.random.default_rng(4).standard_normal(elem.shape[0]))
+    H = HeatBE(ctx, coord, elem, code, k, DT, judge_every=4, judge_keep=0.5)
+    proj = oa.ProjectionGuess(H.A, m=8, minres=True)
+    net = oa.LearnedGuess(H.A, layers=2, seed=5)
+    for s in range(12):
+        info = H.step(torch.as_tensor(source(coord, (s + 1) * DT), device="cuda"),
+                      guess=(proj, net), toler=1e-8)
+        assert info.nb_it >= 0 and info.relres <= 1e-8
+    assert H.judged == 3
+    assert H.net_steps <= H.judged
