@@ -253,9 +253,13 @@ __device__ __forceinline__ int elem_row(const ElemArgs& a, int e, int in, bool w
     const bool var = a.coef != nullptr;
     const double c = var ? 1.0 * a.coef[e] * meas : 0.0;
     if (need_row) {
+      double gi[3];  // d of node `in`, picked by node rather than by a 12-way register index
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+        gi[m] = in == 0 ? d[m] : in == 1 ? d[3 + m] : in == 2 ? d[6 + m] : d[9 + m];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const double dd = d[3 * in] * d[3 * j] + d[3 * in + 1] * d[3 * j + 1] + d[3 * in + 2] * d[3 * j + 2];
+        const double dd = gi[0] * d[3 * j] + gi[1] * d[3 * j + 1] + gi[2] * d[3 * j + 2];
         row[j] = var ? 0.0 + (0.0 + c * dd) : meas * dd;
       }
     }
@@ -269,9 +273,11 @@ __device__ __forceinline__ int elem_row(const ElemArgs& a, int e, int in, bool w
     if (rc) return rc;
     const double k = KIND == OFH_LAPLACE2DT3 ? 0.0 : a.coef[e];
     double ea0[3];
+    const double gi0 = in == 0 ? d[0] : in == 1 ? d[2] : d[4];  // d of node `in`
+    const double gi1 = in == 0 ? d[1] : in == 1 ? d[3] : d[5];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const double dd = d[2 * in] * d[2 * j] + d[2 * in + 1] * d[2 * j + 1] + 0.0 * 0.0;
+      const double dd = gi0 * d[2 * j] + gi1 * d[2 * j + 1] + 0.0 * 0.0;
       ea0[j] = KIND == OFH_LAPLACE2DT3 ? 0.0 + meas * dd : 0.0 + 1.0 * k * meas * dd;
     }
     double ea1[3] = {0.0, 0.0, 0.0};
@@ -328,68 +334,146 @@ __device__ __forceinline__ int elem_row(const ElemArgs& a, int e, int in, bool w
 }
 
 // The element loop of build() and Matrix::Assembly / Vect::Assembly
-// (Matrix_impl.h:172-201, Vect_impl.h:1522-1534) per equation: one thread
-// per row r walks the elements around its node in ascending element order,
-// computes its row of each element matrix and its right-hand-side entry,
-// and adds them to its own nonzeros and b_r.  Each nonzero and b_r is
-// 0 + its element contributions in ascending element order, the
-// reference's sums, with no element-matrix buffer and no atomics.
-constexpr int kAsmBlock = 128;  // threads (rows) per workgroup of k_asm_rows
-constexpr int kAsmAcc = 16;     // row nonzeros summed in LDS; longer rows add in place
-template <int KIND>
-__global__ __launch_bounds__(kAsmBlock) void k_asm_rows(int nb_eq, ElemArgs a,
-                                                        const int* __restrict__ adj_ptr,
-                                                        const int* __restrict__ adj_el,
-                                                        const int* __restrict__ adj_loc,
-                                                        const int* __restrict__ row_node,
-                                                        const int* __restrict__ row_comp,
-                                                        const uint8_t* __restrict__ lmap,
-                                                        const int* __restrict__ row_ptr,
-                                                        double* __restrict__ val,
-                                                        double* __restrict__ b) {
-  constexpr int NEN = (KIND == OFH_LAPLACE3DT4 || KIND == OFH_ELAS2DQ4) ? 4 : 3;
-  constexpr int NB = KIND == OFH_ELAS2DQ4 ? 2 : 1;
-  constexpr int NEE = NEN * NB;
-  __shared__ double acc_all[kAsmAcc * kAsmBlock];
-  OFH_LDS_BUDGET(sizeof(double) * kAsmAcc * kAsmBlock);
-  double* acc = acc_all + threadIdx.x;  // this row's sums, stride kAsmBlock
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nb_eq) return;
-  const int node = row_node[r], comp = row_comp[r];
-  const int ra = row_ptr[r], len = row_ptr[r + 1] - ra;
-  const bool lds = len <= kAsmAcc;
-  if (val) {  // Matrix::clear (Matrix_impl.h:164-169)
-    if (lds)
-      for (int q = 0; q < len; ++q) acc[q * kAsmBlock] = 0.0;
-    else
-      for (int q = 0; q < len; ++q) val[ra + q] = 0.0;
-  }
-  double s = 0.0;
-  for (int t = adj_ptr[node]; t < adj_ptr[node + 1]; ++t) {
-    const int e = adj_el[t], in = adj_loc[t] * NB + comp;
-    double row[NEE], rhs = 0.0;
-    const int rc = elem_row<KIND>(a, e, in, val != nullptr, b != nullptr, row, &rhs);
-    if (rc) {
-      report(a.err, e, rc);
-      continue;
-    }
-    if (val) {
-      const uint8_t* pos = lmap + ((size_t)t * NB + comp) * NEE;
+// (Matrix_impl.h:172-201, Vect_impl.h:1522-1534) per equation.  A
+// workgroup owns R consecutive rows, L lanes per row (L >= the usual row
+// length).  The rows' adjacency entries (element, local node) form one
+// contiguous range, which the workgroup walks in passes: every thread
+// computes element rows and right-hand-side entries of the pass's entries
+// into LDS (adjacency reads coalesced, geometry gathered from a small
+// neighbourhood), then lane l of row r sums, over the entries of r's node
+// in ascending element order, the contributions to nonzeros l and l + L
+// in registers (longer rows: to nonzeros beyond 2L in place), lane 0 the
+// right-hand side.  The LDS reads of that phase are broadcasts within a
+// row's lanes.  Each nonzero and b_r is 0 + its element contributions in
+// ascending element order, the reference's sums, with no element-matrix
+// buffer and no atomics.
+constexpr int kAsmStage = 512;  // element rows staged in LDS per pass
+// The entry of a staged element row (row[j * kAsmStage], positions in
+// words pos[0..W)) that lands on position q of its matrix row, or 0.0 when
+// none does (positions of one element row are distinct; 0xFF never equals
+// q).  Adding that 0.0 leaves a sum that started at +0.0 unchanged, bit for
+// bit.
+template <int W>
+__device__ __forceinline__ double pick(const uint32_t* pos, int q, const double* row) {
+  const uint32_t rep = 0x01010101u * (uint32_t)q;
 #pragma unroll
-      for (int j = 0; j < NEE; ++j) {
-        const int q = pos[j];
-        if (q == 0xFF) continue;
-        if (lds)
-          acc[q * kAsmBlock] += row[j];
-        else
-          val[ra + q] += row[j];
+  for (int w = 0; w < W; ++w) {
+    const uint32_t x = pos[w] ^ rep;
+    const uint32_t z = (x - 0x01010101u) & ~x & 0x80808080u;  // lowest flag: a zero byte
+    if (z) return row[(4 * w + (__builtin_ctz(z) >> 3)) * kAsmStage];
+  }
+  return 0.0;
+}
+constexpr int kAsmThreads = 256;  // threads per workgroup of k_asm_rows
+static_assert(kAsmStage % kAsmThreads == 0, "a pass stages whole rounds of threads");
+template <int KIND>
+struct AsmShape {
+  static constexpr int NEN = (KIND == OFH_LAPLACE3DT4 || KIND == OFH_ELAS2DQ4) ? 4 : 3;
+  static constexpr int NB = KIND == OFH_ELAS2DQ4 ? 2 : 1;
+  static constexpr int NEE = NEN * NB;
+  static constexpr int L = NB == 2 ? 32 : (NEN == 4 ? 16 : 8);  // lanes per row
+  static constexpr int R = kAsmThreads / L;                       // rows per workgroup
+  static constexpr int C = kAsmStage / NB;                        // adjacency entries per pass
+  static constexpr int WAVES = NB == 2 ? 1 : 5;  // waves per SIMD asked of the register allocator
+};
+template <int KIND>
+__global__ __launch_bounds__(kAsmThreads)
+__attribute__((amdgpu_waves_per_eu(AsmShape<KIND>::WAVES))) void k_asm_rows(int nb_eq, ElemArgs a,
+                                                          const int* __restrict__ adj_ptr,
+                                                          const int* __restrict__ adj_el,
+                                                          const int* __restrict__ adj_loc,
+                                                          const int* __restrict__ row_node,
+                                                          const int* __restrict__ row_comp,
+                                                          const uint8_t* __restrict__ lmap,
+                                                          const int* __restrict__ row_ptr,
+                                                          double* __restrict__ val,
+                                                          double* __restrict__ b) {
+  using S = AsmShape<KIND>;
+  constexpr int NB = S::NB, NEE = S::NEE, L = S::L, R = S::R, C = S::C;
+  constexpr int W = (NEE + 3) / 4;              // position words per element row
+  __shared__ double s_row[NEE * kAsmStage];      // element rows of the pass, [j][entry, comp]
+  __shared__ double s_rhs[kAsmStage];
+  __shared__ uint32_t s_pos[kAsmStage * W];      // their column positions (lmap), 4 per word,
+                                                 // padded with 0xFF
+  OFH_LDS_BUDGET(sizeof(double) * kAsmStage * (NEE + 1) + sizeof(uint32_t) * kAsmStage * W);
+  const bool want_mat = val != nullptr, want_rhs = b != nullptr;
+  // consecutive equations have ascending nodes: the workgroup's entries are [t0, t1)
+  const int r0 = blockIdx.x * R;
+  const int r = r0 + threadIdx.x / L, lane = threadIdx.x % L;
+  const int rl = min(r0 + R, nb_eq) - 1;
+  const int t0 = adj_ptr[row_node[r0]], t1 = adj_ptr[row_node[rl] + 1];
+  const bool live = r < nb_eq;
+  int comp = 0, ra = 0, len = 0, ta = 0, tb = 0;
+  if (live) {
+    const int node = row_node[r];
+    comp = row_comp[r];
+    ra = row_ptr[r];
+    len = row_ptr[r + 1] - ra;
+    ta = adj_ptr[node];
+    tb = adj_ptr[node + 1];
+  }
+  if (want_mat)  // Matrix::clear (Matrix_impl.h:164-169) of the in-place nonzeros
+    for (int q = lane + 2 * L; q < len; q += L) val[ra + q] = 0.0;
+  double acc0 = 0.0, acc1 = 0.0, s = 0.0;
+  for (int c0 = t0; c0 < t1; c0 += C) {
+    const int cn = min(C, t1 - c0);
+    __syncthreads();  // the previous pass has been consumed
+#pragma unroll
+    for (int k = 0; k < kAsmStage / kAsmThreads; ++k) {  // unrolled: the loads overlap
+      const int p = threadIdx.x + k * kAsmThreads;
+      if (p >= cn * NB) break;
+      const int t = c0 + p / NB;
+      const int e = adj_el[t], in = adj_loc[t] * NB + p % NB;
+      double row[NEE], rhs = 0.0;
+      const int rc = elem_row<KIND>(a, e, in, want_mat, want_rhs, row, &rhs);
+      if (rc) {
+        report(a.err, e, rc);  // the build fails; the values are not used
+#pragma unroll
+        for (int j = 0; j < NEE; ++j) row[j] = 0.0;
+        rhs = 0.0;
+      }
+      if (want_mat) {
+#pragma unroll
+        for (int j = 0; j < NEE; ++j) s_row[j * kAsmStage + p] = row[j];
+      }
+      s_rhs[p] = rhs;
+    }
+    if (want_mat) {
+      const uint8_t* src = lmap + (size_t)c0 * NB * NEE;
+      if constexpr (NEE % 4 == 0) {  // word-aligned: the plan's map starts on 256 bytes
+        const uint32_t* src4 = reinterpret_cast<const uint32_t*>(src);
+        for (int q = threadIdx.x; q < cn * NB * W; q += kAsmThreads) s_pos[q] = src4[q];
+      } else {
+        for (int q = threadIdx.x; q < cn * NB * W; q += kAsmThreads) {
+          const int pe = q / W, j0 = 4 * (q % W);
+          uint32_t x = 0xFFFFFFFFu;
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (j0 + k < NEE)
+              x = (x & ~(0xFFu << (8 * k))) | ((uint32_t)src[pe * NEE + j0 + k] << (8 * k));
+          s_pos[q] = x;
+        }
       }
     }
-    s += rhs;
+    __syncthreads();
+    const int lo = max(ta, c0), hi = min(tb, c0 + cn);
+    for (int t = lo; t < hi; ++t) {
+      const int p = (t - c0) * NB + comp;
+      if (want_mat) {
+        acc0 += pick<W>(s_pos + p * W, lane, s_row + p);
+        if (len > L) acc1 += pick<W>(s_pos + p * W, lane + L, s_row + p);
+        if (len > 2 * L)
+          for (int q = lane + 2 * L; q < len; q += L) val[ra + q] += pick<W>(s_pos + p * W, q, s_row + p);
+      }
+      s += s_rhs[p];
+    }
   }
-  if (val && lds)
-    for (int q = 0; q < len; ++q) val[ra + q] = acc[q * kAsmBlock];
-  if (b) b[r] = s;
+  if (!live) return;
+  if (want_mat) {
+    if (lane < len) val[ra + lane] = acc0;
+    if (lane + L < len) val[ra + lane + L] = acc1;
+  }
+  if (want_rhs && lane == 0) b[r] = s;
 }
 
 // Elas2DQ4 side tractions (Equa_Solid.h:194-202, Elas2DQ4::BoundaryRHS
@@ -662,10 +746,10 @@ int ofh_asm_build(ofh_asm P, int kind, const double* d_coef, const double* d_f,
   ofh_csr A = P->A;
   double* val = want_matrix ? A->val : nullptr;
   if (!val && !d_b) return OFH_OK;
-  const dim3 grid((P->nb_eq + kAsmBlock - 1) / kAsmBlock), blk(kAsmBlock);
 #define OFH_ASM_ROWS(K)                                                                       \
-  hipLaunchKernelGGL(k_asm_rows<K>, grid, blk, 0, s, P->nb_eq, a, P->adj_ptr, P->adj_el,      \
-                     P->adj_loc, P->row_node, P->row_comp, P->lmap, A->row_ptr, val, d_b)
+  hipLaunchKernelGGL(k_asm_rows<K>, dim3((P->nb_eq + AsmShape<K>::R - 1) / AsmShape<K>::R),     \
+                     dim3(kAsmThreads), 0, s, P->nb_eq, a, P->adj_ptr, P->adj_el, P->adj_loc,   \
+                     P->row_node, P->row_comp, P->lmap, A->row_ptr, val, d_b)
   switch (kind) {
     case OFH_LAPLACE2DT3: OFH_ASM_ROWS(OFH_LAPLACE2DT3); break;
     case OFH_DC2DT3: OFH_ASM_ROWS(OFH_DC2DT3); break;

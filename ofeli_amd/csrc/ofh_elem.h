@@ -36,17 +36,17 @@ __device__ __forceinline__ int tri_shape(const double (&x)[6], double& area, dou
 }
 
 __device__ __forceinline__ int tet_shape(const double (&x)[12], double& vol, double (&d)[12]) {
-  // Jacobian J(r,c) = sum_k dshl_k.c * x_k.r with reference-element
-  // derivatives (-1,-1,-1), (1,0,0), (0,1,0), (0,0,1); accumulated from 0
-  // over k = 0..3 as the reference does.
-  const double L[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  // Jacobian J(r,c) = 0 + sum_k dshl_k.c * x_k.r over k = 0..3 with
+  // reference-element derivatives (-1,-1,-1), (1,0,0), (0,1,0), (0,0,1), as
+  // the reference accumulates it.  For finite coordinates the products by
+  // 0 are +-0 and leave the sum unchanged (a sum that starts at +0 is never
+  // -0 in round-to-nearest, and adding +-0 to it is exact), so
+  // J(r,c) = (0 - x_0.r) + x_{c+1}.r bit for bit.
+  double J[3][3];
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
+  for (int r = 0; r < 3; ++r)
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) J[r][c] += L[k][c] * x[3 * k + r];
+    for (int c = 0; c < 3; ++c) J[r][c] = (0.0 - x[r]) + x[3 * (c + 1) + r];
   double IJ[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {  // cofactors, Tetra4.cpp:97-102 (0-based)
@@ -59,11 +59,16 @@ __device__ __forceinline__ int tet_shape(const double (&x)[12], double& vol, dou
   if (det < 0.0) return -1;
   if (det == 0.0) return -2;
   const double c = 1. / det;
+  // d_i.c = c * sum_m IJ(m,c) * dshl_i.m.  Node 0: c * ((-IJ0c - IJ1c) - IJ2c).
+  // Nodes 1..3 pick one cofactor; the dropped products by 0 can change only
+  // the sign of a zero d, and every element-matrix and right-hand-side entry
+  // built from d enters sums that start at +0, where that sign cannot show.
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    d[3 * i + 0] = c * (IJ[0][0] * L[i][0] + IJ[1][0] * L[i][1] + IJ[2][0] * L[i][2]);
-    d[3 * i + 1] = c * (IJ[0][1] * L[i][0] + IJ[1][1] * L[i][1] + IJ[2][1] * L[i][2]);
-    d[3 * i + 2] = c * (IJ[0][2] * L[i][0] + IJ[1][2] * L[i][1] + IJ[2][2] * L[i][2]);
+  for (int m = 0; m < 3; ++m) {
+    d[m] = c * ((-IJ[0][m] + -IJ[1][m]) + -IJ[2][m]);
+    d[3 * 1 + m] = c * IJ[0][m];
+    d[3 * 2 + m] = c * IJ[1][m];
+    d[3 * 3 + m] = c * IJ[2][m];
   }
   vol = kSixth * det;
   return 0;

@@ -74,6 +74,92 @@ def test_p1_tetra(ctx, with_k):
                bc=rng.standard_normal(m.nb_nodes))
 
 
+def test_p1_tetra_jittered(ctx):
+    """Kuhn mesh of [-1, 1.3] x [-0.7, 0.9] x [-1.1, 0.2] with every interior
+    node moved by up to 0.15 of a cell: general tetrahedra, coordinates of
+    both signs, bitwise."""
+    m = O.mesh3d_kuhn(7, 6, 5, box=(-1.0, 1.3, -0.7, 0.9, -1.1, 0.2))
+    rng = np.random.default_rng(21)
+    h = np.array([2.3 / 7, 1.6 / 6, 1.3 / 5])
+    inner = m.code[:, 0] == 0
+    m.coord[inner] += 0.15 * h * rng.uniform(-1.0, 1.0, (int(inner.sum()), 3))
+    k = np.exp(rng.standard_normal(m.nb_elements))
+    check_plan(ctx, m, O.LAPLACE3DT4, f=rng.standard_normal(m.nb_nodes), coef=k,
+               bc=rng.standard_normal(m.nb_nodes))
+
+
+def _oriented(coord, elem):
+    """Swap the last two nodes of every element with a negative Jacobian."""
+    e = elem.copy()
+    x = coord[e - 1]
+    if e.shape[1] == 3:
+        det = (x[:, 1, 0] - x[:, 0, 0]) * (x[:, 2, 1] - x[:, 0, 1]) - \
+              (x[:, 1, 1] - x[:, 0, 1]) * (x[:, 2, 0] - x[:, 0, 0])
+    else:
+        det = np.linalg.det(x[:, 1:, :] - x[:, :1, :])
+    neg = det < 0
+    e[neg, -2], e[neg, -1] = elem[neg, -1], elem[neg, -2]
+    return e
+
+
+def _fan2d(centre, n, r=1.0):
+    """n triangles around a centre node: the centre row holds n + 1 nonzeros."""
+    t = 2 * np.pi * (np.arange(n) + 0.3) / n
+    coord = np.vstack([centre, np.c_[centre[0] + r * np.cos(t), centre[1] + r * np.sin(t)]])
+    elem = np.array([[1, 2 + k, 2 + (k + 1) % n] for k in range(n)], np.int64)
+    return coord, elem
+
+
+def _double_cone(centre, n, r=1.0, hgt=0.8):
+    """2n tetrahedra around a centre node between a ring of n nodes and two
+    apexes: the centre row holds n + 3 nonzeros, each apex row n + 2."""
+    t = 2 * np.pi * (np.arange(n) + 0.2) / n
+    ring = np.c_[centre[0] + r * np.cos(t), centre[1] + r * np.sin(t), np.full(n, centre[2])]
+    coord = np.vstack([centre, centre + [0, 0, hgt], centre - [0, 0, 0.9 * hgt], ring])
+    elem = []
+    for k in range(n):
+        a, b = 4 + k, 4 + (k + 1) % n
+        elem += [[1, a, b, 2], [1, b, a, 3]]
+    return coord, np.array(elem, np.int64)
+
+
+def _merge(parts, dim):
+    coords, elems, off = [], [], 0
+    for c, e in parts:
+        coords.append(c)
+        elems.append(e + off)
+        off += c.shape[0]
+    return np.vstack(coords).reshape(-1, dim), np.vstack(elems)
+
+
+@pytest.mark.parametrize("kind", ["t3", "t4"])
+def test_long_rows(ctx, kind):
+    """Rows longer than the lanes a row gets (8 for triangles, 16 for
+    tetrahedra) and longer than twice that, which the assembly sums in place:
+    fans of 5, 12 and 40 triangles, double cones of 6, 20 and 40 tetrahedra.
+    One ring node of each part is fixed; matrix and right-hand side bitwise."""
+    rng = np.random.default_rng(22)
+    if kind == "t3":
+        parts = [_fan2d(np.array([3.0 * i, 0.1 * i]), n, 1.0 + 0.1 * i)
+                 for i, n in enumerate((5, 12, 40))]
+        coord, elem = _merge(parts, 2)
+        okind, coef = O.DC2DT3, np.exp(rng.standard_normal(elem.shape[0]))
+    else:
+        parts = [_double_cone(np.array([3.0 * i, 0.2 * i, -0.1 * i]), n, 1.0 + 0.1 * i)
+                 for i, n in enumerate((6, 20, 40))]
+        coord, elem = _merge(parts, 3)
+        okind, coef = O.LAPLACE3DT4, np.exp(rng.standard_normal(elem.shape[0]))
+    elem = _oriented(coord, elem)
+    code = np.zeros((coord.shape[0], 1), np.int32)
+    code[elem[::7, -1] - 1, 0] = 1
+    m = O.Mesh(coord, code, elem, 1)
+    P, a, b, dof = check_plan(ctx, m, okind, f=rng.standard_normal(m.nb_nodes), coef=coef,
+                              bc=rng.standard_normal(m.nb_nodes))
+    lens = np.diff(P.A.pattern()[0])
+    lanes = 8 if kind == "t3" else 16
+    assert lens.max() > 2 * lanes and ((lens > lanes) & (lens <= 2 * lanes)).any()
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_elas2dq4_cantilever(ctx, mode):
     m = O.mesh2d(12, 4, quad=True, nb_dof=2, codes=(1, 0, 0, 0), box=(0.0, 4.0, 0.0, 1.0))
