@@ -484,8 +484,12 @@ int ofh_csr_destroy(ofh_csr A) {
   hipFree(A->dinv);
   hipFree(A->dinv32);
   hipFree(A->diag_pos);
-  hipFree(A->lvl_rows_fwd);
-  hipFree(A->lvl_rows_bwd);
+  for (int d = 0; d < 2; ++d) {
+    hipFree(A->sw_list[d]);
+    hipFree(A->sw_ilev[d]);
+    hipFree(A->sw_chunks[d]);
+  }
+  hipFree(A->sw_loc);
   hipFree(A->sweep_ctr);
   hipFree(A->ssor_tmp);
   hipFree(A->dilu_tpos);

@@ -267,11 +267,16 @@ struct ofh_csr_s {
   int* diag_pos = nullptr;    // device, n (position of a_ii; lazy)
   bool dinv_valid = false, val32_valid = false;
   uint64_t version = 0;  // bumped whenever the values may have changed
-  // level schedule for the SSOR sweeps (lazy)
-  int* lvl_rows_fwd = nullptr;   // rows of the lower triangle listed by level (-1: padding)
-  int* lvl_rows_bwd = nullptr;   // ... of the upper triangle
-  int nlev_fwd = 0, nlev_bwd = 0, nlist_fwd = 0, nlist_bwd = 0;
-  int* sweep_ctr = nullptr;      // 2 ints: chunk ticket and finished workgroups of a sweep
+  // chunk schedules of the triangular sweeps (lazy), [0] forward (lower
+  // triangle), [1] backward (upper triangle)
+  int* sw_list[2] = {nullptr, nullptr};      // rows listed by level
+  int* sw_ilev[2] = {nullptr, nullptr};      // level of each listed row within its chunk
+  int4* sw_chunks[2] = {nullptr, nullptr};   // per chunk: first list entry, rows, levels
+  int sw_nchunk[2] = {0, 0}, sw_nlev[2] = {0, 0};
+  int sw_par[2] = {1, 1};        // chunks that can run side by side, on average
+  int sw_rows[2] = {0, 0};       // rows per chunk
+  int* sw_loc = nullptr;         // nnz: position of an entry's column in its row's chunk, -1
+  int* sweep_ctr = nullptr;      // 3 ints: chunk ticket, finished workgroups, finished chunks
   double* ssor_tmp = nullptr;    // forward-sweep result of the SSOR / DILU application
   int* dilu_tpos = nullptr;      // nnz: position of a_ji for each lower entry a_ij (-1: none)
   double* dilu_inv = nullptr;    // n: 1/D of the DILU factorization

@@ -16,17 +16,22 @@
 // exactly: every row is computed by one thread with the reference's
 // left-to-right sums, so the result is bitwise the reference's.
 //
-// Schedule (synchronization-free, one launch per sweep): the rows are listed
-// by level (a row's level is one more than the deepest row it reads), each
-// level starting a new wave, so every row reads only rows of earlier waves.
-// A persistent grid takes one-wave chunks of that list through an atomic
-// ticket, so a wave only ever waits for waves that are already running.  The
-// output vector is first filled with a signalling-NaN pattern that no
-// arithmetic produces; a thread waits until the entries it reads no longer
-// read as that pattern (64-bit agent-scope atomic loads: value and readiness
-// in one word, no fences) and publishes its own result with one 64-bit atomic
-// store.  The wait is bounded; a row whose inputs never arrive publishes a
-// NaN, which ends the calling solver loudly.
+// Schedule (one launch per sweep): rows are listed by level (a row's level is
+// one more than the deepest row it reads), and consecutive levels are packed
+// into chunks of at most 1024 rows (a level wider than that is split over
+// several chunks of its own).  A persistent grid takes chunks through an
+// atomic ticket, so a workgroup only ever waits for chunks already taken.
+// A workgroup loads every row of its chunk at once, then runs the chunk's
+// levels in order, one barrier apart: an entry whose column lies in the same
+// chunk is read from LDS, where an earlier level of the chunk left it; an
+// entry from an earlier chunk is read from the output vector, which is first
+// filled with a signalling-NaN pattern that no arithmetic produces: the
+// thread waits until the entry no longer reads as that pattern (64-bit
+// agent-scope atomic loads: value and readiness in one word, no fences).
+// Every result is published with one 64-bit atomic store.  A chain of narrow
+// levels thus costs one barrier per level inside a chunk and one hand-off
+// through L2 per chunk.  The wait is bounded; a row whose inputs never
+// arrive publishes a NaN, which ends the calling solver loudly.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,10 +41,22 @@
 
 namespace ofh {
 
-constexpr int kSweepWg = 64;       // threads (list entries) per chunk of a sweep: one wave
-constexpr int kSweepGridMin = 16;  // persistent workgroups of a sweep: 3 x the mean level
-constexpr int kSweepGridMax = 192; // width in waves, within these bounds
-constexpr int kDepBatch = 8;       // entries of a row polled together
+// Rows of a chunk (one per thread of a workgroup): 512 where levels are
+// narrow, so that a chunk holds several levels and the hand-offs through L2
+// are few; 64 where they are wide (a level spans many chunks, and small
+// chunks keep many workgroups waiting on few rows each).
+constexpr int kChunkNarrow = 512;
+constexpr int kChunkWide = 64;
+constexpr int kWideLevel = 256;  // mean level width from which chunks are kChunkWide rows
+constexpr int kLagNarrow = 1;    // chunks a workgroup may run ahead of the finished ones
+constexpr int kLagWide = 0;      // (0: no throttle) before it polls
+// Entries of a row loaded when its chunk is taken (the lower or upper part
+// of a 2D Q4 elasticity row has up to 9, of a 3D P1 row up to 7), and
+// further entries, loaded and polled together, by chunk size.
+template <int K> constexpr int kPre = K == 64 ? 8 : 12;
+template <int K> constexpr int kDepBatch = K == 64 ? 4 : 8;
+constexpr int kSweepGridMin = 16;  // persistent workgroups of a sweep: at least this many,
+constexpr int kSweepGridMax = 192; // at most this many
 constexpr unsigned long long kPending = 0x7FF4000000005EEDull;  // signalling NaN
 constexpr int kSpinLimit = 1 << 20;  // polls before a row gives up (about a second)
 
@@ -71,134 +88,251 @@ __device__ __forceinline__ void store_pub(double* p, double v) {
                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ __launch_bounds__(kSweepWg) void k_fill_pending(double* __restrict__ out, int n,
-                                                          const int* done) {
+__global__ __launch_bounds__(256) void k_fill_pending(double* __restrict__ out, int n,
+                                                     const int* done) {
   if (done && *done) return;  // the sweep that follows is skipped too
-  const int i = blockIdx.x * kSweepWg + threadIdx.x;
+  const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) reinterpret_cast<unsigned long long*>(out)[i] = kPending;
 }
 
-// One row of a sweep, in the reference's operation order.  The entries it
-// reads are taken kDepBatch at a time: their published values are loaded
-// together and re-polled until none reads as pending, then folded in order.
-// Rows of one wave never read each other (see csr_ensure_levels), so a lane
-// may simply wait.  Per op, with v the published value of the entry's column
-// c and w the entry's value:
-//   kSgsFwd/kDiluFwd/kSgsBwd/kDiluBwd  s = 0, s += w v (ascending c)
-//   kDiluFact  s = a_ii, s -= (w v) a_ci      (a_ci: the transposed entry)
-//   kTDiluFwd  s = in_i, s -= (inv_c v) w     (Prec_impl.h:273-278)
-//   kTDiluBwd  s = in_i, s -= w v, descending c (Prec_impl.h:279-283)
+// Entry k of a row (in the row's processing order) for op OP: its column c
+// (-1: an entry the op skips, or use = false), the LDS slot of that
+// column's value when it lies in the row's chunk (lc >= 0), the entry's
+// value w and, for the factorization and the transposed forward sweep, the
+// second factor wt.  k must be a valid entry even when use is false; every
+// load is issued unconditionally, so the loads of a row's entries overlap.
 template <int OP>
-__device__ __forceinline__ void sweep_row(int i, const SweepArgs& a) {
+__device__ __forceinline__ void sweep_entry(const SweepArgs& a, const int* __restrict__ loc,
+                                            int k, bool use, int& c, int& lc, double& w,
+                                            double& wt) {
+  const int col = a.col[k];
+  const int l = loc[k];
+  w = a.val[k];
+  wt = 0.0;
+  if (OP == kDiluFact) {
+    const int t = a.tpos[k];
+    wt = a.val[t >= 0 ? t : k];
+    use = use && t >= 0;
+  }
+  if (OP == kTDiluFwd) wt = a.inv[col];
+  c = use ? col : -1;
+  lc = use ? l : -1;
+}
+
+// The term entry (w, wt) contributes with column value v: the product the
+// row's sum adds (sweeps) or subtracts (factorization, transposed sweeps).
+template <int OP>
+__device__ __forceinline__ double sweep_term(double w, double wt, double v) {
+  if (OP == kDiluFact) return w * v * wt;
+  if (OP == kTDiluFwd) return wt * v * w;
+  return w * v;
+}
+constexpr bool op_subtracts(int op) { return op == kDiluFact || op == kTDiluFwd || op == kTDiluBwd; }
+
+// Waits until the first P entries of a row that lie in earlier chunks are
+// published and turns each into its term (term[j]); false when the wait
+// gives up.  Run when the chunk is taken, so the chunk's levels read LDS only.
+template <int OP, int P>
+__device__ __forceinline__ bool sweep_remote(const SweepArgs& a, const int (&c)[P],
+                                             const int (&lc)[P], const double (&w)[P],
+                                             const double (&wt)[P], double (&term)[P],
+                                             int& spins) {
+  unsigned long long u[P];
+  for (;;) {
+    bool ready = true;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const bool remote = c[j] >= 0 && lc[j] < 0;
+      u[j] = load_pub(a.out + (remote ? c[j] : 0));
+      ready &= !remote || u[j] != kPending;
+    }
+    if (ready) break;
+    if (++spins > kSpinLimit) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+#pragma unroll
+  for (int j = 0; j < P; ++j) term[j] = sweep_term<OP>(w[j], wt[j], __longlong_as_double(u[j]));
+  return true;
+}
+
+// Waits until the batch's entries from earlier chunks are published (the
+// LDS ones are: they were written before a barrier), then folds the batch,
+// in order, into s.  Per op, with v the value of the entry's column:
+//   kSgsFwd/kDiluFwd/kSgsBwd/kDiluBwd  s += w v (ascending c)
+//   kDiluFact  s -= (w v) wt                 (wt = a_ci, the transposed entry)
+//   kTDiluFwd  s -= (wt v) w                 (wt = inv_c, Prec_impl.h:273-278)
+//   kTDiluBwd  s -= w v, descending c        (Prec_impl.h:279-283)
+template <int OP, int B>
+__device__ __forceinline__ bool sweep_fold(const SweepArgs& a, const double* sval, const int (&c)[B],
+                                           const int (&lc)[B], const double (&w)[B],
+                                           const double (&wt)[B], double& s, int& spins) {
+  unsigned long long u[B];
+  for (;;) {
+    bool ready = true;
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      u[j] = c[j] < 0 ? 0ull
+             : lc[j] >= 0 ? (unsigned long long)__double_as_longlong(sval[lc[j]])
+                          : load_pub(a.out + c[j]);
+      ready &= u[j] != kPending;
+    }
+    if (ready) break;
+    if (++spins > kSpinLimit) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+#pragma unroll
+  for (int j = 0; j < B; ++j) {
+    if (c[j] < 0) continue;
+    const double x = sweep_term<OP>(w[j], wt[j], __longlong_as_double(u[j]));
+    if (op_subtracts(OP))
+      s -= x;
+    else
+      s += x;
+  }
+  return true;
+}
+
+// A persistent grid: each workgroup takes chunks in ticket order until none
+// is left; see the schedule above.  chunks[c] = {first list entry, rows,
+// levels}.  ctr[0]: next chunk ticket, ctr[1]: finished workgroups, ctr[2]:
+// finished chunks; the last
+// workgroup to finish resets both, so every sweep starts from zero.
+template <int OP, int K>
+__global__ __launch_bounds__(K) void k_sweep(int nchunk, const int4* __restrict__ chunks,
+                                                      const int* __restrict__ list,
+                                                      const int* __restrict__ ilev,
+                                                      const int* __restrict__ loc, int* ctr,
+                                                      int lag, SweepArgs a) {
   constexpr bool FWD = op_forward(OP);
   constexpr bool DESC = OP == kTDiluBwd;
-  const int d = a.dpos[i];
-  const int kb = FWD ? a.rp[i] : d + 1;
-  const int ke = FWD ? d : a.rp[i + 1];
-  double s = OP == kDiluFact ? a.val[d] : (OP == kTDiluFwd || OP == kTDiluBwd) ? a.in[i] : 0.0;
-  int spins = 0;
-  bool ok = true;
-  for (int b0 = 0; b0 < ke - kb && ok; b0 += kDepBatch) {
-    int c[kDepBatch];
-    double w[kDepBatch], wt[kDepBatch];
-#pragma unroll
-    for (int j = 0; j < kDepBatch; ++j) {
-      const int k = DESC ? ke - 1 - (b0 + j) : kb + b0 + j;
-      c[j] = -1;
-      if (b0 + j < ke - kb) {
-        if (OP == kDiluFact) {
-          const int t = a.tpos[k];
-          if (t >= 0) {
-            c[j] = a.col[k];
-            wt[j] = a.val[t];
-          }
-        } else {
-          c[j] = a.col[k];
-          if (OP == kTDiluFwd) wt[j] = a.inv[c[j]];
-        }
-        w[j] = a.val[k];
-      }
-    }
-    unsigned long long u[kDepBatch];
-    for (;;) {
-#pragma unroll
-      for (int j = 0; j < kDepBatch; ++j) u[j] = c[j] >= 0 ? load_pub(a.out + c[j]) : 0ull;
-      bool ready = true;
-#pragma unroll
-      for (int j = 0; j < kDepBatch; ++j) ready &= u[j] != kPending;
-      if (ready) break;
-      if (++spins > kSpinLimit) {
-        ok = false;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-#pragma unroll
-    for (int j = 0; j < kDepBatch; ++j) {
-      if (c[j] < 0) continue;
-      const double v = __longlong_as_double(u[j]);
-      if (OP == kDiluFact)
-        s -= w[j] * v * wt[j];
-      else if (OP == kTDiluFwd)
-        s -= wt[j] * v * w[j];
-      else if (OP == kTDiluBwd)
-        s -= w[j] * v;
-      else
-        s += w[j] * v;
-    }
-  }
-  double r;
-  if (OP == kDiluFact)
-    r = 1.0 / s;
-  else if (OP == kSgsFwd)
-    r = (a.in[i] - s) / a.val[d];
-  else if (OP == kDiluFwd)
-    r = a.inv[i] * (a.in[i] - s);
-  else if (OP == kSgsBwd)
-    r = a.in[i] - s / a.val[d];
-  else if (OP == kDiluBwd)
-    r = a.in[i] - a.inv[i] * s;
-  else if (OP == kTDiluFwd)
-    r = s;
-  else
-    r = a.inv[i] * s;
-  store_pub(a.out + i, ok ? r : __builtin_nan(""));
-}
-
-// A persistent grid: each workgroup takes chunks of kSweepWg list entries in
-// ticket order until the list is exhausted.  Few workgroups in flight keep the
-// waiting rows from crowding the memory pipes of the rows being computed.
-// ctr[0]: next chunk ticket, ctr[1]: finished workgroups; the last workgroup
-// to finish resets both, so every sweep starts from zero.
-template <int OP>
-__global__ __launch_bounds__(kSweepWg) void k_sweep(int nlist, const int* __restrict__ order,
-                                                    int* ctr, SweepArgs a) {
+  constexpr int P = kPre<K>, B = kDepBatch<K>;
   if (a.done && *a.done) return;  // read alike by every workgroup: none takes a ticket
-  __shared__ int chunk;
-  const int nchunk = (nlist + kSweepWg - 1) / kSweepWg;
+  __shared__ double sval[K];  // results of the chunk's rows, by position
+  __shared__ int ticket;
+  OFH_LDS_BUDGET(sizeof(double) * K + sizeof(int));
+  const int t = threadIdx.x;
   for (;;) {
-    if (threadIdx.x == 0) chunk = atomicAdd(&ctr[0], 1);
+    if (t == 0) ticket = atomicAdd(&ctr[0], 1);
     __syncthreads();
-    const int c = chunk;
-    __syncthreads();  // chunk is rewritten by the next ticket
-    if (c >= nchunk) break;
-    const int t = c * kSweepWg + threadIdx.x;
-    const int i = t < nlist ? order[t] : -1;  // -1: padding after the last row of a level
-    if (i >= 0) sweep_row<OP>(i, a);
+    const int ch_id = ticket;
+    __syncthreads();  // ticket is rewritten by the next chunk
+    if (ch_id >= nchunk) break;
+    const int4 ch = chunks[ch_id];
+    const int nlev = __builtin_amdgcn_readfirstlane(ch.z);
+    // the row of this thread and its first P entries, loaded at once
+    int lv = -1, i = 0, kb = 0, ne = 0, d = 0;
+    int c[P], lc[P];
+    double w[P], wt[P];
+    double s = 0.0, in_i = 0.0, piv = 0.0;
+    if (t < ch.y) {
+      lv = ilev[ch.x + t];
+      i = list[ch.x + t];
+      d = a.dpos[i];
+      kb = FWD ? a.rp[i] : d + 1;
+      ne = (FWD ? d : a.rp[i + 1]) - kb;
+#pragma unroll
+      for (int j = 0; j < P; ++j) {  // past the row's entries: the diagonal, unused
+        const bool use = j < ne;
+        sweep_entry<OP>(a, loc, !use ? d : DESC ? kb + ne - 1 - j : kb + j, use, c[j], lc[j], w[j],
+                        wt[j]);
+      }
+      if (OP == kDiluFact) s = a.val[d];
+      if (OP == kTDiluFwd || OP == kTDiluBwd) s = a.in[i];
+      if (OP == kSgsFwd || OP == kDiluFwd || OP == kSgsBwd || OP == kDiluBwd) in_i = a.in[i];
+      if (OP == kSgsFwd || OP == kSgsBwd) piv = a.val[d];
+      if (OP == kDiluFwd || OP == kDiluBwd || OP == kTDiluBwd) piv = a.inv[i];
+    }
+    // Throttle: start polling only once all but `lag` of the earlier chunks
+    // are done (ctr[2] counts finished chunks), so that the workgroups far
+    // down the line do not load the lines the running chunk is publishing to.
+    if (lag > 0) {
+      if (t == 0)
+        for (int sp = 0; __hip_atomic_load(&ctr[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                             ch_id - lag && sp < kSpinLimit;
+             ++sp)
+          __builtin_amdgcn_s_sleep(2);
+      __syncthreads();
+    }
+    int spins = 0;
+    double term[P];
+    bool ok = true;
+    if (t < ch.y) ok = sweep_remote<OP, P>(a, c, lc, w, wt, term, spins);
+    // From here on w[j] holds what entry j adds when it is not in LDS: its
+    // term when it comes from an earlier chunk, +0 when it is skipped (a sum
+    // that starts at +0 and only adds is never -0 in round-to-nearest, and
+    // x - (+0) = x, so +0 leaves every sum unchanged).
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+      if (lc[j] < 0) w[j] = c[j] < 0 ? 0.0 : term[j];
+    // Every load above has landed (vmcnt 0; expcnt and lgkmcnt untouched):
+    // without this the compiler may wait for one of them inside the levels,
+    // and a vmcnt wait there also waits for the previous level's publishing
+    // store to reach L2.
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    for (int l = 0; l < nlev; ++l) {
+      if (lv == l) {
+        // the reference's order, local and remote entries mixed
+        double v[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) v[j] = sval[lc[j] >= 0 ? lc[j] : 0];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          const double x = lc[j] >= 0 ? sweep_term<OP>(w[j], wt[j], v[j]) : w[j];
+          if (op_subtracts(OP))
+            s -= x;
+          else
+            s += x;
+        }
+        for (int b0 = P; b0 < ne && ok; b0 += B) {  // rows with more than P entries
+          int cb[B], lb[B];
+          double wb[B], wtb[B];
+#pragma unroll
+          for (int j = 0; j < B; ++j) {
+            const bool use = b0 + j < ne;
+            sweep_entry<OP>(a, loc, !use ? d : DESC ? kb + ne - 1 - (b0 + j) : kb + b0 + j, use,
+                            cb[j], lb[j], wb[j], wtb[j]);
+          }
+          ok = sweep_fold<OP, B>(a, sval, cb, lb, wb, wtb, s, spins);
+        }
+        double r;
+        if (OP == kDiluFact)
+          r = 1.0 / s;
+        else if (OP == kSgsFwd)
+          r = (in_i - s) / piv;
+        else if (OP == kDiluFwd)
+          r = piv * (in_i - s);
+        else if (OP == kSgsBwd)
+          r = in_i - s / piv;
+        else if (OP == kDiluBwd)
+          r = in_i - piv * s;
+        else if (OP == kTDiluFwd)
+          r = s;
+        else
+          r = piv * s;
+        if (!ok) r = __builtin_nan("");
+        sval[t] = r;
+        store_pub(a.out + i, r);
+      }
+      __syncthreads();
+    }
+    if (t == 0) atomicAdd(&ctr[2], 1);
   }
-  if (threadIdx.x == 0 && atomicAdd(&ctr[1], 1) == (int)gridDim.x - 1) {
+  if (t == 0 && atomicAdd(&ctr[1], 1) == (int)gridDim.x - 1) {
     __hip_atomic_store(&ctr[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&ctr[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&ctr[2], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
-// Rows of the lower (forward) and upper (backward) triangles listed by level,
-// computed once per pattern on the host.  Every level starts a new wave (its
-// tail is padded with -1 to a multiple of 64 entries), so the rows of one
-// wave never read each other: a wave waits on earlier waves only, whatever
-// the compiler does with the lanes that are ready first.
+// The chunk schedules of the forward (lower triangle) and backward (upper
+// triangle) sweeps, computed once per pattern on the host: rows listed by
+// level, ascending within a level; chunks of consecutive whole levels up to
+// K rows (kChunkNarrow or kChunkWide, by the mean level width), a wider
+// level split into chunks of its own; per listed
+// row its level within the chunk; per entry a sweep reads, the position of
+// the entry's column in the reading row's chunk (-1: an earlier chunk).
 int csr_ensure_levels(ofh_csr A) {
-  if (A->lvl_rows_fwd) return OFH_OK;
+  if (A->sw_list[0]) return OFH_OK;
   int rc = csr_ensure_dinv(A);  // also locates the diagonal; zero pivot -> error
   if (rc) return rc;
   const int n = A->n;
@@ -211,49 +345,104 @@ int csr_ensure_levels(ofh_csr A) {
   for (int i = 0; i < n; ++i)
     OFH_CHECK(dpos[i] >= 0, OFH_E_ZERO_PIVOT,
               "In SpMatrix::SSORSolve(b,x): Zero pivot detected in row " + std::to_string(i + 1));
-  auto build = [&](bool fwd, std::vector<int>& list) {
+  std::vector<int> loc(A->nnz, -1);
+  std::vector<int> list[2], ilev[2];
+  std::vector<int4> chunks[2];
+  for (int dir = 0; dir < 2; ++dir) {
+    const bool fwd = dir == 0;
+    auto lo = [&](int i) { return fwd ? rp[i] : dpos[i] + 1; };
+    auto hi = [&](int i) { return fwd ? dpos[i] : rp[i + 1]; };
     std::vector<int> lev(n, 0);
     int nl = 0;
-    if (fwd) {
-      for (int i = 0; i < n; ++i) {
-        int l = 0;
-        for (int k = rp[i]; k < dpos[i]; ++k) l = std::max(l, lev[col[k]] + 1);
-        lev[i] = l;
-        nl = std::max(nl, l + 1);
-      }
-    } else {
-      for (int i = n - 1; i >= 0; --i) {
-        int l = 0;
-        for (int k = dpos[i] + 1; k < rp[i + 1]; ++k) l = std::max(l, lev[col[k]] + 1);
-        lev[i] = l;
-        nl = std::max(nl, l + 1);
-      }
+    for (int q = 0; q < n; ++q) {
+      const int i = fwd ? q : n - 1 - q;
+      int l = 0;
+      for (int k = lo(i); k < hi(i); ++k) l = std::max(l, lev[col[k]] + 1);
+      lev[i] = l;
+      nl = std::max(nl, l + 1);
     }
-    std::vector<int64_t> start(nl + 1, 0);  // padded start of each level in the list
-    std::vector<int> width(nl, 0);
-    for (int i = 0; i < n; ++i) width[lev[i]]++;
-    for (int l = 0; l < nl; ++l) start[l + 1] = start[l] + (width[l] + kWave - 1) / kWave * kWave;
-    OFH_CHECK(start[nl] <= INT32_MAX, OFH_E_INVALID, "SSOR/DILU sweep: too many rows");
-    list.assign(start[nl], -1);
-    for (int i = 0; i < n; ++i) list[start[lev[i]]++] = i;
-    return nl;
-  };
-  std::vector<int> rf, rb;
-  A->nlev_fwd = build(true, rf);
-  A->nlev_bwd = build(false, rb);
-  A->nlist_fwd = (int)rf.size();
-  A->nlist_bwd = (int)rb.size();
-  int *d_ctr = nullptr, *d_rf = nullptr, *d_rb = nullptr;
-  OFH_HIP(hipMalloc(&d_ctr, sizeof(int) * 2));
-  OFH_HIP(hipMalloc(&d_rf, sizeof(int) * std::max<size_t>(rf.size(), 1)));
-  OFH_HIP(hipMalloc(&d_rb, sizeof(int) * std::max<size_t>(rb.size(), 1)));
-  OFH_HIP(hipMemsetAsync(d_ctr, 0, sizeof(int) * 2, s));
-  OFH_HIP(hipMemcpyAsync(d_rf, rf.data(), sizeof(int) * rf.size(), hipMemcpyHostToDevice, s));
-  OFH_HIP(hipMemcpyAsync(d_rb, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice, s));
-  OFH_HIP(hipStreamSynchronize(s));  // rf, rb leave scope
+    const int K = (int64_t)n >= (int64_t)kWideLevel * nl ? kChunkWide : kChunkNarrow;
+    A->sw_rows[dir] = K;
+    std::vector<int> start(nl + 1, 0);
+    for (int i = 0; i < n; ++i) start[lev[i] + 1]++;
+    for (int l = 0; l < nl; ++l) start[l + 1] += start[l];
+    std::vector<int>& L = list[dir];
+    L.assign(n, 0);
+    {
+      std::vector<int> pos(start.begin(), start.end() - 1);
+      for (int i = 0; i < n; ++i) L[pos[lev[i]]++] = i;
+    }
+    std::vector<int> chunk_of(n), pos_of(n);
+    std::vector<int>& IL = ilev[dir];
+    IL.assign(n, 0);
+    std::vector<int4>& C = chunks[dir];
+    int first = 0, rows = 0, levs = 0, groups = 0;  // groups: chunks that run one after another
+    auto close = [&]() {
+      if (rows) {
+        C.push_back(make_int4(first, rows, levs, 0));
+        groups++;
+      }
+      first += rows;
+      rows = levs = 0;
+    };
+    for (int l = 0; l < nl; ++l) {
+      const int q0 = start[l], w = start[l + 1] - q0;
+      if (w > K) {  // a wide level: chunks of its own, one level each
+        close();
+        for (int o = 0; o < w; o += K) {
+          const int m = std::min(K, w - o);
+          for (int q = q0 + o; q < q0 + o + m; ++q) {
+            chunk_of[L[q]] = (int)C.size();
+            pos_of[L[q]] = q - q0 - o;
+          }
+          C.push_back(make_int4(q0 + o, m, 1, 0));
+        }
+        first = q0 + w;
+        groups++;  // the chunks of one level run side by side
+        continue;
+      }
+      if (rows + w > K) close();
+      for (int q = q0; q < q0 + w; ++q) {
+        IL[q] = levs;
+        chunk_of[L[q]] = (int)C.size();
+        pos_of[L[q]] = rows + q - q0;
+      }
+      rows += w;
+      levs++;
+    }
+    close();
+    for (int i = 0; i < n; ++i)
+      for (int k = lo(i); k < hi(i); ++k)
+        if (chunk_of[col[k]] == chunk_of[i]) loc[k] = pos_of[col[k]];
+    A->sw_nlev[dir] = nl;
+    A->sw_par[dir] = (int)((C.size() + std::max(groups, 1) - 1) / std::max(groups, 1));
+  }
+  int* d_ctr = nullptr;
+  int* d_loc = nullptr;
+  OFH_HIP(hipMalloc(&d_ctr, sizeof(int) * 3));
+  OFH_HIP(hipMemsetAsync(d_ctr, 0, sizeof(int) * 3, s));
+  OFH_HIP(hipMalloc(&d_loc, sizeof(int) * std::max<int64_t>(A->nnz, 1)));
+  OFH_HIP(hipMemcpyAsync(d_loc, loc.data(), sizeof(int) * A->nnz, hipMemcpyHostToDevice, s));
   A->sweep_ctr = d_ctr;
-  A->lvl_rows_bwd = d_rb;
-  A->lvl_rows_fwd = d_rf;  // set last: marks the level lists complete
+  A->sw_loc = d_loc;
+  int* d_list[2] = {nullptr, nullptr};
+  for (int dir = 0; dir < 2; ++dir) {
+    int* d_ilev = nullptr;
+    int4* d_chunks = nullptr;
+    OFH_HIP(hipMalloc(&d_list[dir], sizeof(int) * std::max(n, 1)));
+    OFH_HIP(hipMalloc(&d_ilev, sizeof(int) * std::max(n, 1)));
+    OFH_HIP(hipMalloc(&d_chunks, sizeof(int4) * std::max<size_t>(chunks[dir].size(), 1)));
+    OFH_HIP(hipMemcpyAsync(d_list[dir], list[dir].data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    OFH_HIP(hipMemcpyAsync(d_ilev, ilev[dir].data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    OFH_HIP(hipMemcpyAsync(d_chunks, chunks[dir].data(), sizeof(int4) * chunks[dir].size(),
+                           hipMemcpyHostToDevice, s));
+    A->sw_ilev[dir] = d_ilev;
+    A->sw_chunks[dir] = d_chunks;
+    A->sw_nchunk[dir] = (int)chunks[dir].size();
+  }
+  OFH_HIP(hipStreamSynchronize(s));  // the host vectors leave scope
+  A->sw_list[1] = d_list[1];
+  A->sw_list[0] = d_list[0];  // set last: marks the schedules complete
   return OFH_OK;
 }
 
@@ -268,21 +457,24 @@ static int sweep(ofh_csr A, SweepArgs a, hipStream_t s) {
   a.col = A->col;
   a.val = A->val;
   a.dpos = A->diag_pos;
-  constexpr bool FWD = op_forward(OP);
-  const int nlist = FWD ? A->nlist_fwd : A->nlist_bwd;
-  hipLaunchKernelGGL(k_fill_pending, dim3((n + kSweepWg - 1) / kSweepWg), dim3(kSweepWg), 0, s,
-                     a.out, n, a.done);
-  // Waves in flight: enough for about three levels.  Every extra wave only
-  // polls, and the polling slows the rows being computed (measured on the
-  // 96^3 P1 Poisson matrix, 282 levels of ~47 waves: 128-192 workgroups
-  // 1.16-1.21 ms per DILU application, 512: 1.95 ms, 1024: 2.99 ms; on 2D
-  // matrices with levels of ~4 waves, 16 workgroups are best).
-  const int nlev = FWD ? A->nlev_fwd : A->nlev_bwd;
-  const int nchunk = (nlist + kSweepWg - 1) / kSweepWg;
-  const int want = std::max(kSweepGridMin, std::min(kSweepGridMax, 3 * ((nchunk + nlev - 1) / nlev)));
-  const int grid = std::min(nchunk, want);
-  hipLaunchKernelGGL(k_sweep<OP>, dim3(grid), dim3(kSweepWg), 0, s,
-                     nlist, FWD ? A->lvl_rows_fwd : A->lvl_rows_bwd, A->sweep_ctr, a);
+  const int dir = op_forward(OP) ? 0 : 1;
+  hipLaunchKernelGGL(k_fill_pending, dim3((n + 255) / 256), dim3(256), 0, s, a.out, n, a.done);
+  // Workgroups in flight.  Wide levels: about three levels' worth of chunks
+  // (every further workgroup only polls, and the polling slows the rows being
+  // computed).  Narrow levels (one chunk at a time on the chain): 8 x, so the
+  // chunks after the running one have loaded their rows when their turn comes.
+  const int nchunk = A->sw_nchunk[dir];
+  const int par = A->sw_par[dir];
+  const int want = A->sw_rows[dir] == kChunkWide ? 3 * par : std::min(64, 8 * par);
+  const int grid = std::min(nchunk, std::max(kSweepGridMin, std::min(kSweepGridMax, want)));
+  if (A->sw_rows[dir] == kChunkWide)
+    hipLaunchKernelGGL((k_sweep<OP, kChunkWide>), dim3(grid), dim3(kChunkWide), 0, s, nchunk,
+                       A->sw_chunks[dir], A->sw_list[dir], A->sw_ilev[dir], A->sw_loc, A->sweep_ctr,
+                       kLagWide, a);
+  else
+    hipLaunchKernelGGL((k_sweep<OP, kChunkNarrow>), dim3(grid), dim3(kChunkNarrow), 0, s, nchunk,
+                       A->sw_chunks[dir], A->sw_list[dir], A->sw_ilev[dir], A->sw_loc, A->sweep_ctr,
+                       kLagNarrow, a);
   OFH_HIP(hipGetLastError());
   return OFH_OK;
 }

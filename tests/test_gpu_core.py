@@ -310,6 +310,46 @@ def test_sweep_precs_bitexact_large(ctx, shape, unsym):
     assert np.array_equal(host(z), O.dilu_solve(rp, ci, a, O.dilu_factorize(rp, ci, a), r))
 
 
+def hub_system(seed):
+    """A 60 x 40 grid Laplacian plus hub rows coupled to 30-45 random other
+    rows (symmetric pattern, diagonally dominant): rows whose lower and upper
+    parts are longer than the entries a sweep loads with its chunk."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    rp, ci, a = grid_system((60, 40), seed)
+    n = len(rp) - 1
+    A = sp.csr_matrix((a, ci - 1, rp), shape=(n, n)).tolil()
+    for h in (7, 500, 1500, 2399):
+        for j in rng.choice(n, size=int(rng.integers(30, 46)), replace=False):
+            if j != h:
+                w = -rng.uniform(0.01, 0.1)
+                A[h, j] = w
+                A[j, h] = w
+    A = A.tocsr()
+    A.setdiag(0.0)
+    A.eliminate_zeros()
+    A = (A + sp.diags(-np.asarray(A.sum(axis=1)).ravel() + 0.1)).tocsr()
+    A.sort_indices()
+    return A.indptr.astype(np.int64), A.indices.astype(np.int64) + 1, A.data.copy()
+
+
+@pytest.mark.parametrize("case", ["chain", "hubs"])
+def test_sweep_precs_chains_and_long_rows(ctx, case):
+    """A tridiagonal chain of 3,000 one-row levels (chunks of many levels, each
+    row reading the previous one through LDS), and hub rows with 30-45
+    entries on each side of the diagonal (entries loaded in batches after
+    the first ones): SSOR and DILU sweeps bitwise the oracle's."""
+    import ofeli_amd as oa
+    rp, ci, a = grid_system((3000,), 13) if case == "chain" else hub_system(14)
+    A = gpu_csr(ctx, rp, ci, a)
+    r = np.random.default_rng(15).standard_normal(A.n)
+    z = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+    oa.ssor_solve(A, dev(r), z)
+    assert np.array_equal(host(z), O.ssor_solve(rp, ci, a, r))
+    oa.prec_apply(A, oa.DILU_PREC, dev(r), z)
+    assert np.array_equal(host(z), O.dilu_solve(rp, ci, a, O.dilu_factorize(rp, ci, a), r))
+
+
 @pytest.mark.parametrize("prec", [1, 2, 3, 4, 5])
 def test_cg_matches_oracle(ctx, prec):
     """CG: same return convention, iteration count within 1, and the GPU and
