@@ -69,6 +69,11 @@ def main():
     ap.add_argument("--base-cases", type=int, default=8,
                     help="config 4 cases solved by the per-case baselines")
     ap.add_argument("--train-cases", type=int, default=32)
+    ap.add_argument("--ssor-cases", type=int, default=1,
+                    help="config 4 cases solved by FP64 CG+SSOR_PREC from zero (about 70 s each)")
+    ap.add_argument("--ssor-proj-cases", type=int, default=8,
+                    help="config 4 cases after the first ones solved by FP64 CG+SSOR_PREC (and "
+                         "CG+DIAG) from the projected start")
     ap.add_argument("--n3", type=int, default=512, help="config 3 grid (n x n cells)")
     ap.add_argument("--t-steps", type=int, default=2000, help="config 3 time steps")
     ap.add_argument("--train-steps", type=int, default=200, help="config 3 steps used for training")
@@ -1034,8 +1039,10 @@ def bench_cantilever(args):
     Per-case FP64 CG from zero (the reference solves each case by itself) is
     timed on the first cases; the whole set is then solved with the start
     vector projected on the previous solutions, FP64 and mixed, and with the
-    learned direction on top (trained on the first cases).  The exact
-    level-scheduled SSOR is timed per application."""
+    learned direction on top (trained on the first cases).  SSOR_PREC, the
+    preconditioner the reference's run names, is timed per application, per
+    CG+SSOR solve from zero (--ssor-cases) and per case from the projected
+    start once the basis holds the first solutions (--ssor-proj-cases)."""
     import torch
     import ofeli_amd as oa
     from ofeli_amd import workloads as W
@@ -1085,13 +1092,13 @@ def bench_cantilever(args):
     cg_ssor_it_ms = (time.perf_counter() - t) * 1e3 / max(inf.iterations, 1)
     log("[bench4] SSOR application %.2f ms, CG+SSOR %.2f ms/iteration" % (ssor_ms, cg_ssor_it_ms))
 
-    def per_case(fn, cases):
+    def per_case(fn, cases, prec=oa.DIAG_PREC):
         its, ms, xs = [], [], []
         for c in cases:
             x = torch.zeros(n, dtype=torch.float64, device="cuda")
             torch.cuda.synchronize()
             t = time.perf_counter()
-            info = fn(A, Bc[c], x, prec=oa.DIAG_PREC, **kw)
+            info = fn(A, Bc[c], x, prec=prec, **kw)
             torch.cuda.synchronize()
             ms.append((time.perf_counter() - t) * 1e3)
             its.append(info.iterations)
@@ -1106,15 +1113,16 @@ def bench_cantilever(args):
     ms32, it32, x32 = per_case(oa.cg_mixed, range(nb))
     log("[bench4] mixed zero start: %.1f ms/case, %.0f it" % (ms32.mean(), it32.mean()))
 
-    def all_cases(mixed, net=None, collect=0, tag=""):
+    def all_cases(mixed, net=None, collect=0, tag="", prec=oa.DIAG_PREC, upto=None):
         proj = oa.ProjectionGuess(A, m=16, minres=True)
         x0 = torch.empty(n, dtype=torch.float64, device="cuda")
         xg = torch.empty_like(x0)
         its, samples, sols = [], [], []
         fn = oa.cg_mixed if mixed else oa.cg
+        nc = ncases if upto is None else min(upto, ncases)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        for c in range(ncases):
+        for c in range(nc):
             b = Bc[c]
             if net is None:
                 proj.guess(b, x0)
@@ -1125,8 +1133,8 @@ def bench_cantilever(args):
                 g_ = x0.clone()
             # a start that passes toler ends the case; any other case is
             # solved to toler/100 and joins the projection basis
-            info = oa.solve_with_projection(A, b, x0, proj, fn=fn, toler=tol, tighten=1e-2,
-                                            max_it=kw["max_it"], guess=False)
+            info = oa.solve_with_projection(A, b, x0, proj, fn=fn, prec=prec, toler=tol,
+                                            tighten=1e-2, max_it=kw["max_it"], guess=False)
             its.append(info.iterations if info.nb_it != 0 else 0)
             if c < collect:
                 samples.append((g_, g_, b, x0.clone()))
@@ -1137,7 +1145,7 @@ def bench_cantilever(args):
         torch.cuda.synchronize()
         dt_ = time.perf_counter() - t
         proj.close()
-        return dt_ * 1e3 / ncases, np.array(its), samples, sols
+        return dt_ * 1e3 / nc, np.array(its), samples, sols
 
     res = {}
     ms, its, samples, sols = all_cases(False, collect=args.train_cases, tag="fp64 proj")
@@ -1169,6 +1177,60 @@ def bench_cantilever(args):
     errs = {"mixed_zero": err(x32, x64)}
     for k, v in res.items():
         errs[k] = err(v[2], x64)
+
+    # SSOR_PREC, the preconditioner the reference's elasticity run names: the
+    # exact sweeps keep the reference's sequential order, so each application
+    # walks the matrix's level chain (8,702 levels per sweep on 2048 x 256)
+    ssor = None
+    if args.ssor_cases or args.ssor_proj_cases:
+        ssor = {"ms_per_application": round(ssor_ms, 2),
+                "cg_ms_per_iteration": round(cg_ssor_it_ms, 2)}
+        if args.ssor_cases:
+            mss, its_s, xs_s = per_case(oa.cg, range(min(args.ssor_cases, ncases)), prec=oa.SSOR_PREC)
+            ssor.update({"fp64_zero_ms_per_case": round(float(mss.mean()), 1),
+                         "fp64_zero_iterations": float(its_s.mean()),
+                         "fp64_zero_cases": len(its_s),
+                         "max_rel_err_vs_fp64_diag_zero": err(xs_s, x64[:len(xs_s)])})
+            log("[bench4] fp64 CG+SSOR zero start: %.1f ms/case, %.0f it"
+                % (mss.mean(), its_s.mean()))
+        if args.ssor_proj_cases:
+            # the regime of all but the first few cases: a basis of the first
+            # nb solutions (from the FP64 DIAG projection run), then each of the
+            # next cases from its projected start, SSOR and DIAG alike
+            basis = res["fp64_proj"][2]
+            cases = range(len(basis), min(len(basis) + args.ssor_proj_cases, ncases))
+
+            def proj_run(prec):
+                proj = oa.ProjectionGuess(A, m=16, minres=True)
+                for xb in basis:
+                    proj.update(xb)
+                x0 = torch.empty(n, dtype=torch.float64, device="cuda")
+                its = []
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                for c in cases:
+                    proj.guess(Bc[c], x0)
+                    info = oa.solve_with_projection(A, Bc[c], x0, proj, fn=oa.cg, prec=prec,
+                                                    toler=tol, tighten=1e-2,
+                                                    max_it=kw["max_it"], guess=False)
+                    its.append(info.iterations if info.nb_it != 0 else 0)
+                    log("[bench4]   %s projected case %d: %d it (%.1f s)"
+                        % ("ssor" if prec == oa.SSOR_PREC else "diag", c, its[-1],
+                           time.perf_counter() - t))
+                torch.cuda.synchronize()
+                dt_ = time.perf_counter() - t
+                proj.close()
+                return dt_ * 1e3 / max(len(cases), 1), its
+
+            ms_sp, its_sp = proj_run(oa.SSOR_PREC)
+            ms_dp, its_dp = proj_run(oa.DIAG_PREC)
+            ssor.update({"proj_cases": "%d..%d, basis of the first %d solutions"
+                                       % (cases.start, cases.stop - 1, len(basis)),
+                         "fp64_proj_ms_per_case": round(ms_sp, 1), "fp64_proj_iterations": its_sp,
+                         "diag_fp64_proj_ms_per_case": round(ms_dp, 1),
+                         "diag_fp64_proj_iterations": its_dp})
+            log("[bench4] fp64 CG+SSOR projected, cases %d..%d: %.1f ms/case (DIAG %.1f)"
+                % (cases.start, cases.stop - 1, ms_sp, ms_dp))
 
     cpu = None
     if not args.no_cpu:
@@ -1212,6 +1274,7 @@ def bench_cantilever(args):
         "max_rel_err_vs_fp64_zero_first_cases": errs,
         "ssor_ms_per_application": round(ssor_ms, 2), "cg_ssor_ms_per_iteration": round(cg_ssor_it_ms, 2),
         "assembly_ms": round(asm_ms, 3), "train_seconds": round(train_s, 2), "train_loss": hist,
+        "ssor_prec": ssor,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
