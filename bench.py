@@ -1250,6 +1250,17 @@ def bench_cantilever(args):
                "sample": "%d CG+DIAG iterations of case 0 by the oracle -Ofast on 1 host core, "
                          "%.1f ms/iteration x %.0f GPU iterations" % (iters, per_it * 1e3, it64.mean())}
         log("[bench4] cpu %.1f ms/it" % (per_it * 1e3))
+        if ssor and "fp64_zero_iterations" in ssor:
+            k = max(2, iters // 6)
+            t = time.perf_counter()
+            O.cg(rp.astype(np.int64), col.astype(np.int64) + 1, vals, Bc[0].cpu().numpy(),
+                 prec=O.SSOR, max_it=k, toler=1e-300, fast=True)
+            per_s = (time.perf_counter() - t) / k
+            cpu["ssor"] = {"ms_per_iteration": round(per_s * 1e3, 1),
+                           "ms_per_solve_extrapolated": round(per_s * ssor["fp64_zero_iterations"] * 1e3),
+                           "sample": "%d CG+SSOR iterations of case 0 by the oracle -Ofast on 1 core "
+                                     "x %.0f GPU iterations" % (k, ssor["fp64_zero_iterations"])}
+            log("[bench4] cpu CG+SSOR %.1f ms/it" % (per_s * 1e3))
 
     head = "mixed_proj_learned"
     base_ms = float(ms64.mean())
