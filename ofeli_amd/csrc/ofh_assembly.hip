@@ -197,16 +197,17 @@ __device__ __forceinline__ void report(int* err, int e, int rc) {
 // its right-hand-side entry (*rhs, when want_rhs), computed with the
 // floating-point operations, in the order, of the equation classes' element
 // routines: Laplace2DT3::LHS / BodyRHS, DC2DT3 Diffusion / LCapacity /
-// BodyRHS and the backward-Euler combination, Laplace3DT4 / DC3DT4,
-// Elas2DQ4 Deviator / Dilatation / body force, then updateBC
-// (Equation_impl.h:160-176) and the time term (Equa_Therm.h:203).
+// BodyRHS and the backward-Euler combination, Laplace3DT4 / DC3DT4, then
+// updateBC (Equation_impl.h:160-176) and the time term (Equa_Therm.h:203).
+// (Elas2DQ4: elem_rows_q4.)
 // Returns 0, or -1 / -2 for a negative / null Jacobian.
 template <int KIND>
 __device__ __forceinline__ int elem_row(const ElemArgs& a, int e, int in, bool want_row,
                                         bool want_rhs, double* em_row, double* rhs) {
-  constexpr int NEN = (KIND == OFH_LAPLACE3DT4 || KIND == OFH_ELAS2DQ4) ? 4 : 3;
-  constexpr int NB = KIND == OFH_ELAS2DQ4 ? 2 : 1;
-  constexpr int NEE = NEN * NB;
+  static_assert(KIND != OFH_ELAS2DQ4, "Elas2DQ4 rows come in pairs: elem_rows_q4");
+  constexpr int NEN = KIND == OFH_LAPLACE3DT4 ? 4 : 3;
+  constexpr int NB = 1;
+  constexpr int NEE = NEN;
   constexpr int DIM = KIND == OFH_LAPLACE3DT4 ? 3 : 2;
   int nd[NEN];
 #pragma unroll
@@ -220,33 +221,7 @@ __device__ __forceinline__ int elem_row(const ElemArgs& a, int e, int in, bool w
   double row[NEE];
   double r = 0.0;
 
-  if constexpr (KIND == OFH_ELAS2DQ4) {
-    double dx[4][4], dy[4][4], w[4];
-    const int rc = quad_shape(reinterpret_cast<const double(&)[8]>(x), a.g0, a.g1, a.w0, a.w1, dx,
-                              dy, w);
-    if (rc) return rc;
-    double lambda, G;
-    const int mode = (int)a.p1;
-    elas_moduli(a.coef[e], a.p0, mode, lambda, G);
-    const int i = in / 2, di = in % 2;
-    if (need_row) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int dj = 0; dj < 2; ++dj) row[2 * j + dj] = q4_entry(i, di, j, dj, dx, dy, w, G, lambda, mode);
-    }
-    if (want_rhs && a.f) {  // body force (fx, fy per node) at the Gauss points
-      double sh[4][4];
-      quad_values(a.g0, a.g1, sh);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        double fc = 0.;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) fc += sh[j][k] * a.f[nd[j] * 2 + di];
-        r += w[k] * fc * sh[i][k];
-      }
-    }
-  } else if constexpr (KIND == OFH_LAPLACE3DT4) {
+  if constexpr (KIND == OFH_LAPLACE3DT4) {
     double d[12], meas;
     const int rc = tet_shape(reinterpret_cast<const double(&)[12]>(x), meas, d);
     if (rc) return rc;
@@ -330,6 +305,75 @@ __device__ __forceinline__ int elem_row(const ElemArgs& a, int e, int in, bool w
 #pragma unroll
     for (int j = 0; j < NEE; ++j) em_row[j] = row[j];
   if (want_rhs) *rhs = r;
+  return 0;
+}
+
+// Both rows (2 il and 2 il + 1: the node's two DOFs) of an Elas2DQ4 element
+// matrix and right-hand side, with the element's geometry computed once:
+// each row by the operations of elem_row<OFH_ELAS2DQ4>, in the same order.
+__device__ __forceinline__ int elem_rows_q4(const ElemArgs& a, int e, int il, bool want_row,
+                                            bool want_rhs, double (&row0)[8], double (&row1)[8],
+                                            double& rhs0, double& rhs1) {
+  int nd[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) nd[i] = a.elem[(size_t)e * 4 + i];
+  double x[8];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    x[2 * i] = a.coord[(size_t)nd[i] * 2];
+    x[2 * i + 1] = a.coord[(size_t)nd[i] * 2 + 1];
+  }
+  const bool need_row = want_row || (want_rhs && a.bc);
+  double dx[4][4], dy[4][4], w[4];
+  const int rc = quad_shape(x, a.g0, a.g1, a.w0, a.w1, dx, dy, w);
+  if (rc) return rc;
+  double lambda, G;
+  const int mode = (int)a.p1;
+  elas_moduli(a.coef[e], a.p0, mode, lambda, G);
+  double gx[4], gy[4];  // the node's gradients, picked by node rather than by a register index
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    gx[k] = il == 0 ? dx[0][k] : il == 1 ? dx[1][k] : il == 2 ? dx[2][k] : dx[3][k];
+    gy[k] = il == 0 ? dy[0][k] : il == 1 ? dy[1][k] : il == 2 ? dy[2][k] : dy[3][k];
+  }
+  if (need_row) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int dj = 0; dj < 2; ++dj) {
+        row0[2 * j + dj] = q4_entry_g<0>(gx, gy, j, dj, dx, dy, w, G, lambda, mode);
+        row1[2 * j + dj] = q4_entry_g<1>(gx, gy, j, dj, dx, dy, w, G, lambda, mode);
+      }
+  }
+  double r0 = 0.0, r1 = 0.0;
+  if (want_rhs && a.f) {  // body force (fx, fy per node) at the Gauss points
+    double sh[4][4];
+    quad_values(a.g0, a.g1, sh);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double si = il == 0 ? sh[0][k] : il == 1 ? sh[1][k] : il == 2 ? sh[2][k] : sh[3][k];
+      double fx = 0., fy = 0.;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        fx += sh[j][k] * a.f[nd[j] * 2];
+        fy += sh[j][k] * a.f[nd[j] * 2 + 1];
+      }
+      r0 += w[k] * fx * si;
+      r1 += w[k] * fy * si;
+    }
+  }
+  if (want_rhs && a.bc) {  // updateBC, Equation_impl.h:160-176
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int l = 0; l < 2; ++l)
+        if (a.code[nd[j] * 2 + l] > 0) {
+          r0 -= row0[j * 2 + l] * a.bc[nd[j] * 2 + l];
+          r1 -= row1[j * 2 + l] * a.bc[nd[j] * 2 + l];
+        }
+  }
+  rhs0 = r0;
+  rhs1 = r1;
   return 0;
 }
 
@@ -418,25 +462,49 @@ __attribute__((amdgpu_waves_per_eu(AsmShape<KIND>::WAVES))) void k_asm_rows(int 
   for (int c0 = t0; c0 < t1; c0 += C) {
     const int cn = min(C, t1 - c0);
     __syncthreads();  // the previous pass has been consumed
+    if constexpr (NB == 2) {  // one thread per entry: both DOF rows from one geometry
+      for (int q = threadIdx.x; q < cn; q += kAsmThreads) {
+        const int t = c0 + q;
+        const int e = adj_el[t];
+        double row0[NEE], row1[NEE], rhs0 = 0.0, rhs1 = 0.0;
+        const int rc = elem_rows_q4(a, e, adj_loc[t], want_mat, want_rhs, row0, row1, rhs0, rhs1);
+        if (rc) {
+          report(a.err, e, rc);  // the build fails; the values are not used
 #pragma unroll
-    for (int k = 0; k < kAsmStage / kAsmThreads; ++k) {  // unrolled: the loads overlap
-      const int p = threadIdx.x + k * kAsmThreads;
-      if (p >= cn * NB) break;
-      const int t = c0 + p / NB;
-      const int e = adj_el[t], in = adj_loc[t] * NB + p % NB;
-      double row[NEE], rhs = 0.0;
-      const int rc = elem_row<KIND>(a, e, in, want_mat, want_rhs, row, &rhs);
-      if (rc) {
-        report(a.err, e, rc);  // the build fails; the values are not used
+          for (int j = 0; j < NEE; ++j) row0[j] = row1[j] = 0.0;
+          rhs0 = rhs1 = 0.0;
+        }
+        if (want_mat) {
 #pragma unroll
-        for (int j = 0; j < NEE; ++j) row[j] = 0.0;
-        rhs = 0.0;
+          for (int j = 0; j < NEE; ++j) {
+            s_row[j * kAsmStage + 2 * q] = row0[j];
+            s_row[j * kAsmStage + 2 * q + 1] = row1[j];
+          }
+        }
+        s_rhs[2 * q] = rhs0;
+        s_rhs[2 * q + 1] = rhs1;
       }
-      if (want_mat) {
+    } else {
 #pragma unroll
-        for (int j = 0; j < NEE; ++j) s_row[j * kAsmStage + p] = row[j];
+      for (int k = 0; k < kAsmStage / kAsmThreads; ++k) {  // unrolled: the loads overlap
+        const int p = threadIdx.x + k * kAsmThreads;
+        if (p >= cn) break;
+        const int t = c0 + p;
+        const int e = adj_el[t];
+        double row[NEE], rhs = 0.0;
+        const int rc = elem_row<KIND>(a, e, adj_loc[t], want_mat, want_rhs, row, &rhs);
+        if (rc) {
+          report(a.err, e, rc);  // the build fails; the values are not used
+#pragma unroll
+          for (int j = 0; j < NEE; ++j) row[j] = 0.0;
+          rhs = 0.0;
+        }
+        if (want_mat) {
+#pragma unroll
+          for (int j = 0; j < NEE; ++j) s_row[j * kAsmStage + p] = row[j];
+        }
+        s_rhs[p] = rhs;
       }
-      s_rhs[p] = rhs;
     }
     if (want_mat) {
       const uint8_t* src = lmap + (size_t)c0 * NB * NEE;

@@ -168,6 +168,40 @@ __device__ __forceinline__ double q4_entry(int i, int di, int j, int dj, const d
   return (0.0 + dev) + acc;
 }
 
+// q4_entry with the row's node gradients already picked (gx[k] = dx[i][k],
+// gy[k] = dy[i][k]) and its DOF component DI fixed: the same operations.
+template <int DI>
+__device__ __forceinline__ double q4_entry_g(const double (&gx)[4], const double (&gy)[4], int j,
+                                             int dj, const double (&dx)[4][4],
+                                             const double (&dy)[4][4], const double (&w)[4],
+                                             double G, double lambda, int mode) {
+  double dev = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double c = G * w[k] * 1.0;
+    const double ax = c * gx[k], ay = c * gy[k], bx = dx[j][k], by = dy[j][k];
+    double t;
+    if (DI == 0 && dj == 0) t = 2 * ax * bx + ay * by;
+    else if (DI == 0) t = ay * bx;
+    else if (dj == 0) t = ax * by;
+    else t = 2 * ay * by + ax * bx;
+    dev += t;
+  }
+  double acc = mode == 0 ? dev : 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double c = lambda * w[k] * 1.0;
+    const double ax = c * gx[k], ay = c * gy[k], bx = dx[j][k], by = dy[j][k];
+    double t;
+    if (DI == 0 && dj == 0) t = ax * bx;
+    else if (DI == 0) t = ax * by;
+    else if (dj == 0) t = ay * bx;
+    else t = ay * by;
+    acc += t;
+  }
+  return (0.0 + dev) + acc;
+}
+
 __device__ __forceinline__ void elas_moduli(double E, double nu, int mode, double& lambda,
                                             double& G) {
   double l = nu * E / ((1 + nu) * (1 - 2 * nu));
