@@ -15,7 +15,9 @@
 // (its true-residual reduction per inner iteration is below a fifth of the
 // best cycle's, or the recursive residual needed over 3x the first cycle's
 // length: FP32 no longer resolves the slow modes and the host forced the
-// replacement), the FP64 CG finishes from the current x64.
+// replacement), or the very first cycle had to be forced (2,000 inner
+// iterations without a kDelta drop), the FP64 CG finishes from the current
+// x64.
 // Per inner iteration two kernels, k_mx_upd and k_mx_dirspmv, when the
 // product reads the SELL-64 copy: there q = A p is carried by recurrence,
 // q_new = A z + beta q_old, so the product gathers z alone and runs before
@@ -769,7 +771,9 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     slow = (forced || rate < 0.2 * best_rate) ? slow + 1 : 0;
     it_repl = h.it;
     last_true = std::min(last_true, h.res);
-    if (stagnant >= 3 || slow >= 2) {
+    // a first cycle that had to be forced (FP32 never lowered the recursive
+    // residual by kDelta within the cap) ends the FP32 phase at once
+    if (stagnant >= 3 || slow >= 2 || (forced && first_cycle == 0)) {
       fallback = 1;
       break;
     }
