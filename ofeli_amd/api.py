@@ -137,6 +137,12 @@ class CsrMatrix:
         check(lib().ofh_csr_sell_info(self.h, C.byref(u), C.byref(st), C.byref(cb)))
         return {"in_use": bool(u.value), "stored": st.value, "col_bytes": cb.value}
 
+    def dia_info(self):
+        """The diagonal copy of the FP32 values: {"in_use", "diagonals", "row_floats"}."""
+        u, nd, w = C.c_int(), C.c_int(), C.c_int()
+        check(lib().ofh_csr_dia_info(self.h, C.byref(u), C.byref(nd), C.byref(w)))
+        return {"in_use": bool(u.value), "diagonals": nd.value, "row_floats": w.value}
+
     def pattern(self):
         rp = np.zeros(self.n + 1, np.int32)
         cl = np.zeros(max(self.nnz, 1), np.int32)

@@ -294,6 +294,17 @@ struct ofh_csr_s {
   double* sell_val = nullptr;    // padding: -0.0
   float* sell_val32 = nullptr;
   uint64_t sell_version = ~0ull, sell_version32 = ~0ull;
+  // Diagonal copy of the FP32 values (lazy), for patterns whose entries lie
+  // on at most kDiaMaxDiag diagonals (structured-mesh numberings): row i
+  // holds W floats, slot d the value at column i + dia_off[d] (offsets
+  // ascending), slot W-1 the bit mask of the slots present in the row;
+  // W = 4, 8 or 16, the smallest that holds the diagonals and the mask
+  int dia_state = 0;             // 0: not built, 1: in use, -1: not applicable
+  int dia_nd = 0;                // diagonals
+  int dia_w = 0;                 // floats per row: 4, 8 or 16
+  int dia_off[16] = {};          // column - row of each diagonal (unused slots 0)
+  float* dia_val32 = nullptr;    // n * dia_w
+  uint64_t dia_version32 = ~0ull;
   ofh_csr_s* tr = nullptr;       // transpose (BiCG, TMult, TransSolve; lazy)
   int* tr_perm = nullptr;        // nnz: position in A of each entry of tr
   uint64_t tr_version = ~0ull;   // values version tr->val was gathered from
@@ -334,7 +345,7 @@ int ctx_graph(ofh_ctx ctx, const GraphKey& key, const std::function<int(hipStrea
 // Key of a captured solver chunk over A: the sequence id and kernel variant,
 // the matrix's process-unique serial (a matrix created later at the same
 // address never matches), the solution and work buffers, the value and
-// pivot arrays it reads, every CSR array and the SELL-64 copy's arrays.
+// pivot arrays it reads, every CSR array, the SELL-64 copy's arrays and the diagonal copy.
 inline void csr_graph_key(ofh_csr A, uintptr_t kind, uintptr_t variant, const void* x,
                           const void* work, const void* val, const void* dinv, GraphKey* key) {
   const uintptr_t v[GraphKey::kN] = {
@@ -342,7 +353,8 @@ inline void csr_graph_key(ofh_csr A, uintptr_t kind, uintptr_t variant, const vo
       ((uintptr_t)(uint32_t)A->n << 32) | (uint32_t)A->nblocks, (uintptr_t)val,
       (uintptr_t)dinv, (uintptr_t)A->col16, (uintptr_t)A->col, (uintptr_t)A->row_blocks,
       (uintptr_t)A->row_ptr, (uintptr_t)A->sell_cptr, (uintptr_t)A->sell_col16,
-      (uintptr_t)A->sell_col ^ ((uintptr_t)A->sell_val << 1) ^ (uintptr_t)A->sell_val32,
+      (uintptr_t)A->sell_col ^ ((uintptr_t)A->sell_val << 1) ^ (uintptr_t)A->sell_val32 ^
+          ((uintptr_t)A->dia_val32 << 2),
       (uintptr_t)kGraphIters};
   for (int i = 0; i < GraphKey::kN; ++i) key->v[i] = v[i];
 }
@@ -367,6 +379,10 @@ int launch_dilu_trans(ofh_csr A, const double* r, double* z, const int* d_done, 
 // the pattern pads too much (or 16-bit columns are on) and the CSR-stream
 // kernels are to be used.
 int csr_ensure_sell(ofh_csr A, bool fp32, bool* ok);
+// The diagonal copy with current FP32 values; *ok is false when the pattern
+// has more than kDiaMaxDiag diagonals, a row whose columns do not ascend,
+// or would take more bytes than the SELL-64 copy with 16-bit columns.
+int csr_ensure_dia(ofh_csr A, bool* ok);
 // A->tr, the CSR transpose of A, with values current (built on first use).
 int csr_ensure_transpose(ofh_csr A);
 // Everything derived from the values (FP32 copy, pivots, version) is stale.

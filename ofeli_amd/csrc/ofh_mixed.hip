@@ -18,10 +18,11 @@
 // replacement), or the very first cycle had to be forced (2,000 inner
 // iterations without a kDelta drop), the FP64 CG finishes from the current
 // x64.
-// Per inner iteration two kernels, k_mx_upd and k_mx_dirspmv, when the
-// product reads the SELL-64 copy: there q = A p is carried by recurrence,
-// q_new = A z + beta q_old, so the product gathers z alone and runs before
-// the direction's decision.  Without the SELL copy, L2-resident systems
+// Per inner iteration two kernels, k_mx_upd and k_mx_dirspmv[_dia], when the
+// product reads the diagonal copy (structured-mesh patterns) or the SELL-64
+// copy: there q = A p is carried by recurrence, q_new = A z + beta q_old, so
+// the product gathers z alone and runs before the direction's decision.
+// Without either copy, L2-resident systems
 // (n <= 393,216) form the direction at the product's gathers and larger ones
 // use three kernels: k_mx_spmv, k_mx_upd, k_mx_dir.
 #include <hip/hip_runtime.h>
@@ -281,46 +282,14 @@ struct SellF32 {
   const float* __restrict__ val;
 };
 
-// k_mx_dir<PREC, PHASE> fused with the next SpMV, for L2-resident systems:
-// the same decision (and, on pause or stop, the same early return), then
-// q32 = A32 p_new with p_new = z + beta p_old formed at the gather; p_new of
-// the block's rows is stored; (p,q) -> slot 6.  z32 holds D^-1 r (DIAG) or
-// r (IDENT); p_old and p_new are ping-pong buffers.
-// SELL: the product from the SELL-64 copy (256 rows per workgroup, one per
-// lane), else from the CSR-stream partition.  QREC (SELL only): the product
-// by recurrence, q_new = A z + beta q_old (= A p_new in exact arithmetic, as
-// q_old = A p_old), so the gather reads z alone and p is never gathered; the
-// rounding drift of q against A p lasts one cycle: the launch after each FP64
-// residual replacement (PHASE 2) uses the gathering form, q = A p_new.
-template <int PREC, int PHASE, bool SELL, bool QREC>
-__global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* __restrict__ rb,
-                                                      const int* __restrict__ rp, ColIndex ci,
-                                                      const float* __restrict__ val32,
-                                                      int k_arg, int nparts,
-                                                      const float* __restrict__ z32,
-                                                      float* __restrict__ p0,
-                                                      float* __restrict__ p1,
-                                                      float* __restrict__ q32, KrylovState* st,
-                                                      double* __restrict__ partials, SellF32 sa) {
-  if (PHASE == 0 ? st->halt : st->done) return;
-  const int k = k_arg >= 0 ? k_arg : st->kcur;  // k_arg < 0: index from the state
-  const float* __restrict__ p_old = (k & 1) ? p1 : p0;
-  float* __restrict__ p_new = (k & 1) ? p0 : p1;
-  __shared__ double red[4 * kBlock / kWave];
-  // QREC: A z does not depend on the decision below, so the product runs
-  // first, with the update's partials already loading (prefetch) when few
-  float az = 0.0f;
-  int qrow = 0;
-  const bool pre = QREC && PHASE == 0 && nparts <= PartialsPrefetch<kBlock, 2>::kMax;
-  PartialsPrefetch<kBlock, 2> pf;
-  if constexpr (SELL && QREC) {
-    const int slot[2] = {4, 5};
-    if (pre) pf.load(slot, partials, nparts);
-    qrow = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
-    if (qrow / kSellChunk * kSellChunk < sa.n)  // whole waves past the end skip it
-      az = sell_rowsum_g<float, 4>(qrow, sa.cptr, sa.clen, sa.col, sa.col16, sa.val,
-                                   SellGather<float>{x_rsrc(z32, sa.n)});
-  }
+// The decision of k_mx_dirspmv after `k` inner iterations (PHASE as in
+// k_mx_dir): folds the partials (slots 4, 5 from the prefetch `pf` when
+// `pre`, else slots 0-3 or 4, 5 from memory), records the state (block 0),
+// and returns false when the launch stops or pauses; else beta.
+template <int PHASE, class PF>
+__device__ __forceinline__ bool mx_decide(int k, int nparts, bool pre, const PF& pf,
+                                          const double* __restrict__ partials, KrylovState* st,
+                                          double* red, double& beta_out) {
   double nrm = st->nrm, res, rz;
   bool stop = false, pause = false;
   int status = 0;
@@ -383,7 +352,52 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* _
       st->beta = beta;
     }
   }
-  if (stop || pause) return;
+  beta_out = beta;
+  return !(stop || pause);
+}
+
+// k_mx_dir<PREC, PHASE> fused with the next SpMV, for L2-resident systems:
+// the same decision (and, on pause or stop, the same early return), then
+// q32 = A32 p_new with p_new = z + beta p_old formed at the gather; p_new of
+// the block's rows is stored; (p,q) -> slot 6.  z32 holds D^-1 r (DIAG) or
+// r (IDENT); p_old and p_new are ping-pong buffers.
+// SELL: the product from the SELL-64 copy (256 rows per workgroup, one per
+// lane), else from the CSR-stream partition.  QREC (SELL only): the product
+// by recurrence, q_new = A z + beta q_old (= A p_new in exact arithmetic, as
+// q_old = A p_old), so the gather reads z alone and p is never gathered; the
+// rounding drift of q against A p lasts one cycle: the launch after each FP64
+// residual replacement (PHASE 2) uses the gathering form, q = A p_new.
+template <int PREC, int PHASE, bool SELL, bool QREC>
+__global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* __restrict__ rb,
+                                                      const int* __restrict__ rp, ColIndex ci,
+                                                      const float* __restrict__ val32,
+                                                      int k_arg, int nparts,
+                                                      const float* __restrict__ z32,
+                                                      float* __restrict__ p0,
+                                                      float* __restrict__ p1,
+                                                      float* __restrict__ q32, KrylovState* st,
+                                                      double* __restrict__ partials, SellF32 sa) {
+  if (PHASE == 0 ? st->halt : st->done) return;
+  const int k = k_arg >= 0 ? k_arg : st->kcur;  // k_arg < 0: index from the state
+  const float* __restrict__ p_old = (k & 1) ? p1 : p0;
+  float* __restrict__ p_new = (k & 1) ? p0 : p1;
+  __shared__ double red[4 * kBlock / kWave];
+  // QREC: A z does not depend on the decision below, so the product runs
+  // first, with the update's partials already loading (prefetch) when few
+  float az = 0.0f;
+  int qrow = 0;
+  const bool pre = QREC && PHASE == 0 && nparts <= PartialsPrefetch<kBlock, 2>::kMax;
+  PartialsPrefetch<kBlock, 2> pf;
+  if constexpr (SELL && QREC) {
+    const int slot[2] = {4, 5};
+    if (pre) pf.load(slot, partials, nparts);
+    qrow = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
+    if (qrow / kSellChunk * kSellChunk < sa.n)  // whole waves past the end skip it
+      az = sell_rowsum_g<float, 4>(qrow, sa.cptr, sa.clen, sa.col, sa.col16, sa.val,
+                                   SellGather<float>{x_rsrc(z32, sa.n)});
+  }
+  double beta;
+  if (!mx_decide<PHASE>(k, nparts, pre, pf, partials, st, red, beta)) return;
   double v[1] = {0.0};
   if constexpr (SELL && QREC) {
     const float bf = (float)beta;
@@ -424,6 +438,56 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* _
       q32[i] = sq;
       v[0] = (double)pi * (double)sq;
     }
+  }
+  block_sum_k<kBlock, 1>(v, red);
+  const int slot[1] = {6};
+  write_partials<1>(v, slot, partials, blockIdx.x);
+}
+
+// k_mx_dirspmv from the diagonal copy (W floats per row): the same decision,
+// direction and (p,q) partials.  QREC: q_new = A z + beta q_old, the product
+// formed before the decision with the update's partials loading beside it;
+// otherwise (the launch after a replacement) q = A p_new with p_new formed at
+// the gathers.  The row sums are those of the CSR rows bit for bit.
+template <int PREC, int PHASE, bool QREC, int W>
+__global__ __launch_bounds__(kBlock) void k_mx_dirspmv_dia(int nblocks, DiaF32 da, int k_arg,
+                                                          int nparts,
+                                                          const float* __restrict__ z32,
+                                                          float* __restrict__ p0,
+                                                          float* __restrict__ p1,
+                                                          float* __restrict__ q32, KrylovState* st,
+                                                          double* __restrict__ partials) {
+  if (PHASE == 0 ? st->halt : st->done) return;
+  const int k = k_arg >= 0 ? k_arg : st->kcur;  // k_arg < 0: index from the state
+  const float* __restrict__ p_old = (k & 1) ? p1 : p0;
+  float* __restrict__ p_new = (k & 1) ? p0 : p1;
+  __shared__ double red[4 * kBlock / kWave];
+  const int n = da.n;
+  const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
+  const bool pre = QREC && PHASE == 0 && nparts <= PartialsPrefetch<kBlock, 2>::kMax;
+  PartialsPrefetch<kBlock, 2> pf;
+  float az = 0.0f;
+  if constexpr (QREC) {
+    const int slot[2] = {4, 5};
+    if (pre) pf.load(slot, partials, nparts);
+    if (row < n) az = dia_rowsum_g<W>(row, da, SellGather<float>{x_rsrc(z32, n)});
+  }
+  double beta;
+  if (!mx_decide<PHASE>(k, nparts, pre, pf, partials, st, red, beta)) return;
+  const float bf = (float)beta;
+  double v[1] = {0.0};
+  if (row < n) {
+    const float zi = z32[row];
+    const float pi = k == 0 ? zi : zi + bf * p_old[row];
+    float qi;
+    if constexpr (QREC)
+      qi = k == 0 ? az : az + bf * q32[row];
+    else
+      qi = dia_rowsum_g<W>(row, da,
+                           SellGatherDirF{x_rsrc(z32, n), x_rsrc(p_old, n), bf, k == 0});
+    p_new[row] = pi;
+    q32[row] = qi;
+    v[0] = (double)pi * (double)qi;
   }
   block_sum_k<kBlock, 1>(v, red);
   const int slot[1] = {6};
@@ -574,11 +638,15 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   // The sweep preconditioners (SSOR, DILU) take the three-kernel form with
   // z = M^-1 r between the update and the direction: k_mx_check (pause
   // decision), the FP64 level-scheduled sweeps of the FP32 residual, k_mx_rz.
-  bool fsell = false;
+  // The product reads the diagonal copy when the pattern has one, else the
+  // SELL-64 copy when its padding is small.
+  bool fdia = false, fsell = false;
   const int nbsf = (n + kBlock - 1) / kBlock;
-  if (!sweep_prec(PREC) && nbsf <= kMaxPartials && (rc = csr_ensure_sell(A, true, &fsell)))
+  if (!sweep_prec(PREC) && nbsf <= kMaxPartials && (rc = csr_ensure_dia(A, &fdia))) return rc;
+  if (!sweep_prec(PREC) && !fdia && nbsf <= kMaxPartials &&
+      (rc = csr_ensure_sell(A, true, &fsell)))
     return rc;
-  const bool qrec = fsell;
+  const bool qrec = fsell || fdia;
   const bool fused = !sweep_prec(PREC) && (n <= kFuseRows || qrec);
   void* wsp;
   const size_t vbytes = sizeof(float) * (size_t)n * 6 + (sweep_prec(PREC) ? sizeof(double) * 2 * (size_t)n : 0);
@@ -598,9 +666,31 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   float* zw = fused && PREC == OFH_DIAG_PREC ? z32 : nullptr;  // z written by resid/upd
   const ColIndex ci{A->col, nullptr};  // FP32 partition: 32-bit columns
   const SellF32 sa{n, A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val32};
-  const int npq_fused = fsell ? nbsf : A->nblocks32;  // partials of the fused product
+  const int npq_fused = qrec ? nbsf : A->nblocks32;  // partials of the fused product
+  const DiaF32 da = fdia ? dia_f32(A) : DiaF32{};
   // pk: the profiling kind of this launch (-1: not timed)
   auto dirspmv = [&](int phase, int k, int nparts, hipStream_t st_, int pk = -1) {
+    if constexpr (!sweep_prec(PREC)) {
+      if (fdia) {
+#define OFH_MX_DIA(PH, QR, W)                                                                    \
+  prof_kernel(ctx, pk, k_mx_dirspmv_dia<PREC, PH, QR, W>, dim3(nbsf), dim3(kBlock), st_, nbsf,   \
+              da, k, nparts, z32, pb[0], pb[1], q32, state_ptr(ctx), ctx->ws.partials)
+#define OFH_MX_DIA_W(W)                       \
+  if (phase == 0) OFH_MX_DIA(0, true, W);     \
+  else if (phase == 1) OFH_MX_DIA(1, true, W); \
+  else OFH_MX_DIA(2, false, W);
+        if (A->dia_w == 4) {
+          OFH_MX_DIA_W(4)
+        } else if (A->dia_w == 8) {
+          OFH_MX_DIA_W(8)
+        } else {
+          OFH_MX_DIA_W(16)
+        }
+#undef OFH_MX_DIA_W
+#undef OFH_MX_DIA
+        return;
+      }
+    }
     const int nbf = fsell ? nbsf : A->nblocks32;
 #define OFH_MX_DIRSPMV(PH, SL, QR)                                                              \
   prof_kernel(ctx, pk, k_mx_dirspmv<PREC, PH, SL, QR>, dim3(nbf), dim3(kBlock), st_, nbf,        \
@@ -698,7 +788,9 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   if (ctx->use_graphs && !sweep_prec(PREC)) {
     GraphKey key;
     csr_graph_key(A, 16 * (fused ? 3 : 4) + PREC,
-                  (uintptr_t)fsell | (uintptr_t)qrec << 1 | (uintptr_t)sell << 2, x, wsp,
+                  (uintptr_t)fsell | (uintptr_t)qrec << 1 | (uintptr_t)sell << 2 |
+                      (uintptr_t)fdia << 3,
+                  x, wsp,
                   A->val32, A->dinv32, &key);
     const bool prof = ctx->prof;
     ctx->prof = false;  // no event records inside a captured graph

@@ -191,6 +191,48 @@ __device__ __forceinline__ T sell_rowsum(int row, const int* __restrict__ cptr,
   return sell_rowsum_g<T, U>(row, cptr, clen, col, col16, val, SellGather<T>{xr});
 }
 
+// ---------------------------------------------------------------------------
+// Diagonal copy (FP32): row i stores W floats -- slot d the value of the
+// entry at column i + off[d] (offsets ascending, so slot order is column
+// order), slot W-1 the bit mask of the slots the row really has.  A lane
+// loads its row with W/4 16-byte loads and every x[i + off[d]] at once: the
+// addresses need no column stream, so all loads of a row are in flight
+// together, and across a wave each x load reads 64 consecutive entries.
+// The sum adds the present entries left to right from +0.0, which is the CSR
+// row sum bit for bit; x is read through a buffer descriptor, so a column
+// outside [0, n) of an absent slot reads +0.0 and is masked out anyway.
+// ---------------------------------------------------------------------------
+constexpr int kDiaMaxDiag = 15;
+
+struct DiaF32 {
+  int n;
+  const float4* __restrict__ v;  // n * W / 4
+  int off[kDiaMaxDiag];          // unused slots 0
+};
+
+// The diagonal copy of A as the kernels take it (host side).
+DiaF32 dia_f32(ofh_csr A);
+
+template <int W, class G>
+__device__ __forceinline__ float dia_rowsum_g(int row, const DiaF32& da, const G& g) {
+  static_assert(W == 4 || W == 8 || W == 16, "4, 8 or 16 floats per row");
+  float4 w[W / 4];
+#pragma unroll
+  for (int j = 0; j < W / 4; ++j) w[j] = da.v[(size_t)row * (W / 4) + j];
+  float xv[W - 1];
+#pragma unroll
+  for (int d = 0; d < W - 1; ++d) xv[d] = g((uint32_t)(row + da.off[d]));
+  const float* a = reinterpret_cast<const float*>(w);
+  const uint32_t m = __float_as_uint(a[W - 1]);
+  float s = 0.0f;
+#pragma unroll
+  for (int d = 0; d < W - 1; ++d) {
+    const float t = s + a[d] * xv[d];
+    s = (m >> d & 1u) ? t : s;
+  }
+  return s;
+}
+
 template <typename T, int TILE, bool SCALED = false>
 __device__ __forceinline__ T csr_stream_rowsum(int r0, int r1, const int* __restrict__ rp,
                                                ColIndex ci, const T* __restrict__ val,

@@ -156,6 +156,126 @@ int csr_ensure_sell(ofh_csr A, bool fp32, bool* ok) {
   return OFH_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Diagonal copy (FP32)
+// ---------------------------------------------------------------------------
+struct DiaOffsets {
+  int nd;
+  int off[kDiaMaxDiag];
+};
+
+// thread = row: the row's values into their diagonal slots, the mask last
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_dia_vals(int n, const int* __restrict__ rp,
+                                                     const int* __restrict__ col,
+                                                     const double* __restrict__ val, DiaOffsets o,
+                                                     float4* __restrict__ out) {
+  const int row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= n) return;
+  float r[W];
+#pragma unroll
+  for (int d = 0; d < W - 1; ++d) r[d] = -0.0f;
+  uint32_t m = 0;
+  for (int k = rp[row]; k < rp[row + 1]; ++k) {
+    const int c = col[k] - row;
+    const float v = (float)val[k];
+#pragma unroll
+    for (int d = 0; d < W - 1; ++d)
+      if (d < o.nd && c == o.off[d]) {
+        r[d] = v;
+        m |= 1u << d;
+      }
+  }
+  r[W - 1] = __uint_as_float(m);
+#pragma unroll
+  for (int j = 0; j < W / 4; ++j)
+    out[(size_t)row * (W / 4) + j] = make_float4(r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3]);
+}
+
+int csr_ensure_dia(ofh_csr A, bool* ok) {
+  *ok = false;
+  if (A->dia_state < 0) return OFH_OK;
+  hipStream_t s = A->ctx->stream;
+  const int n = A->n;
+  if (A->dia_state == 0) {
+    std::vector<int> rp(n + 1), col((size_t)std::max<int64_t>(A->nnz, 1));
+    OFH_HIP(hipMemcpyAsync(rp.data(), A->row_ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, s));
+    OFH_HIP(hipMemcpyAsync(col.data(), A->col, sizeof(int) * A->nnz, hipMemcpyDeviceToHost, s));
+    OFH_HIP(hipStreamSynchronize(s));
+    // distinct column - row offsets, kept sorted; within a row the columns
+    // must ascend, so its offsets are met in ascending order and one cursor
+    // walks the sorted list
+    std::vector<int> offs;
+    bool fits = A->nnz > 0;
+    for (int i = 0; i < n && fits; ++i) {
+      size_t j = 0;
+      for (int k = rp[i]; k < rp[i + 1]; ++k) {
+        if (k > rp[i] && col[k] <= col[k - 1]) {
+          fits = false;
+          break;
+        }
+        const int o = col[k] - i;
+        while (j < offs.size() && offs[j] < o) ++j;
+        if (j == offs.size() || offs[j] != o) {
+          if ((int)offs.size() == kDiaMaxDiag) {
+            fits = false;
+            break;
+          }
+          offs.insert(offs.begin() + j, o);
+        }
+        ++j;
+      }
+    }
+    const int nd = (int)offs.size();
+    const int w = nd <= 3 ? 4 : nd <= 7 ? 8 : 16;
+    // no more bytes than the SELL-64 copy's FP32 values and 16-bit columns
+    if (!fits || (int64_t)4 * w * n > 6 * A->nnz) {
+      A->dia_state = -1;
+      return OFH_OK;
+    }
+    OFH_HIP(hipMalloc(&A->dia_val32, sizeof(float) * (size_t)w * n));
+    A->dia_nd = nd;
+    A->dia_w = w;
+    for (int d = 0; d < 16; ++d) A->dia_off[d] = d < nd ? offs[d] : 0;
+    A->dia_version32 = ~0ull;
+    A->dia_state = 1;
+  }
+  if (A->dia_version32 != A->version) {
+    DiaOffsets o{A->dia_nd, {}};
+    for (int d = 0; d < kDiaMaxDiag; ++d) o.off[d] = A->dia_off[d];
+    const int grid = (n + kBlock - 1) / kBlock;
+    float4* out = reinterpret_cast<float4*>(A->dia_val32);
+    if (A->dia_w == 4)
+      hipLaunchKernelGGL(k_dia_vals<4>, dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
+                         A->val, o, out);
+    else if (A->dia_w == 8)
+      hipLaunchKernelGGL(k_dia_vals<8>, dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
+                         A->val, o, out);
+    else
+      hipLaunchKernelGGL(k_dia_vals<16>, dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
+                         A->val, o, out);
+    OFH_HIP(hipGetLastError());
+    A->dia_version32 = A->version;
+  }
+  *ok = true;
+  return OFH_OK;
+}
+
+DiaF32 dia_f32(ofh_csr A) {
+  DiaF32 d{A->n, reinterpret_cast<const float4*>(A->dia_val32), {}};
+  for (int k = 0; k < kDiaMaxDiag; ++k) d.off[k] = A->dia_off[k];
+  return d;
+}
+
+// y = A x from the diagonal copy (FP32)
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_spmv_dia(int nblocks, DiaF32 da, const float* x,
+                                                     float* __restrict__ y) {
+  const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
+  if (row >= da.n) return;
+  y[row] = dia_rowsum_g<W>(row, da, SellGather<float>{x_rsrc(x, da.n)});
+}
+
 // y = A x from the SELL-64 copy
 template <typename T, int U>
 __global__ __launch_bounds__(kBlock) void k_spmv_sell(int n, int nblocks,
@@ -204,6 +324,20 @@ static int launch(ofh_csr A, const T* val, const T* x, T* y, T alpha, hipStream_
 
 }  // namespace ofh
 
+extern "C" int ofh_csr_dia_info(ofh_csr A, int* in_use, int* diagonals, int* row_floats) {
+  using namespace ofh;
+  OFH_CHECK(A && in_use && diagonals && row_floats, OFH_E_INVALID,
+            "ofh_csr_dia_info: null argument");
+  bool ok = false;
+  const int rc = csr_ensure_dia(A, &ok);
+  if (rc) return rc;
+  OFH_HIP(hipStreamSynchronize(A->ctx->stream));
+  *in_use = ok ? 1 : 0;
+  *diagonals = ok ? A->dia_nd : 0;
+  *row_floats = ok ? A->dia_w : 0;
+  return OFH_OK;
+}
+
 extern "C" int ofh_csr_sell_info(ofh_csr A, int* in_use, int64_t* stored, int* col_bytes) {
   using namespace ofh;
   OFH_CHECK(A && in_use && stored && col_bytes, OFH_E_INVALID, "ofh_csr_sell_info: null argument");
@@ -234,9 +368,23 @@ int launch_spmv64(ofh_csr A, const double* x, double* y, hipStream_t s) {
 }
 
 int launch_spmv32(ofh_csr A, const float* x, float* y, hipStream_t s) {
-  bool sell;
-  int rc = csr_ensure_sell(A, true, &sell);
+  bool dia;
+  int rc = csr_ensure_dia(A, &dia);
   if (rc) return rc;
+  if (dia) {
+    const int nb = (A->n + kBlock - 1) / kBlock;
+    const DiaF32 da = dia_f32(A);
+    if (A->dia_w == 4)
+      prof_kernel(A->ctx, OFH_PROF_SPMV32, k_spmv_dia<4>, dim3(nb), dim3(kBlock), s, nb, da, x, y);
+    else if (A->dia_w == 8)
+      prof_kernel(A->ctx, OFH_PROF_SPMV32, k_spmv_dia<8>, dim3(nb), dim3(kBlock), s, nb, da, x, y);
+    else
+      prof_kernel(A->ctx, OFH_PROF_SPMV32, k_spmv_dia<16>, dim3(nb), dim3(kBlock), s, nb, da, x, y);
+    OFH_HIP(hipGetLastError());
+    return OFH_OK;
+  }
+  bool sell;
+  if ((rc = csr_ensure_sell(A, true, &sell))) return rc;
   if (sell) {
     const int nb = (A->n + kBlock - 1) / kBlock;
     prof_kernel(A->ctx, OFH_PROF_SPMV32, k_spmv_sell<float, 4>, dim3(nb), dim3(kBlock), s, A->n,

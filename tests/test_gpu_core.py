@@ -101,6 +101,72 @@ def test_spmv_sell_bitexact_and_value_changes(ctx, shape):
     assert np.array_equal(y32.cpu().numpy(), ref)
 
 
+def seq_f32(rp, ci, a32, x32):
+    """Sequential float32 row sums (the CSR row order, one rounding per step)."""
+    ref = np.zeros(len(rp) - 1, np.float32)
+    with np.errstate(invalid="ignore", over="ignore"):
+        for i in range(len(rp) - 1):
+            s = np.float32(0.0)
+            for k in range(rp[i], rp[i + 1]):
+                s = np.float32(s + np.float32(a32[k] * x32[ci[k] - 1]))
+            ref[i] = s
+    return ref
+
+
+def tridiag_system(n=5000, seed=7):
+    rng = np.random.default_rng(seed)
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < n:
+                rows.append(i)
+                cols.append(j)
+                vals.append(2.5 if j == i else -rng.uniform(0.5, 1.0))
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, np.asarray(rows) + 1, 1)
+    return np.cumsum(rp), np.asarray(cols, np.int64) + 1, np.asarray(vals)
+
+
+@pytest.mark.parametrize("case", ["grid3d", "tridiag"])
+def test_spmv_f32_diagonal_copy_bitexact(ctx, case):
+    """ofh_spmv_f32 from the diagonal copy (7 diagonals of a 3D grid: 8 floats
+    per row; a tridiagonal matrix: 4) bitwise the sequential float32 CSR row
+    sums, after a change of the values (the copy is gathered again) and with
+    non-finite x entries: a row's absent diagonals are masked out, so an
+    inf or nan only reaches the rows that really couple to it."""
+    import ofeli_amd as oa
+    torch = _torch()
+    rp, ci, a = grid_system((28, 28, 28), 31) if case == "grid3d" else tridiag_system()
+    A = gpu_csr(ctx, rp, ci, a)
+    want = {"grid3d": (7, 8), "tridiag": (3, 4)}[case]
+    assert A.dia_info() == {"in_use": True, "diagonals": want[0], "row_floats": want[1]}
+    rng = np.random.default_rng(33)
+    x32 = rng.standard_normal(A.n).astype(np.float32)
+    y32 = torch.zeros(A.n, dtype=torch.float32, device="cuda")
+    for vals in (a, a * (1.0 + 0.5 * np.cos(np.arange(len(a))))):
+        if vals is not a:
+            A.set_values(vals)
+        oa.spmv_f32(A, torch.as_tensor(x32, device="cuda"), y32)
+        assert np.array_equal(y32.cpu().numpy(), seq_f32(rp, ci, vals.astype(np.float32), x32))
+    xs = x32.copy()
+    xs[[0, 5, A.n - 1, A.n // 2]] = [np.inf, np.nan, -np.inf, np.nan]
+    oa.spmv_f32(A, torch.as_tensor(xs, device="cuda"), y32)
+    ref = seq_f32(rp, ci, vals.astype(np.float32), xs)
+    got = y32.cpu().numpy()
+    assert np.isfinite(ref).sum() > A.n - 40
+    assert np.array_equal(got, ref, equal_nan=True)
+
+
+def test_diagonal_copy_not_used_where_it_does_not_pay(ctx):
+    """A 2D 5-point grid (5 diagonals: 8 floats per row against ~30 bytes of
+    SELL-64 entries) and a matrix with far couplings (5 diagonals, but mostly
+    3 entries per row) keep the SELL-64 copy."""
+    rp, ci, a = grid_system((400, 20), 31)
+    assert gpu_csr(ctx, rp, ci, a).dia_info()["in_use"] is False
+    rp, ci, a = far_coupled_system()
+    assert gpu_csr(ctx, rp, ci, a).dia_info()["in_use"] is False
+
+
 def far_coupled_system(n=40000, far=35000):
     """SPD system: a 1D Laplacian plus couplings i <-> i + far.  The far
     columns lie more than 32,767 rows from their 64-row chunk, so the SELL-64

@@ -71,6 +71,27 @@ def test_products_stream_the_sell_copy_with_16bit_columns(system):
     assert system["P"].A.nnz <= info["stored"] <= 1.25 * system["P"].A.nnz
 
 
+def test_fp32_products_read_the_diagonal_copy(system):
+    """The Kuhn numbering of the free nodes is a regular grid: every entry lies
+    on one of 15 diagonals, so ofh_spmv_f32 and the mixed CG's inner product
+    read the diagonal copy (16 floats per row), bitwise the CSR row sums."""
+    import torch
+    import ofeli_amd as oa
+    A = system["P"].A
+    assert A.dia_info() == {"in_use": True, "diagonals": 15, "row_floats": 16}
+    rng = np.random.default_rng(5)
+    x32 = rng.standard_normal(A.n).astype(np.float32)
+    y32 = torch.zeros(A.n, dtype=torch.float32, device="cuda")
+    oa.spmv_f32(A, torch.as_tensor(x32, device="cuda"), y32)
+    rp, ci, a32 = system["rp"], system["ci"], system["a"].astype(np.float32)
+    ref = np.zeros(A.n, np.float32)
+    prod = a32 * x32[ci - 1]
+    for j in range(int(np.diff(rp).max())):  # column j of every row, in order
+        rows = np.nonzero(np.diff(rp) > j)[0]
+        ref[rows] = ref[rows] + prod[rp[rows] + j]
+    assert np.array_equal(y32.cpu().numpy(), ref)
+
+
 def test_fp64_cg_diag_matches_oracle(system):
     import torch
     import ofeli_amd as oa
