@@ -69,6 +69,19 @@ __device__ __forceinline__ float zf(int i, float ri, const float* __restrict__ d
   return PREC == OFH_DIAG_PREC ? ri * d[i] : ri;
 }
 
+// The state scalars a decision reads, loaded together at the start of the
+// kernel (one round trip, overlapped with the kernel's other loads) rather
+// than one dependent load after another where the decision needs them.
+// rho_prev is rho_{k-1} (an unused slot when k == 0).
+struct MxSnap {
+  double nrm, rmax, tol, rho_prev;
+  int max_it;
+};
+
+__device__ __forceinline__ MxSnap mx_snap(const KrylovState* st, int k) {
+  return MxSnap{st->nrm, st->rmax, st->tol, st->rho_buf[(k - 1) & 1], st->max_it};
+}
+
 // FP64 residual r = b - A x64 -> r32 (rounded), x32 = 0; partials
 // (b,b) -> 0, (r,r) -> 1 in FP64, (r32,z32) -> 2, (r32,r32) -> 3.
 template <int PREC, bool SELL>
@@ -116,6 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_dir(int n, int k_arg, int nparts,
                                                   const double* __restrict__ partials) {
   if (PHASE == 0 ? st->halt : st->done) return;
   const int k = k_arg >= 0 ? k_arg : st->kcur;  // k_arg < 0: index from the state
+  const MxSnap sn = mx_snap(st, k);
   __shared__ double red[4 * kBlock / kWave];
   const int base = blockIdx.x * (kBlock * kVec) + threadIdx.x;
   float zv[kVec], pv[kVec];
@@ -127,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_dir(int n, int k_arg, int nparts,
       pv[j] = p32[i];
     }
   }
-  double nrm = st->nrm, res, rz;
+  double nrm = sn.nrm, res, rz;
   bool stop = false, pause = false;
   int status = 0;
   if (PHASE == 0) {
@@ -136,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_dir(int n, int k_arg, int nparts,
     fold_partials<kBlock, 2>(t, slot, partials, nparts, red);
     res = sqrt(t[0]) / nrm;
     rz = t[1];
-    pause = res <= kDelta * st->rmax || res <= st->tol || k >= st->max_it || !(res == res);
+    pause = (res <= kDelta * sn.rmax) | (res <= sn.tol) | (k >= sn.max_it) | !(res == res);
   } else {
     double t[4];
     const int slot[4] = {0, 1, 2, 3};
@@ -147,15 +161,15 @@ __global__ __launch_bounds__(kBlock) void k_mx_dir(int n, int k_arg, int nparts,
     }
     res = sqrt(t[1]) / nrm;
     rz = t[2];
-    if (res <= st->tol) {
+    if (res <= sn.tol) {
       stop = true;
       status = k;
-    } else if (k >= st->max_it || !(res == res)) {
+    } else if (k >= sn.max_it || !(res == res)) {
       stop = true;
-      status = -(st->max_it + 1);
+      status = -(sn.max_it + 1);
     }
   }
-  const double beta = k > 0 ? rz / st->rho_buf[(k - 1) & 1] : 0.0;
+  const double beta = k > 0 ? rz / sn.rho_prev : 0.0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->it = k;
     if (PHASE == 0) {
@@ -289,8 +303,8 @@ struct SellF32 {
 template <int PHASE, class PF>
 __device__ __forceinline__ bool mx_decide(int k, int nparts, bool pre, const PF& pf,
                                           const double* __restrict__ partials, KrylovState* st,
-                                          double* red, double& beta_out) {
-  double nrm = st->nrm, res, rz;
+                                          const MxSnap& sn, double* red, double& beta_out) {
+  double nrm = sn.nrm, res, rz;
   bool stop = false, pause = false;
   int status = 0;
   if (PHASE == 0) {
@@ -302,7 +316,7 @@ __device__ __forceinline__ bool mx_decide(int k, int nparts, bool pre, const PF&
       fold_partials<kBlock, 2>(t, slot, partials, nparts, red);
     res = sqrt(t[0]) / nrm;
     rz = t[1];
-    pause = res <= kDelta * st->rmax || res <= st->tol || k >= st->max_it || !(res == res);
+    pause = (res <= kDelta * sn.rmax) | (res <= sn.tol) | (k >= sn.max_it) | !(res == res);
   } else {
     double t[4];
     const int slot[4] = {0, 1, 2, 3};
@@ -313,15 +327,15 @@ __device__ __forceinline__ bool mx_decide(int k, int nparts, bool pre, const PF&
     }
     res = sqrt(t[1]) / nrm;
     rz = t[2];
-    if (res <= st->tol) {
+    if (res <= sn.tol) {
       stop = true;
       status = k;
-    } else if (k >= st->max_it || !(res == res)) {
+    } else if (k >= sn.max_it || !(res == res)) {
       stop = true;
-      status = -(st->max_it + 1);
+      status = -(sn.max_it + 1);
     }
   }
-  const double beta = k > 0 ? rz / st->rho_buf[(k - 1) & 1] : 0.0;
+  const double beta = k > 0 ? rz / sn.rho_prev : 0.0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->it = k;
     if (PHASE == 0) {
@@ -379,12 +393,13 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* _
                                                       double* __restrict__ partials, SellF32 sa) {
   if (PHASE == 0 ? st->halt : st->done) return;
   const int k = k_arg >= 0 ? k_arg : st->kcur;  // k_arg < 0: index from the state
+  const MxSnap sn = mx_snap(st, k);
   const float* __restrict__ p_old = (k & 1) ? p1 : p0;
   float* __restrict__ p_new = (k & 1) ? p0 : p1;
   __shared__ double red[4 * kBlock / kWave];
   // QREC: A z does not depend on the decision below, so the product runs
   // first, with the update's partials already loading (prefetch) when few
-  float az = 0.0f;
+  float az = 0.0f, zi = 0.0f, po = 0.0f, qo = 0.0f;
   int qrow = 0;
   const bool pre = QREC && PHASE == 0 && nparts <= PartialsPrefetch<kBlock, 2>::kMax;
   PartialsPrefetch<kBlock, 2> pf;
@@ -392,20 +407,24 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* _
     const int slot[2] = {4, 5};
     if (pre) pf.load(slot, partials, nparts);
     qrow = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
+    if (qrow < sa.n) {  // the row's own entries, loaded before the decision
+      zi = z32[qrow];
+      po = p_old[qrow];  // unused when k == 0
+      qo = q32[qrow];
+    }
     if (qrow / kSellChunk * kSellChunk < sa.n)  // whole waves past the end skip it
       az = sell_rowsum_g<float, 4>(qrow, sa.cptr, sa.clen, sa.col, sa.col16, sa.val,
                                    SellGather<float>{x_rsrc(z32, sa.n)});
   }
   double beta;
-  if (!mx_decide<PHASE>(k, nparts, pre, pf, partials, st, red, beta)) return;
+  if (!mx_decide<PHASE>(k, nparts, pre, pf, partials, st, sn, red, beta)) return;
   double v[1] = {0.0};
   if constexpr (SELL && QREC) {
     const float bf = (float)beta;
     const int row = qrow;
     if (row < sa.n) {
-      const float zi = z32[row];
-      const float pi = k == 0 ? zi : zi + bf * p_old[row];
-      const float qi = k == 0 ? az : az + bf * q32[row];
+      const float pi = k == 0 ? zi : zi + bf * po;
+      const float qi = k == 0 ? az : az + bf * qo;
       p_new[row] = pi;
       q32[row] = qi;
       v[0] = (double)pi * (double)qi;
@@ -459,6 +478,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv_dia(int nblocks, DiaF32 d
                                                           double* __restrict__ partials) {
   if (PHASE == 0 ? st->halt : st->done) return;
   const int k = k_arg >= 0 ? k_arg : st->kcur;  // k_arg < 0: index from the state
+  const MxSnap sn = mx_snap(st, k);
   const float* __restrict__ p_old = (k & 1) ? p1 : p0;
   float* __restrict__ p_new = (k & 1) ? p0 : p1;
   __shared__ double red[4 * kBlock / kWave];
@@ -466,22 +486,30 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv_dia(int nblocks, DiaF32 d
   const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
   const bool pre = QREC && PHASE == 0 && nparts <= PartialsPrefetch<kBlock, 2>::kMax;
   PartialsPrefetch<kBlock, 2> pf;
-  float az = 0.0f;
+  // every load of the row is issued before the decision: the partials, the
+  // row's own z, p_old and q_old, the matrix row and its gathers
+  float az = 0.0f, zi = 0.0f, po = 0.0f, qo = 0.0f;
   if constexpr (QREC) {
     const int slot[2] = {4, 5};
     if (pre) pf.load(slot, partials, nparts);
-    if (row < n) az = dia_rowsum_g<W>(row, da, SellGather<float>{x_rsrc(z32, n)});
+  }
+  if (row < n) {
+    zi = z32[row];
+    po = p_old[row];  // unused when k == 0
+    if constexpr (QREC) {
+      qo = q32[row];
+      az = dia_rowsum_g<W>(row, da, SellGather<float>{x_rsrc(z32, n)});
+    }
   }
   double beta;
-  if (!mx_decide<PHASE>(k, nparts, pre, pf, partials, st, red, beta)) return;
+  if (!mx_decide<PHASE>(k, nparts, pre, pf, partials, st, sn, red, beta)) return;
   const float bf = (float)beta;
   double v[1] = {0.0};
   if (row < n) {
-    const float zi = z32[row];
-    const float pi = k == 0 ? zi : zi + bf * p_old[row];
+    const float pi = k == 0 ? zi : zi + bf * po;
     float qi;
     if constexpr (QREC)
-      qi = k == 0 ? az : az + bf * q32[row];
+      qi = k == 0 ? az : az + bf * qo;
     else
       qi = dia_rowsum_g<W>(row, da,
                            SellGatherDirF{x_rsrc(z32, n), x_rsrc(p_old, n), bf, k == 0});
@@ -507,6 +535,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_upd(int n, int k_arg, int nparts_
   if (st->halt) return;
   // k_arg < 0: the iteration after the one the last decision recorded
   const int k = k_arg >= 0 ? k_arg : st->it + 1;
+  const double rho_prev = st->rho_buf[(k - 1) & 1];  // loaded with the vectors
   const float* __restrict__ p32 = (k & 1) ? p1 : p0;
   __shared__ double red[2 * kBlock / kWave];
   // the (p,q) partials load first: their fold (and its barriers) runs while
@@ -533,7 +562,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_upd(int n, int k_arg, int nparts_
     pf.fold(pq, nparts_pq, red);
   else
     fold_partials<kBlock, 1>(pq, sl, partials, nparts_pq, red);
-  const double alpha = st->rho_buf[(k - 1) & 1] / pq[0];
+  const double alpha = rho_prev / pq[0];
   const float af = (float)alpha;
   double v[2] = {0.0, 0.0};
 #pragma unroll
