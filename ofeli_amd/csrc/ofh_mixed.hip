@@ -467,8 +467,9 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* _
 // direction and (p,q) partials.  QREC: q_new = A z + beta q_old, the product
 // formed before the decision with the update's partials loading beside it;
 // otherwise (the launch after a replacement) q = A p_new with p_new formed at
-// the gathers.  The row sums are those of the CSR rows bit for bit.
-template <int PREC, int PHASE, bool QREC, int W>
+// the gathers.  The row sums are those of the CSR rows bit for bit.  A lane
+// owns R rows, kBlock apart (R * kBlock rows per workgroup, one partial each).
+template <int PREC, int PHASE, bool QREC, int W, int R>
 __global__ __launch_bounds__(kBlock) void k_mx_dirspmv_dia(int nblocks, DiaF32 da, int k_arg,
                                                           int nparts,
                                                           const float* __restrict__ z32,
@@ -483,39 +484,48 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv_dia(int nblocks, DiaF32 d
   float* __restrict__ p_new = (k & 1) ? p0 : p1;
   __shared__ double red[4 * kBlock / kWave];
   const int n = da.n;
-  const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
+  const int row0 = xcd_block(blockIdx.x, nblocks) * (kBlock * R) + threadIdx.x;
   const bool pre = QREC && PHASE == 0 && nparts <= PartialsPrefetch<kBlock, 2>::kMax;
   PartialsPrefetch<kBlock, 2> pf;
-  // every load of the row is issued before the decision: the partials, the
-  // row's own z, p_old and q_old, the matrix row and its gathers
-  float az = 0.0f, zi = 0.0f, po = 0.0f, qo = 0.0f;
+  // every load of the rows is issued before the decision: the partials, the
+  // rows' own z, p_old and q_old, the matrix rows and their gathers
+  float az[R], zi[R], po[R], qo[R];
   if constexpr (QREC) {
     const int slot[2] = {4, 5};
     if (pre) pf.load(slot, partials, nparts);
   }
-  if (row < n) {
-    zi = z32[row];
-    po = p_old[row];  // unused when k == 0
-    if constexpr (QREC) {
-      qo = q32[row];
-      az = dia_rowsum_g<W>(row, da, SellGather<float>{x_rsrc(z32, n)});
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int row = row0 + j * kBlock;
+    az[j] = zi[j] = po[j] = qo[j] = 0.0f;
+    if (row < n) {
+      zi[j] = z32[row];
+      po[j] = p_old[row];  // unused when k == 0
+      if constexpr (QREC) {
+        qo[j] = q32[row];
+        az[j] = dia_rowsum_g<W>(row, da, SellGather<float>{x_rsrc(z32, n)});
+      }
     }
   }
   double beta;
   if (!mx_decide<PHASE>(k, nparts, pre, pf, partials, st, sn, red, beta)) return;
   const float bf = (float)beta;
   double v[1] = {0.0};
-  if (row < n) {
-    const float pi = k == 0 ? zi : zi + bf * po;
-    float qi;
-    if constexpr (QREC)
-      qi = k == 0 ? az : az + bf * qo;
-    else
-      qi = dia_rowsum_g<W>(row, da,
-                           SellGatherDirF{x_rsrc(z32, n), x_rsrc(p_old, n), bf, k == 0});
-    p_new[row] = pi;
-    q32[row] = qi;
-    v[0] = (double)pi * (double)qi;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int row = row0 + j * kBlock;
+    if (row < n) {
+      const float pi = k == 0 ? zi[j] : zi[j] + bf * po[j];
+      float qi;
+      if constexpr (QREC)
+        qi = k == 0 ? az[j] : az[j] + bf * qo[j];
+      else
+        qi = dia_rowsum_g<W>(row, da,
+                             SellGatherDirF{x_rsrc(z32, n), x_rsrc(p_old, n), bf, k == 0});
+      p_new[row] = pi;
+      q32[row] = qi;
+      v[0] += (double)pi * (double)qi;
+    }
   }
   block_sum_k<kBlock, 1>(v, red);
   const int slot[1] = {6};
@@ -695,15 +705,19 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   float* zw = fused && PREC == OFH_DIAG_PREC ? z32 : nullptr;  // z written by resid/upd
   const ColIndex ci{A->col, nullptr};  // FP32 partition: 32-bit columns
   const SellF32 sa{n, A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val32};
-  const int npq_fused = qrec ? nbsf : A->nblocks32;  // partials of the fused product
+  // rows per lane of the diagonal product (96^3 mixed solve: 1 row 15.6 ms,
+  // 2 rows 14.7 ms, 4 rows 14.9 ms), and its workgroups (= partials)
+  constexpr int kDiaRows = 2;
+  const int nbd = (n + kBlock * kDiaRows - 1) / (kBlock * kDiaRows);
+  const int npq_fused = fdia ? nbd : qrec ? nbsf : A->nblocks32;  // partials of the fused product
   const DiaF32 da = fdia ? dia_f32(A) : DiaF32{};
   // pk: the profiling kind of this launch (-1: not timed)
   auto dirspmv = [&](int phase, int k, int nparts, hipStream_t st_, int pk = -1) {
     if constexpr (!sweep_prec(PREC)) {
       if (fdia) {
 #define OFH_MX_DIA(PH, QR, W)                                                                    \
-  prof_kernel(ctx, pk, k_mx_dirspmv_dia<PREC, PH, QR, W>, dim3(nbsf), dim3(kBlock), st_, nbsf,   \
-              da, k, nparts, z32, pb[0], pb[1], q32, state_ptr(ctx), ctx->ws.partials)
+  prof_kernel(ctx, pk, k_mx_dirspmv_dia<PREC, PH, QR, W, kDiaRows>, dim3(nbd), dim3(kBlock), st_, \
+              nbd, da, k, nparts, z32, pb[0], pb[1], q32, state_ptr(ctx), ctx->ws.partials)
 #define OFH_MX_DIA_W(W)                       \
   if (phase == 0) OFH_MX_DIA(0, true, W);     \
   else if (phase == 1) OFH_MX_DIA(1, true, W); \
