@@ -305,6 +305,12 @@ struct ofh_csr_s {
   int dia_off[16] = {};          // column - row of each diagonal (unused slots 0)
   float* dia_val32 = nullptr;    // n * dia_w
   uint64_t dia_version32 = ~0ull;
+  // The same copy of the symmetrically scaled matrix D^-1/2 A D^-1/2 (the
+  // mixed CG's FP32 inner operator for DIAG_PREC), and D^-1/2 itself
+  float* dia_val32s = nullptr;   // n * dia_w
+  double* dscale = nullptr;      // n: 1 / sqrt(a_ii)
+  bool dscale_ok = false;        // every a_ii > 0 and finite
+  uint64_t dscale_version = ~0ull;
   ofh_csr_s* tr = nullptr;       // transpose (BiCG, TMult, TransSolve; lazy)
   int* tr_perm = nullptr;        // nnz: position in A of each entry of tr
   uint64_t tr_version = ~0ull;   // values version tr->val was gathered from
@@ -354,7 +360,7 @@ inline void csr_graph_key(ofh_csr A, uintptr_t kind, uintptr_t variant, const vo
       (uintptr_t)dinv, (uintptr_t)A->col16, (uintptr_t)A->col, (uintptr_t)A->row_blocks,
       (uintptr_t)A->row_ptr, (uintptr_t)A->sell_cptr, (uintptr_t)A->sell_col16,
       (uintptr_t)A->sell_col ^ ((uintptr_t)A->sell_val << 1) ^ (uintptr_t)A->sell_val32 ^
-          ((uintptr_t)A->dia_val32 << 2),
+          ((uintptr_t)A->dia_val32 << 2) ^ ((uintptr_t)A->dia_val32s << 3),
       (uintptr_t)kGraphIters};
   for (int i = 0; i < GraphKey::kN; ++i) key->v[i] = v[i];
 }
@@ -383,6 +389,11 @@ int csr_ensure_sell(ofh_csr A, bool fp32, bool* ok);
 // has more than kDiaMaxDiag diagonals, a row whose columns do not ascend,
 // or would take more bytes than the SELL-64 copy with 16-bit columns.
 int csr_ensure_dia(ofh_csr A, bool* ok);
+// The diagonal copy of D^-1/2 A D^-1/2 (values a_ij (s_i s_j), s = D^-1/2,
+// rounded once to FP32) and A->dscale = s in FP64, current with the values;
+// *ok is false when A has no diagonal copy or a diagonal entry is not
+// positive and finite.
+int csr_ensure_dia_scaled(ofh_csr A, bool* ok);
 // A->tr, the CSR transpose of A, with values current (built on first use).
 int csr_ensure_transpose(ofh_csr A);
 // Everything derived from the values (FP32 copy, pivots, version) is stale.

@@ -37,6 +37,11 @@
 namespace ofh {
 
 constexpr double kDelta = 0.1;  // recursive-residual drop that triggers a replacement
+// DIAG_PREC applied as a symmetric scaling: the FP32 inner iteration is CG
+// without preconditioner on D^-1/2 A D^-1/2 (the same iterates as Jacobi-
+// preconditioned CG on A in exact arithmetic) with r32 = D^-1/2 r and
+// x64 += D^-1/2 x32, so z = r and the update reads no pivots and writes no z.
+constexpr int kScaledDiag = 100;
 
 // Inner iterations to enqueue before the host looks at the state again.  The
 // device pauses by itself at the replacement point and the launches queued
@@ -74,21 +79,23 @@ __device__ __forceinline__ float zf(int i, float ri, const float* __restrict__ d
 // than one dependent load after another where the decision needs them.
 // rho_prev is rho_{k-1} (an unused slot when k == 0).
 struct MxSnap {
-  double nrm, rmax, tol, rho_prev;
+  double nrm, rmax, tol, rho_prev, rscale;
   int max_it;
 };
 
 __device__ __forceinline__ MxSnap mx_snap(const KrylovState* st, int k) {
-  return MxSnap{st->nrm, st->rmax, st->tol, st->rho_buf[(k - 1) & 1], st->max_it};
+  return MxSnap{st->nrm, st->rmax, st->tol, st->rho_buf[(k - 1) & 1], st->rscale, st->max_it};
 }
 
 // FP64 residual r = b - A x64 -> r32 (rounded), x32 = 0; partials
 // (b,b) -> 0, (r,r) -> 1 in FP64, (r32,z32) -> 2, (r32,r32) -> 3.
+// (kScaledDiag: r32 = D^-1/2 r, sc = D^-1/2; the FP64 partials stay unscaled.)
 template <int PREC, bool SELL>
 __global__ __launch_bounds__(kBlock) void k_mx_resid(
     MatRows64 m, const double* __restrict__ b, const double* __restrict__ x,
     float* __restrict__ r32, float* __restrict__ x32, const float* __restrict__ dinv32,
-    float* __restrict__ z32, const KrylovState* __restrict__ st, double* __restrict__ partials) {
+    float* __restrict__ z32, const double* __restrict__ sc, const KrylovState* __restrict__ st,
+    double* __restrict__ partials) {
   if (st->done) return;
   __shared__ double red[4 * kBlock / kWave];
   int i;
@@ -97,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_resid(
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (valid) {
     const double bi = b[i], ri = bi - s;
-    const float rf = (float)ri;
+    const float rf = PREC == kScaledDiag ? (float)(sc[i] * ri) : (float)ri;
     r32[i] = rf;
     x32[i] = 0.0f;
     v[0] = bi * bi;
@@ -300,11 +307,11 @@ struct SellF32 {
 // k_mx_dir): folds the partials (slots 4, 5 from the prefetch `pf` when
 // `pre`, else slots 0-3 or 4, 5 from memory), records the state (block 0),
 // and returns false when the launch stops or pauses; else beta.
-template <int PHASE, class PF>
+template <int PHASE, int PREC, class PF>
 __device__ __forceinline__ bool mx_decide(int k, int nparts, bool pre, const PF& pf,
                                           const double* __restrict__ partials, KrylovState* st,
                                           const MxSnap& sn, double* red, double& beta_out) {
-  double nrm = sn.nrm, res, rz;
+  double nrm = sn.nrm, res, rz, rs = 0.0;
   bool stop = false, pause = false;
   int status = 0;
   if (PHASE == 0) {
@@ -314,7 +321,9 @@ __device__ __forceinline__ bool mx_decide(int k, int nparts, bool pre, const PF&
       pf.fold(t, nparts, red);
     else
       fold_partials<kBlock, 2>(t, slot, partials, nparts, red);
-    res = sqrt(t[0]) / nrm;
+    // the recursive residual as a true-relres estimate (scaled: its ratio to
+    // the scaled residual at the last replacement)
+    res = PREC == kScaledDiag ? sqrt(t[0]) * sn.rscale : sqrt(t[0]) / nrm;
     rz = t[1];
     pause = (res <= kDelta * sn.rmax) | (res <= sn.tol) | (k >= sn.max_it) | !(res == res);
   } else {
@@ -327,6 +336,7 @@ __device__ __forceinline__ bool mx_decide(int k, int nparts, bool pre, const PF&
     }
     res = sqrt(t[1]) / nrm;
     rz = t[2];
+    rs = t[3] > 0 ? res / sqrt(t[3]) : 0.0;
     if (res <= sn.tol) {
       stop = true;
       status = k;
@@ -348,6 +358,7 @@ __device__ __forceinline__ bool mx_decide(int k, int nparts, bool pre, const PF&
       st->sconv_res = res;
       st->res = res;
       st->rmax = res;
+      if (PREC == kScaledDiag) st->rscale = rs;
       if (PHASE == 1) {
         st->nrm = nrm;
         st->res0 = res;
@@ -417,7 +428,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* _
                                    SellGather<float>{x_rsrc(z32, sa.n)});
   }
   double beta;
-  if (!mx_decide<PHASE>(k, nparts, pre, pf, partials, st, sn, red, beta)) return;
+  if (!mx_decide<PHASE, PREC>(k, nparts, pre, pf, partials, st, sn, red, beta)) return;
   double v[1] = {0.0};
   if constexpr (SELL && QREC) {
     const float bf = (float)beta;
@@ -508,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv_dia(int nblocks, DiaF32 d
     }
   }
   double beta;
-  if (!mx_decide<PHASE>(k, nparts, pre, pf, partials, st, sn, red, beta)) return;
+  if (!mx_decide<PHASE, PREC>(k, nparts, pre, pf, partials, st, sn, red, beta)) return;
   const float bf = (float)beta;
   double v[1] = {0.0};
 #pragma unroll
@@ -649,12 +660,13 @@ __global__ __launch_bounds__(kBlock) void k_mx_rz(int n, const int* __restrict__
   write_partials<1>(v, slot, partials, blockIdx.x);
 }
 
-// x64 += x32 (the correction accumulated since the last replacement)
+// x64 += x32 (the correction accumulated since the last replacement), or
+// x64 += D^-1/2 x32 on the scaled system (sc = D^-1/2)
 __global__ void k_mx_accum(int n, double* __restrict__ x, const float* __restrict__ x32,
-                           const KrylovState* __restrict__ st) {
+                           const double* __restrict__ sc, const KrylovState* __restrict__ st) {
   if (st->done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] += (double)x32[i];
+  if (i < n) x[i] += sc ? sc[i] * (double)x32[i] : (double)x32[i];
 }
 
 template <int PREC>
@@ -710,7 +722,13 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   constexpr int kDiaRows = 2;
   const int nbd = (n + kBlock * kDiaRows - 1) / (kBlock * kDiaRows);
   const int npq_fused = fdia ? nbd : qrec ? nbsf : A->nblocks32;  // partials of the fused product
-  const DiaF32 da = fdia ? dia_f32(A) : DiaF32{};
+  DiaF32 da = fdia ? dia_f32(A) : DiaF32{};
+  // the scaled system's copy and D^-1/2 (kScaledDiag: the dispatch checked both)
+  const double* sc = PREC == kScaledDiag ? A->dscale : nullptr;
+  if (PREC == kScaledDiag) {
+    if (!fdia || !sc) return fail(OFH_E_INVALID, "ofh_cg_mixed: no scaled diagonal copy");
+    da.v = reinterpret_cast<const float4*>(A->dia_val32s);
+  }
   // pk: the profiling kind of this launch (-1: not timed)
   auto dirspmv = [&](int phase, int k, int nparts, hipStream_t st_, int pk = -1) {
     if constexpr (!sweep_prec(PREC)) {
@@ -779,10 +797,10 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   auto resid = [&]() {
     if (rsell)
       hipLaunchKernelGGL((k_mx_resid<PREC, true>), dim3(nbr), dim3(kBlock), 0, s, mr, b, x, r32,
-                         x32, A->dinv32, zw, st, part);
+                         x32, A->dinv32, zw, sc, st, part);
     else
       hipLaunchKernelGGL((k_mx_resid<PREC, false>), dim3(nbr), dim3(kBlock), 0, s, mr, b, x, r32,
-                         x32, A->dinv32, zw, st, part);
+                         x32, A->dinv32, zw, sc, st, part);
   };
   // z32 = M^-1 r32 by the FP64 sweeps and (r32, z32) -> slot (sweep precs)
   auto sweep_z = [&](const int* skip, int nparts, int slot_id, hipStream_t st_) -> int {
@@ -830,11 +848,11 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   hipGraphExec_t chunk_graph = nullptr;
   if (ctx->use_graphs && !sweep_prec(PREC)) {
     GraphKey key;
-    csr_graph_key(A, 16 * (fused ? 3 : 4) + PREC,
+    csr_graph_key(A, 16 * (fused ? 3 : 4) + (PREC == kScaledDiag ? 15 : PREC),
                   (uintptr_t)fsell | (uintptr_t)qrec << 1 | (uintptr_t)sell << 2 |
                       (uintptr_t)fdia << 3,
                   x, wsp,
-                  A->val32, A->dinv32, &key);
+                  A->val32, PREC == kScaledDiag ? (const void*)A->dscale : A->dinv32, &key);
     const bool prof = ctx->prof;
     ctx->prof = false;  // no event records inside a captured graph
     rc = ctx_graph(ctx, key, [&](hipStream_t cs) {
@@ -882,7 +900,7 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     const int kr = forced ? h.it + 1 : h.it;
     last_cycle = kr - it_repl;
     // residual replacement in FP64 after kr inner iterations
-    hipLaunchKernelGGL(k_mx_accum, dim3(ge), dim3(256), 0, s, n, x, x32, st);
+    hipLaunchKernelGGL(k_mx_accum, dim3(ge), dim3(256), 0, s, n, x, x32, sc, st);
     resid();
     if (sweep_prec(PREC) && (rc = sweep_z(&st->done, nbr, 2, s))) return rc;
     if (fused)
@@ -916,7 +934,8 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   if (fallback) {
     ofh_solve_info sub{};
     const int left = std::max(0, max_it - h.it);
-    if ((rc = ofh_cg(A, PREC, b, x, left, toler, &sub))) return rc;
+    const int prec64 = PREC == kScaledDiag ? OFH_DIAG_PREC : PREC;
+    if ((rc = ofh_cg(A, prec64, b, x, left, toler, &sub))) return rc;
     if (info) {
       *info = sub;
       info->iterations += h.it;
@@ -949,7 +968,17 @@ extern "C" int ofh_cg_mixed(ofh_csr A, int prec, const double* b, double* x, int
   OFH_CHECK(A && b && x && max_it >= 0, OFH_E_INVALID, "ofh_cg_mixed: bad argument");
   switch (prec) {
     case OFH_IDENT_PREC: return mixed_impl<OFH_IDENT_PREC>(A, b, x, max_it, toler, info);
-    case OFH_DIAG_PREC: return mixed_impl<OFH_DIAG_PREC>(A, b, x, max_it, toler, info);
+    case OFH_DIAG_PREC: {
+      // with a diagonal copy and positive pivots: the inner iteration on
+      // D^-1/2 A D^-1/2 (no pivot read, no z written per iteration)
+      bool scaled = false;
+      if (A->n <= kMaxPartials * kBlock) {
+        const int rc = csr_ensure_dia_scaled(A, &scaled);
+        if (rc) return rc;
+      }
+      if (scaled) return mixed_impl<kScaledDiag>(A, b, x, max_it, toler, info);
+      return mixed_impl<OFH_DIAG_PREC>(A, b, x, max_it, toler, info);
+    }
     case OFH_SSOR_PREC: return mixed_impl<OFH_SSOR_PREC>(A, b, x, max_it, toler, info);
     case OFH_DILU_PREC:  // the reference's ILU is its DILU
     case OFH_ILU_PREC: return mixed_impl<OFH_DILU_PREC>(A, b, x, max_it, toler, info);

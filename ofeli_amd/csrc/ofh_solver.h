@@ -25,6 +25,8 @@ struct KrylovState {
   int kcur;  // iteration index of the current update kernel, for kernels
              // launched with it = -1 (captured chunks of iterations)
   int pad[2];
+  double rscale;  // mixed CG on D^-1/2 A D^-1/2: true relres / ||r32|| at the
+                  // last replacement (estimates the relres from the scaled ||r32||)
 };
 static_assert(sizeof(KrylovState) <= 512, "state block");
 

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "ofh_internal.h"
@@ -164,11 +165,13 @@ struct DiaOffsets {
   int off[kDiaMaxDiag];
 };
 
-// thread = row: the row's values into their diagonal slots, the mask last
+// thread = row: the row's values into their diagonal slots, the mask last;
+// with `scale` (D^-1/2) the values of D^-1/2 A D^-1/2, a_ij (s_i s_j)
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_dia_vals(int n, const int* __restrict__ rp,
                                                      const int* __restrict__ col,
                                                      const double* __restrict__ val, DiaOffsets o,
+                                                     const double* __restrict__ scale,
                                                      float4* __restrict__ out) {
   const int row = blockIdx.x * kBlock + threadIdx.x;
   if (row >= n) return;
@@ -176,9 +179,10 @@ __global__ __launch_bounds__(kBlock) void k_dia_vals(int n, const int* __restric
 #pragma unroll
   for (int d = 0; d < W - 1; ++d) r[d] = -0.0f;
   uint32_t m = 0;
+  const double si = scale ? scale[row] : 1.0;
   for (int k = rp[row]; k < rp[row + 1]; ++k) {
     const int c = col[k] - row;
-    const float v = (float)val[k];
+    const float v = scale ? (float)(val[k] * (si * scale[col[k]])) : (float)val[k];
 #pragma unroll
     for (int d = 0; d < W - 1; ++d)
       if (d < o.nd && c == o.off[d]) {
@@ -190,6 +194,27 @@ __global__ __launch_bounds__(kBlock) void k_dia_vals(int n, const int* __restric
 #pragma unroll
   for (int j = 0; j < W / 4; ++j)
     out[(size_t)row * (W / 4) + j] = make_float4(r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3]);
+}
+
+// values of the diagonal copy (scale non-null: of D^-1/2 A D^-1/2) into out
+static int launch_dia_vals(ofh_csr A, const double* scale, float* out_) {
+  DiaOffsets o{A->dia_nd, {}};
+  for (int d = 0; d < kDiaMaxDiag; ++d) o.off[d] = A->dia_off[d];
+  const int n = A->n;
+  const int grid = (n + kBlock - 1) / kBlock;
+  hipStream_t s = A->ctx->stream;
+  float4* out = reinterpret_cast<float4*>(out_);
+  if (A->dia_w == 4)
+    hipLaunchKernelGGL(k_dia_vals<4>, dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
+                       A->val, o, scale, out);
+  else if (A->dia_w == 8)
+    hipLaunchKernelGGL(k_dia_vals<8>, dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
+                       A->val, o, scale, out);
+  else
+    hipLaunchKernelGGL(k_dia_vals<16>, dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
+                       A->val, o, scale, out);
+  OFH_HIP(hipGetLastError());
+  return OFH_OK;
 }
 
 int csr_ensure_dia(ofh_csr A, bool* ok) {
@@ -241,23 +266,53 @@ int csr_ensure_dia(ofh_csr A, bool* ok) {
     A->dia_state = 1;
   }
   if (A->dia_version32 != A->version) {
-    DiaOffsets o{A->dia_nd, {}};
-    for (int d = 0; d < kDiaMaxDiag; ++d) o.off[d] = A->dia_off[d];
-    const int grid = (n + kBlock - 1) / kBlock;
-    float4* out = reinterpret_cast<float4*>(A->dia_val32);
-    if (A->dia_w == 4)
-      hipLaunchKernelGGL(k_dia_vals<4>, dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
-                         A->val, o, out);
-    else if (A->dia_w == 8)
-      hipLaunchKernelGGL(k_dia_vals<8>, dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
-                         A->val, o, out);
-    else
-      hipLaunchKernelGGL(k_dia_vals<16>, dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
-                         A->val, o, out);
-    OFH_HIP(hipGetLastError());
+    const int rc = launch_dia_vals(A, nullptr, A->dia_val32);
+    if (rc) return rc;
     A->dia_version32 = A->version;
   }
   *ok = true;
+  return OFH_OK;
+}
+
+// s_i = 1 / sqrt(a_ii); *bad = 1 when a pivot is not positive and finite
+__global__ void k_dscale(int n, const int* __restrict__ diag_pos, const double* __restrict__ val,
+                         double* __restrict__ s, int* __restrict__ bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int p = diag_pos[i];
+  const double a = p < 0 ? 0.0 : val[p];
+  if (!(a > 0.0) || !(a < INFINITY)) {
+    *bad = 1;  // every writer stores the same value
+    s[i] = 0.0;
+    return;
+  }
+  s[i] = 1.0 / sqrt(a);
+}
+
+int csr_ensure_dia_scaled(ofh_csr A, bool* ok) {
+  *ok = false;
+  bool dia = false;
+  int rc = csr_ensure_dia(A, &dia);
+  if (rc || !dia) return rc;
+  if (A->dscale_version != A->version) {
+    if ((rc = csr_ensure_dinv(A))) return rc;  // diag_pos; a zero pivot fails as in DIAG_PREC
+    hipStream_t s = A->ctx->stream;
+    const int n = A->n;
+    if (!A->dscale) OFH_HIP(hipMalloc(&A->dscale, sizeof(double) * n));
+    if (!A->dia_val32s) OFH_HIP(hipMalloc(&A->dia_val32s, sizeof(float) * (size_t)A->dia_w * n));
+    int* bad = (int*)A->ctx->dev_scratch;
+    OFH_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_dscale, dim3((n + 255) / 256), dim3(256), 0, s, n, A->diag_pos, A->val,
+                       A->dscale, bad);
+    OFH_HIP(hipGetLastError());
+    int hbad = 0;
+    OFH_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    OFH_HIP(hipStreamSynchronize(s));
+    A->dscale_ok = hbad == 0;
+    if (A->dscale_ok && (rc = launch_dia_vals(A, A->dscale, A->dia_val32s))) return rc;
+    A->dscale_version = A->version;
+  }
+  *ok = A->dscale_ok;
   return OFH_OK;
 }
 

@@ -157,6 +157,34 @@ def test_spmv_f32_diagonal_copy_bitexact(ctx, case):
     assert np.array_equal(got, ref, equal_nan=True)
 
 
+def test_mixed_cg_diag_scaled_and_unscaled_inner_iterations(ctx):
+    """Mixed CG with DIAG_PREC on a matrix with a diagonal copy runs its FP32
+    inner iteration on D^-1/2 A D^-1/2 (positive pivots); on -A (negative
+    pivots) it keeps the Jacobi-preconditioned inner iteration on A.  Both
+    reach the FP64 true-residual tolerance and the same solution, and the
+    scaled form needs no more inner iterations than FP64 CG needs by much."""
+    import ofeli_amd as oa
+    torch = _torch()
+    rp, ci, a = grid_system((28, 28, 28), 31)
+    xs = np.cos(np.arange(len(rp) - 1) * 0.01)
+    b = O.spmv(rp, ci, a, xs)
+    out = {}
+    for sign in (1.0, -1.0):
+        A = gpu_csr(ctx, rp, ci, sign * a)
+        assert A.dia_info()["in_use"]
+        x = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+        info = oa.cg_mixed(A, dev(sign * b), x, prec=oa.DIAG_PREC, max_it=5000, toler=1e-10)
+        assert info.nb_it > 0 and info.fallback == 0, info.as_dict()
+        assert info.true_relres <= 1e-10
+        assert rel(host(x), xs) <= 1e-8
+        out[sign] = info.iterations
+    x = torch.zeros(len(xs), dtype=torch.float64, device="cuda")
+    it64 = oa.cg(gpu_csr(ctx, rp, ci, a), dev(b), x, prec=oa.DIAG_PREC, max_it=5000,
+                 toler=1e-10).iterations
+    print("inner iterations: scaled %d, unscaled %d, FP64 CG %d" % (out[1.0], out[-1.0], it64))
+    assert out[1.0] <= 1.1 * it64 + 5
+
+
 def test_diagonal_copy_not_used_where_it_does_not_pay(ctx):
     """A 2D 5-point grid (5 diagonals: 8 floats per row against ~30 bytes of
     SELL-64 entries) and a matrix with far couplings (5 diagonals, but mostly
