@@ -110,13 +110,13 @@ def b_spmv32(n, nnz):
 
 def kernel_bytes(n, nnz):
     """Algorithmic bytes per launch of each kernel of the CG iterations
-    (FP64: B_spmv64 + 88 n per iteration; mixed: B_spmv32 + 44 n)."""
+    (FP64: B_spmv64 + 88 n per iteration; mixed: B_spmv32 + 36 n)."""
     return {
         # mixed, two kernels: product of z with q by recurrence (reads p_old,
-        # q_old, writes p_new, q beside the product), update (reads x, p, r, q,
-        # 1/a_ii; writes x, r, z)
+        # q_old, writes p_new, q beside the product), update on the scaled
+        # system D^-1/2 A D^-1/2 (reads x, p, r, q; writes x, r; z = r)
         "k_mx_dirspmv": b_spmv32(n, nnz) + 12 * n,
-        "k_mx_upd": 32 * n,
+        "k_mx_upd": 24 * n,
         # FP64, three kernels: q = A p, update (reads x, p, r, q, 1/a_ii;
         # writes x, r), direction (reads r, 1/a_ii, p; writes p)
         "k_cg_spmv": b_spmv64(n, nnz),
@@ -383,6 +383,7 @@ def bench_poisson3d(args):
 
     # standalone SpMV, warm and cold
     sell = A.sell_info()
+    dia = A.dia_info()
     warm64, cold64 = spmv_timing(oa, ctx, A, args.spmv_reps, args.cold_reps)
     warm32, _ = spmv_timing(oa, ctx, A, args.spmv_reps, 0, fp32=True)
     B64, B32 = b_spmv64(n, nnz), b_spmv32(n, nnz)
@@ -514,10 +515,19 @@ def bench_poisson3d(args):
             "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": dom["frac"], "traffic": None if traffic is None else traffic["bytes"],
             "traffic_source": None if traffic is None else traffic["source"],
-            "kernel": "k_mx_dirspmv<2,0,true,true> (mixed CG: FP32 SELL-64 product of z, "
-                      "q by recurrence, direction, (p,q) partials)",
+            "kernel": "k_mx_dirspmv_dia (mixed CG inner product: FP32 diagonal copy of "
+                      "D^-1/2 A D^-1/2, %d floats per row, 2 rows per lane; q by recurrence, "
+                      "direction, (p,q) partials)" % dia["row_floats"],
             "bytes_per_launch": dom["bytes_per_launch"], "avg_ms": round(dom["avg_us"] / 1e3, 5),
-            "bytes_definition": "B_spmv32 + 12n = 8 nnz + 4(n+1) + 20n"},
+            "bytes_definition": "B_spmv32 + 12n = 8 nnz + 4(n+1) + 20n (FP32 CSR SpMV + the "
+                                "direction and recurrence vectors)",
+            "own_bytes_per_launch": (4 * dia["row_floats"] + 20) * n,
+            "own_bytes_definition": "diagonal copy 4*%d n + z 4n + p_old, q_old read, p_new, q "
+                                    "written 16n" % dia["row_floats"],
+            "own_frac": round((4 * dia["row_floats"] + 20) * n / (dom["avg_us"] * 1e3)
+                              / HBM_PEAK_GBS, 4)},
+        "fp32_copy": {"diagonal_copy": dia, "inner_operator": "D^-1/2 A D^-1/2 (DIAG_PREC as a "
+                      "symmetric scaling of the FP32 inner iteration)"},
         "roofline_spmv64": {
             "kernel": "k_spmv_sell<double,2> (SELL-64, %d-byte columns, %d stored entries)"
                       % (sell["col_bytes"], sell["stored"]),
@@ -529,7 +539,9 @@ def bench_poisson3d(args):
             "cold_protocol": "each launch after a 1 GiB streaming write and a 1 GiB read",
             "traffic": (pmc_traffic("k_spmv_sell<double,2>", n) or {}).get("bytes"),
             "spmv32_warm_ms": round(float(spmv32_ms), 5),
-            "spmv32_warm_GBs": round(B32 / spmv32_ms / 1e6, 1)},
+            "spmv32_warm_GBs": round(B32 / spmv32_ms / 1e6, 1),
+            "spmv32_kernel": "k_spmv_dia<%d>" % dia["row_floats"] if dia["in_use"]
+                             else "k_spmv_sell<float,4>"},
         "roofline_cold": None if replica is None else dict(
             replica, kernel="k_spmv_sell<double,2>", workload="%d^3 Kuhn replica" % args.replica,
             bound="hbm", peak=HBM_PEAK_GBS, unit="GB/s",

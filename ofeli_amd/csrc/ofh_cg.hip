@@ -94,6 +94,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int n, int it_arg, int nparts
                                                   const double* __restrict__ partials) {
   if (st->done) return;
   const int it = it_arg >= 0 ? it_arg : st->kcur;
+  const CgSnap sn = cg_snap(st, it);
   __shared__ double red[3 * kBlock / kWave];
   // issue this thread's loads first: they are in flight while the partials fold
   const int base = blockIdx.x * (kBlock * kVec) + threadIdx.x;
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int n, int it_arg, int nparts
     double u[1];
     fold_partials<kBlock, 1>(u, slot, partials, nparts, red);  // written by k_cg_rz
     t[2] = u[0];
-    nrm = st->nrm;
+    nrm = sn.nrm;
   } else if (it == 0) {
     const int slot[3] = {0, 1, 2};
     fold_partials<kBlock, 3>(t, slot, partials, nparts0, red);
@@ -126,23 +127,23 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int n, int it_arg, int nparts
     fold_partials<kBlock, 2>(u, slot, partials, nparts, red);
     t[1] = u[0];
     t[2] = u[1];
-    nrm = st->nrm;
+    nrm = sn.nrm;
   }
   bool stop = false;
   int status = 0;
   double res = 0;
   if (!sweep_prec(PREC)) {
     res = sqrt(t[1]) / nrm;
-    if (res <= st->tol) {
+    if (res <= sn.tol) {
       stop = true;
       status = it;  // 0 when the initial guess passes (CG.h:116-123)
-    } else if (it >= st->max_it || !(res == res)) {
+    } else if (it >= sn.max_it || !(res == res)) {
       stop = true;  // the loop counter ends at max_it+1 (CG.h:154-156)
-      status = -(st->max_it + 1);
+      status = -(sn.max_it + 1);
     }
   }
   const double rho = t[2];
-  const double beta = it > 0 ? rho / st->rho_buf[(it - 1) & 1] : 0.0;  // CG.h:131
+  const double beta = it > 0 ? rho / sn.rho_prev : 0.0;  // CG.h:131
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->it = it;
     if (!sweep_prec(PREC)) {
@@ -198,6 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_dirspmv(int nblocks, const int* _
                                                       double* __restrict__ partials) {
   if (st->done) return;
   const int it = it_arg >= 0 ? it_arg : st->kcur;
+  const CgSnap sn = cg_snap(st, it);
   const double* __restrict__ p_old = (it & 1) ? p1 : p0;
   double* __restrict__ p_new = (it & 1) ? p0 : p1;
   __shared__ double lds[kTile64];
@@ -217,20 +219,20 @@ __global__ __launch_bounds__(kBlock) void k_cg_dirspmv(int nblocks, const int* _
     fold_partials<kBlock, 2>(u, slot, partials, nparts, red);
     t[1] = u[0];
     t[2] = u[1];
-    nrm = st->nrm;
+    nrm = sn.nrm;
   }
   bool stop = false;
   int status = 0;
   const double res = sqrt(t[1]) / nrm;
-  if (res <= st->tol) {
+  if (res <= sn.tol) {
     stop = true;
     status = it;  // 0 when the initial guess passes (CG.h:116-123)
-  } else if (it >= st->max_it || !(res == res)) {
+  } else if (it >= sn.max_it || !(res == res)) {
     stop = true;  // the loop counter ends at max_it+1 (CG.h:154-156)
-    status = -(st->max_it + 1);
+    status = -(sn.max_it + 1);
   }
   const double rho = t[2];
-  const double beta = it > 0 ? rho / st->rho_buf[(it - 1) & 1] : 0.0;  // CG.h:131
+  const double beta = it > 0 ? rho / sn.rho_prev : 0.0;  // CG.h:131
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->it = it;
     st->res = res;
@@ -312,6 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_dirspmv_sell(
     double* __restrict__ q, KrylovState* st, double* __restrict__ partials) {
   if (st->done) return;
   const int it = it_arg >= 0 ? it_arg : st->kcur;
+  const CgSnap sn = cg_snap(st, it);
   const double* __restrict__ p_old = (it & 1) ? p1 : p0;
   double* __restrict__ p_new = (it & 1) ? p0 : p1;
   __shared__ double red[3 * kBlock / kWave];
@@ -328,20 +331,20 @@ __global__ __launch_bounds__(kBlock) void k_cg_dirspmv_sell(
     fold_partials<kBlock, 2>(u, slot, partials, nparts, red);
     t[1] = u[0];
     t[2] = u[1];
-    nrm = st->nrm;
+    nrm = sn.nrm;
   }
   bool stop = false;
   int status = 0;
   const double res = sqrt(t[1]) / nrm;
-  if (res <= st->tol) {
+  if (res <= sn.tol) {
     stop = true;
     status = it;
-  } else if (it >= st->max_it || !(res == res)) {
+  } else if (it >= sn.max_it || !(res == res)) {
     stop = true;
-    status = -(st->max_it + 1);
+    status = -(sn.max_it + 1);
   }
   const double rho = t[2];
-  const double beta = it > 0 ? rho / st->rho_buf[(it - 1) & 1] : 0.0;  // CG.h:131
+  const double beta = it > 0 ? rho / sn.rho_prev : 0.0;  // CG.h:131
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->it = it;
     st->res = res;
@@ -416,6 +419,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_upd(int n, int it_arg, int nparts
   if (st->done) return;
   // it_arg < 0: the iteration after the one the last decision recorded
   const int it = it_arg >= 0 ? it_arg : st->it + 1;
+  const double rho_prev = st->rho_buf[(it - 1) & 1];  // loaded with the vectors
   const double* __restrict__ p = (it & 1) ? p1 : p0;
   __shared__ double red[2 * kBlock / kWave];
   // the (p,q) partials load first: their fold (and its barriers) runs while
@@ -442,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_upd(int n, int it_arg, int nparts
     pf.fold(pq, nparts_pq, red);
   else
     fold_partials<kBlock, 1>(pq, sl, partials, nparts_pq, red);
-  const double alpha = st->rho_buf[(it - 1) & 1] / pq[0];
+  const double alpha = rho_prev / pq[0];
   double v[2] = {0.0, 0.0};
 #pragma unroll
   for (int j = 0; j < kVec; ++j) {

@@ -30,6 +30,19 @@ struct KrylovState {
 };
 static_assert(sizeof(KrylovState) <= 512, "state block");
 
+// The state scalars a CG decision reads, loaded together at the start of a
+// kernel so that they arrive with its other loads instead of one dependent
+// load after another behind the partial fold.  rho_prev is rho_{it-1} (an
+// unused slot when it == 0).
+struct CgSnap {
+  double nrm, tol, rho_prev;
+  int max_it;
+};
+
+__device__ __forceinline__ CgSnap cg_snap(const KrylovState* st, int it) {
+  return CgSnap{st->nrm, st->tol, st->rho_buf[(it - 1) & 1], st->max_it};
+}
+
 inline KrylovState* state_ptr(ofh_ctx ctx) {
   return reinterpret_cast<KrylovState*>(reinterpret_cast<char*>(ctx->dev_scratch) + 512);
 }
