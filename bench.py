@@ -112,10 +112,11 @@ def kernel_bytes(n, nnz):
     """Algorithmic bytes per launch of each kernel of the CG iterations
     (FP64: B_spmv64 + 88 n per iteration; mixed: B_spmv32 + 36 n)."""
     return {
-        # mixed, two kernels: product of z with q by recurrence (reads p_old,
-        # q_old, writes p_new, q beside the product), update on the scaled
-        # system D^-1/2 A D^-1/2 (reads x, p, r, q; writes x, r; z = r)
-        "k_mx_dirspmv": b_spmv32(n, nnz) + 12 * n,
+        # mixed, two kernels: the product of z (the survey's FP32 SpMV bytes;
+        # it also reads p_old, q_old and writes p_new beside q, which the
+        # survey counts in the iteration's 44n vector term), and the update
+        # on the scaled system D^-1/2 A D^-1/2 (reads x, p, r, q; writes x, r)
+        "k_mx_dirspmv": b_spmv32(n, nnz),
         "k_mx_upd": 24 * n,
         # FP64, three kernels: q = A p, update (reads x, p, r, q, 1/a_ii;
         # writes x, r), direction (reads r, 1/a_ii, p; writes p)
@@ -519,8 +520,7 @@ def bench_poisson3d(args):
                       "D^-1/2 A D^-1/2, %d floats per row, 2 rows per lane; q by recurrence, "
                       "direction, (p,q) partials)" % dia["row_floats"],
             "bytes_per_launch": dom["bytes_per_launch"], "avg_ms": round(dom["avg_us"] / 1e3, 5),
-            "bytes_definition": "B_spmv32 + 12n = 8 nnz + 4(n+1) + 20n (FP32 CSR SpMV + the "
-                                "direction and recurrence vectors)",
+            "bytes_definition": "B_spmv32 = 8 nnz + 4(n+1) + 8n (the survey's FP32 CSR SpMV)",
             "own_bytes_per_launch": (4 * dia["row_floats"] + 20) * n,
             "own_bytes_definition": "diagonal copy 4*%d n + z 4n + p_old, q_old read, p_new, q "
                                     "written 16n" % dia["row_floats"],
@@ -528,6 +528,15 @@ def bench_poisson3d(args):
                               / HBM_PEAK_GBS, 4)},
         "fp32_copy": {"diagonal_copy": dia, "inner_operator": "D^-1/2 A D^-1/2 (DIAG_PREC as a "
                       "symmetric scaling of the FP32 inner iteration)"},
+        "mixed_iteration": None if dom is None or "k_mx_upd" not in kernels else {
+            "us": round(dom["avg_us"] + kernels["k_mx_upd"]["avg_us"], 2),
+            "bytes_per_iteration": b_spmv32(n, nnz) + 44 * n,
+            "bytes_definition": "B_cg32_iter = B_spmv32 + 44n (the survey's FP32 inner iteration)",
+            "GBs": round((b_spmv32(n, nnz) + 44 * n)
+                         / ((dom["avg_us"] + kernels["k_mx_upd"]["avg_us"]) * 1e3), 1),
+            "frac": round((b_spmv32(n, nnz) + 44 * n)
+                          / ((dom["avg_us"] + kernels["k_mx_upd"]["avg_us"]) * 1e3)
+                          / HBM_PEAK_GBS, 4)},
         "roofline_spmv64": {
             "kernel": "k_spmv_sell<double,2> (SELL-64, %d-byte columns, %d stored entries)"
                       % (sell["col_bytes"], sell["stored"]),
