@@ -473,9 +473,12 @@ def bench_poisson3d(args):
                  f=torch.as_tensor(dr["f"], device="cuda"), matrix=True, b=br)
         torch.cuda.synchronize()
         nr, nnzr = Pr.A.n, Pr.A.nnz
+        dia_r = Pr.A.dia_info()
         wr, cr = spmv_timing(oa, ctx, Pr.A, args.spmv_reps, args.cold_reps)
         Br = b_spmv64(nr, nnzr)
         replica = {"rows": nr, "nnz": nnzr, "bytes_per_launch": Br,
+                   "kernel": ("k_spmv_dia<%d,double>" % dia_r["row_slots"]) if dia_r["fp64"]
+                             else "k_spmv_sell<double,2>",
                    "warm_ms": round(float(wr.mean()), 4), "cold_ms": round(float(cr.mean()), 4),
                    "warm_GBs": round(Br / wr.mean() / 1e6, 1), "cold_GBs": round(Br / cr.mean() / 1e6, 1)}
         log("[bench] replica %d^3: n=%d nnz=%d spmv64 warm %.4f ms (%.0f GB/s) cold %.4f ms (%.0f GB/s) (%.0fs)"
@@ -518,13 +521,13 @@ def bench_poisson3d(args):
             "traffic_source": None if traffic is None else traffic["source"],
             "kernel": "k_mx_dirspmv_dia (mixed CG inner product: FP32 diagonal copy of "
                       "D^-1/2 A D^-1/2, %d floats per row, 2 rows per lane; q by recurrence, "
-                      "direction, (p,q) partials)" % dia["row_floats"],
+                      "direction, (p,q) partials)" % dia["row_slots"],
             "bytes_per_launch": dom["bytes_per_launch"], "avg_ms": round(dom["avg_us"] / 1e3, 5),
             "bytes_definition": "B_spmv32 = 8 nnz + 4(n+1) + 8n (the survey's FP32 CSR SpMV)",
-            "own_bytes_per_launch": (4 * dia["row_floats"] + 20) * n,
+            "own_bytes_per_launch": (4 * dia["row_slots"] + 20) * n,
             "own_bytes_definition": "diagonal copy 4*%d n + z 4n + p_old, q_old read, p_new, q "
-                                    "written 16n" % dia["row_floats"],
-            "own_frac": round((4 * dia["row_floats"] + 20) * n / (dom["avg_us"] * 1e3)
+                                    "written 16n" % dia["row_slots"],
+            "own_frac": round((4 * dia["row_slots"] + 20) * n / (dom["avg_us"] * 1e3)
                               / HBM_PEAK_GBS, 4)},
         "fp32_copy": {"diagonal_copy": dia, "inner_operator": "D^-1/2 A D^-1/2 (DIAG_PREC as a "
                       "symmetric scaling of the FP32 inner iteration)"},
@@ -538,7 +541,10 @@ def bench_poisson3d(args):
                           / ((dom["avg_us"] + kernels["k_mx_upd"]["avg_us"]) * 1e3)
                           / HBM_PEAK_GBS, 4)},
         "roofline_spmv64": {
-            "kernel": "k_spmv_sell<double,2> (SELL-64, %d-byte columns, %d stored entries)"
+            "kernel": ("k_spmv_dia<%d,double> (FP64 diagonal copy, %d slots per row, chunks of 64 "
+                       "rows stored slot-major)" % (dia["row_slots"], dia["row_slots"]))
+                      if dia["fp64"] else
+                      "k_spmv_sell<double,2> (SELL-64, %d-byte columns, %d stored entries)"
                       % (sell["col_bytes"], sell["stored"]),
             "bytes_per_launch": B64, "bytes_definition": "B_spmv64 = 12 nnz + 4(n+1) + 16n",
             "warm_ms": round(float(spmv_ms), 5), "warm_GBs": round(B64 / spmv_ms / 1e6, 1),
@@ -546,13 +552,14 @@ def bench_poisson3d(args):
             "cold_ms": round(float(spmv_cold_ms), 5), "cold_GBs": round(B64 / spmv_cold_ms / 1e6, 1),
             "cold_frac": round(B64 / spmv_cold_ms / 1e6 / HBM_PEAK_GBS, 4),
             "cold_protocol": "each launch after a 1 GiB streaming write and a 1 GiB read",
-            "traffic": (pmc_traffic("k_spmv_sell<double,2>", n) or {}).get("bytes"),
+            "traffic": (pmc_traffic("k_spmv_dia<%d,double>" % dia["row_slots"] if dia["fp64"]
+                                    else "k_spmv_sell<double,2>", n) or {}).get("bytes"),
             "spmv32_warm_ms": round(float(spmv32_ms), 5),
             "spmv32_warm_GBs": round(B32 / spmv32_ms / 1e6, 1),
-            "spmv32_kernel": "k_spmv_dia<%d>" % dia["row_floats"] if dia["in_use"]
+            "spmv32_kernel": "k_spmv_dia<%d,float>" % dia["row_slots"] if dia["fp32"]
                              else "k_spmv_sell<float,4>"},
         "roofline_cold": None if replica is None else dict(
-            replica, kernel="k_spmv_sell<double,2>", workload="%d^3 Kuhn replica" % args.replica,
+            replica, workload="%d^3 Kuhn replica" % args.replica,
             bound="hbm", peak=HBM_PEAK_GBS, unit="GB/s",
             frac=round(replica["cold_GBs"] / HBM_PEAK_GBS, 4),
             bytes_definition="B_spmv64 = 12 nnz + 4(n+1) + 16n",

@@ -142,15 +142,18 @@ int ofh_csr_compress_columns(ofh_csr A, int enable);
  * chunk's first row) or 4 (32-bit columns). */
 int ofh_csr_sell_info(ofh_csr A, int* in_use, int64_t* stored, int* col_bytes);
 
-/* The diagonal copy of the FP32 values that ofh_spmv_f32 and the mixed
- * CG's inner product read (no reference counterpart: an internal format,
- * reported for measurement).  Builds it if the matrix has none yet;
- * *in_use = 1 when every entry lies on at most 15 diagonals (column - row
- * offsets), each row's columns ascend and the copy takes no more bytes than
- * the SELL-64 copy with 16-bit columns, with *diagonals = their number and
- * *row_floats = 4, 8 or 16 (the values of a row and its presence mask);
- * else 0 and the SELL-64 or CSR-stream kernels run. */
-int ofh_csr_dia_info(ofh_csr A, int* in_use, int* diagonals, int* row_floats);
+/* The diagonal copies of the values (no reference counterpart: an internal
+ * format, reported for measurement): where every entry lies on at most 15
+ * diagonals (column - row offsets) and each row's columns ascend, row i is
+ * stored as 4, 8 or 16 slots (the values of the diagonals and a presence
+ * mask) and no column index.  The FP32 copy (read by ofh_spmv_f32 and the
+ * mixed CG's inner product) is used when it takes no more bytes than the
+ * SELL-64 FP32 copy with 16-bit columns, the FP64 copy (ofh_spmv_f64, the
+ * FP64 CG products) when it takes no more than the SELL-64 FP64 copy.
+ * Builds what the matrix has not yet; *in_use32 / *in_use64 = 1 for a copy in
+ * use, *diagonals and *row_slots describe the layout (0 when the pattern does
+ * not fit); otherwise the SELL-64 or CSR-stream kernels run. */
+int ofh_csr_dia_info(ofh_csr A, int* in_use32, int* in_use64, int* diagonals, int* row_slots);
 
 /* y = A x: SpMatrix::Mult (SpMatrix_impl.h:728-734). */
 int ofh_spmv_f64(ofh_csr A, const double* d_x, double* d_y);

@@ -138,10 +138,12 @@ class CsrMatrix:
         return {"in_use": bool(u.value), "stored": st.value, "col_bytes": cb.value}
 
     def dia_info(self):
-        """The diagonal copy of the FP32 values: {"in_use", "diagonals", "row_floats"}."""
-        u, nd, w = C.c_int(), C.c_int(), C.c_int()
-        check(lib().ofh_csr_dia_info(self.h, C.byref(u), C.byref(nd), C.byref(w)))
-        return {"in_use": bool(u.value), "diagonals": nd.value, "row_floats": w.value}
+        """The diagonal copies of the values: {"fp32", "fp64" (in use), "diagonals",
+        "row_slots"}."""
+        u32, u64, nd, w = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().ofh_csr_dia_info(self.h, C.byref(u32), C.byref(u64), C.byref(nd), C.byref(w)))
+        return {"fp32": bool(u32.value), "fp64": bool(u64.value), "diagonals": nd.value,
+                "row_slots": w.value}
 
     def pattern(self):
         rp = np.zeros(self.n + 1, np.int32)

@@ -302,14 +302,15 @@ struct SellGatherDir {
   }
 };
 
-// k_cg_dirspmv with the product from the SELL-64 copy (one row per lane,
-// 256 rows per workgroup): the same stop test, beta, p_new = z + beta p_old
-// at the gathers, p_new of the block's rows stored, partial (p,q) -> slot 3.
-template <int PREC>
+// k_cg_dirspmv with the product from the SELL-64 copy (W = 0) or the FP64
+// diagonal copy of W slots per row (one row per lane, 256 rows per
+// workgroup): the same stop test, beta, p_new = z + beta p_old at the
+// gathers, p_new of the block's rows stored, partial (p,q) -> slot 3.
+template <int PREC, int W>
 __global__ __launch_bounds__(kBlock) void k_cg_dirspmv_sell(
     int n, int nblocks, const int* __restrict__ cptr, const int* __restrict__ clen,
     const uint32_t* __restrict__ col, const int16_t* __restrict__ col16,
-    const double* __restrict__ val, int it_arg, int nparts0,
+    const double* __restrict__ val, DiaF64 da, int it_arg, int nparts0,
     int nparts, const double* __restrict__ z, double* __restrict__ p0, double* __restrict__ p1,
     double* __restrict__ q, KrylovState* st, double* __restrict__ partials) {
   if (st->done) return;
@@ -364,8 +365,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_dirspmv_sell(
   const SellGatherDir g{x_rsrc(z, n), x_rsrc(p_old, n), beta, it == 0};
   const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
   double v[1] = {0.0};
-  if (row / kSellChunk * kSellChunk < n) {  // whole waves past the end only reduce
-    const double sq = sell_rowsum_g<double, 2>(row, cptr, clen, col, col16, val, g);
+  if (W > 0 ? row < n : row / kSellChunk * kSellChunk < n) {  // SELL: whole waves past the end only reduce
+    double sq;
+    if constexpr (W > 0)
+      sq = dia_rowsum_g<W>(row, da, g);
+    else
+      sq = sell_rowsum_g<double, 2>(row, cptr, clen, col, col16, val, g);
     if (row < n) {
       const double pi = it == 0 ? z[row] : z[row] + beta * p_old[row];
       p_new[row] = pi;
@@ -399,6 +404,27 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_sell(int n, int nblocks,
       q[row] = s;
       v[0] = p[row] * s;
     }
+  }
+  block_sum_k<kBlock, 1>(v, red);
+  const int slot[1] = {3};
+  write_partials<1>(v, slot, partials, blockIdx.x);
+}
+
+// q = A p from the FP64 diagonal copy (W slots per row), partials of (p,q) -> slot 3
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_dia(int nblocks, DiaF64 da,
+                                                       const double* __restrict__ p,
+                                                       double* __restrict__ q,
+                                                       const KrylovState* __restrict__ st,
+                                                       double* __restrict__ partials) {
+  if (st->done) return;
+  __shared__ double red[kBlock / kWave];
+  const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
+  double v[1] = {0.0};
+  if (row < da.n) {
+    const double s = dia_rowsum_g<W>(row, da, SellGather<double>{x_rsrc(p, da.n)});
+    q[row] = s;
+    v[0] = p[row] * s;
   }
   block_sum_k<kBlock, 1>(v, red);
   const int slot[1] = {3};
@@ -561,16 +587,28 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   if (!sweep_prec(PREC) && n <= kFuseRows) {
     const ColIndex ci{A->col, A->col16};
     double* zw = PREC == OFH_DIAG_PREC ? z : nullptr;
-    // the product from the SELL-64 copy when it is in use (one partial per 256 rows)
-    bool sell = false;
+    // the product from the FP64 diagonal copy or the SELL-64 copy when one is
+    // in use (one partial per 256 rows)
+    bool dia = false, sell = false;
     const int nbs = (n + kBlock - 1) / kBlock;
-    if (nbs <= kMaxPartials && (rc = csr_ensure_sell(A, false, &sell))) return rc;
-    const int npq = sell ? nbs : nb;
+    if (nbs <= kMaxPartials && (rc = csr_ensure_dia64(A, &dia))) return rc;
+    if (!dia && nbs <= kMaxPartials && (rc = csr_ensure_sell(A, false, &sell))) return rc;
+    const int npq = dia || sell ? nbs : nb;
+    const DiaF64 da = dia ? dia_f64(A) : DiaF64{};
     auto dirspmv = [&](int it, hipStream_t st_, int pk = -1) {  // pk: profiling kind
-      if (sell)
-        prof_kernel(ctx, pk, k_cg_dirspmv_sell<PREC>, dim3(nbs), dim3(kBlock), st_, n, nbs,
-                    A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val, it, nb,
-                    g, z, p, p1, q, st, part);
+#define OFH_CG_DIRSPMV_S(W)                                                                    \
+  prof_kernel(ctx, pk, k_cg_dirspmv_sell<PREC, W>, dim3(nbs), dim3(kBlock), st_, n, nbs,       \
+              A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val, da, it, nb, \
+              g, z, p, p1, q, st, part)
+      if (dia && A->dia_w == 4)
+        OFH_CG_DIRSPMV_S(4);
+      else if (dia && A->dia_w == 8)
+        OFH_CG_DIRSPMV_S(8);
+      else if (dia)
+        OFH_CG_DIRSPMV_S(16);
+      else if (sell)
+        OFH_CG_DIRSPMV_S(0);
+#undef OFH_CG_DIRSPMV_S
       else
         prof_kernel(ctx, pk, k_cg_dirspmv<PREC>, dim3(nb), dim3(kBlock), st_, nb, A->row_blocks,
                     A->row_ptr, ci, A->val, it, nb, g, z, p, p1, q, st, part);
@@ -587,7 +625,8 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
     hipGraphExec_t chunk_graph = nullptr;
     if (ctx->use_graphs) {
       GraphKey key;
-      csr_graph_key(A, 16 * 1 + PREC, (uintptr_t)sell, x, wsp, A->val, A->dinv, &key);
+      csr_graph_key(A, 16 * 1 + PREC, (uintptr_t)sell | (uintptr_t)dia << 1, x, wsp, A->val,
+                    A->dinv, &key);
       const bool prof = ctx->prof;
       ctx->prof = false;  // no event records inside a captured graph
       rc = ctx_graph(ctx, key, [&](hipStream_t cs) {
@@ -636,14 +675,26 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   if ((rc = read_state(ctx, &h))) return rc;
   int enq = 0, it_prev = 0;
   double res_prev = h.res;
-  // the SpMV from the SELL-64 copy when it is in use (one partial per 256 rows)
-  bool sell = false;
+  // the SpMV from the FP64 diagonal copy or the SELL-64 copy when one is in
+  // use (one partial per 256 rows)
+  bool dia = false, sell = false;
   const int nbs = (n + kBlock - 1) / kBlock;
-  if (nbs <= kMaxPartials && (rc = csr_ensure_sell(A, false, &sell))) return rc;
-  const int npq = sell ? nbs : nb;
+  if (nbs <= kMaxPartials && (rc = csr_ensure_dia64(A, &dia))) return rc;
+  if (!dia && nbs <= kMaxPartials && (rc = csr_ensure_sell(A, false, &sell))) return rc;
+  const int npq = dia || sell ? nbs : nb;
+  const DiaF64 da = dia ? dia_f64(A) : DiaF64{};
   // one iteration: k_cg_spmv, k_cg_upd(it), [SSOR sweeps], k_cg_dir(it)
   auto iteration = [&](int it, hipStream_t st_) -> int {
-    if (sell)
+    if (dia && A->dia_w == 4)
+      prof_kernel(ctx, OFH_PROF_CG_SPMV, k_cg_spmv_dia<4>, dim3(nbs), dim3(kBlock), st_, nbs, da,
+                  p, q, st, part);
+    else if (dia && A->dia_w == 8)
+      prof_kernel(ctx, OFH_PROF_CG_SPMV, k_cg_spmv_dia<8>, dim3(nbs), dim3(kBlock), st_, nbs, da,
+                  p, q, st, part);
+    else if (dia)
+      prof_kernel(ctx, OFH_PROF_CG_SPMV, k_cg_spmv_dia<16>, dim3(nbs), dim3(kBlock), st_, nbs, da,
+                  p, q, st, part);
+    else if (sell)
       prof_kernel(ctx, OFH_PROF_CG_SPMV, k_cg_spmv_sell, dim3(nbs), dim3(kBlock), st_, n, nbs,
                   A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val, p, q, st,
                   part);
@@ -663,7 +714,8 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   hipGraphExec_t chunk_graph = nullptr;
   if (ctx->use_graphs && !sweep_prec(PREC)) {
     GraphKey key;
-    csr_graph_key(A, 16 * 2 + PREC, (uintptr_t)sell, x, wsp, A->val, A->dinv, &key);
+    csr_graph_key(A, 16 * 2 + PREC, (uintptr_t)sell | (uintptr_t)dia << 1, x, wsp, A->val,
+                  A->dinv, &key);
     const bool prof = ctx->prof;
     ctx->prof = false;  // no event records inside a captured graph
     rc = ctx_graph(ctx, key, [&](hipStream_t cs) {

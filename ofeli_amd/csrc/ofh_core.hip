@@ -503,6 +503,7 @@ int ofh_csr_destroy(ofh_csr A) {
   hipFree(A->sell_val);
   hipFree(A->sell_val32);
   hipFree(A->dia_val32);
+  hipFree(A->dia_val64);
   hipFree(A->dia_val32s);
   hipFree(A->dscale);
   delete A;

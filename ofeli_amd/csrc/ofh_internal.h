@@ -294,20 +294,24 @@ struct ofh_csr_s {
   double* sell_val = nullptr;    // padding: -0.0
   float* sell_val32 = nullptr;
   uint64_t sell_version = ~0ull, sell_version32 = ~0ull;
-  // Diagonal copy of the FP32 values (lazy), for patterns whose entries lie
-  // on at most kDiaMaxDiag diagonals (structured-mesh numberings): row i
-  // holds W floats, slot d the value at column i + dia_off[d] (offsets
-  // ascending), slot W-1 the bit mask of the slots present in the row;
-  // W = 4, 8 or 16, the smallest that holds the diagonals and the mask
-  int dia_state = 0;             // 0: not built, 1: in use, -1: not applicable
+  // Diagonal copies of the values (lazy), for patterns whose entries lie on
+  // at most kDiaMaxDiag diagonals (structured-mesh numberings): row i has W
+  // slots, slot d the value at column i + dia_off[d] (offsets ascending),
+  // slot W-1 the bit mask of the slots present in the row; W = 4, 8 or 16,
+  // the smallest that holds the diagonals and the mask; rows stored by
+  // chunks of 64, slot-major inside a chunk
+  int dia_state = 0;             // 0: not detected, 1: pattern fits, -1: it does not
   int dia_nd = 0;                // diagonals
-  int dia_w = 0;                 // floats per row: 4, 8 or 16
+  int dia_w = 0;                 // slots per row: 4, 8 or 16
   int dia_off[16] = {};          // column - row of each diagonal (unused slots 0)
-  float* dia_val32 = nullptr;    // n * dia_w
-  uint64_t dia_version32 = ~0ull;
-  // The same copy of the symmetrically scaled matrix D^-1/2 A D^-1/2 (the
-  // mixed CG's FP32 inner operator for DIAG_PREC), and D^-1/2 itself
-  float* dia_val32s = nullptr;   // n * dia_w
+  bool dia_use32 = false;        // the FP32 copy takes no more bytes than SELL-64 FP32
+  bool dia_use64 = false;        // the FP64 copy takes no more bytes than SELL-64 FP64
+  float* dia_val32 = nullptr;    // ceil(n / 64) * 64 * dia_w
+  double* dia_val64 = nullptr;
+  uint64_t dia_version32 = ~0ull, dia_version64 = ~0ull;
+  // The symmetrically scaled matrix D^-1/2 A D^-1/2 (the mixed CG's FP32
+  // inner operator for DIAG_PREC) in the FP32 copy's layout, and D^-1/2
+  float* dia_val32s = nullptr;
   double* dscale = nullptr;      // n: 1 / sqrt(a_ii)
   bool dscale_ok = false;        // every a_ii > 0 and finite
   uint64_t dscale_version = ~0ull;
@@ -360,7 +364,8 @@ inline void csr_graph_key(ofh_csr A, uintptr_t kind, uintptr_t variant, const vo
       (uintptr_t)dinv, (uintptr_t)A->col16, (uintptr_t)A->col, (uintptr_t)A->row_blocks,
       (uintptr_t)A->row_ptr, (uintptr_t)A->sell_cptr, (uintptr_t)A->sell_col16,
       (uintptr_t)A->sell_col ^ ((uintptr_t)A->sell_val << 1) ^ (uintptr_t)A->sell_val32 ^
-          ((uintptr_t)A->dia_val32 << 2) ^ ((uintptr_t)A->dia_val32s << 3),
+          ((uintptr_t)A->dia_val32 << 2) ^ ((uintptr_t)A->dia_val32s << 3) ^
+          ((uintptr_t)A->dia_val64 << 4),
       (uintptr_t)kGraphIters};
   for (int i = 0; i < GraphKey::kN; ++i) key->v[i] = v[i];
 }
@@ -387,8 +392,11 @@ int launch_dilu_trans(ofh_csr A, const double* r, double* z, const int* d_done, 
 int csr_ensure_sell(ofh_csr A, bool fp32, bool* ok);
 // The diagonal copy with current FP32 values; *ok is false when the pattern
 // has more than kDiaMaxDiag diagonals, a row whose columns do not ascend,
-// or would take more bytes than the SELL-64 copy with 16-bit columns.
+// or the copy would take more bytes than the SELL-64 FP32 copy with 16-bit
+// columns (6 B per entry).
 int csr_ensure_dia(ofh_csr A, bool* ok);
+// The same with FP64 values (no more bytes than SELL-64 FP64: 10 B per entry).
+int csr_ensure_dia64(ofh_csr A, bool* ok);
 // The diagonal copy of D^-1/2 A D^-1/2 (values a_ij (s_i s_j), s = D^-1/2,
 // rounded once to FP32) and A->dscale = s in FP64, current with the values;
 // *ok is false when A has no diagonal copy or a diagonal entry is not

@@ -727,7 +727,7 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   const double* sc = PREC == kScaledDiag ? A->dscale : nullptr;
   if (PREC == kScaledDiag) {
     if (!fdia || !sc) return fail(OFH_E_INVALID, "ofh_cg_mixed: no scaled diagonal copy");
-    da.v = reinterpret_cast<const float4*>(A->dia_val32s);
+    da.v = A->dia_val32s;
   }
   // pk: the profiling kind of this launch (-1: not timed)
   auto dirspmv = [&](int phase, int k, int nparts, hipStream_t st_, int pk = -1) {

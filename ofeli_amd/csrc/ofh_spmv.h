@@ -192,43 +192,64 @@ __device__ __forceinline__ T sell_rowsum(int row, const int* __restrict__ cptr,
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal copy (FP32): row i stores W floats -- slot d the value of the
+// Diagonal copy (FP32 and FP64): row i has W slots -- slot d the value of the
 // entry at column i + off[d] (offsets ascending, so slot order is column
-// order), slot W-1 the bit mask of the slots the row really has.  A lane
-// loads its row with W/4 16-byte loads and every x[i + off[d]] at once: the
-// addresses need no column stream, so all loads of a row are in flight
-// together, and across a wave each x load reads 64 consecutive entries.
-// The sum adds the present entries left to right from +0.0, which is the CSR
-// row sum bit for bit; x is read through a buffer descriptor, so a column
-// outside [0, n) of an absent slot reads +0.0 and is masked out anyway.
+// order), slot W-1 the bit mask of the slots the row really has.  Rows are
+// stored by chunks of 64 (one wave), slot-major inside a chunk: slot d of
+// the chunk's 64 rows occupies 64 consecutive elements, so every load of a
+// wave reads one contiguous run.  A lane loads its row's W slots and every
+// x[i + off[d]] at once: the addresses need no column stream, so all loads of
+// a row are in flight together, and across a wave each x load reads 64
+// consecutive entries.  The sum adds the present entries left to right from
+// +0.0, which is the CSR row sum bit for bit; x is read through a buffer
+// descriptor, so a column outside [0, n) of an absent slot reads +0.0 and is
+// masked out anyway.
 // ---------------------------------------------------------------------------
 constexpr int kDiaMaxDiag = 15;
+constexpr int kDiaChunk = 64;
 
-struct DiaF32 {
+template <typename T>
+struct DiaRows {
   int n;
-  const float4* __restrict__ v;  // n * W / 4
-  int off[kDiaMaxDiag];          // unused slots 0
+  const T* __restrict__ v;  // ceil(n / 64) * W * 64
+  int off[kDiaMaxDiag];     // unused slots 0
+};
+using DiaF32 = DiaRows<float>;
+using DiaF64 = DiaRows<double>;
+
+// The diagonal copies of A as the kernels take them (host side).
+DiaF32 dia_f32(ofh_csr A);
+DiaF64 dia_f64(ofh_csr A);
+
+template <typename T>
+struct DiaMaskBits;
+template <>
+struct DiaMaskBits<float> {
+  using U = uint32_t;
+};
+template <>
+struct DiaMaskBits<double> {
+  using U = uint64_t;
 };
 
-// The diagonal copy of A as the kernels take it (host side).
-DiaF32 dia_f32(ofh_csr A);
-
-template <int W, class G>
-__device__ __forceinline__ float dia_rowsum_g(int row, const DiaF32& da, const G& g) {
-  static_assert(W == 4 || W == 8 || W == 16, "4, 8 or 16 floats per row");
-  float4 w[W / 4];
+template <int W, typename T, class G>
+__device__ __forceinline__ T dia_rowsum_g(int row, const DiaRows<T>& da, const G& g) {
+  static_assert(W == 4 || W == 8 || W == 16, "4, 8 or 16 slots per row");
+  const T* __restrict__ base =
+      da.v + (size_t)(row / kDiaChunk) * (W * kDiaChunk) + (row % kDiaChunk);
+  T a[W];
 #pragma unroll
-  for (int j = 0; j < W / 4; ++j) w[j] = da.v[(size_t)row * (W / 4) + j];
-  float xv[W - 1];
+  for (int d = 0; d < W; ++d) a[d] = base[d * kDiaChunk];
+  T xv[W - 1];
 #pragma unroll
   for (int d = 0; d < W - 1; ++d) xv[d] = g((uint32_t)(row + da.off[d]));
-  const float* a = reinterpret_cast<const float*>(w);
-  const uint32_t m = __float_as_uint(a[W - 1]);
-  float s = 0.0f;
+  using U = typename DiaMaskBits<T>::U;
+  const U m = __builtin_bit_cast(U, a[W - 1]);
+  T s = T(0);
 #pragma unroll
   for (int d = 0; d < W - 1; ++d) {
-    const float t = s + a[d] * xv[d];
-    s = (m >> d & 1u) ? t : s;
+    const T t = s + a[d] * xv[d];
+    s = (m >> d & U(1)) ? t : s;
   }
   return s;
 }
@@ -243,8 +264,9 @@ __device__ __forceinline__ T csr_stream_rowsum(int r0, int r1, const int* __rest
 }
 
 // The FP64 matrix as a product kernel reads it: the CSR-stream partition
-// (nblocks row blocks) and, when sell.cptr is set, its SELL-64 copy (256 rows
-// per workgroup, nblocks = ceil(n / 256)).
+// (nblocks row blocks) and, when sell.cptr is set, its SELL-64 copy, or when
+// dia.v is set its FP64 diagonal copy of dia_w slots per row (either: 256
+// rows per workgroup, nblocks = ceil(n / 256)).
 struct Sell64 {
   int n;
   const int* __restrict__ cptr;
@@ -260,6 +282,8 @@ struct MatRows64 {
   ColIndex ci;
   const double* __restrict__ val;
   Sell64 sell;
+  DiaF64 dia;
+  int dia_w;
 };
 
 // (A x)_i for this lane's row i of the workgroup; `valid` is false for lanes
@@ -272,6 +296,13 @@ __device__ __forceinline__ double mat_rowsum64(const MatRows64& m, const double*
     i = xcd_block(blockIdx.x, m.nblocks) * kBlock + threadIdx.x;
     valid = i < m.sell.n;
     if (i / kSellChunk * kSellChunk >= m.sell.n) return 0.0;  // a whole wave past the end
+    if (m.dia.v) {  // uniform: the diagonal copy
+      if (!valid) return 0.0;
+      const SellGather<double> g{x_rsrc(x, m.dia.n)};
+      if (m.dia_w == 4) return dia_rowsum_g<4>(i, m.dia, g);
+      if (m.dia_w == 8) return dia_rowsum_g<8>(i, m.dia, g);
+      return dia_rowsum_g<16>(i, m.dia, g);
+    }
     return sell_rowsum<double, 2>(i, m.sell.cptr, m.sell.clen, m.sell.col, m.sell.col16, m.sell.val,
                                   x_rsrc(x, m.sell.n));
   } else {
@@ -285,19 +316,24 @@ __device__ __forceinline__ double mat_rowsum64(const MatRows64& m, const double*
   }
 }
 
-// The FP64 matrix for mat_rowsum64<*sell>: the SELL-64 copy when it is worth
-// using and one partial per workgroup fits the reduction slots.
+// The FP64 matrix for mat_rowsum64<*sell>: the FP64 diagonal copy, else the
+// SELL-64 copy, when it is worth using and one partial per workgroup fits
+// the reduction slots (*sell then selects the 256-rows-per-workgroup form).
 inline int mat_rows64(ofh_csr A, MatRows64* m, bool* sell) {
   *sell = false;
+  bool dia = false;
   const int nbs = (A->n + kBlock - 1) / kBlock;
   if (nbs <= kMaxPartials) {
-    const int rc = csr_ensure_sell(A, false, sell);
+    int rc = csr_ensure_dia64(A, &dia);
+    if (!rc && !dia) rc = csr_ensure_sell(A, false, sell);
     if (rc) return rc;
   }
-  *m = MatRows64{*sell ? nbs : A->nblocks, A->row_blocks, A->row_ptr,
+  *m = MatRows64{*sell || dia ? nbs : A->nblocks, A->row_blocks, A->row_ptr,
                  ColIndex{A->col, A->col16}, A->val,
                  Sell64{A->n, A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16,
-                        A->sell_val}};
+                        A->sell_val},
+                 dia ? dia_f64(A) : DiaF64{}, dia ? A->dia_w : 0};
+  if (dia) *sell = true;
   return OFH_OK;
 }
 

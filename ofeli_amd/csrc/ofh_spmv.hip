@@ -158,31 +158,37 @@ int csr_ensure_sell(ofh_csr A, bool fp32, bool* ok) {
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal copy (FP32)
+// Diagonal copies (FP32, FP64)
 // ---------------------------------------------------------------------------
 struct DiaOffsets {
   int nd;
   int off[kDiaMaxDiag];
 };
 
+template <typename T>
+__device__ __forceinline__ T dia_mask_value(uint32_t m) {
+  using U = typename DiaMaskBits<T>::U;
+  return __builtin_bit_cast(T, (U)m);
+}
+
 // thread = row: the row's values into their diagonal slots, the mask last;
 // with `scale` (D^-1/2) the values of D^-1/2 A D^-1/2, a_ij (s_i s_j)
-template <int W>
+template <int W, typename T>
 __global__ __launch_bounds__(kBlock) void k_dia_vals(int n, const int* __restrict__ rp,
                                                      const int* __restrict__ col,
                                                      const double* __restrict__ val, DiaOffsets o,
                                                      const double* __restrict__ scale,
-                                                     float4* __restrict__ out) {
+                                                     T* __restrict__ out) {
   const int row = blockIdx.x * kBlock + threadIdx.x;
   if (row >= n) return;
-  float r[W];
+  T r[W];
 #pragma unroll
-  for (int d = 0; d < W - 1; ++d) r[d] = -0.0f;
+  for (int d = 0; d < W - 1; ++d) r[d] = T(-0.0);
   uint32_t m = 0;
   const double si = scale ? scale[row] : 1.0;
   for (int k = rp[row]; k < rp[row + 1]; ++k) {
     const int c = col[k] - row;
-    const float v = scale ? (float)(val[k] * (si * scale[col[k]])) : (float)val[k];
+    const T v = scale ? (T)(val[k] * (si * scale[col[k]])) : (T)val[k];
 #pragma unroll
     for (int d = 0; d < W - 1; ++d)
       if (d < o.nd && c == o.off[d]) {
@@ -190,85 +196,108 @@ __global__ __launch_bounds__(kBlock) void k_dia_vals(int n, const int* __restric
         m |= 1u << d;
       }
   }
-  r[W - 1] = __uint_as_float(m);
+  r[W - 1] = dia_mask_value<T>(m);
+  T* base = out + (size_t)(row / kDiaChunk) * (W * kDiaChunk) + (row % kDiaChunk);
 #pragma unroll
-  for (int j = 0; j < W / 4; ++j)
-    out[(size_t)row * (W / 4) + j] = make_float4(r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3]);
+  for (int d = 0; d < W; ++d) base[d * kDiaChunk] = r[d];
 }
 
-// values of the diagonal copy (scale non-null: of D^-1/2 A D^-1/2) into out
-static int launch_dia_vals(ofh_csr A, const double* scale, float* out_) {
+static size_t dia_elems(ofh_csr A) {
+  return (size_t)((A->n + kDiaChunk - 1) / kDiaChunk) * kDiaChunk * A->dia_w;
+}
+
+// values of a diagonal copy (scale non-null: of D^-1/2 A D^-1/2) into out
+template <typename T>
+static int launch_dia_vals(ofh_csr A, const double* scale, T* out) {
   DiaOffsets o{A->dia_nd, {}};
   for (int d = 0; d < kDiaMaxDiag; ++d) o.off[d] = A->dia_off[d];
   const int n = A->n;
   const int grid = (n + kBlock - 1) / kBlock;
   hipStream_t s = A->ctx->stream;
-  float4* out = reinterpret_cast<float4*>(out_);
   if (A->dia_w == 4)
-    hipLaunchKernelGGL(k_dia_vals<4>, dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
+    hipLaunchKernelGGL((k_dia_vals<4, T>), dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
                        A->val, o, scale, out);
   else if (A->dia_w == 8)
-    hipLaunchKernelGGL(k_dia_vals<8>, dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
+    hipLaunchKernelGGL((k_dia_vals<8, T>), dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
                        A->val, o, scale, out);
   else
-    hipLaunchKernelGGL(k_dia_vals<16>, dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
-                       A->val, o, scale, out);
+    hipLaunchKernelGGL((k_dia_vals<16, T>), dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr,
+                       A->col, A->val, o, scale, out);
   OFH_HIP(hipGetLastError());
+  return OFH_OK;
+}
+
+// The pattern's diagonals, once per pattern (host pass over row_ptr / col).
+static int dia_detect(ofh_csr A) {
+  if (A->dia_state != 0) return OFH_OK;
+  hipStream_t s = A->ctx->stream;
+  const int n = A->n;
+  std::vector<int> rp(n + 1), col((size_t)std::max<int64_t>(A->nnz, 1));
+  OFH_HIP(hipMemcpyAsync(rp.data(), A->row_ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, s));
+  OFH_HIP(hipMemcpyAsync(col.data(), A->col, sizeof(int) * A->nnz, hipMemcpyDeviceToHost, s));
+  OFH_HIP(hipStreamSynchronize(s));
+  // distinct column - row offsets, kept sorted; within a row the columns
+  // must ascend, so its offsets are met in ascending order and one cursor
+  // walks the sorted list
+  std::vector<int> offs;
+  bool fits = A->nnz > 0;
+  for (int i = 0; i < n && fits; ++i) {
+    size_t j = 0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      if (k > rp[i] && col[k] <= col[k - 1]) {
+        fits = false;
+        break;
+      }
+      const int o = col[k] - i;
+      while (j < offs.size() && offs[j] < o) ++j;
+      if (j == offs.size() || offs[j] != o) {
+        if ((int)offs.size() == kDiaMaxDiag) {
+          fits = false;
+          break;
+        }
+        offs.insert(offs.begin() + j, o);
+      }
+      ++j;
+    }
+  }
+  if (!fits) {
+    A->dia_state = -1;
+    return OFH_OK;
+  }
+  const int nd = (int)offs.size();
+  A->dia_nd = nd;
+  A->dia_w = nd <= 3 ? 4 : nd <= 7 ? 8 : 16;
+  for (int d = 0; d < 16; ++d) A->dia_off[d] = d < nd ? offs[d] : 0;
+  // each copy only where it takes no more bytes than the SELL-64 copy of the
+  // same precision with 16-bit columns (6 and 10 bytes per entry)
+  const int64_t rows = (int64_t)((n + kDiaChunk - 1) / kDiaChunk) * kDiaChunk;
+  A->dia_use32 = 4 * A->dia_w * rows <= 6 * A->nnz;
+  A->dia_use64 = 8 * A->dia_w * rows <= 10 * A->nnz;
+  A->dia_state = A->dia_use32 || A->dia_use64 ? 1 : -1;
   return OFH_OK;
 }
 
 int csr_ensure_dia(ofh_csr A, bool* ok) {
   *ok = false;
-  if (A->dia_state < 0) return OFH_OK;
-  hipStream_t s = A->ctx->stream;
-  const int n = A->n;
-  if (A->dia_state == 0) {
-    std::vector<int> rp(n + 1), col((size_t)std::max<int64_t>(A->nnz, 1));
-    OFH_HIP(hipMemcpyAsync(rp.data(), A->row_ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, s));
-    OFH_HIP(hipMemcpyAsync(col.data(), A->col, sizeof(int) * A->nnz, hipMemcpyDeviceToHost, s));
-    OFH_HIP(hipStreamSynchronize(s));
-    // distinct column - row offsets, kept sorted; within a row the columns
-    // must ascend, so its offsets are met in ascending order and one cursor
-    // walks the sorted list
-    std::vector<int> offs;
-    bool fits = A->nnz > 0;
-    for (int i = 0; i < n && fits; ++i) {
-      size_t j = 0;
-      for (int k = rp[i]; k < rp[i + 1]; ++k) {
-        if (k > rp[i] && col[k] <= col[k - 1]) {
-          fits = false;
-          break;
-        }
-        const int o = col[k] - i;
-        while (j < offs.size() && offs[j] < o) ++j;
-        if (j == offs.size() || offs[j] != o) {
-          if ((int)offs.size() == kDiaMaxDiag) {
-            fits = false;
-            break;
-          }
-          offs.insert(offs.begin() + j, o);
-        }
-        ++j;
-      }
-    }
-    const int nd = (int)offs.size();
-    const int w = nd <= 3 ? 4 : nd <= 7 ? 8 : 16;
-    // no more bytes than the SELL-64 copy's FP32 values and 16-bit columns
-    if (!fits || (int64_t)4 * w * n > 6 * A->nnz) {
-      A->dia_state = -1;
-      return OFH_OK;
-    }
-    OFH_HIP(hipMalloc(&A->dia_val32, sizeof(float) * (size_t)w * n));
-    A->dia_nd = nd;
-    A->dia_w = w;
-    for (int d = 0; d < 16; ++d) A->dia_off[d] = d < nd ? offs[d] : 0;
-    A->dia_version32 = ~0ull;
-    A->dia_state = 1;
-  }
+  int rc = dia_detect(A);
+  if (rc || A->dia_state < 0 || !A->dia_use32) return rc;
+  if (!A->dia_val32) OFH_HIP(hipMalloc(&A->dia_val32, sizeof(float) * dia_elems(A)));
   if (A->dia_version32 != A->version) {
-    const int rc = launch_dia_vals(A, nullptr, A->dia_val32);
-    if (rc) return rc;
+    if ((rc = launch_dia_vals<float>(A, nullptr, A->dia_val32))) return rc;
     A->dia_version32 = A->version;
+  }
+  *ok = true;
+  return OFH_OK;
+}
+
+int csr_ensure_dia64(ofh_csr A, bool* ok) {
+  *ok = false;
+  int rc = dia_detect(A);
+  if (rc || A->dia_state < 0 || !A->dia_use64) return rc;
+  if (!A->dia_val64) OFH_HIP(hipMalloc(&A->dia_val64, sizeof(double) * dia_elems(A)));
+  if (A->dia_version64 != A->version) {
+    if ((rc = launch_dia_vals<double>(A, nullptr, A->dia_val64))) return rc;
+    A->dia_version64 = A->version;
   }
   *ok = true;
   return OFH_OK;
@@ -299,7 +328,7 @@ int csr_ensure_dia_scaled(ofh_csr A, bool* ok) {
     hipStream_t s = A->ctx->stream;
     const int n = A->n;
     if (!A->dscale) OFH_HIP(hipMalloc(&A->dscale, sizeof(double) * n));
-    if (!A->dia_val32s) OFH_HIP(hipMalloc(&A->dia_val32s, sizeof(float) * (size_t)A->dia_w * n));
+    if (!A->dia_val32s) OFH_HIP(hipMalloc(&A->dia_val32s, sizeof(float) * dia_elems(A)));
     int* bad = (int*)A->ctx->dev_scratch;
     OFH_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_dscale, dim3((n + 255) / 256), dim3(256), 0, s, n, A->diag_pos, A->val,
@@ -309,7 +338,7 @@ int csr_ensure_dia_scaled(ofh_csr A, bool* ok) {
     OFH_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
     OFH_HIP(hipStreamSynchronize(s));
     A->dscale_ok = hbad == 0;
-    if (A->dscale_ok && (rc = launch_dia_vals(A, A->dscale, A->dia_val32s))) return rc;
+    if (A->dscale_ok && (rc = launch_dia_vals<float>(A, A->dscale, A->dia_val32s))) return rc;
     A->dscale_version = A->version;
   }
   *ok = A->dscale_ok;
@@ -317,18 +346,38 @@ int csr_ensure_dia_scaled(ofh_csr A, bool* ok) {
 }
 
 DiaF32 dia_f32(ofh_csr A) {
-  DiaF32 d{A->n, reinterpret_cast<const float4*>(A->dia_val32), {}};
+  DiaF32 d{A->n, A->dia_val32, {}};
   for (int k = 0; k < kDiaMaxDiag; ++k) d.off[k] = A->dia_off[k];
   return d;
 }
 
-// y = A x from the diagonal copy (FP32)
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_spmv_dia(int nblocks, DiaF32 da, const float* x,
-                                                     float* __restrict__ y) {
+DiaF64 dia_f64(ofh_csr A) {
+  DiaF64 d{A->n, A->dia_val64, {}};
+  for (int k = 0; k < kDiaMaxDiag; ++k) d.off[k] = A->dia_off[k];
+  return d;
+}
+
+// y = A x from a diagonal copy
+template <int W, typename T>
+__global__ __launch_bounds__(kBlock) void k_spmv_dia(int nblocks, DiaRows<T> da, const T* x,
+                                                     T* __restrict__ y) {
   const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
   if (row >= da.n) return;
-  y[row] = dia_rowsum_g<W>(row, da, SellGather<float>{x_rsrc(x, da.n)});
+  y[row] = dia_rowsum_g<W>(row, da, SellGather<T>{x_rsrc(x, da.n)});
+}
+
+template <typename T>
+static int launch_spmv_dia(ofh_csr A, const DiaRows<T>& da, const T* x, T* y, int kind,
+                           hipStream_t s) {
+  const int nb = (A->n + kBlock - 1) / kBlock;
+  if (A->dia_w == 4)
+    prof_kernel(A->ctx, kind, k_spmv_dia<4, T>, dim3(nb), dim3(kBlock), s, nb, da, x, y);
+  else if (A->dia_w == 8)
+    prof_kernel(A->ctx, kind, k_spmv_dia<8, T>, dim3(nb), dim3(kBlock), s, nb, da, x, y);
+  else
+    prof_kernel(A->ctx, kind, k_spmv_dia<16, T>, dim3(nb), dim3(kBlock), s, nb, da, x, y);
+  OFH_HIP(hipGetLastError());
+  return OFH_OK;
 }
 
 // y = A x from the SELL-64 copy
@@ -379,17 +428,20 @@ static int launch(ofh_csr A, const T* val, const T* x, T* y, T alpha, hipStream_
 
 }  // namespace ofh
 
-extern "C" int ofh_csr_dia_info(ofh_csr A, int* in_use, int* diagonals, int* row_floats) {
+extern "C" int ofh_csr_dia_info(ofh_csr A, int* in_use32, int* in_use64, int* diagonals,
+                                int* row_slots) {
   using namespace ofh;
-  OFH_CHECK(A && in_use && diagonals && row_floats, OFH_E_INVALID,
+  OFH_CHECK(A && in_use32 && in_use64 && diagonals && row_slots, OFH_E_INVALID,
             "ofh_csr_dia_info: null argument");
-  bool ok = false;
-  const int rc = csr_ensure_dia(A, &ok);
+  bool ok32 = false, ok64 = false;
+  int rc = csr_ensure_dia(A, &ok32);
+  if (!rc) rc = csr_ensure_dia64(A, &ok64);
   if (rc) return rc;
   OFH_HIP(hipStreamSynchronize(A->ctx->stream));
-  *in_use = ok ? 1 : 0;
-  *diagonals = ok ? A->dia_nd : 0;
-  *row_floats = ok ? A->dia_w : 0;
+  *in_use32 = ok32 ? 1 : 0;
+  *in_use64 = ok64 ? 1 : 0;
+  *diagonals = A->dia_state > 0 ? A->dia_nd : 0;
+  *row_slots = A->dia_state > 0 ? A->dia_w : 0;
   return OFH_OK;
 }
 
@@ -409,9 +461,12 @@ extern "C" int ofh_csr_sell_info(ofh_csr A, int* in_use, int64_t* stored, int* c
 namespace ofh {
 
 int launch_spmv64(ofh_csr A, const double* x, double* y, hipStream_t s) {
-  bool sell;
-  int rc = csr_ensure_sell(A, false, &sell);
+  bool dia;
+  int rc = csr_ensure_dia64(A, &dia);
   if (rc) return rc;
+  if (dia) return launch_spmv_dia<double>(A, dia_f64(A), x, y, OFH_PROF_SPMV64, s);
+  bool sell;
+  if ((rc = csr_ensure_sell(A, false, &sell))) return rc;
   if (sell) {
     const int nb = (A->n + kBlock - 1) / kBlock;
     prof_kernel(A->ctx, OFH_PROF_SPMV64, k_spmv_sell<double, 2>, dim3(nb), dim3(kBlock), s, A->n,
@@ -426,18 +481,7 @@ int launch_spmv32(ofh_csr A, const float* x, float* y, hipStream_t s) {
   bool dia;
   int rc = csr_ensure_dia(A, &dia);
   if (rc) return rc;
-  if (dia) {
-    const int nb = (A->n + kBlock - 1) / kBlock;
-    const DiaF32 da = dia_f32(A);
-    if (A->dia_w == 4)
-      prof_kernel(A->ctx, OFH_PROF_SPMV32, k_spmv_dia<4>, dim3(nb), dim3(kBlock), s, nb, da, x, y);
-    else if (A->dia_w == 8)
-      prof_kernel(A->ctx, OFH_PROF_SPMV32, k_spmv_dia<8>, dim3(nb), dim3(kBlock), s, nb, da, x, y);
-    else
-      prof_kernel(A->ctx, OFH_PROF_SPMV32, k_spmv_dia<16>, dim3(nb), dim3(kBlock), s, nb, da, x, y);
-    OFH_HIP(hipGetLastError());
-    return OFH_OK;
-  }
+  if (dia) return launch_spmv_dia<float>(A, dia_f32(A), x, y, OFH_PROF_SPMV32, s);
   bool sell;
   if ((rc = csr_ensure_sell(A, true, &sell))) return rc;
   if (sell) {

@@ -129,17 +129,22 @@ def tridiag_system(n=5000, seed=7):
 
 @pytest.mark.parametrize("case", ["grid3d", "tridiag"])
 def test_spmv_f32_diagonal_copy_bitexact(ctx, case):
-    """ofh_spmv_f32 from the diagonal copy (7 diagonals of a 3D grid: 8 floats
+    """ofh_spmv_f32 from the diagonal copy (7 diagonals of a 3D grid: 8 slots
     per row; a tridiagonal matrix: 4) bitwise the sequential float32 CSR row
-    sums, after a change of the values (the copy is gathered again) and with
-    non-finite x entries: a row's absent diagonals are masked out, so an
-    inf or nan only reaches the rows that really couple to it."""
+    sums, and ofh_spmv_f64 from the FP64 copy (the 3D grid; the tridiagonal
+    matrix keeps SELL-64, whose 3 entries per row take fewer bytes) bitwise
+    the oracle's, after a change of the values (the copies are gathered
+    again) and with non-finite x entries: a row's absent diagonals are masked
+    out, so an inf or nan only reaches the rows that really couple to it."""
     import ofeli_amd as oa
     torch = _torch()
     rp, ci, a = grid_system((28, 28, 28), 31) if case == "grid3d" else tridiag_system()
     A = gpu_csr(ctx, rp, ci, a)
-    want = {"grid3d": (7, 8), "tridiag": (3, 4)}[case]
-    assert A.dia_info() == {"in_use": True, "diagonals": want[0], "row_floats": want[1]}
+    want = {"grid3d": (True, 7, 8), "tridiag": (False, 3, 4)}[case]
+    assert A.dia_info() == {"fp32": True, "fp64": want[0], "diagonals": want[1],
+                            "row_slots": want[2]}
+    x64 = np.random.default_rng(34).standard_normal(A.n)
+    y64 = torch.zeros(A.n, dtype=torch.float64, device="cuda")
     rng = np.random.default_rng(33)
     x32 = rng.standard_normal(A.n).astype(np.float32)
     y32 = torch.zeros(A.n, dtype=torch.float32, device="cuda")
@@ -148,6 +153,8 @@ def test_spmv_f32_diagonal_copy_bitexact(ctx, case):
             A.set_values(vals)
         oa.spmv_f32(A, torch.as_tensor(x32, device="cuda"), y32)
         assert np.array_equal(y32.cpu().numpy(), seq_f32(rp, ci, vals.astype(np.float32), x32))
+        oa.spmv(A, dev(x64), y64)
+        assert np.array_equal(host(y64), O.spmv(rp, ci, vals, x64))
     xs = x32.copy()
     xs[[0, 5, A.n - 1, A.n // 2]] = [np.inf, np.nan, -np.inf, np.nan]
     oa.spmv_f32(A, torch.as_tensor(xs, device="cuda"), y32)
@@ -155,6 +162,11 @@ def test_spmv_f32_diagonal_copy_bitexact(ctx, case):
     got = y32.cpu().numpy()
     assert np.isfinite(ref).sum() > A.n - 40
     assert np.array_equal(got, ref, equal_nan=True)
+    xs64 = x64.copy()
+    xs64[[0, 5, A.n - 1, A.n // 2]] = [np.inf, np.nan, -np.inf, np.nan]
+    oa.spmv(A, dev(xs64), y64)
+    with np.errstate(invalid="ignore"):
+        assert np.array_equal(host(y64), O.spmv(rp, ci, vals, xs64), equal_nan=True)
 
 
 def test_mixed_cg_diag_scaled_and_unscaled_inner_iterations(ctx):
@@ -171,7 +183,7 @@ def test_mixed_cg_diag_scaled_and_unscaled_inner_iterations(ctx):
     out = {}
     for sign in (1.0, -1.0):
         A = gpu_csr(ctx, rp, ci, sign * a)
-        assert A.dia_info()["in_use"]
+        assert A.dia_info()["fp32"]
         x = torch.zeros(A.n, dtype=torch.float64, device="cuda")
         info = oa.cg_mixed(A, dev(sign * b), x, prec=oa.DIAG_PREC, max_it=5000, toler=1e-10)
         assert info.nb_it > 0 and info.fallback == 0, info.as_dict()
@@ -188,11 +200,13 @@ def test_mixed_cg_diag_scaled_and_unscaled_inner_iterations(ctx):
 def test_diagonal_copy_not_used_where_it_does_not_pay(ctx):
     """A 2D 5-point grid (5 diagonals: 8 floats per row against ~30 bytes of
     SELL-64 entries) and a matrix with far couplings (5 diagonals, but mostly
-    3 entries per row) keep the SELL-64 copy."""
+    3 entries per row) keep the SELL-64 copies in both precisions."""
     rp, ci, a = grid_system((400, 20), 31)
-    assert gpu_csr(ctx, rp, ci, a).dia_info()["in_use"] is False
+    info = gpu_csr(ctx, rp, ci, a).dia_info()
+    assert info["fp32"] is False and info["fp64"] is False
     rp, ci, a = far_coupled_system()
-    assert gpu_csr(ctx, rp, ci, a).dia_info()["in_use"] is False
+    info = gpu_csr(ctx, rp, ci, a).dia_info()
+    assert info["fp32"] is False and info["fp64"] is False
 
 
 def far_coupled_system(n=40000, far=35000):

@@ -71,14 +71,19 @@ def test_products_stream_the_sell_copy_with_16bit_columns(system):
     assert system["P"].A.nnz <= info["stored"] <= 1.25 * system["P"].A.nnz
 
 
-def test_fp32_products_read_the_diagonal_copy(system):
+def test_products_read_the_diagonal_copies(system):
     """The Kuhn numbering of the free nodes is a regular grid: every entry lies
     on one of 15 diagonals, so ofh_spmv_f32 and the mixed CG's inner product
-    read the diagonal copy (16 floats per row), bitwise the CSR row sums."""
+    read the FP32 diagonal copy and ofh_spmv_f64 and FP64 CG the FP64 one (16
+    slots per row), bitwise the CSR row sums."""
     import torch
     import ofeli_amd as oa
     A = system["P"].A
-    assert A.dia_info() == {"in_use": True, "diagonals": 15, "row_floats": 16}
+    assert A.dia_info() == {"fp32": True, "fp64": True, "diagonals": 15, "row_slots": 16}
+    x64 = np.random.default_rng(6).standard_normal(A.n)
+    y64 = torch.zeros(A.n, dtype=torch.float64, device="cuda")
+    oa.spmv(A, torch.as_tensor(x64, device="cuda"), y64)
+    assert np.array_equal(y64.cpu().numpy(), O.spmv(system["rp"], system["ci"], system["a"], x64))
     rng = np.random.default_rng(5)
     x32 = rng.standard_normal(A.n).astype(np.float32)
     y32 = torch.zeros(A.n, dtype=torch.float32, device="cuda")
