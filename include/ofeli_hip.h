@@ -86,6 +86,15 @@ int ofh_ctx_synchronize(ofh_ctx ctx);
  * directly (enable = 0, the default: measured faster on MI355X).  Results
  * are identical either way. */
 int ofh_ctx_use_graphs(ofh_ctx ctx, int enable);
+/* Form of ofh_cg_mixed's FP32 inner iteration (no reference counterpart; a
+ * tuning and test control): 0 (default) chooses by size, 1 takes two kernels
+ * per inner iteration (direction + product, update), 2 takes one kernel (the
+ * single-reduction iteration) wherever it applies: IDENT_PREC or DIAG_PREC
+ * on a matrix with an FP32 diagonal copy.  By default the one-kernel form is
+ * taken there when the copy and ten vector streams fit in 48 MiB.  Both
+ * forms meet the same FP64-verified tolerance; their iterates differ in
+ * rounding. */
+int ofh_ctx_mixed_form(ofh_ctx ctx, int form);
 /* Per-kernel timing (no reference counterpart; the reference's Timer
  * measures whole calls with clock(), Timer.h:93-129).  enable = 1 clears
  * earlier records and, until disabled, gives each launch of the kernels
@@ -104,6 +113,7 @@ int ofh_ctx_use_graphs(ofh_ctx ctx, int enable);
 #define OFH_PROF_CG_DIR 5     /* FP64 CG, three-kernel form: direction     */
 #define OFH_PROF_SPMV64 6     /* ofh_spmv_f64 from the SELL-64 copy        */
 #define OFH_PROF_SPMV32 7     /* ofh_spmv_f32 from the SELL-64 copy        */
+#define OFH_PROF_MX_ITER 8    /* mixed CG, one-kernel inner iteration      */
 int ofh_ctx_profile(ofh_ctx ctx, int enable);
 int ofh_ctx_profile_read(ofh_ctx ctx, int kind, double* h_ms, int cap, int* count);
 int ofh_malloc(ofh_ctx ctx, size_t bytes, void** d_ptr);

@@ -18,7 +18,7 @@ IDENT_PREC, DIAG_PREC, DILU_PREC, ILU_PREC, SSOR_PREC = 1, 2, 3, 4, 5
 LAPLACE2DT3, DC2DT3, DC2DT3_BE, LAPLACE3DT4, ELAS2DQ4 = 1, 2, 3, 4, 5
 # per-kernel timing kinds (ofh_ctx_profile)
 (PROF_MX_DIRSPMV, PROF_MX_UPD, PROF_CG_DIRSPMV, PROF_CG_UPD, PROF_CG_SPMV, PROF_CG_DIR,
- PROF_SPMV64, PROF_SPMV32) = range(8)
+ PROF_SPMV64, PROF_SPMV32, PROF_MX_ITER) = range(9)
 
 OFH_OK = 0
 ERRORS = {-1: "OFH_E_INVALID", -2: "OFH_E_HIP", -3: "OFH_E_ZERO_PIVOT", -4: "OFH_E_NEGDET",

@@ -18,6 +18,7 @@ SIGNATURES = {
     "ofh_ctx_set_stream": (i32, [vp, vp]),
     "ofh_ctx_synchronize": (i32, [vp]),
     "ofh_ctx_use_graphs": (i32, [vp, i32]),
+    "ofh_ctx_mixed_form": (i32, [vp, i32]),
     "ofh_malloc": (i32, [vp, C.c_size_t, C.POINTER(vp)]),
     "ofh_free": (i32, [vp, vp]),
     "ofh_memcpy_h2d": (i32, [vp, vp, vp, C.c_size_t]),

@@ -88,6 +88,11 @@ class Context:
         """Replay chunks of solver iterations from captured graphs (default off)."""
         check(lib().ofh_ctx_use_graphs(self.h, int(bool(enable))))
 
+    def mixed_form(self, form=0):
+        """Form of cg_mixed's inner iteration: 0 by size (default), 1 two kernels,
+        2 one kernel wherever z = r on a diagonal copy."""
+        check(lib().ofh_ctx_mixed_form(self.h, int(form)))
+
     def close(self):
         if self.h:
             for ch in list(self._children):

@@ -344,6 +344,12 @@ int ofh_ctx_use_graphs(ofh_ctx c, int enable) {
   return OFH_OK;
 }
 
+int ofh_ctx_mixed_form(ofh_ctx c, int form) {
+  OFH_CHECK(c && form >= 0 && form <= 2, OFH_E_INVALID, "ofh_ctx_mixed_form: bad argument");
+  c->mixed_form = form;
+  return OFH_OK;
+}
+
 int ofh_ctx_profile(ofh_ctx c, int enable) {
   OFH_CHECK(c, OFH_E_INVALID, "ofh_ctx_profile: null ctx");
   OFH_HIP(hipStreamSynchronize(c->stream));

@@ -239,6 +239,9 @@ struct ofh_ctx_s {
   // (ofh_ctx_use_graphs).  Off by default: measured on MI355X it is 2-3 %
   // slower than direct launches at 261k and 857k rows.
   bool use_graphs = false;
+  // form of ofh_cg_mixed's inner iteration (ofh_ctx_mixed_form): 0 by size,
+  // 1 two kernels, 2 one kernel wherever z = r on a diagonal copy
+  int mixed_form = 0;
   hipStream_t cap = nullptr;  // capture-only stream
   std::vector<std::pair<ofh::GraphKey, hipGraphExec_t>> graphs;  // small cache, oldest first
   // per-kernel timing of the solver loops (ofh_ctx_profile), off by default

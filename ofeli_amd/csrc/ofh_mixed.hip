@@ -88,14 +88,14 @@ __device__ __forceinline__ MxSnap mx_snap(const KrylovState* st, int k) {
 }
 
 // FP64 residual r = b - A x64 -> r32 (rounded), x32 = 0; partials
-// (b,b) -> 0, (r,r) -> 1 in FP64, (r32,z32) -> 2, (r32,r32) -> 3.
+// (b,b) -> sb, (r,r) -> sb + 1 in FP64, (r32,z32) -> sb + 2, (r32,r32) -> sb + 3.
 // (kScaledDiag: r32 = D^-1/2 r, sc = D^-1/2; the FP64 partials stay unscaled.)
 template <int PREC, bool SELL>
 __global__ __launch_bounds__(kBlock) void k_mx_resid(
     MatRows64 m, const double* __restrict__ b, const double* __restrict__ x,
     float* __restrict__ r32, float* __restrict__ x32, const float* __restrict__ dinv32,
     float* __restrict__ z32, const double* __restrict__ sc, const KrylovState* __restrict__ st,
-    double* __restrict__ partials) {
+    double* __restrict__ partials, int sb) {
   if (st->done) return;
   __shared__ double red[4 * kBlock / kWave];
   int i;
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_resid(
     v[3] = (double)rf * (double)rf;
   }
   block_sum_k<kBlock, 4>(v, red);
-  const int slot[4] = {0, 1, 2, 3};
+  const int slot[4] = {sb, sb + 1, sb + 2, sb + 3};
   write_partials<4>(v, slot, partials, blockIdx.x);
 }
 
@@ -305,12 +305,13 @@ struct SellF32 {
 
 // The decision of k_mx_dirspmv after `k` inner iterations (PHASE as in
 // k_mx_dir): folds the partials (slots 4, 5 from the prefetch `pf` when
-// `pre`, else slots 0-3 or 4, 5 from memory), records the state (block 0),
-// and returns false when the launch stops or pauses; else beta.
+// `pre`, else slots sb..sb+3 (PHASE 1, 2) or 4, 5 from memory), records the
+// state (block 0), and returns false when the launch stops or pauses; else beta.
 template <int PHASE, int PREC, class PF>
 __device__ __forceinline__ bool mx_decide(int k, int nparts, bool pre, const PF& pf,
                                           const double* __restrict__ partials, KrylovState* st,
-                                          const MxSnap& sn, double* red, double& beta_out) {
+                                          const MxSnap& sn, double* red, double& beta_out,
+                                          int sb = 0) {
   double nrm = sn.nrm, res, rz, rs = 0.0;
   bool stop = false, pause = false;
   int status = 0;
@@ -328,7 +329,7 @@ __device__ __forceinline__ bool mx_decide(int k, int nparts, bool pre, const PF&
     pause = (res <= kDelta * sn.rmax) | (res <= sn.tol) | (k >= sn.max_it) | !(res == res);
   } else {
     double t[4];
-    const int slot[4] = {0, 1, 2, 3};
+    const int slot[4] = {sb, sb + 1, sb + 2, sb + 3};
     fold_partials<kBlock, 4>(t, slot, partials, nparts, red);
     if (PHASE == 1) {
       nrm = sqrt(t[0]);
@@ -660,6 +661,214 @@ __global__ __launch_bounds__(kBlock) void k_mx_rz(int n, const int* __restrict__
   write_partials<1>(v, slot, partials, blockIdx.x);
 }
 
+// ---------------------------------------------------------------------------
+// One kernel per inner iteration, for z = r (IDENT, or DIAG as the
+// iteration on D^-1/2 A D^-1/2) on a diagonal copy whose rows and vectors
+// stay in the L2 caches: the single-reduction CG of Chronopoulos & Gear
+// (1989).  Launch j holds r_j, w_j = A r_j, s_{j-1} = A p_{j-1}, p_{j-1},
+// x_j and the dots (r_j,r_j), (w_j,r_j), (r_j,s_{j-1}), (p_{j-1},s_{j-1})
+// of the launch before it, and
+//   takes k_mx_dirspmv's PHASE 0 decision on r_j (pause, or go on);
+//   beta = (r_j,r_j) / (r_{j-1},r_{j-1}),  p_j = r_j + beta p_{j-1},
+//   s_j = w_j + beta s_{j-1} (= A p_j);
+//   (p_j,s_j) = (w_j,r_j) + 2 beta (r_j,s_{j-1}) + beta^2 (p_{j-1},s_{j-1})
+//   (A symmetric; formed from three dots, FP32 directions stay conjugate,
+//   where the two-term recurrence (w,r) - beta (r,r) / alpha loses them);
+//   alpha = (r_j,r_j) / (p_j,s_j),  x_{j+1} = x_j + alpha p_j,
+//   r_{j+1} = r_j - alpha s_j,  w_{j+1} = A r_{j+1}: each neighbour's r_{j+1}
+//   is formed at the gathers from its r_j, w_j and s_{j-1};
+//   the four dots of launch j + 1.
+// r, w and s alternate between two buffers (launch j reads set j & 1 and
+// writes the other), the dots between partial slots 0-3 and 4-7 (launch j
+// reads slots 4 (j & 1)).  x and p are updated in place, row by row.
+// A lane owns R rows, kBlock apart.
+
+// The new direction's product and residual of one row from its r_j, w_j and
+// s_{j-1}: the same arithmetic for a row's own values and for the copies its
+// neighbours form at their gathers.
+struct Cg1Step {
+  float alpha, beta;
+  bool first;  // j == 0: p_0 = r_0, s_0 = w_0
+  __device__ __forceinline__ float s(float w, float s_old) const {
+    return first ? w : w + beta * s_old;
+  }
+  __device__ __forceinline__ float r(float r_old, float s_new) const {
+    return r_old - alpha * s_new;
+  }
+};
+
+template <int PREC, int W, int R>
+__global__ __launch_bounds__(kBlock) void k_mx_cg1(int nblocks, DiaF32 da, int j, int nparts,
+                                                  const float* __restrict__ r_in,
+                                                  const float* __restrict__ w_in,
+                                                  const float* __restrict__ s_in,
+                                                  float* __restrict__ r_out,
+                                                  float* __restrict__ w_out,
+                                                  float* __restrict__ s_out,
+                                                  float* __restrict__ x32,
+                                                  float* __restrict__ p32, KrylovState* st,
+                                                  double* __restrict__ partials) {
+  if (st->halt) return;
+  const MxSnap sn = mx_snap(st, j);
+  __shared__ double red[4 * kBlock / kWave];
+  const int n = da.n;
+  const int row0 = xcd_block(blockIdx.x, nblocks) * (kBlock * R) + threadIdx.x;
+  const int sb = 4 * (j & 1);
+  const int slot[4] = {sb, sb + 1, sb + 2, sb + 3};
+  PartialsPrefetch<kBlock, 4> pf;
+  const bool pre = nparts <= pf.kMax;
+  if (pre) pf.load(slot, partials, nparts);
+  // every load is issued before the decision: the dots, the rows' own
+  // values, the matrix rows and the neighbours' r, w, s
+  const XRsrc rr = x_rsrc(r_in, n), wr = x_rsrc(w_in, n), sr = x_rsrc(s_in, n);
+  float a[R][W], rg[R][W - 1], wg[R][W - 1], sg[R][W - 1];
+  float ri[R], wi[R], si[R], xi[R], pi[R];
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int row = row0 + u * kBlock;
+    ri[u] = wi[u] = si[u] = xi[u] = pi[u] = 0.0f;
+#pragma unroll
+    for (int d = 0; d < W; ++d) a[u][d] = 0.0f;
+#pragma unroll
+    for (int d = 0; d < W - 1; ++d) rg[u][d] = wg[u][d] = sg[u][d] = 0.0f;
+    if (row < n) {
+      ri[u] = r_in[row];
+      wi[u] = w_in[row];
+      si[u] = s_in[row];  // unused when j == 0
+      xi[u] = x32[row];
+      pi[u] = p32[row];   // unused when j == 0
+      const float* __restrict__ base =
+          da.v + (size_t)(row / kDiaChunk) * (W * kDiaChunk) + (row % kDiaChunk);
+#pragma unroll
+      for (int d = 0; d < W; ++d) a[u][d] = base[d * kDiaChunk];
+#pragma unroll
+      for (int d = 0; d < W - 1; ++d) {
+        const uint32_t c = (uint32_t)(row + da.off[d]);
+        rg[u][d] = x_load<float>(rr, c);
+        wg[u][d] = x_load<float>(wr, c);
+        sg[u][d] = x_load<float>(sr, c);
+      }
+    }
+  }
+  double t[4];
+  if (pre)
+    pf.fold(t, nparts, red);
+  else
+    fold_partials<kBlock, 4>(t, slot, partials, nparts, red);
+  const double res = PREC == kScaledDiag ? sqrt(t[0]) * sn.rscale : sqrt(t[0]) / sn.nrm;
+  const bool pause =
+      (res <= kDelta * sn.rmax) | (res <= sn.tol) | (j >= sn.max_it) | !(res == res);
+  const double beta = j > 0 ? t[0] / sn.rho_prev : 0.0;
+  const double pap = j > 0 ? t[1] + beta * (2.0 * t[2] + beta * t[3]) : t[1];
+  const double alpha = t[0] / pap;
+  // st->it counts the steps taken: j on a pause (x_j is the iterate the
+  // replacement accumulates), j + 1 once this launch has formed x_{j+1}
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->res = res;
+    if (pause) {
+      st->it = j;
+      st->pause = 1;
+      st->halt = 1;
+    } else {
+      st->it = j + 1;
+      st->rho_buf[j & 1] = t[0];
+      st->beta = beta;
+      st->alpha = alpha;
+    }
+  }
+  if (pause) return;
+  const Cg1Step f{(float)alpha, (float)beta, j == 0};
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int row = row0 + u * kBlock;
+    if (row < n) {
+      const uint32_t m = __builtin_bit_cast(uint32_t, a[u][W - 1]);
+      float w = 0.0f;  // A r_{j+1}, the present slots left to right
+#pragma unroll
+      for (int d = 0; d < W - 1; ++d) {
+        const float rn = f.r(rg[u][d], f.s(wg[u][d], sg[u][d]));
+        const float t1 = w + a[u][d] * rn;
+        w = (m >> d & 1u) ? t1 : w;
+      }
+      const float sj = f.s(wi[u], si[u]);
+      const float rn = f.r(ri[u], sj);
+      const float pj = f.first ? ri[u] : ri[u] + f.beta * pi[u];
+      x32[row] = xi[u] + f.alpha * pj;
+      p32[row] = pj;
+      s_out[row] = sj;
+      r_out[row] = rn;
+      w_out[row] = w;
+      v[0] += (double)rn * (double)rn;
+      v[1] += (double)w * (double)rn;
+      v[2] += (double)rn * (double)sj;
+      v[3] += (double)pj * (double)sj;
+    }
+  }
+  block_sum_k<kBlock, 4>(v, red);
+  const int wslot[4] = {sb ^ 4, (sb ^ 4) + 1, (sb ^ 4) + 2, (sb ^ 4) + 3};
+  write_partials<4>(v, wslot, partials, blockIdx.x);
+}
+
+// The launch after k_mx_resid, ahead of launch j of k_mx_cg1 (PHASE 1: the
+// initial residual, j = 0; PHASE 2: a replacement after j inner
+// iterations): k_mx_dirspmv's PHASE 1/2 decision on the FP64 residual (the
+// partials k_mx_resid wrote to slots 4 (j & 1) ^ 4), then w = A r and, in
+// PHASE 2, s = A p_{j-1} from the diagonal copy (the product re-formed from
+// p once per cycle, so the rounding drift of the recurrence-carried s does
+// not outlive a cycle), and the four dots launch j folds (slots 4 (j & 1)).
+// r, w and s are the buffers of set j & 1.
+template <int PREC, int PHASE, int W, int R>
+__global__ __launch_bounds__(kBlock) void k_mx_cg1_start(int nblocks, DiaF32 da, int j, int nparts,
+                                                        const float* __restrict__ r,
+                                                        const float* __restrict__ p32,
+                                                        float* __restrict__ w,
+                                                        float* __restrict__ s, KrylovState* st,
+                                                        double* __restrict__ partials) {
+  if (st->done) return;
+  const MxSnap sn = mx_snap(st, j);
+  __shared__ double red[4 * kBlock / kWave];
+  const int n = da.n;
+  const int row0 = xcd_block(blockIdx.x, nblocks) * (kBlock * R) + threadIdx.x;
+  const bool gp = PHASE == 2 && j > 0;  // p_{j-1} exists
+  float ri[R], pi[R], wr[R], sp[R];
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int row = row0 + u * kBlock;
+    ri[u] = pi[u] = wr[u] = sp[u] = 0.0f;
+    if (row < n) {
+      ri[u] = r[row];
+      wr[u] = dia_rowsum_g<W>(row, da, SellGather<float>{x_rsrc(r, n)});
+      if (gp) {
+        pi[u] = p32[row];
+        sp[u] = dia_rowsum_g<W>(row, da, SellGather<float>{x_rsrc(p32, n)});
+      }
+    }
+  }
+  const int sb = 4 * (j & 1);
+  double beta;
+  PartialsPrefetch<kBlock, 2> pf;  // not used: PHASE 1, 2 fold from memory
+  if (!mx_decide<PHASE, PREC>(j, nparts, false, pf, partials, st, sn, red, beta, sb ^ 4)) return;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int row = row0 + u * kBlock;
+    if (row < n) {
+      w[row] = wr[u];
+      if (gp) s[row] = sp[u];
+      v[0] += (double)ri[u] * (double)ri[u];
+      v[1] += (double)wr[u] * (double)ri[u];
+      if (gp) {
+        v[2] += (double)ri[u] * (double)sp[u];
+        v[3] += (double)pi[u] * (double)sp[u];
+      }
+    }
+  }
+  block_sum_k<kBlock, 4>(v, red);
+  const int slot[4] = {sb, sb + 1, sb + 2, sb + 3};
+  write_partials<4>(v, slot, partials, blockIdx.x);
+}
+
 // x64 += x32 (the correction accumulated since the last replacement), or
 // x64 += D^-1/2 x32 on the scaled system (sc = D^-1/2)
 __global__ void k_mx_accum(int n, double* __restrict__ x, const float* __restrict__ x32,
@@ -699,8 +908,19 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     return rc;
   const bool qrec = fsell || fdia;
   const bool fused = !sweep_prec(PREC) && (n <= kFuseRows || qrec);
+  // One kernel per inner iteration (k_mx_cg1) where z = r and the diagonal
+  // copy and the vectors stay within reach of the L2 caches; measured per
+  // inner iteration against the two kernels: 8 slots per row, 261,121 rows
+  // 7.2 vs 9.5 us, 524,288 rows 9.1 vs 12.0 us; 16 slots, 857,375 rows
+  // 25.9 vs 20.6 us.  ofh_ctx_mixed_form overrides the choice.
+  constexpr size_t kOneKernelBytes = size_t(48) << 20;
+  const bool one = fdia && (PREC == kScaledDiag || PREC == OFH_IDENT_PREC) &&
+                   ctx->mixed_form != 1 &&
+                   (ctx->mixed_form == 2 ||
+                    (size_t)n * (4 * (size_t)A->dia_w + 40) <= kOneKernelBytes);
   void* wsp;
-  const size_t vbytes = sizeof(float) * (size_t)n * 6 + (sweep_prec(PREC) ? sizeof(double) * 2 * (size_t)n : 0);
+  const size_t vbytes = sizeof(float) * (size_t)n * (one ? 8 : 6) +
+                        (sweep_prec(PREC) ? sizeof(double) * 2 * (size_t)n : 0);
   if ((rc = ensure_vws(ctx, vbytes, &wsp))) return rc;
   float* r32 = (float*)wsp;
   float* p32 = r32 + n;
@@ -714,6 +934,11 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   // what k_mx_dir / k_mx_resid read as `dinv32`: the pivots, or z for the sweeps
   const float* dz32 = sweep_prec(PREC) ? z32 : A->dinv32;
   float* pb[2] = {p32, p32b};
+  // the one-kernel form's alternating r, w and s (z = r; x32 and p32 as above)
+  float* const ext = r32 + 5 * (size_t)n;
+  float* const r1[2] = {r32, ext};
+  float* const w1[2] = {q32, ext + n};
+  float* const s1[2] = {p32b, ext + 2 * (size_t)n};
   float* zw = fused && PREC == OFH_DIAG_PREC ? z32 : nullptr;  // z written by resid/upd
   const ColIndex ci{A->col, nullptr};  // FP32 partition: 32-bit columns
   const SellF32 sa{n, A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val32};
@@ -794,13 +1019,54 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   bool rsell = false;
   if ((rc = mat_rows64(A, &mr, &rsell))) return rc;
   const int nbr = mr.nblocks;  // its workgroups (= partials)
-  auto resid = [&]() {
+  // r = b - A x into rd, its partials to slots sb..sb+3
+  auto resid = [&](float* rd, int sb) {
     if (rsell)
-      hipLaunchKernelGGL((k_mx_resid<PREC, true>), dim3(nbr), dim3(kBlock), 0, s, mr, b, x, r32,
-                         x32, A->dinv32, zw, sc, st, part);
+      hipLaunchKernelGGL((k_mx_resid<PREC, true>), dim3(nbr), dim3(kBlock), 0, s, mr, b, x, rd,
+                         x32, A->dinv32, zw, sc, st, part, sb);
     else
-      hipLaunchKernelGGL((k_mx_resid<PREC, false>), dim3(nbr), dim3(kBlock), 0, s, mr, b, x, r32,
-                         x32, A->dinv32, zw, sc, st, part);
+      hipLaunchKernelGGL((k_mx_resid<PREC, false>), dim3(nbr), dim3(kBlock), 0, s, mr, b, x, rd,
+                         x32, A->dinv32, zw, sc, st, part, sb);
+  };
+  // the one-kernel form: launch j of k_mx_cg1, and k_mx_cg1_start ahead of
+  // launch j (rows per lane: 4, 2 with 16 slots per row)
+  const int nb1 = (n + kBlock * (A->dia_w == 16 ? 2 : 4) - 1) / (kBlock * (A->dia_w == 16 ? 2 : 4));
+  auto cg1 = [&](int j, hipStream_t st_) {
+    if constexpr (PREC == kScaledDiag || PREC == OFH_IDENT_PREC) {
+      const int u = j & 1;
+#define OFH_CG1(W, R)                                                                            \
+  prof_kernel(ctx, OFH_PROF_MX_ITER, k_mx_cg1<PREC, W, R>, dim3(nb1), dim3(kBlock), st_, nb1, da, \
+              j, nb1, r1[u], w1[u], s1[u], r1[u ^ 1], w1[u ^ 1], s1[u ^ 1], x32, p32, st, part)
+      if (A->dia_w == 4)
+        OFH_CG1(4, 4);
+      else if (A->dia_w == 8)
+        OFH_CG1(8, 4);
+      else
+        OFH_CG1(16, 2);
+#undef OFH_CG1
+    }
+  };
+  auto cg1_start = [&](int phase, int j) {
+    if constexpr (PREC == kScaledDiag || PREC == OFH_IDENT_PREC) {
+      const int u = j & 1;
+#define OFH_CG1S(PH, W, R)                                                                   \
+  hipLaunchKernelGGL((k_mx_cg1_start<PREC, PH, W, R>), dim3(nb1), dim3(kBlock), 0, s, nb1, da, j, \
+                     nbr, r1[u], p32, w1[u], s1[u], st, part)
+#define OFH_CG1S_W(PH)        \
+  if (A->dia_w == 4)          \
+    OFH_CG1S(PH, 4, 4);       \
+  else if (A->dia_w == 8)     \
+    OFH_CG1S(PH, 8, 4);       \
+  else                        \
+    OFH_CG1S(PH, 16, 2);
+      if (phase == 1) {
+        OFH_CG1S_W(1)
+      } else {
+        OFH_CG1S_W(2)
+      }
+#undef OFH_CG1S_W
+#undef OFH_CG1S
+    }
   };
   // z32 = M^-1 r32 by the FP64 sweeps and (r32, z32) -> slot (sweep precs)
   auto sweep_z = [&](const int* skip, int nparts, int slot_id, hipStream_t st_) -> int {
@@ -811,16 +1077,25 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
                        slot_id, part);
     return OFH_OK;
   };
-  resid();
-  if (sweep_prec(PREC) && (rc = sweep_z(&st->done, nbr, 2, s))) return rc;
-  if (fused)
-    dirspmv(1, 0, nbr, s);
-  else
-    hipLaunchKernelGGL((k_mx_dir<PREC, 1>), dim3(g), dim3(kBlock), 0, s, n, 0, nbr, r32,
-                       dz32, p32, st, part);
+  if (one) {  // the residual into set 0, its partials to the slots launch 0 does not read
+    resid(r1[0], 4);
+    cg1_start(1, 0);
+  } else {
+    resid(r32, 0);
+    if (sweep_prec(PREC) && (rc = sweep_z(&st->done, nbr, 2, s))) return rc;
+    if (fused)
+      dirspmv(1, 0, nbr, s);
+    else
+      hipLaunchKernelGGL((k_mx_dir<PREC, 1>), dim3(g), dim3(kBlock), 0, s, n, 0, nbr, r32,
+                         dz32, p32, st, part);
+  }
   OFH_HIP(hipGetLastError());
   // one inner iteration; k = -1 takes the index from the state (captured chunks)
   auto iteration = [&](int k, hipStream_t st_) -> int {
+    if (one) {  // launch k - 1 decides on r_{k-1} and forms r_k, x_k (state it = k)
+      cg1(k - 1, st_);
+      return OFH_OK;
+    }
     if (fused) {
       prof_kernel(ctx, OFH_PROF_MX_UPD, k_mx_upd<PREC>, dim3(g), dim3(kBlock), st_, n, k,
                   npq_fused, x32, r32, pb[0], pb[1], q32, A->dinv32, zw, st, part);
@@ -846,7 +1121,7 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     return OFH_OK;
   };
   hipGraphExec_t chunk_graph = nullptr;
-  if (ctx->use_graphs && !sweep_prec(PREC)) {
+  if (ctx->use_graphs && !sweep_prec(PREC) && !one) {
     GraphKey key;
     csr_graph_key(A, 16 * (fused ? 3 : 4) + (PREC == kScaledDiag ? 15 : PREC),
                   (uintptr_t)fsell | (uintptr_t)qrec << 1 | (uintptr_t)sell << 2 |
@@ -897,17 +1172,24 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     const bool forced = !h.pause && cyc_now >= cap;
     if (!h.pause && !forced) continue;
     if (h.pause && first_cycle == 0) first_cycle = std::max(1, cyc_now);
-    const int kr = forced ? h.it + 1 : h.it;
+    // (the one-kernel form has already taken the step along p_{it-1}: x32 is
+    // x_it, so its replacement acts as iteration it)
+    const int kr = forced && !one ? h.it + 1 : h.it;
     last_cycle = kr - it_repl;
     // residual replacement in FP64 after kr inner iterations
     hipLaunchKernelGGL(k_mx_accum, dim3(ge), dim3(256), 0, s, n, x, x32, sc, st);
-    resid();
-    if (sweep_prec(PREC) && (rc = sweep_z(&st->done, nbr, 2, s))) return rc;
-    if (fused)
-      dirspmv(2, kr, nbr, s);
-    else
-      hipLaunchKernelGGL((k_mx_dir<PREC, 2>), dim3(g), dim3(kBlock), 0, s, n, kr, nbr, r32,
-                         dz32, p32, st, part);
+    if (one) {
+      resid(r1[kr & 1], 4 * (kr & 1) ^ 4);
+      cg1_start(2, kr);
+    } else {
+      resid(r32, 0);
+      if (sweep_prec(PREC) && (rc = sweep_z(&st->done, nbr, 2, s))) return rc;
+      if (fused)
+        dirspmv(2, kr, nbr, s);
+      else
+        hipLaunchKernelGGL((k_mx_dir<PREC, 2>), dim3(g), dim3(kBlock), 0, s, n, kr, nbr, r32,
+                           dz32, p32, st, part);
+    }
     OFH_HIP(hipGetLastError());
     if ((rc = read_state(ctx, &h))) return rc;
     it_prev = h.it;

@@ -559,6 +559,61 @@ def test_mixed_cg_fp64_verified(ctx, prec):
     assert e <= 1e-7
 
 
+def sym_tridiag_system(n=6000, seed=11):
+    """Symmetric positive definite tridiagonal system (3 diagonals: the FP32
+    diagonal copy with 4 slots per row)."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    off = -rng.uniform(0.5, 1.0, n - 1)
+    d = 2.05 + rng.uniform(0.0, 0.1, n)
+    M = sp.diags([off, d, off], [-1, 0, 1], format="csr")
+    M.sort_indices()
+    b = rng.standard_normal(n)
+    return M.indptr.astype(np.int64), M.indices.astype(np.int64) + 1, M.data, b
+
+
+@pytest.mark.parametrize("system", ["dc2d", "tridiag"])
+@pytest.mark.parametrize("prec", ["DIAG_PREC", "IDENT_PREC"])
+def test_mixed_cg_one_and_two_kernel_inner_forms(ctx, prec, system):
+    """ofh_ctx_mixed_form: the single-reduction inner iteration (one kernel per
+    inner iteration, z = r on the FP32 diagonal copy; 8 and 4 slots per row
+    here) and the two-kernel form both stop on the true FP64 residual, at the
+    oracle's rtol-1e-12 solution to 1e-7 with about the FP64 iteration count
+    of the same preconditioner.  The per-kind kernel timings show which form
+    ran.  The max_it convention (-(max_it+1), max_it inner iterations) holds
+    for both."""
+    import ofeli_amd as oa
+    rp, ci, a, b = dc2d_system(64) if system == "dc2d" else sym_tridiag_system()
+    A = gpu_csr(ctx, rp, ci, a)
+    assert A.dia_info()["fp32"] and A.dia_info()["row_slots"] == (8 if system == "dc2d" else 4)
+    _, xo, _ = O.cg(rp, ci, a, b, max_it=20000, toler=1e-12)
+    r8, _, _ = O.cg(rp, ci, a, b, prec=getattr(O, prec.split("_")[0]), max_it=20000, toler=1e-8)
+    assert r8 > 0
+    try:
+        for form in (1, 2):
+            ctx.mixed_form(form)
+            ctx.profile(True)
+            x = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+            info = oa.cg_mixed(A, dev(b), x, prec=getattr(oa, prec), max_it=20000, toler=1e-8)
+            n_one = len(ctx.profile_read(oa.PROF_MX_ITER))
+            n_two = len(ctx.profile_read(oa.PROF_MX_UPD))
+            ctx.profile(False)
+            assert (n_one > 0 and n_two == 0) if form == 2 else (n_one == 0 and n_two > 0)
+            assert info.nb_it > 0 and info.fallback == 0, info.as_dict()
+            assert info.true_relres <= 1e-8
+            assert info.iterations <= 1.3 * r8 + 10, (info.iterations, r8)
+            e = rel(host(x), xo)
+            print("%s %s form %d: %d inner it (FP64 %d), %d replacements, err vs oracle %.2e"
+                  % (system, prec, form, info.iterations, r8, info.outer, e))
+            assert e <= 1e-7
+            x = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+            info = oa.cg_mixed(A, dev(b), x, prec=getattr(oa, prec), max_it=7, toler=1e-12)
+            assert info.nb_it == -8 and info.iterations == 7
+    finally:
+        ctx.profile(False)
+        ctx.mixed_form(0)
+
+
 def test_mixed_cg_many_partials(ctx):
     """Mixed CG on 2,000,000 rows: more update partials than the product
     kernel prefetches (it folds them after the product instead); the solve

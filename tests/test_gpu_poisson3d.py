@@ -128,6 +128,28 @@ def test_mixed_cg_matches_oracle(system):
     assert e <= 1e-7
 
 
+@pytest.mark.parametrize("form", [1, 2])
+def test_mixed_cg_inner_forms(system, ctx, form):
+    """Both forms of the inner iteration on the 15-diagonal copy (16 slots per
+    row): two kernels, and the single-reduction one kernel.  Same bars as the
+    default solve."""
+    import torch
+    import ofeli_amd as oa
+    P = system["P"]
+    try:
+        ctx.mixed_form(form)
+        x = torch.zeros_like(system["b"])
+        info = oa.cg_mixed(P.A, system["b"], x, prec=oa.DIAG_PREC, max_it=20000, toler=1e-8)
+    finally:
+        ctx.mixed_form(0)
+    assert info.nb_it > 0 and info.fallback == 0, info.as_dict()
+    assert info.true_relres <= 1e-8
+    e = rel(x.cpu().numpy(), system["x12"])
+    print("mixed cg form %d: %d inner it, %d replacements, err vs oracle %.2e"
+          % (form, info.iterations, info.outer, e))
+    assert e <= 1e-7
+
+
 def test_mixed_cg_ident_prec(system):
     import torch
     import ofeli_amd as oa
