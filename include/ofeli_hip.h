@@ -91,7 +91,8 @@ int ofh_ctx_use_graphs(ofh_ctx ctx, int enable);
  * per inner iteration (direction + product, update), 2 takes one kernel (the
  * single-reduction iteration) wherever it applies: IDENT_PREC or DIAG_PREC
  * on a matrix with an FP32 diagonal copy.  By default the one-kernel form is
- * taken there when the copy and ten vector streams fit in 48 MiB.  Both
+ * taken there up to 300,000 rows with 16 slots per row of the copy and
+ * 650,000 rows with 8 or 4 (measured crossovers on MI355X).  Both
  * forms meet the same FP64-verified tolerance; their iterates differ in
  * rounding. */
 int ofh_ctx_mixed_form(ofh_ctx ctx, int form);
