@@ -135,6 +135,30 @@ def working(ms):
     return ms[ms >= 0.5 * np.median(ms)]
 
 
+def dia_row_bytes(dia, value_bytes):
+    """Matrix bytes per row of the diagonal copy the products read: row_slots
+    values (the last slot the mask), or the symmetric form's row_slots / 2
+    values and a 16-bit mask."""
+    if dia["symmetric"]:
+        return value_bytes * dia["row_slots"] // 2 + 2
+    return value_bytes * dia["row_slots"]
+
+
+def dia_kernel(dia, typ):
+    """Name of the standalone diagonal-copy SpMV kernel as rocprofv3 lists it
+    (compacted) for the copy in use."""
+    return "k_spmv_dia<%d,%s,%s>" % (dia["row_slots"], typ, "true" if dia["symmetric"] else "false")
+
+
+def dia_layout(dia, what):
+    if dia["symmetric"]:
+        return ("%s: symmetric form of the diagonal copy, %d value slots per row (diagonal and "
+                "upper values; a lower value read from its mirror) + a 16-bit mask, chunks of 64 "
+                "rows stored slot-major" % (what, dia["row_slots"] // 2))
+    return ("%s: diagonal copy, %d slots per row, chunks of 64 rows stored slot-major"
+            % (what, dia["row_slots"]))
+
+
 def pmc_traffic(kernel, n):
     """HBM bytes per launch of `kernel` on an n-row system from the committed
     rocprofv3 --pmc summaries (profiles/*.md lines "traffic <kernel> n=<rows>
@@ -476,11 +500,16 @@ def bench_poisson3d(args):
         dia_r = Pr.A.dia_info()
         wr, cr = spmv_timing(oa, ctx, Pr.A, args.spmv_reps, args.cold_reps)
         Br = b_spmv64(nr, nnzr)
+        own_r = (dia_row_bytes(dia_r, 8) + 16) * nr if dia_r["fp64"] else None
         replica = {"rows": nr, "nnz": nnzr, "bytes_per_launch": Br,
-                   "kernel": ("k_spmv_dia<%d,double>" % dia_r["row_slots"]) if dia_r["fp64"]
+                   "kernel": dia_kernel(dia_r, "double") if dia_r["fp64"]
                              else "k_spmv_sell<double,2>",
                    "warm_ms": round(float(wr.mean()), 4), "cold_ms": round(float(cr.mean()), 4),
-                   "warm_GBs": round(Br / wr.mean() / 1e6, 1), "cold_GBs": round(Br / cr.mean() / 1e6, 1)}
+                   "warm_GBs": round(Br / wr.mean() / 1e6, 1), "cold_GBs": round(Br / cr.mean() / 1e6, 1),
+                   "own_bytes_per_launch": own_r,
+                   "own_bytes_definition": "matrix copy + x read 8n + y written 8n",
+                   "own_cold_frac": None if own_r is None
+                                    else round(own_r / cr.mean() / 1e6 / HBM_PEAK_GBS, 4)}
         log("[bench] replica %d^3: n=%d nnz=%d spmv64 warm %.4f ms (%.0f GB/s) cold %.4f ms (%.0f GB/s) (%.0fs)"
             % (args.replica, nr, nnzr, wr.mean(), Br / wr.mean() / 1e6, cr.mean(),
                Br / cr.mean() / 1e6, time.time() - t0))
@@ -519,15 +548,15 @@ def bench_poisson3d(args):
             "bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": dom["frac"], "traffic": None if traffic is None else traffic["bytes"],
             "traffic_source": None if traffic is None else traffic["source"],
-            "kernel": "k_mx_dirspmv_dia (mixed CG inner product: FP32 diagonal copy of "
-                      "D^-1/2 A D^-1/2, %d floats per row, 2 rows per lane; q by recurrence, "
-                      "direction, (p,q) partials)" % dia["row_slots"],
+            "kernel": "k_mx_dirspmv_dia (mixed CG inner product, 2 rows per lane; q by "
+                      "recurrence, direction, (p,q) partials) -- "
+                      + dia_layout(dia, "FP32 D^-1/2 A D^-1/2"),
             "bytes_per_launch": dom["bytes_per_launch"], "avg_ms": round(dom["avg_us"] / 1e3, 5),
             "bytes_definition": "B_spmv32 = 8 nnz + 4(n+1) + 8n (the survey's FP32 CSR SpMV)",
-            "own_bytes_per_launch": (4 * dia["row_slots"] + 20) * n,
-            "own_bytes_definition": "diagonal copy 4*%d n + z 4n + p_old, q_old read, p_new, q "
-                                    "written 16n" % dia["row_slots"],
-            "own_frac": round((4 * dia["row_slots"] + 20) * n / (dom["avg_us"] * 1e3)
+            "own_bytes_per_launch": (dia_row_bytes(dia, 4) + 20) * n,
+            "own_bytes_definition": "matrix copy %d n + z 4n + p_old, q_old read, p_new, q "
+                                    "written 16n" % dia_row_bytes(dia, 4),
+            "own_frac": round((dia_row_bytes(dia, 4) + 20) * n / (dom["avg_us"] * 1e3)
                               / HBM_PEAK_GBS, 4)},
         "fp32_copy": {"diagonal_copy": dia, "inner_operator": "D^-1/2 A D^-1/2 (DIAG_PREC as a "
                       "symmetric scaling of the FP32 inner iteration)"},
@@ -541,8 +570,7 @@ def bench_poisson3d(args):
                           / ((dom["avg_us"] + kernels["k_mx_upd"]["avg_us"]) * 1e3)
                           / HBM_PEAK_GBS, 4)},
         "roofline_spmv64": {
-            "kernel": ("k_spmv_dia<%d,double> (FP64 diagonal copy, %d slots per row, chunks of 64 "
-                       "rows stored slot-major)" % (dia["row_slots"], dia["row_slots"]))
+            "kernel": (dia_kernel(dia, "double") + " (" + dia_layout(dia, "FP64") + ")")
                       if dia["fp64"] else
                       "k_spmv_sell<double,2> (SELL-64, %d-byte columns, %d stored entries)"
                       % (sell["col_bytes"], sell["stored"]),
@@ -552,11 +580,15 @@ def bench_poisson3d(args):
             "cold_ms": round(float(spmv_cold_ms), 5), "cold_GBs": round(B64 / spmv_cold_ms / 1e6, 1),
             "cold_frac": round(B64 / spmv_cold_ms / 1e6 / HBM_PEAK_GBS, 4),
             "cold_protocol": "each launch after a 1 GiB streaming write and a 1 GiB read",
-            "traffic": (pmc_traffic("k_spmv_dia<%d,double>" % dia["row_slots"] if dia["fp64"]
+            "own_bytes_per_launch": (dia_row_bytes(dia, 8) + 16) * n if dia["fp64"] else None,
+            "own_bytes_definition": "matrix copy + x read 8n + y written 8n",
+            "own_cold_frac": round((dia_row_bytes(dia, 8) + 16) * n / spmv_cold_ms / 1e6
+                                   / HBM_PEAK_GBS, 4) if dia["fp64"] else None,
+            "traffic": (pmc_traffic(dia_kernel(dia, "double") if dia["fp64"]
                                     else "k_spmv_sell<double,2>", n) or {}).get("bytes"),
             "spmv32_warm_ms": round(float(spmv32_ms), 5),
             "spmv32_warm_GBs": round(B32 / spmv32_ms / 1e6, 1),
-            "spmv32_kernel": "k_spmv_dia<%d,float>" % dia["row_slots"] if dia["fp32"]
+            "spmv32_kernel": dia_kernel(dia, "float") if dia["fp32"]
                              else "k_spmv_sell<float,4>"},
         "roofline_cold": None if replica is None else dict(
             replica, workload="%d^3 Kuhn replica" % args.replica,

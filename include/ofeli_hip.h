@@ -163,8 +163,14 @@ int ofh_csr_sell_info(ofh_csr A, int* in_use, int64_t* stored, int* col_bytes);
  * FP64 CG products) when it takes no more than the SELL-64 FP64 copy.
  * Builds what the matrix has not yet; *in_use32 / *in_use64 = 1 for a copy in
  * use, *diagonals and *row_slots describe the layout (0 when the pattern does
- * not fit); otherwise the SELL-64 or CSR-stream kernels run. */
-int ofh_csr_dia_info(ofh_csr A, int* in_use32, int* in_use64, int* diagonals, int* row_slots);
+ * not fit); otherwise the SELL-64 or CSR-stream kernels run.  *symmetric = 1
+ * when the diagonals pair up (+-o) and the values are symmetric bit for bit
+ * (a_ij == a_ji, checked on the GPU for each set of values): the products
+ * then read the symmetric form, row_slots / 2 slots per row (the diagonal
+ * and the upper values; a lower value is read from its mirror) and a 16-bit
+ * presence mask per row, with the same row sums bit for bit. */
+int ofh_csr_dia_info(ofh_csr A, int* in_use32, int* in_use64, int* diagonals, int* row_slots,
+                     int* symmetric);
 
 /* y = A x: SpMatrix::Mult (SpMatrix_impl.h:728-734). */
 int ofh_spmv_f64(ofh_csr A, const double* d_x, double* d_y);

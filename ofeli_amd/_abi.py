@@ -35,7 +35,7 @@ SIGNATURES = {
     "ofh_csr_compress_columns": (i32, [vp, i32]),
     "ofh_csr_sell_info": (i32, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "ofh_csr_dia_info": (i32, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
-                               C.POINTER(C.c_int)]),
+                               C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ofh_spmv_f64": (i32, [vp, vp, vp]),
     "ofh_spmv_add_f64": (i32, [vp, f64, vp, vp]),
     "ofh_spmv_t_f64": (i32, [vp, vp, vp]),

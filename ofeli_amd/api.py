@@ -144,11 +144,12 @@ class CsrMatrix:
 
     def dia_info(self):
         """The diagonal copies of the values: {"fp32", "fp64" (in use), "diagonals",
-        "row_slots"}."""
-        u32, u64, nd, w = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        check(lib().ofh_csr_dia_info(self.h, C.byref(u32), C.byref(u64), C.byref(nd), C.byref(w)))
+        "row_slots", "symmetric" (the products read the symmetric form)}."""
+        u32, u64, nd, w, sym = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().ofh_csr_dia_info(self.h, C.byref(u32), C.byref(u64), C.byref(nd), C.byref(w),
+                                     C.byref(sym)))
         return {"fp32": bool(u32.value), "fp64": bool(u64.value), "diagonals": nd.value,
-                "row_slots": w.value}
+                "row_slots": w.value, "symmetric": bool(sym.value)}
 
     def pattern(self):
         rp = np.zeros(self.n + 1, np.int32)

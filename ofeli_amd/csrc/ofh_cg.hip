@@ -303,14 +303,15 @@ struct SellGatherDir {
 };
 
 // k_cg_dirspmv with the product from the SELL-64 copy (W = 0) or the FP64
-// diagonal copy of W slots per row (one row per lane, 256 rows per
-// workgroup): the same stop test, beta, p_new = z + beta p_old at the
-// gathers, p_new of the block's rows stored, partial (p,q) -> slot 3.
-template <int PREC, int W>
+// diagonal copy of W slots per row (SYM: its symmetric form; one row per
+// lane, 256 rows per workgroup): the same stop test, beta, p_new = z + beta
+// p_old at the gathers, p_new of the block's rows stored, partial (p,q) ->
+// slot 3.
+template <int PREC, int W, bool SYM>
 __global__ __launch_bounds__(kBlock) void k_cg_dirspmv_sell(
     int n, int nblocks, const int* __restrict__ cptr, const int* __restrict__ clen,
     const uint32_t* __restrict__ col, const int16_t* __restrict__ col16,
-    const double* __restrict__ val, DiaF64 da, int it_arg, int nparts0,
+    const double* __restrict__ val, DiaF64 da, DiaSymRows<double> ds, int it_arg, int nparts0,
     int nparts, const double* __restrict__ z, double* __restrict__ p0, double* __restrict__ p1,
     double* __restrict__ q, KrylovState* st, double* __restrict__ partials) {
   if (st->done) return;
@@ -368,7 +369,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_dirspmv_sell(
   if (W > 0 ? row < n : row / kSellChunk * kSellChunk < n) {  // SELL: whole waves past the end only reduce
     double sq;
     if constexpr (W > 0)
-      sq = dia_rowsum_g<W>(row, da, g);
+      sq = dia_row<W, SYM>(row, da, ds, g);
     else
       sq = sell_rowsum_g<double, 2>(row, cptr, clen, col, col16, val, g);
     if (row < n) {
@@ -410,9 +411,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_sell(int n, int nblocks,
   write_partials<1>(v, slot, partials, blockIdx.x);
 }
 
-// q = A p from the FP64 diagonal copy (W slots per row), partials of (p,q) -> slot 3
-template <int W>
+// q = A p from the FP64 diagonal copy (W slots per row; SYM: its symmetric
+// form), partials of (p,q) -> slot 3
+template <int W, bool SYM>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_dia(int nblocks, DiaF64 da,
+                                                       DiaSymRows<double> ds,
                                                        const double* __restrict__ p,
                                                        double* __restrict__ q,
                                                        const KrylovState* __restrict__ st,
@@ -422,7 +425,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_dia(int nblocks, DiaF64 da,
   const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
   double v[1] = {0.0};
   if (row < da.n) {
-    const double s = dia_rowsum_g<W>(row, da, SellGather<double>{x_rsrc(p, da.n)});
+    const double s = dia_row<W, SYM>(row, da, ds, SellGather<double>{x_rsrc(p, da.n)});
     q[row] = s;
     v[0] = p[row] * s;
   }
@@ -593,21 +596,24 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
     const int nbs = (n + kBlock - 1) / kBlock;
     if (nbs <= kMaxPartials && (rc = csr_ensure_dia64(A, &dia))) return rc;
     if (!dia && nbs <= kMaxPartials && (rc = csr_ensure_sell(A, false, &sell))) return rc;
+    bool sym = false;  // the symmetric form of the diagonal copy
+    if (dia && (rc = csr_ensure_dias(A, 0, &sym))) return rc;
     const int npq = dia || sell ? nbs : nb;
     const DiaF64 da = dia ? dia_f64(A) : DiaF64{};
+    const DiaSymRows<double> ds = sym ? dias_f64(A) : DiaSymRows<double>{};
     auto dirspmv = [&](int it, hipStream_t st_, int pk = -1) {  // pk: profiling kind
-#define OFH_CG_DIRSPMV_S(W)                                                                    \
-  prof_kernel(ctx, pk, k_cg_dirspmv_sell<PREC, W>, dim3(nbs), dim3(kBlock), st_, n, nbs,       \
-              A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val, da, it, nb, \
-              g, z, p, p1, q, st, part)
+#define OFH_CG_DIRSPMV_S(W, SY)                                                                 \
+  prof_kernel(ctx, pk, k_cg_dirspmv_sell<PREC, W, SY>, dim3(nbs), dim3(kBlock), st_, n, nbs,     \
+              A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val, da, ds, it,  \
+              nb, g, z, p, p1, q, st, part)
       if (dia && A->dia_w == 4)
-        OFH_CG_DIRSPMV_S(4);
+        sym ? OFH_CG_DIRSPMV_S(4, true) : OFH_CG_DIRSPMV_S(4, false);
       else if (dia && A->dia_w == 8)
-        OFH_CG_DIRSPMV_S(8);
+        sym ? OFH_CG_DIRSPMV_S(8, true) : OFH_CG_DIRSPMV_S(8, false);
       else if (dia)
-        OFH_CG_DIRSPMV_S(16);
+        sym ? OFH_CG_DIRSPMV_S(16, true) : OFH_CG_DIRSPMV_S(16, false);
       else if (sell)
-        OFH_CG_DIRSPMV_S(0);
+        OFH_CG_DIRSPMV_S(0, false);
 #undef OFH_CG_DIRSPMV_S
       else
         prof_kernel(ctx, pk, k_cg_dirspmv<PREC>, dim3(nb), dim3(kBlock), st_, nb, A->row_blocks,
@@ -681,19 +687,23 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   const int nbs = (n + kBlock - 1) / kBlock;
   if (nbs <= kMaxPartials && (rc = csr_ensure_dia64(A, &dia))) return rc;
   if (!dia && nbs <= kMaxPartials && (rc = csr_ensure_sell(A, false, &sell))) return rc;
+  bool sym = false;  // the symmetric form of the diagonal copy
+  if (dia && (rc = csr_ensure_dias(A, 0, &sym))) return rc;
   const int npq = dia || sell ? nbs : nb;
   const DiaF64 da = dia ? dia_f64(A) : DiaF64{};
+  const DiaSymRows<double> ds = sym ? dias_f64(A) : DiaSymRows<double>{};
   // one iteration: k_cg_spmv, k_cg_upd(it), [SSOR sweeps], k_cg_dir(it)
   auto iteration = [&](int it, hipStream_t st_) -> int {
+#define OFH_CG_SPMV_DIA(W, SY)                                                                  \
+  prof_kernel(ctx, OFH_PROF_CG_SPMV, k_cg_spmv_dia<W, SY>, dim3(nbs), dim3(kBlock), st_, nbs, da, \
+              ds, p, q, st, part)
     if (dia && A->dia_w == 4)
-      prof_kernel(ctx, OFH_PROF_CG_SPMV, k_cg_spmv_dia<4>, dim3(nbs), dim3(kBlock), st_, nbs, da,
-                  p, q, st, part);
+      sym ? OFH_CG_SPMV_DIA(4, true) : OFH_CG_SPMV_DIA(4, false);
     else if (dia && A->dia_w == 8)
-      prof_kernel(ctx, OFH_PROF_CG_SPMV, k_cg_spmv_dia<8>, dim3(nbs), dim3(kBlock), st_, nbs, da,
-                  p, q, st, part);
+      sym ? OFH_CG_SPMV_DIA(8, true) : OFH_CG_SPMV_DIA(8, false);
     else if (dia)
-      prof_kernel(ctx, OFH_PROF_CG_SPMV, k_cg_spmv_dia<16>, dim3(nbs), dim3(kBlock), st_, nbs, da,
-                  p, q, st, part);
+      sym ? OFH_CG_SPMV_DIA(16, true) : OFH_CG_SPMV_DIA(16, false);
+#undef OFH_CG_SPMV_DIA
     else if (sell)
       prof_kernel(ctx, OFH_PROF_CG_SPMV, k_cg_spmv_sell, dim3(nbs), dim3(kBlock), st_, n, nbs,
                   A->sell_cptr, A->sell_clen, A->sell_col, A->sell_col16, A->sell_val, p, q, st,

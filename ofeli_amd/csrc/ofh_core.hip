@@ -512,6 +512,10 @@ int ofh_csr_destroy(ofh_csr A) {
   hipFree(A->dia_val64);
   hipFree(A->dia_val32s);
   hipFree(A->dscale);
+  hipFree(A->dias_mask);
+  hipFree(A->dias_val64);
+  hipFree(A->dias_val32);
+  hipFree(A->dias_val32s);
   delete A;
   return OFH_OK;
 }

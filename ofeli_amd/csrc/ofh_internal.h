@@ -318,6 +318,19 @@ struct ofh_csr_s {
   double* dscale = nullptr;      // n: 1 / sqrt(a_ii)
   bool dscale_ok = false;        // every a_ii > 0 and finite
   uint64_t dscale_version = ~0ull;
+  // Symmetric diagonal copies (DiaSymRows): the pattern's diagonals pair up
+  // (+-o) with as many entries above the diagonal as below (dias_pat, found
+  // with the diagonals), and the values are symmetric bit for bit (dias_ok,
+  // checked on the GPU once per value version)
+  bool dias_pat = false;
+  int dias_up[8] = {};           // up[0] = 0, up[k] the k-th upper offset (unused 0)
+  bool dias_ok = false;
+  uint64_t dias_check_version = ~0ull;
+  uint16_t* dias_mask = nullptr;  // n
+  double* dias_val64 = nullptr;   // ceil(n / 64) * 64 * dia_w / 2
+  float* dias_val32 = nullptr;
+  float* dias_val32s = nullptr;   // of D^-1/2 A D^-1/2
+  uint64_t dias_version64 = ~0ull, dias_version32 = ~0ull, dias_version32s = ~0ull;
   ofh_csr_s* tr = nullptr;       // transpose (BiCG, TMult, TransSolve; lazy)
   int* tr_perm = nullptr;        // nnz: position in A of each entry of tr
   uint64_t tr_version = ~0ull;   // values version tr->val was gathered from
@@ -368,7 +381,8 @@ inline void csr_graph_key(ofh_csr A, uintptr_t kind, uintptr_t variant, const vo
       (uintptr_t)A->row_ptr, (uintptr_t)A->sell_cptr, (uintptr_t)A->sell_col16,
       (uintptr_t)A->sell_col ^ ((uintptr_t)A->sell_val << 1) ^ (uintptr_t)A->sell_val32 ^
           ((uintptr_t)A->dia_val32 << 2) ^ ((uintptr_t)A->dia_val32s << 3) ^
-          ((uintptr_t)A->dia_val64 << 4),
+          ((uintptr_t)A->dia_val64 << 4) ^ ((uintptr_t)A->dias_val32 << 5) ^
+          ((uintptr_t)A->dias_val32s << 6) ^ ((uintptr_t)A->dias_val64 << 7),
       (uintptr_t)kGraphIters};
   for (int i = 0; i < GraphKey::kN; ++i) key->v[i] = v[i];
 }
@@ -405,6 +419,11 @@ int csr_ensure_dia64(ofh_csr A, bool* ok);
 // *ok is false when A has no diagonal copy or a diagonal entry is not
 // positive and finite.
 int csr_ensure_dia_scaled(ofh_csr A, bool* ok);
+// The symmetric form of a diagonal copy that is in use (kind 0: FP64, 1:
+// FP32, 2: FP32 of D^-1/2 A D^-1/2, after csr_ensure_dia_scaled), current
+// with the values; *ok when the matrix is symmetric bit for bit and the copy
+// is made.
+int csr_ensure_dias(ofh_csr A, int kind, bool* ok);
 // A->tr, the CSR transpose of A, with values current (built on first use).
 int csr_ensure_transpose(ofh_csr A);
 // Everything derived from the values (FP32 copy, pivots, version) is stale.

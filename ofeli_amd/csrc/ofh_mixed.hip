@@ -479,10 +479,12 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv(int nblocks, const int* _
 // direction and (p,q) partials.  QREC: q_new = A z + beta q_old, the product
 // formed before the decision with the update's partials loading beside it;
 // otherwise (the launch after a replacement) q = A p_new with p_new formed at
-// the gathers.  The row sums are those of the CSR rows bit for bit.  A lane
-// owns R rows, kBlock apart (R * kBlock rows per workgroup, one partial each).
-template <int PREC, int PHASE, bool QREC, int W, int R>
-__global__ __launch_bounds__(kBlock) void k_mx_dirspmv_dia(int nblocks, DiaF32 da, int k_arg,
+// the gathers.  The row sums are those of the CSR rows bit for bit (SYM: from
+// the symmetric form of the copy).  A lane owns R rows, kBlock apart
+// (R * kBlock rows per workgroup, one partial each).
+template <int PREC, int PHASE, bool QREC, int W, int R, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_mx_dirspmv_dia(int nblocks, DiaF32 da,
+                                                          DiaSymRows<float> ds, int k_arg,
                                                           int nparts,
                                                           const float* __restrict__ z32,
                                                           float* __restrict__ p0,
@@ -515,7 +517,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv_dia(int nblocks, DiaF32 d
       po[j] = p_old[row];  // unused when k == 0
       if constexpr (QREC) {
         qo[j] = q32[row];
-        az[j] = dia_rowsum_g<W>(row, da, SellGather<float>{x_rsrc(z32, n)});
+        az[j] = dia_row<W, SYM>(row, da, ds, SellGather<float>{x_rsrc(z32, n)});
       }
     }
   }
@@ -532,7 +534,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_dirspmv_dia(int nblocks, DiaF32 d
       if constexpr (QREC)
         qi = k == 0 ? az[j] : az[j] + bf * qo[j];
       else
-        qi = dia_rowsum_g<W>(row, da,
+        qi = dia_row<W, SYM>(row, da, ds,
                              SellGatherDirF{x_rsrc(z32, n), x_rsrc(p_old, n), bf, k == 0});
       p_new[row] = pi;
       q32[row] = qi;
@@ -681,7 +683,10 @@ __global__ __launch_bounds__(kBlock) void k_mx_rz(int n, const int* __restrict__
 // r, w and s alternate between two buffers (launch j reads set j & 1 and
 // writes the other), the dots between partial slots 0-3 and 4-7 (launch j
 // reads slots 4 (j & 1)).  x and p are updated in place, row by row.
-// A lane owns R rows, kBlock apart.
+// A lane owns R rows, kBlock apart.  The matrix comes from the full diagonal
+// copy even where a symmetric form exists: this kernel is latency-bound at
+// the sizes it runs at, and the symmetric form's partner loads made it
+// slower (512^2 heat matrix, mixed CG from zero: 7.0 vs 6.7 ms).
 
 // The new direction's product and residual of one row from its r_j, w_j and
 // s_{j-1}: the same arithmetic for a row's own values and for the copies its
@@ -954,13 +959,20 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     if (!fdia || !sc) return fail(OFH_E_INVALID, "ofh_cg_mixed: no scaled diagonal copy");
     da.v = A->dia_val32s;
   }
+  // the symmetric form of that copy, for the two-kernel product
+  bool fsym = false;
+  if (fdia && !one && (rc = csr_ensure_dias(A, PREC == kScaledDiag ? 2 : 1, &fsym))) return rc;
+  const DiaSymRows<float> dsy = fsym ? dias_f32(A, PREC == kScaledDiag) : DiaSymRows<float>{};
   // pk: the profiling kind of this launch (-1: not timed)
   auto dirspmv = [&](int phase, int k, int nparts, hipStream_t st_, int pk = -1) {
     if constexpr (!sweep_prec(PREC)) {
       if (fdia) {
-#define OFH_MX_DIA(PH, QR, W)                                                                    \
-  prof_kernel(ctx, pk, k_mx_dirspmv_dia<PREC, PH, QR, W, kDiaRows>, dim3(nbd), dim3(kBlock), st_, \
-              nbd, da, k, nparts, z32, pb[0], pb[1], q32, state_ptr(ctx), ctx->ws.partials)
+#define OFH_MX_DIA_S(PH, QR, W, SY)                                                             \
+  prof_kernel(ctx, pk, k_mx_dirspmv_dia<PREC, PH, QR, W, kDiaRows, SY>, dim3(nbd), dim3(kBlock),  \
+              st_, nbd, da, dsy, k, nparts, z32, pb[0], pb[1], q32, state_ptr(ctx),             \
+              ctx->ws.partials)
+#define OFH_MX_DIA(PH, QR, W) \
+  fsym ? OFH_MX_DIA_S(PH, QR, W, true) : OFH_MX_DIA_S(PH, QR, W, false)
 #define OFH_MX_DIA_W(W)                       \
   if (phase == 0) OFH_MX_DIA(0, true, W);     \
   else if (phase == 1) OFH_MX_DIA(1, true, W); \
@@ -974,6 +986,7 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
         }
 #undef OFH_MX_DIA_W
 #undef OFH_MX_DIA
+#undef OFH_MX_DIA_S
         return;
       }
     }

@@ -254,6 +254,78 @@ __device__ __forceinline__ T dia_rowsum_g(int row, const DiaRows<T>& da, const G
   return s;
 }
 
+// Symmetric diagonal copy, for a matrix whose diagonals come in pairs +-o and
+// whose values are symmetric bit for bit (a_ij == a_ji, checked when the copy
+// is made): row i keeps K = W / 2 slots -- slot 0 the diagonal, slot k the
+// value at column i + up[k] (up ascending; unused slots 0) -- in the same
+// chunked, slot-major layout, and the lower value a_{i, i-up[k]} is read as
+// a_{i-up[k], i}, slot k of row i - up[k]: across a wave one more contiguous
+// run per diagonal, read at about the time the partner rows' own workgroup
+// reads it.  A 16-bit mask per row holds the present entries in summation
+// order: bit K-1-k the lower entry k, bit K-1 the diagonal, bit K-1+k the
+// upper entry k.  The sum runs over the lower entries (k from K-1 down to
+// 1), the diagonal and the upper entries in ascending column order from
+// +0.0: the CSR row sum bit for bit, with half the value bytes.
+template <typename T>
+struct DiaSymRows {
+  int n;
+  const T* __restrict__ u;         // ceil(n / 64) * K * 64
+  const uint16_t* __restrict__ m;  // n
+  int up[8];                       // up[0] = 0 (the diagonal)
+};
+
+template <int K, typename T, class G>
+__device__ __forceinline__ T dias_rowsum_g(int row, const DiaSymRows<T>& ds, const G& g) {
+  static_assert(K == 2 || K == 4 || K == 8, "2, 4 or 8 slots per row");
+  constexpr int cs = K * kDiaChunk;
+  const T* __restrict__ base = ds.u + (size_t)(row / kDiaChunk) * cs + (row % kDiaChunk);
+  T au[K], al[K], xu[K], xl[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) au[k] = base[k * kDiaChunk];
+#pragma unroll
+  for (int k = 1; k < K; ++k) {
+    // the partner row; below row 0 the entry is absent (masked): any valid row
+    const int pr = max(row - ds.up[k], 0);
+    al[k] = ds.u[(size_t)(pr / kDiaChunk) * cs + k * kDiaChunk + (pr % kDiaChunk)];
+    xl[k] = g((uint32_t)(row - ds.up[k]));
+    xu[k] = g((uint32_t)(row + ds.up[k]));
+  }
+  xu[0] = g((uint32_t)row);
+  const uint32_t m = ds.m[row];
+  T s = T(0);
+#pragma unroll
+  for (int k = K - 1; k >= 1; --k) {
+    const T t = s + al[k] * xl[k];
+    s = (m >> (K - 1 - k) & 1u) ? t : s;
+  }
+  {
+    const T t = s + au[0] * xu[0];
+    s = (m >> (K - 1) & 1u) ? t : s;
+  }
+#pragma unroll
+  for (int k = 1; k < K; ++k) {
+    const T t = s + au[k] * xu[k];
+    s = (m >> (K - 1 + k) & 1u) ? t : s;
+  }
+  return s;
+}
+
+// The row sum from the diagonal copy of W slots per row, or (SYM) from its
+// symmetric form of W / 2 slots.
+template <int W, bool SYM, typename T, class G>
+__device__ __forceinline__ T dia_row(int row, const DiaRows<T>& da, const DiaSymRows<T>& ds,
+                                     const G& g) {
+  if constexpr (SYM)
+    return dias_rowsum_g<W / 2>(row, ds, g);
+  else
+    return dia_rowsum_g<W>(row, da, g);
+}
+
+// The symmetric copies as the kernels take them (host side): FP64, FP32,
+// FP32 of D^-1/2 A D^-1/2.
+DiaSymRows<double> dias_f64(ofh_csr A);
+DiaSymRows<float> dias_f32(ofh_csr A, bool scaled);
+
 template <typename T, int TILE, bool SCALED = false>
 __device__ __forceinline__ T csr_stream_rowsum(int r0, int r1, const int* __restrict__ rp,
                                                ColIndex ci, const T* __restrict__ val,

@@ -241,6 +241,7 @@ static int dia_detect(ofh_csr A) {
   // walks the sorted list
   std::vector<int> offs;
   bool fits = A->nnz > 0;
+  int64_t n_lower = 0, n_upper = 0;
   for (int i = 0; i < n && fits; ++i) {
     size_t j = 0;
     for (int k = rp[i]; k < rp[i + 1]; ++k) {
@@ -249,6 +250,8 @@ static int dia_detect(ofh_csr A) {
         break;
       }
       const int o = col[k] - i;
+      n_lower += o < 0;
+      n_upper += o > 0;
       while (j < offs.size() && offs[j] < o) ++j;
       if (j == offs.size() || offs[j] != o) {
         if ((int)offs.size() == kDiaMaxDiag) {
@@ -268,6 +271,14 @@ static int dia_detect(ofh_csr A) {
   A->dia_nd = nd;
   A->dia_w = nd <= 3 ? 4 : nd <= 7 ? 8 : 16;
   for (int d = 0; d < 16; ++d) A->dia_off[d] = d < nd ? offs[d] : 0;
+  // the symmetric form needs the diagonals in pairs +-o around the main one
+  // and as many entries above it as below (with every upper entry's mirror
+  // present, checked with the values, the patterns then match)
+  const int h = (nd - 1) / 2;
+  bool pairs = nd % 2 == 1 && offs[h] == 0 && n_lower == n_upper;
+  for (int d = 0; d < h && pairs; ++d) pairs = offs[d] == -offs[nd - 1 - d];
+  A->dias_pat = pairs;
+  for (int k = 0; k < 8; ++k) A->dias_up[k] = pairs && k >= 1 && k <= h ? offs[h + k] : 0;
   // each copy only where it takes no more bytes than the SELL-64 copy of the
   // same precision with 16-bit columns (6 and 10 bytes per entry)
   const int64_t rows = (int64_t)((n + kDiaChunk - 1) / kDiaChunk) * kDiaChunk;
@@ -345,6 +356,147 @@ int csr_ensure_dia_scaled(ofh_csr A, bool* ok) {
   return OFH_OK;
 }
 
+// *bad = 1 unless every entry above the diagonal has its mirror a_ji with
+// the same bits (thread = row; rows of a diagonal copy hold <= 15 entries)
+__global__ void k_dias_check(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                             const double* __restrict__ val, int* __restrict__ bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int k = rp[i]; k < rp[i + 1]; ++k) {
+    const int c = col[k];
+    if (c <= i) continue;
+    bool same = false;
+    for (int q = rp[c]; q < rp[c + 1]; ++q)
+      if (col[q] == i) {
+        same = __builtin_bit_cast(uint64_t, val[q]) == __builtin_bit_cast(uint64_t, val[k]);
+        break;
+      }
+    if (!same) *bad = 1;  // every writer stores the same value
+  }
+}
+
+struct DiaSymOffsets {
+  int up[8];
+};
+
+// thread = row: the diagonal and the upper values into the K slots of the
+// symmetric copy (scale: of D^-1/2 A D^-1/2), and the row's mask
+template <int K, typename T>
+__global__ __launch_bounds__(kBlock) void k_dias_vals(int n, const int* __restrict__ rp,
+                                                      const int* __restrict__ col,
+                                                      const double* __restrict__ val,
+                                                      DiaSymOffsets o,
+                                                      const double* __restrict__ scale,
+                                                      T* __restrict__ out,
+                                                      uint16_t* __restrict__ mask) {
+  const int row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= n) return;
+  T r[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) r[k] = T(0);
+  uint32_t m = 0;
+  const double si = scale ? scale[row] : 1.0;
+  for (int q = rp[row]; q < rp[row + 1]; ++q) {
+    const int c = col[q] - row;
+    const T v = scale ? (T)(val[q] * (si * scale[col[q]])) : (T)val[q];
+    if (c == 0) {
+      r[0] = v;
+      m |= 1u << (K - 1);
+    }
+#pragma unroll
+    for (int k = 1; k < K; ++k) {
+      if (c == o.up[k]) {
+        r[k] = v;
+        m |= 1u << (K - 1 + k);
+      }
+      if (c == -o.up[k]) m |= 1u << (K - 1 - k);  // read from the partner row
+    }
+  }
+  T* base = out + (size_t)(row / kDiaChunk) * (K * kDiaChunk) + (row % kDiaChunk);
+#pragma unroll
+  for (int k = 0; k < K; ++k) base[k * kDiaChunk] = r[k];
+  mask[row] = (uint16_t)m;
+}
+
+template <typename T>
+static int launch_dias_vals(ofh_csr A, const double* scale, T* out) {
+  DiaSymOffsets o{};
+  for (int k = 0; k < 8; ++k) o.up[k] = A->dias_up[k];
+  const int n = A->n;
+  const int grid = (n + kBlock - 1) / kBlock;
+  hipStream_t s = A->ctx->stream;
+  const int K = A->dia_w / 2;
+  if (K == 2)
+    hipLaunchKernelGGL((k_dias_vals<2, T>), dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
+                       A->val, o, scale, out, A->dias_mask);
+  else if (K == 4)
+    hipLaunchKernelGGL((k_dias_vals<4, T>), dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
+                       A->val, o, scale, out, A->dias_mask);
+  else
+    hipLaunchKernelGGL((k_dias_vals<8, T>), dim3(grid), dim3(kBlock), 0, s, n, A->row_ptr, A->col,
+                       A->val, o, scale, out, A->dias_mask);
+  OFH_HIP(hipGetLastError());
+  return OFH_OK;
+}
+
+int csr_ensure_dias(ofh_csr A, int kind, bool* ok) {
+  *ok = false;
+  int rc = dia_detect(A);
+  if (rc || A->dia_state < 0 || !A->dias_pat) return rc;
+  if (kind == 0 ? !A->dia_use64 : !A->dia_use32) return OFH_OK;
+  if (kind == 2 && !(A->dscale_ok && A->dscale_version == A->version)) return OFH_OK;
+  hipStream_t s = A->ctx->stream;
+  const int n = A->n;
+  if (A->dias_check_version != A->version) {
+    int* bad = (int*)A->ctx->dev_scratch;
+    OFH_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_dias_check, dim3((n + 255) / 256), dim3(256), 0, s, n, A->row_ptr,
+                       A->col, A->val, bad);
+    OFH_HIP(hipGetLastError());
+    int hbad = 0;
+    OFH_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    OFH_HIP(hipStreamSynchronize(s));
+    A->dias_ok = hbad == 0;
+    A->dias_check_version = A->version;
+  }
+  if (!A->dias_ok) return OFH_OK;
+  const size_t elems = dia_elems(A) / 2;
+  if (!A->dias_mask) OFH_HIP(hipMalloc(&A->dias_mask, sizeof(uint16_t) * (size_t)n));
+  if (kind == 0) {
+    if (!A->dias_val64) OFH_HIP(hipMalloc(&A->dias_val64, sizeof(double) * elems));
+    if (A->dias_version64 != A->version) {
+      if ((rc = launch_dias_vals<double>(A, nullptr, A->dias_val64))) return rc;
+      A->dias_version64 = A->version;
+    }
+  } else if (kind == 1) {
+    if (!A->dias_val32) OFH_HIP(hipMalloc(&A->dias_val32, sizeof(float) * elems));
+    if (A->dias_version32 != A->version) {
+      if ((rc = launch_dias_vals<float>(A, nullptr, A->dias_val32))) return rc;
+      A->dias_version32 = A->version;
+    }
+  } else {
+    if (!A->dias_val32s) OFH_HIP(hipMalloc(&A->dias_val32s, sizeof(float) * elems));
+    if (A->dias_version32s != A->version) {
+      if ((rc = launch_dias_vals<float>(A, A->dscale, A->dias_val32s))) return rc;
+      A->dias_version32s = A->version;
+    }
+  }
+  *ok = true;
+  return OFH_OK;
+}
+
+DiaSymRows<double> dias_f64(ofh_csr A) {
+  DiaSymRows<double> d{A->n, A->dias_val64, A->dias_mask, {}};
+  for (int k = 0; k < 8; ++k) d.up[k] = A->dias_up[k];
+  return d;
+}
+
+DiaSymRows<float> dias_f32(ofh_csr A, bool scaled) {
+  DiaSymRows<float> d{A->n, scaled ? A->dias_val32s : A->dias_val32, A->dias_mask, {}};
+  for (int k = 0; k < 8; ++k) d.up[k] = A->dias_up[k];
+  return d;
+}
+
 DiaF32 dia_f32(ofh_csr A) {
   DiaF32 d{A->n, A->dia_val32, {}};
   for (int k = 0; k < kDiaMaxDiag; ++k) d.off[k] = A->dia_off[k];
@@ -357,25 +509,36 @@ DiaF64 dia_f64(ofh_csr A) {
   return d;
 }
 
-// y = A x from a diagonal copy
-template <int W, typename T>
-__global__ __launch_bounds__(kBlock) void k_spmv_dia(int nblocks, DiaRows<T> da, const T* x,
+// y = A x from a diagonal copy (SYM: its symmetric form)
+template <int W, typename T, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_spmv_dia(int nblocks, DiaRows<T> da,
+                                                     DiaSymRows<T> ds, const T* x,
                                                      T* __restrict__ y) {
   const int row = xcd_block(blockIdx.x, nblocks) * kBlock + threadIdx.x;
   if (row >= da.n) return;
-  y[row] = dia_rowsum_g<W>(row, da, SellGather<T>{x_rsrc(x, da.n)});
+  y[row] = dia_row<W, SYM>(row, da, ds, SellGather<T>{x_rsrc(x, da.n)});
 }
 
 template <typename T>
-static int launch_spmv_dia(ofh_csr A, const DiaRows<T>& da, const T* x, T* y, int kind,
-                           hipStream_t s) {
+static int launch_spmv_dia(ofh_csr A, const DiaRows<T>& da, const DiaSymRows<T>* ds, const T* x,
+                           T* y, int kind, hipStream_t s) {
   const int nb = (A->n + kBlock - 1) / kBlock;
-  if (A->dia_w == 4)
-    prof_kernel(A->ctx, kind, k_spmv_dia<4, T>, dim3(nb), dim3(kBlock), s, nb, da, x, y);
-  else if (A->dia_w == 8)
-    prof_kernel(A->ctx, kind, k_spmv_dia<8, T>, dim3(nb), dim3(kBlock), s, nb, da, x, y);
-  else
-    prof_kernel(A->ctx, kind, k_spmv_dia<16, T>, dim3(nb), dim3(kBlock), s, nb, da, x, y);
+  const DiaSymRows<T> dsv = ds ? *ds : DiaSymRows<T>{};
+#define OFH_SPMV_DIA(W)                                                                         \
+  if (ds)                                                                                       \
+    prof_kernel(A->ctx, kind, k_spmv_dia<W, T, true>, dim3(nb), dim3(kBlock), s, nb, da, dsv, x, \
+                y);                                                                             \
+  else                                                                                          \
+    prof_kernel(A->ctx, kind, k_spmv_dia<W, T, false>, dim3(nb), dim3(kBlock), s, nb, da, dsv, \
+                x, y);
+  if (A->dia_w == 4) {
+    OFH_SPMV_DIA(4)
+  } else if (A->dia_w == 8) {
+    OFH_SPMV_DIA(8)
+  } else {
+    OFH_SPMV_DIA(16)
+  }
+#undef OFH_SPMV_DIA
   OFH_HIP(hipGetLastError());
   return OFH_OK;
 }
@@ -429,19 +592,21 @@ static int launch(ofh_csr A, const T* val, const T* x, T* y, T alpha, hipStream_
 }  // namespace ofh
 
 extern "C" int ofh_csr_dia_info(ofh_csr A, int* in_use32, int* in_use64, int* diagonals,
-                                int* row_slots) {
+                                int* row_slots, int* symmetric) {
   using namespace ofh;
-  OFH_CHECK(A && in_use32 && in_use64 && diagonals && row_slots, OFH_E_INVALID,
+  OFH_CHECK(A && in_use32 && in_use64 && diagonals && row_slots && symmetric, OFH_E_INVALID,
             "ofh_csr_dia_info: null argument");
-  bool ok32 = false, ok64 = false;
+  bool ok32 = false, ok64 = false, sym = false;
   int rc = csr_ensure_dia(A, &ok32);
   if (!rc) rc = csr_ensure_dia64(A, &ok64);
+  if (!rc && (ok32 || ok64)) rc = csr_ensure_dias(A, ok64 ? 0 : 1, &sym);
   if (rc) return rc;
   OFH_HIP(hipStreamSynchronize(A->ctx->stream));
   *in_use32 = ok32 ? 1 : 0;
   *in_use64 = ok64 ? 1 : 0;
   *diagonals = A->dia_state > 0 ? A->dia_nd : 0;
   *row_slots = A->dia_state > 0 ? A->dia_w : 0;
+  *symmetric = sym ? 1 : 0;
   return OFH_OK;
 }
 
@@ -464,7 +629,12 @@ int launch_spmv64(ofh_csr A, const double* x, double* y, hipStream_t s) {
   bool dia;
   int rc = csr_ensure_dia64(A, &dia);
   if (rc) return rc;
-  if (dia) return launch_spmv_dia<double>(A, dia_f64(A), x, y, OFH_PROF_SPMV64, s);
+  if (dia) {
+    bool sym;
+    if ((rc = csr_ensure_dias(A, 0, &sym))) return rc;
+    const DiaSymRows<double> ds = sym ? dias_f64(A) : DiaSymRows<double>{};
+    return launch_spmv_dia<double>(A, dia_f64(A), sym ? &ds : nullptr, x, y, OFH_PROF_SPMV64, s);
+  }
   bool sell;
   if ((rc = csr_ensure_sell(A, false, &sell))) return rc;
   if (sell) {
@@ -481,7 +651,12 @@ int launch_spmv32(ofh_csr A, const float* x, float* y, hipStream_t s) {
   bool dia;
   int rc = csr_ensure_dia(A, &dia);
   if (rc) return rc;
-  if (dia) return launch_spmv_dia<float>(A, dia_f32(A), x, y, OFH_PROF_SPMV32, s);
+  if (dia) {
+    bool sym;
+    if ((rc = csr_ensure_dias(A, 1, &sym))) return rc;
+    const DiaSymRows<float> ds = sym ? dias_f32(A, false) : DiaSymRows<float>{};
+    return launch_spmv_dia<float>(A, dia_f32(A), sym ? &ds : nullptr, x, y, OFH_PROF_SPMV32, s);
+  }
   bool sell;
   if ((rc = csr_ensure_sell(A, true, &sell))) return rc;
   if (sell) {

@@ -133,40 +133,48 @@ def test_spmv_f32_diagonal_copy_bitexact(ctx, case):
     per row; a tridiagonal matrix: 4) bitwise the sequential float32 CSR row
     sums, and ofh_spmv_f64 from the FP64 copy (the 3D grid; the tridiagonal
     matrix keeps SELL-64, whose 3 entries per row take fewer bytes) bitwise
-    the oracle's, after a change of the values (the copies are gathered
+    the oracle's, through changes of the values (the copies are gathered
     again) and with non-finite x entries: a row's absent diagonals are masked
-    out, so an inf or nan only reaches the rows that really couple to it."""
+    out, so an inf or nan only reaches the rows that really couple to it.
+    The 3D grid's values are symmetric bit for bit, so the products read the
+    symmetric form (4 slots per row, lower values from their mirrors); values
+    scaled entry by entry are not, and the full copy returns; values doubled
+    are symmetric again.  The random tridiagonal matrix is not symmetric."""
     import ofeli_amd as oa
     torch = _torch()
     rp, ci, a = grid_system((28, 28, 28), 31) if case == "grid3d" else tridiag_system()
     A = gpu_csr(ctx, rp, ci, a)
-    want = {"grid3d": (True, 7, 8), "tridiag": (False, 3, 4)}[case]
+    want = {"grid3d": (True, 7, 8, True), "tridiag": (False, 3, 4, False)}[case]
     assert A.dia_info() == {"fp32": True, "fp64": want[0], "diagonals": want[1],
-                            "row_slots": want[2]}
+                            "row_slots": want[2], "symmetric": want[3]}
     x64 = np.random.default_rng(34).standard_normal(A.n)
     y64 = torch.zeros(A.n, dtype=torch.float64, device="cuda")
     rng = np.random.default_rng(33)
     x32 = rng.standard_normal(A.n).astype(np.float32)
     y32 = torch.zeros(A.n, dtype=torch.float32, device="cuda")
-    for vals in (a, a * (1.0 + 0.5 * np.cos(np.arange(len(a))))):
+    unsym = a * (1.0 + 0.5 * np.cos(np.arange(len(a))))
+    for vals, sym in ((a, want[3]), (unsym, False), (2.0 * a, want[3]), (unsym, False)):
         if vals is not a:
             A.set_values(vals)
+        assert A.dia_info()["symmetric"] is sym
         oa.spmv_f32(A, torch.as_tensor(x32, device="cuda"), y32)
         assert np.array_equal(y32.cpu().numpy(), seq_f32(rp, ci, vals.astype(np.float32), x32))
         oa.spmv(A, dev(x64), y64)
         assert np.array_equal(host(y64), O.spmv(rp, ci, vals, x64))
-    xs = x32.copy()
-    xs[[0, 5, A.n - 1, A.n // 2]] = [np.inf, np.nan, -np.inf, np.nan]
-    oa.spmv_f32(A, torch.as_tensor(xs, device="cuda"), y32)
-    ref = seq_f32(rp, ci, vals.astype(np.float32), xs)
-    got = y32.cpu().numpy()
-    assert np.isfinite(ref).sum() > A.n - 40
-    assert np.array_equal(got, ref, equal_nan=True)
-    xs64 = x64.copy()
-    xs64[[0, 5, A.n - 1, A.n // 2]] = [np.inf, np.nan, -np.inf, np.nan]
-    oa.spmv(A, dev(xs64), y64)
-    with np.errstate(invalid="ignore"):
-        assert np.array_equal(host(y64), O.spmv(rp, ci, vals, xs64), equal_nan=True)
+        if vals is a:
+            continue
+        xs = x32.copy()
+        xs[[0, 5, A.n - 1, A.n // 2]] = [np.inf, np.nan, -np.inf, np.nan]
+        oa.spmv_f32(A, torch.as_tensor(xs, device="cuda"), y32)
+        ref = seq_f32(rp, ci, vals.astype(np.float32), xs)
+        got = y32.cpu().numpy()
+        assert np.isfinite(ref).sum() > A.n - 40
+        assert np.array_equal(got, ref, equal_nan=True)
+        xs64 = x64.copy()
+        xs64[[0, 5, A.n - 1, A.n // 2]] = [np.inf, np.nan, -np.inf, np.nan]
+        oa.spmv(A, dev(xs64), y64)
+        with np.errstate(invalid="ignore"):
+            assert np.array_equal(host(y64), O.spmv(rp, ci, vals, xs64), equal_nan=True)
 
 
 def test_mixed_cg_diag_scaled_and_unscaled_inner_iterations(ctx):

@@ -75,11 +75,15 @@ def test_products_read_the_diagonal_copies(system):
     """The Kuhn numbering of the free nodes is a regular grid: every entry lies
     on one of 15 diagonals, so ofh_spmv_f32 and the mixed CG's inner product
     read the FP32 diagonal copy and ofh_spmv_f64 and FP64 CG the FP64 one (16
-    slots per row), bitwise the CSR row sums."""
+    slots per row), bitwise the CSR row sums.  The assembled values are
+    symmetric bit for bit (the element matrices are, and every entry sums
+    its elements in ascending order), so the products read the symmetric
+    form (8 slots per row)."""
     import torch
     import ofeli_amd as oa
     A = system["P"].A
-    assert A.dia_info() == {"fp32": True, "fp64": True, "diagonals": 15, "row_slots": 16}
+    assert A.dia_info() == {"fp32": True, "fp64": True, "diagonals": 15, "row_slots": 16,
+                            "symmetric": True}
     x64 = np.random.default_rng(6).standard_normal(A.n)
     y64 = torch.zeros(A.n, dtype=torch.float64, device="cuda")
     oa.spmv(A, torch.as_tensor(x64, device="cuda"), y64)
