@@ -45,23 +45,30 @@ constexpr int kScaledDiag = 100;
 
 // Inner iterations to enqueue before the host looks at the state again.  The
 // device pauses by itself at the replacement point and the launches queued
-// past it return at once, but each still costs its dispatch and starves the
-// GPU while the host issues it; a look costs one round trip.  So approach the
-// pause from below: 3/4 of the predicted distance (right after a replacement
-// the previous cycle's length, otherwise the residual's rate over the last
-// chunk), at least 2.
-inline int mixed_chunk(int it, double res, int it_prev, double res_prev, double target,
-                       int remaining, int cycle_start, int last_cycle) {
-  int c;
-  if (it == cycle_start && last_cycle > 0) {
-    c = last_cycle * 3 / 4;
-  } else if (it > it_prev && res_prev > 0 && res > 0 && res < res_prev && res > target) {
-    const double rate = std::pow(res / res_prev, 1.0 / (it - it_prev));
-    c = (int)(0.75 * std::log(target / res) / std::log(rate));
-  } else {
-    c = 8;
+// past it return at once, but each still costs its dispatch; a look costs one
+// round trip.  So approach the pause from below: 0.9 of the predicted
+// distance from the residual `res` after `it` iterations to `target`, the
+// rate taken over the whole cycle so far (from `res_start` at `cycle_start`:
+// the norm of a CG residual is not monotone, and a rate over one short chunk
+// can be ~1 on a plateau); at the start of a cycle the previous cycle's
+// length, scaled by how far this one has to go against a drop of kDelta (the
+// last cycle only has to reach the tolerance).  At least 2, at most the
+// previous cycle's length, at most max(32, it) while the solve is young.
+// (512^2 heat steps with projected starts: 2.73 ms per step against 2.90 ms
+// with a rate over the last chunk and 3/4 of the distance, which enqueued up
+// to 190 launches past the pause per solve.)
+inline int mixed_chunk(int it, double res, int cycle_start, double res_start, double target,
+                       int remaining, int last_cycle) {
+  int c = 8;  // no usable rate yet
+  const double need = std::log(target / res);  // < 0 while res is above target
+  if (it == cycle_start) {
+    if (last_cycle > 0) c = (int)(0.9 * last_cycle * need / std::log(kDelta));
+  } else if (res > 0 && res < res_start && need < 0) {
+    const double lr = std::log(res / res_start) / (it - cycle_start);  // < 0
+    c = (int)(0.9 * need / lr);
   }
-  c = std::max(2, std::min(c, std::max(16, it)));
+  if (last_cycle > 0) c = std::min(c, last_cycle);
+  c = std::max(2, std::min(c, std::max(32, it)));
   return std::max(1, std::min(c, remaining));
 }
 
@@ -1157,17 +1164,13 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   KrylovState h{};
   if ((rc = read_state(ctx, &h))) return rc;
   double last_true = h.res;
-  int stagnant = 0, slow = 0, fallback = 0, it_prev = 0, it_repl = 0, first_cycle = 0;
+  int stagnant = 0, slow = 0, fallback = 0, it_repl = 0, first_cycle = 0;
   int last_cycle = 0;
   double best_rate = 0.0;
-  double res_prev = h.res;
   while (!h.done) {
     int k = h.it;
     const double target = std::max(kDelta * h.rmax, toler);
-    const int chunk =
-        mixed_chunk(k, h.res, it_prev, res_prev, target, max_it - k, it_repl, last_cycle);
-    it_prev = k;
-    res_prev = h.res;
+    const int chunk = mixed_chunk(k, h.res, it_repl, h.rmax, target, max_it - k, last_cycle);
     // whole captured chunks, then the remainder launched directly
     const int full = chunk_graph ? chunk / kGraphIters : 0;
     for (int c = 0; c < full; ++c) OFH_HIP(hipGraphLaunch(chunk_graph, s));
@@ -1205,8 +1208,6 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     }
     OFH_HIP(hipGetLastError());
     if ((rc = read_state(ctx, &h))) return rc;
-    it_prev = h.it;
-    res_prev = h.res;
     if (h.done) break;
     // stagnation: the true residual must keep falling ...
     stagnant = (h.res > 0.5 * last_true) ? stagnant + 1 : 0;
