@@ -205,6 +205,57 @@ def test_mixed_cg_diag_scaled_and_unscaled_inner_iterations(ctx):
     assert out[1.0] <= 1.1 * it64 + 5
 
 
+def _drop_entry(rp, ci, a, k):
+    """The CSR arrays without stored entry k."""
+    row = int(np.searchsorted(rp, k, side="right")) - 1
+    rp2 = rp.copy()
+    rp2[row + 1:] -= 1
+    return rp2, np.delete(ci, k), np.delete(a, k)
+
+
+def _entry(rp, ci, i, j):
+    """Index of stored entry (i, j), 0-based row and column."""
+    k = rp[i] + int(np.searchsorted(ci[rp[i]:rp[i + 1]], j + 1))
+    assert ci[k] == j + 1
+    return k
+
+
+def test_symmetric_form_needs_mirrored_bits(ctx):
+    """The products read the symmetric form of the diagonal copy only where
+    every entry's mirror is stored with the same 64 bits: the 3D grid system
+    does; one value moved by one ulp, a mirrored pair stored as +0.0 / -0.0,
+    or one off-diagonal entry dropped without its mirror (the diagonals still
+    pair up) do not; a mirrored pair dropped together does.  In every case
+    ofh_spmv_f64 and ofh_spmv_f32 stay the CSR row sums bit for bit."""
+    import ofeli_amd as oa
+    torch = _torch()
+    rp, ci, a = grid_system((12, 12, 12), 31)
+    n = len(rp) - 1
+    i, j = 500, 501  # a coupled pair of rows
+    kij, kji = _entry(rp, ci, i, j), _entry(rp, ci, j, i)
+    ulp = a.copy()
+    ulp[kij] = np.nextafter(ulp[kij], np.inf)
+    zeros = a.copy()
+    zeros[kij], zeros[kji] = 0.0, -0.0
+    one = _drop_entry(rp, ci, a, kij)
+    both = _drop_entry(*_drop_entry(rp, ci, a, max(kij, kji)), min(kij, kji))
+    cases = [((rp, ci, a), True), ((rp, ci, ulp), False), ((rp, ci, zeros), False),
+             (one, False), (both, True)]
+    x64 = np.random.default_rng(41).standard_normal(n)
+    x32 = x64.astype(np.float32)
+    for (r, c, v), sym in cases:
+        A = gpu_csr(ctx, r, c, v)
+        info = A.dia_info()
+        assert info["fp32"] and info["diagonals"] == 7 and info["symmetric"] is sym
+        y64 = torch.zeros(n, dtype=torch.float64, device="cuda")
+        oa.spmv(A, dev(x64), y64)
+        assert np.array_equal(host(y64), O.spmv(r, c, v, x64))
+        y32 = torch.zeros(n, dtype=torch.float32, device="cuda")
+        oa.spmv_f32(A, torch.as_tensor(x32, device="cuda"), y32)
+        assert np.array_equal(y32.cpu().numpy(), seq_f32(r, c, v.astype(np.float32), x32))
+        A.close()
+
+
 def test_diagonal_copy_not_used_where_it_does_not_pay(ctx):
     """A 2D 5-point grid (5 diagonals: 8 floats per row against ~30 bytes of
     SELL-64 entries) and a matrix with far couplings (5 diagonals, but mostly
