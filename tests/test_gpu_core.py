@@ -631,18 +631,20 @@ def sym_tridiag_system(n=6000, seed=11):
     return M.indptr.astype(np.int64), M.indices.astype(np.int64) + 1, M.data, b
 
 
-@pytest.mark.parametrize("system", ["dc2d", "tridiag"])
+@pytest.mark.parametrize("system", ["dc2d", "tridiag", "tiny"])
 @pytest.mark.parametrize("prec", ["DIAG_PREC", "IDENT_PREC"])
 def test_mixed_cg_one_and_two_kernel_inner_forms(ctx, prec, system):
     """ofh_ctx_mixed_form: the single-reduction inner iteration (one kernel per
     inner iteration, z = r on the FP32 diagonal copy; 8 and 4 slots per row
-    here) and the two-kernel form both stop on the true FP64 residual, at the
-    oracle's rtol-1e-12 solution to 1e-7 with about the FP64 iteration count
-    of the same preconditioner.  The per-kind kernel timings show which form
+    here; "tiny": 63 rows, less than one workgroup's rows) and the two-kernel
+    form both stop on the true FP64 residual, at the oracle's rtol-1e-12
+    solution to 1e-7 with about the FP64 iteration count of the same
+    preconditioner.  The per-kind kernel timings show which form
     ran.  The max_it convention (-(max_it+1), max_it inner iterations) holds
     for both."""
     import ofeli_amd as oa
-    rp, ci, a, b = dc2d_system(64) if system == "dc2d" else sym_tridiag_system()
+    rp, ci, a, b = (dc2d_system(64) if system == "dc2d" else
+                    sym_tridiag_system(63) if system == "tiny" else sym_tridiag_system())
     A = gpu_csr(ctx, rp, ci, a)
     assert A.dia_info()["fp32"] and A.dia_info()["row_slots"] == (8 if system == "dc2d" else 4)
     _, xo, _ = O.cg(rp, ci, a, b, max_it=20000, toler=1e-12)
