@@ -90,7 +90,8 @@ int ofh_ctx_use_graphs(ofh_ctx ctx, int enable);
  * tuning and test control): 0 (default) chooses by size, 1 takes two kernels
  * per inner iteration (direction + product, update), 2 takes one kernel (the
  * single-reduction iteration) wherever it applies: IDENT_PREC or DIAG_PREC
- * on a matrix with an FP32 diagonal copy.  By default the one-kernel form is
+ * on a matrix with an FP32 diagonal copy whose values are symmetric bit for
+ * bit (its (p, Ap) uses the symmetry).  By default the one-kernel form is
  * taken there up to 300,000 rows with 16 slots per row of the copy and
  * 650,000 rows with 8 or 4 (measured crossovers on MI355X).  Both
  * forms meet the same FP64-verified tolerance; their iterates differ in

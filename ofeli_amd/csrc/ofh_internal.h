@@ -419,6 +419,9 @@ int csr_ensure_dia64(ofh_csr A, bool* ok);
 // *ok is false when A has no diagonal copy or a diagonal entry is not
 // positive and finite.
 int csr_ensure_dia_scaled(ofh_csr A, bool* ok);
+// *sym when the pattern has diagonals (a diagonal copy fits), they pair up
+// and the values are symmetric bit for bit (checked once per value version).
+int csr_dia_symmetric(ofh_csr A, bool* sym);
 // The symmetric form of a diagonal copy that is in use (kind 0: FP64, 1:
 // FP32, 2: FP32 of D^-1/2 A D^-1/2, after csr_ensure_dia_scaled), current
 // with the values; *ok when the matrix is symmetric bit for bit and the copy

@@ -920,15 +920,21 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     return rc;
   const bool qrec = fsell || fdia;
   const bool fused = !sweep_prec(PREC) && (n <= kFuseRows || qrec);
-  // One kernel per inner iteration (k_mx_cg1) where z = r, up to the size
-  // where its gathers of three vectors per neighbour stop hitting the L2
-  // caches.  Measured per inner iteration, two kernels vs one (mixed CG+DIAG
-  // from zero): 16 slots per row (3D P1), 250,047 rows 15.4 vs 12.4 us,
-  // 357,911 rows 16.6 vs 16.9 us, 493,039 rows 19.1 vs 20.8 us; 8 slots (2D
-  // P1), 261,121 rows 12.9 vs 9.3 us, 522,729 rows 14.6 vs 10.4 us, 808,201
-  // rows 18.7 vs 18.5 us.  ofh_ctx_mixed_form overrides the choice.
+  // One kernel per inner iteration (k_mx_cg1) where z = r and A is
+  // symmetric bit for bit (its (p, Ap) comes from three dot products through
+  // (p_old, A r) = (A p_old, r)), up to the size where its gathers of three
+  // vectors per neighbour stop hitting the L2 caches.  Measured per inner
+  // iteration, two kernels vs one (mixed CG+DIAG from zero): 16 slots per row
+  // (3D P1), 250,047 rows 15.4 vs 12.4 us, 357,911 rows 16.6 vs 16.9 us,
+  // 493,039 rows 19.1 vs 20.8 us; 8 slots (2D P1), 261,121 rows 12.9 vs
+  // 9.3 us, 522,729 rows 14.6 vs 10.4 us, 808,201 rows 18.7 vs 18.5 us.
+  // ofh_ctx_mixed_form overrides the choice.
   const int one_rows = A->dia_w == 16 ? 300000 : 650000;
-  const bool one = fdia && (PREC == kScaledDiag || PREC == OFH_IDENT_PREC) &&
+  bool symA = false;
+  if (fdia && (PREC == kScaledDiag || PREC == OFH_IDENT_PREC) && ctx->mixed_form != 1 &&
+      (rc = csr_dia_symmetric(A, &symA)))
+    return rc;
+  const bool one = fdia && symA && (PREC == kScaledDiag || PREC == OFH_IDENT_PREC) &&
                    ctx->mixed_form != 1 && (ctx->mixed_form == 2 || n <= one_rows);
   void* wsp;
   const size_t vbytes = sizeof(float) * (size_t)n * (one ? 8 : 6) +

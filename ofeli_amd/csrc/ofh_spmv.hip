@@ -439,12 +439,10 @@ static int launch_dias_vals(ofh_csr A, const double* scale, T* out) {
   return OFH_OK;
 }
 
-int csr_ensure_dias(ofh_csr A, int kind, bool* ok) {
-  *ok = false;
+int csr_dia_symmetric(ofh_csr A, bool* sym) {
+  *sym = false;
   int rc = dia_detect(A);
   if (rc || A->dia_state < 0 || !A->dias_pat) return rc;
-  if (kind == 0 ? !A->dia_use64 : !A->dia_use32) return OFH_OK;
-  if (kind == 2 && !(A->dscale_ok && A->dscale_version == A->version)) return OFH_OK;
   hipStream_t s = A->ctx->stream;
   const int n = A->n;
   if (A->dias_check_version != A->version) {
@@ -459,7 +457,18 @@ int csr_ensure_dias(ofh_csr A, int kind, bool* ok) {
     A->dias_ok = hbad == 0;
     A->dias_check_version = A->version;
   }
-  if (!A->dias_ok) return OFH_OK;
+  *sym = A->dias_ok;
+  return OFH_OK;
+}
+
+int csr_ensure_dias(ofh_csr A, int kind, bool* ok) {
+  *ok = false;
+  bool sym = false;
+  int rc = csr_dia_symmetric(A, &sym);
+  if (rc || !sym) return rc;
+  if (kind == 0 ? !A->dia_use64 : !A->dia_use32) return OFH_OK;
+  if (kind == 2 && !(A->dscale_ok && A->dscale_version == A->version)) return OFH_OK;
+  const int n = A->n;
   const size_t elems = dia_elems(A) / 2;
   if (!A->dias_mask) OFH_HIP(hipMalloc(&A->dias_mask, sizeof(uint16_t) * (size_t)n));
   if (kind == 0) {

@@ -675,6 +675,31 @@ def test_mixed_cg_one_and_two_kernel_inner_forms(ctx, prec, system):
         ctx.mixed_form(0)
 
 
+def test_mixed_cg_one_kernel_form_needs_symmetric_values(ctx):
+    """The one-kernel inner iteration forms (p, Ap) through (p_old, A r) =
+    (A p_old, r), so it runs only where the values are symmetric bit for bit:
+    one value moved by one ulp makes even a requested one-kernel form take the
+    two-kernel iteration (no k_mx_cg1 launch), which still meets the FP64
+    tolerance."""
+    import ofeli_amd as oa
+    rp, ci, a, b = sym_tridiag_system()
+    a = a.copy()
+    a[5] = np.nextafter(a[5], -np.inf)
+    A = gpu_csr(ctx, rp, ci, a)
+    assert A.dia_info()["fp32"] and not A.dia_info()["symmetric"]
+    try:
+        ctx.mixed_form(2)
+        ctx.profile(True)
+        x = _torch().zeros(A.n, dtype=_torch().float64, device="cuda")
+        info = oa.cg_mixed(A, dev(b), x, prec=oa.DIAG_PREC, max_it=20000, toler=1e-8)
+        assert len(ctx.profile_read(oa.PROF_MX_ITER)) == 0
+        assert len(ctx.profile_read(oa.PROF_MX_UPD)) > 0
+    finally:
+        ctx.profile(False)
+        ctx.mixed_form(0)
+    assert info.nb_it > 0 and info.fallback == 0 and info.true_relres <= 1e-8
+
+
 def test_mixed_cg_many_partials(ctx):
     """Mixed CG on 2,000,000 rows: more update partials than the product
     kernel prefetches (it folds them after the product instead); the solve
