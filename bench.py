@@ -98,8 +98,8 @@ def main():
 
 # ---------------------------------------------------------------- config 1
 def b_spmv64(n, nnz):
-    """Algorithmic bytes of y = A x, FP64 CSR with int32 indices (the survey's
-    B_spmv64): values and columns once, row pointers, x once, y written."""
+    """Algorithmic bytes of y = A x, FP64 CSR with int32 indices (B_spmv64):
+    values and columns once, row pointers, x once, y written."""
     return 12 * nnz + 4 * (n + 1) + 16 * n
 
 
@@ -112,9 +112,9 @@ def kernel_bytes(n, nnz):
     """Algorithmic bytes per launch of each kernel of the CG iterations
     (FP64: B_spmv64 + 88 n per iteration; mixed: B_spmv32 + 36 n)."""
     return {
-        # mixed, two kernels: the product of z (the survey's FP32 SpMV bytes;
-        # it also reads p_old, q_old and writes p_new beside q, which the
-        # survey counts in the iteration's 44n vector term), and the update
+        # mixed, two kernels: the product of z (the FP32 CSR SpMV bytes; it
+        # also reads p_old, q_old and writes p_new beside q, which B_cg32_iter
+        # counts in the iteration's 44n vector term), and the update
         # on the scaled system D^-1/2 A D^-1/2 (reads x, p, r, q; writes x, r)
         "k_mx_dirspmv": b_spmv32(n, nnz),
         "k_mx_upd": 24 * n,
@@ -552,7 +552,7 @@ def bench_poisson3d(args):
                       "recurrence, direction, (p,q) partials) -- "
                       + dia_layout(dia, "FP32 D^-1/2 A D^-1/2"),
             "bytes_per_launch": dom["bytes_per_launch"], "avg_ms": round(dom["avg_us"] / 1e3, 5),
-            "bytes_definition": "B_spmv32 = 8 nnz + 4(n+1) + 8n (the survey's FP32 CSR SpMV)",
+            "bytes_definition": "B_spmv32 = 8 nnz + 4(n+1) + 8n (the FP32 CSR SpMV)",
             "own_bytes_per_launch": (dia_row_bytes(dia, 4) + 20) * n,
             "own_bytes_definition": "matrix copy %d n + z 4n + p_old, q_old read, p_new, q "
                                     "written 16n" % dia_row_bytes(dia, 4),
@@ -563,7 +563,7 @@ def bench_poisson3d(args):
         "mixed_iteration": None if dom is None or "k_mx_upd" not in kernels else {
             "us": round(dom["avg_us"] + kernels["k_mx_upd"]["avg_us"], 2),
             "bytes_per_iteration": b_spmv32(n, nnz) + 44 * n,
-            "bytes_definition": "B_cg32_iter = B_spmv32 + 44n (the survey's FP32 inner iteration)",
+            "bytes_definition": "B_cg32_iter = B_spmv32 + 44n (the FP32 CSR inner CG iteration)",
             "GBs": round((b_spmv32(n, nnz) + 44 * n)
                          / ((dom["avg_us"] + kernels["k_mx_upd"]["avg_us"]) * 1e3), 1),
             "frac": round((b_spmv32(n, nnz) + 44 * n)
