@@ -328,12 +328,18 @@ static int cgs_impl(ofh_csr A, const double* b, double* x, int max_it, double to
   hipLaunchKernelGGL(k_kr_init, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr, ci,
                      A->val, b, x, r, sv, part);
   KrylovState h{};
-  int enq = 0, it_prev = 0;
-  double res_prev = 0.0;
+  int enq = 0, it_prev = 0, it0 = 0;
+  double res_prev = 0.0, res0 = 0.0;
   bool first = true;
   while (!h.done && enq <= max_it) {
-    const int chunk = first ? 1 : next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it + 1 - enq);
+    const int chunk = first ? 1
+                            : next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it + 1 - enq,
+                                         it0, res0);
     if (!first) {
+      if (res0 == 0.0) {  // the first look: the rate since the start is measured from here
+        it0 = h.it;
+        res0 = h.res;
+      }
       it_prev = h.it;
       res_prev = h.res;
     }
@@ -521,12 +527,18 @@ static int bicg_impl(ofh_csr A, const double* b, double* x, int max_it, double t
   hipLaunchKernelGGL(k_kr_init, dim3(nb), dim3(kBlock), 0, s, nb, A->row_blocks, A->row_ptr, ci,
                      A->val, b, x, r, rr, part);
   KrylovState h{};
-  int enq = 0, it_prev = 0;
-  double res_prev = 0.0;
+  int enq = 0, it_prev = 0, it0 = 0;
+  double res_prev = 0.0, res0 = 0.0;
   bool first = true;
   while (!h.done && enq <= max_it) {
-    const int chunk = first ? 1 : next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it + 1 - enq);
+    const int chunk = first ? 1
+                            : next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it + 1 - enq,
+                                         it0, res0);
     if (!first) {
+      if (res0 == 0.0) {  // the first look: the rate since the start is measured from here
+        it0 = h.it;
+        res0 = h.res;
+      }
       it_prev = h.it;
       res_prev = h.res;
     }

@@ -647,8 +647,11 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
     if ((rc = read_state(ctx, &h))) return rc;
     int enq = 0, it_prev = 0;
     double res_prev = h.res;
+    const int it0 = h.it;
+    const double res0 = h.res;
     while (!h.done && enq < max_it) {
-      const int chunk = next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq);
+      const int chunk =
+          next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq, it0, res0);
       it_prev = h.it;
       res_prev = h.res;
       // whole captured chunks, then the remainder launched directly
@@ -739,8 +742,11 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
     ctx->prof = prof;
     if (rc) return rc;
   }
+  const int it0 = h.it;
+  const double res0 = h.res;
   while (!h.done && enq < max_it) {
-    const int chunk = next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq);
+    const int chunk =
+        next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq, it0, res0);
     it_prev = h.it;
     res_prev = h.res;
     // whole captured chunks, then the remainder launched directly
@@ -899,8 +905,11 @@ static int cg_noalloc_impl(ofh_csr A, double* sc, const double* b, double* x, in
   if ((rc = read(&h))) return rc;
   int enq = 0, it_prev = 0;
   double res_prev = h.res;
+  const int it0 = h.it;
+  const double res0 = h.res;
   while (!h.done && enq < max_it) {
-    const int chunk = next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq);
+    const int chunk =
+        next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq, it0, res0);
     it_prev = h.it;
     res_prev = h.res;
     for (int c = 0; c < chunk; ++c) {

@@ -381,8 +381,11 @@ static int bicgstab_impl(ofh_csr A, const double* b, double* x, int max_it, doub
   if ((rc = read_state(ctx, &h))) return rc;
   int enq = 0, it_prev = 0;
   double res_prev = h.res;
+  const int it0 = h.it;
+  const double res0 = h.res;
   while (!h.done && enq < max_it) {
-    const int chunk = next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq);
+    const int chunk =
+        next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq, it0, res0);
     it_prev = h.it;
     res_prev = h.res;
     for (int c = 0; c < chunk; ++c) {

@@ -59,15 +59,20 @@ inline int read_dev(ofh_ctx ctx, const S* d, S* h) {
 inline int read_state(ofh_ctx ctx, KrylovState* h) { return read_dev(ctx, state_ptr(ctx), h); }
 
 // Iterations to enqueue before the next host check.  The residual reduction
-// observed since the previous check predicts the remaining count; a chunk is
-// at least 4 and at most as long as the iterations already done (>= 16), so a
-// poor prediction costs at most a doubling of the no-op launches.
+// observed since the previous check predicts the remaining count, capped by
+// the prediction from the rate since the solve's first check (`res0` after
+// `it0` iterations): the norm of a Krylov residual is not monotone, and on a
+// plateau the rate over one chunk is ~1, which alone would predict hundreds
+// of iterations past the stop.  A chunk is at least 4 and at most as long as
+// the iterations already done (>= 16).
 inline int next_chunk(int it, double res, int it_prev, double res_prev, double target,
-                      int remaining) {
+                      int remaining, int it0 = 0, double res0 = 0.0) {
   int c = 8;
   if (it > it_prev && res_prev > 0 && res > 0 && res < res_prev && target > 0 && res > target) {
     const double rate = std::pow(res / res_prev, 1.0 / (it - it_prev));
-    const double pred = std::log(target / res) / std::log(rate);
+    double pred = std::log(target / res) / std::log(rate);
+    if (it > it0 && res0 > 0 && res < res0)
+      pred = std::min(pred, std::log(target / res) * (it - it0) / std::log(res / res0));
     c = (int)std::ceil(pred);
   }
   c = std::max(4, std::min(c, std::max(16, it)));
