@@ -137,17 +137,24 @@ __global__ __launch_bounds__(256) void k_mp_feat(int n, int n_pad, const int* __
 // H0 = relu(F W_enc + b_enc); one thread per (row, channel).
 __global__ __launch_bounds__(256) void k_mp_enc(int n, int n_pad, const float* __restrict__ F,
                                                 const float* __restrict__ P, float* H0) {
+  // a thread per row and 4 channels (16-byte stores)
   const int idx = blockIdx.x * 256 + threadIdx.x;
-  const int row = idx / kH, c = idx % kH;
+  const int row = idx / (kH / 4), c = 4 * (idx % (kH / 4));
   if (row >= n_pad) return;
-  float h = 0.0f;
+  float h[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   if (row < n) {
-    h = P[off_enc_b() + c];
 #pragma unroll
-    for (int f = 0; f < kFin; ++f) h += F[(size_t)row * kFin + f] * P[off_enc_W() + f * kH + c];
-    h = fmaxf(h, 0.0f);
+    for (int q = 0; q < 4; ++q) h[q] = P[off_enc_b() + c + q];
+#pragma unroll
+    for (int f = 0; f < kFin; ++f) {
+      const float ff = F[(size_t)row * kFin + f];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) h[q] += ff * P[off_enc_W() + f * kH + c + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) h[q] = fmaxf(h[q], 0.0f);
   }
-  H0[(size_t)row * kH + c] = h;
+  *reinterpret_cast<float4*>(H0 + (size_t)row * kH + c) = make_float4(h[0], h[1], h[2], h[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,27 +371,46 @@ __global__ __launch_bounds__(256) void k_mp_edge_fwd(int n, int n_pad, const int
                                                      const float* __restrict__ Z,
                                                      const float* __restrict__ we,
                                                      const float* __restrict__ bm, float* M) {
-  const int c = threadIdx.x & 63;
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  // 16 lanes per node, 4 channels per lane (16-byte loads and stores); the
+  // row's columns and edge weights are loaded lane-parallel, 16 at a time,
+  // and broadcast within the node's lanes, and the gathers of a group of 8
+  // neighbour rows are issued before any is summed.  Per channel the sum
+  // runs over the row's entries in order, as one lane per channel would.
+  constexpr int G = 16;
+  const int l = threadIdx.x & (G - 1);
+  const int i = blockIdx.x * (256 / G) + (threadIdx.x / G);
   if (i >= n_pad) return;
-  float acc = 0.f;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   if (i < n) {
-    const float za = Z[(size_t)i * kH2 + c], w = we[c], bb = bm[c];
-    // the row's columns and edge weights are loaded lane-parallel, 64 at a
-    // time, and broadcast: the gathers of Z no longer wait on index loads
-    for (int k0 = rp[i], k1 = rp[i + 1]; k0 < k1; k0 += 64) {
-      const int m = min(64, k1 - k0);
-      const int jl = c < m ? col[k0 + c] : i;
-      const float el = c < m ? e[k0 + c] : 0.f;
-#pragma unroll 4
-      for (int t = 0; t < m; ++t) {
-        const int j = __shfl(jl, t, 64);
-        const float et = __shfl(el, t, 64);
-        if (j != i) acc += fmaxf(za + Z[(size_t)j * kH2 + kH + c] + et * w + bb, 0.f);
+    const float4 za = *reinterpret_cast<const float4*>(Z + (size_t)i * kH2 + 4 * l);
+    const float4 w = *reinterpret_cast<const float4*>(we + 4 * l);
+    const float4 bb = *reinterpret_cast<const float4*>(bm + 4 * l);
+    for (int k0 = rp[i], k1 = rp[i + 1]; k0 < k1; k0 += G) {
+      const int m = min(G, k1 - k0);
+      const int jl = l < m ? col[k0 + l] : i;
+      const float el = l < m ? e[k0 + l] : 0.f;
+      for (int t0 = 0; t0 < m; t0 += 8) {
+        float4 zv[8];
+        int jt[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          jt[t] = __shfl(jl, t0 + t, G);  // lanes past m hold i: an in-range row
+          zv[t] = *reinterpret_cast<const float4*>(Z + (size_t)jt[t] * kH2 + kH + 4 * l);
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const float et = __shfl(el, t0 + t, G);
+          if (t0 + t < m && jt[t] != i) {
+            acc.x += fmaxf(za.x + zv[t].x + et * w.x + bb.x, 0.f);
+            acc.y += fmaxf(za.y + zv[t].y + et * w.y + bb.y, 0.f);
+            acc.z += fmaxf(za.z + zv[t].z + et * w.z + bb.z, 0.f);
+            acc.w += fmaxf(za.w + zv[t].w + et * w.w + bb.w, 0.f);
+          }
+        }
       }
     }
   }
-  M[(size_t)i * kH + c] = acc;
+  *reinterpret_cast<float4*>(M + (size_t)i * kH + 4 * l) = acc;
 }
 
 // Backward of the aggregation.  dM = T[:, 64:].  With s_ij = dM_i [pre_ij > 0]:
@@ -444,21 +470,25 @@ __global__ __launch_bounds__(256) void k_mp_edge_bwd(int n, int n_pad, const int
 // ---------------------------------------------------------------------------
 // decoder, loss, encoder backward
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float wave_sum_f(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// d = s (H w_dec + b_dec), the learned direction; one wavefront per row.
+// d = s (H w_dec + b_dec), the learned direction.
 __global__ __launch_bounds__(256) void k_mp_dec(int n, const float* __restrict__ H,
                                                 const float* __restrict__ P, int off,
                                                 const double* __restrict__ scal, double* dv) {
-  const int c = threadIdx.x & 63;
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  // 16 lanes per row, 4 channels per lane; the sum is the butterfly over the
+  // 64 channels (partners 32, 16, 8, 4 across lanes, then 2 and 1 within a
+  // lane), the order one lane per channel with xor shuffles adds in
+  const int l = threadIdx.x & 15;
+  const int i = blockIdx.x * 16 + (threadIdx.x >> 4);
   if (i >= n) return;
-  const float d = wave_sum_f(H[(size_t)i * kH + c] * P[off + c]) + P[off + kH];
-  if (c == 0) dv[i] = scal[0] * (double)d;
+  const float4 h = *reinterpret_cast<const float4*>(H + (size_t)i * kH + 4 * l);
+  float v[4] = {h.x * P[off + 4 * l], h.y * P[off + 4 * l + 1], h.z * P[off + 4 * l + 2],
+                h.w * P[off + 4 * l + 3]};
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] += __shfl_xor(v[q], o, 16);
+  const float d = ((v[0] + v[2]) + (v[1] + v[3])) + P[off + kH];
+  if (l == 0) dv[i] = scal[0] * (double)d;
 }
 
 // q = A d; line-search partials.  A-norm objective (obj 0): d.q -> slot 0,
@@ -782,7 +812,7 @@ static int forward(ofh_mpnn M, const double* xn, const double* xp, const double*
                      xn, xp, M->ext, b, M->r, M->z, M->dpart);
   hipLaunchKernelGGL(k_mp_feat, dim3(gp), dim3(256), 0, s, n, n_pad, A->row_ptr, A->col, A->val,
                      A->dinv, M->r, M->z, xn, xp, M->ld, M->dpart, g1, M->F, M->scal);
-  hipLaunchKernelGGL(k_mp_enc, dim3((n_pad * kH + 255) / 256), dim3(256), 0, s, n, n_pad, M->F,
+  hipLaunchKernelGGL(k_mp_enc, dim3((n_pad * (kH / 4) + 255) / 256), dim3(256), 0, s, n, n_pad, M->F,
                      M->params, Hl(M, 0));
   const int gg = gemm_grid(n_pad);
   for (int l = 0; l < M->L; ++l) {
@@ -795,7 +825,7 @@ static int forward(ofh_mpnn M, const double* xn, const double* xp, const double*
     a.W = Wab_bf(M, l);
     a.Y = Zl(M, l);
     hipLaunchKernelGGL((k_mp_rowgemm<kH, kH2, false, false, false>), dim3(gg), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_mp_edge_fwd, dim3((n_pad + 3) / 4), dim3(256), 0, s, n, n_pad,
+    hipLaunchKernelGGL(k_mp_edge_fwd, dim3((n_pad + 15) / 16), dim3(256), 0, s, n, n_pad,
                        A->row_ptr, A->col, M->e, Zl(M, l), M->params + off_we(l),
                        M->params + off_bm(l), Ml(M, l));
     RowGemmArgs u{};
@@ -814,7 +844,7 @@ static int forward(ofh_mpnn M, const double* xn, const double* xp, const double*
     u.Y2 = training ? Ul(M, l) : nullptr;
     hipLaunchKernelGGL((k_mp_rowgemm<kH2, kH, false, true, false>), dim3(gg), dim3(256), 0, s, u);
   }
-  hipLaunchKernelGGL(k_mp_dec, dim3((n + 3) / 4), dim3(256), 0, s, n, Hl(M, M->L), M->params,
+  hipLaunchKernelGGL(k_mp_dec, dim3((n + 15) / 16), dim3(256), 0, s, n, Hl(M, M->L), M->params,
                      off_dec(M->L), M->scal, M->dv);
   hipLaunchKernelGGL(k_mp_ls, dim3(g1), dim3(256), 0, s, n, A->row_ptr, A->col, A->val, M->dv,
                      M->r, M->q, M->dpart, g1, M->obj);
