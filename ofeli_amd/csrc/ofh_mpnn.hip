@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "ofh_internal.h"
+#include "ofh_spmv.h"
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -74,20 +75,44 @@ __device__ __forceinline__ double xbase(const double* __restrict__ xn,
 }
 
 // r = b - A x_b, z = D^-1 r; partial sums of z^2 per workgroup.
+// x_b gathered at column c (+0.0 past either end, where the entry is absent)
+struct XbGather {
+  XRsrc xn, xp;
+  double ext;
+  __device__ __forceinline__ double operator()(uint32_t c) const {
+    const double a = x_load<double>(xn, c), p = x_load<double>(xp, c);
+    return a + ext * (a - p);
+  }
+};
+
+// The row sums of the next three kernels: with W > 0 from the FP64 diagonal
+// copy of W slots per row (a row per lane, workgroups remapped across the
+// XCDs, each writing the partial of its own 256 rows), else a thread per row
+// over its CSR entries; both are the CSR row sum bit for bit.
+template <int W>
+__device__ __forceinline__ int mp_row_block() {
+  return W ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+}
+
+template <int W>
 __global__ __launch_bounds__(256) void k_mp_resid(int n, const int* __restrict__ rp,
                                                   const int* __restrict__ col,
-                                                  const double* __restrict__ val,
+                                                  const double* __restrict__ val, DiaF64 dia,
                                                   const double* __restrict__ dinv,
                                                   const double* __restrict__ xn,
                                                   const double* __restrict__ xp, double ext,
                                                   const double* __restrict__ b, double* r,
                                                   double* z, double* part) {
   __shared__ double red[4];
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int blk = mp_row_block<W>();
+  const int i = blk * 256 + threadIdx.x;
   double zz = 0.0;
   if (i < n) {
     double s = 0.0;
-    for (int k = rp[i]; k < rp[i + 1]; ++k) s += val[k] * xbase(xn, xp, ext, col[k]);
+    if constexpr (W > 0)
+      s = dia_rowsum_g<W>(i, dia, XbGather{x_rsrc(xn, n), x_rsrc(xp, n), ext});
+    else
+      for (int k = rp[i]; k < rp[i + 1]; ++k) s += val[k] * xbase(xn, xp, ext, col[k]);
     const double ri = b[i] - s;
     const double zi = dinv[i] * ri;
     r[i] = ri;
@@ -95,13 +120,14 @@ __global__ __launch_bounds__(256) void k_mp_resid(int n, const int* __restrict__
     zz = zi * zi;
   }
   zz = block_sum<256>(zz, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = zz;
+  if (threadIdx.x == 0) part[blk] = zz;
 }
 
 // s = rms(z) (folded by every workgroup), z2 = D^-1 (r - A z), features.
+template <int W>
 __global__ __launch_bounds__(256) void k_mp_feat(int n, int n_pad, const int* __restrict__ rp,
                                                  const int* __restrict__ col,
-                                                 const double* __restrict__ val,
+                                                 const double* __restrict__ val, DiaF64 dia,
                                                  const double* __restrict__ dinv,
                                                  const double* __restrict__ r,
                                                  const double* __restrict__ z,
@@ -117,12 +143,15 @@ __global__ __launch_bounds__(256) void k_mp_feat(int n, int n_pad, const int* __
   double s = sqrt(t / (double)n);
   if (!(s > 0.0)) s = 1.0;
   if (blockIdx.x == 0 && threadIdx.x == 0) scal[0] = s;
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int i = mp_row_block<W>() * 256 + threadIdx.x;
   if (i >= n_pad) return;
   float f[kFin] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (i < n) {
     double az = 0.0;
-    for (int k = rp[i]; k < rp[i + 1]; ++k) az += val[k] * z[col[k]];
+    if constexpr (W > 0)
+      az = dia_rowsum_g<W>(i, dia, SellGather<double>{x_rsrc(z, n)});
+    else
+      for (int k = rp[i]; k < rp[i + 1]; ++k) az += val[k] * z[col[k]];
     const double z2 = dinv[i] * (r[i] - az);
     f[0] = (float)(z[i] / s);
     f[1] = (float)(z2 / s);
@@ -187,7 +216,7 @@ struct RowGemmArgs {
 //   A fragment: A[l%16][4*(l/16)+t];  B fragment: B[4*(l/16)+t][l%16];
 //   accumulator: D[4*(l/16)+t][l%16].
 template <int K, int O, bool TRANS, bool RELU, bool MASK>
-__global__ __launch_bounds__(256) void k_mp_rowgemm(RowGemmArgs a) {
+__global__ __launch_bounds__(256, 3) void k_mp_rowgemm(RowGemmArgs a) {
   constexpr int KB = K / 16, JB = O / 16;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lg = lane >> 4;
@@ -493,18 +522,23 @@ __global__ __launch_bounds__(256) void k_mp_dec(int n, const float* __restrict__
 
 // q = A d; line-search partials.  A-norm objective (obj 0): d.q -> slot 0,
 // d.r -> slot 1; minimum residual (obj 1): q.q -> 0, q.r -> 1, r.r -> 2.
+template <int W>
 __global__ __launch_bounds__(256) void k_mp_ls(int n, const int* __restrict__ rp,
                                                const int* __restrict__ col,
-                                               const double* __restrict__ val,
+                                               const double* __restrict__ val, DiaF64 dia,
                                                const double* __restrict__ dv,
                                                const double* __restrict__ r, double* q,
                                                double* part, int nparts, int obj) {
   __shared__ double red[12];
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int blk = mp_row_block<W>();
+  const int i = blk * 256 + threadIdx.x;
   double v[3] = {0.0, 0.0, 0.0};
   if (i < n) {
     double a = 0.0;
-    for (int k = rp[i]; k < rp[i + 1]; ++k) a += val[k] * dv[col[k]];
+    if constexpr (W > 0)
+      a = dia_rowsum_g<W>(i, dia, SellGather<double>{x_rsrc(dv, n)});
+    else
+      for (int k = rp[i]; k < rp[i + 1]; ++k) a += val[k] * dv[col[k]];
     q[i] = a;
     const double w = obj ? a : dv[i];
     v[0] = w * a;
@@ -513,9 +547,9 @@ __global__ __launch_bounds__(256) void k_mp_ls(int n, const int* __restrict__ rp
   }
   block_sum_k<256, 3>(v, red);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = v[0];
-    part[nparts + blockIdx.x] = v[1];
-    if (obj) part[2 * nparts + blockIdx.x] = v[2];
+    part[blk] = v[0];
+    part[nparts + blk] = v[1];
+    if (obj) part[2 * nparts + blk] = v[2];
   }
 }
 
@@ -808,10 +842,26 @@ static int forward(ofh_mpnn M, const double* xn, const double* xp, const double*
   ofh_csr A = M->A;
   const int n = M->n, n_pad = M->n_pad;
   const int g1 = (n + 255) / 256, gp = (n_pad + 255) / 256;
-  hipLaunchKernelGGL(k_mp_resid, dim3(g1), dim3(256), 0, s, n, A->row_ptr, A->col, A->val, A->dinv,
-                     xn, xp, M->ext, b, M->r, M->z, M->dpart);
-  hipLaunchKernelGGL(k_mp_feat, dim3(gp), dim3(256), 0, s, n, n_pad, A->row_ptr, A->col, A->val,
-                     A->dinv, M->r, M->z, xn, xp, M->ld, M->dpart, g1, M->F, M->scal);
+  // the FP64 diagonal copy when the entry point made it current (never here:
+  // this may run under stream capture)
+  const bool dia = A->dia_use64 && A->dia_val64 && A->dia_version64 == A->version;
+  const DiaF64 da = dia ? dia_f64(A) : DiaF64{};
+  const int dw = dia ? A->dia_w : 0;
+#define OFH_MP_ROWS(KERNEL, GRID, ...)                                                        \
+  do {                                                                                        \
+    if (dw == 4)                                                                              \
+      hipLaunchKernelGGL(KERNEL<4>, dim3(GRID), dim3(256), 0, s, __VA_ARGS__);                \
+    else if (dw == 8)                                                                         \
+      hipLaunchKernelGGL(KERNEL<8>, dim3(GRID), dim3(256), 0, s, __VA_ARGS__);                \
+    else if (dw == 16)                                                                        \
+      hipLaunchKernelGGL(KERNEL<16>, dim3(GRID), dim3(256), 0, s, __VA_ARGS__);               \
+    else                                                                                      \
+      hipLaunchKernelGGL(KERNEL<0>, dim3(GRID), dim3(256), 0, s, __VA_ARGS__);                \
+  } while (0)
+  OFH_MP_ROWS(k_mp_resid, g1, n, A->row_ptr, A->col, A->val, da, A->dinv, xn, xp, M->ext, b,
+              M->r, M->z, M->dpart);
+  OFH_MP_ROWS(k_mp_feat, gp, n, n_pad, A->row_ptr, A->col, A->val, da, A->dinv, M->r, M->z, xn,
+              xp, M->ld, M->dpart, g1, M->F, M->scal);
   hipLaunchKernelGGL(k_mp_enc, dim3((n_pad * (kH / 4) + 255) / 256), dim3(256), 0, s, n, n_pad, M->F,
                      M->params, Hl(M, 0));
   const int gg = gemm_grid(n_pad);
@@ -846,8 +896,9 @@ static int forward(ofh_mpnn M, const double* xn, const double* xp, const double*
   }
   hipLaunchKernelGGL(k_mp_dec, dim3((n + 15) / 16), dim3(256), 0, s, n, Hl(M, M->L), M->params,
                      off_dec(M->L), M->scal, M->dv);
-  hipLaunchKernelGGL(k_mp_ls, dim3(g1), dim3(256), 0, s, n, A->row_ptr, A->col, A->val, M->dv,
-                     M->r, M->q, M->dpart, g1, M->obj);
+  OFH_MP_ROWS(k_mp_ls, g1, n, A->row_ptr, A->col, A->val, da, M->dv, M->r, M->q, M->dpart, g1,
+              M->obj);
+#undef OFH_MP_ROWS
   hipLaunchKernelGGL(k_mp_apply, dim3(g1), dim3(256), 0, s, n, M->dpart, g1, xn, xp, M->ext,
                      M->dv, M->scal, x0);
   OFH_HIP(hipGetLastError());
@@ -1202,6 +1253,8 @@ int ofh_mpnn_guess(ofh_mpnn M, const double* d_xn, const double* d_xp, const dou
                    double* d_x0) {
   OFH_CHECK(M && d_xn && d_xp && d_b && d_x0, OFH_E_INVALID, "ofh_mpnn_guess: bad argument");
   hipStream_t s = M->A->ctx->stream;
+  bool dia = false;  // the FP64 diagonal copy the row sums read, made current outside any capture
+  if (int rc = csr_ensure_dia64(M->A, &dia)) return rc;
   if (!M->use_graph) return forward(M, d_xn, d_xp, d_b, d_x0, false, s);
   // the ~16 launches of an inference are captured once per argument set and
   // replayed (graph of kernels only)
@@ -1246,7 +1299,9 @@ int ofh_mpnn_loss_grad(ofh_mpnn M, const double* d_xn, const double* d_xp, const
   OFH_CHECK(M && d_xn && d_xp && d_b && (d_xstar || M->obj == 1) && d_x0, OFH_E_INVALID,
             "ofh_mpnn_loss_grad: bad argument");
   hipStream_t s = M->A->ctx->stream;
-  int rc = forward(M, d_xn, d_xp, d_b, d_x0, true, s);
+  bool dia = false;  // the FP64 diagonal copy the forward row sums read
+  int rc = csr_ensure_dia64(M->A, &dia);
+  if (!rc) rc = forward(M, d_xn, d_xp, d_b, d_x0, true, s);
   if (rc) return rc;
   if ((rc = backward(M, d_xn, d_xp, d_xstar, s))) return rc;
   OFH_HIP(hipMemcpyAsync(M->h_pinned, M->scal, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
@@ -1262,7 +1317,9 @@ int ofh_mpnn_train_step(ofh_mpnn M, const double* d_xn, const double* d_xp, cons
             OFH_E_INVALID,
             "ofh_mpnn_train_step: bad argument");
   hipStream_t s = M->A->ctx->stream;
-  int rc = forward(M, d_xn, d_xp, d_b, d_x0, true, s);
+  bool dia = false;  // the FP64 diagonal copy the forward row sums read
+  int rc = csr_ensure_dia64(M->A, &dia);
+  if (!rc) rc = forward(M, d_xn, d_xp, d_b, d_x0, true, s);
   if (rc) return rc;
   if ((rc = backward(M, d_xn, d_xp, d_xstar, s))) return rc;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
