@@ -325,6 +325,7 @@ int ofh_ctx_destroy(ofh_ctx c) {
   hipFree(c->ws.partials);
   hipFree(c->dev_scratch);
   hipHostFree(c->host_pinned);
+  if (c->look_ev) (void)hipEventDestroy(c->look_ev);
   if (c->vws) hipFree(c->vws);
   for (auto& e : c->graphs) (void)hipGraphExecDestroy(e.second);
   for (auto& r : c->prof_recs) {

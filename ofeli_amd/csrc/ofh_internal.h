@@ -232,6 +232,7 @@ struct ofh_ctx_s {
   int num_cu = 0;
   ofh::Workspace ws;
   void* host_pinned = nullptr;  // 4 KiB staging for scalars
+  hipEvent_t look_ev = nullptr;  // marks the state copy of a look taken ahead
   double* dev_scratch = nullptr;  // small per-solve state (4 KiB)
   void* vws = nullptr;            // solver work vectors, grown on demand
   size_t vws_bytes = 0;

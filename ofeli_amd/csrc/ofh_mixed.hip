@@ -1173,10 +1173,17 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   int stagnant = 0, slow = 0, fallback = 0, it_repl = 0, first_cycle = 0;
   int last_cycle = 0;
   double best_rate = 0.0;
+  // Each look at the state is taken while kAhead more iterations run: the
+  // state is copied after the chunk, the kAhead launches are enqueued behind
+  // the copy, and the host waits for the copy only, so the GPU does not idle
+  // while the host decides.  Launches past a pause return at once.
+  constexpr int kAhead = 4;
+  int ahead = 0;  // iterations enqueued after the copy h was taken
   while (!h.done) {
-    int k = h.it;
+    int k = h.it + ahead;
     const double target = std::max(kDelta * h.rmax, toler);
-    const int chunk = mixed_chunk(k, h.res, it_repl, h.rmax, target, max_it - k, last_cycle);
+    const int chunk = std::max(
+        1, mixed_chunk(h.it, h.res, it_repl, h.rmax, target, max_it - h.it, last_cycle) - ahead);
     // whole captured chunks, then the remainder launched directly
     const int full = chunk_graph ? chunk / kGraphIters : 0;
     for (int c = 0; c < full; ++c) OFH_HIP(hipGraphLaunch(chunk_graph, s));
@@ -1184,15 +1191,29 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     for (int c = full * kGraphIters; c < chunk; ++c)
       if ((rc = iteration(++k, s))) return rc;
     OFH_HIP(hipGetLastError());
-    if ((rc = read_state(ctx, &h))) return rc;
+    ahead = kAhead;  // past max_it a launch pauses (the first) or returns
+    if ((rc = read_state_ahead(ctx, &h, [&]() -> int {
+           for (int c = 0; c < ahead; ++c)
+             if (const int e = iteration(++k, s)) return e;
+           OFH_HIP(hipGetLastError());
+           return OFH_OK;
+         })))
+      return rc;
     if (h.done) break;
     // A cycle far longer than the first one means FP32 can no longer lower
     // the recursive residual: replace it now (the direction p_{k+1} and
     // rho_k are already in place, so the replacement acts as iteration k+1).
-    const int cyc_now = h.it - it_repl;
     const int cap = first_cycle > 0 ? std::max(200, 3 * first_cycle) : 2000;
+    if (!h.pause && ahead > 0 && h.it + ahead - it_repl >= cap) {
+      // the iterations run ahead may reach the cap: the exact state decides
+      if ((rc = read_state(ctx, &h))) return rc;
+      ahead = 0;
+      if (h.done) break;
+    }
+    const int cyc_now = h.it - it_repl;
     const bool forced = !h.pause && cyc_now >= cap;
     if (!h.pause && !forced) continue;
+    ahead = 0;  // paused: the iterations run ahead returned at once
     if (h.pause && first_cycle == 0) first_cycle = std::max(1, cyc_now);
     // (the one-kernel form has already taken the step along p_{it-1}: x32 is
     // x_it, so its replacement acts as iteration it)

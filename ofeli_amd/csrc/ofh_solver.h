@@ -58,6 +58,20 @@ inline int read_dev(ofh_ctx ctx, const S* d, S* h) {
 
 inline int read_state(ofh_ctx ctx, KrylovState* h) { return read_dev(ctx, state_ptr(ctx), h); }
 
+// The state as it is once everything enqueued so far has run, read while
+// the launches `more()` enqueues behind its copy keep the GPU busy.
+template <class F>
+inline int read_state_ahead(ofh_ctx ctx, KrylovState* h, F&& more) {
+  if (!ctx->look_ev) OFH_HIP(hipEventCreateWithFlags(&ctx->look_ev, hipEventDisableTiming));
+  OFH_HIP(hipMemcpyAsync(ctx->host_pinned, state_ptr(ctx), sizeof(KrylovState),
+                         hipMemcpyDeviceToHost, ctx->stream));
+  OFH_HIP(hipEventRecord(ctx->look_ev, ctx->stream));
+  if (const int rc = more()) return rc;
+  OFH_HIP(hipEventSynchronize(ctx->look_ev));
+  std::memcpy(h, ctx->host_pinned, sizeof(KrylovState));
+  return OFH_OK;
+}
+
 // Iterations to enqueue before the next host check.  The residual reduction
 // observed since the previous check predicts the remaining count, capped by
 // the prediction from the rate since the solve's first check (`res0` after
