@@ -649,19 +649,29 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
     double res_prev = h.res;
     const int it0 = h.it;
     const double res0 = h.res;
+    // each look at the state is taken while up to kLookAhead more
+    // iterations run (launches past the stop return at once)
+    int ahead = 0;
     while (!h.done && enq < max_it) {
-      const int chunk =
-          next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq, it0, res0);
+      const int chunk = std::max(
+          1, next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq, it0, res0) - ahead);
       it_prev = h.it;
       res_prev = h.res;
       // whole captured chunks, then the remainder launched directly
       const int full = chunk_graph ? chunk / kGraphIters : 0;
       for (int c = 0; c < full; ++c) OFH_HIP(hipGraphLaunch(chunk_graph, s));
       enq += full * kGraphIters;
-      for (int c = full * kGraphIters; c < chunk; ++c) iteration(++enq, s);
+      for (int c = full * kGraphIters; c < chunk && enq < max_it; ++c) iteration(++enq, s);
       OFH_HIP(hipGetLastError());
-      if ((rc = read_state(ctx, &h))) return rc;
+      ahead = std::min(kLookAhead, max_it - enq);
+      if ((rc = read_state_ahead(ctx, &h, [&]() -> int {
+             for (int c = 0; c < ahead; ++c) iteration(++enq, s);
+             OFH_HIP(hipGetLastError());
+             return OFH_OK;
+           })))
+        return rc;
     }
+    if (ahead > 0 && (rc = read_state(ctx, &h))) return rc;  // the final state
     if (!h.done) {  // max_it == 0 with a non-converged guess
       h.done = 1;
       h.status = -(max_it + 1);
@@ -744,20 +754,29 @@ static int cg_impl(ofh_csr A, const double* b, double* x, int max_it, double tol
   }
   const int it0 = h.it;
   const double res0 = h.res;
+  int ahead = 0;  // as in the two-kernel loop
   while (!h.done && enq < max_it) {
-    const int chunk =
-        next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq, it0, res0);
+    const int chunk = std::max(
+        1, next_chunk(h.it, h.res, it_prev, res_prev, toler, max_it - enq, it0, res0) - ahead);
     it_prev = h.it;
     res_prev = h.res;
     // whole captured chunks, then the remainder launched directly
     const int full = chunk_graph ? chunk / kGraphIters : 0;
     for (int c = 0; c < full; ++c) OFH_HIP(hipGraphLaunch(chunk_graph, s));
     enq += full * kGraphIters;
-    for (int c = full * kGraphIters; c < chunk; ++c)
+    for (int c = full * kGraphIters; c < chunk && enq < max_it; ++c)
       if ((rc = iteration(++enq, s))) return rc;
     OFH_HIP(hipGetLastError());
-    if ((rc = read_state(ctx, &h))) return rc;
+    ahead = std::min(kLookAhead, max_it - enq);
+    if ((rc = read_state_ahead(ctx, &h, [&]() -> int {
+           for (int c = 0; c < ahead; ++c)
+             if (const int e = iteration(++enq, s)) return e;
+           OFH_HIP(hipGetLastError());
+           return OFH_OK;
+         })))
+      return rc;
   }
+  if (ahead > 0 && (rc = read_state(ctx, &h))) return rc;  // the final state
   if (!h.done) {  // max_it == 0 with a non-converged guess
     h.done = 1;
     h.status = -(max_it + 1);

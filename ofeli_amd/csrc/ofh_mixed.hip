@@ -1173,11 +1173,10 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
   int stagnant = 0, slow = 0, fallback = 0, it_repl = 0, first_cycle = 0;
   int last_cycle = 0;
   double best_rate = 0.0;
-  // Each look at the state is taken while kAhead more iterations run: the
-  // state is copied after the chunk, the kAhead launches are enqueued behind
+  // Each look at the state is taken while kLookAhead more iterations run: the
+  // state is copied after the chunk, those launches are enqueued behind
   // the copy, and the host waits for the copy only, so the GPU does not idle
   // while the host decides.  Launches past a pause return at once.
-  constexpr int kAhead = 4;
   int ahead = 0;  // iterations enqueued after the copy h was taken
   while (!h.done) {
     int k = h.it + ahead;
@@ -1191,7 +1190,7 @@ static int mixed_impl(ofh_csr A, const double* b, double* x, int max_it, double 
     for (int c = full * kGraphIters; c < chunk; ++c)
       if ((rc = iteration(++k, s))) return rc;
     OFH_HIP(hipGetLastError());
-    ahead = kAhead;  // past max_it a launch pauses (the first) or returns
+    ahead = kLookAhead;  // past max_it a launch pauses (the first) or returns
     if ((rc = read_state_ahead(ctx, &h, [&]() -> int {
            for (int c = 0; c < ahead; ++c)
              if (const int e = iteration(++k, s)) return e;

@@ -58,6 +58,9 @@ inline int read_dev(ofh_ctx ctx, const S* d, S* h) {
 
 inline int read_state(ofh_ctx ctx, KrylovState* h) { return read_dev(ctx, state_ptr(ctx), h); }
 
+// Iterations a solver loop enqueues behind each copy of its state.
+constexpr int kLookAhead = 4;
+
 // The state as it is once everything enqueued so far has run, read while
 // the launches `more()` enqueues behind its copy keep the GPU busy.
 template <class F>
